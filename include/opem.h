@@ -1,0 +1,140 @@
+/* opem.h - C ABI of the MI355X per-frequency solve path (OpenParEM3D drop-in for one path).
+ *
+ * One context = one GPU + one HIP stream.  Caller owns every host buffer; the context owns all
+ * device memory.  Every call returns 0 on success, a negative value for misuse (bad argument,
+ * wrong call order) and a positive value for a runtime failure (HIP error, allocation);
+ * opem_last_error() gives the message.  Nonconvergence of a Krylov solve is NOT an error: it
+ * is reported through per-system `converged` flags, as the reference does
+ * (src/OpenParEM3D/solveComplexLinearSystem.c:204-206, *converged).
+ *
+ * Complex numbers are interleaved (re, im) doubles.  Boundary attributes and volume attributes are
+ * 1-based, as in MFEM meshes.  No exceptions cross the ABI.  One host thread per context.
+ */
+#ifndef OPEM_H
+#define OPEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct opem_ctx opem_ctx;
+
+/* Boundary kinds (src/OpenParEM3D/port.cpp:733-755 Boundary types; ports from Port definitions). */
+enum {
+  OPEM_BDR_PEC = 1,        /* essential E_t = 0 (fem3D.cpp:80-99, eliminatePEC)                     */
+  OPEM_BDR_PMC = 2,        /* natural, nothing assembled                                            */
+  OPEM_BDR_PORT = 3,       /* wave port: Dirichlet when driven, 1st-order 2D ABC otherwise           */
+  OPEM_BDR_IMPEDANCE = 4,  /* surface impedance; value = conductivity sigma [S/m] (R_s from sigma)   */
+  OPEM_BDR_RADIATION = 5   /* 1st-order radiation; value = wave impedance eta [ohm] (0 -> eta0)      */
+};
+
+enum { OPEM_PREC_FP64 = 0, OPEM_PREC_MIXED = 1 };
+
+/* Context lifecycle. */
+int opem_create(opem_ctx **ctx, int device);
+int opem_destroy(opem_ctx *ctx);
+const char *opem_last_error(const opem_ctx *ctx);
+
+/* Mesh (replaces the reference's gmsh/MFEM mesh load, src/OpenParEMCommon/mesh.cpp:386-449).
+ * xyz: nv*3, tets: nt*4 (0-based vertex ids), tet_attr: nt, bfaces: nbf*3, bface_attr: nbf. */
+int opem_mesh_set(opem_ctx *ctx, int64_t nv, const double *xyz, int64_t nt, const int32_t *tets,
+                  const int32_t *tet_attr, int64_t nbf, const int32_t *bfaces, const int32_t *bface_attr);
+
+/* ND_p space, DOF numbering, shared CSR pattern, and the unit-coefficient base matrices
+ * (curl-curl, mass, mixed curl per volume attribute) assembled on the GPU.
+ * Replaces fem3D::build_fe_spaces (fem3D.cpp:65) + the MFEM assembly inside build_A/build_P/build_Q
+ * (fem3D.cpp:103-225).  Materials and boundaries must be set before opem_sweep, not before this. */
+int opem_space_build(opem_ctx *ctx, int order, int64_t *ndof, int64_t *nnz);
+
+/* Read back the numbering / pattern (any pointer may be NULL).  elem_dofs: nt*nd, bface_dofs:
+ * nbf*ndf, rowptr: ndof+1, colind: nnz. */
+int opem_space_get(opem_ctx *ctx, int32_t *elem_dofs, int32_t *bface_dofs, int32_t *rowptr, int32_t *colind);
+
+/* Unit-coefficient base volume matrix values on the pattern for one volume attribute.
+ * which: 0 = curl-curl K, 1 = mass M, 2 = mixed curl Q (Q[a][b] = int curl(phi_b).phi_a). out: nnz. */
+int opem_base_matrix(opem_ctx *ctx, int which, int attr, double *out);
+
+/* Material of a volume attribute: eps = er*eps0*(1 - j tand), mu = mur*mu0
+ * (src/OpenParEMCommon/OpenParEMmaterials.cpp:600-708). */
+int opem_material_set(opem_ctx *ctx, int attr, double er, double tand, double mur);
+
+/* Boundary attribute kind and parameter (see OPEM_BDR_*). */
+int opem_boundary_set(opem_ctx *ctx, int attr, int kind, double value);
+
+/* Quadrature points of a port at which the caller samples its modal fields (npts*3 doubles).
+ * Call with xyz == NULL to query npts. */
+int opem_port_points(opem_ctx *ctx, int attr, int64_t *npts, double *xyz);
+
+/* Complex system matrix A(omega) values (nnz complex) with the 2D ABC on every port
+ * (gamma: one complex per port, ports in ascending attribute order).  Mirrors fem3D::build_A
+ * (fem3D.cpp:103-190) + hypre_ParCSRMatrixToMat (csr.c:171-223), before any elimination. */
+int opem_system_matrix(opem_ctx *ctx, double freq, const double *gamma, double *vals);
+
+/* Complex CSR SpMV on caller-provided host arrays (parity check of the device kernel):
+ * y = A x with A n x n (rowptr n+1, colind nnz, vals nnz complex); prec OPEM_PREC_FP64 computes in
+ * FP64, OPEM_PREC_MIXED stores values and vectors in FP32 and accumulates in FP64. */
+int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind,
+              const double *vals, const double *x, double *y);
+
+/* Element matrices of ne independent tets (the assembly kernel alone):
+ * A_e = K_e - k0^2 er (1 - j tand) M_e, written as {Re A_e (nd*nd row-major), Im A_e} per element.
+ * verts: ne*12 (4 vertices per tet, element-local order as given), er/tand: ne.
+ * prec 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16-input MFMA (per-element scaled), FP32 output for 1 and 2.
+ * The kernel is run nrep times on device-resident inputs; *avg_seconds is its mean duration
+ * (HIP events).  out (ne*2*nd*nd doubles) may be NULL.  Counterpart: the element loop inside MFEM's
+ * CurlCurlIntegrator / VectorFEMassIntegrator::AssembleElementMatrix used by fem3D.cpp:115-117. */
+int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const double *verts, const double *er,
+                          const double *tand, double k0, int nrep, double *out, double *avg_seconds);
+
+typedef struct {
+  int nfreq;
+  const double *freqs;          /* [nfreq] Hz                                                     */
+  int nports;                   /* S-ports (one mode per port)                                    */
+  const int32_t *port_attr;     /* [nports] boundary attributes, S-port order                     */
+  const double *mode_E;         /* complex [nfreq][sum_k npts_k*3] modal E_t at port points       */
+  const double *mode_H;         /* complex, same layout: modal H_t of the wave leaving the domain */
+  const double *gamma;          /* complex [nfreq][nports] propagation constant alpha + j beta    */
+  const double *V;              /* complex [nfreq][nports] modal voltage                          */
+  const double *Z;              /* complex [nfreq][nports] modal impedance                        */
+  int precision;                /* OPEM_PREC_FP64 or OPEM_PREC_MIXED                              */
+  int batch;                    /* frequencies per batched launch; 1 = one system group at a time */
+  int restart;                  /* GMRES(m)                                                       */
+  int maxit;                    /* Krylov iteration limit per system                              */
+  double tol;                   /* true relative residual ||b - A x|| / ||b|| target              */
+  int reciprocal;               /* 1: S <- (S + S^T)/2 (results.cpp:401-419)                      */
+  double h_tol;                 /* H-projection CG relative tolerance (0 -> 1e-13)                */
+} opem_sweep_in;
+
+typedef struct {
+  double *S;                    /* complex [nfreq][nports][nports], S[i][j] row-major             */
+  double *S_raw;                /* optional: before forced reciprocity                            */
+  double *Cp, *Cm;              /* optional complex [nfreq][nports drive][nports]                 */
+  int32_t *iters;               /* optional [nfreq][nports]                                       */
+  double *true_res;             /* optional [nfreq][nports] final ||b-Ax||/||b|| in FP64          */
+  int32_t *converged;           /* optional [nfreq][nports]                                       */
+  double *x_out;                /* optional complex [nfreq][nports][ndof]  E DOFs                 */
+  double *h_out;                /* optional complex [nfreq][nports][ndof]  H DOFs                 */
+  double seconds_total;         /* wall time of the sweep (device-synchronised)                   */
+  double seconds_setup;         /* coefficient combine + preconditioner build                     */
+  double seconds_solve;         /* E solves                                                       */
+  double seconds_post;          /* H projection + port projections + S                            */
+} opem_sweep_out;
+
+/* The per-frequency hot path for a list of frequencies and all port excitations:
+ * coefficients -> A(omega) -> PEC + driven-port elimination -> GMRES (element-block Jacobi)
+ * -> H projection -> port projections (C+, C-) -> S.  Mirrors the driving-set loop of
+ * OpenParEM3D.cpp:497-566 around fem3D::solve (fem3D.cpp:712-799) and
+ * BoundaryDatabase::calculateS (port.cpp:7499-7606). */
+int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out);
+
+/* S from splits (BoundaryDatabase::calculateS, port.cpp:7499-7606, with Zsolve.c LUP inverse):
+ * Cp/Cm complex [n drive][n], V/Z complex [n]; S complex [n][n]. Host-only, no context. */
+int opem_calculate_s(int n, const double *Cp, const double *Cm, const double *V, const double *Z,
+                     int reciprocal, double *S);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPEM_H */

@@ -1,0 +1,91 @@
+"""ctypes binding of libopem.so (include/opem.h).  Fails loudly when the library is missing."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libopem.so")
+
+SYMBOLS = [
+    "opem_create", "opem_destroy", "opem_last_error", "opem_mesh_set", "opem_space_build", "opem_space_get",
+    "opem_base_matrix", "opem_material_set", "opem_boundary_set", "opem_port_points", "opem_system_matrix",
+    "opem_spmv", "opem_element_matrices", "opem_sweep", "opem_calculate_s",
+]
+
+BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
+PREC_FP64, PREC_MIXED = 0, 1
+
+_lib = None
+
+P = C.c_void_p
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+
+
+class SweepIn(C.Structure):
+    _fields_ = [("nfreq", C.c_int), ("freqs", f64p), ("nports", C.c_int), ("port_attr", i32p),
+                ("mode_E", f64p), ("mode_H", f64p), ("gamma", f64p), ("V", f64p), ("Z", f64p),
+                ("precision", C.c_int), ("batch", C.c_int), ("restart", C.c_int), ("maxit", C.c_int),
+                ("tol", C.c_double), ("reciprocal", C.c_int), ("h_tol", C.c_double)]
+
+
+class SweepOut(C.Structure):
+    _fields_ = [("S", f64p), ("S_raw", f64p), ("Cp", f64p), ("Cm", f64p), ("iters", i32p),
+                ("true_res", f64p), ("converged", i32p), ("x_out", f64p), ("h_out", f64p),
+                ("seconds_total", C.c_double), ("seconds_setup", C.c_double),
+                ("seconds_solve", C.c_double), ("seconds_post", C.c_double)]
+
+
+def lib():
+    """Load libopem.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m openparem_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        L.opem_create.argtypes = [C.POINTER(P), C.c_int]
+        L.opem_destroy.argtypes = [P]
+        L.opem_last_error.argtypes = [P]
+        L.opem_last_error.restype = C.c_char_p
+        L.opem_mesh_set.argtypes = [P, C.c_int64, f64p, C.c_int64, i32p, i32p, C.c_int64, i32p, i32p]
+        L.opem_space_build.argtypes = [P, C.c_int, i64p, i64p]
+        L.opem_space_get.argtypes = [P, i32p, i32p, i32p, i32p]
+        L.opem_base_matrix.argtypes = [P, C.c_int, C.c_int, f64p]
+        L.opem_material_set.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.opem_boundary_set.argtypes = [P, C.c_int, C.c_int, C.c_double]
+        L.opem_port_points.argtypes = [P, C.c_int, i64p, f64p]
+        L.opem_system_matrix.argtypes = [P, C.c_double, f64p, f64p]
+        L.opem_spmv.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
+        L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
+                                            C.c_int, f64p, f64p]
+        L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]
+        L.opem_calculate_s.argtypes = [C.c_int, f64p, f64p, f64p, f64p, C.c_int, f64p]
+        _lib = L
+    return _lib
+
+
+def ptr(a, ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def cplx_view(a):
+    """complex128 array -> float64 view with interleaved (re, im)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64)
+
+
+class OpemError(RuntimeError):
+    pass
+
+
+def check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = lib().opem_last_error(ctx).decode() if ctx else ""
+        raise OpemError(f"{what} failed (rc={rc}): {msg}")

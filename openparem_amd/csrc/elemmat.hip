@@ -1,0 +1,272 @@
+// Element matrices of independent affine tets on MFMA (gfx950).
+//
+// For one element, A_e = K_e - k0^2 er (1 - j tand) M_e with
+//   K_e = sum_c gk_c TK_c,  M_e = sum_c gm_c TM_c      (6 symmetric Jacobian components c)
+// so a tile of 16 elements is a GEMM  X[16 x K] * T[K x nd^2]:
+//   Re: X = [gk(6), -k0^2 er gm(6), 0(4)],   T = [TK; TM; 0]     (K = 16)
+//   Im: X = [k0^2 er tand gm(6), 0(2)],       T = [TM; 0]         (K = 8)
+// One wavefront owns 16 elements and walks the nd^2 output in 16-wide tiles.
+//   prec 0: v_mfma_f64_16x16x4f64, FP64 out
+//   prec 1: v_mfma_f32_16x16x4f32, FP32 out (factors formed in FP64, rounded once)
+//   prec 2: v_mfma_f32_16x16x32_bf16 with per-element scaling of X to max|X| = 1, FP32 out
+// Operand layouts (CDNA3/4 16x16 MFMA): A[i][k] in lane i + 16*(k/kpl), B[k][j] in lane j + 16*(k/kpl),
+// D[i][j] in register i%4 of lane j + 16*(i/4) for f32 outputs, register i/4 of lane j + 16*(i%4) for
+// v_mfma_f64_16x16x4f64; kpl = values per lane (1 for f64/f32, 8 for bf16).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "kernels.h"
+#include "nd_tables.h"
+
+namespace opem {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+static inline int padded(int nd) { return ((nd * nd + 15) / 16) * 16; }
+
+// geometric factors of one tet in FP64 (same formulas as tet_factors, device side)
+__device__ void dev_factors(const double *v, double gm[6], double gk[6]) {
+  double J[3][3];
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) J[r][c] = v[(c + 1) * 3 + r] - v[r];
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  const double id = 1.0 / det;
+  double inv[3][3];
+  inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+  inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+  inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+  inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  const int comp[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  const double ad = fabs(det);
+  for (int c = 0; c < 6; c++) {
+    const int k = comp[c][0], l = comp[c][1];
+    double G = 0.0, C = 0.0;
+    for (int m = 0; m < 3; m++) {
+      G += inv[k][m] * inv[l][m];
+      C += J[m][k] * J[m][l];
+    }
+    gm[c] = ad * G;
+    gk[c] = C / ad;
+  }
+}
+
+template <int PREC>
+struct ElemT {
+  using T = double;  // table / A-operand type
+  using O = double;  // output type
+};
+template <>
+struct ElemT<1> {
+  using T = float;
+  using O = float;
+};
+template <>
+struct ElemT<2> {
+  using T = __bf16;
+  using O = float;
+};
+
+// tab layout: Tre[16][NP] then Tim[8][NP] (K rows, padded N); bf16: Tre[32][NP], Tim[32][NP]
+template <int PREC>
+__global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const double *__restrict__ verts,
+                                              const double *__restrict__ er, const double *__restrict__ tand,
+                                              double k0, const typename ElemT<PREC>::T *__restrict__ tab,
+                                              typename ElemT<PREC>::O *__restrict__ out) {
+  using T = typename ElemT<PREC>::T;
+  using O = typename ElemT<PREC>::O;
+  __shared__ double X[4][16][24];  // per wave: 16 elements x (16 re + 8 im) factors
+  __shared__ double sc[4][16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t e0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+  if (lane < 16) {
+    const int64_t e = e0 + lane;
+    double row[24];
+    for (int k = 0; k < 24; k++) row[k] = 0.0;
+    if (e < ne) {
+      double gm[6], gk[6];
+      dev_factors(verts + e * 12, gm, gk);
+      const double ke = k0 * k0 * er[e];
+      for (int c = 0; c < 6; c++) {
+        row[c] = gk[c];
+        row[6 + c] = -ke * gm[c];
+        row[16 + c] = ke * tand[e] * gm[c];
+      }
+    }
+    double s = 1.0;
+    if (PREC == 2) {
+      double mr = 0.0, mi = 0.0;
+      for (int k = 0; k < 16; k++) mr = fmax(mr, fabs(row[k]));
+      for (int k = 16; k < 24; k++) mi = fmax(mi, fabs(row[k]));
+      s = mr > 0.0 ? mr : 1.0;
+      const double si = mi > 0.0 ? mi : 1.0;
+      for (int k = 0; k < 16; k++) row[k] /= s;
+      for (int k = 16; k < 24; k++) row[k] /= si;
+      sc[wave][lane] = s;
+      X[wave][lane][23] = si;  // stash imag scale in the pad slot (row[22..23] are zero padding)
+      row[23] = si;
+    }
+    for (int k = 0; k < 24; k++) X[wave][lane][k] = row[k];
+  }
+  __syncthreads();
+  if (e0 >= ne) return;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int ndd = nd * nd;
+  O *ob = out + e0 * (int64_t)(2 * ndd);
+  const int ntile = NP / 16;
+  if constexpr (PREC == 0 || PREC == 1) {
+    using V4 = typename std::conditional<PREC == 0, d4_t, f4_t>::type;
+    // A operands (fixed over N tiles): Re K=16 -> 4 steps, Im K=8 -> 2 steps
+    T are[4], aim[2];
+    for (int kb = 0; kb < 4; kb++) are[kb] = (T)X[wave][i16][kb * 4 + kq];
+    for (int kb = 0; kb < 2; kb++) aim[kb] = (T)X[wave][i16][16 + kb * 4 + kq];
+    const T *Tre = tab;
+    const T *Tim = tab + 16 * NP;
+    for (int t = 0; t < ntile; t++) {
+      const int n = t * 16 + i16;
+      V4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) {
+        const T b = Tre[(kb * 4 + kq) * NP + n];
+        if constexpr (PREC == 0)
+          cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kb], b, cre, 0, 0, 0);
+        else
+          cre = __builtin_amdgcn_mfma_f32_16x16x4f32(are[kb], b, cre, 0, 0, 0);
+      }
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++) {
+        const T b = Tim[(kb * 4 + kq) * NP + n];
+        if constexpr (PREC == 0)
+          cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kb], b, cim, 0, 0, 0);
+        else
+          cim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[kb], b, cim, 0, 0, 0);
+      }
+      if (n < ndd) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          // f64 16x16x4: D row i in lane group i%4, register i/4; f32 16x16x4: group i/4, register i%4
+          const int el = PREC == 0 ? r * 4 + kq : kq * 4 + r;
+          if (e0 + el < ne) {
+            ob[(int64_t)el * 2 * ndd + n] = (O)cre[r];
+            ob[(int64_t)el * 2 * ndd + ndd + n] = (O)cim[r];
+          }
+        }
+      }
+    }
+  } else {
+    // bf16: one K=32 step each; lanes kq = 0,1 carry K rows 0..15, kq = 2,3 zeros
+    bf8_t are, aim;
+    for (int t = 0; t < 8; t++) {
+      const int k = kq * 8 + t;
+      are[t] = (__bf16)(k < 16 ? (float)X[wave][i16][k] : 0.0f);
+      aim[t] = (__bf16)(k < 6 ? (float)X[wave][i16][16 + k] : 0.0f);
+    }
+    const __bf16 *Tre = tab;
+    const __bf16 *Tim = tab + 32 * NP;
+    for (int tt = 0; tt < ntile; tt++) {
+      const int n = tt * 16 + i16;
+      bf8_t bre, bim;
+      for (int t = 0; t < 8; t++) {
+        bre[t] = Tre[(kq * 8 + t) * NP + n];
+        bim[t] = Tim[(kq * 8 + t) * NP + n];
+      }
+      f4_t cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+      cre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(are, bre, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim, bim, cim, 0, 0, 0);
+      if (n < ndd) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int el = kq * 4 + r;
+          if (e0 + el < ne) {
+            ob[(int64_t)el * 2 * ndd + n] = (O)(cre[r] * sc[wave][el]);
+            ob[(int64_t)el * 2 * ndd + ndd + n] = (O)(cim[r] * X[wave][el][23]);
+          }
+        }
+      }
+    }
+  }
+}
+
+size_t element_tab_bytes(int prec, int p) {
+  const int nd = nd_tet_ndof(p);
+  const int NP = padded(nd);
+  if (prec == 0) return (size_t)24 * NP * 8;
+  if (prec == 1) return (size_t)24 * NP * 4;
+  return (size_t)64 * NP * 2;
+}
+
+size_t element_geo_bytes(int, int64_t) { return 0; }
+
+// Host: build the padded tables in the layout the kernel expects.
+static std::vector<unsigned char> make_tab(int prec, int p) {
+  const TetTables &T = tet_tables(p);
+  const int nd = T.nd, ndd = nd * nd, NP = padded(nd);
+  const int kre = prec == 2 ? 32 : 16, kim = prec == 2 ? 32 : 8;
+  std::vector<double> re((size_t)kre * NP, 0.0), im((size_t)kim * NP, 0.0);
+  for (int c = 0; c < 6; c++)
+    for (int n = 0; n < ndd; n++) {
+      re[(size_t)c * NP + n] = T.K[(size_t)c * ndd + n];
+      re[(size_t)(6 + c) * NP + n] = T.M[(size_t)c * ndd + n];
+      im[(size_t)c * NP + n] = T.M[(size_t)c * ndd + n];
+    }
+  std::vector<unsigned char> out(element_tab_bytes(prec, p));
+  size_t k = 0;
+  auto put = [&](const std::vector<double> &v) {
+    for (double x : v) {
+      if (prec == 0) {
+        std::memcpy(&out[k], &x, 8);
+        k += 8;
+      } else if (prec == 1) {
+        float f = (float)x;
+        std::memcpy(&out[k], &f, 4);
+        k += 4;
+      } else {
+        __bf16 b = (__bf16)(float)x;
+        std::memcpy(&out[k], &b, 2);
+        k += 2;
+      }
+    }
+  };
+  put(re);
+  put(im);
+  return out;
+}
+
+void element_table(int prec, int p, void *host_out) {
+  auto t = make_tab(prec, p);
+  std::memcpy(host_out, t.data(), t.size());
+}
+
+void launch_element_matrices(hipStream_t st, int prec, int p, int64_t ne, const double *verts, const double *er,
+                             const double *tand, double k0, const void *tab, void *, void *out) {
+  const int nd = nd_tet_ndof(p), NP = padded(nd);
+  const unsigned blocks = (unsigned)((ne + 63) / 64);
+  if (prec == 0)
+    hipLaunchKernelGGL((k_elem<0>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                       (const double *)tab, (double *)out);
+  else if (prec == 1)
+    hipLaunchKernelGGL((k_elem<1>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                       (const float *)tab, (float *)out);
+  else if (prec == 2)
+    hipLaunchKernelGGL((k_elem<2>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                       (const __bf16 *)tab, (float *)out);
+  else
+    throw std::runtime_error("element matrices: bad precision");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw std::runtime_error(std::string("element kernel launch: ") + hipGetErrorString(e));
+}
+
+}  // namespace opem

@@ -1,0 +1,109 @@
+// Device kernels of the per-frequency solve path (gfx950) and their host launchers.
+// All launchers are asynchronous on the given stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace opem {
+
+// ---------------------------------------------------------------- assembly
+// Base volume matrices on the shared pattern, one wavefront per row (deterministic gather):
+//   K[a][nz], M[a][nz], Q[a][nz] for attribute slot a (unit coefficients).
+void launch_assemble_volume(hipStream_t st, int64_t ndof, const int32_t *rowptr, const int32_t *colind,
+                            const int32_t *d2e_ptr, const int32_t *d2e, const int32_t *elem_slot,
+                            const double *gm, const double *gk, const double *sdet, const double *TM,
+                            const double *TK, const double *TQ, int nd, int nslot, int64_t nnz, double *K,
+                            double *M, double *Q);
+
+// Element matrices of independent tets (assembly kernel alone), via MFMA contraction of
+// per-element geometric factors with the reference tensors.
+//   prec 0: FP64 (v_mfma_f64_16x16x4f64), 1: FP32 (v_mfma_f32_16x16x4f32), 2: BF16 inputs
+//   (v_mfma_f32_16x16x32_bf16) with per-element scaling, FP32 accumulate.
+// geo: ne x 16 factors (6 K, 6 M-real, 4 pad) and ne x 8 (6 M-imag, 2 pad) are formed inside
+// from vertices; out: ne x (2*nd*nd) {Re A_e, Im A_e}, FP64 (prec 0) or FP32.
+void launch_element_matrices(hipStream_t st, int prec, int p, int64_t ne, const double *verts,
+                             const double *er, const double *tand, double k0, const void *tab, void *geo,
+                             void *out);
+size_t element_tab_bytes(int prec, int p);
+void element_table(int prec, int p, void *host_out);
+size_t element_geo_bytes(int prec, int64_t ne);
+
+// ---------------------------------------------------------------- per-frequency values
+// A_f[nz] = sum_a cK[f][a] K[a][nz] + cM[f][a] M[a][nz]  (complex coefficients, interleaved)
+// then A_f[ov_nz[i]] += sum_t cT[f][t] ov_val[t][i].  Writes FP64 and (if a32) FP32 copies.
+void launch_combine(hipStream_t st, int nf, int64_t nnz, int nslot, const double *cK, const double *cM,
+                    const double *K, const double *M, int64_t nov, int nterm, const int32_t *ov_nz,
+                    const double *ov_val, const double *cT, double *a64, float *a32);
+
+// ---------------------------------------------------------------- SpMV
+// For matrix group g (values vals + g*vstride; vstride 0 = one shared matrix) and its R right-hand sides (systems g*R..g*R+R-1):
+//   y_s = rowmask_s .* (A_g x_s)          (mode 0)
+//   y_s = rowmask_s .* (b_s - A_g x_s)    (mode 1, residual)   b, y may alias nothing
+// rowmask: per excitation r in [0,R): uint8 N (1 = free row).  Vectors: s-major, N complex.
+template <typename V, typename X>
+void launch_spmv(hipStream_t st, int ngroups, int R, int64_t n, const int32_t *rowptr, const int32_t *colind,
+                 const V *vals, int64_t nnz, const uint8_t *rowmask, const X *x, X *y, const X *b,
+                 const int32_t *active, int mode, int lanes_per_row, int64_t vstride);
+
+// ---------------------------------------------------------------- Krylov helpers (batched, S systems)
+// partial[s][blk][i] = sum_{chunk} conj(Vb[s][i][n]) w[s][n], i < k;  vectors in basis stride ldv.
+template <typename X>
+void launch_mdot(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *w,
+                 int64_t ldw_sys, double *partial, int nblk, const int32_t *active);
+// w[s] -= sum_i Vb[s][i] h[s][i]  ; optionally partial norms^2 of the result into part2[s][blk]
+template <typename X>
+void launch_mupdate(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *w,
+                    int64_t ldw_sys, const double *h, int ldh, double *part2, int nblk, const int32_t *active);
+// out[s][i] = sum_blk partial[s][blk][i]  (fixed order) ; if acc: Hcol[s][i] += out
+void launch_reduce(hipStream_t st, int S, int nblk, int k, const double *partial, double *out, int ldo,
+                   double *Hcol, int ldH, const int32_t *active);
+// v[s] = w[s] / nrm[s]  ; nrm from nrm2[s] (sqrt taken inside), also stored into H[s][(k+1)*ldH? ]
+template <typename X>
+void launch_scale(hipStream_t st, int S, int64_t n, const X *w, int64_t ldw_sys, X *v, int64_t ldv_sys,
+                  const double *nrm2, const int32_t *active);
+
+// ---------------------------------------------------------------- block Jacobi
+// Build owned-row inverse blocks for systems s (matrix group s/R, mask r = s%R).
+// elem_nz[e][a*nd+b] = CSR position of (dof_a, dof_b); a64 = FP64 values of all matrix groups.
+template <typename P>
+void launch_bj_build(hipStream_t st, int S, int R, int nd, const int32_t *elem_dofs, const int32_t *elem_nz,
+                     const double *a64, int64_t nnz, const uint8_t *rowmask, int64_t n, const int32_t *owner_elem,
+                     P *inv_rows, const int32_t *elem_of_block, int64_t nblocks);
+// z[s][r] = sum_b inv_rows[s][r][b] v[s][elem_dofs[owner_elem[r]][b]]
+template <typename P, typename X>
+void launch_bj_apply(hipStream_t st, int S, int64_t n, int nd, const int32_t *elem_dofs,
+                     const int32_t *owner_elem, const P *inv_rows, const X *v, int64_t ldv_sys, X *z,
+                     const int32_t *active, int64_t inv_stride);
+
+// ---------------------------------------------------------------- misc vector ops (batched)
+template <typename X, typename Y>
+void launch_convert(hipStream_t st, int64_t count, const X *in, Y *out);
+// x[s] += alpha * z[s] (alpha = 1) for active systems
+template <typename X, typename Z>
+void launch_axpy(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32_t *active);
+// partial norms^2: part[s][blk]
+template <typename X>
+void launch_norm2(hipStream_t st, int S, int64_t n, const X *x, double *part, int nblk, const int32_t *active);
+int reduce_blocks(int64_t n);
+
+// ---------------------------------------------------------------- Givens / Hessenberg (device)
+// Apply previous rotations to column j of H[s] ((m+1) x m, column-major, complex), new rotation,
+// update g; est[s] = |g[j+1]|; deactivate systems whose est <= thresh[s] (records kfinal[s] = j+1).
+// nrm2[s] = |w|^2 of the new Arnoldi vector (becomes H[j+1][j]).
+void launch_givens_n(hipStream_t st, int S, int m, int j, double *H, double *cs, double *sn, double *g,
+                     double *est, const double *thresh, int32_t *active, int32_t *kfinal, const double *nrm2);
+// y[s] = R^{-1} g  (k = kfinal[s] or j+1 for still active)
+void launch_backsolve(hipStream_t st, int S, int m, const double *H, const double *g, const int32_t *kfinal,
+                      double *y, const int32_t *cycle_active);
+// u[s] = sum_{i<k_s} V[s][i] y[s][i]
+template <typename X>
+void launch_combine_basis(hipStream_t st, int S, int64_t n, int m, const X *Vb, int64_t ldv_sys,
+                          const double *y, const int32_t *kfinal, X *u, const int32_t *cycle_active);
+
+// ---------------------------------------------------------------- port projections
+// out[s][k] = sum_b conj(u[s][off_k + b]) x[s][pd[off_k + b]]  (ports concatenated with offsets)
+void launch_port_dots(hipStream_t st, int S, int nport, const int32_t *pd_off, const int32_t *pd,
+                      const double *u, int64_t ldu, const double *x, int64_t ldx, double *out);
+
+}  // namespace opem

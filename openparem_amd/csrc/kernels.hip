@@ -1,0 +1,874 @@
+// Device kernels of the per-frequency solve path for gfx950 (wave64, CDNA4).
+// See kernels.h for the contracts.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+
+namespace opem {
+
+#define HIP_LAUNCH_CHECK()                                                                         \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ complex helpers
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {  // conj(a)*b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ double2 to_d2(T v) {
+  return make_double2((double)v.x, (double)v.y);
+}
+template <typename T>
+__device__ __forceinline__ T from_d2(double2 v);
+template <>
+__device__ __forceinline__ double2 from_d2<double2>(double2 v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ float2 from_d2<float2>(double2 v) {
+  return make_float2((float)v.x, (float)v.y);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+constexpr int kRedThreads = 256;
+constexpr int kRedElems = 4;
+constexpr int kChunk = kRedThreads * kRedElems;
+
+int reduce_blocks(int64_t n) {
+  int64_t b = (n + kChunk - 1) / kChunk;
+  return (int)(b < 1 ? 1 : b);
+}
+
+// ------------------------------------------------------------------ volume assembly
+constexpr int kMaxSlot = 8;
+
+__global__ __launch_bounds__(256) void k_assemble_volume(
+    int64_t ndof, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+    const int32_t *__restrict__ d2e_ptr, const int32_t *__restrict__ d2e, const int32_t *__restrict__ elem_slot,
+    const double *__restrict__ gm, const double *__restrict__ gk, const double *__restrict__ sdet,
+    const double *__restrict__ TM, const double *__restrict__ TK, const double *__restrict__ TQ, int nd,
+    int nslot, int64_t nnz, double *__restrict__ K, double *__restrict__ M, double *__restrict__ Q) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= ndof) return;
+  const int rb = d2e_ptr[r], re = d2e_ptr[r + 1];
+  const int ndd = nd * nd;
+  for (int nz = rowptr[r] + lane; nz < rowptr[r + 1]; nz += 64) {
+    const int c = colind[nz];
+    const int cb = d2e_ptr[c], ce = d2e_ptr[c + 1];
+    double k[kMaxSlot], m[kMaxSlot], q[kMaxSlot];
+#pragma unroll
+    for (int t = 0; t < kMaxSlot; t++) k[t] = m[t] = q[t] = 0.0;
+    int i = rb, j = cb;
+    while (i < re && j < ce) {
+      const int pr = d2e[i], pc = d2e[j];
+      const int er = pr / nd, ec = pc / nd;
+      if (er < ec) {
+        i++;
+      } else if (ec < er) {
+        j++;
+      } else {
+        const int a = pr - er * nd, b = pc - ec * nd;
+        const int sl = elem_slot[er];
+        double kk = 0.0, mm = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+          kk += gk[(int64_t)er * 6 + t] * TK[t * ndd + a * nd + b];
+          mm += gm[(int64_t)er * 6 + t] * TM[t * ndd + a * nd + b];
+        }
+        const double qq = sdet[er] * TQ[a * nd + b];
+#pragma unroll
+        for (int t = 0; t < kMaxSlot; t++)
+          if (t == sl) {
+            k[t] += kk;
+            m[t] += mm;
+            q[t] += qq;
+          }
+        i++;
+        j++;
+      }
+    }
+    for (int t = 0; t < nslot; t++) {
+      K[t * nnz + nz] = k[t];
+      M[t * nnz + nz] = m[t];
+      Q[t * nnz + nz] = q[t];
+    }
+  }
+}
+
+void launch_assemble_volume(hipStream_t st, int64_t ndof, const int32_t *rowptr, const int32_t *colind,
+                            const int32_t *d2e_ptr, const int32_t *d2e, const int32_t *elem_slot,
+                            const double *gm, const double *gk, const double *sdet, const double *TM,
+                            const double *TK, const double *TQ, int nd, int nslot, int64_t nnz, double *K,
+                            double *M, double *Q) {
+  if (nslot > kMaxSlot) throw std::runtime_error("too many volume attributes (max 8)");
+  dim3 grid((unsigned)((ndof + 3) / 4));
+  hipLaunchKernelGGL(k_assemble_volume, grid, dim3(256), 0, st, ndof, rowptr, colind, d2e_ptr, d2e, elem_slot,
+                     gm, gk, sdet, TM, TK, TQ, nd, nslot, nnz, K, M, Q);
+  HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ per-frequency values
+__global__ __launch_bounds__(256) void k_combine(int nf, int64_t nnz, int nslot, const double *__restrict__ cK,
+                                                 const double *__restrict__ cM, const double *__restrict__ K,
+                                                 const double *__restrict__ M, double2 *__restrict__ a64,
+                                                 float2 *__restrict__ a32) {
+  for (int64_t nz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nz < nnz;
+       nz += (int64_t)gridDim.x * blockDim.x) {
+    double kv[kMaxSlot], mv[kMaxSlot];
+    for (int t = 0; t < nslot; t++) {
+      kv[t] = K[t * nnz + nz];
+      mv[t] = M[t * nnz + nz];
+    }
+    for (int f = 0; f < nf; f++) {
+      double re = 0.0, im = 0.0;
+      for (int t = 0; t < nslot; t++) {
+        const double *ck = cK + ((int64_t)f * nslot + t) * 2;
+        const double *cm = cM + ((int64_t)f * nslot + t) * 2;
+        re += ck[0] * kv[t] + cm[0] * mv[t];
+        im += ck[1] * kv[t] + cm[1] * mv[t];
+      }
+      a64[(int64_t)f * nnz + nz] = make_double2(re, im);
+      if (a32) a32[(int64_t)f * nnz + nz] = make_float2((float)re, (float)im);
+    }
+  }
+}
+
+__global__ void k_overlay(int nf, int64_t nnz, int64_t nov, int nterm, const int32_t *__restrict__ ov_nz,
+                          const double *__restrict__ ov_val, const double *__restrict__ cT, double2 *a64,
+                          float2 *a32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= nov) return;
+  double re = 0.0, im = 0.0;
+  for (int t = 0; t < nterm; t++) {
+    const double v = ov_val[(int64_t)t * nov + i];
+    re += cT[((int64_t)f * nterm + t) * 2] * v;
+    im += cT[((int64_t)f * nterm + t) * 2 + 1] * v;
+  }
+  const int64_t nz = (int64_t)f * nnz + ov_nz[i];
+  double2 a = a64[nz];
+  a.x += re;
+  a.y += im;
+  a64[nz] = a;
+  if (a32) a32[nz] = make_float2((float)a.x, (float)a.y);
+}
+
+void launch_combine(hipStream_t st, int nf, int64_t nnz, int nslot, const double *cK, const double *cM,
+                    const double *K, const double *M, int64_t nov, int nterm, const int32_t *ov_nz,
+                    const double *ov_val, const double *cT, double *a64, float *a32) {
+  int64_t blocks = (nnz + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)blocks), dim3(256), 0, st, nf, nnz, nslot, cK, cM, K, M,
+                     (double2 *)a64, (float2 *)a32);
+  HIP_LAUNCH_CHECK();
+  if (nov > 0 && nterm > 0) {
+    hipLaunchKernelGGL(k_overlay, dim3((unsigned)((nov + 255) / 256), nf), dim3(256), 0, st, nf, nnz, nov, nterm,
+                       ov_nz, ov_val, cT, (double2 *)a64, (float2 *)a32);
+    HIP_LAUNCH_CHECK();
+  }
+}
+
+// ------------------------------------------------------------------ SpMV
+// LPR lanes cooperate on one row; R right-hand sides share each value load.
+template <typename V, typename X, int LPR, int R>
+__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ colind, const V *__restrict__ vals,
+                                              int64_t nnz, const uint8_t *__restrict__ rowmask,
+                                              const X *__restrict__ x, X *__restrict__ y, const X *__restrict__ b,
+                                              const int32_t *__restrict__ active, int mode, int64_t vstride) {
+  const int g = blockIdx.y;
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < R; r++) any |= (active == nullptr) || active[g * R + r];
+  if (!any) return;
+  const int sub = threadIdx.x % LPR;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= n) return;
+  const V *__restrict__ A = vals + (int64_t)g * vstride;
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  double2 acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = make_double2(0.0, 0.0);
+  for (int k = beg + sub; k < end; k += LPR) {
+    const int c = colind[k];
+    const V a = A[k];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const X xv = x[((int64_t)g * R + r) * n + c];
+      acc[r].x += (double)a.x * (double)xv.x - (double)a.y * (double)xv.y;
+      acc[r].y += (double)a.x * (double)xv.y + (double)a.y * (double)xv.x;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+      acc[r].x += __shfl_xor(acc[r].x, o, LPR);
+      acc[r].y += __shfl_xor(acc[r].y, o, LPR);
+    }
+  }
+  if (sub == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t s = (int64_t)g * R + r;
+      if (active && !active[s]) continue;
+      const bool free_row = rowmask[(int64_t)r * n + row] != 0;
+      double2 out = make_double2(0.0, 0.0);
+      if (free_row) {
+        if (mode == 0) {
+          out = acc[r];
+        } else {
+          double2 bv = b ? to_d2(b[s * n + row]) : make_double2(0.0, 0.0);
+          out = make_double2(bv.x - acc[r].x, bv.y - acc[r].y);
+        }
+      }
+      y[s * n + row] = from_d2<X>(out);
+    }
+  }
+}
+
+template <typename V, typename X, int LPR>
+static void spmv_r(hipStream_t st, int ngroups, int R, int64_t n, const int32_t *rowptr, const int32_t *colind,
+                   const V *vals, int64_t nnz, const uint8_t *rowmask, const X *x, X *y, const X *b,
+                   const int32_t *active, int mode, int64_t vstride) {
+  dim3 grid((unsigned)((n * LPR + 255) / 256), ngroups);
+  switch (R) {
+    case 1:
+      hipLaunchKernelGGL((k_spmv<V, X, LPR, 1>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
+                         y, b, active, mode, vstride);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_spmv<V, X, LPR, 2>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
+                         y, b, active, mode, vstride);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_spmv<V, X, LPR, 3>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
+                         y, b, active, mode, vstride);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_spmv<V, X, LPR, 4>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
+                         y, b, active, mode, vstride);
+      break;
+    default:
+      throw std::runtime_error("spmv: R must be 1..4");
+  }
+  HIP_LAUNCH_CHECK();
+}
+
+template <typename V, typename X>
+void launch_spmv(hipStream_t st, int ngroups, int R, int64_t n, const int32_t *rowptr, const int32_t *colind,
+                 const V *vals, int64_t nnz, const uint8_t *rowmask, const X *x, X *y, const X *b,
+                 const int32_t *active, int mode, int lanes_per_row, int64_t vstride) {
+  switch (lanes_per_row) {
+    case 4:
+      spmv_r<V, X, 4>(st, ngroups, R, n, rowptr, colind, vals, nnz, rowmask, x, y, b, active, mode, vstride);
+      break;
+    case 8:
+      spmv_r<V, X, 8>(st, ngroups, R, n, rowptr, colind, vals, nnz, rowmask, x, y, b, active, mode, vstride);
+      break;
+    case 16:
+      spmv_r<V, X, 16>(st, ngroups, R, n, rowptr, colind, vals, nnz, rowmask, x, y, b, active, mode, vstride);
+      break;
+    case 32:
+      spmv_r<V, X, 32>(st, ngroups, R, n, rowptr, colind, vals, nnz, rowmask, x, y, b, active, mode, vstride);
+      break;
+    case 64:
+      spmv_r<V, X, 64>(st, ngroups, R, n, rowptr, colind, vals, nnz, rowmask, x, y, b, active, mode, vstride);
+      break;
+    default:
+      throw std::runtime_error("spmv: lanes_per_row must be 4..64");
+  }
+}
+
+template void launch_spmv<double2, double2>(hipStream_t, int, int, int64_t, const int32_t *, const int32_t *,
+                                            const double2 *, int64_t, const uint8_t *, const double2 *, double2 *,
+                                            const double2 *, const int32_t *, int, int, int64_t);
+template void launch_spmv<float2, float2>(hipStream_t, int, int, int64_t, const int32_t *, const int32_t *,
+                                          const float2 *, int64_t, const uint8_t *, const float2 *, float2 *,
+                                          const float2 *, const int32_t *, int, int, int64_t);
+
+// ------------------------------------------------------------------ Krylov helpers
+constexpr int kMaxK = 130;  // basis vectors per reduction (restart + 1)
+
+template <typename X>
+__global__ __launch_bounds__(kRedThreads) void k_mdot(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
+                                                      const X *__restrict__ w, int64_t ldw_sys,
+                                                      double *__restrict__ partial, int nblk,
+                                                      const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  __shared__ double2 red[kRedThreads / 64][kMaxK];
+  const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const X *ws = w + s * ldw_sys;
+  const X *Vs = Vb + s * ldv_sys;
+  double2 wl[kRedElems];
+#pragma unroll
+  for (int e = 0; e < kRedElems; e++) {
+    const int64_t i = base + e * kRedThreads;
+    wl[e] = i < n ? to_d2(ws[i]) : make_double2(0.0, 0.0);
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int v = 0; v < k; v++) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int e = 0; e < kRedElems; e++) {
+      const int64_t i = base + e * kRedThreads;
+      if (i < n) {
+        const double2 a = to_d2(Vs[(int64_t)v * n + i]);
+        sx += a.x * wl[e].x + a.y * wl[e].y;
+        sy += a.x * wl[e].y - a.y * wl[e].x;
+      }
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane == 0) red[wave][v] = make_double2(sx, sy);
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < k; v += kRedThreads) {
+    double2 t = red[0][v];
+    for (int q = 1; q < kRedThreads / 64; q++) {
+      t.x += red[q][v].x;
+      t.y += red[q][v].y;
+    }
+    double *p = partial + (((int64_t)s * nblk + blockIdx.x) * kMaxK + v) * 2;
+    p[0] = t.x;
+    p[1] = t.y;
+  }
+}
+
+template <typename X>
+void launch_mdot(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *w,
+                 int64_t ldw_sys, double *partial, int nblk, const int32_t *active) {
+  if (k > kMaxK) throw std::runtime_error("restart too large");
+  hipLaunchKernelGGL((k_mdot<X>), dim3(nblk, S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, w, ldw_sys, partial,
+                     nblk, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_mdot<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, const double2 *,
+                                   int64_t, double *, int, const int32_t *);
+template void launch_mdot<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, const float2 *, int64_t,
+                                  double *, int, const int32_t *);
+
+template <typename X>
+__global__ __launch_bounds__(kRedThreads) void k_mupdate(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
+                                                         X *__restrict__ w, int64_t ldw_sys,
+                                                         const double *__restrict__ h, int ldh,
+                                                         double *__restrict__ part2, int nblk,
+                                                         const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  __shared__ double2 hs[kMaxK];
+  __shared__ double red[kRedThreads / 64];
+  for (int v = threadIdx.x; v < k; v += kRedThreads)
+    hs[v] = make_double2(h[((int64_t)s * ldh + v) * 2], h[((int64_t)s * ldh + v) * 2 + 1]);
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  X *ws = w + s * ldw_sys;
+  const X *Vs = Vb + s * ldv_sys;
+  double nrm = 0.0;
+#pragma unroll
+  for (int e = 0; e < kRedElems; e++) {
+    const int64_t i = base + e * kRedThreads;
+    if (i < n) {
+      double2 acc = to_d2(ws[i]);
+      for (int v = 0; v < k; v++) {
+        const double2 a = to_d2(Vs[(int64_t)v * n + i]);
+        const double2 c = cmul(a, hs[v]);
+        acc.x -= c.x;
+        acc.y -= c.y;
+      }
+      const X out = from_d2<X>(acc);
+      ws[i] = out;
+      const double2 o = to_d2(out);
+      nrm += o.x * o.x + o.y * o.y;
+    }
+  }
+  if (part2) {
+    nrm = wave_sum(nrm);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int q = 0; q < kRedThreads / 64; q++) t += red[q];
+      part2[(int64_t)s * nblk + blockIdx.x] = t;
+    }
+  }
+}
+
+template <typename X>
+void launch_mupdate(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *w, int64_t ldw_sys,
+                    const double *h, int ldh, double *part2, int nblk, const int32_t *active) {
+  hipLaunchKernelGGL((k_mupdate<X>), dim3(nblk, S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, w, ldw_sys, h, ldh,
+                     part2, nblk, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_mupdate<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, double2 *, int64_t,
+                                      const double *, int, double *, int, const int32_t *);
+template void launch_mupdate<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, float2 *, int64_t,
+                                     const double *, int, double *, int, const int32_t *);
+
+// partial layout for k>=1 complex: [s][blk][kMaxK] double2 ; for norms (k == 0): [s][blk] double
+__global__ void k_reduce(int nblk, int k, const double *__restrict__ partial, double *__restrict__ out, int ldo,
+                         double *__restrict__ Hcol, int ldH, const int32_t *__restrict__ active) {
+  const int s = blockIdx.x;
+  if (active && !active[s]) return;
+  if (k == 0) {
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int b = 0; b < nblk; b++) t += partial[(int64_t)s * nblk + b];
+      out[(int64_t)s * ldo] = t;
+    }
+    return;
+  }
+  for (int v = threadIdx.x; v < k; v += blockDim.x) {
+    double tx = 0.0, ty = 0.0;
+    for (int b = 0; b < nblk; b++) {
+      const double *p = partial + (((int64_t)s * nblk + b) * kMaxK + v) * 2;
+      tx += p[0];
+      ty += p[1];
+    }
+    out[((int64_t)s * ldo + v) * 2] = tx;
+    out[((int64_t)s * ldo + v) * 2 + 1] = ty;
+    if (Hcol) {
+      Hcol[((int64_t)s * ldH + v) * 2] += tx;
+      Hcol[((int64_t)s * ldH + v) * 2 + 1] += ty;
+    }
+  }
+}
+
+void launch_reduce(hipStream_t st, int S, int nblk, int k, const double *partial, double *out, int ldo,
+                   double *Hcol, int ldH, const int32_t *active) {
+  hipLaunchKernelGGL(k_reduce, dim3(S), dim3(128), 0, st, nblk, k, partial, out, ldo, Hcol, ldH, active);
+  HIP_LAUNCH_CHECK();
+}
+
+template <typename X>
+__global__ void k_scale(int64_t n, const X *__restrict__ w, int64_t ldw_sys, X *__restrict__ v, int64_t ldv_sys,
+                        const double *__restrict__ nrm2, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const double nr = sqrt(nrm2[s]);
+  const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = to_d2(w[s * ldw_sys + i]);
+    v[s * ldv_sys + i] = from_d2<X>(make_double2(a.x * inv, a.y * inv));
+  }
+}
+
+template <typename X>
+void launch_scale(hipStream_t st, int S, int64_t n, const X *w, int64_t ldw_sys, X *v, int64_t ldv_sys,
+                  const double *nrm2, const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_scale<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, w, ldw_sys, v, ldv_sys, nrm2, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_scale<double2>(hipStream_t, int, int64_t, const double2 *, int64_t, double2 *, int64_t,
+                                    const double *, const int32_t *);
+template void launch_scale<float2>(hipStream_t, int, int64_t, const float2 *, int64_t, float2 *, int64_t,
+                                   const double *, const int32_t *);
+
+// ------------------------------------------------------------------ block Jacobi
+// One wavefront per (block, system): gather nd x nd block of A (masked), invert in LDS with
+// partial pivoting (Gauss-Jordan, in place), write owned rows transposed: inv[s][b][r].
+template <typename P>
+__global__ __launch_bounds__(64) void k_bj_build(int R, int nd, const int32_t *__restrict__ elem_dofs,
+                                                 const int32_t *__restrict__ elem_nz, const double2 *__restrict__ a64,
+                                                 int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
+                                                 const int32_t *__restrict__ owner_elem, P *__restrict__ inv,
+                                                 const int32_t *__restrict__ elem_of_block) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds;                                   // nd*nd
+  double2 *colk = lds + nd * nd;                      // nd
+  int *perm = (int *)(lds + nd * nd + nd);            // nd
+  const int s = blockIdx.y;
+  const int e = elem_of_block[blockIdx.x];
+  const int g = s / R, r = s % R;
+  const int lane = threadIdx.x;
+  const int32_t *dofs = elem_dofs + (int64_t)e * nd;
+  const double2 *Ag = a64 + (int64_t)g * nnz;
+  const uint8_t *mask = rowmask + (int64_t)r * n;
+  const int ndd = nd * nd;
+  for (int t = lane; t < ndd; t += 64) {
+    const int a = t / nd, b = t - a * nd;
+    const bool fa = mask[dofs[a]] != 0, fb = mask[dofs[b]] != 0;
+    double2 v;
+    if (fa && fb)
+      v = Ag[elem_nz[(int64_t)e * ndd + t]];
+    else
+      v = make_double2(a == b ? 1.0 : 0.0, 0.0);
+    A[t] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < nd; k++) {
+    // pivot search in column k, rows k..nd-1
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + lane; i < nd; i += 64) {
+      const double2 v = A[i * nd + k];
+      const double m = v.x * v.x + v.y * v.y;
+      if (m > best) {
+        best = m;
+        bi = i;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) perm[k] = bi;
+    if (bi != k)
+      for (int c = lane; c < nd; c += 64) {
+        const double2 t0 = A[k * nd + c];
+        A[k * nd + c] = A[bi * nd + c];
+        A[bi * nd + c] = t0;
+      }
+    __syncthreads();
+    const double2 piv = A[k * nd + k];
+    const double pm = piv.x * piv.x + piv.y * piv.y;
+    const double2 pinv = pm > 0.0 ? make_double2(piv.x / pm, -piv.y / pm) : make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int c = lane; c < nd; c += 64) A[k * nd + c] = (c == k) ? pinv : cmul(A[k * nd + c], pinv);
+    for (int i = lane; i < nd; i += 64) colk[i] = A[i * nd + k];
+    __syncthreads();
+    for (int t = lane; t < ndd; t += 64) {
+      const int i = t / nd, c = t - i * nd;
+      if (i == k) continue;
+      const double2 f = colk[i];
+      if (c == k)
+        A[t] = make_double2(-(f.x * pinv.x - f.y * pinv.y), -(f.x * pinv.y + f.y * pinv.x));
+      else {
+        const double2 u = cmul(f, A[k * nd + c]);
+        A[t] = make_double2(A[t].x - u.x, A[t].y - u.y);
+      }
+    }
+    __syncthreads();
+  }
+  // undo column permutation (reverse order)
+  for (int k = nd - 1; k >= 0; k--) {
+    const int pk = perm[k];
+    if (pk != k)
+      for (int i = lane; i < nd; i += 64) {
+        const double2 t0 = A[i * nd + k];
+        A[i * nd + k] = A[i * nd + pk];
+        A[i * nd + pk] = t0;
+      }
+    __syncthreads();
+  }
+  // owned rows
+  for (int t = lane; t < ndd; t += 64) {
+    const int a = t / nd, b = t - a * nd;
+    const int dof = dofs[a];
+    if (owner_elem[dof] != e) continue;
+    // first occurrence of the DOF in the element is its owner slot (a DOF appears once per element)
+    inv[((int64_t)s * nd + b) * n + dof] = from_d2<P>(A[a * nd + b]);
+  }
+}
+
+template <typename P>
+void launch_bj_build(hipStream_t st, int S, int R, int nd, const int32_t *elem_dofs, const int32_t *elem_nz,
+                     const double *a64, int64_t nnz, const uint8_t *rowmask, int64_t n, const int32_t *owner_elem,
+                     P *inv_rows, const int32_t *elem_of_block, int64_t nblocks) {
+  size_t lds = (size_t)(nd * nd + nd) * sizeof(double2) + nd * sizeof(int) + 16;
+  hipLaunchKernelGGL((k_bj_build<P>), dim3((unsigned)nblocks, S), dim3(64), lds, st, R, nd, elem_dofs, elem_nz,
+                     (const double2 *)a64, nnz, rowmask, n, owner_elem, inv_rows, elem_of_block);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_bj_build<double2>(hipStream_t, int, int, int, const int32_t *, const int32_t *, const double *,
+                                       int64_t, const uint8_t *, int64_t, const int32_t *, double2 *,
+                                       const int32_t *, int64_t);
+template void launch_bj_build<float2>(hipStream_t, int, int, int, const int32_t *, const int32_t *, const double *,
+                                      int64_t, const uint8_t *, int64_t, const int32_t *, float2 *, const int32_t *,
+                                      int64_t);
+
+template <typename P, typename X>
+__global__ __launch_bounds__(256) void k_bj_apply(int64_t n, int nd, const int32_t *__restrict__ elem_dofs,
+                                                  const int32_t *__restrict__ owner_elem, const P *__restrict__ inv,
+                                                  const X *__restrict__ v, int64_t ldv_sys, X *__restrict__ z,
+                                                  const int32_t *__restrict__ active, int64_t inv_stride) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t *dofs = elem_dofs + (int64_t)owner_elem[r] * nd;
+  const X *vs = v + s * ldv_sys;
+  const P *is = inv + (int64_t)s * inv_stride;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int b = 0; b < nd; b++) {
+    const double2 p = to_d2(is[(int64_t)b * n + r]);
+    const double2 x = to_d2(vs[dofs[b]]);
+    acc.x += p.x * x.x - p.y * x.y;
+    acc.y += p.x * x.y + p.y * x.x;
+  }
+  z[s * n + r] = from_d2<X>(acc);
+}
+
+template <typename P, typename X>
+void launch_bj_apply(hipStream_t st, int S, int64_t n, int nd, const int32_t *elem_dofs, const int32_t *owner_elem,
+                     const P *inv_rows, const X *v, int64_t ldv_sys, X *z, const int32_t *active, int64_t inv_stride) {
+  hipLaunchKernelGGL((k_bj_apply<P, X>), dim3((unsigned)((n + 255) / 256), S), dim3(256), 0, st, n, nd, elem_dofs,
+                     owner_elem, inv_rows, v, ldv_sys, z, active, inv_stride);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_bj_apply<double2, double2>(hipStream_t, int, int64_t, int, const int32_t *, const int32_t *,
+                                                const double2 *, const double2 *, int64_t, double2 *,
+                                                const int32_t *, int64_t);
+template void launch_bj_apply<float2, float2>(hipStream_t, int, int64_t, int, const int32_t *, const int32_t *,
+                                              const float2 *, const float2 *, int64_t, float2 *, const int32_t *,
+                                              int64_t);
+
+// ------------------------------------------------------------------ misc vector ops
+template <typename X, typename Y>
+__global__ void k_convert(int64_t count, const X *__restrict__ in, Y *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = from_d2<Y>(to_d2(in[i]));
+}
+template <typename X, typename Y>
+void launch_convert(hipStream_t st, int64_t count, const X *in, Y *out) {
+  int64_t bl = (count + 255) / 256;
+  if (bl > 16384) bl = 16384;
+  if (bl < 1) bl = 1;
+  hipLaunchKernelGGL((k_convert<X, Y>), dim3((unsigned)bl), dim3(256), 0, st, count, in, out);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_convert<double2, float2>(hipStream_t, int64_t, const double2 *, float2 *);
+template void launch_convert<float2, double2>(hipStream_t, int64_t, const float2 *, double2 *);
+template void launch_convert<double2, double2>(hipStream_t, int64_t, const double2 *, double2 *);
+
+template <typename X, typename Z>
+__global__ void k_axpy(int64_t n, const Z *__restrict__ z, X *__restrict__ x, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = to_d2(x[s * n + i]);
+    const double2 b = to_d2(z[s * n + i]);
+    x[s * n + i] = from_d2<X>(make_double2(a.x + b.x, a.y + b.y));
+  }
+}
+template <typename X, typename Z>
+void launch_axpy(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_axpy<X, Z>), dim3((unsigned)bl, S), dim3(256), 0, st, n, z, x, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_axpy<double2, double2>(hipStream_t, int, int64_t, const double2 *, double2 *, const int32_t *);
+template void launch_axpy<double2, float2>(hipStream_t, int, int64_t, const float2 *, double2 *, const int32_t *);
+
+template <typename X>
+__global__ __launch_bounds__(kRedThreads) void k_norm2(int64_t n, const X *__restrict__ x, double *__restrict__ part,
+                                                       int nblk, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  __shared__ double red[kRedThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  double t = 0.0;
+#pragma unroll
+  for (int e = 0; e < kRedElems; e++) {
+    const int64_t i = base + e * kRedThreads;
+    if (i < n) {
+      const double2 a = to_d2(x[s * n + i]);
+      t += a.x * a.x + a.y * a.y;
+    }
+  }
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double u = 0.0;
+    for (int q = 0; q < kRedThreads / 64; q++) u += red[q];
+    part[(int64_t)s * nblk + blockIdx.x] = u;
+  }
+}
+template <typename X>
+void launch_norm2(hipStream_t st, int S, int64_t n, const X *x, double *part, int nblk, const int32_t *active) {
+  hipLaunchKernelGGL((k_norm2<X>), dim3(nblk, S), dim3(kRedThreads), 0, st, n, x, part, nblk, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_norm2<double2>(hipStream_t, int, int64_t, const double2 *, double *, int, const int32_t *);
+template void launch_norm2<float2>(hipStream_t, int, int64_t, const float2 *, double *, int, const int32_t *);
+
+// ------------------------------------------------------------------ Givens / Hessenberg
+__global__ void k_givens(int S, int m, int j, double2 *__restrict__ H, double *__restrict__ cs,
+                         double2 *__restrict__ sn, double2 *__restrict__ g, double *__restrict__ est,
+                         const double *__restrict__ thresh, int32_t *__restrict__ active,
+                         int32_t *__restrict__ kfinal, const double *__restrict__ nrm2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || !active[s]) return;
+  double2 *h = H + ((int64_t)s * m + j) * (m + 1);
+  h[j + 1] = make_double2(sqrt(nrm2[s]), 0.0);
+  double *c = cs + (int64_t)s * m;
+  double2 *sv = sn + (int64_t)s * m;
+  for (int i = 0; i < j; i++) {
+    const double2 a = h[i], b = h[i + 1];
+    const double2 sb = cmul(sv[i], b);
+    const double2 t = make_double2(c[i] * a.x + sb.x, c[i] * a.y + sb.y);
+    const double2 csa = cconjmul(sv[i], a);  // conj(s)*a
+    h[i + 1] = make_double2(-csa.x + c[i] * b.x, -csa.y + c[i] * b.y);
+    h[i] = t;
+  }
+  const double2 a = h[j], b = h[j + 1];
+  const double na = sqrt(a.x * a.x + a.y * a.y), nb = sqrt(b.x * b.x + b.y * b.y);
+  double cc;
+  double2 ss, rr;
+  if (nb == 0.0) {
+    cc = 1.0;
+    ss = make_double2(0.0, 0.0);
+    rr = a;
+  } else if (na == 0.0) {
+    cc = 0.0;
+    ss = make_double2(b.x / nb, -b.y / nb);  // conj(b)/|b|
+    rr = make_double2(nb, 0.0);
+  } else {
+    const double nr = sqrt(na * na + nb * nb);
+    cc = na / nr;
+    const double2 ph = make_double2(a.x / na, a.y / na);
+    ss = cmul(ph, make_double2(b.x / nr, -b.y / nr));  // (a/|a|) conj(b)/r
+    rr = make_double2(ph.x * nr, ph.y * nr);
+  }
+  c[j] = cc;
+  sv[j] = ss;
+  h[j] = rr;
+  h[j + 1] = make_double2(0.0, 0.0);
+  double2 *gs = g + (int64_t)s * (m + 1);
+  const double2 gj = gs[j];
+  const double2 csg = cconjmul(ss, gj);
+  gs[j + 1] = make_double2(-csg.x, -csg.y);
+  gs[j] = make_double2(cc * gj.x, cc * gj.y);
+  const double e = sqrt(gs[j + 1].x * gs[j + 1].x + gs[j + 1].y * gs[j + 1].y);
+  est[s] = e;
+  const bool breakdown = !(nrm2[s] > 0.0);
+  if (e <= thresh[s] || breakdown || j == m - 1) {
+    kfinal[s] = j + 1;
+    active[s] = 0;
+  }
+}
+
+void launch_givens_n(hipStream_t st, int S, int m, int j, double *H, double *cs, double *sn, double *g,
+                     double *est, const double *thresh, int32_t *active, int32_t *kfinal, const double *nrm2) {
+  hipLaunchKernelGGL(k_givens, dim3((S + 63) / 64), dim3(64), 0, st, S, m, j, (double2 *)H, cs, (double2 *)sn,
+                     (double2 *)g, est, thresh, active, kfinal, nrm2);
+  HIP_LAUNCH_CHECK();
+}
+
+__global__ void k_backsolve(int S, int m, const double2 *__restrict__ H, const double2 *__restrict__ g,
+                            const int32_t *__restrict__ kfinal, double2 *__restrict__ y,
+                            const int32_t *__restrict__ cycle_active) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || !cycle_active[s]) return;
+  const int k = kfinal[s];
+  const double2 *Hs = H + (int64_t)s * m * (m + 1);
+  const double2 *gs = g + (int64_t)s * (m + 1);
+  double2 *ys = y + (int64_t)s * m;
+  for (int i = k - 1; i >= 0; i--) {
+    double2 t = gs[i];
+    for (int l = i + 1; l < k; l++) {
+      const double2 u = cmul(Hs[(int64_t)l * (m + 1) + i], ys[l]);
+      t.x -= u.x;
+      t.y -= u.y;
+    }
+    const double2 d = Hs[(int64_t)i * (m + 1) + i];
+    const double dm = d.x * d.x + d.y * d.y;
+    ys[i] = dm > 0.0 ? cmul(t, make_double2(d.x / dm, -d.y / dm)) : make_double2(0.0, 0.0);
+  }
+  for (int i = k; i < m; i++) ys[i] = make_double2(0.0, 0.0);
+}
+
+void launch_backsolve(hipStream_t st, int S, int m, const double *H, const double *g, const int32_t *kfinal,
+                      double *y, const int32_t *cycle_active) {
+  hipLaunchKernelGGL(k_backsolve, dim3((S + 63) / 64), dim3(64), 0, st, S, m, (const double2 *)H,
+                     (const double2 *)g, kfinal, (double2 *)y, cycle_active);
+  HIP_LAUNCH_CHECK();
+}
+
+template <typename X>
+__global__ void k_combine_basis(int64_t n, int m, const X *__restrict__ Vb, int64_t ldv_sys,
+                                const double2 *__restrict__ y, const int32_t *__restrict__ kfinal, X *__restrict__ u,
+                                const int32_t *__restrict__ cycle_active) {
+  const int s = blockIdx.y;
+  if (!cycle_active[s]) return;
+  const int k = kfinal[s];
+  __shared__ double2 ys[kMaxK];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) ys[i] = y[(int64_t)s * m + i];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int v = 0; v < k; v++) {
+      const double2 c = cmul(to_d2(Vb[s * ldv_sys + (int64_t)v * n + i]), ys[v]);
+      acc.x += c.x;
+      acc.y += c.y;
+    }
+    u[s * n + i] = from_d2<X>(acc);
+  }
+}
+
+template <typename X>
+void launch_combine_basis(hipStream_t st, int S, int64_t n, int m, const X *Vb, int64_t ldv_sys, const double *y,
+                          const int32_t *kfinal, X *u, const int32_t *cycle_active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_combine_basis<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, m, Vb, ldv_sys,
+                     (const double2 *)y, kfinal, u, cycle_active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_combine_basis<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t,
+                                            const double *, const int32_t *, double2 *, const int32_t *);
+template void launch_combine_basis<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, const double *,
+                                           const int32_t *, float2 *, const int32_t *);
+
+// ------------------------------------------------------------------ port projections
+__global__ void k_port_dots(int nport, const int32_t *__restrict__ pd_off, const int32_t *__restrict__ pd,
+                            const double2 *__restrict__ u, int64_t ldu, const double2 *__restrict__ x, int64_t ldx,
+                            double2 *__restrict__ out) {
+  const int s = blockIdx.y, k = blockIdx.x;
+  __shared__ double red[2][4];
+  const int b0 = pd_off[k], b1 = pd_off[k + 1];
+  double sx = 0.0, sy = 0.0;
+  for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const double2 c = cconjmul(u[s * ldu + i], x[s * ldx + pd[i]]);
+    sx += c.x;
+    sy += c.y;
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = sx;
+    red[1][threadIdx.x >> 6] = sy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tx = 0.0, ty = 0.0;
+    for (int q = 0; q < 4; q++) {
+      tx += red[0][q];
+      ty += red[1][q];
+    }
+    out[(int64_t)s * nport + k] = make_double2(tx, ty);
+  }
+}
+
+void launch_port_dots(hipStream_t st, int S, int nport, const int32_t *pd_off, const int32_t *pd, const double *u,
+                      int64_t ldu, const double *x, int64_t ldx, double *out) {
+  hipLaunchKernelGGL(k_port_dots, dim3(nport, S), dim3(256), 0, st, nport, pd_off, pd, (const double2 *)u, ldu,
+                     (const double2 *)x, ldx, (double2 *)out);
+  HIP_LAUNCH_CHECK();
+}
+
+}  // namespace opem

@@ -1,0 +1,1436 @@
+// Host orchestration of the per-frequency solve path and the C ABI (include/opem.h).
+//
+// Per frequency (OpenParEM3D.cpp:373-437) the complex system is a fixed linear combination of
+// base matrices on one sparsity pattern:
+//   A(w) = sum_attr (1/mu_r) K_attr - k0^2 er (1 - j tand) M_attr
+//        + sum_ports [ 1/(mu_r gamma) Ks_port,attr - k0^2 er(1-j tand)/gamma Ms_port,attr ]   (port.cpp:5783-5890)
+//        + sum_Zs j w mu0 / R_s  Ms_bdr  + sum_rad j w mu0 / eta Ms_bdr                         (port.cpp:1009-1041)
+// The ABC of the driven port touches only that port's DOFs, which the excitation eliminates
+// (solveComplexLinearSystem.c:115-129), so one A(w) per frequency serves every excitation; each
+// excitation differs only in its Dirichlet row mask (PEC U driven-port DOFs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/opem.h"
+#include "kernels.h"
+#include "nd_tables.h"
+#include "quadrature.h"
+#include "space.h"
+
+namespace opem {
+
+static const double kMu0 = 4e-7 * M_PI;
+static const double kEps0 = 8.8541878176e-12;  // OpenParEM3D.cpp:216
+
+#define HIPCHK(x)                                                                                 \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    if (b <= bytes && p) return;
+    release();
+    if (b == 0) b = 16;
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+  template <typename T>
+  void upload(const std::vector<T> &v, hipStream_t st) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+};
+
+struct Material {
+  double er = 1.0, tand = 0.0, mur = 1.0;
+};
+struct Bdr {
+  int kind = 0;
+  double value = 0.0;
+};
+
+struct PortTrace {
+  int attr = 0;
+  std::vector<int64_t> faces;
+  std::vector<int32_t> dofs;            // sorted global DOFs on the port
+  std::vector<uint8_t> pec_fixed;       // per port DOF: shared with a PEC boundary
+  std::vector<int32_t> loc;             // faces*ndf -> port-local index
+  // trace mass matrix (port-local CSR)
+  std::vector<int32_t> mp_ptr, mp_col;
+  std::vector<double> mp_val;
+  // quadrature
+  int64_t npts = 0;
+  std::vector<double> pts;              // npts*3
+  std::vector<double> wq;               // npts (physical weights)
+  std::vector<double> phi;              // npts*ndf*3 basis values
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  Mesh mesh;
+  bool have_mesh = false;
+  Space sp;
+  bool have_space = false;
+  std::map<int, Material> mat;
+  std::map<int, Bdr> bdr;
+  std::vector<int> slot_attr;           // volume attribute per slot
+  std::vector<int32_t> elem_slot;
+  std::vector<PortTrace> ports;         // ascending attribute
+  // device data
+  DevBuf d_rowptr, d_colind, d_elem_dofs, d_owner_elem, d_elem_nz, d_blocks, d_K, d_M, d_Q;
+  DevBuf d_P, d_Qs, d_Pdinv, d_onesmask;
+  int64_t nblocks = 0;
+  int lpr = 16;
+};
+
+// ------------------------------------------------------------------ small kernels
+__global__ void k_scatter_vals(int S, int64_t n, int npd, const int32_t *__restrict__ pd,
+                               const double2 *__restrict__ vals, double2 *__restrict__ x) {
+  const int s = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npd; i += gridDim.x * blockDim.x)
+    x[(int64_t)s * n + pd[(int64_t)s * npd + i]] = vals[(int64_t)s * npd + i];
+}
+
+__global__ void k_real_to_c(int64_t nnz, const double *__restrict__ in, double2 *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = make_double2(in[i], 0.0);
+}
+
+__global__ void k_sum_slots(int64_t nnz, int nslot, const double *__restrict__ in, const double *__restrict__ w,
+                            double2 *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = 0.0;
+    for (int k = 0; k < nslot; k++) t += w[k] * in[(int64_t)k * nnz + i];
+    out[i] = make_double2(t, 0.0);
+  }
+}
+
+// y[s] = a[s]*x[s] + b[s]*y[s]   (complex per-system scalars; a,b may be null = 0/1)
+__global__ void k_lincomb(int64_t n, const double2 *__restrict__ a, const double2 *__restrict__ x,
+                          const double2 *__restrict__ b, double2 *__restrict__ y, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const double2 as = a ? a[s] : make_double2(1.0, 0.0);
+  const double2 bs = b ? b[s] : make_double2(1.0, 0.0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 xv = x[(int64_t)s * n + i], yv = y[(int64_t)s * n + i];
+    y[(int64_t)s * n + i] = make_double2(as.x * xv.x - as.y * xv.y + bs.x * yv.x - bs.y * yv.y,
+                                         as.x * xv.y + as.y * xv.x + bs.x * yv.y + bs.y * yv.x);
+  }
+}
+
+__global__ void k_diag_inv(int64_t n, const int32_t *__restrict__ dpos, const double2 *__restrict__ P,
+                           double *__restrict__ dinv) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dinv[i] = 1.0 / P[dpos[i]].x;
+}
+
+// z[s] = dinv .* r[s]  (point Jacobi, symmetric: used by CG on the SPD mass matrix)
+__global__ void k_diag_apply(int64_t n, const double *__restrict__ dinv, const double2 *__restrict__ r,
+                             double2 *__restrict__ z, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = dinv[i];
+    const double2 v = r[(int64_t)s * n + i];
+    z[(int64_t)s * n + i] = make_double2(d * v.x, d * v.y);
+  }
+}
+
+// CG scalar updates. mode 0: alpha = rz/pq, nalpha = -alpha.  mode 1: beta = rz_new/rz, rz = rz_new.
+__global__ void k_cg_scalars(int S, int mode, double2 *__restrict__ rz, const double2 *__restrict__ t,
+                             double2 *__restrict__ alpha, double2 *__restrict__ nalpha,
+                             const int32_t *__restrict__ active) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || (active && !active[s])) return;
+  if (mode == 0) {
+    const double den = t[s].x;
+    const double a = den != 0.0 ? rz[s].x / den : 0.0;
+    alpha[s] = make_double2(a, 0.0);
+    nalpha[s] = make_double2(-a, 0.0);
+  } else {
+    const double b = rz[s].x != 0.0 ? t[s].x / rz[s].x : 0.0;
+    alpha[s] = make_double2(b, 0.0);
+    rz[s] = t[s];
+  }
+}
+
+static inline int blocks_for(int64_t n, int cap = 4096) {
+  int64_t b = (n + 255) / 256;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ------------------------------------------------------------------ host helpers
+static int choose_lpr(const Space &s) {
+  const double avg = (double)s.nnz / (double)std::max<int64_t>(1, s.ndof);
+  if (avg <= 12) return 8;
+  if (avg <= 48) return 16;
+  if (avg <= 100) return 32;
+  return 64;
+}
+
+// triangle basis values (tangential, physical) at barycentric L for face with sorted vertices v
+static void tri_basis_eval(int p, const double v[3][3], const double *L, double *out /* ndf*3 */) {
+  static thread_local std::vector<BasisFn> basis;
+  static thread_local int bp = -1;
+  if (bp != p) {
+    basis = nd_tri_basis(p);
+    bp = p;
+  }
+  double e1[3], e2[3], n[3];
+  for (int d = 0; d < 3; d++) {
+    e1[d] = v[1][d] - v[0][d];
+    e2[d] = v[2][d] - v[0][d];
+  }
+  n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  const double a2 = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int d = 0; d < 3; d++) n[d] /= a2;
+  double g[3][3];
+  // grad l1 = (e2 x n)/a2 ; grad l2 = (n x e1)/a2 ; grad l0 = -(g1+g2)
+  g[1][0] = (e2[1] * n[2] - e2[2] * n[1]) / a2;
+  g[1][1] = (e2[2] * n[0] - e2[0] * n[2]) / a2;
+  g[1][2] = (e2[0] * n[1] - e2[1] * n[0]) / a2;
+  g[2][0] = (n[1] * e1[2] - n[2] * e1[1]) / a2;
+  g[2][1] = (n[2] * e1[0] - n[0] * e1[2]) / a2;
+  g[2][2] = (n[0] * e1[1] - n[1] * e1[0]) / a2;
+  for (int d = 0; d < 3; d++) g[0][d] = -(g[1][d] + g[2][d]);
+  for (size_t a = 0; a < basis.size(); a++) {
+    const BasisFn &f = basis[a];
+    double la = 1.0;
+    for (int k = 0; k < 3; k++)
+      for (int q = 0; q < f.alpha[k]; q++) la *= L[k];
+    for (int d = 0; d < 3; d++) out[a * 3 + d] = la * (L[f.i] * g[f.j][d] - L[f.j] * g[f.i][d]);
+  }
+}
+
+static void build_ports(Ctx &c) {
+  c.ports.clear();
+  const Space &s = c.sp;
+  const int ndf = s.ndf;
+  std::vector<uint8_t> is_pec(s.ndof, 0);
+  for (int64_t f = 0; f < c.mesh.nbf; f++) {
+    auto it = c.bdr.find(c.mesh.bface_attr[f]);
+    if (it != c.bdr.end() && it->second.kind == OPEM_BDR_PEC)
+      for (int k = 0; k < ndf; k++) is_pec[s.bface_dofs[f * ndf + k]] = 1;
+  }
+  std::vector<double> Lq, wq;
+  tri_rule(s.p + 2, Lq, wq);
+  const int nq = (int)wq.size();
+  for (auto &kv : c.bdr) {
+    if (kv.second.kind != OPEM_BDR_PORT) continue;
+    PortTrace P;
+    P.attr = kv.first;
+    for (int64_t f = 0; f < c.mesh.nbf; f++)
+      if (c.mesh.bface_attr[f] == P.attr) P.faces.push_back(f);
+    if (P.faces.empty()) throw std::runtime_error("port attribute has no faces");
+    for (int64_t f : P.faces)
+      for (int k = 0; k < ndf; k++) P.dofs.push_back(s.bface_dofs[f * ndf + k]);
+    std::sort(P.dofs.begin(), P.dofs.end());
+    P.dofs.erase(std::unique(P.dofs.begin(), P.dofs.end()), P.dofs.end());
+    const int np = (int)P.dofs.size();
+    P.pec_fixed.resize(np);
+    for (int i = 0; i < np; i++) P.pec_fixed[i] = is_pec[P.dofs[i]];
+    P.loc.resize(P.faces.size() * ndf);
+    for (size_t fi = 0; fi < P.faces.size(); fi++)
+      for (int k = 0; k < ndf; k++) {
+        int32_t d = s.bface_dofs[P.faces[fi] * ndf + k];
+        P.loc[fi * ndf + k] = (int32_t)(std::lower_bound(P.dofs.begin(), P.dofs.end(), d) - P.dofs.begin());
+      }
+    // trace mass matrix (exact) and quadrature data
+    std::vector<std::map<int, double>> rows(np);
+    std::vector<double> Kf(ndf * ndf), Mf(ndf * ndf);
+    P.npts = (int64_t)P.faces.size() * nq;
+    P.pts.resize(P.npts * 3);
+    P.wq.resize(P.npts);
+    P.phi.resize(P.npts * ndf * 3);
+    for (size_t fi = 0; fi < P.faces.size(); fi++) {
+      const int64_t f = P.faces[fi];
+      face_matrices(c.mesh, s, f, Kf.data(), Mf.data());
+      for (int a = 0; a < ndf; a++)
+        for (int b = 0; b < ndf; b++) rows[P.loc[fi * ndf + a]][P.loc[fi * ndf + b]] += Mf[a * ndf + b];
+      double v[3][3];
+      for (int k = 0; k < 3; k++)
+        for (int d = 0; d < 3; d++) v[k][d] = c.mesh.xyz[(int64_t)s.bf_sorted[f * 3 + k] * 3 + d];
+      double e1[3], e2[3];
+      for (int d = 0; d < 3; d++) {
+        e1[d] = v[1][d] - v[0][d];
+        e2[d] = v[2][d] - v[0][d];
+      }
+      const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                   cz = e1[0] * e2[1] - e1[1] * e2[0];
+      const double a2 = std::sqrt(cx * cx + cy * cy + cz * cz);
+      for (int q = 0; q < nq; q++) {
+        const int64_t pi = (int64_t)fi * nq + q;
+        for (int d = 0; d < 3; d++)
+          P.pts[pi * 3 + d] = Lq[q * 3] * v[0][d] + Lq[q * 3 + 1] * v[1][d] + Lq[q * 3 + 2] * v[2][d];
+        P.wq[pi] = wq[q] * a2;
+        tri_basis_eval(s.p, v, &Lq[q * 3], &P.phi[pi * ndf * 3]);
+      }
+    }
+    P.mp_ptr.assign(np + 1, 0);
+    for (int i = 0; i < np; i++) {
+      P.mp_ptr[i + 1] = P.mp_ptr[i] + (int)rows[i].size();
+      for (auto &e : rows[i]) {
+        P.mp_col.push_back(e.first);
+        P.mp_val.push_back(e.second);
+      }
+    }
+    c.ports.push_back(std::move(P));
+  }
+}
+
+// complex CG on the (real SPD) port trace mass matrix restricted to free DOFs
+static void port_mass_solve(const PortTrace &P, const std::vector<cplx> &b, std::vector<cplx> &x, bool fix_pec) {
+  const int n = (int)P.dofs.size();
+  std::vector<uint8_t> fr(n, 1);
+  if (fix_pec)
+    for (int i = 0; i < n; i++) fr[i] = !P.pec_fixed[i];
+  auto mv = [&](const std::vector<cplx> &in, std::vector<cplx> &out) {
+    for (int i = 0; i < n; i++) {
+      cplx t = 0;
+      if (fr[i])
+        for (int k = P.mp_ptr[i]; k < P.mp_ptr[i + 1]; k++)
+          if (fr[P.mp_col[k]]) t += P.mp_val[k] * in[P.mp_col[k]];
+      out[i] = t;
+    }
+  };
+  std::vector<double> dg(n, 1.0);
+  for (int i = 0; i < n; i++)
+    for (int k = P.mp_ptr[i]; k < P.mp_ptr[i + 1]; k++)
+      if (P.mp_col[k] == i) dg[i] = P.mp_val[k];
+  x.assign(n, 0.0);
+  std::vector<cplx> r(n), z(n), p(n), q(n);
+  double bn = 0;
+  for (int i = 0; i < n; i++) {
+    r[i] = fr[i] ? b[i] : 0.0;
+    bn += std::norm(r[i]);
+  }
+  bn = std::sqrt(bn);
+  if (bn == 0) return;
+  for (int i = 0; i < n; i++) z[i] = r[i] / dg[i];
+  p = z;
+  cplx rz = 0;
+  for (int i = 0; i < n; i++) rz += std::conj(r[i]) * z[i];
+  for (int it = 0; it < 20 * n + 100; it++) {
+    mv(p, q);
+    cplx pq = 0;
+    for (int i = 0; i < n; i++) pq += std::conj(p[i]) * q[i];
+    const cplx al = rz / pq;
+    double rn = 0;
+    for (int i = 0; i < n; i++) {
+      x[i] += al * p[i];
+      r[i] -= al * q[i];
+      rn += std::norm(r[i]);
+    }
+    if (std::sqrt(rn) <= 1e-15 * bn) break;
+    for (int i = 0; i < n; i++) z[i] = r[i] / dg[i];
+    cplx rz2 = 0;
+    for (int i = 0; i < n; i++) rz2 += std::conj(r[i]) * z[i];
+    const cplx be = rz2 / rz;
+    rz = rz2;
+    for (int i = 0; i < n; i++) p[i] = z[i] + be * p[i];
+  }
+}
+
+// L2 projection of sampled tangential field values onto the port trace space
+static void port_project(const PortTrace &P, int ndf, const double *samples /* complex npts*3 */, bool fix_pec,
+                         std::vector<cplx> &out) {
+  const int n = (int)P.dofs.size();
+  std::vector<cplx> b(n, 0.0);
+  const int nq = (int)(P.npts / (int64_t)P.faces.size());
+  for (size_t fi = 0; fi < P.faces.size(); fi++)
+    for (int q = 0; q < nq; q++) {
+      const int64_t pi = (int64_t)fi * nq + q;
+      const cplx ex(samples[(pi * 3 + 0) * 2], samples[(pi * 3 + 0) * 2 + 1]);
+      const cplx ey(samples[(pi * 3 + 1) * 2], samples[(pi * 3 + 1) * 2 + 1]);
+      const cplx ez(samples[(pi * 3 + 2) * 2], samples[(pi * 3 + 2) * 2 + 1]);
+      for (int a = 0; a < ndf; a++) {
+        const double *ph = &P.phi[(pi * ndf + a) * 3];
+        b[P.loc[fi * ndf + a]] += P.wq[pi] * (ph[0] * ex + ph[1] * ey + ph[2] * ez);
+      }
+    }
+  port_mass_solve(P, b, out, fix_pec);
+}
+
+static void port_mass_apply(const PortTrace &P, const std::vector<cplx> &x, std::vector<cplx> &y) {
+  const int n = (int)P.dofs.size();
+  y.assign(n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int k = P.mp_ptr[i]; k < P.mp_ptr[i + 1]; k++) y[i] += P.mp_val[k] * x[P.mp_col[k]];
+}
+
+// boundary overlay terms (ports per adjacent material slot, impedance, radiation)
+struct Overlay {
+  std::vector<int32_t> nz;               // unique CSR positions
+  std::vector<double> val;               // nterm * nov
+  struct Term {
+    int kind;        // 0 port curl-curl, 1 port mass, 2 impedance mass, 3 radiation mass
+    int port;        // index into ports (kind 0/1)
+    int slot;        // adjacent material slot (kind 0/1)
+    int attr;        // boundary attribute
+  };
+  std::vector<Term> terms;
+};
+
+static void build_overlay(Ctx &c, Overlay &ov) {
+  const Space &s = c.sp;
+  const int ndf = s.ndf;
+  std::map<std::tuple<int, int, int>, int> term_of;  // (kind, attr, slot)
+  std::vector<std::map<int32_t, double>> tv;
+  std::vector<double> Kf(ndf * ndf), Mf(ndf * ndf);
+  std::map<int, int> port_index;
+  for (size_t i = 0; i < c.ports.size(); i++) port_index[c.ports[i].attr] = (int)i;
+  std::map<int, int> attr_slot;
+  for (size_t i = 0; i < c.slot_attr.size(); i++) attr_slot[c.slot_attr[i]] = (int)i;
+  auto term = [&](int kind, int attr, int slot) {
+    auto key = std::make_tuple(kind, attr, slot);
+    auto it = term_of.find(key);
+    if (it != term_of.end()) return it->second;
+    int id = (int)ov.terms.size();
+    Overlay::Term t{kind, kind <= 1 ? port_index[attr] : -1, slot, attr};
+    ov.terms.push_back(t);
+    tv.emplace_back();
+    term_of[key] = id;
+    return id;
+  };
+  for (int64_t f = 0; f < c.mesh.nbf; f++) {
+    auto it = c.bdr.find(c.mesh.bface_attr[f]);
+    if (it == c.bdr.end()) continue;
+    const int kind = it->second.kind;
+    if (kind != OPEM_BDR_PORT && kind != OPEM_BDR_IMPEDANCE && kind != OPEM_BDR_RADIATION) continue;
+    face_matrices(c.mesh, s, f, Kf.data(), Mf.data());
+    const int32_t *dofs = &s.bface_dofs[f * ndf];
+    auto add = [&](int t, const std::vector<double> &E) {
+      for (int a = 0; a < ndf; a++)
+        for (int b = 0; b < ndf; b++) {
+          int64_t nz = csr_find(s, dofs[a], dofs[b]);
+          if (nz < 0) throw std::runtime_error("boundary entry outside pattern");
+          tv[t][(int32_t)nz] += E[a * ndf + b];
+        }
+    };
+    if (kind == OPEM_BDR_PORT) {
+      const int slot = c.elem_slot[s.bface_owner[f]];
+      add(term(0, it->first, slot), Kf);
+      add(term(1, it->first, slot), Mf);
+    } else {
+      add(term(kind == OPEM_BDR_IMPEDANCE ? 2 : 3, it->first, 0), Mf);
+    }
+  }
+  std::vector<int32_t> all;
+  for (auto &m : tv)
+    for (auto &e : m) all.push_back(e.first);
+  std::sort(all.begin(), all.end());
+  all.erase(std::unique(all.begin(), all.end()), all.end());
+  ov.nz = all;
+  const size_t nov = all.size();
+  ov.val.assign(ov.terms.size() * nov, 0.0);
+  for (size_t t = 0; t < tv.size(); t++)
+    for (auto &e : tv[t]) {
+      size_t i = std::lower_bound(all.begin(), all.end(), e.first) - all.begin();
+      ov.val[t * nov + i] = e.second;
+    }
+}
+
+// per-frequency coefficients (OpenParEM3D.cpp:396-437, port.cpp:1009-1041,5783-5890)
+static void frequency_coefs(const Ctx &c, const Overlay &ov, double freq, const std::vector<cplx> &gamma_by_port,
+                            std::vector<double> &cK, std::vector<double> &cM, std::vector<double> &cT) {
+  const double k0 = 2 * M_PI * freq * std::sqrt(kMu0 * kEps0);
+  const double w = 2 * M_PI * freq;
+  const int ns = (int)c.slot_attr.size();
+  for (int sl = 0; sl < ns; sl++) {
+    const Material &m = c.mat.at(c.slot_attr[sl]);
+    const cplx ec(m.er, -m.er * m.tand);
+    const cplx kk = 1.0 / m.mur, mm = -k0 * k0 * ec;
+    cK.push_back(kk.real());
+    cK.push_back(kk.imag());
+    cM.push_back(mm.real());
+    cM.push_back(mm.imag());
+  }
+  for (auto &t : ov.terms) {
+    cplx v = 0;
+    if (t.kind <= 1) {
+      const Material &m = c.mat.at(c.slot_attr[t.slot]);
+      const cplx g = gamma_by_port[t.port];
+      const cplx ec(m.er, -m.er * m.tand);
+      v = t.kind == 0 ? cplx(1.0 / m.mur) / g : (-k0 * k0 * ec) / g;
+    } else if (t.kind == 2) {
+      const double sigma = c.bdr.at(t.attr).value;
+      const double rs = std::sqrt(M_PI * freq * kMu0 / sigma);  // OpenParEMmaterials.cpp:860 (mur = 1, no roughness)
+      v = cplx(0.0, w * kMu0 / rs);
+    } else {
+      double eta = c.bdr.at(t.attr).value;
+      if (eta <= 0) eta = std::sqrt(M_PI * 4e-7 / kEps0);  // port.cpp:725-727 default
+      v = cplx(0.0, w * kMu0 / eta);
+    }
+    cT.push_back(v.real());
+    cT.push_back(v.imag());
+  }
+}
+
+// ------------------------------------------------------------------ space build on device
+static void device_space(Ctx &c) {
+  Space &s = c.sp;
+  hipStream_t st = c.st;
+  // slots
+  c.slot_attr.clear();
+  std::map<int, int> aslot;
+  for (int64_t e = 0; e < c.mesh.nt; e++) aslot[c.mesh.tet_attr[e]] = 0;
+  for (auto &kv : aslot) {
+    kv.second = (int)c.slot_attr.size();
+    c.slot_attr.push_back(kv.first);
+  }
+  c.elem_slot.resize(c.mesh.nt);
+  for (int64_t e = 0; e < c.mesh.nt; e++) c.elem_slot[e] = aslot[c.mesh.tet_attr[e]];
+  const int nslot = (int)c.slot_attr.size();
+  const int nd = s.nd;
+  const TetTables &T = tet_tables(s.p);
+  // element -> CSR position map, blocks owning DOFs
+  std::vector<int32_t> elem_nz((size_t)c.mesh.nt * nd * nd);
+  {
+    const int64_t nt = c.mesh.nt;
+    auto work = [&](int64_t lo, int64_t hi) {
+      for (int64_t e = lo; e < hi; e++)
+        for (int a = 0; a < nd; a++)
+          for (int b = 0; b < nd; b++)
+            elem_nz[(e * nd + a) * nd + b] = (int32_t)csr_find(s, s.elem_dofs[e * nd + a], s.elem_dofs[e * nd + b]);
+    };
+    std::vector<std::thread> th;
+    int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    int64_t ch = (nt + nth - 1) / nth;
+    for (int t = 0; t < nth; t++) th.emplace_back(work, t * ch, std::min(nt, (t + 1) * ch));
+    for (auto &t : th) t.join();
+  }
+  std::vector<uint8_t> owns(c.mesh.nt, 0);
+  for (int64_t r = 0; r < s.ndof; r++) owns[s.owner_elem[r]] = 1;
+  std::vector<int32_t> blocks;
+  for (int64_t e = 0; e < c.mesh.nt; e++)
+    if (owns[e]) blocks.push_back((int32_t)e);
+  c.nblocks = (int64_t)blocks.size();
+  std::vector<double> sdet(c.mesh.nt);
+  for (int64_t e = 0; e < c.mesh.nt; e++) sdet[e] = s.detj[e] > 0 ? 1.0 : -1.0;
+  c.d_rowptr.upload(s.rowptr, st);
+  c.d_colind.upload(s.colind, st);
+  c.d_elem_dofs.upload(s.elem_dofs, st);
+  c.d_owner_elem.upload(s.owner_elem, st);
+  c.d_elem_nz.upload(elem_nz, st);
+  c.d_blocks.upload(blocks, st);
+  DevBuf d_d2e_ptr, d_d2e, d_slot, d_gm, d_gk, d_sdet, d_TM, d_TK, d_TQ;
+  d_d2e_ptr.upload(s.d2e_ptr, st);
+  d_d2e.upload(s.d2e, st);
+  d_slot.upload(c.elem_slot, st);
+  d_gm.upload(s.gm, st);
+  d_gk.upload(s.gk, st);
+  d_sdet.upload(sdet, st);
+  d_TM.upload(T.M, st);
+  d_TK.upload(T.K, st);
+  d_TQ.upload(T.Q, st);
+  const int64_t nnz = s.nnz;
+  c.d_K.alloc((size_t)nslot * nnz * 8);
+  c.d_M.alloc((size_t)nslot * nnz * 8);
+  c.d_Q.alloc((size_t)nslot * nnz * 8);
+  launch_assemble_volume(st, s.ndof, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), d_d2e_ptr.as<int32_t>(),
+                         d_d2e.as<int32_t>(), d_slot.as<int32_t>(), d_gm.as<double>(), d_gk.as<double>(),
+                         d_sdet.as<double>(), d_TM.as<double>(), d_TK.as<double>(), d_TQ.as<double>(), nd, nslot, nnz,
+                         c.d_K.as<double>(), c.d_M.as<double>(), c.d_Q.as<double>());
+  // P = sum of unit masses (complex storage), block-Jacobi inverse of P (shared by all systems)
+  c.d_P.alloc((size_t)nnz * 16);
+  std::vector<double> ones(nslot, 1.0);
+  DevBuf d_w;
+  d_w.upload(ones, st);
+  hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for(nnz, 65536)), dim3(256), 0, st, nnz, nslot, c.d_M.as<double>(),
+                     d_w.as<double>(), c.d_P.as<double2>());
+  std::vector<uint8_t> ones_mask(s.ndof, 1);
+  c.d_onesmask.upload(ones_mask, st);
+  // point-Jacobi inverse diagonal of P: symmetric, as CG requires
+  std::vector<int32_t> dpos(s.ndof);
+  for (int64_t r = 0; r < s.ndof; r++) dpos[r] = (int32_t)csr_find(s, (int32_t)r, (int32_t)r);
+  DevBuf d_dpos;
+  d_dpos.upload(dpos, st);
+  c.d_Pdinv.alloc((size_t)s.ndof * 8);
+  hipLaunchKernelGGL(k_diag_inv, dim3(blocks_for(s.ndof)), dim3(256), 0, st, s.ndof, d_dpos.as<int32_t>(),
+                     c.d_P.as<double2>(), c.d_Pdinv.as<double>());
+  c.lpr = choose_lpr(s);
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+// ------------------------------------------------------------------ batched solve
+struct Work {
+  // sized per batch
+  DevBuf A64, A32, masks, x, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
+      hbuf, nrm2, inv, pdv, pd, bn;
+};
+
+template <typename X>
+struct VecT;
+template <>
+struct VecT<double2> {
+  using P = double2;
+};
+template <>
+struct VecT<float2> {
+  using P = float2;
+};
+
+struct BatchResult {
+  std::vector<double> xh;  // not used
+};
+
+// GMRES(m) with element-block-Jacobi right preconditioning, restarted with the true residual in
+// FP64 at every cycle (for X = float2 this is FP32 GMRES inside FP64 iterative refinement).
+template <typename X>
+static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double tol, std::vector<int> &iters,
+                        std::vector<double> &relres, std::vector<int> &conv, const std::vector<double> &bnorm) {
+  using P = typename VecT<X>::P;
+  const Space &s = c.sp;
+  hipStream_t st = c.st;
+  const int64_t n = s.ndof, nnz = s.nnz;
+  const int nd = s.nd;
+  const int nblk = reduce_blocks(n);
+  const bool mixed = std::is_same<X, float2>::value;
+  const X *vals = mixed ? (const X *)W.A32.p : (const X *)W.A64.p;
+  // preconditioner
+  W.inv.alloc((size_t)S * nd * n * sizeof(P));
+  launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(), nnz,
+                     W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(), c.d_blocks.as<int32_t>(),
+                     c.nblocks);
+  const int64_t ldv = (int64_t)(m + 1) * n;
+  W.V.alloc((size_t)S * ldv * sizeof(X));
+  W.w.alloc((size_t)S * n * sizeof(X));
+  W.z.alloc((size_t)S * n * sizeof(X));
+  W.u.alloc((size_t)S * n * sizeof(X));
+  W.r.alloc((size_t)S * n * 16);
+  W.Hh.alloc((size_t)S * m * (m + 1) * 16);
+  W.cs.alloc((size_t)S * m * 8);
+  W.sn.alloc((size_t)S * m * 16);
+  W.g.alloc((size_t)S * (m + 1) * 16);
+  W.y.alloc((size_t)S * m * 16);
+  W.est.alloc((size_t)S * 8);
+  W.thresh.alloc((size_t)S * 8);
+  W.active.alloc((size_t)S * 4);
+  W.cyc_active.alloc((size_t)S * 4);
+  W.kfinal.alloc((size_t)S * 4);
+  W.partial.alloc((size_t)S * nblk * 130 * 16);
+  W.hbuf.alloc((size_t)S * 130 * 16);
+  W.nrm2.alloc((size_t)S * 8);
+  std::vector<double> rn2(S), thresh(S);
+  std::vector<int32_t> act(S), kf(S);
+  iters.assign(S, 0);
+  relres.assign(S, 0.0);
+  conv.assign(S, 0);
+  const double2 *x64 = W.x.as<double2>();
+  while (true) {
+    // true residual in FP64: r = mask .* (-A x)
+    launch_spmv<double2, double2>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
+                                  W.A64.as<double2>(), nnz, W.masks.as<uint8_t>(), x64, W.r.as<double2>(), nullptr,
+                                  nullptr, 1, c.lpr, nnz);
+    launch_norm2<double2>(st, S, n, W.r.as<double2>(), W.partial.as<double>(), nblk, nullptr);
+    launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0, nullptr);
+    HIPCHK(hipMemcpyAsync(rn2.data(), W.nrm2.p, S * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool any = false;
+    for (int i = 0; i < S; i++) {
+      const double rn = std::sqrt(rn2[i]);
+      relres[i] = bnorm[i] > 0 ? rn / bnorm[i] : 0.0;
+      const bool done = rn <= tol * bnorm[i] || iters[i] >= maxit;
+      conv[i] = rn <= tol * bnorm[i];
+      act[i] = !done;
+      any |= !done;
+      // inner target: the outer tolerance, or 1e-5 of the current residual in FP32
+      thresh[i] = std::max(tol * bnorm[i], (mixed ? 1e-5 : 0.0) * rn) * (mixed ? 1.0 : 0.5);
+    }
+    if (!any) break;
+    HIPCHK(hipMemcpyAsync(W.active.p, act.data(), S * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(W.cyc_active.p, act.data(), S * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(W.thresh.p, thresh.data(), S * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(W.Hh.p, 0, (size_t)S * m * (m + 1) * 16, st));
+    // g = (||r||, 0, ...), V0 = r/||r||
+    {
+      std::vector<double> g0((size_t)S * (m + 1) * 2, 0.0);
+      for (int i = 0; i < S; i++) g0[(size_t)i * (m + 1) * 2] = std::sqrt(rn2[i]);
+      HIPCHK(hipMemcpyAsync(W.g.p, g0.data(), g0.size() * 8, hipMemcpyHostToDevice, st));
+    }
+    if (mixed) {
+      launch_convert<double2, float2>(st, (int64_t)S * n, W.r.as<double2>(), (float2 *)W.w.p);
+      launch_scale<X>(st, S, n, W.w.as<X>(), n, W.V.as<X>(), ldv, W.nrm2.as<double>(), W.active.as<int32_t>());
+    } else {
+      launch_scale<X>(st, S, n, (const X *)W.r.p, n, W.V.as<X>(), ldv, W.nrm2.as<double>(), W.active.as<int32_t>());
+    }
+    int j = 0;
+    for (j = 0; j < m; j++) {
+      X *Vj = W.V.as<X>() + (int64_t)j * n;
+      launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
+                            Vj, ldv, W.z.as<X>(), W.active.as<int32_t>(), (int64_t)nd * n);
+      launch_spmv<X, X>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), vals, nnz,
+                        W.masks.as<uint8_t>(), W.z.as<X>(), W.w.as<X>(), nullptr, W.active.as<int32_t>(), 0, c.lpr,
+                        nnz);
+      double *Hcol = W.Hh.as<double>() + (int64_t)j * (m + 1) * 2;
+      const int ldH = m * (m + 1);
+      for (int pass = 0; pass < 2; pass++) {
+        launch_mdot<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.partial.as<double>(), nblk,
+                       W.active.as<int32_t>());
+        launch_reduce(st, S, nblk, j + 1, W.partial.as<double>(), W.hbuf.as<double>(), 130, Hcol, ldH,
+                      W.active.as<int32_t>());
+        launch_mupdate<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.hbuf.as<double>(), 130,
+                          pass == 1 ? W.partial.as<double>() : nullptr, nblk, W.active.as<int32_t>());
+      }
+      launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0,
+                    W.active.as<int32_t>());
+      launch_givens_n(st, S, m, j, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
+                      W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
+                      W.nrm2.as<double>());
+      if (j + 1 < m)
+        launch_scale<X>(st, S, n, W.w.as<X>(), n, W.V.as<X>() + (int64_t)(j + 1) * n, ldv, W.nrm2.as<double>(),
+                        W.active.as<int32_t>());
+      if ((j & 7) == 7) {
+        HIPCHK(hipMemcpyAsync(act.data(), W.active.p, S * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        bool a = false;
+        for (int i = 0; i < S; i++) a |= act[i] != 0;
+        if (!a) break;
+      }
+    }
+    // x += M^{-1} (V y)
+    launch_backsolve(st, S, m, W.Hh.as<double>(), W.g.as<double>(), W.kfinal.as<int32_t>(), W.y.as<double>(),
+                     W.cyc_active.as<int32_t>());
+    launch_combine_basis<X>(st, S, n, m, W.V.as<X>(), ldv, W.y.as<double>(), W.kfinal.as<int32_t>(), W.u.as<X>(),
+                            W.cyc_active.as<int32_t>());
+    launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
+                          W.u.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>(), (int64_t)nd * n);
+    launch_axpy<double2, X>(st, S, n, W.z.as<X>(), W.x.as<double2>(), W.cyc_active.as<int32_t>());
+    HIPCHK(hipMemcpyAsync(kf.data(), W.kfinal.p, S * 4, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> ca(S);
+    HIPCHK(hipMemcpyAsync(ca.data(), W.cyc_active.p, S * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < S; i++)
+      if (ca[i]) iters[i] += kf[i];
+  }
+}
+
+// H = P^{-1} (j/w) Qs x  by batched CG with point Jacobi on P (fem3D.cpp:610-648)
+static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, double htol, DevBuf &h) {
+  const Space &s = c.sp;
+  hipStream_t st = c.st;
+  const int64_t n = s.ndof, nnz = s.nnz;
+  const int nd = s.nd;
+  const int nblk = reduce_blocks(n);
+  DevBuf b, r, z, p, q, rz, t, al, nal, act, part, part2;
+  b.alloc((size_t)S * n * 16);
+  r.alloc((size_t)S * n * 16);
+  z.alloc((size_t)S * n * 16);
+  p.alloc((size_t)S * n * 16);
+  q.alloc((size_t)S * n * 16);
+  h.alloc((size_t)S * n * 16);
+  rz.alloc(S * 16);
+  t.alloc(S * 16);
+  al.alloc(S * 16);
+  nal.alloc(S * 16);
+  act.alloc(S * 4);
+  part.alloc((size_t)S * nblk * 130 * 16);
+  part2.alloc((size_t)S * 16);
+  // b = Qs x, then scale by j/w per system
+  launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), c.d_Qs.as<double2>(),
+                                nnz, c.d_onesmask.as<uint8_t>(), W.x.as<double2>(), b.as<double2>(), nullptr, nullptr,
+                                0, c.lpr, 0);
+  std::vector<double> sc(S * 2);
+  for (int i = 0; i < S; i++) {
+    sc[i * 2] = 0.0;
+    sc[i * 2 + 1] = 1.0 / omega[i];
+  }
+  DevBuf dsc;
+  dsc.upload(sc, st);
+  std::vector<double> zero(S * 2, 0.0);
+  DevBuf dzero;
+  dzero.upload(zero, st);
+  hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, dsc.as<double2>(), b.as<double2>(),
+                     dzero.as<double2>(), b.as<double2>(), nullptr);
+  // h = 0, r = b, z = M r, p = z, rz = r.z
+  HIPCHK(hipMemsetAsync(h.p, 0, (size_t)S * n * 16, st));
+  HIPCHK(hipMemcpyAsync(r.p, b.p, (size_t)S * n * 16, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(),
+                     r.as<double2>(), z.as<double2>(), nullptr);
+  HIPCHK(hipMemcpyAsync(p.p, z.p, (size_t)S * n * 16, hipMemcpyDeviceToDevice, st));
+  launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nblk, nullptr);
+  launch_reduce(st, S, nblk, 1, part.as<double>(), rz.as<double>(), 1, nullptr, 0, nullptr);
+  // ||b||
+  launch_norm2<double2>(st, S, n, b.as<double2>(), part.as<double>(), nblk, nullptr);
+  launch_reduce(st, S, nblk, 0, part.as<double>(), part2.as<double>(), 1, nullptr, 0, nullptr);
+  std::vector<double> bn2(S), rn2(S);
+  HIPCHK(hipMemcpyAsync(bn2.data(), part2.p, S * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<int32_t> a(S, 1);
+  for (int i = 0; i < S; i++) a[i] = bn2[i] > 0;
+  HIPCHK(hipMemcpyAsync(act.p, a.data(), S * 4, hipMemcpyHostToDevice, st));
+  for (int it = 0; it < 20000; it++) {
+    launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
+                                  c.d_P.as<double2>(), nnz, c.d_onesmask.as<uint8_t>(), p.as<double2>(),
+                                  q.as<double2>(), nullptr, act.as<int32_t>(), 0, c.lpr, 0);
+    launch_mdot<double2>(st, S, n, 1, p.as<double2>(), n, q.as<double2>(), n, part.as<double>(), nblk,
+                         act.as<int32_t>());
+    launch_reduce(st, S, nblk, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
+    hipLaunchKernelGGL(k_cg_scalars, dim3((S + 63) / 64), dim3(64), 0, st, S, 0, rz.as<double2>(), t.as<double2>(),
+                       al.as<double2>(), nal.as<double2>(), act.as<int32_t>());
+    hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, al.as<double2>(), p.as<double2>(),
+                       nullptr, h.as<double2>(), act.as<int32_t>());
+    hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, nal.as<double2>(), q.as<double2>(),
+                       nullptr, r.as<double2>(), act.as<int32_t>());
+    if ((it % 8) == 7) {
+      launch_norm2<double2>(st, S, n, r.as<double2>(), part.as<double>(), nblk, act.as<int32_t>());
+      launch_reduce(st, S, nblk, 0, part.as<double>(), part2.as<double>(), 1, nullptr, 0, act.as<int32_t>());
+      HIPCHK(hipMemcpyAsync(rn2.data(), part2.p, S * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      bool any = false;
+      for (int i = 0; i < S; i++) {
+        if (a[i] && rn2[i] <= htol * htol * bn2[i]) a[i] = 0;
+        any |= a[i] != 0;
+      }
+      HIPCHK(hipMemcpyAsync(act.p, a.data(), S * 4, hipMemcpyHostToDevice, st));
+      if (!any) break;
+    }
+    hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(),
+                       r.as<double2>(), z.as<double2>(), act.as<int32_t>());
+    launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nblk,
+                         act.as<int32_t>());
+    launch_reduce(st, S, nblk, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
+    hipLaunchKernelGGL(k_cg_scalars, dim3((S + 63) / 64), dim3(64), 0, st, S, 1, rz.as<double2>(), t.as<double2>(),
+                       al.as<double2>(), nal.as<double2>(), act.as<int32_t>());
+    hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, nullptr, z.as<double2>(),
+                       al.as<double2>(), p.as<double2>(), act.as<int32_t>());
+  }
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void calculate_s_host(int N, const cplx *Cp, const cplx *Cm, const cplx *V, const cplx *Z, cplx *S) {
+  // port.cpp:7499-7606: row (i,j): sum_k Cm[i][k] w_k S[j][k] = Cp[i][j] w_j, w = V/sqrt(Z)
+  const int n = N * N;
+  std::vector<cplx> a((size_t)n * n, 0.0), b(n, 0.0);
+  std::vector<cplx> w(N);
+  for (int k = 0; k < N; k++) w[k] = V[k] / std::sqrt(Z[k]);
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++) {
+      b[i * N + j] = Cp[i * N + j] * w[j];
+      for (int k = 0; k < N; k++) a[(size_t)(i * N + j) * n + (j * N + k)] = Cm[i * N + k] * w[k];
+    }
+  // LU with partial pivoting
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::abs(a[(size_t)i * n + k]) > std::abs(a[(size_t)p * n + k])) p = i;
+    if (std::abs(a[(size_t)p * n + k]) == 0.0) throw std::runtime_error("singular S system");
+    if (p != k) {
+      for (int c = 0; c < n; c++) std::swap(a[(size_t)k * n + c], a[(size_t)p * n + c]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < n; i++) {
+      const cplx f = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+      for (int c = k; c < n; c++) a[(size_t)i * n + c] -= f * a[(size_t)k * n + c];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    cplx t = b[i];
+    for (int c = i + 1; c < n; c++) t -= a[(size_t)i * n + c] * S[c];
+    S[i] = t / a[(size_t)i * n + i];
+  }
+}
+
+static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
+  if (!c.have_space) throw std::invalid_argument("space not built");
+  const Space &s = c.sp;
+  hipStream_t st = c.st;
+  for (int sl = 0; sl < (int)c.slot_attr.size(); sl++)
+    if (!c.mat.count(c.slot_attr[sl])) throw std::invalid_argument("material not set for a volume attribute");
+  build_ports(c);
+  const int np = in->nports;
+  if (np < 1 || np > 4) throw std::invalid_argument("nports must be 1..4");
+  std::vector<int> port_of(np);
+  for (int k = 0; k < np; k++) {
+    port_of[k] = -1;
+    for (size_t i = 0; i < c.ports.size(); i++)
+      if (c.ports[i].attr == in->port_attr[k]) port_of[k] = (int)i;
+    if (port_of[k] < 0) throw std::invalid_argument("port_attr is not an OPEM_BDR_PORT boundary");
+  }
+  if ((int)c.ports.size() != np) throw std::invalid_argument("every port boundary must be an S-port");
+  Overlay ov;
+  build_overlay(c, ov);
+  DevBuf d_ovnz, d_ovval;
+  d_ovnz.upload(ov.nz, st);
+  d_ovval.upload(ov.val, st);
+  const int64_t n = s.ndof, nnz = s.nnz;
+  const int nslot = (int)c.slot_attr.size();
+  // Qs = sum_slot Q_slot / (mu0 mur_slot): H rhs = (j/w) Qs x
+  {
+    std::vector<double> wq(nslot);
+    for (int sl = 0; sl < nslot; sl++) wq[sl] = 1.0 / (kMu0 * c.mat.at(c.slot_attr[sl]).mur);
+    DevBuf dwq;
+    dwq.upload(wq, st);
+    c.d_Qs.alloc((size_t)nnz * 16);
+    hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for(nnz, 65536)), dim3(256), 0, st, nnz, nslot, c.d_Q.as<double>(),
+                       dwq.as<double>(), c.d_Qs.as<double2>());
+  }
+  // Dirichlet masks per excitation: PEC U driven port
+  std::vector<uint8_t> pec(n, 0);
+  for (int64_t f = 0; f < c.mesh.nbf; f++) {
+    auto it = c.bdr.find(c.mesh.bface_attr[f]);
+    if (it != c.bdr.end() && it->second.kind == OPEM_BDR_PEC)
+      for (int k = 0; k < s.ndf; k++) pec[s.bface_dofs[f * s.ndf + k]] = 1;
+  }
+  const int R = np;
+  std::vector<uint8_t> masks((size_t)R * n);
+  for (int r = 0; r < R; r++) {
+    for (int64_t i = 0; i < n; i++) masks[(size_t)r * n + i] = !pec[i];
+    for (int32_t d : c.ports[port_of[r]].dofs) masks[(size_t)r * n + d] = 0;
+  }
+  Work W;
+  W.masks.upload(masks, st);
+  // mode sample offsets
+  std::vector<int64_t> moff(np + 1, 0);
+  for (int k = 0; k < np; k++) moff[k + 1] = moff[k] + c.ports[port_of[k]].npts * 3;
+  const int64_t mstride = moff[np];
+  // port DOF lists for projections (S-port order), concatenated
+  std::vector<int32_t> pd_off(np + 1, 0), pd;
+  for (int k = 0; k < np; k++) {
+    const auto &P = c.ports[port_of[k]];
+    pd.insert(pd.end(), P.dofs.begin(), P.dofs.end());
+    pd_off[k + 1] = (int32_t)pd.size();
+  }
+  DevBuf d_pdoff, d_pd;
+  d_pdoff.upload(pd_off, st);
+  d_pd.upload(pd, st);
+  const int nf = in->nfreq;
+  const int batch = std::max(1, std::min(in->batch > 0 ? in->batch : nf, nf));
+  const int m = std::max(2, std::min(in->restart > 0 ? in->restart : 50, 128));
+  const double tol = in->tol > 0 ? in->tol : 1e-10;
+  const double htol = in->h_tol > 0 ? in->h_tol : 1e-13;
+  HIPCHK(hipStreamSynchronize(st));
+  const double t0 = now_s();
+  double t_setup = 0, t_solve = 0, t_post = 0;
+  for (int f0 = 0; f0 < nf; f0 += batch) {
+    const int F = std::min(batch, nf - f0);
+    const int S = F * R;
+    const double ta = now_s();
+    // modal data per frequency: projected E (Dirichlet values), projected H; u vectors
+    std::vector<std::vector<cplx>> mE((size_t)F * np), mH((size_t)F * np);
+    std::vector<cplx> e2((size_t)F * np), h2((size_t)F * np);
+    std::vector<double> uvec((size_t)S * pd.size() * 2, 0.0);
+    for (int fi = 0; fi < F; fi++)
+      for (int k = 0; k < np; k++) {
+        const auto &P = c.ports[port_of[k]];
+        const double *Es = in->mode_E + ((int64_t)(f0 + fi) * mstride + moff[k]) * 2;
+        const double *Hs = in->mode_H + ((int64_t)(f0 + fi) * mstride + moff[k]) * 2;
+        auto &me = mE[fi * np + k];
+        auto &mh = mH[fi * np + k];
+        port_project(P, s.ndf, Es, true, me);
+        port_project(P, s.ndf, Hs, false, mh);
+        std::vector<cplx> ue, uh;
+        port_mass_apply(P, me, ue);
+        port_mass_apply(P, mh, uh);
+        cplx a = 0, b = 0;
+        for (size_t i = 0; i < me.size(); i++) {
+          a += std::conj(me[i]) * ue[i];
+          b += std::conj(mh[i]) * uh[i];
+        }
+        e2[fi * np + k] = a;
+        h2[fi * np + k] = b;
+        // u for every system of this frequency: E part goes to x-dots, H part to h-dots
+        for (int r = 0; r < R; r++) {
+          const int sidx = fi * R + r;
+          for (size_t i = 0; i < ue.size(); i++) {
+            uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2] = ue[i].real();
+            uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2 + 1] = ue[i].imag();
+          }
+        }
+        // stash uh in mh's place for later (kept in mH via a second vector)
+        mh.insert(mh.end(), uh.begin(), uh.end());
+      }
+    // coefficients and A(w)
+    std::vector<double> cK, cM, cT;
+    for (int fi = 0; fi < F; fi++) {
+      std::vector<cplx> gam(c.ports.size());
+      for (int k = 0; k < np; k++) {
+        const double *g = in->gamma + ((int64_t)(f0 + fi) * np + k) * 2;
+        gam[port_of[k]] = cplx(g[0], g[1]);
+      }
+      frequency_coefs(c, ov, in->freqs[f0 + fi], gam, cK, cM, cT);
+    }
+    DevBuf dcK, dcM, dcT;
+    dcK.upload(cK, st);
+    dcM.upload(cM, st);
+    dcT.upload(cT, st);
+    const bool mixed = in->precision == OPEM_PREC_MIXED;
+    W.A64.alloc((size_t)F * nnz * 16);
+    if (mixed) W.A32.alloc((size_t)F * nnz * 8);
+    launch_combine(st, F, nnz, nslot, dcK.as<double>(), dcM.as<double>(), c.d_K.as<double>(), c.d_M.as<double>(),
+                   (int64_t)ov.nz.size(), (int)ov.terms.size(), d_ovnz.as<int32_t>(), d_ovval.as<double>(),
+                   dcT.as<double>(), W.A64.as<double>(), mixed ? W.A32.as<float>() : nullptr);
+    // x = Dirichlet values of the driven port
+    W.x.alloc((size_t)S * n * 16);
+    HIPCHK(hipMemsetAsync(W.x.p, 0, (size_t)S * n * 16, st));
+    std::vector<double> bn(S, 0.0);
+    for (int fi = 0; fi < F; fi++)
+      for (int r = 0; r < R; r++) {
+        const auto &P = c.ports[port_of[r]];
+        const auto &me = mE[fi * np + r];
+        std::vector<double> vals(me.size() * 2);
+        double xn = 0;
+        for (size_t i = 0; i < me.size(); i++) {
+          vals[i * 2] = me[i].real();
+          vals[i * 2 + 1] = me[i].imag();
+          xn += std::norm(me[i]);
+        }
+        bn[fi * R + r] = xn;
+        DevBuf dv, dp;
+        dv.upload(vals, st);
+        dp.upload(P.dofs, st);
+        hipLaunchKernelGGL(k_scatter_vals, dim3(blocks_for((int64_t)me.size()), 1), dim3(256), 0, st, 1, n,
+                           (int)me.size(), dp.as<int32_t>(), dv.as<double2>(),
+                           W.x.as<double2>() + (int64_t)(fi * R + r) * n);
+        HIPCHK(hipStreamSynchronize(st));
+      }
+    // ||b~||^2 = ||mask .* A x_D||^2 + ||x_D||^2
+    {
+      const int nblk = reduce_blocks(n);
+      W.r.alloc((size_t)S * n * 16);
+      W.partial.alloc((size_t)S * nblk * 130 * 16);
+      W.nrm2.alloc((size_t)S * 8);
+      launch_spmv<double2, double2>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
+                                    W.A64.as<double2>(), nnz, W.masks.as<uint8_t>(), W.x.as<double2>(),
+                                    W.r.as<double2>(), nullptr, nullptr, 1, c.lpr, nnz);
+      launch_norm2<double2>(st, S, n, W.r.as<double2>(), W.partial.as<double>(), nblk, nullptr);
+      launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0, nullptr);
+      std::vector<double> an(S);
+      HIPCHK(hipMemcpyAsync(an.data(), W.nrm2.p, S * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int i = 0; i < S; i++) bn[i] = std::sqrt(bn[i] + an[i]);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    const double tb = now_s();
+    t_setup += tb - ta;
+    std::vector<int> iters, conv;
+    std::vector<double> relres;
+    if (mixed)
+      gmres_batch<float2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, iters, relres, conv, bn);
+    else
+      gmres_batch<double2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, iters, relres, conv, bn);
+    HIPCHK(hipStreamSynchronize(st));
+    const double tc = now_s();
+    t_solve += tc - tb;
+    // H projection
+    std::vector<double> omega(S);
+    for (int i = 0; i < S; i++) omega[i] = 2 * M_PI * in->freqs[f0 + i / R];
+    DevBuf h;
+    h_solve(c, W, S, omega, htol, h);
+    // projections: e0 (x with uE), h0 (h with uH)
+    DevBuf du, dout;
+    du.upload(uvec, st);
+    dout.alloc((size_t)S * np * 16);
+    launch_port_dots(st, S, np, d_pdoff.as<int32_t>(), d_pd.as<int32_t>(), du.as<double>(), (int64_t)pd.size(),
+                     W.x.as<double>(), n, dout.as<double>());
+    std::vector<double> e0((size_t)S * np * 2), h0((size_t)S * np * 2);
+    HIPCHK(hipMemcpyAsync(e0.data(), dout.p, e0.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int fi = 0; fi < F; fi++)
+      for (int r = 0; r < R; r++) {
+        const int sidx = fi * R + r;
+        for (int k = 0; k < np; k++) {
+          const auto &mh = mH[fi * np + k];
+          const size_t half = mh.size() / 2;
+          for (size_t i = 0; i < half; i++) {
+            uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2] = mh[half + i].real();
+            uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2 + 1] = mh[half + i].imag();
+          }
+        }
+      }
+    du.upload(uvec, st);
+    launch_port_dots(st, S, np, d_pdoff.as<int32_t>(), d_pd.as<int32_t>(), du.as<double>(), (int64_t)pd.size(),
+                     h.as<double>(), n, dout.as<double>());
+    HIPCHK(hipMemcpyAsync(h0.data(), dout.p, h0.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int fi = 0; fi < F; fi++) {
+      const int fg = f0 + fi;
+      std::vector<cplx> Cp(np * np), Cm(np * np), V(np), Z(np), Sm(np * np);
+      for (int k = 0; k < np; k++) {
+        const double *vv = in->V + ((int64_t)fg * np + k) * 2;
+        const double *zz = in->Z + ((int64_t)fg * np + k) * 2;
+        V[k] = cplx(vv[0], vv[1]);
+        Z[k] = cplx(zz[0], zz[1]);
+      }
+      for (int i = 0; i < R; i++)
+        for (int k = 0; k < np; k++) {
+          const int sidx = fi * R + i;
+          const cplx ee(e0[((size_t)sidx * np + k) * 2], e0[((size_t)sidx * np + k) * 2 + 1]);
+          const cplx hh(h0[((size_t)sidx * np + k) * 2], h0[((size_t)sidx * np + k) * 2 + 1]);
+          const cplx re = ee / e2[fi * np + k], rh = hh / h2[fi * np + k];
+          Cp[i * np + k] = 0.5 * (re + rh);  // port.cpp:3564-3565
+          Cm[i * np + k] = 0.5 * (re - rh);
+        }
+      calculate_s_host(np, Cp.data(), Cm.data(), V.data(), Z.data(), Sm.data());
+      for (int i = 0; i < np * np; i++) {
+        if (out->S_raw) {
+          out->S_raw[((size_t)fg * np * np + i) * 2] = Sm[i].real();
+          out->S_raw[((size_t)fg * np * np + i) * 2 + 1] = Sm[i].imag();
+        }
+        if (out->Cp) {
+          out->Cp[((size_t)fg * np * np + i) * 2] = Cp[i].real();
+          out->Cp[((size_t)fg * np * np + i) * 2 + 1] = Cp[i].imag();
+          out->Cm[((size_t)fg * np * np + i) * 2] = Cm[i].real();
+          out->Cm[((size_t)fg * np * np + i) * 2 + 1] = Cm[i].imag();
+        }
+      }
+      for (int i = 0; i < np; i++)
+        for (int j = 0; j < np; j++) {
+          cplx v = Sm[i * np + j];
+          if (in->reciprocal) v = 0.5 * (Sm[i * np + j] + Sm[j * np + i]);  // results.cpp:401-419
+          out->S[((size_t)fg * np * np + i * np + j) * 2] = v.real();
+          out->S[((size_t)fg * np * np + i * np + j) * 2 + 1] = v.imag();
+        }
+      for (int r = 0; r < R; r++) {
+        const int sidx = fi * R + r;
+        if (out->iters) out->iters[(size_t)fg * np + r] = iters[sidx];
+        if (out->true_res) out->true_res[(size_t)fg * np + r] = relres[sidx];
+        if (out->converged) out->converged[(size_t)fg * np + r] = conv[sidx];
+      }
+    }
+    if (out->x_out)
+      HIPCHK(hipMemcpyAsync(out->x_out + (size_t)f0 * R * n * 2, W.x.p, (size_t)S * n * 16, hipMemcpyDeviceToHost, st));
+    if (out->h_out)
+      HIPCHK(hipMemcpyAsync(out->h_out + (size_t)f0 * R * n * 2, h.p, (size_t)S * n * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    t_post += now_s() - tc;
+  }
+  out->seconds_total = now_s() - t0;
+  out->seconds_setup = t_setup;
+  out->seconds_solve = t_solve;
+  out->seconds_post = t_post;
+}
+
+}  // namespace opem
+
+// ====================================================================== C ABI
+using namespace opem;
+
+struct opem_ctx : public Ctx {};
+
+template <class F>
+static int guarded(opem_ctx *ctx, F f) {
+  if (!ctx) return -1;
+  try {
+    HIPCHK(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    ctx->err = e.what();
+    return -2;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    return 1;
+  }
+}
+
+extern "C" {
+
+int opem_create(opem_ctx **ctx, int device) {
+  if (!ctx) return -1;
+  *ctx = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return 2;
+  if (device < 0 || device >= ndev) return -2;
+  opem_ctx *c = new (std::nothrow) opem_ctx();
+  if (!c) return 3;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return 4;
+  }
+  *ctx = c;
+  return 0;
+}
+
+int opem_destroy(opem_ctx *ctx) {
+  if (!ctx) return -1;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->st);
+  hipStream_t st = ctx->st;
+  delete ctx;
+  (void)hipStreamDestroy(st);
+  return 0;
+}
+
+const char *opem_last_error(const opem_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int opem_mesh_set(opem_ctx *ctx, int64_t nv, const double *xyz, int64_t nt, const int32_t *tets,
+                  const int32_t *tet_attr, int64_t nbf, const int32_t *bfaces, const int32_t *bface_attr) {
+  return guarded(ctx, [&] {
+    if (nv <= 0 || nt <= 0 || nbf < 0 || !xyz || !tets || !tet_attr || (nbf > 0 && (!bfaces || !bface_attr)))
+      throw std::invalid_argument("bad mesh arguments");
+    Mesh &m = ctx->mesh;
+    m.nv = nv;
+    m.nt = nt;
+    m.nbf = nbf;
+    m.xyz.assign(xyz, xyz + nv * 3);
+    m.tets.assign(tets, tets + nt * 4);
+    m.tet_attr.assign(tet_attr, tet_attr + nt);
+    m.bfaces.assign(bfaces, bfaces + nbf * 3);
+    m.bface_attr.assign(bface_attr, bface_attr + nbf);
+    ctx->have_mesh = true;
+    ctx->have_space = false;
+  });
+}
+
+int opem_space_build(opem_ctx *ctx, int order, int64_t *ndof, int64_t *nnz) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_mesh) throw std::invalid_argument("mesh not set");
+    if (order < 1 || order > 3) throw std::invalid_argument("order must be 1..3");
+    int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    build_space(ctx->mesh, order, ctx->sp, nth);
+    device_space(*ctx);
+    ctx->have_space = true;
+    if (ndof) *ndof = ctx->sp.ndof;
+    if (nnz) *nnz = ctx->sp.nnz;
+  });
+}
+
+int opem_space_get(opem_ctx *ctx, int32_t *elem_dofs, int32_t *bface_dofs, int32_t *rowptr, int32_t *colind) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    const Space &s = ctx->sp;
+    if (elem_dofs) std::memcpy(elem_dofs, s.elem_dofs.data(), s.elem_dofs.size() * 4);
+    if (bface_dofs) std::memcpy(bface_dofs, s.bface_dofs.data(), s.bface_dofs.size() * 4);
+    if (rowptr) std::memcpy(rowptr, s.rowptr.data(), s.rowptr.size() * 4);
+    if (colind) std::memcpy(colind, s.colind.data(), s.colind.size() * 4);
+  });
+}
+
+int opem_base_matrix(opem_ctx *ctx, int which, int attr, double *out) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (!out || which < 0 || which > 2) throw std::invalid_argument("bad arguments");
+    int slot = -1;
+    for (size_t i = 0; i < ctx->slot_attr.size(); i++)
+      if (ctx->slot_attr[i] == attr) slot = (int)i;
+    if (slot < 0) throw std::invalid_argument("unknown volume attribute");
+    const DevBuf &b = which == 0 ? ctx->d_K : which == 1 ? ctx->d_M : ctx->d_Q;
+    HIPCHK(hipMemcpy(out, b.as<double>() + (int64_t)slot * ctx->sp.nnz, ctx->sp.nnz * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int opem_material_set(opem_ctx *ctx, int attr, double er, double tand, double mur) {
+  return guarded(ctx, [&] {
+    if (!(er > 0) || !(mur > 0) || tand < 0) throw std::invalid_argument("bad material");
+    ctx->mat[attr] = Material{er, tand, mur};
+  });
+}
+
+int opem_boundary_set(opem_ctx *ctx, int attr, int kind, double value) {
+  return guarded(ctx, [&] {
+    if (kind < OPEM_BDR_PEC || kind > OPEM_BDR_RADIATION) throw std::invalid_argument("bad boundary kind");
+    if (kind == OPEM_BDR_IMPEDANCE && !(value > 0)) throw std::invalid_argument("conductivity must be > 0");
+    ctx->bdr[attr] = Bdr{kind, value};
+  });
+}
+
+int opem_port_points(opem_ctx *ctx, int attr, int64_t *npts, double *xyz) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    build_ports(*ctx);
+    for (auto &P : ctx->ports)
+      if (P.attr == attr) {
+        if (npts) *npts = P.npts;
+        if (xyz) std::memcpy(xyz, P.pts.data(), P.pts.size() * 8);
+        return;
+      }
+    throw std::invalid_argument("attribute is not a port");
+  });
+}
+
+int opem_system_matrix(opem_ctx *ctx, double freq, const double *gamma, double *vals) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (!vals || !(freq > 0)) throw std::invalid_argument("bad arguments");
+    for (int sl = 0; sl < (int)ctx->slot_attr.size(); sl++)
+      if (!ctx->mat.count(ctx->slot_attr[sl])) throw std::invalid_argument("material not set");
+    build_ports(*ctx);
+    Overlay ov;
+    build_overlay(*ctx, ov);
+    std::vector<cplx> gam(ctx->ports.size());
+    for (size_t k = 0; k < ctx->ports.size(); k++) gam[k] = cplx(gamma[2 * k], gamma[2 * k + 1]);
+    std::vector<double> cK, cM, cT;
+    frequency_coefs(*ctx, ov, freq, gam, cK, cM, cT);
+    DevBuf dcK, dcM, dcT, dnz, dval, A;
+    hipStream_t st = ctx->st;
+    dcK.upload(cK, st);
+    dcM.upload(cM, st);
+    dcT.upload(cT, st);
+    dnz.upload(ov.nz, st);
+    dval.upload(ov.val, st);
+    A.alloc((size_t)ctx->sp.nnz * 16);
+    launch_combine(st, 1, ctx->sp.nnz, (int)ctx->slot_attr.size(), dcK.as<double>(), dcM.as<double>(),
+                   ctx->d_K.as<double>(), ctx->d_M.as<double>(), (int64_t)ov.nz.size(), (int)ov.terms.size(),
+                   dnz.as<int32_t>(), dval.as<double>(), dcT.as<double>(), A.as<double>(), nullptr);
+    HIPCHK(hipMemcpyAsync(vals, A.p, (size_t)ctx->sp.nnz * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
+              const double *x, double *y) {
+  return guarded(ctx, [&] {
+    if (n <= 0 || !rowptr || !colind || !vals || !x || !y) throw std::invalid_argument("bad arguments");
+    const int64_t nnz = rowptr[n];
+    for (int64_t i = 0; i < n; i++)
+      if (rowptr[i + 1] < rowptr[i]) throw std::invalid_argument("rowptr not monotone");
+    for (int64_t k = 0; k < nnz; k++)
+      if (colind[k] < 0 || colind[k] >= n) throw std::invalid_argument("colind out of range");
+    hipStream_t st = ctx->st;
+    DevBuf drp, dci, dv, dx, dy, dm;
+    drp.upload(std::vector<int32_t>(rowptr, rowptr + n + 1), st);
+    dci.upload(std::vector<int32_t>(colind, colind + nnz), st);
+    dm.upload(std::vector<uint8_t>(n, 1), st);
+    const int lpr = nnz / n <= 12 ? 8 : nnz / n <= 48 ? 16 : 32;
+    if (prec == OPEM_PREC_FP64) {
+      dv.upload(std::vector<double>(vals, vals + 2 * nnz), st);
+      dx.upload(std::vector<double>(x, x + 2 * n), st);
+      dy.alloc(n * 16);
+      launch_spmv<double2, double2>(st, 1, 1, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<double2>(), nnz,
+                                    dm.as<uint8_t>(), dx.as<double2>(), dy.as<double2>(), nullptr, nullptr, 0, lpr,
+                                    nnz);
+      HIPCHK(hipMemcpyAsync(y, dy.p, n * 16, hipMemcpyDeviceToHost, st));
+    } else {
+      std::vector<float> v32(2 * nnz), x32(2 * n), y32(2 * n);
+      for (int64_t i = 0; i < 2 * nnz; i++) v32[i] = (float)vals[i];
+      for (int64_t i = 0; i < 2 * n; i++) x32[i] = (float)x[i];
+      dv.upload(v32, st);
+      dx.upload(x32, st);
+      dy.alloc(n * 8);
+      launch_spmv<float2, float2>(st, 1, 1, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<float2>(), nnz,
+                                  dm.as<uint8_t>(), dx.as<float2>(), dy.as<float2>(), nullptr, nullptr, 0, lpr, nnz);
+      HIPCHK(hipMemcpyAsync(y32.data(), dy.p, n * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int64_t i = 0; i < 2 * n; i++) y[i] = y32[i];
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
+  return guarded(ctx, [&] {
+    if (!in || !out || !out->S || in->nfreq < 1 || !in->freqs || !in->port_attr || !in->mode_E || !in->mode_H ||
+        !in->gamma || !in->V || !in->Z)
+      throw std::invalid_argument("bad sweep arguments");
+    if (in->precision != OPEM_PREC_FP64 && in->precision != OPEM_PREC_MIXED)
+      throw std::invalid_argument("bad precision");
+    sweep(*ctx, in, out);
+  });
+}
+
+int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const double *verts, const double *er,
+                          const double *tand, double k0, int nrep, double *out, double *avg_seconds) {
+  return guarded(ctx, [&] {
+    if (order < 1 || order > 3 || prec < 0 || prec > 2 || ne <= 0 || !verts || !er || !tand || nrep < 1)
+      throw std::invalid_argument("bad element-matrix arguments");
+    hipStream_t st = ctx->st;
+    const int nd = nd_tet_ndof(order);
+    const size_t osz = prec == 0 ? 8 : 4;
+    DevBuf dv, de, dt, dtab, dout;
+    dv.upload(std::vector<double>(verts, verts + ne * 12), st);
+    de.upload(std::vector<double>(er, er + ne), st);
+    dt.upload(std::vector<double>(tand, tand + ne), st);
+    std::vector<unsigned char> tab(element_tab_bytes(prec, order));
+    element_table(prec, order, tab.data());
+    dtab.upload(tab, st);
+    dout.alloc((size_t)ne * 2 * nd * nd * osz);
+    launch_element_matrices(st, prec, order, ne, dv.as<double>(), de.as<double>(), dt.as<double>(), k0, dtab.p,
+                            nullptr, dout.p);  // warm-up
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, st));
+    for (int r = 0; r < nrep; r++)
+      launch_element_matrices(st, prec, order, ne, dv.as<double>(), de.as<double>(), dt.as<double>(), k0, dtab.p,
+                              nullptr, dout.p);
+    HIPCHK(hipEventRecord(b, st));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (avg_seconds) *avg_seconds = ms * 1e-3 / nrep;
+    if (out) {
+      const size_t cnt = (size_t)ne * 2 * nd * nd;
+      if (prec == 0) {
+        HIPCHK(hipMemcpy(out, dout.p, cnt * 8, hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> tmp(cnt);
+        HIPCHK(hipMemcpy(tmp.data(), dout.p, cnt * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cnt; i++) out[i] = tmp[i];
+      }
+    }
+  });
+}
+
+int opem_calculate_s(int n, const double *Cp, const double *Cm, const double *V, const double *Z, int reciprocal,
+                     double *S) {
+  if (n < 1 || n > 16 || !Cp || !Cm || !V || !Z || !S) return -1;
+  try {
+    std::vector<cplx> cp(n * n), cm(n * n), v(n), z(n), s(n * n);
+    for (int i = 0; i < n * n; i++) {
+      cp[i] = cplx(Cp[2 * i], Cp[2 * i + 1]);
+      cm[i] = cplx(Cm[2 * i], Cm[2 * i + 1]);
+    }
+    for (int i = 0; i < n; i++) {
+      v[i] = cplx(V[2 * i], V[2 * i + 1]);
+      z[i] = cplx(Z[2 * i], Z[2 * i + 1]);
+    }
+    calculate_s_host(n, cp.data(), cm.data(), v.data(), z.data(), s.data());
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        cplx t = s[i * n + j];
+        if (reciprocal) t = 0.5 * (s[i * n + j] + s[j * n + i]);
+        S[2 * (i * n + j)] = t.real();
+        S[2 * (i * n + j) + 1] = t.imag();
+      }
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""Host-side mirror of the reference's per-frequency solve path over the C ABI (libopem.so).
+
+Call surface, in the reference's order (``src/OpenParEM3D/OpenParEM3D.cpp:367-566``):
+
+    s = Solver(mesh, order)                       # fem3D::build_fe_spaces + MFEM assembly on GPU
+    r = s.sweep(freqs, precision="fp64"|"mixed")  # per frequency: build_A, eliminatePEC, fillX,
+                                                  # solveComplexLinearSystem, buildHgrids,
+                                                  # calculateSplits, calculateS, forceReciprocal
+    r["S"][f]                                     # S matrix per frequency (S-port order)
+
+Nonconvergence is reported in r["converged"], never raised (solveComplexLinearSystem.c:204-206).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BDR, check, cplx_view, f64p, i32p, i64p, lib, ptr
+from .ports import provider_for
+
+
+class Solver:
+    def __init__(self, mesh, order, device=0, materials=None, boundaries=None):
+        L = lib()
+        self.mesh = mesh
+        self.order = order
+        self.ctx = C.c_void_p()
+        rc = L.opem_create(C.byref(self.ctx), device)
+        if rc != 0:
+            raise _lib.OpemError(f"opem_create failed (rc={rc}): no usable GPU device {device}")
+        xyz = np.ascontiguousarray(mesh.xyz, np.float64)
+        tets = np.ascontiguousarray(mesh.tets, np.int32)
+        ta = np.ascontiguousarray(mesh.tet_attr, np.int32)
+        bf = np.ascontiguousarray(mesh.bfaces, np.int32)
+        ba = np.ascontiguousarray(mesh.bface_attr, np.int32)
+        check(L.opem_mesh_set(self.ctx, xyz.shape[0], ptr(xyz, C.c_double), tets.shape[0], ptr(tets, C.c_int32),
+                              ptr(ta, C.c_int32), bf.shape[0], ptr(bf, C.c_int32), ptr(ba, C.c_int32)),
+              self.ctx, "opem_mesh_set")
+        nd, nz = C.c_int64(), C.c_int64()
+        check(L.opem_space_build(self.ctx, order, C.byref(nd), C.byref(nz)), self.ctx, "opem_space_build")
+        self.ndof, self.nnz = nd.value, nz.value
+        for attr, m in (materials or mesh.materials).items():
+            check(L.opem_material_set(self.ctx, int(attr), m["er"], m["tand"], m["mur"]), self.ctx,
+                  "opem_material_set")
+        bvals = boundaries or {}
+        for attr, kind in mesh.bdr_kind.items():
+            val = bvals.get(attr, 0.0)
+            if kind == "zs" and not val:
+                val = mesh.conductivity.get(attr, 5.8e7) if hasattr(mesh, "conductivity") else 5.8e7
+            check(L.opem_boundary_set(self.ctx, int(attr), BDR[kind], float(val)), self.ctx, "opem_boundary_set")
+        self.ports = sorted(a for a, k in mesh.bdr_kind.items() if k == "port")
+
+    def close(self):
+        if self.ctx:
+            lib().opem_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ inspection
+    def space(self):
+        nt, nbf = self.mesh.nt, self.mesh.bfaces.shape[0]
+        nd = self.order * (self.order + 2) * (self.order + 3) // 2
+        ndf = self.order * (self.order + 2)
+        ed = np.empty((nt, nd), np.int32)
+        bd = np.empty((nbf, ndf), np.int32)
+        rp = np.empty(self.ndof + 1, np.int32)
+        ci = np.empty(self.nnz, np.int32)
+        check(lib().opem_space_get(self.ctx, ptr(ed, C.c_int32), ptr(bd, C.c_int32), ptr(rp, C.c_int32),
+                                   ptr(ci, C.c_int32)), self.ctx, "opem_space_get")
+        return dict(elem_dofs=ed, bface_dofs=bd, rowptr=rp, colind=ci)
+
+    def base_matrix(self, which, attr):
+        out = np.empty(self.nnz, np.float64)
+        w = {"K": 0, "M": 1, "Q": 2}[which]
+        check(lib().opem_base_matrix(self.ctx, w, attr, ptr(out, C.c_double)), self.ctx, "opem_base_matrix")
+        return out
+
+    def system_matrix(self, freq, gammas):
+        g = cplx_view(np.asarray([gammas[a] for a in self.ports], np.complex128))
+        out = np.empty(self.nnz, np.complex128)
+        check(lib().opem_system_matrix(self.ctx, freq, ptr(g, C.c_double), ptr(out.view(np.float64), C.c_double)),
+              self.ctx, "opem_system_matrix")
+        return out
+
+    def port_points(self, attr):
+        n = C.c_int64()
+        check(lib().opem_port_points(self.ctx, attr, C.byref(n), ptr(None, C.c_double)), self.ctx, "port_points")
+        pts = np.empty((n.value, 3), np.float64)
+        check(lib().opem_port_points(self.ctx, attr, C.byref(n), ptr(pts, C.c_double)), self.ctx, "port_points")
+        return pts
+
+    # ------------------------------------------------------------------ hot path
+    def sweep(self, freqs, precision="fp64", batch=0, restart=50, maxit=20000, tol=1e-10, reciprocal=True,
+              h_tol=1e-13, want_fields=False, mode_fn=None):
+        freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
+        nf, np_ = freqs.size, len(self.ports)
+        mode_fn = mode_fn or provider_for(self.mesh)
+        pts = [self.port_points(a) for a in self.ports]
+        E, H = [], []
+        gam = np.empty((nf, np_), np.complex128)
+        V = np.empty((nf, np_), np.complex128)
+        Z = np.empty((nf, np_), np.complex128)
+        for i, f in enumerate(freqs):
+            for k, a in enumerate(self.ports):
+                md = mode_fn(self.mesh, a, f)
+                E.append(md["E"](pts[k]).ravel())
+                H.append(md["H"](pts[k]).ravel())
+                gam[i, k], V[i, k], Z[i, k] = md["gamma"], md["V"], md["Z"]
+        E = cplx_view(np.concatenate(E))
+        H = cplx_view(np.concatenate(H))
+        pa = np.asarray(self.ports, np.int32)
+        gv, Vv, Zv = cplx_view(gam), cplx_view(V), cplx_view(Z)
+        sin = _lib.SweepIn(nfreq=nf, freqs=ptr(freqs, C.c_double), nports=np_, port_attr=ptr(pa, C.c_int32),
+                           mode_E=ptr(E, C.c_double), mode_H=ptr(H, C.c_double), gamma=ptr(gv, C.c_double),
+                           V=ptr(Vv, C.c_double), Z=ptr(Zv, C.c_double),
+                           precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64,
+                           batch=int(batch), restart=int(restart), maxit=int(maxit), tol=float(tol),
+                           reciprocal=int(bool(reciprocal)), h_tol=float(h_tol))
+        S = np.zeros((nf, np_, np_), np.complex128)
+        Sr = np.zeros_like(S)
+        Cp = np.zeros_like(S)
+        Cm = np.zeros_like(S)
+        it = np.zeros((nf, np_), np.int32)
+        res = np.zeros((nf, np_), np.float64)
+        cv = np.zeros((nf, np_), np.int32)
+        x = np.zeros((nf, np_, self.ndof), np.complex128) if want_fields else None
+        h = np.zeros((nf, np_, self.ndof), np.complex128) if want_fields else None
+        fv = lambda a: ptr(a.view(np.float64), C.c_double) if a is not None else ptr(None, C.c_double)
+        sout = _lib.SweepOut(S=fv(S), S_raw=fv(Sr), Cp=fv(Cp), Cm=fv(Cm), iters=ptr(it, C.c_int32),
+                             true_res=ptr(res, C.c_double), converged=ptr(cv, C.c_int32), x_out=fv(x), h_out=fv(h))
+        check(lib().opem_sweep(self.ctx, C.byref(sin), C.byref(sout)), self.ctx, "opem_sweep")
+        return dict(S=S, S_raw=Sr, Cp=Cp, Cm=Cm, iters=it, true_res=res, converged=cv, x=x, h=h,
+                    seconds=dict(total=sout.seconds_total, setup=sout.seconds_setup, solve=sout.seconds_solve,
+                                 post=sout.seconds_post))
+
+
+def spmv(ctx_solver, rowptr, colind, vals, x, precision="fp64"):
+    """y = A x on the device (complex CSR parity helper)."""
+    n = rowptr.size - 1
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colind, np.int32)
+    v = cplx_view(vals)
+    xv = cplx_view(x)
+    y = np.empty(n, np.complex128)
+    check(lib().opem_spmv(ctx_solver.ctx, 0 if precision == "fp64" else 1, n, ptr(rp, C.c_int32),
+                          ptr(ci, C.c_int32), ptr(v, C.c_double), ptr(xv, C.c_double),
+                          ptr(y.view(np.float64), C.c_double)), ctx_solver.ctx, "opem_spmv")
+    return y
+
+
+class Device:
+    """Bare context (no mesh) for kernel-level calls."""
+
+    def __init__(self, device=0):
+        self.ctx = C.c_void_p()
+        rc = lib().opem_create(C.byref(self.ctx), device)
+        if rc != 0:
+            raise _lib.OpemError(f"opem_create failed (rc={rc})")
+
+    def close(self):
+        if self.ctx:
+            lib().opem_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def element_matrices(self, order, prec, verts, er, tand, k0, nrep=1, want=True):
+        ne = verts.shape[0]
+        nd = order * (order + 2) * (order + 3) // 2
+        v = np.ascontiguousarray(verts, np.float64)
+        e = np.ascontiguousarray(er, np.float64)
+        t = np.ascontiguousarray(tand, np.float64)
+        out = np.empty((ne, 2, nd, nd), np.float64) if want else None
+        sec = C.c_double()
+        check(lib().opem_element_matrices(self.ctx, order, prec, ne, ptr(v, C.c_double), ptr(e, C.c_double),
+                                          ptr(t, C.c_double), k0, nrep, ptr(out, C.c_double) if want else
+                                          ptr(None, C.c_double), C.byref(sec)), self.ctx, "opem_element_matrices")
+        return out, sec.value

@@ -1,0 +1,157 @@
+"""GPU path (libopem.so through the C ABI) against the FP64 oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from openparem_amd.mesh import microstrip_mesh, random_tets, waveguide_mesh
+from oracle import assemble as oasm
+from oracle import nd as ond
+from oracle.pipeline import Problem
+
+pytestmark = pytest.mark.gpu
+
+COPPER = 5.813e7
+
+
+def _solver(mesh, p):
+    from openparem_amd.solver import Solver
+    bnd = {a: COPPER for a, k in mesh.bdr_kind.items() if k == "zs"}
+    return Solver(mesh, p, boundaries=bnd)
+
+
+def _csr(s, vals):
+    sp_ = s.space()
+    return sp.csr_matrix((vals, sp_["colind"], sp_["rowptr"]), shape=(s.ndof, s.ndof))
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-6), (2, 3e-2)])
+def test_element_matrices(p, prec, tol):
+    from openparem_amd.solver import Device
+    print(f"element matrices p={p} prec={prec}", flush=True)
+    v, er, td = random_tets(300, seed=11)
+    k0 = 2 * np.pi * 5e9 / 299792458.0 * 1e-2  # unit-cube tets: scale k0 so both terms matter
+    dev = Device()
+    try:
+        out, _ = dev.element_matrices(p, prec, v, er, td, k0)
+    finally:
+        dev.close()
+    K, M = ond.element_matrices(v, p)
+    ref_re = K - (k0 * k0 * er)[:, None, None] * M
+    ref_im = (k0 * k0 * er * td)[:, None, None] * M
+    ref = np.stack([ref_re, ref_im], 1)
+    err = np.linalg.norm((out - ref).reshape(300, -1), axis=1) / np.linalg.norm(ref.reshape(300, -1), axis=1)
+    assert err.max() < tol, err.max()
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_numbering_and_pattern(p):
+    m = waveguide_mesh(nx=3, ny=2, nz=4, seed=5)
+    s = _solver(m, p)
+    try:
+        g = s.space()
+    finally:
+        s.close()
+    o = oasm.Space(m, p)
+    assert s.ndof == o.ndof
+    np.testing.assert_array_equal(g["elem_dofs"], o.elem_dofs)
+    np.testing.assert_array_equal(g["bface_dofs"], o.bface_dofs)
+    rp, ci = oasm.pattern(o)
+    np.testing.assert_array_equal(g["rowptr"], rp)
+    np.testing.assert_array_equal(g["colind"], ci)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_base_matrices(p):
+    m = microstrip_mesh(nx=6, ny=4, nz=4, n_sub=1, seed=2)
+    s = _solver(m, p)
+    try:
+        mats = {(w, a): _csr(s, s.base_matrix(w, a)) for w in "KMQ" for a in (1, 2)}
+    finally:
+        s.close()
+    o = oasm.Space(m, p)
+    v = o.sorted_coords()
+    K, M = ond.element_matrices(v, p)
+    Q = ond.mixed_curl_matrices(v, p)
+    for a in (1, 2):
+        sel = (m.tet_attr == a).astype(float)
+        for w, E in (("K", K), ("M", M), ("Q", Q)):
+            ref = oasm._scatter(o.elem_dofs, E * sel[:, None, None], o.ndof)
+            d = abs(mats[(w, a)] - ref).max()
+            assert d <= 1e-12 * abs(ref).max(), (w, a, d)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_system_matrix_microstrip(p):
+    m = microstrip_mesh(nx=6, ny=4, nz=4, n_sub=1, seed=2)
+    f = 5e9
+    gam = {2: 0.3 + 180j, 3: 0.2 + 150j}
+    s = _solver(m, p)
+    try:
+        A = _csr(s, s.system_matrix(f, gam))
+    finally:
+        s.close()
+    o = oasm.Space(m, p)
+    rs = np.sqrt(np.pi * f * oasm.MU0 / COPPER)
+    ref = oasm.system_matrix(o, f, gam, rs_of_attr={4: rs})
+    d = abs(A - ref).max()
+    assert d <= 1e-11 * abs(ref).max(), d
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-14), ("fp32", 2e-6)])
+def test_spmv(prec, tol):
+    from openparem_amd.solver import Device, spmv
+    m = waveguide_mesh(nx=4, ny=3, nz=5, seed=9)
+    o = oasm.Space(m, 2)
+    A = oasm.system_matrix(o, 9e9, {2: 130j, 3: 130j})
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(o.ndof) + 1j * rng.standard_normal(o.ndof)
+    dev = Device()
+    try:
+        y = spmv(dev, A.indptr, A.indices, A.data, x, prec)
+    finally:
+        dev.close()
+    ref = A @ x
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_sweep_waveguide_fp64(p):
+    """E, H fields and S of the GPU sweep against the oracle's direct solve on a small guide."""
+    m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
+    f = np.array([9e9, 11e9])
+    s = _solver(m, p)
+    try:
+        r = s.sweep(f, precision="fp64", tol=1e-12, restart=60, want_fields=True)
+    finally:
+        s.close()
+    assert r["converged"].all(), r
+    pr = Problem(m, p)
+    from openparem_amd.ports import te10
+    for i, fi in enumerate(f):
+        modes = {a: te10(m, a, fi) for a in pr.ports}
+        o = pr.solve_point(fi, modes)
+        for k in range(2):
+            xo, ho = o["info"][k]["x"], o["info"][k]["h"]
+            ex = np.abs(r["x"][i, k] - xo).max() / np.abs(xo).max()
+            eh = np.abs(r["h"][i, k] - ho).max() / np.abs(ho).max()
+            print(f"f={fi:.3g} exc={k} |dx|={ex:.3e} |dh|={eh:.3e}", flush=True)
+            assert ex < 1e-9 and eh < 1e-9
+        print("Cp", np.abs(r["Cp"][i] - o["Cp"]).max(), "Cm", np.abs(r["Cm"][i] - o["Cm"]).max(), flush=True)
+        dS = np.abs(r["S"][i] - o["S"]).max()
+        assert dS < 1e-9, (fi, dS, r["S"][i], o["S"])
+
+
+def test_sweep_mixed_vs_fp64():
+    m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
+    f = np.array([8e9, 10e9, 12e9])
+    s = _solver(m, 2)
+    try:
+        r64 = s.sweep(f, precision="fp64", tol=1e-10, batch=1)
+        r32 = s.sweep(f, precision="mixed", tol=1e-10, batch=3)
+    finally:
+        s.close()
+    assert r64["converged"].all() and r32["converged"].all()
+    assert np.abs(r64["S"] - r32["S"]).max() < 1e-6
+    assert (r32["true_res"] <= 1e-10).all()
