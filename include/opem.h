@@ -105,6 +105,8 @@ typedef struct {
   double tol;                   /* true relative residual ||b - A x|| / ||b|| target              */
   int reciprocal;               /* 1: S <- (S + S^T)/2 (results.cpp:401-419)                      */
   double h_tol;                 /* H-projection CG relative tolerance (0 -> 1e-13)                */
+  int precond;                  /* 0: vertex-patch additive Schwarz (element blocks if a patch has */
+                                /*    more than 100 DOFs), 1: element-block Jacobi                */
 } opem_sweep_in;
 
 typedef struct {

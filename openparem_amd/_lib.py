@@ -30,7 +30,7 @@ class SweepIn(C.Structure):
     _fields_ = [("nfreq", C.c_int), ("freqs", f64p), ("nports", C.c_int), ("port_attr", i32p),
                 ("mode_E", f64p), ("mode_H", f64p), ("gamma", f64p), ("V", f64p), ("Z", f64p),
                 ("precision", C.c_int), ("batch", C.c_int), ("restart", C.c_int), ("maxit", C.c_int),
-                ("tol", C.c_double), ("reciprocal", C.c_int), ("h_tol", C.c_double)]
+                ("tol", C.c_double), ("reciprocal", C.c_int), ("h_tol", C.c_double), ("precond", C.c_int)]
 
 
 class SweepOut(C.Structure):

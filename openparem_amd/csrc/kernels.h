@@ -76,6 +76,19 @@ void launch_bj_apply(hipStream_t st, int S, int64_t n, int nd, const int32_t *el
                      const int32_t *owner_elem, const P *inv_rows, const X *v, int64_t ldv_sys, X *z,
                      const int32_t *active, int64_t inv_stride);
 
+// ---------------------------------------------------------------- vertex-patch additive Schwarz
+// inv[s] = blockwise (A_s restricted to each patch, driven-port DOFs of system s as identity)^-1.
+// One workgroup per (patch, system), Gauss-Jordan with partial pivoting in LDS (FP64, n_p <= 100).
+template <typename P>
+void launch_patch_build(hipStream_t st, int S, int R, int64_t npatch, int nmax, const int32_t *idx_off,
+                        const int32_t *idx, const int64_t *val_off, const int32_t *pnz, const double *a64,
+                        int64_t nnz, const uint8_t *rowmask, int64_t n, P *inv, int64_t inv_stride);
+// z[s][d] = sum_{patches p containing d} (inv_p r_p)[d]   (deterministic gather over DOF rows)
+template <typename P, typename X>
+void launch_patch_apply(hipStream_t st, int S, int64_t n, const int32_t *d2p_ptr, const int32_t *d2p_patch,
+                        const int32_t *d2p_row, const int32_t *idx_off, const int32_t *idx, const int64_t *val_off,
+                        const P *inv, int64_t inv_stride, const X *v, int64_t ldv_sys, X *z, const int32_t *active);
+
 // ---------------------------------------------------------------- misc vector ops (batched)
 template <typename X, typename Y>
 void launch_convert(hipStream_t st, int64_t count, const X *in, Y *out);

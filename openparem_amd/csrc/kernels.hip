@@ -635,6 +635,190 @@ template void launch_bj_apply<float2, float2>(hipStream_t, int, int64_t, int, co
                                               const float2 *, const float2 *, int64_t, float2 *, const int32_t *,
                                               int64_t);
 
+// ------------------------------------------------------------------ vertex-patch additive Schwarz
+template <typename P>
+__global__ __launch_bounds__(256) void k_patch_build(int R, const int32_t *__restrict__ idx_off,
+                                                     const int32_t *__restrict__ idx,
+                                                     const int64_t *__restrict__ val_off,
+                                                     const int32_t *__restrict__ pnz, const double2 *__restrict__ a64,
+                                                     int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
+                                                     P *__restrict__ inv, int64_t inv_stride) {
+  extern __shared__ double2 lds[];
+  const int p = blockIdx.x, s = blockIdx.y;
+  const int g = s / R, r = s % R;
+  const int np = idx_off[p + 1] - idx_off[p];
+  double2 *A = lds;                          // np*np
+  double2 *colk = lds + np * np;             // np
+  __shared__ int perm[128];
+  __shared__ double2 pinv_s;
+  const int32_t *I = idx + idx_off[p];
+  const int32_t *nz = pnz + val_off[p];
+  const double2 *Ag = a64 + (int64_t)g * nnz;
+  const uint8_t *mask = rowmask + (int64_t)r * n;
+  const int tid = threadIdx.x;
+  const int nn = np * np;
+  for (int t = tid; t < nn; t += blockDim.x) {
+    const int a = t / np, b = t - a * np;
+    const bool fa = mask[I[a]] != 0, fb = mask[I[b]] != 0;
+    double2 v = make_double2(a == b ? 1.0 : 0.0, 0.0);
+    if (fa && fb) {
+      const int32_t q = nz[t];
+      v = q >= 0 ? Ag[q] : make_double2(0.0, 0.0);
+    }
+    A[t] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < np; k++) {
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + tid; i < np; i += 64) {
+        const double2 v = A[i * np + k];
+        const double m = v.x * v.x + v.y * v.y;
+        if (m > best) {
+          best = m;
+          bi = i;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) perm[k] = bi;
+    }
+    __syncthreads();
+    const int bi = perm[k];
+    if (bi != k)
+      for (int c = tid; c < np; c += blockDim.x) {
+        const double2 t0 = A[k * np + c];
+        A[k * np + c] = A[bi * np + c];
+        A[bi * np + c] = t0;
+      }
+    __syncthreads();
+    if (tid == 0) {
+      const double2 piv = A[k * np + k];
+      const double pm = piv.x * piv.x + piv.y * piv.y;
+      pinv_s = pm > 0.0 ? make_double2(piv.x / pm, -piv.y / pm) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const double2 pinv = pinv_s;
+    for (int c = tid; c < np; c += blockDim.x) A[k * np + c] = (c == k) ? pinv : cmul(A[k * np + c], pinv);
+    for (int i = tid; i < np; i += blockDim.x) colk[i] = A[i * np + k];
+    __syncthreads();
+    for (int t = tid; t < nn; t += blockDim.x) {
+      const int i = t / np, c = t - i * np;
+      if (i == k) continue;
+      const double2 f = colk[i];
+      if (c == k) {
+        A[t] = make_double2(-(f.x * pinv.x - f.y * pinv.y), -(f.x * pinv.y + f.y * pinv.x));
+      } else {
+        const double2 u = cmul(f, A[k * np + c]);
+        A[t] = make_double2(A[t].x - u.x, A[t].y - u.y);
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = np - 1; k >= 0; k--) {
+    const int pk = perm[k];
+    if (pk != k)
+      for (int i = tid; i < np; i += blockDim.x) {
+        const double2 t0 = A[i * np + k];
+        A[i * np + k] = A[i * np + pk];
+        A[i * np + pk] = t0;
+      }
+    __syncthreads();
+  }
+  P *out = inv + (int64_t)s * inv_stride + val_off[p];
+  for (int t = tid; t < nn; t += blockDim.x) out[t] = from_d2<P>(A[t]);
+}
+
+template <typename P>
+void launch_patch_build(hipStream_t st, int S, int R, int64_t npatch, int nmax, const int32_t *idx_off,
+                        const int32_t *idx, const int64_t *val_off, const int32_t *pnz, const double *a64,
+                        int64_t nnz, const uint8_t *rowmask, int64_t n, P *inv, int64_t inv_stride) {
+  if (nmax > 100) throw std::runtime_error("vertex patch larger than 100 DOFs");
+  const size_t lds = (size_t)(nmax * nmax + nmax) * sizeof(double2);
+  static size_t attr_bytes = 0;
+  if (lds > attr_bytes) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_patch_build<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw std::runtime_error(std::string("patch build: LDS request failed: ") + hipGetErrorString(e));
+    }
+    attr_bytes = lds;
+  }
+  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)npatch, S), dim3(256), lds, st, R, idx_off, idx, val_off,
+                     pnz, (const double2 *)a64, nnz, rowmask, n, inv, inv_stride);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_patch_build<double2>(hipStream_t, int, int, int64_t, int, const int32_t *, const int32_t *,
+                                          const int64_t *, const int32_t *, const double *, int64_t, const uint8_t *,
+                                          int64_t, double2 *, int64_t);
+template void launch_patch_build<float2>(hipStream_t, int, int, int64_t, int, const int32_t *, const int32_t *,
+                                         const int64_t *, const int32_t *, const double *, int64_t, const uint8_t *,
+                                         int64_t, float2 *, int64_t);
+
+constexpr int kPatchLanes = 16;
+
+template <typename P, typename X>
+__global__ __launch_bounds__(256) void k_patch_apply(int64_t n, const int32_t *__restrict__ d2p_ptr,
+                                                     const int32_t *__restrict__ d2p_patch,
+                                                     const int32_t *__restrict__ d2p_row,
+                                                     const int32_t *__restrict__ idx_off,
+                                                     const int32_t *__restrict__ idx,
+                                                     const int64_t *__restrict__ val_off, const P *__restrict__ inv,
+                                                     int64_t inv_stride, const X *__restrict__ v, int64_t ldv_sys,
+                                                     X *__restrict__ z, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const int sub = threadIdx.x % kPatchLanes;
+  const int64_t d = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPatchLanes;
+  if (d >= n) return;
+  const X *vs = v + s * ldv_sys;
+  const P *is = inv + s * inv_stride;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int k = d2p_ptr[d]; k < d2p_ptr[d + 1]; k++) {
+    const int p = d2p_patch[k], l = d2p_row[k];
+    const int b0 = idx_off[p], np = idx_off[p + 1] - b0;
+    const P *row = is + val_off[p] + (int64_t)l * np;
+    const int32_t *I = idx + b0;
+    for (int j = sub; j < np; j += kPatchLanes) {
+      const double2 a = to_d2(row[j]);
+      const double2 x = to_d2(vs[I[j]]);
+      acc.x += a.x * x.x - a.y * x.y;
+      acc.y += a.x * x.y + a.y * x.x;
+    }
+  }
+#pragma unroll
+  for (int o = kPatchLanes / 2; o > 0; o >>= 1) {
+    acc.x += __shfl_xor(acc.x, o, kPatchLanes);
+    acc.y += __shfl_xor(acc.y, o, kPatchLanes);
+  }
+  if (sub == 0) z[s * n + d] = from_d2<X>(acc);
+}
+
+template <typename P, typename X>
+void launch_patch_apply(hipStream_t st, int S, int64_t n, const int32_t *d2p_ptr, const int32_t *d2p_patch,
+                        const int32_t *d2p_row, const int32_t *idx_off, const int32_t *idx, const int64_t *val_off,
+                        const P *inv, int64_t inv_stride, const X *v, int64_t ldv_sys, X *z, const int32_t *active) {
+  hipLaunchKernelGGL((k_patch_apply<P, X>), dim3((unsigned)((n * kPatchLanes + 255) / 256), S), dim3(256), 0, st, n,
+                     d2p_ptr, d2p_patch, d2p_row, idx_off, idx, val_off, inv, inv_stride, v, ldv_sys, z, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_patch_apply<double2, double2>(hipStream_t, int, int64_t, const int32_t *, const int32_t *,
+                                                   const int32_t *, const int32_t *, const int32_t *,
+                                                   const int64_t *, const double2 *, int64_t, const double2 *,
+                                                   int64_t, double2 *, const int32_t *);
+template void launch_patch_apply<float2, float2>(hipStream_t, int, int64_t, const int32_t *, const int32_t *,
+                                                 const int32_t *, const int32_t *, const int32_t *, const int64_t *,
+                                                 const float2 *, int64_t, const float2 *, int64_t, float2 *,
+                                                 const int32_t *);
+
 // ------------------------------------------------------------------ misc vector ops
 template <typename X, typename Y>
 __global__ void k_convert(int64_t count, const X *__restrict__ in, Y *__restrict__ out) {

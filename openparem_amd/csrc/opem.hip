@@ -110,6 +110,12 @@ struct Ctx {
   DevBuf d_rowptr, d_colind, d_elem_dofs, d_owner_elem, d_elem_nz, d_blocks, d_K, d_M, d_Q;
   DevBuf d_P, d_Qs, d_Pdinv, d_onesmask;
   int64_t nblocks = 0;
+  // vertex-patch Schwarz (built per PEC set)
+  Patches patches;
+  bool have_patches = false;
+  std::vector<uint8_t> patch_excl;
+  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row;
+  int precond = 0;  // 0 vertex patches (falls back to element blocks when a patch exceeds 100 DOFs), 1 element blocks
   int lpr = 16;
 };
 
@@ -622,11 +628,27 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   const int nblk = reduce_blocks(n);
   const bool mixed = std::is_same<X, float2>::value;
   const X *vals = mixed ? (const X *)W.A32.p : (const X *)W.A64.p;
-  // preconditioner
-  W.inv.alloc((size_t)S * nd * n * sizeof(P));
-  launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(), nnz,
-                     W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(), c.d_blocks.as<int32_t>(),
-                     c.nblocks);
+  // preconditioner: vertex-patch additive Schwarz, or element-block Jacobi
+  const bool use_patch = c.have_patches && c.precond == 0;
+  const int64_t inv_stride = use_patch ? c.patches.val_off.back() : (int64_t)nd * n;
+  W.inv.alloc((size_t)S * inv_stride * sizeof(P));
+  if (use_patch)
+    launch_patch_build<P>(st, S, R, c.patches.npatch, c.patches.nmax, c.dp_idx_off.as<int32_t>(),
+                          c.dp_idx.as<int32_t>(), c.dp_val_off.as<int64_t>(), c.dp_nz.as<int32_t>(),
+                          W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<P>(), inv_stride);
+  else
+    launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
+                       nnz, W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
+                       c.d_blocks.as<int32_t>(), c.nblocks);
+  auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
+    if (use_patch)
+      launch_patch_apply<P, X>(st, S, n, c.dp_d2p_ptr.as<int32_t>(), c.dp_d2p_patch.as<int32_t>(),
+                               c.dp_d2p_row.as<int32_t>(), c.dp_idx_off.as<int32_t>(), c.dp_idx.as<int32_t>(),
+                               c.dp_val_off.as<int64_t>(), W.inv.as<P>(), inv_stride, v, ldv_sys, z, act);
+    else
+      launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
+                            v, ldv_sys, z, act, (int64_t)nd * n);
+  };
   const int64_t ldv = (int64_t)(m + 1) * n;
   W.V.alloc((size_t)S * ldv * sizeof(X));
   W.w.alloc((size_t)S * n * sizeof(X));
@@ -692,8 +714,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     int j = 0;
     for (j = 0; j < m; j++) {
       X *Vj = W.V.as<X>() + (int64_t)j * n;
-      launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
-                            Vj, ldv, W.z.as<X>(), W.active.as<int32_t>(), (int64_t)nd * n);
+      precond(Vj, ldv, W.z.as<X>(), W.active.as<int32_t>());
       launch_spmv<X, X>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), vals, nnz,
                         W.masks.as<uint8_t>(), W.z.as<X>(), W.w.as<X>(), nullptr, W.active.as<int32_t>(), 0, c.lpr,
                         nnz);
@@ -728,8 +749,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
                      W.cyc_active.as<int32_t>());
     launch_combine_basis<X>(st, S, n, m, W.V.as<X>(), ldv, W.y.as<double>(), W.kfinal.as<int32_t>(), W.u.as<X>(),
                             W.cyc_active.as<int32_t>());
-    launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
-                          W.u.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>(), (int64_t)nd * n);
+    precond(W.u.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>());
     launch_axpy<double2, X>(st, S, n, W.z.as<X>(), W.x.as<double2>(), W.cyc_active.as<int32_t>());
     HIPCHK(hipMemcpyAsync(kf.data(), W.kfinal.p, S * 4, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> ca(S);
@@ -911,6 +931,21 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     auto it = c.bdr.find(c.mesh.bface_attr[f]);
     if (it != c.bdr.end() && it->second.kind == OPEM_BDR_PEC)
       for (int k = 0; k < s.ndf; k++) pec[s.bface_dofs[f * s.ndf + k]] = 1;
+  }
+  if (c.precond == 0 && (!c.have_patches || c.patch_excl != pec)) {
+    int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    build_patches(s, pec, c.patches, nth);
+    c.patch_excl = pec;
+    c.have_patches = c.patches.nmax <= 100;
+    if (c.have_patches) {
+      c.dp_idx_off.upload(c.patches.idx_off, st);
+      c.dp_idx.upload(c.patches.idx, st);
+      c.dp_val_off.upload(c.patches.val_off, st);
+      c.dp_nz.upload(c.patches.nz, st);
+      c.dp_d2p_ptr.upload(c.patches.d2p_ptr, st);
+      c.dp_d2p_patch.upload(c.patches.d2p_patch, st);
+      c.dp_d2p_row.upload(c.patches.d2p_row, st);
+    }
   }
   const int R = np;
   std::vector<uint8_t> masks((size_t)R * n);
@@ -1357,6 +1392,8 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
       throw std::invalid_argument("bad sweep arguments");
     if (in->precision != OPEM_PREC_FP64 && in->precision != OPEM_PREC_MIXED)
       throw std::invalid_argument("bad precision");
+    if (in->precond < 0 || in->precond > 1) throw std::invalid_argument("bad preconditioner");
+    ctx->precond = in->precond;
     sweep(*ctx, in, out);
   });
 }

@@ -230,6 +230,67 @@ void build_space(const Mesh &m, int p, Space &s, int nthreads) {
     if (!(std::fabs(s.detj[e]) > 0)) throw std::runtime_error("zero-volume tet");
 }
 
+void build_patches(const Space &s, const std::vector<uint8_t> &exclude, Patches &P, int nthreads) {
+  P = Patches();
+  const int nd = s.nd;
+  const int64_t nt = (int64_t)s.tets_sorted.size() / 4;
+  // entity vertex sets per element-local DOF
+  std::vector<std::vector<int>> ent(nd);
+  {
+    int a = 0;
+    for (int k = 0; k < 6; k++)
+      for (int t = 0; t < s.k_edge; t++) ent[a++] = {TET_E[k][0], TET_E[k][1]};
+    for (int k = 0; k < 4; k++)
+      for (int t = 0; t < s.k_face; t++) ent[a++] = {TET_F[k][0], TET_F[k][1], TET_F[k][2]};
+    for (int t = 0; t < s.k_cell; t++) ent[a++] = {0, 1, 2, 3};
+  }
+  int32_t nv = 0;
+  for (int32_t v : s.tets_sorted) nv = std::max(nv, v + 1);
+  std::vector<std::vector<int32_t>> pv(nv);
+  for (int64_t e = 0; e < nt; e++)
+    for (int a = 0; a < nd; a++) {
+      const int32_t d = s.elem_dofs[e * nd + a];
+      if (exclude[d]) continue;
+      for (int k : ent[a]) pv[s.tets_sorted[e * 4 + k]].push_back(d);
+    }
+  P.idx_off.push_back(0);
+  P.val_off.push_back(0);
+  for (int32_t v = 0; v < nv; v++) {
+    auto &L = pv[v];
+    std::sort(L.begin(), L.end());
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+    if (L.empty()) continue;
+    P.idx.insert(P.idx.end(), L.begin(), L.end());
+    P.idx_off.push_back((int32_t)P.idx.size());
+    P.val_off.push_back(P.val_off.back() + (int64_t)L.size() * (int64_t)L.size());
+    P.nmax = std::max(P.nmax, (int)L.size());
+    std::vector<int32_t>().swap(L);
+  }
+  P.npatch = (int64_t)P.idx_off.size() - 1;
+  P.nz.resize(P.val_off.back());
+  parallel_for(P.npatch, nthreads, [&](int64_t lo, int64_t hi) {
+    for (int64_t p = lo; p < hi; p++) {
+      const int32_t *I = &P.idx[P.idx_off[p]];
+      const int n = P.idx_off[p + 1] - P.idx_off[p];
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) P.nz[P.val_off[p] + (int64_t)i * n + j] = (int32_t)csr_find(s, I[i], I[j]);
+    }
+  });
+  P.d2p_ptr.assign(s.ndof + 1, 0);
+  for (int32_t d : P.idx) P.d2p_ptr[d + 1]++;
+  for (int64_t i = 0; i < s.ndof; i++) P.d2p_ptr[i + 1] += P.d2p_ptr[i];
+  P.d2p_patch.resize(P.idx.size());
+  P.d2p_row.resize(P.idx.size());
+  std::vector<int32_t> fill(P.d2p_ptr.begin(), P.d2p_ptr.end() - 1);
+  for (int64_t p = 0; p < P.npatch; p++)
+    for (int32_t k = P.idx_off[p]; k < P.idx_off[p + 1]; k++) {
+      const int32_t d = P.idx[k];
+      P.d2p_patch[fill[d]] = (int32_t)p;
+      P.d2p_row[fill[d]] = k - P.idx_off[p];
+      fill[d]++;
+    }
+}
+
 int64_t csr_find(const Space &s, int32_t row, int32_t col) {
   auto b = s.colind.begin() + s.rowptr[row], e = s.colind.begin() + s.rowptr[row + 1];
   auto it = std::lower_bound(b, e, col);

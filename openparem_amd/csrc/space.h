@@ -44,6 +44,21 @@ struct Space {
 // Build the space; throws std::runtime_error on bad input.
 void build_space(const Mesh &m, int p, Space &s, int nthreads);
 
+// Vertex-patch additive Schwarz structure (Arnold-Falk-Winther H(curl) subspace decomposition):
+// patch v = DOFs of every entity (edge, face, cell) incident to vertex v, minus excluded DOFs.
+struct Patches {
+  int64_t npatch = 0;
+  int nmax = 0;
+  std::vector<int32_t> idx_off;   // npatch+1, into idx
+  std::vector<int32_t> idx;       // global DOFs, ascending per patch
+  std::vector<int64_t> val_off;   // npatch+1, into n_p*n_p blocks
+  std::vector<int32_t> nz;        // CSR position of (idx[i], idx[j]) or -1, per block
+  std::vector<int32_t> d2p_ptr;   // ndof+1
+  std::vector<int32_t> d2p_patch; // patch of each (DOF, patch-row) pair
+  std::vector<int32_t> d2p_row;   // local row
+};
+void build_patches(const Space &s, const std::vector<uint8_t> &exclude, Patches &P, int nthreads);
+
 // Position of (row, col) in the CSR pattern, -1 if absent.
 int64_t csr_find(const Space &s, int32_t row, int32_t col);
 

@@ -98,7 +98,7 @@ class Solver:
 
     # ------------------------------------------------------------------ hot path
     def sweep(self, freqs, precision="fp64", batch=0, restart=50, maxit=20000, tol=1e-10, reciprocal=True,
-              h_tol=1e-13, want_fields=False, mode_fn=None):
+              h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch"):
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
         nf, np_ = freqs.size, len(self.ports)
         mode_fn = mode_fn or provider_for(self.mesh)
@@ -122,7 +122,8 @@ class Solver:
                            V=ptr(Vv, C.c_double), Z=ptr(Zv, C.c_double),
                            precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64,
                            batch=int(batch), restart=int(restart), maxit=int(maxit), tol=float(tol),
-                           reciprocal=int(bool(reciprocal)), h_tol=float(h_tol))
+                           reciprocal=int(bool(reciprocal)), h_tol=float(h_tol),
+                           precond={"patch": 0, "element": 1}[precond])
         S = np.zeros((nf, np_, np_), np.complex128)
         Sr = np.zeros_like(S)
         Cp = np.zeros_like(S)

@@ -116,14 +116,15 @@ def test_spmv(prec, tol):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("precond", ["patch", "element"])
 @pytest.mark.parametrize("p", [1, 2])
-def test_sweep_waveguide_fp64(p):
+def test_sweep_waveguide_fp64(p, precond):
     """E, H fields and S of the GPU sweep against the oracle's direct solve on a small guide."""
     m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
     f = np.array([9e9, 11e9])
     s = _solver(m, p)
     try:
-        r = s.sweep(f, precision="fp64", tol=1e-12, restart=60, want_fields=True)
+        r = s.sweep(f, precision="fp64", tol=1e-12, restart=60, want_fields=True, precond=precond)
     finally:
         s.close()
     assert r["converged"].all(), r
