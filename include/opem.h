@@ -88,6 +88,13 @@ int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const i
 int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const double *verts, const double *er,
                           const double *tand, double k0, int nrep, double *out, double *avg_seconds);
 
+/* Times the batched complex CSR SpMV kernel on the context's pattern (device-resident synthetic values
+ * and vectors): ngroups matrices, R right-hand sides each (y = mask .* A x), prec as opem_spmv.
+ * *avg_seconds = mean kernel duration over nrep launches (HIP events on the context stream);
+ * *alg_bytes = algorithmic HBM bytes of one launch: 4 nnz + 4 (N+1) + ngroups s_v nnz + ngroups R 2 N s_v
+ * (s_v = 16 FP64, 8 FP32). */
+int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, double *avg_seconds, double *alg_bytes);
+
 typedef struct {
   int nfreq;
   const double *freqs;          /* [nfreq] Hz                                                     */
@@ -100,13 +107,15 @@ typedef struct {
   const double *Z;              /* complex [nfreq][nports] modal impedance                        */
   int precision;                /* OPEM_PREC_FP64 or OPEM_PREC_MIXED                              */
   int batch;                    /* frequencies per batched launch; 1 = one system group at a time */
-  int restart;                  /* GMRES(m)                                                       */
+  int restart;                  /* GMRES(m), m <= 400 (0 -> 150)                                  */
   int maxit;                    /* Krylov iteration limit per system                              */
   double tol;                   /* true relative residual ||b - A x|| / ||b|| target              */
   int reciprocal;               /* 1: S <- (S + S^T)/2 (results.cpp:401-419)                      */
   double h_tol;                 /* H-projection CG relative tolerance (0 -> 1e-13)                */
   int precond;                  /* 0: vertex-patch additive Schwarz (element blocks if a patch has */
                                 /*    more than 100 DOFs), 1: element-block Jacobi                */
+  int coarse_size;              /* aggregates of the gradient coarse level (0 = one-level, <= 128) */
+  int orth_passes;              /* classical Gram-Schmidt passes per Arnoldi step: 2 (default) or 1 */
 } opem_sweep_in;
 
 typedef struct {

@@ -7,6 +7,9 @@
 
 namespace opem {
 
+// Largest Krylov basis (restart + 1) the reduction kernels accept.
+constexpr int kMaxBasis = 402;
+
 // ---------------------------------------------------------------- assembly
 // Base volume matrices on the shared pattern, one wavefront per row (deterministic gather):
 //   K[a][nz], M[a][nz], Q[a][nz] for attribute slot a (unit coefficients).
@@ -88,6 +91,20 @@ template <typename P, typename X>
 void launch_patch_apply(hipStream_t st, int S, int64_t n, const int32_t *d2p_ptr, const int32_t *d2p_patch,
                         const int32_t *d2p_row, const int32_t *idx_off, const int32_t *idx, const int64_t *val_off,
                         const P *inv, int64_t inv_stride, const X *v, int64_t ldv_sys, X *z, const int32_t *active);
+
+// ---------------------------------------------------------------- coarse space (two-level Schwarz)
+// rc[s][a] = sum_i C_ia (v[s][i] - t[s][i])   (C^T by coarse rows; t may be null)
+template <typename X>
+void launch_coarse_restrict(hipStream_t st, int S, int nc, const int32_t *col_ptr, const int32_t *col_row,
+                            const double *col_val, const X *v, int64_t ldv_sys, const X *t, int64_t ldt_sys,
+                            double *rc, const int32_t *active);
+// yc[s] = Ainv[s / R] rc[s]   (dense nc x nc complex per matrix group)
+void launch_coarse_solve(hipStream_t st, int S, int R, int nc, const double *Ainv, const double *rc, double *yc,
+                         const int32_t *active);
+// z[s][i] += sum_a C_ia yc[s][a]
+template <typename X>
+void launch_coarse_prolong(hipStream_t st, int S, int64_t n, int nc, const int32_t *row_ptr, const int32_t *row_col,
+                           const double *row_val, const double *yc, X *z, const int32_t *active);
 
 // ---------------------------------------------------------------- misc vector ops (batched)
 template <typename X, typename Y>

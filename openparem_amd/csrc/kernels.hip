@@ -302,7 +302,7 @@ template void launch_spmv<float2, float2>(hipStream_t, int, int, int64_t, const 
                                           const float2 *, const int32_t *, int, int, int64_t);
 
 // ------------------------------------------------------------------ Krylov helpers
-constexpr int kMaxK = 130;  // basis vectors per reduction (restart + 1)
+constexpr int kMaxK = kMaxBasis;  // basis vectors per reduction (restart + 1)
 
 template <typename X>
 __global__ __launch_bounds__(kRedThreads) void k_mdot(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
@@ -818,6 +818,114 @@ template void launch_patch_apply<float2, float2>(hipStream_t, int, int64_t, cons
                                                  const int32_t *, const int32_t *, const int32_t *, const int64_t *,
                                                  const float2 *, int64_t, const float2 *, int64_t, float2 *,
                                                  const int32_t *);
+
+// ------------------------------------------------------------------ coarse space
+template <typename X>
+__global__ __launch_bounds__(256) void k_coarse_restrict(int nc, const int32_t *__restrict__ col_ptr,
+                                                         const int32_t *__restrict__ col_row,
+                                                         const double *__restrict__ col_val, const X *__restrict__ v,
+                                                         int64_t ldv_sys, const X *__restrict__ t, int64_t ldt_sys,
+                                                         double2 *__restrict__ rc, const int32_t *__restrict__ active) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  if (active && !active[s]) return;
+  __shared__ double red[2][4];
+  const X *vs = v + s * ldv_sys;
+  double sx = 0.0, sy = 0.0;
+  const X *ts = t ? t + s * ldt_sys : nullptr;
+  for (int k = col_ptr[a] + threadIdx.x; k < col_ptr[a + 1]; k += blockDim.x) {
+    double2 x = to_d2(vs[col_row[k]]);
+    if (ts) {
+      const double2 y = to_d2(ts[col_row[k]]);
+      x.x -= y.x;
+      x.y -= y.y;
+    }
+    const double c = col_val[k];
+    sx += c * x.x;
+    sy += c * x.y;
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = sx;
+    red[1][threadIdx.x >> 6] = sy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    rc[(int64_t)s * nc + a] = make_double2(red[0][0] + red[0][1] + red[0][2] + red[0][3],
+                                           red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+}
+
+template <typename X>
+void launch_coarse_restrict(hipStream_t st, int S, int nc, const int32_t *col_ptr, const int32_t *col_row,
+                            const double *col_val, const X *v, int64_t ldv_sys, const X *t, int64_t ldt_sys,
+                            double *rc, const int32_t *active) {
+  hipLaunchKernelGGL((k_coarse_restrict<X>), dim3(nc, S), dim3(256), 0, st, nc, col_ptr, col_row, col_val, v, ldv_sys,
+                     t, ldt_sys, (double2 *)rc, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_coarse_restrict<double2>(hipStream_t, int, int, const int32_t *, const int32_t *, const double *,
+                                              const double2 *, int64_t, const double2 *, int64_t, double *,
+                                              const int32_t *);
+template void launch_coarse_restrict<float2>(hipStream_t, int, int, const int32_t *, const int32_t *, const double *,
+                                             const float2 *, int64_t, const float2 *, int64_t, double *,
+                                             const int32_t *);
+
+__global__ void k_coarse_solve(int R, int nc, const double2 *__restrict__ Ainv, const double2 *__restrict__ rc,
+                               double2 *__restrict__ yc, const int32_t *__restrict__ active) {
+  const int s = blockIdx.x;
+  if (active && !active[s]) return;
+  const double2 *A = Ainv + (int64_t)(s / R) * nc * nc;
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < nc; j++) {
+      const double2 u = cmul(A[(int64_t)i * nc + j], rc[(int64_t)s * nc + j]);
+      acc.x += u.x;
+      acc.y += u.y;
+    }
+    yc[(int64_t)s * nc + i] = acc;
+  }
+}
+
+void launch_coarse_solve(hipStream_t st, int S, int R, int nc, const double *Ainv, const double *rc, double *yc,
+                         const int32_t *active) {
+  hipLaunchKernelGGL(k_coarse_solve, dim3(S), dim3(128), 0, st, R, nc, (const double2 *)Ainv, (const double2 *)rc,
+                     (double2 *)yc, active);
+  HIP_LAUNCH_CHECK();
+}
+
+template <typename X>
+__global__ void k_coarse_prolong(int64_t n, int nc, const int32_t *__restrict__ row_ptr,
+                                 const int32_t *__restrict__ row_col, const double *__restrict__ row_val,
+                                 const double2 *__restrict__ yc, X *__restrict__ z, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    if (b == e) continue;
+    double2 acc = to_d2(z[s * n + i]);
+    for (int k = b; k < e; k++) {
+      const double c = row_val[k];
+      const double2 y = yc[(int64_t)s * nc + row_col[k]];
+      acc.x += c * y.x;
+      acc.y += c * y.y;
+    }
+    z[s * n + i] = from_d2<X>(acc);
+  }
+}
+
+template <typename X>
+void launch_coarse_prolong(hipStream_t st, int S, int64_t n, int nc, const int32_t *row_ptr, const int32_t *row_col,
+                           const double *row_val, const double *yc, X *z, const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_coarse_prolong<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, nc, row_ptr, row_col, row_val,
+                     (const double2 *)yc, z, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_coarse_prolong<double2>(hipStream_t, int, int64_t, int, const int32_t *, const int32_t *,
+                                             const double *, const double *, double2 *, const int32_t *);
+template void launch_coarse_prolong<float2>(hipStream_t, int, int64_t, int, const int32_t *, const int32_t *,
+                                            const double *, const double *, float2 *, const int32_t *);
 
 // ------------------------------------------------------------------ misc vector ops
 template <typename X, typename Y>

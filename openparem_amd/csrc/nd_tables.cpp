@@ -182,6 +182,40 @@ TetTables build_tet(int p) {
       for (int k = 0; k < 3; k++) s += prod_int(phi[a], k, curl[b], k, 4);
       T.Q[a * nd + b] = s;
     }
+  // grad(lambda_k) = sum_a G[a][k] phi_a: solve Mref G = int phi . ghat_k on the reference tet
+  std::vector<double> Mr(nd * nd), rhs(nd * 4, 0.0);
+  for (int a = 0; a < nd; a++)
+    for (int b = 0; b < nd; b++) Mr[a * nd + b] = T.M[(0 * nd + a) * nd + b] + T.M[(1 * nd + a) * nd + b] +
+                                                  T.M[(2 * nd + a) * nd + b];
+  for (int a = 0; a < nd; a++)
+    for (auto &t : phi[a]) {
+      const double w = mono_int(t.b, 4);
+      for (int k = 0; k < 4; k++) rhs[a * 4 + k] += w * (t.c[0] * g[k][0] + t.c[1] * g[k][1] + t.c[2] * g[k][2]);
+    }
+  // Gaussian elimination with partial pivoting (nd <= 45, 4 right-hand sides)
+  for (int k = 0; k < nd; k++) {
+    int p = k;
+    for (int i = k + 1; i < nd; i++)
+      if (std::fabs(Mr[i * nd + k]) > std::fabs(Mr[p * nd + k])) p = i;
+    if (p != k) {
+      for (int c = 0; c < nd; c++) std::swap(Mr[k * nd + c], Mr[p * nd + c]);
+      for (int c = 0; c < 4; c++) std::swap(rhs[k * 4 + c], rhs[p * 4 + c]);
+    }
+    for (int i = k + 1; i < nd; i++) {
+      const double f = Mr[i * nd + k] / Mr[k * nd + k];
+      for (int c = k; c < nd; c++) Mr[i * nd + c] -= f * Mr[k * nd + c];
+      for (int c = 0; c < 4; c++) rhs[i * 4 + c] -= f * rhs[k * 4 + c];
+    }
+  }
+  T.G.assign(nd * 4, 0.0);
+  for (int i = nd - 1; i >= 0; i--)
+    for (int c = 0; c < 4; c++) {
+      double t = rhs[i * 4 + c];
+      for (int j = i + 1; j < nd; j++) t -= Mr[i * nd + j] * T.G[j * 4 + c];
+      T.G[i * 4 + c] = t / Mr[i * nd + i];
+    }
+  for (double &v : T.G)  // entries are small rationals; remove solve round-off
+    v = std::round(v * 1e9) / 1e9;
   return T;
 }
 

@@ -41,6 +41,7 @@ struct TetTables {
   std::vector<double> M;   // 6*nd*nd
   std::vector<double> K;   // 6*nd*nd
   std::vector<double> Q;   // nd*nd
+  std::vector<double> G;   // nd*4: coefficients of grad(lambda_k) in the basis (affine invariant)
 };
 struct TriTables {
   int p = 0, nd = 0;

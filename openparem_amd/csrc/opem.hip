@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/opem.h"
+#include "coarse.h"
 #include "kernels.h"
 #include "nd_tables.h"
 #include "quadrature.h"
@@ -115,6 +116,14 @@ struct Ctx {
   bool have_patches = false;
   std::vector<uint8_t> patch_excl;
   DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row;
+  // gradient coarse space (second level)
+  Coarse coarse;
+  bool have_coarse = false;
+  std::vector<uint8_t> coarse_excl;
+  std::vector<std::vector<double>> coarse_M;   // per volume slot: C^T M_slot C
+  std::vector<std::vector<double>> coarse_T;   // per overlay term
+  DevBuf dc_row_ptr, dc_row_col, dc_row_val, dc_col_ptr, dc_col_row, dc_col_val;
+  int coarse_size = 96;                        // 0 disables the coarse level
   int precond = 0;  // 0 vertex patches (falls back to element blocks when a patch exceeds 100 DOFs), 1 element blocks
   int lpr = 16;
 };
@@ -170,6 +179,15 @@ __global__ void k_diag_apply(int64_t n, const double *__restrict__ dinv, const d
     const double d = dinv[i];
     const double2 v = r[(int64_t)s * n + i];
     z[(int64_t)s * n + i] = make_double2(d * v.x, d * v.y);
+  }
+}
+
+template <typename T>
+__global__ void k_fill(int64_t count, T *__restrict__ a, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const double u = (double)((i * 2654435761ull) % 1000003ull) / 1000003.0;
+    a[i].x = (decltype(a[i].x))(scale * (u - 0.5));
+    a[i].y = (decltype(a[i].y))(scale * (0.25 - u * u));
   }
 }
 
@@ -597,7 +615,7 @@ static void device_space(Ctx &c) {
 struct Work {
   // sized per batch
   DevBuf A64, A32, masks, x, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
-      hbuf, nrm2, inv, pdv, pd, bn;
+      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t;
 };
 
 template <typename X>
@@ -618,8 +636,9 @@ struct BatchResult {
 // GMRES(m) with element-block-Jacobi right preconditioning, restarted with the true residual in
 // FP64 at every cycle (for X = float2 this is FP32 GMRES inside FP64 iterative refinement).
 template <typename X>
-static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double tol, std::vector<int> &iters,
-                        std::vector<double> &relres, std::vector<int> &conv, const std::vector<double> &bnorm) {
+static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double tol, int passes,
+                        std::vector<int> &iters, std::vector<double> &relres, std::vector<int> &conv,
+                        const std::vector<double> &bnorm) {
   using P = typename VecT<X>::P;
   const Space &s = c.sp;
   hipStream_t st = c.st;
@@ -640,6 +659,12 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
                        nnz, W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                        c.d_blocks.as<int32_t>(), c.nblocks);
+  const bool use_coarse = c.have_coarse && c.coarse.nc > 0;
+  if (use_coarse) {
+    W.t.alloc((size_t)S * n * sizeof(X));
+    W.rc.alloc((size_t)S * c.coarse.nc * 16);
+    W.yc.alloc((size_t)S * c.coarse.nc * 16);
+  }
   auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
     if (use_patch)
       launch_patch_apply<P, X>(st, S, n, c.dp_d2p_ptr.as<int32_t>(), c.dp_d2p_patch.as<int32_t>(),
@@ -648,6 +673,16 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     else
       launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                             v, ldv_sys, z, act, (int64_t)nd * n);
+    if (use_coarse) {  // multiplicative second level: z += C Ac^-1 C^T (v - A z)
+      const int nc = c.coarse.nc;
+      launch_spmv<X, X>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), vals, nnz,
+                        W.masks.as<uint8_t>(), z, W.t.as<X>(), nullptr, act, 0, c.lpr, nnz);
+      launch_coarse_restrict<X>(st, S, nc, c.dc_col_ptr.as<int32_t>(), c.dc_col_row.as<int32_t>(),
+                                c.dc_col_val.as<double>(), v, ldv_sys, W.t.as<X>(), n, W.rc.as<double>(), act);
+      launch_coarse_solve(st, S, R, nc, W.cinv.as<double>(), W.rc.as<double>(), W.yc.as<double>(), act);
+      launch_coarse_prolong<X>(st, S, n, nc, c.dc_row_ptr.as<int32_t>(), c.dc_row_col.as<int32_t>(),
+                               c.dc_row_val.as<double>(), W.yc.as<double>(), z, act);
+    }
   };
   const int64_t ldv = (int64_t)(m + 1) * n;
   W.V.alloc((size_t)S * ldv * sizeof(X));
@@ -665,8 +700,8 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   W.active.alloc((size_t)S * 4);
   W.cyc_active.alloc((size_t)S * 4);
   W.kfinal.alloc((size_t)S * 4);
-  W.partial.alloc((size_t)S * nblk * 130 * 16);
-  W.hbuf.alloc((size_t)S * 130 * 16);
+  W.partial.alloc((size_t)S * nblk * kMaxBasis * 16);
+  W.hbuf.alloc((size_t)S * kMaxBasis * 16);
   W.nrm2.alloc((size_t)S * 8);
   std::vector<double> rn2(S), thresh(S);
   std::vector<int32_t> act(S), kf(S);
@@ -720,13 +755,13 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
                         nnz);
       double *Hcol = W.Hh.as<double>() + (int64_t)j * (m + 1) * 2;
       const int ldH = m * (m + 1);
-      for (int pass = 0; pass < 2; pass++) {
+      for (int pass = 0; pass < passes; pass++) {
         launch_mdot<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.partial.as<double>(), nblk,
                        W.active.as<int32_t>());
-        launch_reduce(st, S, nblk, j + 1, W.partial.as<double>(), W.hbuf.as<double>(), 130, Hcol, ldH,
+        launch_reduce(st, S, nblk, j + 1, W.partial.as<double>(), W.hbuf.as<double>(), kMaxBasis, Hcol, ldH,
                       W.active.as<int32_t>());
-        launch_mupdate<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.hbuf.as<double>(), 130,
-                          pass == 1 ? W.partial.as<double>() : nullptr, nblk, W.active.as<int32_t>());
+        launch_mupdate<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.hbuf.as<double>(), kMaxBasis,
+                          pass == passes - 1 ? W.partial.as<double>() : nullptr, nblk, W.active.as<int32_t>());
       }
       launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0,
                     W.active.as<int32_t>());
@@ -779,7 +814,7 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   al.alloc(S * 16);
   nal.alloc(S * 16);
   act.alloc(S * 4);
-  part.alloc((size_t)S * nblk * 130 * 16);
+  part.alloc((size_t)S * nblk * kMaxBasis * 16);
   part2.alloc((size_t)S * 16);
   // b = Qs x, then scale by j/w per system
   launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), c.d_Qs.as<double2>(),
@@ -856,6 +891,38 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Gauss-Jordan inverse with partial pivoting (small dense complex matrices on the host)
+static void dense_inverse(int n, const std::vector<cplx> &A0, std::vector<cplx> &X) {
+  std::vector<cplx> A(A0);
+  X.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) X[(size_t)i * n + i] = 1.0;
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::abs(A[(size_t)i * n + k]) > std::abs(A[(size_t)p * n + k])) p = i;
+    if (std::abs(A[(size_t)p * n + k]) == 0.0) throw std::runtime_error("singular coarse matrix");
+    if (p != k)
+      for (int c2 = 0; c2 < n; c2++) {
+        std::swap(A[(size_t)k * n + c2], A[(size_t)p * n + c2]);
+        std::swap(X[(size_t)k * n + c2], X[(size_t)p * n + c2]);
+      }
+    const cplx inv = 1.0 / A[(size_t)k * n + k];
+    for (int c2 = 0; c2 < n; c2++) {
+      A[(size_t)k * n + c2] *= inv;
+      X[(size_t)k * n + c2] *= inv;
+    }
+    for (int i = 0; i < n; i++) {
+      if (i == k) continue;
+      const cplx f = A[(size_t)i * n + k];
+      if (f == 0.0) continue;
+      for (int c2 = 0; c2 < n; c2++) {
+        A[(size_t)i * n + c2] -= f * A[(size_t)k * n + c2];
+        X[(size_t)i * n + c2] -= f * X[(size_t)k * n + c2];
+      }
+    }
+  }
+}
+
 static void calculate_s_host(int N, const cplx *Cp, const cplx *Cm, const cplx *V, const cplx *Z, cplx *S) {
   // port.cpp:7499-7606: row (i,j): sum_k Cm[i][k] w_k S[j][k] = Cp[i][j] w_j, w = V/sqrt(Z)
   const int n = N * N;
@@ -910,6 +977,40 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   if ((int)c.ports.size() != np) throw std::invalid_argument("every port boundary must be an S-port");
   Overlay ov;
   build_overlay(c, ov);
+  // second level: vertices on PEC and port faces are excluded
+  if (c.coarse_size > 0) {
+    std::vector<uint8_t> vex(c.mesh.nv, 0);
+    for (int64_t f = 0; f < c.mesh.nbf; f++) {
+      auto it = c.bdr.find(c.mesh.bface_attr[f]);
+      if (it != c.bdr.end() && (it->second.kind == OPEM_BDR_PEC || it->second.kind == OPEM_BDR_PORT))
+        for (int k = 0; k < 3; k++) vex[c.mesh.bfaces[f * 3 + k]] = 1;
+    }
+    if (!c.have_coarse || vex != c.coarse_excl || c.coarse_T.size() != ov.terms.size()) {
+      int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      build_coarse(c.mesh, c.sp, vex, c.coarse_size, c.coarse);
+      c.coarse_excl = vex;
+      const int ns = (int)c.slot_attr.size();
+      std::vector<double> Mh((size_t)ns * c.sp.nnz);
+      HIPCHK(hipMemcpy(Mh.data(), c.d_M.p, Mh.size() * 8, hipMemcpyDeviceToHost));
+      c.coarse_M.assign(ns, {});
+      for (int sl = 0; sl < ns; sl++)
+        coarse_galerkin(c.sp, c.coarse, Mh.data() + (size_t)sl * c.sp.nnz, nullptr, nullptr, 0, c.coarse_M[sl], nth);
+      const size_t nov = ov.nz.size();
+      c.coarse_T.assign(ov.terms.size(), {});
+      for (size_t t = 0; t < ov.terms.size(); t++)
+        coarse_galerkin(c.sp, c.coarse, nullptr, ov.nz.data(), ov.val.data() + t * nov, (int64_t)nov, c.coarse_T[t],
+                        nth);
+      c.dc_row_ptr.upload(c.coarse.row_ptr, st);
+      c.dc_row_col.upload(c.coarse.row_col, st);
+      c.dc_row_val.upload(c.coarse.row_val, st);
+      c.dc_col_ptr.upload(c.coarse.col_ptr, st);
+      c.dc_col_row.upload(c.coarse.col_row, st);
+      c.dc_col_val.upload(c.coarse.col_val, st);
+      c.have_coarse = true;
+    }
+  } else {
+    c.have_coarse = false;
+  }
   DevBuf d_ovnz, d_ovval;
   d_ovnz.upload(ov.nz, st);
   d_ovval.upload(ov.val, st);
@@ -971,7 +1072,8 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   d_pd.upload(pd, st);
   const int nf = in->nfreq;
   const int batch = std::max(1, std::min(in->batch > 0 ? in->batch : nf, nf));
-  const int m = std::max(2, std::min(in->restart > 0 ? in->restart : 50, 128));
+  const int m = std::max(2, std::min(in->restart > 0 ? in->restart : 150, kMaxBasis - 2));
+  const int passes = in->orth_passes == 1 ? 1 : 2;
   const double tol = in->tol > 0 ? in->tol : 1e-10;
   const double htol = in->h_tol > 0 ? in->h_tol : 1e-13;
   HIPCHK(hipStreamSynchronize(st));
@@ -1025,6 +1127,30 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       }
       frequency_coefs(c, ov, in->freqs[f0 + fi], gam, cK, cM, cT);
     }
+    if (c.have_coarse && c.coarse.nc > 0) {
+      // Ac(w) = sum_slot cM C^T M_slot C + sum_t cT C^T T_t C  (C^T K C = 0: curl grad = 0), inverted on the host
+      const int nc = c.coarse.nc;
+      std::vector<double> cinv((size_t)F * nc * nc * 2);
+      for (int fi = 0; fi < F; fi++) {
+        std::vector<cplx> Ac((size_t)nc * nc, 0.0);
+        for (int sl = 0; sl < nslot; sl++) {
+          const cplx cm(cM[(fi * nslot + sl) * 2], cM[(fi * nslot + sl) * 2 + 1]);
+          for (size_t k = 0; k < Ac.size(); k++) Ac[k] += cm * c.coarse_M[sl][k];
+        }
+        const int nt_ = (int)ov.terms.size();
+        for (int t = 0; t < nt_; t++) {
+          const cplx ct(cT[(fi * nt_ + t) * 2], cT[(fi * nt_ + t) * 2 + 1]);
+          for (size_t k = 0; k < Ac.size(); k++) Ac[k] += ct * c.coarse_T[t][k];
+        }
+        std::vector<cplx> inv;
+        dense_inverse(nc, Ac, inv);
+        for (size_t k = 0; k < inv.size(); k++) {
+          cinv[((size_t)fi * nc * nc + k) * 2] = inv[k].real();
+          cinv[((size_t)fi * nc * nc + k) * 2 + 1] = inv[k].imag();
+        }
+      }
+      W.cinv.upload(cinv, st);
+    }
     DevBuf dcK, dcM, dcT;
     dcK.upload(cK, st);
     dcM.upload(cM, st);
@@ -1063,7 +1189,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     {
       const int nblk = reduce_blocks(n);
       W.r.alloc((size_t)S * n * 16);
-      W.partial.alloc((size_t)S * nblk * 130 * 16);
+      W.partial.alloc((size_t)S * nblk * kMaxBasis * 16);
       W.nrm2.alloc((size_t)S * 8);
       launch_spmv<double2, double2>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
                                     W.A64.as<double2>(), nnz, W.masks.as<uint8_t>(), W.x.as<double2>(),
@@ -1081,9 +1207,9 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     std::vector<int> iters, conv;
     std::vector<double> relres;
     if (mixed)
-      gmres_batch<float2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, iters, relres, conv, bn);
+      gmres_batch<float2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, iters, relres, conv, bn);
     else
-      gmres_batch<double2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, iters, relres, conv, bn);
+      gmres_batch<double2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, iters, relres, conv, bn);
     HIPCHK(hipStreamSynchronize(st));
     const double tc = now_s();
     t_solve += tc - tb;
@@ -1393,7 +1519,10 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
     if (in->precision != OPEM_PREC_FP64 && in->precision != OPEM_PREC_MIXED)
       throw std::invalid_argument("bad precision");
     if (in->precond < 0 || in->precond > 1) throw std::invalid_argument("bad preconditioner");
+    if (in->coarse_size < 0 || in->coarse_size > 128) throw std::invalid_argument("coarse_size must be 0..128");
     ctx->precond = in->precond;
+    if (ctx->coarse_size != in->coarse_size) ctx->have_coarse = false;
+    ctx->coarse_size = in->coarse_size;
     sweep(*ctx, in, out);
   });
 }
@@ -1440,6 +1569,56 @@ int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const 
         for (size_t i = 0; i < cnt; i++) out[i] = tmp[i];
       }
     }
+  });
+}
+
+int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, double *avg_seconds, double *alg_bytes) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (ngroups < 1 || R < 1 || R > 4 || nrep < 1 || (prec != 0 && prec != 1))
+      throw std::invalid_argument("bad spmv bench arguments");
+    const Space &s = ctx->sp;
+    hipStream_t st = ctx->st;
+    const int64_t n = s.ndof, nnz = s.nnz;
+    const int S = ngroups * R;
+    const size_t sv = prec == 0 ? 16 : 8;
+    DevBuf vals, x, y, mask;
+    vals.alloc((size_t)ngroups * nnz * sv);
+    x.alloc((size_t)S * n * sv);
+    y.alloc((size_t)S * n * sv);
+    mask.upload(std::vector<uint8_t>((size_t)R * n, 1), st);
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    auto run = [&]() {
+      if (prec == 0)
+        launch_spmv<double2, double2>(st, ngroups, R, n, ctx->d_rowptr.as<int32_t>(), ctx->d_colind.as<int32_t>(),
+                                      vals.as<double2>(), nnz, mask.as<uint8_t>(), x.as<double2>(), y.as<double2>(),
+                                      nullptr, nullptr, 0, ctx->lpr, nnz);
+      else
+        launch_spmv<float2, float2>(st, ngroups, R, n, ctx->d_rowptr.as<int32_t>(), ctx->d_colind.as<int32_t>(),
+                                    vals.as<float2>(), nnz, mask.as<uint8_t>(), x.as<float2>(), y.as<float2>(),
+                                    nullptr, nullptr, 0, ctx->lpr, nnz);
+    };
+    if (prec == 0) {
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)ngroups * nnz, vals.as<double2>(), 1.0);
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, x.as<double2>(), 1.0);
+    } else {
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)ngroups * nnz, vals.as<float2>(), 1.0);
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, x.as<float2>(), 1.0);
+    }
+    run();
+    HIPCHK(hipEventRecord(a, st));
+    for (int r = 0; r < nrep; r++) run();
+    HIPCHK(hipEventRecord(b, st));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (avg_seconds) *avg_seconds = ms * 1e-3 / nrep;
+    if (alg_bytes)
+      *alg_bytes = 4.0 * nnz + 4.0 * (n + 1) + (double)ngroups * sv * nnz + (double)S * 2.0 * n * sv;
   });
 }
 

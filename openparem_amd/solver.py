@@ -96,9 +96,16 @@ class Solver:
         check(lib().opem_port_points(self.ctx, attr, C.byref(n), ptr(pts, C.c_double)), self.ctx, "port_points")
         return pts
 
+    def spmv_bench(self, precision="fp32", ngroups=1, R=2, nrep=20):
+        """(mean kernel seconds, algorithmic bytes) of the batched SpMV on this pattern."""
+        t, b = C.c_double(), C.c_double()
+        check(lib().opem_spmv_bench(self.ctx, 0 if precision == "fp64" else 1, ngroups, R, nrep, C.byref(t),
+                                    C.byref(b)), self.ctx, "opem_spmv_bench")
+        return t.value, b.value
+
     # ------------------------------------------------------------------ hot path
-    def sweep(self, freqs, precision="fp64", batch=0, restart=50, maxit=20000, tol=1e-10, reciprocal=True,
-              h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch"):
+    def sweep(self, freqs, precision="fp64", batch=0, restart=150, maxit=20000, tol=1e-10, reciprocal=True,
+              h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch", coarse=0, orth_passes=2):
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
         nf, np_ = freqs.size, len(self.ports)
         mode_fn = mode_fn or provider_for(self.mesh)
@@ -123,7 +130,7 @@ class Solver:
                            precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64,
                            batch=int(batch), restart=int(restart), maxit=int(maxit), tol=float(tol),
                            reciprocal=int(bool(reciprocal)), h_tol=float(h_tol),
-                           precond={"patch": 0, "element": 1}[precond])
+                           precond={"patch": 0, "element": 1}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes))
         S = np.zeros((nf, np_, np_), np.complex128)
         Sr = np.zeros_like(S)
         Cp = np.zeros_like(S)

@@ -189,13 +189,14 @@ def system_matrix(space, freq, port_gamma, rs_of_attr=None, eta_rad=None, Ke=Non
     return A
 
 
-def h_matrices(space, freq):
+def h_matrices(space, freq, Me=None):
     """P (mass, coefficient 1) and Q with 1/(omega mu) (fem3D.cpp:192-225); Q is real, used as jQ."""
     mesh = space.mesh
     omega = 2 * np.pi * freq
     mu = materials_per_element(mesh, "mur") * MU0
     v = space.sorted_coords()
-    _, Me = nd.element_matrices(v, space.p)
+    if Me is None:
+        _, Me = nd.element_matrices(v, space.p)
     Qe = nd.mixed_curl_matrices(v, space.p)
     P = _scatter(space.elem_dofs, Me, space.ndof)
     Q = _scatter(space.elem_dofs, Qe / (omega * mu)[:, None, None], space.ndof)

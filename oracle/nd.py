@@ -193,9 +193,12 @@ def element_matrices(v, p, nquad=None):
     L, w = tet_rule(nquad or p + 2)
     g, det = tet_gradients(v)
     phi, curl = eval_basis(basis, L, g)
-    wd = w[None, :] * np.abs(det)[:, None]                                 # (ne,nq)
-    M = np.einsum("eq,eqak,eqbk->eab", wd, phi, phi, optimize=True)
-    K = np.einsum("eq,eqak,eqbk->eab", wd, curl, curl, optimize=True)
+    wd = np.sqrt(w[None, :] * np.abs(det)[:, None])[:, :, None, None]     # (ne,nq,1,1)
+    ne, nq, nd_ = phi.shape[:3]
+    P = (phi * wd).transpose(0, 1, 3, 2).reshape(ne, nq * 3, nd_)
+    C = (curl * wd).transpose(0, 1, 3, 2).reshape(ne, nq * 3, nd_)
+    M = np.matmul(P.transpose(0, 2, 1), P)
+    K = np.matmul(C.transpose(0, 2, 1), C)
     return K, M
 
 
@@ -218,5 +221,8 @@ def mixed_curl_matrices(v, p, nquad=None):
     L, w = tet_rule(nquad or p + 2)
     g, det = tet_gradients(v)
     phi, curl = eval_basis(basis, L, g)
-    wd = w[None, :] * np.abs(det)[:, None]
-    return np.einsum("eq,eqak,eqbk->eab", wd, phi, curl, optimize=True)
+    wd = np.sqrt(w[None, :] * np.abs(det)[:, None])[:, :, None, None]
+    ne, nq, nd_ = phi.shape[:3]
+    P = (phi * wd).transpose(0, 1, 3, 2).reshape(ne, nq * 3, nd_)
+    C = (curl * wd).transpose(0, 1, 3, 2).reshape(ne, nq * 3, nd_)
+    return np.matmul(P.transpose(0, 2, 1), C)

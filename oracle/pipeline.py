@@ -62,7 +62,7 @@ class Problem:
         sp_ = self.space
         gam = {a: modes[a]["gamma"] for a in self.ports}
         A = asm.system_matrix(sp_, freq, gam, rs_of_attr=rs, Ke=self.Ke, Me=self.Me)
-        P, Q = asm.h_matrices(sp_, freq)
+        P, Q = asm.h_matrices(sp_, freq, Me=self.Me)
         Em, Hm, xm = {}, {}, {}
         for a in self.ports:
             tr = self.traces[a]
@@ -90,7 +90,12 @@ class Problem:
                 dmask = np.zeros(sp_.ndof, bool)
                 dmask[self.pec] = True
                 dmask[tr.port_dofs] = True
-                M = slv.BlockJacobi(At, sp_.elem_dofs, dmask) if solver == "gmres-bj" else None
+                if solver == "gmres-bj":
+                    M = slv.BlockJacobi(At, sp_.elem_dofs, dmask)
+                elif solver == "gmres-patch":
+                    M = slv.VertexPatchASM(At, sp_, dmask)
+                else:
+                    M = None
                 x, its, ok, _ = slv.gmres(At, b, M=M, restart=restart, rtol=rtol)
             h = slv.h_field(P, Q, x)
             res = np.linalg.norm(b - At @ x) / np.linalg.norm(b)

@@ -82,6 +82,59 @@ class BlockJacobi:
         return z
 
 
+class VertexPatchASM:
+    """Additive Schwarz over vertex patches (Arnold, Falk, Winther, Numer. Math. 85 (2000)):
+    patch v = DOFs of the edges, faces and cells incident to vertex v, minus Dirichlet DOFs;
+    M^{-1} r = sum_v R_v^T (R_v A R_v^T)^{-1} R_v r.  Patches of equal size are inverted and
+    applied as one batched dense product."""
+
+    def __init__(self, A, space, dirichlet_mask):
+        A = A.tocsr()
+        p = space.p
+        ke, kf, kc = nd.ndofs_per_entity(p)
+        ents = []
+        for e in nd.TET_EDGES:
+            ents += [e] * ke
+        for f in nd.TET_FACES:
+            ents += [f] * kf
+        ents += [(0, 1, 2, 3)] * kc
+        ts = space.tets_sorted
+        rows, cols = [], []
+        for a, ent in enumerate(ents):
+            d = space.elem_dofs[:, a]
+            for k in ent:
+                rows.append(ts[:, k])
+                cols.append(d)
+        v = np.concatenate(rows)
+        d = np.concatenate(cols)
+        keep = ~dirichlet_mask[d]
+        pairs = np.unique(np.stack([v[keep], d[keep]], 1), axis=0)       # (vertex, dof), sorted
+        verts, start, counts = np.unique(pairs[:, 0], return_index=True, return_counts=True)
+        self.groups = []
+        self.n = n = A.shape[0]
+        self.dmask = dirichlet_mask
+        A.sort_indices()
+        keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(A.indptr)) * n + A.indices
+        for size in np.unique(counts):
+            sel = np.flatnonzero(counts == size)
+            idx = pairs[start[sel][:, None] + np.arange(size)[None, :], 1]  # (np_, size)
+            q = (idx[:, :, None].astype(np.int64) * n + idx[:, None, :]).ravel()
+            pos = np.searchsorted(keys, q)
+            pos = np.minimum(pos, keys.size - 1)
+            hit = keys[pos] == q
+            blocks = np.where(hit, A.data[pos], 0.0).reshape(len(sel), size, size)
+            self.groups.append((idx, np.linalg.inv(blocks)))
+
+    def __call__(self, r):
+        z = np.zeros(self.n, np.complex128)
+        for idx, inv in self.groups:
+            zi = np.matmul(inv, r[idx][..., None])[..., 0].ravel()
+            f = idx.ravel()
+            z += np.bincount(f, zi.real, self.n) + 1j * np.bincount(f, zi.imag, self.n)
+        z[self.dmask] = r[self.dmask]
+        return z
+
+
 def gmres(A, b, M=None, x0=None, restart=50, rtol=1e-10, maxit=20000):
     """Right-preconditioned restarted GMRES(m) with CGS2 Arnoldi; converged on the true residual."""
     n = b.size
@@ -107,7 +160,7 @@ def gmres(A, b, M=None, x0=None, restart=50, rtol=1e-10, maxit=20000):
             Z[k] = M(V[k])
             w = A @ Z[k]
             for _ in range(2):
-                h = V[: k + 1].conj() @ w
+                h = np.conj(V[: k + 1] @ np.conj(w))
                 w = w - h @ V[: k + 1]
                 H[: k + 1, k] += h
             H[k + 1, k] = np.linalg.norm(w)
@@ -122,9 +175,28 @@ def gmres(A, b, M=None, x0=None, restart=50, rtol=1e-10, maxit=20000):
     return x, it, np.linalg.norm(r) <= rtol * bn, hist
 
 
-def h_field(P, Q, x):
-    """h = P^{-1} (j Q x)   (fem3D.cpp:610-648)."""
-    return spla.splu(P.tocsc().astype(np.complex128)).solve(1j * (Q @ x))
+def h_field(P, Q, x, rtol=1e-14):
+    """h = P^{-1} (j Q x)   (fem3D.cpp:610-648), P SPD: Jacobi-preconditioned CG, FP64."""
+    b = 1j * (Q @ x)
+    d = 1.0 / P.diagonal()
+    h = np.zeros_like(b)
+    r = b.copy()
+    z = d * r
+    p = z.copy()
+    rz = np.vdot(r, z).real
+    bn = np.linalg.norm(b)
+    for _ in range(20 * b.size):
+        q = P @ p
+        a = rz / np.vdot(p, q).real
+        h += a * p
+        r -= a * q
+        if np.linalg.norm(r) <= rtol * bn:
+            break
+        z = d * r
+        rz2 = np.vdot(r, z).real
+        p = z + (rz2 / rz) * p
+        rz = rz2
+    return h
 
 
 # ----------------------------------------------------------------------------- ports

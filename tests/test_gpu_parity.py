@@ -156,3 +156,60 @@ def test_sweep_mixed_vs_fp64():
     assert r64["converged"].all() and r32["converged"].all()
     assert np.abs(r64["S"] - r32["S"]).max() < 1e-6
     assert (r32["true_res"] <= 1e-10).all()
+
+
+def test_reference_mesh_gpu():
+    """GPU S on the reference's own 30-tet WR90 mesh: equal to the oracle, and to the reference's
+    recorded S (results.ods) at the discretization level of the port mode (see test_oracle)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import wr90
+    from openparem_amd.ports import te10
+    m, d = wr90.load()
+    for p, tol in ((2, 0.03), (3, 0.005)):
+        s = _solver(m, p)
+        try:
+            r = s.sweep(np.array([9e9]), precision="fp64", tol=1e-12)
+        finally:
+            s.close()
+        assert r["converged"].all()
+        pr = Problem(m, p)
+        o = pr.solve_point(9e9, {a: te10(m, a, 9e9) for a in pr.ports})
+        assert np.abs(r["S"][0] - o["S"]).max() < 1e-9
+        s11, s21 = wr90.reference_s(d, p)
+        assert abs(r["S"][0, 0, 0] - s11) < tol and abs(r["S"][0, 1, 0] - s21) < tol
+
+
+@pytest.mark.parametrize("coarse,passes", [(0, 2), (32, 1)])
+def test_sweep_microstrip(coarse, passes):
+    """Lossy substrate, surface-impedance trace, radiation lid, quasi-TEM ports: GPU vs oracle."""
+    from openparem_amd.microstrip_mode import quasi_tem
+    m = microstrip_mesh(nx=8, ny=5, nz=8, n_sub=2, seed=6)
+    f = np.array([3e9, 6e9])
+    s = _solver(m, 2)
+    try:
+        r = s.sweep(f, precision="mixed", tol=1e-11, batch=2, coarse=coarse, orth_passes=passes)
+    finally:
+        s.close()
+    assert r["converged"].all()
+    pr = Problem(m, 2)
+    for i, fi in enumerate(f):
+        rs = np.sqrt(np.pi * fi * oasm.MU0 / COPPER)
+        o = pr.solve_point(fi, {a: quasi_tem(m, a, fi) for a in pr.ports}, rs={4: rs})
+        assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
+
+
+def test_sweep_nd3():
+    """ND3 (vertex patches exceed 100 DOFs: element-block Jacobi is used)."""
+    from openparem_amd.ports import te10
+    m = waveguide_mesh(nx=3, ny=2, nz=4, seed=7)
+    s = _solver(m, 3)
+    try:
+        r = s.sweep(np.array([10e9]), precision="fp64", tol=1e-11, restart=200)
+    finally:
+        s.close()
+    assert r["converged"].all()
+    pr = Problem(m, 3)
+    o = pr.solve_point(10e9, {a: te10(m, a, 10e9) for a in pr.ports})
+    assert np.abs(r["S"][0] - o["S"]).max() < 1e-8
