@@ -1,6 +1,7 @@
 // Device kernels of the per-frequency solve path (gfx950) and their host launchers.
 // All launchers are asynchronous on the given stream.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -80,17 +81,26 @@ void launch_bj_apply(hipStream_t st, int S, int64_t n, int nd, const int32_t *el
                      const int32_t *active, int64_t inv_stride);
 
 // ---------------------------------------------------------------- vertex-patch additive Schwarz
-// inv[s] = blockwise (A_s restricted to each patch, driven-port DOFs of system s as identity)^-1.
-// One workgroup per (patch, system), Gauss-Jordan with partial pivoting in LDS (FP64, n_p <= 100).
+// Device view of the patch structure.  Patches without port DOFs ("shared") have one inverse per
+// matrix group; patches touching a port DOF have one per excitation (their Dirichlet rows differ).
+struct PatchLayout {
+  const int32_t *idx_off, *idx;      // patch DOF lists
+  const int64_t *val_off;            // offset of the n_p x n_p block in its region
+  const int64_t *nz_off;             // offset of the block's CSR-position map in pnz
+  const uint8_t *port;               // per patch: excitation-dependent
+  const int32_t *pnz;                // CSR position of (idx[i], idx[j]) or -1
+  const int32_t *d2p_ptr, *d2p_patch, *d2p_row;  // DOF -> (patch, local row)
+  int64_t group_stride, shared_size, exc_size, npatch;
+  int nmax;
+};
+// inv: per-group storage (group_stride elements each); FP16 storage takes per-block scales.
 template <typename P>
-void launch_patch_build(hipStream_t st, int S, int R, int64_t npatch, int nmax, const int32_t *idx_off,
-                        const int32_t *idx, const int64_t *val_off, const int32_t *pnz, const double *a64,
-                        int64_t nnz, const uint8_t *rowmask, int64_t n, P *inv, int64_t inv_stride);
+void launch_patch_build(hipStream_t st, const PatchLayout &L, int S, int R, const double *a64, int64_t nnz,
+                        const uint8_t *rowmask, int64_t n, P *inv, float *scale);
 // z[s][d] = sum_{patches p containing d} (inv_p r_p)[d]   (deterministic gather over DOF rows)
 template <typename P, typename X>
-void launch_patch_apply(hipStream_t st, int S, int64_t n, const int32_t *d2p_ptr, const int32_t *d2p_patch,
-                        const int32_t *d2p_row, const int32_t *idx_off, const int32_t *idx, const int64_t *val_off,
-                        const P *inv, int64_t inv_stride, const X *v, int64_t ldv_sys, X *z, const int32_t *active);
+void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R, int64_t n, const P *inv,
+                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active);
 
 // ---------------------------------------------------------------- coarse space (two-level Schwarz)
 // rc[s][a] = sum_i C_ia (v[s][i] - t[s][i])   (C^T by coarse rows; t may be null)

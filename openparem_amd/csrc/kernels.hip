@@ -2,6 +2,7 @@
 // See kernels.h for the contracts.
 #include "kernels.h"
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -40,6 +41,11 @@ __device__ __forceinline__ float2 from_d2<float2>(double2 v) {
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -636,23 +642,42 @@ template void launch_bj_apply<float2, float2>(hipStream_t, int, int64_t, int, co
                                               int64_t);
 
 // ------------------------------------------------------------------ vertex-patch additive Schwarz
+// Storage per matrix group g: [shared region: patches with no port DOF, one inverse for all excitations]
+// [R excitation regions: patches touching a port DOF].  Element offset of patch p, group g, excitation r:
+//   g*group_stride + (port[p] ? shared_size + r*exc_size : 0) + val_off[p].
+// FP16 storage keeps one scale per (group, excitation, patch) block: value = half * scale.
+__device__ __forceinline__ double2 ldp(double2 v) { return v; }
+__device__ __forceinline__ double2 ldp(__half2 v) {
+  return make_double2((double)__half2float(v.x), (double)__half2float(v.y));
+}
+__device__ __forceinline__ void stp(double2 *o, double2 v, double) { *o = v; }
+__device__ __forceinline__ void stp(__half2 *o, double2 v, double inv_scale) {
+  __half2 h;
+  h.x = __float2half((float)(v.x * inv_scale));
+  h.y = __float2half((float)(v.y * inv_scale));
+  *o = h;
+}
+
+__device__ __forceinline__ int64_t patch_offset(const PatchLayout &L, int p, int g, int r) {
+  return (int64_t)g * L.group_stride + (L.port[p] ? L.shared_size + (int64_t)r * L.exc_size : 0) + L.val_off[p];
+}
+
 template <typename P>
-__global__ __launch_bounds__(256) void k_patch_build(int R, const int32_t *__restrict__ idx_off,
-                                                     const int32_t *__restrict__ idx,
-                                                     const int64_t *__restrict__ val_off,
-                                                     const int32_t *__restrict__ pnz, const double2 *__restrict__ a64,
+__global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int R, const double2 *__restrict__ a64,
                                                      int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
-                                                     P *__restrict__ inv, int64_t inv_stride) {
+                                                     P *__restrict__ inv, float *__restrict__ scale) {
   extern __shared__ double2 lds[];
   const int p = blockIdx.x, s = blockIdx.y;
   const int g = s / R, r = s % R;
-  const int np = idx_off[p + 1] - idx_off[p];
+  if (!L.port[p] && r != 0) return;  // shared inverse: built once per group
+  const int np = L.idx_off[p + 1] - L.idx_off[p];
   double2 *A = lds;                          // np*np
   double2 *colk = lds + np * np;             // np
   __shared__ int perm[128];
   __shared__ double2 pinv_s;
-  const int32_t *I = idx + idx_off[p];
-  const int32_t *nz = pnz + val_off[p];
+  __shared__ double red[4];
+  const int32_t *I = L.idx + L.idx_off[p];
+  const int32_t *nz = L.pnz + L.nz_off[p];
   const double2 *Ag = a64 + (int64_t)g * nnz;
   const uint8_t *mask = rowmask + (int64_t)r * n;
   const int tid = threadIdx.x;
@@ -732,16 +757,27 @@ __global__ __launch_bounds__(256) void k_patch_build(int R, const int32_t *__res
       }
     __syncthreads();
   }
-  P *out = inv + (int64_t)s * inv_stride + val_off[p];
-  for (int t = tid; t < nn; t += blockDim.x) out[t] = from_d2<P>(A[t]);
+  double sc = 1.0;
+  if (scale) {  // block scale: largest |Re|,|Im| maps to 1 in the half-precision store
+    double m = 0.0;
+    for (int t = tid; t < nn; t += blockDim.x) m = fmax(m, fmax(fabs(A[t].x), fabs(A[t].y)));
+    m = wave_sum_max(m);
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    sc = m > 0.0 ? m : 1.0;
+    if (tid == 0) scale[((int64_t)g * R + (L.port[p] ? r : 0)) * L.npatch + p] = (float)sc;
+  }
+  P *out = inv + patch_offset(L, p, g, r);
+  const double isc = 1.0 / sc;
+  for (int t = tid; t < nn; t += blockDim.x) stp(out + t, A[t], isc);
 }
 
 template <typename P>
-void launch_patch_build(hipStream_t st, int S, int R, int64_t npatch, int nmax, const int32_t *idx_off,
-                        const int32_t *idx, const int64_t *val_off, const int32_t *pnz, const double *a64,
-                        int64_t nnz, const uint8_t *rowmask, int64_t n, P *inv, int64_t inv_stride) {
-  if (nmax > 100) throw std::runtime_error("vertex patch larger than 100 DOFs");
-  const size_t lds = (size_t)(nmax * nmax + nmax) * sizeof(double2);
+void launch_patch_build(hipStream_t st, const PatchLayout &L, int S, int R, const double *a64, int64_t nnz,
+                        const uint8_t *rowmask, int64_t n, P *inv, float *scale) {
+  if (L.nmax > 100) throw std::runtime_error("vertex patch larger than 100 DOFs");
+  const size_t lds = (size_t)(L.nmax * L.nmax + L.nmax) * sizeof(double2);
   static size_t attr_bytes = 0;
   if (lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute((const void *)k_patch_build<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -752,72 +788,123 @@ void launch_patch_build(hipStream_t st, int S, int R, int64_t npatch, int nmax, 
     }
     attr_bytes = lds;
   }
-  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)npatch, S), dim3(256), lds, st, R, idx_off, idx, val_off,
-                     pnz, (const double2 *)a64, nnz, rowmask, n, inv, inv_stride);
+  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)L.npatch, S), dim3(256), lds, st, L, R,
+                     (const double2 *)a64, nnz, rowmask, n, inv, scale);
   HIP_LAUNCH_CHECK();
 }
-template void launch_patch_build<double2>(hipStream_t, int, int, int64_t, int, const int32_t *, const int32_t *,
-                                          const int64_t *, const int32_t *, const double *, int64_t, const uint8_t *,
-                                          int64_t, double2 *, int64_t);
-template void launch_patch_build<float2>(hipStream_t, int, int, int64_t, int, const int32_t *, const int32_t *,
-                                         const int64_t *, const int32_t *, const double *, int64_t, const uint8_t *,
-                                         int64_t, float2 *, int64_t);
+template void launch_patch_build<double2>(hipStream_t, const PatchLayout &, int, int, const double *, int64_t,
+                                          const uint8_t *, int64_t, double2 *, float *);
+template void launch_patch_build<__half2>(hipStream_t, const PatchLayout &, int, int, const double *, int64_t,
+                                          const uint8_t *, int64_t, __half2 *, float *);
 
 constexpr int kPatchLanes = 16;
 
-template <typename P, typename X>
-__global__ __launch_bounds__(256) void k_patch_apply(int64_t n, const int32_t *__restrict__ d2p_ptr,
-                                                     const int32_t *__restrict__ d2p_patch,
-                                                     const int32_t *__restrict__ d2p_row,
-                                                     const int32_t *__restrict__ idx_off,
-                                                     const int32_t *__restrict__ idx,
-                                                     const int64_t *__restrict__ val_off, const P *__restrict__ inv,
-                                                     int64_t inv_stride, const X *__restrict__ v, int64_t ldv_sys,
-                                                     X *__restrict__ z, const int32_t *__restrict__ active) {
-  const int s = blockIdx.y;
-  if (active && !active[s]) return;
+// One 16-lane group per DOF row d and matrix group g; the R excitations of the group share every
+// shared-patch row read.
+template <typename P, typename X, int R>
+__global__ __launch_bounds__(256) void k_patch_apply(PatchLayout L, int64_t n, const P *__restrict__ inv,
+                                                     const float *__restrict__ scale, const X *__restrict__ v,
+                                                     int64_t ldv_sys, X *__restrict__ z,
+                                                     const int32_t *__restrict__ active) {
+  const int g = blockIdx.y;
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < R; r++) any |= (active == nullptr) || active[g * R + r];
+  if (!any) return;
   const int sub = threadIdx.x % kPatchLanes;
   const int64_t d = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPatchLanes;
   if (d >= n) return;
-  const X *vs = v + s * ldv_sys;
-  const P *is = inv + s * inv_stride;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int k = d2p_ptr[d]; k < d2p_ptr[d + 1]; k++) {
-    const int p = d2p_patch[k], l = d2p_row[k];
-    const int b0 = idx_off[p], np = idx_off[p + 1] - b0;
-    const P *row = is + val_off[p] + (int64_t)l * np;
-    const int32_t *I = idx + b0;
-    for (int j = sub; j < np; j += kPatchLanes) {
-      const double2 a = to_d2(row[j]);
-      const double2 x = to_d2(vs[I[j]]);
-      acc.x += a.x * x.x - a.y * x.y;
-      acc.y += a.x * x.y + a.y * x.x;
+  double2 acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = make_double2(0.0, 0.0);
+  for (int k = L.d2p_ptr[d]; k < L.d2p_ptr[d + 1]; k++) {
+    const int p = L.d2p_patch[k], l = L.d2p_row[k];
+    const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0;
+    const int32_t *I = L.idx + b0;
+    if (!L.port[p]) {
+      const P *row = inv + patch_offset(L, p, g, 0) + (int64_t)l * np;
+      const double sc = scale ? (double)scale[((int64_t)g * R) * L.npatch + p] : 1.0;
+      double2 t[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) t[r] = make_double2(0.0, 0.0);
+      for (int j = sub; j < np; j += kPatchLanes) {
+        const double2 a = ldp(row[j]);
+        const int c = I[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const double2 x = to_d2(v[((int64_t)g * R + r) * ldv_sys + c]);
+          t[r].x += a.x * x.x - a.y * x.y;
+          t[r].y += a.x * x.y + a.y * x.x;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        acc[r].x += sc * t[r].x;
+        acc[r].y += sc * t[r].y;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const P *row = inv + patch_offset(L, p, g, r) + (int64_t)l * np;
+        const double sc = scale ? (double)scale[((int64_t)g * R + r) * L.npatch + p] : 1.0;
+        double2 t = make_double2(0.0, 0.0);
+        for (int j = sub; j < np; j += kPatchLanes) {
+          const double2 a = ldp(row[j]);
+          const double2 x = to_d2(v[((int64_t)g * R + r) * ldv_sys + I[j]]);
+          t.x += a.x * x.x - a.y * x.y;
+          t.y += a.x * x.y + a.y * x.x;
+        }
+        acc[r].x += sc * t.x;
+        acc[r].y += sc * t.y;
+      }
     }
   }
 #pragma unroll
-  for (int o = kPatchLanes / 2; o > 0; o >>= 1) {
-    acc.x += __shfl_xor(acc.x, o, kPatchLanes);
-    acc.y += __shfl_xor(acc.y, o, kPatchLanes);
+  for (int r = 0; r < R; r++) {
+#pragma unroll
+    for (int o = kPatchLanes / 2; o > 0; o >>= 1) {
+      acc[r].x += __shfl_xor(acc[r].x, o, kPatchLanes);
+      acc[r].y += __shfl_xor(acc[r].y, o, kPatchLanes);
+    }
   }
-  if (sub == 0) z[s * n + d] = from_d2<X>(acc);
+  if (sub == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t s = (int64_t)g * R + r;
+      if (active && !active[s]) continue;
+      z[s * n + d] = from_d2<X>(acc[r]);
+    }
+  }
 }
 
 template <typename P, typename X>
-void launch_patch_apply(hipStream_t st, int S, int64_t n, const int32_t *d2p_ptr, const int32_t *d2p_patch,
-                        const int32_t *d2p_row, const int32_t *idx_off, const int32_t *idx, const int64_t *val_off,
-                        const P *inv, int64_t inv_stride, const X *v, int64_t ldv_sys, X *z, const int32_t *active) {
-  hipLaunchKernelGGL((k_patch_apply<P, X>), dim3((unsigned)((n * kPatchLanes + 255) / 256), S), dim3(256), 0, st, n,
-                     d2p_ptr, d2p_patch, d2p_row, idx_off, idx, val_off, inv, inv_stride, v, ldv_sys, z, active);
+void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R, int64_t n, const P *inv,
+                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active) {
+  dim3 grid((unsigned)((n * kPatchLanes + 255) / 256), ngroups);
+  switch (R) {
+    case 1:
+      hipLaunchKernelGGL((k_patch_apply<P, X, 1>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_patch_apply<P, X, 2>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_patch_apply<P, X, 3>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_patch_apply<P, X, 4>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      break;
+    default:
+      throw std::runtime_error("patch apply: R must be 1..4");
+  }
   HIP_LAUNCH_CHECK();
 }
-template void launch_patch_apply<double2, double2>(hipStream_t, int, int64_t, const int32_t *, const int32_t *,
-                                                   const int32_t *, const int32_t *, const int32_t *,
-                                                   const int64_t *, const double2 *, int64_t, const double2 *,
-                                                   int64_t, double2 *, const int32_t *);
-template void launch_patch_apply<float2, float2>(hipStream_t, int, int64_t, const int32_t *, const int32_t *,
-                                                 const int32_t *, const int32_t *, const int32_t *, const int64_t *,
-                                                 const float2 *, int64_t, const float2 *, int64_t, float2 *,
-                                                 const int32_t *);
+template void launch_patch_apply<double2, double2>(hipStream_t, const PatchLayout &, int, int, int64_t,
+                                                   const double2 *, const float *, const double2 *, int64_t,
+                                                   double2 *, const int32_t *);
+template void launch_patch_apply<__half2, float2>(hipStream_t, const PatchLayout &, int, int, int64_t,
+                                                  const __half2 *, const float *, const float2 *, int64_t, float2 *,
+                                                  const int32_t *);
 
 // ------------------------------------------------------------------ coarse space
 template <typename X>

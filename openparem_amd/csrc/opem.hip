@@ -8,6 +8,7 @@
 // The ABC of the driven port touches only that port's DOFs, which the excitation eliminates
 // (solveComplexLinearSystem.c:115-129), so one A(w) per frequency serves every excitation; each
 // excitation differs only in its Dirichlet row mask (PEC U driven-port DOFs).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -115,7 +116,9 @@ struct Ctx {
   Patches patches;
   bool have_patches = false;
   std::vector<uint8_t> patch_excl;
-  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row;
+  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row;
+  int64_t patch_shared_size = 0, patch_exc_size = 0;
+  std::vector<uint8_t> patch_port_dofs;  // port-DOF mask the patch regions were split with
   // gradient coarse space (second level)
   Coarse coarse;
   bool have_coarse = false;
@@ -615,7 +618,7 @@ static void device_space(Ctx &c) {
 struct Work {
   // sized per batch
   DevBuf A64, A32, masks, x, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
-      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t;
+      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale;
 };
 
 template <typename X>
@@ -647,18 +650,40 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   const int nblk = reduce_blocks(n);
   const bool mixed = std::is_same<X, float2>::value;
   const X *vals = mixed ? (const X *)W.A32.p : (const X *)W.A64.p;
-  // preconditioner: vertex-patch additive Schwarz, or element-block Jacobi
+  // preconditioner: vertex-patch additive Schwarz (inverses shared by the excitations of a frequency
+  // except on port patches; FP16 with per-block scales inside the FP32 solve), or element-block Jacobi
+  using PP = typename std::conditional<std::is_same<X, float2>::value, __half2, double2>::type;
   const bool use_patch = c.have_patches && c.precond == 0;
-  const int64_t inv_stride = use_patch ? c.patches.val_off.back() : (int64_t)nd * n;
-  W.inv.alloc((size_t)S * inv_stride * sizeof(P));
-  if (use_patch)
-    launch_patch_build<P>(st, S, R, c.patches.npatch, c.patches.nmax, c.dp_idx_off.as<int32_t>(),
-                          c.dp_idx.as<int32_t>(), c.dp_val_off.as<int64_t>(), c.dp_nz.as<int32_t>(),
-                          W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<P>(), inv_stride);
-  else
+  PatchLayout PL{};
+  const int G = S / R;
+  if (use_patch) {
+    PL.idx_off = c.dp_idx_off.as<int32_t>();
+    PL.idx = c.dp_idx.as<int32_t>();
+    PL.val_off = c.dp_val_off.as<int64_t>();
+    PL.nz_off = c.dp_nz_off.as<int64_t>();
+    PL.port = c.dp_port.as<uint8_t>();
+    PL.pnz = c.dp_nz.as<int32_t>();
+    PL.d2p_ptr = c.dp_d2p_ptr.as<int32_t>();
+    PL.d2p_patch = c.dp_d2p_patch.as<int32_t>();
+    PL.d2p_row = c.dp_d2p_row.as<int32_t>();
+    PL.shared_size = c.patch_shared_size;
+    PL.exc_size = c.patch_exc_size;
+    PL.group_stride = c.patch_shared_size + (int64_t)R * c.patch_exc_size;
+    PL.npatch = c.patches.npatch;
+    PL.nmax = c.patches.nmax;
+    W.inv.alloc((size_t)G * PL.group_stride * sizeof(PP));
+    float *scale = nullptr;
+    if (mixed) {
+      W.pscale.alloc((size_t)S * PL.npatch * sizeof(float));
+      scale = W.pscale.as<float>();
+    }
+    launch_patch_build<PP>(st, PL, S, R, W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(), scale);
+  } else {
+    W.inv.alloc((size_t)S * nd * n * sizeof(P));
     launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
                        nnz, W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                        c.d_blocks.as<int32_t>(), c.nblocks);
+  }
   const bool use_coarse = c.have_coarse && c.coarse.nc > 0;
   if (use_coarse) {
     W.t.alloc((size_t)S * n * sizeof(X));
@@ -667,9 +692,8 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   }
   auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
     if (use_patch)
-      launch_patch_apply<P, X>(st, S, n, c.dp_d2p_ptr.as<int32_t>(), c.dp_d2p_patch.as<int32_t>(),
-                               c.dp_d2p_row.as<int32_t>(), c.dp_idx_off.as<int32_t>(), c.dp_idx.as<int32_t>(),
-                               c.dp_val_off.as<int64_t>(), W.inv.as<P>(), inv_stride, v, ldv_sys, z, act);
+      launch_patch_apply<PP, X>(st, PL, G, R, n, W.inv.as<PP>(), mixed ? W.pscale.as<float>() : nullptr, v, ldv_sys,
+                                z, act);
     else
       launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                             v, ldv_sys, z, act, (int64_t)nd * n);
@@ -1033,15 +1057,35 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     if (it != c.bdr.end() && it->second.kind == OPEM_BDR_PEC)
       for (int k = 0; k < s.ndf; k++) pec[s.bface_dofs[f * s.ndf + k]] = 1;
   }
-  if (c.precond == 0 && (!c.have_patches || c.patch_excl != pec)) {
+  std::vector<uint8_t> portdof(n, 0);
+  for (auto &P : c.ports)
+    for (int32_t d : P.dofs) portdof[d] = 1;
+  if (c.precond == 0 && (!c.have_patches || c.patch_excl != pec || c.patch_port_dofs != portdof)) {
     int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     build_patches(s, pec, c.patches, nth);
     c.patch_excl = pec;
+    c.patch_port_dofs = portdof;
     c.have_patches = c.patches.nmax <= 100;
     if (c.have_patches) {
+      // split: patches touching a port DOF get one inverse per excitation
+      const int64_t npch = c.patches.npatch;
+      std::vector<uint8_t> pport(npch, 0);
+      std::vector<int64_t> voff(npch);
+      int64_t sh = 0, ex = 0;
+      for (int64_t p = 0; p < npch; p++) {
+        for (int32_t k = c.patches.idx_off[p]; k < c.patches.idx_off[p + 1]; k++)
+          if (portdof[c.patches.idx[k]]) pport[p] = 1;
+        const int64_t sz = c.patches.val_off[p + 1] - c.patches.val_off[p];
+        voff[p] = pport[p] ? ex : sh;
+        (pport[p] ? ex : sh) += sz;
+      }
+      c.patch_shared_size = sh;
+      c.patch_exc_size = ex;
       c.dp_idx_off.upload(c.patches.idx_off, st);
       c.dp_idx.upload(c.patches.idx, st);
-      c.dp_val_off.upload(c.patches.val_off, st);
+      c.dp_val_off.upload(voff, st);
+      c.dp_nz_off.upload(c.patches.val_off, st);
+      c.dp_port.upload(pport, st);
       c.dp_nz.upload(c.patches.nz, st);
       c.dp_d2p_ptr.upload(c.patches.d2p_ptr, st);
       c.dp_d2p_patch.upload(c.patches.d2p_patch, st);
