@@ -90,17 +90,18 @@ struct PatchLayout {
   const uint8_t *port;               // per patch: excitation-dependent
   const int32_t *pnz;                // CSR position of (idx[i], idx[j]) or -1
   const int32_t *d2p_ptr, *d2p_patch, *d2p_row;  // DOF -> (patch, local row)
-  int64_t group_stride, shared_size, exc_size, npatch;
+  const int64_t *yoff;               // offset of patch p's rows in the per-system patch-output buffer
+  int64_t group_stride, shared_size, exc_size, npatch, ytot;
   int nmax;
 };
 // inv: per-group storage (group_stride elements each); FP16 storage takes per-block scales.
 template <typename P>
 void launch_patch_build(hipStream_t st, const PatchLayout &L, int S, int R, const double *a64, int64_t nnz,
                         const uint8_t *rowmask, int64_t n, P *inv, float *scale);
-// z[s][d] = sum_{patches p containing d} (inv_p r_p)[d]   (deterministic gather over DOF rows)
+// z[s][d] = sum_{patches p containing d} (inv_p r_p)[d]; ybuf holds S * ytot complex patch outputs.
 template <typename P, typename X>
 void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R, int64_t n, const P *inv,
-                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active);
+                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active, double *ybuf);
 
 // ---------------------------------------------------------------- coarse space (two-level Schwarz)
 // rc[s][a] = sum_i C_ia (v[s][i] - t[s][i])   (C^T by coarse rows; t may be null)

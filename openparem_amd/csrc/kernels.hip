@@ -797,114 +797,118 @@ template void launch_patch_build<double2>(hipStream_t, const PatchLayout &, int,
 template void launch_patch_build<__half2>(hipStream_t, const PatchLayout &, int, int, const double *, int64_t,
                                           const uint8_t *, int64_t, __half2 *, float *);
 
-constexpr int kPatchLanes = 16;
-
-// One 16-lane group per DOF row d and matrix group g; the R excitations of the group share every
-// shared-patch row read.
+// Apply in two deterministic passes.
+// 1) one workgroup per (patch, matrix group): y_p = inv_p v_p for the R excitations, v_p staged in LDS;
+//    the inverse of a complex-symmetric block is complex symmetric, so thread t reads row j at column t
+//    (coalesced across threads) for sum_j inv[j][t] v[j].
+// 2) one thread per DOF: z[d] = sum over the (patch, row) pairs of d of y.
 template <typename P, typename X, int R>
-__global__ __launch_bounds__(256) void k_patch_apply(PatchLayout L, int64_t n, const P *__restrict__ inv,
+__global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, const P *__restrict__ inv,
                                                      const float *__restrict__ scale, const X *__restrict__ v,
-                                                     int64_t ldv_sys, X *__restrict__ z,
+                                                     int64_t ldv_sys, double2 *__restrict__ y, int64_t ldy_sys,
                                                      const int32_t *__restrict__ active) {
-  const int g = blockIdx.y;
+  const int p = blockIdx.x, g = blockIdx.y;
   bool any = false;
 #pragma unroll
   for (int r = 0; r < R; r++) any |= (active == nullptr) || active[g * R + r];
   if (!any) return;
-  const int sub = threadIdx.x % kPatchLanes;
-  const int64_t d = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPatchLanes;
-  if (d >= n) return;
-  double2 acc[R];
+  __shared__ double2 vs[R][128];
+  const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0;
+  const int t = threadIdx.x;
+  if (t < np) {
+    const int32_t c = L.idx[b0 + t];
 #pragma unroll
-  for (int r = 0; r < R; r++) acc[r] = make_double2(0.0, 0.0);
-  for (int k = L.d2p_ptr[d]; k < L.d2p_ptr[d + 1]; k++) {
-    const int p = L.d2p_patch[k], l = L.d2p_row[k];
-    const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0;
-    const int32_t *I = L.idx + b0;
-    if (!L.port[p]) {
-      const P *row = inv + patch_offset(L, p, g, 0) + (int64_t)l * np;
-      const double sc = scale ? (double)scale[((int64_t)g * R) * L.npatch + p] : 1.0;
-      double2 t[R];
+    for (int r = 0; r < R; r++) vs[r][t] = to_d2(v[((int64_t)g * R + r) * ldv_sys + c]);
+  }
+  __syncthreads();
+  if (t >= np) return;
+  const int64_t yo = L.yoff[p] + t;
+  if (!L.port[p]) {
+    const P *A = inv + patch_offset(L, p, g, 0);
+    const double sc = scale ? (double)scale[((int64_t)g * R) * L.npatch + p] : 1.0;
+    double2 acc[R];
 #pragma unroll
-      for (int r = 0; r < R; r++) t[r] = make_double2(0.0, 0.0);
-      for (int j = sub; j < np; j += kPatchLanes) {
-        const double2 a = ldp(row[j]);
-        const int c = I[j];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const double2 x = to_d2(v[((int64_t)g * R + r) * ldv_sys + c]);
-          t[r].x += a.x * x.x - a.y * x.y;
-          t[r].y += a.x * x.y + a.y * x.x;
-        }
-      }
+    for (int r = 0; r < R; r++) acc[r] = make_double2(0.0, 0.0);
+    for (int j = 0; j < np; j++) {
+      const double2 a = ldp(A[(int64_t)j * np + t]);
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        acc[r].x += sc * t[r].x;
-        acc[r].y += sc * t[r].y;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const P *row = inv + patch_offset(L, p, g, r) + (int64_t)l * np;
-        const double sc = scale ? (double)scale[((int64_t)g * R + r) * L.npatch + p] : 1.0;
-        double2 t = make_double2(0.0, 0.0);
-        for (int j = sub; j < np; j += kPatchLanes) {
-          const double2 a = ldp(row[j]);
-          const double2 x = to_d2(v[((int64_t)g * R + r) * ldv_sys + I[j]]);
-          t.x += a.x * x.x - a.y * x.y;
-          t.y += a.x * x.y + a.y * x.x;
-        }
-        acc[r].x += sc * t.x;
-        acc[r].y += sc * t.y;
+        const double2 x = vs[r][j];
+        acc[r].x += a.x * x.x - a.y * x.y;
+        acc[r].y += a.x * x.y + a.y * x.x;
       }
     }
-  }
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-#pragma unroll
-    for (int o = kPatchLanes / 2; o > 0; o >>= 1) {
-      acc[r].x += __shfl_xor(acc[r].x, o, kPatchLanes);
-      acc[r].y += __shfl_xor(acc[r].y, o, kPatchLanes);
-    }
-  }
-  if (sub == 0) {
+    for (int r = 0; r < R; r++)
+      y[((int64_t)g * R + r) * ldy_sys + yo] = make_double2(sc * acc[r].x, sc * acc[r].y);
+  } else {
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int64_t s = (int64_t)g * R + r;
-      if (active && !active[s]) continue;
-      z[s * n + d] = from_d2<X>(acc[r]);
+      const P *A = inv + patch_offset(L, p, g, r);
+      const double sc = scale ? (double)scale[((int64_t)g * R + r) * L.npatch + p] : 1.0;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int j = 0; j < np; j++) {
+        const double2 a = ldp(A[(int64_t)j * np + t]);
+        const double2 x = vs[r][j];
+        acc.x += a.x * x.x - a.y * x.y;
+        acc.y += a.x * x.y + a.y * x.x;
+      }
+      y[((int64_t)g * R + r) * ldy_sys + yo] = make_double2(sc * acc.x, sc * acc.y);
     }
   }
 }
 
+template <typename X>
+__global__ __launch_bounds__(256) void k_patch_gather(PatchLayout L, int64_t n, int S, const double2 *__restrict__ y,
+                                                      int64_t ldy_sys, X *__restrict__ z,
+                                                      const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  double2 acc = make_double2(0.0, 0.0);
+  const double2 *ys = y + (int64_t)s * ldy_sys;
+  for (int k = L.d2p_ptr[d]; k < L.d2p_ptr[d + 1]; k++) {
+    const double2 t = ys[L.yoff[L.d2p_patch[k]] + L.d2p_row[k]];
+    acc.x += t.x;
+    acc.y += t.y;
+  }
+  z[(int64_t)s * n + d] = from_d2<X>(acc);
+}
+
 template <typename P, typename X>
 void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R, int64_t n, const P *inv,
-                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active) {
-  dim3 grid((unsigned)((n * kPatchLanes + 255) / 256), ngroups);
+                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active, double *ybuf) {
+  if (L.nmax > 128) throw std::runtime_error("patch apply: patch larger than 128 DOFs");
+  dim3 grid((unsigned)L.npatch, ngroups);
+  double2 *y = (double2 *)ybuf;
   switch (R) {
     case 1:
-      hipLaunchKernelGGL((k_patch_apply<P, X, 1>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 1>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
       break;
     case 2:
-      hipLaunchKernelGGL((k_patch_apply<P, X, 2>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
       break;
     case 3:
-      hipLaunchKernelGGL((k_patch_apply<P, X, 3>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 3>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
       break;
     case 4:
-      hipLaunchKernelGGL((k_patch_apply<P, X, 4>), grid, dim3(256), 0, st, L, n, inv, scale, v, ldv_sys, z, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 4>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
       break;
     default:
       throw std::runtime_error("patch apply: R must be 1..4");
   }
   HIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_patch_gather<X>), dim3((unsigned)((n + 255) / 256), ngroups * R), dim3(256), 0, st, L, n,
+                     ngroups * R, y, L.ytot, z, active);
+  HIP_LAUNCH_CHECK();
 }
 template void launch_patch_apply<double2, double2>(hipStream_t, const PatchLayout &, int, int, int64_t,
                                                    const double2 *, const float *, const double2 *, int64_t,
-                                                   double2 *, const int32_t *);
+                                                   double2 *, const int32_t *, double *);
 template void launch_patch_apply<__half2, float2>(hipStream_t, const PatchLayout &, int, int, int64_t,
                                                   const __half2 *, const float *, const float2 *, int64_t, float2 *,
-                                                  const int32_t *);
+                                                  const int32_t *, double *);
 
 // ------------------------------------------------------------------ coarse space
 template <typename X>

@@ -116,7 +116,7 @@ struct Ctx {
   Patches patches;
   bool have_patches = false;
   std::vector<uint8_t> patch_excl;
-  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row;
+  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row, dp_yoff;
   int64_t patch_shared_size = 0, patch_exc_size = 0;
   std::vector<uint8_t> patch_port_dofs;  // port-DOF mask the patch regions were split with
   // gradient coarse space (second level)
@@ -618,7 +618,7 @@ static void device_space(Ctx &c) {
 struct Work {
   // sized per batch
   DevBuf A64, A32, masks, x, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
-      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale;
+      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch;
 };
 
 template <typename X>
@@ -666,6 +666,9 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     PL.d2p_ptr = c.dp_d2p_ptr.as<int32_t>();
     PL.d2p_patch = c.dp_d2p_patch.as<int32_t>();
     PL.d2p_row = c.dp_d2p_row.as<int32_t>();
+    PL.yoff = c.dp_yoff.as<int64_t>();
+    PL.ytot = (int64_t)c.patches.idx.size();
+    W.ypatch.alloc((size_t)S * PL.ytot * 16);
     PL.shared_size = c.patch_shared_size;
     PL.exc_size = c.patch_exc_size;
     PL.group_stride = c.patch_shared_size + (int64_t)R * c.patch_exc_size;
@@ -693,7 +696,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
     if (use_patch)
       launch_patch_apply<PP, X>(st, PL, G, R, n, W.inv.as<PP>(), mixed ? W.pscale.as<float>() : nullptr, v, ldv_sys,
-                                z, act);
+                                z, act, W.ypatch.as<double>());
     else
       launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                             v, ldv_sys, z, act, (int64_t)nd * n);
@@ -1086,6 +1089,9 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       c.dp_val_off.upload(voff, st);
       c.dp_nz_off.upload(c.patches.val_off, st);
       c.dp_port.upload(pport, st);
+      std::vector<int64_t> yoff(npch);
+      for (int64_t p = 0; p < npch; p++) yoff[p] = c.patches.idx_off[p];
+      c.dp_yoff.upload(yoff, st);
       c.dp_nz.upload(c.patches.nz, st);
       c.dp_d2p_ptr.upload(c.patches.d2p_ptr, st);
       c.dp_d2p_patch.upload(c.patches.d2p_patch, st);
