@@ -214,3 +214,17 @@ def test_sweep_nd3():
     pr = Problem(m, 3)
     o = pr.solve_point(10e9, {a: te10(m, a, 10e9) for a in pr.ports})
     assert np.abs(r["S"][0] - o["S"]).max() < 1e-8
+
+
+def test_batched_multigroup_spmv_sweep():
+    """Nine frequencies in one batch (multi-group SpMV path for >= 8 matrix groups) against one-at-a-time FP64."""
+    m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
+    f = np.linspace(8e9, 12e9, 9)
+    s = _solver(m, 2)
+    try:
+        rb = s.sweep(f, precision="mixed", tol=1e-10, batch=9, orth_passes=1)
+        r1 = s.sweep(f, precision="fp64", tol=1e-10, batch=1)
+    finally:
+        s.close()
+    assert rb["converged"].all() and r1["converged"].all()
+    assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
