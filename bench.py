@@ -77,6 +77,36 @@ def _cpu_baseline(work, its_per_system, budget_s):
                         f"{its_per_system:.0f} iterations (the GPU's mean)"))
 
 
+def _assembly_c2(order, n_elem, nrep=5):
+    """Assembly kernel alone (config 2): element matrices of n_elem random tets on MFMA in FP64, FP32 and
+    BF16-scaled; error against the FP64 oracle on the first 2000 elements."""
+    from openparem_amd.mesh import random_tets
+    from openparem_amd.solver import Device
+    from oracle import nd as ond
+
+    v, er, td = random_tets(n_elem, seed=7)
+    k0 = 2 * np.pi * 5e9 / 299792458.0 * 1e-2
+    nd = order * (order + 2) * (order + 3) // 2
+    K, M = ond.element_matrices(v[:2000], order)
+    ref = np.stack([K - (k0 * k0 * er[:2000])[:, None, None] * M, (k0 * k0 * er[:2000] * td[:2000])[:, None, None] * M], 1)
+    rn = np.linalg.norm(ref.reshape(2000, -1), axis=1)
+    out = {}
+    dev = Device()
+    try:
+        for prec, name, osz in ((0, "fp64", 8), (1, "fp32", 4), (2, "bf16", 4)):
+            _, t = dev.element_matrices(order, prec, v, er, td, k0, nrep=nrep, want=False)
+            got, _ = dev.element_matrices(order, prec, v[:2000], er[:2000], td[:2000], k0, nrep=1)
+            err = np.linalg.norm((got - ref).reshape(2000, -1), axis=1) / rn
+            b = n_elem * (12 * 8 + 2 * 8 + 2 * nd * nd * osz)
+            f = n_elem * (2 * 16 + 2 * 8) * nd * nd
+            out[name] = {"ms": t * 1e3, "GB/s": b / t / 1e9, "TFLOP/s": f / t / 1e12, "frac_hbm": b / t / 1e9 / HBM_PEAK_GBS,
+                         "rel_err_max": float(err.max()), "rel_err_p99": float(np.quantile(err, 0.99))}
+    finally:
+        dev.close()
+    return {"elements": n_elem, "order": order, "nd": nd, "bytes_per_element": "12*8 + 2*8 + 2*nd^2*s_out",
+            **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +115,7 @@ def main():
     ap.add_argument("--workload", default="waveguide-sweep")
     ap.add_argument("--fp64-points", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--restart", type=int, default=150)
     ap.add_argument("--coarse", type=int, default=0)
     args = ap.parse_args()
@@ -130,6 +161,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(work, its_mean, 30.0)
+    asm = None if args.no_assembly else _assembly_c2(2, 1_000_000)
 
     line = {
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",
@@ -162,6 +194,7 @@ def main():
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes_per_launch": b_spmv, "launch_seconds": t_spmv},
         "cpu_baseline": cpu,
+        "assembly_c2": asm,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -127,6 +127,9 @@ void launch_axpy(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32
 template <typename X>
 void launch_norm2(hipStream_t st, int S, int64_t n, const X *x, double *part, int nblk, const int32_t *active);
 int reduce_blocks(int64_t n);
+// OPEM_XCD_REMAP=1 remaps block ids so each XCD works on one contiguous range (measured slower on the
+// sweep: off by default).
+int xcd_remap_enabled();
 
 // ---------------------------------------------------------------- Givens / Hessenberg (device)
 // Apply previous rotations to column j of H[s] ((m+1) x m, column-major, complex), new rotation,

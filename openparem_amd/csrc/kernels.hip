@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 
 namespace opem {
@@ -47,6 +48,14 @@ __device__ __forceinline__ double wave_sum_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  Remapping the linear block
+// id so that each XCD receives one contiguous range of logical blocks keeps the x-vector slice a range of
+// rows touches in that XCD's own L2.
+__device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, rem = nb % 8, k = b % 8, i = b / 8;
+  return k * q + (k < rem ? k : rem) + i;
 }
 
 constexpr int kRedThreads = 256;
@@ -195,14 +204,22 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t *__restri
                                               const int32_t *__restrict__ colind, const V *__restrict__ vals,
                                               int64_t nnz, const uint8_t *__restrict__ rowmask,
                                               const X *__restrict__ x, X *__restrict__ y, const X *__restrict__ b,
-                                              const int32_t *__restrict__ active, int mode, int64_t vstride) {
-  const int g = blockIdx.y;
+                                              const int32_t *__restrict__ active, int mode, int64_t vstride,
+                                              int xcd_remap) {
+  int64_t bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_remap) {
+    const int64_t nbx = gridDim.x;
+    const int64_t lg = xcd_logical(by * nbx + bx, nbx * gridDim.y);
+    bx = lg % nbx;
+    by = lg / nbx;
+  }
+  const int g = (int)by;
   bool any = false;
 #pragma unroll
   for (int r = 0; r < R; r++) any |= (active == nullptr) || active[g * R + r];
   if (!any) return;
   const int sub = threadIdx.x % LPR;
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  const int64_t row = (bx * blockDim.x + threadIdx.x) / LPR;
   if (row >= n) return;
   const V *__restrict__ A = vals + (int64_t)g * vstride;
   const int beg = rowptr[row], end = rowptr[row + 1];
@@ -247,6 +264,15 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t *__restri
   }
 }
 
+int xcd_remap_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("OPEM_XCD_REMAP");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
+}
+
 template <typename V, typename X, int LPR>
 static void spmv_r(hipStream_t st, int ngroups, int R, int64_t n, const int32_t *rowptr, const int32_t *colind,
                    const V *vals, int64_t nnz, const uint8_t *rowmask, const X *x, X *y, const X *b,
@@ -255,19 +281,19 @@ static void spmv_r(hipStream_t st, int ngroups, int R, int64_t n, const int32_t 
   switch (R) {
     case 1:
       hipLaunchKernelGGL((k_spmv<V, X, LPR, 1>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
-                         y, b, active, mode, vstride);
+                         y, b, active, mode, vstride, xcd_remap_enabled());
       break;
     case 2:
       hipLaunchKernelGGL((k_spmv<V, X, LPR, 2>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
-                         y, b, active, mode, vstride);
+                         y, b, active, mode, vstride, xcd_remap_enabled());
       break;
     case 3:
       hipLaunchKernelGGL((k_spmv<V, X, LPR, 3>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
-                         y, b, active, mode, vstride);
+                         y, b, active, mode, vstride, xcd_remap_enabled());
       break;
     case 4:
       hipLaunchKernelGGL((k_spmv<V, X, LPR, 4>), grid, dim3(256), 0, st, n, rowptr, colind, vals, nnz, rowmask, x,
-                         y, b, active, mode, vstride);
+                         y, b, active, mode, vstride, xcd_remap_enabled());
       break;
     default:
       throw std::runtime_error("spmv: R must be 1..4");
@@ -806,8 +832,15 @@ template <typename P, typename X, int R>
 __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, const P *__restrict__ inv,
                                                      const float *__restrict__ scale, const X *__restrict__ v,
                                                      int64_t ldv_sys, double2 *__restrict__ y, int64_t ldy_sys,
-                                                     const int32_t *__restrict__ active) {
-  const int p = blockIdx.x, g = blockIdx.y;
+                                                     const int32_t *__restrict__ active, int xcd_remap) {
+  int64_t bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_remap) {
+    const int64_t nbx = gridDim.x;
+    const int64_t lg = xcd_logical(by * nbx + bx, nbx * gridDim.y);
+    bx = lg % nbx;
+    by = lg / nbx;
+  }
+  const int p = (int)bx, g = (int)by;
   bool any = false;
 #pragma unroll
   for (int r = 0; r < R; r++) any |= (active == nullptr) || active[g * R + r];
@@ -884,16 +917,20 @@ void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R
   double2 *y = (double2 *)ybuf;
   switch (R) {
     case 1:
-      hipLaunchKernelGGL((k_patch_local<P, X, 1>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 1>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
+                         xcd_remap_enabled());
       break;
     case 2:
-      hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
+                         xcd_remap_enabled());
       break;
     case 3:
-      hipLaunchKernelGGL((k_patch_local<P, X, 3>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 3>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
+                         xcd_remap_enabled());
       break;
     case 4:
-      hipLaunchKernelGGL((k_patch_local<P, X, 4>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active);
+      hipLaunchKernelGGL((k_patch_local<P, X, 4>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
+                         xcd_remap_enabled());
       break;
     default:
       throw std::runtime_error("patch apply: R must be 1..4");

@@ -221,6 +221,10 @@ static inline int blocks_for(int64_t n, int cap = 4096) {
 
 // ------------------------------------------------------------------ host helpers
 static int choose_lpr(const Space &s) {
+  if (const char *e = getenv("OPEM_SPMV_LPR")) {
+    const int v = atoi(e);
+    if (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) return v;
+  }
   const double avg = (double)s.nnz / (double)std::max<int64_t>(1, s.ndof);
   if (avg <= 12) return 8;
   if (avg <= 48) return 16;
