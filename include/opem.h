@@ -72,6 +72,14 @@ int opem_port_points(opem_ctx *ctx, int attr, int64_t *npts, double *xyz);
  * (fem3D.cpp:103-190) + hypre_ParCSRMatrixToMat (csr.c:171-223), before any elimination. */
 int opem_system_matrix(opem_ctx *ctx, double freq, const double *gamma, double *vals);
 
+/* The batched operator of a sweep on caller vectors (parity check of the product SpMV):
+ * y_f = A(freqs[f]) x_f for f < nfreq <= 256, A(w) as opem_system_matrix (gamma: complex [nfreq][nports],
+ * ports in ascending attribute order), applied as Kt - k0^2 (Mr - j Mi) + boundary terms over one pass
+ * of the frequency-independent matrices (the MatMult of KSPSolve, solveComplexLinearSystem.c:100-190,
+ * for every frequency at once).  prec 0 FP64, 1 FP32 values and vectors.  x, y complex [nfreq][N]. */
+int opem_sweep_apply(opem_ctx *ctx, int prec, int nfreq, const double *freqs, const double *gamma,
+                     const double *x, double *y);
+
 /* Complex CSR SpMV on caller-provided host arrays (parity check of the device kernel):
  * y = A x with A n x n (rowptr n+1, colind nnz, vals nnz complex); prec OPEM_PREC_FP64 computes in
  * FP64, OPEM_PREC_MIXED stores values and vectors in FP32 and accumulates in FP64. */
@@ -88,12 +96,22 @@ int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const i
 int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const double *verts, const double *er,
                           const double *tand, double k0, int nrep, double *out, double *avg_seconds);
 
-/* Times the batched complex CSR SpMV kernel on the context's pattern (device-resident synthetic values
- * and vectors): ngroups matrices, R right-hand sides each (y = mask .* A x), prec as opem_spmv.
- * *avg_seconds = mean kernel duration over nrep launches (HIP events on the context stream);
- * *alg_bytes = algorithmic HBM bytes of one launch: 4 nnz + 4 (N+1) + ngroups s_v nnz + ngroups R 2 N s_v
- * (s_v = 16 FP64, 8 FP32). */
+/* Times the batched operator of a sweep (the SpMV inside GMRES, PETSc MatMult in KSPSolve,
+ * solveComplexLinearSystem.c:100-190) on the context's pattern with device-resident synthetic inputs:
+ * S = ngroups*R systems A_s = Kt + c_s M + sum_t d_{s,t} T_t over one pass of the frequency-independent
+ * matrices, x system-interleaved; prec 0 FP64, 1 FP32.  Needs a space and, for the boundary terms, the
+ * boundaries set.  *avg_seconds = mean kernel duration over nrep launches (HIP events on the context
+ * stream); *alg_bytes = algorithmic HBM bytes of one launch:
+ *   4 (N+1) + 4 nnz + 2 s_r nnz + 4 (N+1) + 4 nov + s_r nterm nov + R N + 2 S N s_v
+ * (s_r = 8 FP64 / 4 FP32 real, s_v = 16 / 8 complex; nov, nterm = boundary-term entries and terms). */
 int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, double *avg_seconds, double *alg_bytes);
+
+/* Times the Arnoldi orthogonalisation kernels of the batched GMRES (classical Gram-Schmidt inside
+ * KSPGMRES, the VecMDot / VecMAXPY pair of solveComplexLinearSystem.c's KSPSolve) for S systems and a
+ * basis of k vectors: avg_seconds[0] = projection h = V^H w, avg_seconds[1] = update w -= V h (+ norm);
+ * alg_bytes[0] = S (k+1) N s_v + S nblk k 16, alg_bytes[1] = S (k+2) N s_v + S k 16 + S nblk 8
+ * (nblk = ceil(N / 1024) partial blocks). */
+int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *avg_seconds, double *alg_bytes);
 
 typedef struct {
   int nfreq;
@@ -116,6 +134,8 @@ typedef struct {
                                 /*    more than 100 DOFs), 1: element-block Jacobi                */
   int coarse_size;              /* aggregates of the gradient coarse level (0 = one-level, <= 128) */
   int orth_passes;              /* classical Gram-Schmidt passes per Arnoldi step: 2 (default) or 1 */
+  int precond_share;            /* vertex patches: adjacent frequencies of a batch sharing one set of */
+                                /*    patch inverses, built from their mean matrix (0 -> 1, <= 8 / nports) */
 } opem_sweep_in;
 
 typedef struct {

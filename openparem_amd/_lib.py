@@ -13,6 +13,7 @@ SYMBOLS = [
     "opem_create", "opem_destroy", "opem_last_error", "opem_mesh_set", "opem_space_build", "opem_space_get",
     "opem_base_matrix", "opem_material_set", "opem_boundary_set", "opem_port_points", "opem_system_matrix",
     "opem_spmv", "opem_element_matrices", "opem_sweep", "opem_calculate_s", "opem_spmv_bench",
+    "opem_krylov_bench", "opem_sweep_apply",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -31,7 +32,8 @@ class SweepIn(C.Structure):
                 ("mode_E", f64p), ("mode_H", f64p), ("gamma", f64p), ("V", f64p), ("Z", f64p),
                 ("precision", C.c_int), ("batch", C.c_int), ("restart", C.c_int), ("maxit", C.c_int),
                 ("tol", C.c_double), ("reciprocal", C.c_int), ("h_tol", C.c_double), ("precond", C.c_int),
-                ("coarse_size", C.c_int), ("orth_passes", C.c_int)]
+                ("coarse_size", C.c_int), ("orth_passes", C.c_int),
+                ("precond_share", C.c_int)]
 
 
 class SweepOut(C.Structure):
@@ -65,6 +67,8 @@ def lib():
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]
         L.opem_spmv_bench.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+        L.opem_krylov_bench.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+        L.opem_sweep_apply.argtypes = [P, C.c_int, C.c_int, f64p, f64p, f64p, f64p]
         L.opem_calculate_s.argtypes = [C.c_int, f64p, f64p, f64p, f64p, C.c_int, f64p]
         _lib = L
     return _lib

@@ -50,6 +50,36 @@ void launch_spmv(hipStream_t st, int ngroups, int R, int64_t n, const int32_t *r
                  const V *vals, int64_t nnz, const uint8_t *rowmask, const X *x, X *y, const X *b,
                  const int32_t *active, int mode, int lanes_per_row, int64_t vstride);
 
+// ---------------------------------------------------------------- batched sweep SpMV
+// The S systems of a batch on one pattern: A_s = Kt + c_s (Mr - j Mi) + sum_t d_{s,t} T_t, with Kt, Mr, Mi
+// frequency independent (Mi null when every material is lossless) and the boundary terms T_t stored on
+// their rows only (ov_ptr n+1, ov_col, ov_val[t][nov]).
+template <typename V>
+struct SweepMatrix {
+  int64_t n, nnz;
+  const int32_t *rowptr, *colind;
+  const V *Kt, *Mr, *Mi;
+  const int32_t *ov_ptr, *ov_col;
+  const V *ov_val;
+  int64_t nov;
+  int nterm;
+};
+// y[s*n + row] = rowmask[s%R][row] * (mode ? -(A_s x_s) : A_s x_s)[row] for active s; x system-interleaved
+// (x[j*S + s]); csys[s] = c_s, dsys[s][t] complex.  S <= 256.
+template <typename V, typename X>
+void launch_spmv_sweep(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
+                       const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr);
+// out64 = [Kt | Mr | Mi] (nnz each; Mi only if lossy), Kt = sum_a w[a][0] K_a, Mr = sum_a w[a][1] M_a,
+// Mi = sum_a w[a][2] M_a; out32 (optional) the same in FP32.
+void launch_base_matrices(hipStream_t st, int64_t nnz, int nslot, const double *K, const double *M, const double *w,
+                          double *out64, float *out32, int lossy);
+// system-major [S][n] <-> system-interleaved [n][S]
+template <typename X>
+void launch_transpose(hipStream_t st, int64_t n, int S, const X *in, X *out, bool to_il);
+// x += z, both system-interleaved, for active systems
+template <typename X, typename Z>
+void launch_axpy_il(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32_t *active);
+
 // ---------------------------------------------------------------- Krylov helpers (batched, S systems)
 // partial[s][blk][i] = sum_{chunk} conj(Vb[s][i][n]) w[s][n], i < k;  vectors in basis stride ldv.
 template <typename X>
@@ -74,7 +104,7 @@ template <typename P>
 void launch_bj_build(hipStream_t st, int S, int R, int nd, const int32_t *elem_dofs, const int32_t *elem_nz,
                      const double *a64, int64_t nnz, const uint8_t *rowmask, int64_t n, const int32_t *owner_elem,
                      P *inv_rows, const int32_t *elem_of_block, int64_t nblocks);
-// z[s][r] = sum_b inv_rows[s][r][b] v[s][elem_dofs[owner_elem[r]][b]]
+// z[r*S + s] = sum_b inv_rows[s][r][b] v[s][elem_dofs[owner_elem[r]][b]]  (z system-interleaved)
 template <typename P, typename X>
 void launch_bj_apply(hipStream_t st, int S, int64_t n, int nd, const int32_t *elem_dofs,
                      const int32_t *owner_elem, const P *inv_rows, const X *v, int64_t ldv_sys, X *z,
@@ -82,10 +112,12 @@ void launch_bj_apply(hipStream_t st, int S, int64_t n, int nd, const int32_t *el
 
 // ---------------------------------------------------------------- vertex-patch additive Schwarz
 // Device view of the patch structure.  Patches without port DOFs ("shared") have one inverse per
-// matrix group; patches touching a port DOF have one per excitation (their Dirichlet rows differ).
+// preconditioner group; patches touching a port DOF have one per excitation (their Dirichlet rows
+// differ).  A preconditioner group is pg consecutive frequency groups of the batch; its inverses are
+// those of the mean of the groups' matrices.
 struct PatchLayout {
   const int32_t *idx_off, *idx;      // patch DOF lists
-  const int64_t *val_off;            // offset of the n_p x n_p block in its region
+  const int64_t *val_off;            // offset of the block in its region (FP64 n_p^2, FP16 n_p*npk)
   const int64_t *nz_off;             // offset of the block's CSR-position map in pnz
   const uint8_t *port;               // per patch: excitation-dependent
   const int32_t *pnz;                // CSR position of (idx[i], idx[j]) or -1
@@ -93,15 +125,18 @@ struct PatchLayout {
   const int64_t *yoff;               // offset of patch p's rows in the per-system patch-output buffer
   int64_t group_stride, shared_size, exc_size, npatch, ytot;
   int nmax;
+  int pg;                            // frequency groups per preconditioner group
 };
-// inv: per-group storage (group_stride elements each); FP16 storage takes per-block scales.
+// inv: per-preconditioner-group storage (group_stride elements each); FP16 storage takes per-block
+// scales.  G = number of frequency groups in the batch (systems s = g*R + r).
 template <typename P>
-void launch_patch_build(hipStream_t st, const PatchLayout &L, int S, int R, const double *a64, int64_t nnz,
+void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, const double *a64, int64_t nnz,
                         const uint8_t *rowmask, int64_t n, P *inv, float *scale);
-// z[s][d] = sum_{patches p containing d} (inv_p r_p)[d]; ybuf holds S * ytot complex patch outputs.
+// z[d*S + s] = sum_{patches p containing d} (inv_p v_p)[d] (z system-interleaved, v system-major with stride
+// ldv_sys); ybuf holds ytot * S patch outputs of type X (system-interleaved).
 template <typename P, typename X>
 void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R, int64_t n, const P *inv,
-                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active, double *ybuf);
+                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active, void *ybuf);
 
 // ---------------------------------------------------------------- coarse space (two-level Schwarz)
 // rc[s][a] = sum_i C_ia (v[s][i] - t[s][i])   (C^T by coarse rows; t may be null)
@@ -112,7 +147,7 @@ void launch_coarse_restrict(hipStream_t st, int S, int nc, const int32_t *col_pt
 // yc[s] = Ainv[s / R] rc[s]   (dense nc x nc complex per matrix group)
 void launch_coarse_solve(hipStream_t st, int S, int R, int nc, const double *Ainv, const double *rc, double *yc,
                          const int32_t *active);
-// z[s][i] += sum_a C_ia yc[s][a]
+// z[i*S + s] += sum_a C_ia yc[s][a]  (z system-interleaved)
 template <typename X>
 void launch_coarse_prolong(hipStream_t st, int S, int64_t n, int nc, const int32_t *row_ptr, const int32_t *row_col,
                            const double *row_val, const double *yc, X *z, const int32_t *active);
@@ -130,6 +165,8 @@ int reduce_blocks(int64_t n);
 // OPEM_XCD_REMAP=1 remaps block ids so each XCD works on one contiguous range (measured slower on the
 // sweep: off by default).
 int xcd_remap_enabled();
+// OPEM_PATCH_MFMA=0 applies FP16 patch inverses with vector FMAs instead of MFMA (default 1).
+int patch_mfma_enabled();
 
 // ---------------------------------------------------------------- Givens / Hessenberg (device)
 // Apply previous rotations to column j of H[s] ((m+1) x m, column-major, complex), new rotation,

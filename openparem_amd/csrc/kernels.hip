@@ -5,9 +5,11 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
+#include <type_traits>
 
 namespace opem {
 
@@ -264,6 +266,15 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t *__restri
   }
 }
 
+int patch_mfma_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("OPEM_PATCH_MFMA");
+    v = e ? atoi(e) : 1;
+  }
+  return v;
+}
+
 int xcd_remap_enabled() {
   static int v = -1;
   if (v < 0) {
@@ -332,6 +343,326 @@ template void launch_spmv<double2, double2>(hipStream_t, int, int, int64_t, cons
 template void launch_spmv<float2, float2>(hipStream_t, int, int, int64_t, const int32_t *, const int32_t *,
                                           const float2 *, int64_t, const uint8_t *, const float2 *, float2 *,
                                           const float2 *, const int32_t *, int, int, int64_t);
+
+// ------------------------------------------------------------------ batched sweep SpMV
+// y[s][row] = rowmask[s % R][row] * (+/-) sum_j (A_s)_{row j} x[j][s]
+//   A_s = Kt + c_s (Mr - j Mi) + sum_t d_{s,t} T_t       (T_t on the boundary overlay rows only)
+// x is system-interleaved (x[j*S + s]): one pass over the frequency-independent matrices serves all S
+// systems and each gathered column is S contiguous values.  y is system-major (y[s*n + row]).
+// Wide form (S > 8): one wavefront per row, lane l owns systems l + 64m (m < MS); a block covers 16
+// consecutive rows (4 per wavefront) so each written cache line of y comes from one workgroup.
+template <typename V, typename X, int MS, bool LOSSY>
+__global__ __launch_bounds__(256) void k_spmv_sweep_w(SweepMatrix<V> M, int S, int R, const V *__restrict__ csys,
+                                                      const V *__restrict__ dsys, const uint8_t *__restrict__ rowmask,
+                                                      const X *__restrict__ x, X *__restrict__ y, int mode,
+                                                      const int32_t *__restrict__ active) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  V cs[MS];
+  bool on[MS];
+#pragma unroll
+  for (int m = 0; m < MS; m++) {
+    const int s = lane + 64 * m;
+    on[m] = s < S && (active == nullptr || active[s] != 0);
+    cs[m] = s < S ? csys[s] : (V)0;
+  }
+  for (int rr = 0; rr < 4; rr++) {
+    const int64_t row = (int64_t)blockIdx.x * 16 + wave * 4 + rr;
+    if (row >= M.n) break;
+    const int beg = M.rowptr[row], end = M.rowptr[row + 1];
+    V ax[MS], ay[MS];
+#pragma unroll
+    for (int m = 0; m < MS; m++) ax[m] = ay[m] = (V)0;
+    for (int k = beg; k < end; k++) {
+      const int64_t j = M.colind[k];
+      const V kv = M.Kt[k], mr = M.Mr[k];
+      V mi = (V)0;
+      if constexpr (LOSSY) mi = M.Mi[k];
+#pragma unroll
+      for (int m = 0; m < MS; m++) {
+        if (on[m]) {
+          const X xv = x[j * S + lane + 64 * m];
+          const V ar = kv + cs[m] * mr;
+          if constexpr (LOSSY) {
+            const V ai = -cs[m] * mi;
+            ax[m] += ar * xv.x - ai * xv.y;
+            ay[m] += ar * xv.y + ai * xv.x;
+          } else {
+            ax[m] += ar * xv.x;
+            ay[m] += ar * xv.y;
+          }
+        }
+      }
+    }
+    if (M.nterm > 0) {
+      for (int e = M.ov_ptr[row]; e < M.ov_ptr[row + 1]; e++) {
+        const int64_t j = M.ov_col[e];
+#pragma unroll
+        for (int m = 0; m < MS; m++) {
+          if (on[m]) {
+            const int s = lane + 64 * m;
+            V cr = (V)0, ci = (V)0;
+            for (int t = 0; t < M.nterm; t++) {
+              const V tv = M.ov_val[(int64_t)t * M.nov + e];
+              cr += dsys[((int64_t)s * M.nterm + t) * 2] * tv;
+              ci += dsys[((int64_t)s * M.nterm + t) * 2 + 1] * tv;
+            }
+            const X xv = x[j * S + s];
+            ax[m] += cr * xv.x - ci * xv.y;
+            ay[m] += cr * xv.y + ci * xv.x;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      if (on[m]) {
+        const int s = lane + 64 * m;
+        const bool free_row = rowmask[(int64_t)(s % R) * M.n + row] != 0;
+        X o;
+        o.x = free_row ? (mode ? -ax[m] : ax[m]) : (V)0;
+        o.y = free_row ? (mode ? -ay[m] : ay[m]) : (V)0;
+        y[(int64_t)s * M.n + row] = o;
+      }
+    }
+  }
+}
+
+// Narrow form (S <= 8): LPR lanes split a row's entries, each lane carries all S systems.
+template <typename V, typename X, int LPR, int SP, bool LOSSY>
+__global__ __launch_bounds__(256) void k_spmv_sweep_n(SweepMatrix<V> M, int S, int R, const V *__restrict__ csys,
+                                                      const V *__restrict__ dsys, const uint8_t *__restrict__ rowmask,
+                                                      const X *__restrict__ x, X *__restrict__ y, int mode,
+                                                      const int32_t *__restrict__ active) {
+  const int sub = threadIdx.x % LPR;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= M.n) return;
+  V cs[SP], ax[SP], ay[SP];
+#pragma unroll
+  for (int s = 0; s < SP; s++) {
+    cs[s] = s < S ? csys[s] : (V)0;
+    ax[s] = ay[s] = (V)0;
+  }
+  for (int k = M.rowptr[row] + sub; k < M.rowptr[row + 1]; k += LPR) {
+    const int64_t j = M.colind[k];
+    const V kv = M.Kt[k], mr = M.Mr[k];
+    V mi = (V)0;
+    if constexpr (LOSSY) mi = M.Mi[k];
+#pragma unroll
+    for (int s = 0; s < SP; s++) {
+      if (s < S) {
+        const X xv = x[j * S + s];
+        const V ar = kv + cs[s] * mr;
+        if constexpr (LOSSY) {
+          const V ai = -cs[s] * mi;
+          ax[s] += ar * xv.x - ai * xv.y;
+          ay[s] += ar * xv.y + ai * xv.x;
+        } else {
+          ax[s] += ar * xv.x;
+          ay[s] += ar * xv.y;
+        }
+      }
+    }
+  }
+  if (M.nterm > 0) {
+    for (int e = M.ov_ptr[row] + sub; e < M.ov_ptr[row + 1]; e += LPR) {
+      const int64_t j = M.ov_col[e];
+#pragma unroll
+      for (int s = 0; s < SP; s++) {
+        if (s < S) {
+          V cr = (V)0, ci = (V)0;
+          for (int t = 0; t < M.nterm; t++) {
+            const V tv = M.ov_val[(int64_t)t * M.nov + e];
+            cr += dsys[((int64_t)s * M.nterm + t) * 2] * tv;
+            ci += dsys[((int64_t)s * M.nterm + t) * 2 + 1] * tv;
+          }
+          const X xv = x[j * S + s];
+          ax[s] += cr * xv.x - ci * xv.y;
+          ay[s] += cr * xv.y + ci * xv.x;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < SP; s++) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+      ax[s] += __shfl_xor(ax[s], o, LPR);
+      ay[s] += __shfl_xor(ay[s], o, LPR);
+    }
+  }
+  if (sub == 0) {
+#pragma unroll
+    for (int s = 0; s < SP; s++) {
+      if (s < S && (active == nullptr || active[s] != 0)) {
+        const bool free_row = rowmask[(int64_t)(s % R) * M.n + row] != 0;
+        X o;
+        o.x = free_row ? (mode ? -ax[s] : ax[s]) : (V)0;
+        o.y = free_row ? (mode ? -ay[s] : ay[s]) : (V)0;
+        y[(int64_t)s * M.n + row] = o;
+      }
+    }
+  }
+}
+
+template <typename V, typename X, bool LOSSY>
+static void spmv_sweep_l(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
+                         const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr) {
+  if (S > 8) {
+    dim3 grid((unsigned)((M.n + 15) / 16));
+    if (S <= 64)
+      hipLaunchKernelGGL((k_spmv_sweep_w<V, X, 1, LOSSY>), grid, dim3(256), 0, st, M, S, R, csys, dsys, rowmask, x,
+                         y, mode, active);
+    else if (S <= 128)
+      hipLaunchKernelGGL((k_spmv_sweep_w<V, X, 2, LOSSY>), grid, dim3(256), 0, st, M, S, R, csys, dsys, rowmask, x,
+                         y, mode, active);
+    else if (S <= 256)
+      hipLaunchKernelGGL((k_spmv_sweep_w<V, X, 4, LOSSY>), grid, dim3(256), 0, st, M, S, R, csys, dsys, rowmask, x,
+                         y, mode, active);
+    else
+      throw std::runtime_error("sweep spmv: at most 256 systems per batch");
+  } else {
+    const int L = lpr <= 8 ? 8 : lpr <= 16 ? 16 : 32;
+    dim3 grid((unsigned)((M.n * L + 255) / 256));
+#define OPEM_SPMV_N(LL, SS)                                                                                   \
+  hipLaunchKernelGGL((k_spmv_sweep_n<V, X, LL, SS, LOSSY>), grid, dim3(256), 0, st, M, S, R, csys, dsys, rowmask, \
+                     x, y, mode, active)
+#define OPEM_SPMV_NS(LL)        \
+  if (S <= 1)                   \
+    OPEM_SPMV_N(LL, 1);         \
+  else if (S <= 2)              \
+    OPEM_SPMV_N(LL, 2);         \
+  else if (S <= 4)              \
+    OPEM_SPMV_N(LL, 4);         \
+  else                          \
+    OPEM_SPMV_N(LL, 8);
+    if (L == 8) {
+      OPEM_SPMV_NS(8)
+    } else if (L == 16) {
+      OPEM_SPMV_NS(16)
+    } else {
+      OPEM_SPMV_NS(32)
+    }
+#undef OPEM_SPMV_NS
+#undef OPEM_SPMV_N
+  }
+  HIP_LAUNCH_CHECK();
+}
+
+template <typename V, typename X>
+void launch_spmv_sweep(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
+                       const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr) {
+  if (M.Mi)
+    spmv_sweep_l<V, X, true>(st, M, S, R, csys, dsys, rowmask, x, y, mode, active, lpr);
+  else
+    spmv_sweep_l<V, X, false>(st, M, S, R, csys, dsys, rowmask, x, y, mode, active, lpr);
+}
+template void launch_spmv_sweep<double, double2>(hipStream_t, const SweepMatrix<double> &, int, int, const double *,
+                                                 const double *, const uint8_t *, const double2 *, double2 *, int,
+                                                 const int32_t *, int);
+template void launch_spmv_sweep<float, float2>(hipStream_t, const SweepMatrix<float> &, int, int, const float *,
+                                               const float *, const uint8_t *, const float2 *, float2 *, int,
+                                               const int32_t *, int);
+
+// Kt = sum_a wk[a] K[a], Mr = sum_a wr[a] M[a], Mi = sum_a wi[a] M[a] (FP64, plus FP32 copies if f32 != null)
+__global__ void k_base_matrices(int64_t nnz, int nslot, const double *__restrict__ K, const double *__restrict__ M,
+                                const double *__restrict__ w, double *__restrict__ out64, float *__restrict__ out32,
+                                int lossy) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+    double kt = 0.0, mr = 0.0, mi = 0.0;
+    for (int a = 0; a < nslot; a++) {
+      const double kv = K[(int64_t)a * nnz + i], mv = M[(int64_t)a * nnz + i];
+      kt += w[a * 3] * kv;
+      mr += w[a * 3 + 1] * mv;
+      mi += w[a * 3 + 2] * mv;
+    }
+    out64[i] = kt;
+    out64[nnz + i] = mr;
+    if (lossy) out64[2 * nnz + i] = mi;
+    if (out32) {
+      out32[i] = (float)kt;
+      out32[nnz + i] = (float)mr;
+      if (lossy) out32[2 * nnz + i] = (float)mi;
+    }
+  }
+}
+
+void launch_base_matrices(hipStream_t st, int64_t nnz, int nslot, const double *K, const double *M, const double *w,
+                          double *out64, float *out32, int lossy) {
+  int64_t bl = (nnz + 255) / 256;
+  if (bl > 65536) bl = 65536;
+  hipLaunchKernelGGL(k_base_matrices, dim3((unsigned)bl), dim3(256), 0, st, nnz, nslot, K, M, w, out64, out32, lossy);
+  HIP_LAUNCH_CHECK();
+}
+
+// out[i*S + s] = in[s*n + i] (to_il) or out[s*n + i] = in[i*S + s], through 32 x 32 LDS tiles
+template <typename X, bool TO_IL>
+__global__ __launch_bounds__(256) void k_transpose(int64_t n, int S, const X *__restrict__ in, X *__restrict__ out) {
+  __shared__ X tile[32][33];
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int s0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  if (TO_IL) {
+    for (int k = ty; k < 32; k += 8) {  // read rows s, columns i (coalesced over i)
+      const int s = s0 + k;
+      const int64_t i = i0 + tx;
+      if (s < S && i < n) tile[k][tx] = in[(int64_t)s * n + i];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {  // write rows i, columns s (coalesced over s)
+      const int64_t i = i0 + k;
+      const int s = s0 + tx;
+      if (s < S && i < n) out[i * S + s] = tile[tx][k];
+    }
+  } else {
+    for (int k = ty; k < 32; k += 8) {
+      const int64_t i = i0 + k;
+      const int s = s0 + tx;
+      if (s < S && i < n) tile[tx][k] = in[i * S + s];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+      const int s = s0 + k;
+      const int64_t i = i0 + tx;
+      if (s < S && i < n) out[(int64_t)s * n + i] = tile[k][tx];
+    }
+  }
+}
+
+template <typename X>
+void launch_transpose(hipStream_t st, int64_t n, int S, const X *in, X *out, bool to_il) {
+  dim3 grid((unsigned)((n + 31) / 32), (unsigned)((S + 31) / 32));
+  if (to_il)
+    hipLaunchKernelGGL((k_transpose<X, true>), grid, dim3(256), 0, st, n, S, in, out);
+  else
+    hipLaunchKernelGGL((k_transpose<X, false>), grid, dim3(256), 0, st, n, S, in, out);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_transpose<double2>(hipStream_t, int64_t, int, const double2 *, double2 *, bool);
+template void launch_transpose<float2>(hipStream_t, int64_t, int, const float2 *, float2 *, bool);
+
+// x[i*S + s] += z[i*S + s] for active s (both system-interleaved)
+template <typename X, typename Z>
+__global__ void k_axpy_il(int64_t n, int S, const Z *__restrict__ z, X *__restrict__ x,
+                          const int32_t *__restrict__ active) {
+  const int64_t tot = n * S;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e % S);
+    if (active && !active[s]) continue;
+    const double2 a = to_d2(x[e]), b = to_d2(z[e]);
+    x[e] = from_d2<X>(make_double2(a.x + b.x, a.y + b.y));
+  }
+}
+template <typename X, typename Z>
+void launch_axpy_il(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32_t *active) {
+  int64_t bl = (n * S + 255) / 256;
+  if (bl > 65536) bl = 65536;
+  hipLaunchKernelGGL((k_axpy_il<X, Z>), dim3((unsigned)bl), dim3(256), 0, st, n, S, z, x, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_axpy_il<double2, double2>(hipStream_t, int, int64_t, const double2 *, double2 *,
+                                               const int32_t *);
+template void launch_axpy_il<double2, float2>(hipStream_t, int, int64_t, const float2 *, double2 *, const int32_t *);
 
 // ------------------------------------------------------------------ Krylov helpers
 constexpr int kMaxK = kMaxBasis;  // basis vectors per reduction (restart + 1)
@@ -636,7 +967,7 @@ __global__ __launch_bounds__(256) void k_bj_apply(int64_t n, int nd, const int32
                                                   const int32_t *__restrict__ owner_elem, const P *__restrict__ inv,
                                                   const X *__restrict__ v, int64_t ldv_sys, X *__restrict__ z,
                                                   const int32_t *__restrict__ active, int64_t inv_stride) {
-  const int s = blockIdx.y;
+  const int s = blockIdx.y, S = gridDim.y;
   if (active && !active[s]) return;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -650,7 +981,7 @@ __global__ __launch_bounds__(256) void k_bj_apply(int64_t n, int nd, const int32
     acc.x += p.x * x.x - p.y * x.y;
     acc.y += p.x * x.y + p.y * x.x;
   }
-  z[s * n + r] = from_d2<X>(acc);
+  z[r * S + s] = from_d2<X>(acc);
 }
 
 template <typename P, typename X>
@@ -668,34 +999,30 @@ template void launch_bj_apply<float2, float2>(hipStream_t, int, int64_t, int, co
                                               int64_t);
 
 // ------------------------------------------------------------------ vertex-patch additive Schwarz
-// Storage per matrix group g: [shared region: patches with no port DOF, one inverse for all excitations]
-// [R excitation regions: patches touching a port DOF].  Element offset of patch p, group g, excitation r:
-//   g*group_stride + (port[p] ? shared_size + r*exc_size : 0) + val_off[p].
-// FP16 storage keeps one scale per (group, excitation, patch) block: value = half * scale.
-__device__ __forceinline__ double2 ldp(double2 v) { return v; }
-__device__ __forceinline__ double2 ldp(__half2 v) {
-  return make_double2((double)__half2float(v.x), (double)__half2float(v.y));
+// Storage per preconditioner group q (pg consecutive frequency groups): [shared region: patches with no
+// port DOF, one inverse for all excitations] [R excitation regions: patches touching a port DOF].
+// Element offset of patch p, group q, excitation r:
+//   q*group_stride + (port[p] ? shared_size + r*exc_size : 0) + val_off[p].
+// FP64 storage: the full n_p x n_p inverse, row-major.  FP16 storage: the symmetrised inverse as two
+// row-major planes of halves, real then imaginary, each n_p x npk with npk = n_p rounded up to a multiple
+// of 8 (zero columns n_p..npk-1, so every row starts 16-byte aligned); block size n_p*npk in __half2
+// units.  FP16 blocks carry one scale per (group, excitation, patch): value = half * scale.
+__device__ __forceinline__ int64_t patch_offset(const PatchLayout &L, int p, int q, int r) {
+  return (int64_t)q * L.group_stride + (L.port[p] ? L.shared_size + (int64_t)r * L.exc_size : 0) + L.val_off[p];
 }
-__device__ __forceinline__ void stp(double2 *o, double2 v, double) { *o = v; }
-__device__ __forceinline__ void stp(__half2 *o, double2 v, double inv_scale) {
-  __half2 h;
-  h.x = __float2half((float)(v.x * inv_scale));
-  h.y = __float2half((float)(v.y * inv_scale));
-  *o = h;
-}
-
-__device__ __forceinline__ int64_t patch_offset(const PatchLayout &L, int p, int g, int r) {
-  return (int64_t)g * L.group_stride + (L.port[p] ? L.shared_size + (int64_t)r * L.exc_size : 0) + L.val_off[p];
-}
+__device__ __forceinline__ int pad8(int n) { return (n + 7) & ~7; }
 
 template <typename P>
-__global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int R, const double2 *__restrict__ a64,
+__global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R, const double2 *__restrict__ a64,
                                                      int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
                                                      P *__restrict__ inv, float *__restrict__ scale) {
+  constexpr bool kPacked = std::is_same<P, __half2>::value;
   extern __shared__ double2 lds[];
-  const int p = blockIdx.x, s = blockIdx.y;
-  const int g = s / R, r = s % R;
+  const int p = blockIdx.x;
+  const int q = blockIdx.y / R, r = blockIdx.y % R;
   if (!L.port[p] && r != 0) return;  // shared inverse: built once per group
+  const int g0 = q * L.pg, ng = min(L.pg, G - g0);
+  const double wg = 1.0 / ng;
   const int np = L.idx_off[p + 1] - L.idx_off[p];
   double2 *A = lds;                          // np*np
   double2 *colk = lds + np * np;             // np
@@ -704,17 +1031,25 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int R, const
   __shared__ double red[4];
   const int32_t *I = L.idx + L.idx_off[p];
   const int32_t *nz = L.pnz + L.nz_off[p];
-  const double2 *Ag = a64 + (int64_t)g * nnz;
   const uint8_t *mask = rowmask + (int64_t)r * n;
   const int tid = threadIdx.x;
   const int nn = np * np;
+  // block of the mean matrix of the group's frequencies (the frequencies of a group are adjacent)
   for (int t = tid; t < nn; t += blockDim.x) {
     const int a = t / np, b = t - a * np;
     const bool fa = mask[I[a]] != 0, fb = mask[I[b]] != 0;
     double2 v = make_double2(a == b ? 1.0 : 0.0, 0.0);
     if (fa && fb) {
-      const int32_t q = nz[t];
-      v = q >= 0 ? Ag[q] : make_double2(0.0, 0.0);
+      const int32_t qz = nz[t];
+      v = make_double2(0.0, 0.0);
+      if (qz >= 0)
+        for (int g = g0; g < g0 + ng; g++) {
+          const double2 e = a64[(int64_t)g * nnz + qz];
+          v.x += e.x;
+          v.y += e.y;
+        }
+      v.x *= wg;
+      v.y *= wg;
     }
     A[t] = v;
   }
@@ -783,26 +1118,41 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int R, const
       }
     __syncthreads();
   }
-  double sc = 1.0;
-  if (scale) {  // block scale: largest |Re|,|Im| maps to 1 in the half-precision store
+  P *out = inv + patch_offset(L, p, q, r);
+  if constexpr (kPacked) {
+    // block scale: the largest |Re|,|Im| maps to 1 in the half-precision store
     double m = 0.0;
     for (int t = tid; t < nn; t += blockDim.x) m = fmax(m, fmax(fabs(A[t].x), fabs(A[t].y)));
     m = wave_sum_max(m);
     if ((tid & 63) == 0) red[tid >> 6] = m;
     __syncthreads();
     m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    sc = m > 0.0 ? m : 1.0;
-    if (tid == 0) scale[((int64_t)g * R + (L.port[p] ? r : 0)) * L.npatch + p] = (float)sc;
+    const double sc = m > 0.0 ? m : 1.0;
+    if (tid == 0) scale[((int64_t)q * R + (L.port[p] ? r : 0)) * L.npatch + p] = (float)sc;
+    const double h = 0.5 / sc;
+    const int npk = pad8(np);
+    __half *Pr = (__half *)out, *Pi = Pr + (int64_t)np * npk;
+    for (int t = tid; t < np * npk; t += blockDim.x) {
+      const int i = t / npk, j = t - i * npk;
+      double re = 0.0, im = 0.0;
+      if (j < np) {
+        const double2 u = A[i * np + j], w = A[j * np + i];
+        re = (u.x + w.x) * h;
+        im = (u.y + w.y) * h;
+      }
+      Pr[t] = __float2half((float)re);
+      Pi[t] = __float2half((float)im);
+    }
+  } else {
+    for (int t = tid; t < nn; t += blockDim.x) out[t] = A[t];
   }
-  P *out = inv + patch_offset(L, p, g, r);
-  const double isc = 1.0 / sc;
-  for (int t = tid; t < nn; t += blockDim.x) stp(out + t, A[t], isc);
 }
 
 template <typename P>
-void launch_patch_build(hipStream_t st, const PatchLayout &L, int S, int R, const double *a64, int64_t nnz,
+void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, const double *a64, int64_t nnz,
                         const uint8_t *rowmask, int64_t n, P *inv, float *scale) {
   if (L.nmax > 100) throw std::runtime_error("vertex patch larger than 100 DOFs");
+  if (L.pg < 1) throw std::runtime_error("patch build: frequencies per preconditioner must be >= 1");
   const size_t lds = (size_t)(L.nmax * L.nmax + L.nmax) * sizeof(double2);
   static size_t attr_bytes = 0;
   if (lds > attr_bytes) {
@@ -814,7 +1164,8 @@ void launch_patch_build(hipStream_t st, const PatchLayout &L, int S, int R, cons
     }
     attr_bytes = lds;
   }
-  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)L.npatch, S), dim3(256), lds, st, L, R,
+  const int nq = (G + L.pg - 1) / L.pg;
+  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)L.npatch, nq * R), dim3(256), lds, st, L, G, R,
                      (const double2 *)a64, nnz, rowmask, n, inv, scale);
   HIP_LAUNCH_CHECK();
 }
@@ -824,128 +1175,246 @@ template void launch_patch_build<__half2>(hipStream_t, const PatchLayout &, int,
                                           const uint8_t *, int64_t, __half2 *, float *);
 
 // Apply in two deterministic passes.
-// 1) one workgroup per (patch, matrix group): y_p = inv_p v_p for the R excitations, v_p staged in LDS;
-//    the inverse of a complex-symmetric block is complex symmetric, so thread t reads row j at column t
-//    (coalesced across threads) for sum_j inv[j][t] v[j].
+// 1) one workgroup per (patch, preconditioner group): y_p = inv_p v_p for the group's ns = pg*R systems,
+//    v_p staged in LDS.  The inverse of a complex-symmetric block is complex symmetric, so thread t forms
+//    row t as column t, reading A[j][t] (coalesced across t).  A patch touching a port DOF has one
+//    inverse per excitation r, applied to the systems with s % R == r.  FP16 storage computes in FP32.
 // 2) one thread per DOF: z[d] = sum over the (patch, row) pairs of d of y.
-template <typename P, typename X, int R>
-__global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, const P *__restrict__ inv,
+template <typename P, typename X, int NSM>
+__global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R, const P *__restrict__ inv,
                                                      const float *__restrict__ scale, const X *__restrict__ v,
-                                                     int64_t ldv_sys, double2 *__restrict__ y, int64_t ldy_sys,
-                                                     const int32_t *__restrict__ active, int xcd_remap) {
-  int64_t bx = blockIdx.x, by = blockIdx.y;
-  if (xcd_remap) {
-    const int64_t nbx = gridDim.x;
-    const int64_t lg = xcd_logical(by * nbx + bx, nbx * gridDim.y);
-    bx = lg % nbx;
-    by = lg / nbx;
-  }
-  const int p = (int)bx, g = (int)by;
+                                                     int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
+                                                     const int32_t *__restrict__ active) {
+  using T = typename std::conditional<std::is_same<P, __half2>::value, float, double>::type;
+  const int p = blockIdx.x, q = blockIdx.y;
+  const int g0 = q * L.pg;
+  const int ns = min(L.pg, G - g0) * R;
+  const int64_t s0 = (int64_t)g0 * R;
+  bool act[NSM];
   bool any = false;
 #pragma unroll
-  for (int r = 0; r < R; r++) any |= (active == nullptr) || active[g * R + r];
+  for (int k = 0; k < NSM; k++) {
+    act[k] = k < ns && (active == nullptr || active[s0 + k]);
+    any |= act[k];
+  }
   if (!any) return;
-  __shared__ double2 vs[R][128];
+  __shared__ X vs[NSM][128];
   const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0;
   const int t = threadIdx.x;
   if (t < np) {
     const int32_t c = L.idx[b0 + t];
 #pragma unroll
-    for (int r = 0; r < R; r++) vs[r][t] = to_d2(v[((int64_t)g * R + r) * ldv_sys + c]);
+    for (int k = 0; k < NSM; k++)
+      if (k < ns) vs[k][t] = v[(s0 + k) * ldv_sys + c];
   }
   __syncthreads();
   if (t >= np) return;
   const int64_t yo = L.yoff[p] + t;
-  if (!L.port[p]) {
-    const P *A = inv + patch_offset(L, p, g, 0);
-    const double sc = scale ? (double)scale[((int64_t)g * R) * L.npatch + p] : 1.0;
-    double2 acc[R];
+  const int nreg = L.port[p] ? R : 1;
+  const int npk = pad8(np);
+  for (int ri = 0; ri < nreg; ri++) {
+    const P *A = inv + patch_offset(L, p, q, ri);
+    const T sc = scale ? (T)scale[((int64_t)q * R + ri) * L.npatch + p] : (T)1;
+    T ax[NSM], ay[NSM];
 #pragma unroll
-    for (int r = 0; r < R; r++) acc[r] = make_double2(0.0, 0.0);
+    for (int k = 0; k < NSM; k++) ax[k] = ay[k] = (T)0;
     for (int j = 0; j < np; j++) {
-      const double2 a = ldp(A[(int64_t)j * np + t]);
+      T a_x, a_y;
+      if constexpr (std::is_same<P, __half2>::value) {
+        const __half *Pr = (const __half *)A;
+        a_x = __half2float(Pr[(int64_t)j * npk + t]);
+        a_y = __half2float(Pr[(int64_t)np * npk + (int64_t)j * npk + t]);
+      } else {
+        const double2 a = A[(int64_t)j * np + t];
+        a_x = a.x;
+        a_y = a.y;
+      }
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const double2 x = vs[r][j];
-        acc[r].x += a.x * x.x - a.y * x.y;
-        acc[r].y += a.x * x.y + a.y * x.x;
+      for (int k = 0; k < NSM; k++) {
+        if (k < ns) {
+          const X xv = vs[k][j];
+          ax[k] += a_x * (T)xv.x - a_y * (T)xv.y;
+          ay[k] += a_x * (T)xv.y + a_y * (T)xv.x;
+        }
       }
     }
 #pragma unroll
-    for (int r = 0; r < R; r++)
-      y[((int64_t)g * R + r) * ldy_sys + yo] = make_double2(sc * acc[r].x, sc * acc[r].y);
-  } else {
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const P *A = inv + patch_offset(L, p, g, r);
-      const double sc = scale ? (double)scale[((int64_t)g * R + r) * L.npatch + p] : 1.0;
-      double2 acc = make_double2(0.0, 0.0);
-      for (int j = 0; j < np; j++) {
-        const double2 a = ldp(A[(int64_t)j * np + t]);
-        const double2 x = vs[r][j];
-        acc.x += a.x * x.x - a.y * x.y;
-        acc.y += a.x * x.y + a.y * x.x;
+    for (int k = 0; k < NSM; k++) {
+      const bool mine = nreg == 1 || (k % R) == ri;
+      if (act[k] && mine) {
+        X o;
+        o.x = sc * ax[k];
+        o.y = sc * ay[k];
+        y[yo * ldy_sys + s0 + k] = o;
       }
-      y[((int64_t)g * R + r) * ldy_sys + yo] = make_double2(sc * acc.x, sc * acc.y);
+    }
+  }
+}
+
+// FP16 storage on MFMA: one wavefront per (patch, preconditioner group).  The complex product is two real
+// products, [Yr | Yi] = Ar [Vr | Vi] + Ai [-Vi | Vr], an (n_p x n_p) x (n_p x 2ns) GEMM on
+// v_mfma_f32_16x16x32_f16 (FP32 accumulation): column j < ns is Re of system j, ns <= j < 2ns is Im of
+// system j - ns.  Each system's patch vector is scaled to max |Re|,|Im| = 1 and split into two halves,
+// hi = f16(v), lo = f16(v - hi), so the operand keeps about 22 bits; the inverse is exact in its FP16
+// store.  Operand layouts (16x16x32, 8 values per lane): A[i][k] in lane i%16 + 16*(k%32/8), B[k][j] in
+// lane j + 16*(k%32/8), D[i][j] in register i%4 of lane j + 16*(i/4).
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+
+template <typename X>
+__global__ __launch_bounds__(64) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
+                                                         const float *__restrict__ scale, const X *__restrict__ v,
+                                                         int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
+                                                         const int32_t *__restrict__ active) {
+  const int p = blockIdx.x, q = blockIdx.y;
+  const int g0 = q * L.pg;
+  const int ns = min(L.pg, G - g0) * R;  // <= 8
+  const int64_t s0 = (int64_t)g0 * R;
+  bool any = false;
+  for (int k = 0; k < ns; k++) any |= active == nullptr || active[s0 + k] != 0;
+  if (!any) return;
+  __shared__ float2 vs[8][128];
+  __shared__ unsigned int mx[8];
+  const int lane = threadIdx.x;
+  const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0, npk = pad8(np);
+  if (lane < 8) mx[lane] = 0u;
+  __syncthreads();
+  for (int e = lane; e < ns * 128; e += 64) {
+    const int sy = e >> 7, t = e & 127;
+    float2 val = make_float2(0.f, 0.f);
+    if (t < np) {
+      const X xv = v[(s0 + sy) * ldv_sys + L.idx[b0 + t]];
+      val = make_float2((float)xv.x, (float)xv.y);
+    }
+    vs[sy][t] = val;
+    atomicMax(&mx[sy], __float_as_uint(fmaxf(fabsf(val.x), fabsf(val.y))));
+  }
+  __syncthreads();
+  const int j = lane & 15, kq = lane >> 4;
+  const int sy = j < 2 * ns ? (j < ns ? j : j - ns) : -1;
+  const int comp = j < ns ? 0 : 1;
+  const float ms = sy >= 0 ? __uint_as_float(mx[sy]) : 0.f;
+  const float ims = ms > 0.f ? 1.f / ms : 0.f;
+  const int nkb = (npk + 31) >> 5;  // <= 4 for n_p <= 100
+  h8_t b1h[4], b1l[4], b2h[4], b2l[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k = kb * 32 + kq * 8 + e;
+      float2 x = make_float2(0.f, 0.f);
+      if (sy >= 0) x = vs[sy][k];
+      x.x *= ims;
+      x.y *= ims;
+      const float c1 = comp == 0 ? x.x : x.y;
+      const float c2 = comp == 0 ? -x.y : x.x;
+      const _Float16 h1 = (_Float16)c1, h2 = (_Float16)c2;
+      b1h[kb][e] = h1;
+      b1l[kb][e] = (_Float16)(c1 - (float)h1);
+      b2h[kb][e] = h2;
+      b2l[kb][e] = (_Float16)(c2 - (float)h2);
+    }
+  }
+  const int nreg = L.port[p] ? R : 1;
+  for (int ri = 0; ri < nreg; ri++) {
+    const _Float16 *Ar = (const _Float16 *)(inv + patch_offset(L, p, q, ri));
+    const _Float16 *Ai = Ar + (int64_t)np * npk;
+    const float sc = scale[((int64_t)q * R + ri) * L.npatch + p];
+    const int64_t s = s0 + sy;
+    const bool mine = sy >= 0 && (nreg == 1 || sy % R == ri) && (active == nullptr || active[s] != 0);
+    for (int mt = 0; mt < (np + 15) >> 4; mt++) {
+      const int i = mt * 16 + j;
+      f4v_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) {
+        if (kb < nkb) {
+          const int k0 = kb * 32 + kq * 8;
+          h8_t ar = {0, 0, 0, 0, 0, 0, 0, 0}, ai = {0, 0, 0, 0, 0, 0, 0, 0};
+          if (i < np && k0 < npk) {
+            ar = *(const h8_t *)(Ar + (int64_t)i * npk + k0);
+            ai = *(const h8_t *)(Ai + (int64_t)i * npk + k0);
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, b1h[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, b1l[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, b2h[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, b2l[kb], acc, 0, 0, 0);
+        }
+      }
+      if (mine) {
+        const float f = sc * ms;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = mt * 16 + kq * 4 + r;
+          if (row < np) {
+            float *o = (float *)(y + (L.yoff[p] + row) * ldy_sys + s);
+            o[comp] = acc[r] * f;
+          }
+        }
+      }
     }
   }
 }
 
 template <typename X>
-__global__ __launch_bounds__(256) void k_patch_gather(PatchLayout L, int64_t n, int S, const double2 *__restrict__ y,
-                                                      int64_t ldy_sys, X *__restrict__ z,
-                                                      const int32_t *__restrict__ active) {
-  const int s = blockIdx.y;
-  if (active && !active[s]) return;
-  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= n) return;
-  double2 acc = make_double2(0.0, 0.0);
-  const double2 *ys = y + (int64_t)s * ldy_sys;
-  for (int k = L.d2p_ptr[d]; k < L.d2p_ptr[d + 1]; k++) {
-    const double2 t = ys[L.yoff[L.d2p_patch[k]] + L.d2p_row[k]];
-    acc.x += t.x;
-    acc.y += t.y;
+__global__ __launch_bounds__(256) void k_patch_gather(PatchLayout L, int64_t n, int S, const X *__restrict__ y,
+                                                      X *__restrict__ z, const int32_t *__restrict__ active) {
+  const int64_t tot = n * S;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e % S);
+    if (active && !active[s]) continue;
+    const int64_t d = e / S;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = L.d2p_ptr[d]; k < L.d2p_ptr[d + 1]; k++) {
+      const X t = y[(L.yoff[L.d2p_patch[k]] + L.d2p_row[k]) * S + s];
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    z[e] = from_d2<X>(acc);
   }
-  z[(int64_t)s * n + d] = from_d2<X>(acc);
 }
 
 template <typename P, typename X>
 void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R, int64_t n, const P *inv,
-                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active, double *ybuf) {
-  if (L.nmax > 128) throw std::runtime_error("patch apply: patch larger than 128 DOFs");
-  dim3 grid((unsigned)L.npatch, ngroups);
-  double2 *y = (double2 *)ybuf;
-  switch (R) {
-    case 1:
-      hipLaunchKernelGGL((k_patch_local<P, X, 1>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
-                         xcd_remap_enabled());
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
-                         xcd_remap_enabled());
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_patch_local<P, X, 3>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
-                         xcd_remap_enabled());
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_patch_local<P, X, 4>), grid, dim3(128), 0, st, L, inv, scale, v, ldv_sys, y, L.ytot, active,
-                         xcd_remap_enabled());
-      break;
-    default:
-      throw std::runtime_error("patch apply: R must be 1..4");
+                        const float *scale, const X *v, int64_t ldv_sys, X *z, const int32_t *active, void *ybuf) {
+  if (L.nmax > 100) throw std::runtime_error("patch apply: patch larger than 100 DOFs");
+  const int nsm = L.pg * R;
+  const int nq = (ngroups + L.pg - 1) / L.pg;
+  dim3 grid((unsigned)L.npatch, nq);
+  X *y = (X *)ybuf;
+  const int S = ngroups * R;
+  const unsigned gblocks = (unsigned)std::min<int64_t>(65536, (n * S + 255) / 256);
+  if (nsm > 8) throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
+  if constexpr (std::is_same<P, __half2>::value) {
+    if (patch_mfma_enabled()) {
+      hipLaunchKernelGGL((k_patch_local_mfma<X>), grid, dim3(64), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                         (int64_t)S, active);
+      HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_patch_gather<X>), dim3(gblocks), dim3(256), 0, st, L, n, S, y, z, active);
+      HIP_LAUNCH_CHECK();
+      return;
+    }
   }
+  if (nsm <= 2)
+    hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                       (int64_t)S, active);
+  else if (nsm <= 4)
+    hipLaunchKernelGGL((k_patch_local<P, X, 4>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                       (int64_t)S, active);
+  else if (nsm <= 8)
+    hipLaunchKernelGGL((k_patch_local<P, X, 8>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                       (int64_t)S, active);
+  else
+    throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
   HIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_patch_gather<X>), dim3((unsigned)((n + 255) / 256), ngroups * R), dim3(256), 0, st, L, n,
-                     ngroups * R, y, L.ytot, z, active);
+  hipLaunchKernelGGL((k_patch_gather<X>), dim3(gblocks), dim3(256), 0, st, L, n, S, y, z, active);
   HIP_LAUNCH_CHECK();
 }
 template void launch_patch_apply<double2, double2>(hipStream_t, const PatchLayout &, int, int, int64_t,
                                                    const double2 *, const float *, const double2 *, int64_t,
-                                                   double2 *, const int32_t *, double *);
+                                                   double2 *, const int32_t *, void *);
 template void launch_patch_apply<__half2, float2>(hipStream_t, const PatchLayout &, int, int, int64_t,
                                                   const __half2 *, const float *, const float2 *, int64_t, float2 *,
-                                                  const int32_t *, double *);
+                                                  const int32_t *, void *);
 
 // ------------------------------------------------------------------ coarse space
 template <typename X>
@@ -1025,19 +1494,19 @@ template <typename X>
 __global__ void k_coarse_prolong(int64_t n, int nc, const int32_t *__restrict__ row_ptr,
                                  const int32_t *__restrict__ row_col, const double *__restrict__ row_val,
                                  const double2 *__restrict__ yc, X *__restrict__ z, const int32_t *__restrict__ active) {
-  const int s = blockIdx.y;
+  const int s = blockIdx.y, S = gridDim.y;
   if (active && !active[s]) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int b = row_ptr[i], e = row_ptr[i + 1];
     if (b == e) continue;
-    double2 acc = to_d2(z[s * n + i]);
+    double2 acc = to_d2(z[i * S + s]);
     for (int k = b; k < e; k++) {
       const double c = row_val[k];
       const double2 y = yc[(int64_t)s * nc + row_col[k]];
       acc.x += c * y.x;
       acc.y += c * y.y;
     }
-    z[s * n + i] = from_d2<X>(acc);
+    z[i * S + s] = from_d2<X>(acc);
   }
 }
 

@@ -116,8 +116,10 @@ struct Ctx {
   Patches patches;
   bool have_patches = false;
   std::vector<uint8_t> patch_excl;
-  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_patch, dp_d2p_row, dp_yoff;
-  int64_t patch_shared_size = 0, patch_exc_size = 0;
+  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_val_off_h, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_patch,
+      dp_d2p_row, dp_yoff;
+  int64_t patch_shared_size = 0, patch_exc_size = 0;      // FP64 storage: n_p^2 per block
+  int64_t patch_shared_size_h = 0, patch_exc_size_h = 0;  // FP16 storage: n_p * roundup8(n_p) per block
   std::vector<uint8_t> patch_port_dofs;  // port-DOF mask the patch regions were split with
   // gradient coarse space (second level)
   Coarse coarse;
@@ -128,6 +130,7 @@ struct Ctx {
   DevBuf dc_row_ptr, dc_row_col, dc_row_val, dc_col_ptr, dc_col_row, dc_col_val;
   int coarse_size = 96;                        // 0 disables the coarse level
   int precond = 0;  // 0 vertex patches (falls back to element blocks when a patch exceeds 100 DOFs), 1 element blocks
+  int precond_share = 1;  // frequency groups per set of patch inverses
   int lpr = 16;
 };
 
@@ -532,6 +535,94 @@ static void frequency_coefs(const Ctx &c, const Overlay &ov, double freq, const 
   }
 }
 
+// Frequency-independent parts of the batched operator A_s = Kt + c_s (Mr - j Mi) + sum_t d_{s,t} T_t:
+// Kt = sum_a K_a / mur_a, Mr - j Mi = sum_a er_a (1 - j tand_a) M_a (so c_s = -k0^2, matching
+// frequency_coefs), in FP64 and FP32, and the boundary terms as a CSR over their rows.  With
+// unit_materials every slot counts with mur = er = 1, tand = 0.
+struct SweepOperator {
+  DevBuf base64, base32, ovptr, ovcol, ovval64, ovval32;
+  bool lossy = false;
+  int64_t nov = 0;
+  int nterm = 0;
+  SweepMatrix<double> m64(const Ctx &c) const {
+    const int64_t nnz = c.sp.nnz;
+    return SweepMatrix<double>{c.sp.ndof, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
+                               base64.as<double>(), base64.as<double>() + nnz,
+                               lossy ? base64.as<double>() + 2 * nnz : nullptr, ovptr.as<int32_t>(),
+                               ovcol.as<int32_t>(), ovval64.as<double>(), nov, nterm};
+  }
+  SweepMatrix<float> m32(const Ctx &c) const {
+    const int64_t nnz = c.sp.nnz;
+    return SweepMatrix<float>{c.sp.ndof, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
+                              base32.as<float>(), base32.as<float>() + nnz,
+                              lossy ? base32.as<float>() + 2 * nnz : nullptr, ovptr.as<int32_t>(),
+                              ovcol.as<int32_t>(), ovval32.as<float>(), nov, nterm};
+  }
+};
+
+static void build_sweep_operator(Ctx &c, const Overlay &ov, bool unit_materials, SweepOperator &op) {
+  const Space &s = c.sp;
+  hipStream_t st = c.st;
+  const int64_t n = s.ndof, nnz = s.nnz;
+  const int nslot = (int)c.slot_attr.size();
+  std::vector<double> wb((size_t)nslot * 3, 0.0);
+  op.lossy = false;
+  for (int sl = 0; sl < nslot; sl++) {
+    if (unit_materials) {
+      wb[sl * 3] = wb[sl * 3 + 1] = 1.0;
+      continue;
+    }
+    const Material &m = c.mat.at(c.slot_attr[sl]);
+    wb[sl * 3] = 1.0 / m.mur;
+    wb[sl * 3 + 1] = m.er;
+    wb[sl * 3 + 2] = m.er * m.tand;
+    op.lossy |= m.tand != 0.0;
+  }
+  DevBuf dwb;
+  dwb.upload(wb, st);
+  op.base64.alloc((size_t)3 * nnz * 8);
+  op.base32.alloc((size_t)3 * nnz * 4);
+  launch_base_matrices(st, nnz, nslot, c.d_K.as<double>(), c.d_M.as<double>(), dwb.as<double>(),
+                       op.base64.as<double>(), op.base32.as<float>(), op.lossy ? 1 : 0);
+  op.nov = (int64_t)ov.nz.size();
+  op.nterm = (int)ov.terms.size();
+  std::vector<int32_t> optr(n + 1, 0), ocol(std::max<int64_t>(op.nov, 1), 0);
+  for (int64_t i = 0; i < op.nov; i++) {
+    const int64_t row = std::upper_bound(s.rowptr.begin(), s.rowptr.end(), ov.nz[i]) - s.rowptr.begin() - 1;
+    optr[row + 1]++;
+    ocol[i] = s.colind[ov.nz[i]];
+  }
+  for (int64_t i = 0; i < n; i++) optr[i + 1] += optr[i];
+  std::vector<double> ovv(ov.val);
+  if (ovv.empty()) ovv.push_back(0.0);
+  op.ovptr.upload(optr, st);
+  op.ovcol.upload(ocol, st);
+  op.ovval64.upload(ovv, st);
+  op.ovval32.upload(std::vector<float>(ovv.begin(), ovv.end()), st);
+}
+
+// per-system coefficients: system s = f*R + r takes c_s = -k0(f)^2 and d_{s,t} = cT[f][t]
+static void sweep_coefs(const std::vector<double> &freqs, int R, const std::vector<double> &cT, int nterm,
+                        hipStream_t st, DevBuf &cs64, DevBuf &ds64, DevBuf &cs32, DevBuf &ds32) {
+  const int F = (int)freqs.size(), S = F * R;
+  std::vector<double> c(S), d((size_t)S * std::max(1, nterm) * 2, 0.0);
+  for (int fi = 0; fi < F; fi++) {
+    const double k0 = 2 * M_PI * freqs[fi] * std::sqrt(kMu0 * kEps0);
+    for (int r = 0; r < R; r++) {
+      const int sidx = fi * R + r;
+      c[sidx] = -k0 * k0;
+      for (int t = 0; t < nterm; t++) {
+        d[((size_t)sidx * nterm + t) * 2] = cT[((size_t)fi * nterm + t) * 2];
+        d[((size_t)sidx * nterm + t) * 2 + 1] = cT[((size_t)fi * nterm + t) * 2 + 1];
+      }
+    }
+  }
+  cs64.upload(c, st);
+  ds64.upload(d, st);
+  cs32.upload(std::vector<float>(c.begin(), c.end()), st);
+  ds32.upload(std::vector<float>(d.begin(), d.end()), st);
+}
+
 // ------------------------------------------------------------------ space build on device
 static void device_space(Ctx &c) {
   Space &s = c.sp;
@@ -621,9 +712,27 @@ static void device_space(Ctx &c) {
 // ------------------------------------------------------------------ batched solve
 struct Work {
   // sized per batch
-  DevBuf A64, A32, masks, x, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
+  DevBuf A64, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
       hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch;
+  // batched operator: frequency-independent matrices (per sweep) and per-system coefficients (per batch)
+  SweepMatrix<double> M64{};
+  SweepMatrix<float> M32{};
+  DevBuf csys64, dsys64, csys32, dsys32;
 };
+
+// y = mask .* (A_s x) (mode 0) or mask .* (-A_s x) (mode 1) for the batch, x system-interleaved
+template <typename X>
+static void sweep_spmv(Ctx &c, Work &W, int S, int R, const X *x, X *y, int mode, const int32_t *act);
+template <>
+void sweep_spmv<double2>(Ctx &c, Work &W, int S, int R, const double2 *x, double2 *y, int mode, const int32_t *act) {
+  launch_spmv_sweep<double, double2>(c.st, W.M64, S, R, W.csys64.as<double>(), W.dsys64.as<double>(),
+                                     W.masks.as<uint8_t>(), x, y, mode, act, c.lpr);
+}
+template <>
+void sweep_spmv<float2>(Ctx &c, Work &W, int S, int R, const float2 *x, float2 *y, int mode, const int32_t *act) {
+  launch_spmv_sweep<float, float2>(c.st, W.M32, S, R, W.csys32.as<float>(), W.dsys32.as<float>(),
+                                   W.masks.as<uint8_t>(), x, y, mode, act, c.lpr);
+}
 
 template <typename X>
 struct VecT;
@@ -653,7 +762,6 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   const int nd = s.nd;
   const int nblk = reduce_blocks(n);
   const bool mixed = std::is_same<X, float2>::value;
-  const X *vals = mixed ? (const X *)W.A32.p : (const X *)W.A64.p;
   // preconditioner: vertex-patch additive Schwarz (inverses shared by the excitations of a frequency
   // except on port patches; FP16 with per-block scales inside the FP32 solve), or element-block Jacobi
   using PP = typename std::conditional<std::is_same<X, float2>::value, __half2, double2>::type;
@@ -663,7 +771,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   if (use_patch) {
     PL.idx_off = c.dp_idx_off.as<int32_t>();
     PL.idx = c.dp_idx.as<int32_t>();
-    PL.val_off = c.dp_val_off.as<int64_t>();
+    PL.val_off = mixed ? c.dp_val_off_h.as<int64_t>() : c.dp_val_off.as<int64_t>();
     PL.nz_off = c.dp_nz_off.as<int64_t>();
     PL.port = c.dp_port.as<uint8_t>();
     PL.pnz = c.dp_nz.as<int32_t>();
@@ -673,18 +781,21 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     PL.yoff = c.dp_yoff.as<int64_t>();
     PL.ytot = (int64_t)c.patches.idx.size();
     W.ypatch.alloc((size_t)S * PL.ytot * 16);
-    PL.shared_size = c.patch_shared_size;
-    PL.exc_size = c.patch_exc_size;
-    PL.group_stride = c.patch_shared_size + (int64_t)R * c.patch_exc_size;
+    PL.shared_size = mixed ? c.patch_shared_size_h : c.patch_shared_size;
+    PL.exc_size = mixed ? c.patch_exc_size_h : c.patch_exc_size;
+    PL.group_stride = PL.shared_size + (int64_t)R * PL.exc_size;
     PL.npatch = c.patches.npatch;
     PL.nmax = c.patches.nmax;
-    W.inv.alloc((size_t)G * PL.group_stride * sizeof(PP));
+    PL.pg = std::max(1, std::min(c.precond_share, G));
+    if (PL.pg * R > 8) throw std::invalid_argument("precond_share x nports must be <= 8");
+    const int nq = (G + PL.pg - 1) / PL.pg;
+    W.inv.alloc((size_t)nq * PL.group_stride * sizeof(PP));
     float *scale = nullptr;
     if (mixed) {
-      W.pscale.alloc((size_t)S * PL.npatch * sizeof(float));
+      W.pscale.alloc((size_t)nq * R * PL.npatch * sizeof(float));
       scale = W.pscale.as<float>();
     }
-    launch_patch_build<PP>(st, PL, S, R, W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(), scale);
+    launch_patch_build<PP>(st, PL, G, R, W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(), scale);
   } else {
     W.inv.alloc((size_t)S * nd * n * sizeof(P));
     launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
@@ -700,14 +811,13 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
     if (use_patch)
       launch_patch_apply<PP, X>(st, PL, G, R, n, W.inv.as<PP>(), mixed ? W.pscale.as<float>() : nullptr, v, ldv_sys,
-                                z, act, W.ypatch.as<double>());
+                                z, act, (void *)W.ypatch.p);
     else
       launch_bj_apply<P, X>(st, S, n, nd, c.d_elem_dofs.as<int32_t>(), c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                             v, ldv_sys, z, act, (int64_t)nd * n);
     if (use_coarse) {  // multiplicative second level: z += C Ac^-1 C^T (v - A z)
       const int nc = c.coarse.nc;
-      launch_spmv<X, X>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), vals, nnz,
-                        W.masks.as<uint8_t>(), z, W.t.as<X>(), nullptr, act, 0, c.lpr, nnz);
+      sweep_spmv<X>(c, W, S, R, z, W.t.as<X>(), 0, act);
       launch_coarse_restrict<X>(st, S, nc, c.dc_col_ptr.as<int32_t>(), c.dc_col_row.as<int32_t>(),
                                 c.dc_col_val.as<double>(), v, ldv_sys, W.t.as<X>(), n, W.rc.as<double>(), act);
       launch_coarse_solve(st, S, R, nc, W.cinv.as<double>(), W.rc.as<double>(), W.yc.as<double>(), act);
@@ -739,12 +849,13 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   iters.assign(S, 0);
   relres.assign(S, 0.0);
   conv.assign(S, 0);
-  const double2 *x64 = W.x.as<double2>();
+  // the iterate lives system-interleaved (the layout the batched operator reads) until the end
+  W.xil.alloc((size_t)S * n * 16);
+  launch_transpose<double2>(st, n, S, W.x.as<double2>(), W.xil.as<double2>(), true);
+  const double2 *x64 = W.xil.as<double2>();
   while (true) {
     // true residual in FP64: r = mask .* (-A x)
-    launch_spmv<double2, double2>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
-                                  W.A64.as<double2>(), nnz, W.masks.as<uint8_t>(), x64, W.r.as<double2>(), nullptr,
-                                  nullptr, 1, c.lpr, nnz);
+    sweep_spmv<double2>(c, W, S, R, x64, W.r.as<double2>(), 1, nullptr);
     launch_norm2<double2>(st, S, n, W.r.as<double2>(), W.partial.as<double>(), nblk, nullptr);
     launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0, nullptr);
     HIPCHK(hipMemcpyAsync(rn2.data(), W.nrm2.p, S * 8, hipMemcpyDeviceToHost, st));
@@ -781,9 +892,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     for (j = 0; j < m; j++) {
       X *Vj = W.V.as<X>() + (int64_t)j * n;
       precond(Vj, ldv, W.z.as<X>(), W.active.as<int32_t>());
-      launch_spmv<X, X>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), vals, nnz,
-                        W.masks.as<uint8_t>(), W.z.as<X>(), W.w.as<X>(), nullptr, W.active.as<int32_t>(), 0, c.lpr,
-                        nnz);
+      sweep_spmv<X>(c, W, S, R, W.z.as<X>(), W.w.as<X>(), 0, W.active.as<int32_t>());
       double *Hcol = W.Hh.as<double>() + (int64_t)j * (m + 1) * 2;
       const int ldH = m * (m + 1);
       for (int pass = 0; pass < passes; pass++) {
@@ -816,7 +925,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     launch_combine_basis<X>(st, S, n, m, W.V.as<X>(), ldv, W.y.as<double>(), W.kfinal.as<int32_t>(), W.u.as<X>(),
                             W.cyc_active.as<int32_t>());
     precond(W.u.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>());
-    launch_axpy<double2, X>(st, S, n, W.z.as<X>(), W.x.as<double2>(), W.cyc_active.as<int32_t>());
+    launch_axpy_il<double2, X>(st, S, n, W.z.as<X>(), W.xil.as<double2>(), W.cyc_active.as<int32_t>());
     HIPCHK(hipMemcpyAsync(kf.data(), W.kfinal.p, S * 4, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> ca(S);
     HIPCHK(hipMemcpyAsync(ca.data(), W.cyc_active.p, S * 4, hipMemcpyDeviceToHost, st));
@@ -824,6 +933,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     for (int i = 0; i < S; i++)
       if (ca[i]) iters[i] += kf[i];
   }
+  launch_transpose<double2>(st, n, S, W.xil.as<double2>(), W.x.as<double2>(), false);
 }
 
 // H = P^{-1} (j/w) Qs x  by batched CG with point Jacobi on P (fem3D.cpp:610-648)
@@ -1047,6 +1157,9 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   d_ovval.upload(ov.val, st);
   const int64_t n = s.ndof, nnz = s.nnz;
   const int nslot = (int)c.slot_attr.size();
+  SweepOperator sop;
+  build_sweep_operator(c, ov, false, sop);
+
   // Qs = sum_slot Q_slot / (mu0 mur_slot): H rhs = (j/w) Qs x
   {
     std::vector<double> wq(nslot);
@@ -1077,17 +1190,23 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       // split: patches touching a port DOF get one inverse per excitation
       const int64_t npch = c.patches.npatch;
       std::vector<uint8_t> pport(npch, 0);
-      std::vector<int64_t> voff(npch);
-      int64_t sh = 0, ex = 0;
+      std::vector<int64_t> voff(npch), voff_h(npch);
+      int64_t sh = 0, ex = 0, shh = 0, exh = 0;
       for (int64_t p = 0; p < npch; p++) {
         for (int32_t k = c.patches.idx_off[p]; k < c.patches.idx_off[p + 1]; k++)
           if (portdof[c.patches.idx[k]]) pport[p] = 1;
-        const int64_t sz = c.patches.val_off[p + 1] - c.patches.val_off[p];
+        const int64_t np_ = c.patches.idx_off[p + 1] - c.patches.idx_off[p];
+        const int64_t sz = np_ * np_, szh = np_ * ((np_ + 7) / 8 * 8);
         voff[p] = pport[p] ? ex : sh;
+        voff_h[p] = pport[p] ? exh : shh;
         (pport[p] ? ex : sh) += sz;
+        (pport[p] ? exh : shh) += szh;
       }
       c.patch_shared_size = sh;
       c.patch_exc_size = ex;
+      c.patch_shared_size_h = shh;
+      c.patch_exc_size_h = exh;
+      c.dp_val_off_h.upload(voff_h, st);
       c.dp_idx_off.upload(c.patches.idx_off, st);
       c.dp_idx.upload(c.patches.idx, st);
       c.dp_val_off.upload(voff, st);
@@ -1110,6 +1229,8 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   }
   Work W;
   W.masks.upload(masks, st);
+  W.M64 = sop.m64(c);
+  W.M32 = sop.m32(c);
   // mode sample offsets
   std::vector<int64_t> moff(np + 1, 0);
   for (int k = 0; k < np; k++) moff[k + 1] = moff[k] + c.ports[port_of[k]].npts * 3;
@@ -1210,11 +1331,13 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     dcM.upload(cM, st);
     dcT.upload(cT, st);
     const bool mixed = in->precision == OPEM_PREC_MIXED;
+    // per-frequency values (FP64) for the preconditioner builds
     W.A64.alloc((size_t)F * nnz * 16);
-    if (mixed) W.A32.alloc((size_t)F * nnz * 8);
     launch_combine(st, F, nnz, nslot, dcK.as<double>(), dcM.as<double>(), c.d_K.as<double>(), c.d_M.as<double>(),
                    (int64_t)ov.nz.size(), (int)ov.terms.size(), d_ovnz.as<int32_t>(), d_ovval.as<double>(),
-                   dcT.as<double>(), W.A64.as<double>(), mixed ? W.A32.as<float>() : nullptr);
+                   dcT.as<double>(), W.A64.as<double>(), nullptr);
+    sweep_coefs(std::vector<double>(in->freqs + f0, in->freqs + f0 + F), R, cT, (int)ov.terms.size(), st, W.csys64,
+                W.dsys64, W.csys32, W.dsys32);
     // x = Dirichlet values of the driven port
     W.x.alloc((size_t)S * n * 16);
     HIPCHK(hipMemsetAsync(W.x.p, 0, (size_t)S * n * 16, st));
@@ -1245,9 +1368,9 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       W.r.alloc((size_t)S * n * 16);
       W.partial.alloc((size_t)S * nblk * kMaxBasis * 16);
       W.nrm2.alloc((size_t)S * 8);
-      launch_spmv<double2, double2>(st, S / R, R, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
-                                    W.A64.as<double2>(), nnz, W.masks.as<uint8_t>(), W.x.as<double2>(),
-                                    W.r.as<double2>(), nullptr, nullptr, 1, c.lpr, nnz);
+      W.xil.alloc((size_t)S * n * 16);
+      launch_transpose<double2>(st, n, S, W.x.as<double2>(), W.xil.as<double2>(), true);
+      sweep_spmv<double2>(c, W, S, R, W.xil.as<double2>(), W.r.as<double2>(), 1, nullptr);
       launch_norm2<double2>(st, S, n, W.r.as<double2>(), W.partial.as<double>(), nblk, nullptr);
       launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0, nullptr);
       std::vector<double> an(S);
@@ -1525,6 +1648,59 @@ int opem_system_matrix(opem_ctx *ctx, double freq, const double *gamma, double *
   });
 }
 
+int opem_sweep_apply(opem_ctx *ctx, int prec, int nfreq, const double *freqs, const double *gamma, const double *x,
+                     double *y) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (nfreq < 1 || nfreq > 256 || !freqs || !gamma || !x || !y || (prec != 0 && prec != 1))
+      throw std::invalid_argument("bad sweep-apply arguments");
+    for (int f = 0; f < nfreq; f++)
+      if (!(freqs[f] > 0)) throw std::invalid_argument("frequencies must be positive");
+    Ctx &c = *ctx;
+    for (int sl = 0; sl < (int)c.slot_attr.size(); sl++)
+      if (!c.mat.count(c.slot_attr[sl])) throw std::invalid_argument("material not set");
+    hipStream_t st = c.st;
+    const int64_t n = c.sp.ndof;
+    build_ports(c);
+    Overlay ov;
+    build_overlay(c, ov);
+    SweepOperator sop;
+    build_sweep_operator(c, ov, false, sop);
+    std::vector<double> cK, cM, cT;
+    const size_t npt = c.ports.size();
+    for (int f = 0; f < nfreq; f++) {
+      std::vector<cplx> gam(npt);
+      for (size_t k = 0; k < npt; k++) gam[k] = cplx(gamma[(f * npt + k) * 2], gamma[(f * npt + k) * 2 + 1]);
+      frequency_coefs(c, ov, freqs[f], gam, cK, cM, cT);
+    }
+    DevBuf cs64, ds64, cs32, ds32, mask, xs, xi, yd;
+    sweep_coefs(std::vector<double>(freqs, freqs + nfreq), 1, cT, sop.nterm, st, cs64, ds64, cs32, ds32);
+    mask.upload(std::vector<uint8_t>(n, 1), st);
+    const size_t tot = (size_t)nfreq * n;
+    if (prec == 0) {
+      xs.upload(std::vector<double>(x, x + 2 * tot), st);
+      xi.alloc(tot * 16);
+      yd.alloc(tot * 16);
+      launch_transpose<double2>(st, n, nfreq, xs.as<double2>(), xi.as<double2>(), true);
+      launch_spmv_sweep<double, double2>(st, sop.m64(c), nfreq, 1, cs64.as<double>(), ds64.as<double>(),
+                                         mask.as<uint8_t>(), xi.as<double2>(), yd.as<double2>(), 0, nullptr, c.lpr);
+      HIPCHK(hipMemcpyAsync(y, yd.p, tot * 16, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    } else {
+      std::vector<float> x32(x, x + 2 * tot), y32(2 * tot);
+      xs.upload(x32, st);
+      xi.alloc(tot * 8);
+      yd.alloc(tot * 8);
+      launch_transpose<float2>(st, n, nfreq, xs.as<float2>(), xi.as<float2>(), true);
+      launch_spmv_sweep<float, float2>(st, sop.m32(c), nfreq, 1, cs32.as<float>(), ds32.as<float>(),
+                                       mask.as<uint8_t>(), xi.as<float2>(), yd.as<float2>(), 0, nullptr, c.lpr);
+      HIPCHK(hipMemcpyAsync(y32.data(), yd.p, tot * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (size_t i = 0; i < 2 * tot; i++) y[i] = y32[i];
+    }
+  });
+}
+
 int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
               const double *x, double *y) {
   return guarded(ctx, [&] {
@@ -1574,7 +1750,10 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
       throw std::invalid_argument("bad precision");
     if (in->precond < 0 || in->precond > 1) throw std::invalid_argument("bad preconditioner");
     if (in->coarse_size < 0 || in->coarse_size > 128) throw std::invalid_argument("coarse_size must be 0..128");
+    if (in->precond_share < 0 || in->precond_share * in->nports > 8)
+      throw std::invalid_argument("precond_share x nports must be <= 8");
     ctx->precond = in->precond;
+    ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
     if (ctx->coarse_size != in->coarse_size) ctx->have_coarse = false;
     ctx->coarse_size = in->coarse_size;
     sweep(*ctx, in, out);
@@ -1629,39 +1808,49 @@ int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const 
 int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, double *avg_seconds, double *alg_bytes) {
   return guarded(ctx, [&] {
     if (!ctx->have_space) throw std::invalid_argument("space not built");
-    if (ngroups < 1 || R < 1 || R > 4 || nrep < 1 || (prec != 0 && prec != 1))
+    if (ngroups < 1 || R < 1 || R > 4 || nrep < 1 || (prec != 0 && prec != 1) || ngroups * R > 256)
       throw std::invalid_argument("bad spmv bench arguments");
-    const Space &s = ctx->sp;
-    hipStream_t st = ctx->st;
+    Ctx &c = *ctx;
+    const Space &s = c.sp;
+    hipStream_t st = c.st;
     const int64_t n = s.ndof, nnz = s.nnz;
     const int S = ngroups * R;
+    // the operator of a sweep on this context: unit-material volume part plus its boundary terms
+    build_ports(c);
+    Overlay ov;
+    build_overlay(c, ov);
+    SweepOperator sop;
+    build_sweep_operator(c, ov, true, sop);
+    const int64_t nov = sop.nov;
+    const int nterm = sop.nterm;
+    std::vector<double> fr(ngroups);
+    for (int g = 0; g < ngroups; g++) fr[g] = 1e9 * (1.0 + 0.01 * g);
+    std::vector<double> cT((size_t)ngroups * std::max(1, nterm) * 2, 0.0);
+    for (size_t i = 1; i < cT.size(); i += 2) cT[i] = 1.0;
+    DevBuf cs64, ds64, cs32, ds32, x, y, mask;
+    sweep_coefs(fr, R, cT, nterm, st, cs64, ds64, cs32, ds32);
     const size_t sv = prec == 0 ? 16 : 8;
-    DevBuf vals, x, y, mask;
-    vals.alloc((size_t)ngroups * nnz * sv);
     x.alloc((size_t)S * n * sv);
     y.alloc((size_t)S * n * sv);
     mask.upload(std::vector<uint8_t>((size_t)R * n, 1), st);
+    const SweepMatrix<double> M64 = sop.m64(c);
+    const SweepMatrix<float> M32 = sop.m32(c);
+    auto run = [&]() {
+      if (prec == 0)
+        launch_spmv_sweep<double, double2>(st, M64, S, R, cs64.as<double>(), ds64.as<double>(), mask.as<uint8_t>(),
+                                           x.as<double2>(), y.as<double2>(), 0, nullptr, c.lpr);
+      else
+        launch_spmv_sweep<float, float2>(st, M32, S, R, cs32.as<float>(), ds32.as<float>(), mask.as<uint8_t>(),
+                                         x.as<float2>(), y.as<float2>(), 0, nullptr, c.lpr);
+    };
+    if (prec == 0)
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, x.as<double2>(), 1.0);
+    else
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, x.as<float2>(), 1.0);
+    run();
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
-    auto run = [&]() {
-      if (prec == 0)
-        launch_spmv<double2, double2>(st, ngroups, R, n, ctx->d_rowptr.as<int32_t>(), ctx->d_colind.as<int32_t>(),
-                                      vals.as<double2>(), nnz, mask.as<uint8_t>(), x.as<double2>(), y.as<double2>(),
-                                      nullptr, nullptr, 0, ctx->lpr, nnz);
-      else
-        launch_spmv<float2, float2>(st, ngroups, R, n, ctx->d_rowptr.as<int32_t>(), ctx->d_colind.as<int32_t>(),
-                                    vals.as<float2>(), nnz, mask.as<uint8_t>(), x.as<float2>(), y.as<float2>(),
-                                    nullptr, nullptr, 0, ctx->lpr, nnz);
-    };
-    if (prec == 0) {
-      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)ngroups * nnz, vals.as<double2>(), 1.0);
-      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, x.as<double2>(), 1.0);
-    } else {
-      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)ngroups * nnz, vals.as<float2>(), 1.0);
-      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, x.as<float2>(), 1.0);
-    }
-    run();
     HIPCHK(hipEventRecord(a, st));
     for (int r = 0; r < nrep; r++) run();
     HIPCHK(hipEventRecord(b, st));
@@ -1670,9 +1859,77 @@ int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, doubl
     HIPCHK(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    const double sr = prec == 0 ? 8.0 : 4.0;
     if (avg_seconds) *avg_seconds = ms * 1e-3 / nrep;
     if (alg_bytes)
-      *alg_bytes = 4.0 * nnz + 4.0 * (n + 1) + (double)ngroups * sv * nnz + (double)S * 2.0 * n * sv;
+      *alg_bytes = 4.0 * (n + 1) + 4.0 * nnz + 2.0 * sr * nnz + 4.0 * (n + 1) + 4.0 * nov + sr * nterm * nov +
+                   (double)R * n + 2.0 * S * n * sv;
+  });
+}
+
+int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *avg_seconds, double *alg_bytes) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (S < 1 || k < 1 || k + 1 > kMaxBasis || nrep < 1 || (prec != 0 && prec != 1))
+      throw std::invalid_argument("bad krylov bench arguments");
+    hipStream_t st = ctx->st;
+    const int64_t n = ctx->sp.ndof;
+    const int nblk = reduce_blocks(n);
+    const size_t sv = prec == 0 ? 16 : 8;
+    const int64_t ldv = (int64_t)(k + 1) * n;
+    DevBuf V, w, part, h;
+    V.alloc((size_t)S * ldv * sv);
+    w.alloc((size_t)S * n * sv);
+    part.alloc((size_t)S * nblk * kMaxBasis * 16);
+    h.alloc((size_t)S * kMaxBasis * 16);
+    if (prec == 0) {
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * ldv, V.as<double2>(), 1e-3);
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, w.as<double2>(), 1e-3);
+    } else {
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * ldv, V.as<float2>(), 1e-3);
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, w.as<float2>(), 1e-3);
+    }
+    HIPCHK(hipMemsetAsync(h.p, 0, (size_t)S * kMaxBasis * 16, st));
+    auto mdot = [&]() {
+      if (prec == 0)
+        launch_mdot<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, part.as<double>(), nblk, nullptr);
+      else
+        launch_mdot<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, part.as<double>(), nblk, nullptr);
+    };
+    auto mupd = [&]() {
+      if (prec == 0)
+        launch_mupdate<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, h.as<double>(), kMaxBasis,
+                                part.as<double>(), nblk, nullptr);
+      else
+        launch_mupdate<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, h.as<double>(), kMaxBasis,
+                               part.as<double>(), nblk, nullptr);
+    };
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    mdot();
+    mupd();
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < nrep; r++) mdot();
+    HIPCHK(hipEventRecord(e1, st));
+    for (int r = 0; r < nrep; r++) mupd();
+    HIPCHK(hipEventRecord(e2, st));
+    HIPCHK(hipEventSynchronize(e2));
+    float m1 = 0, m2 = 0;
+    HIPCHK(hipEventElapsedTime(&m1, e0, e1));
+    HIPCHK(hipEventElapsedTime(&m2, e1, e2));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (avg_seconds) {
+      avg_seconds[0] = m1 * 1e-3 / nrep;
+      avg_seconds[1] = m2 * 1e-3 / nrep;
+    }
+    if (alg_bytes) {
+      alg_bytes[0] = (double)S * (k + 1) * n * sv + (double)S * nblk * k * 16;
+      alg_bytes[1] = (double)S * (k + 2) * n * sv + (double)S * k * 16 + (double)S * nblk * 8;
+    }
   });
 }
 

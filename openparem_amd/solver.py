@@ -89,6 +89,18 @@ class Solver:
               self.ctx, "opem_system_matrix")
         return out
 
+    def sweep_apply(self, freqs, gammas, x, precision="fp64"):
+        """y[f] = A(freqs[f]) x[f] through the batched sweep operator (gammas[f] = {attr: gamma})."""
+        freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
+        g = np.asarray([[gm[a] for a in self.ports] for gm in gammas], np.complex128)
+        g = cplx_view(np.ascontiguousarray(g))
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.empty_like(x)
+        check(lib().opem_sweep_apply(self.ctx, 0 if precision == "fp64" else 1, freqs.size, ptr(freqs, C.c_double),
+                                     ptr(g, C.c_double), ptr(x.view(np.float64), C.c_double),
+                                     ptr(y.view(np.float64), C.c_double)), self.ctx, "opem_sweep_apply")
+        return y
+
     def port_points(self, attr):
         n = C.c_int64()
         check(lib().opem_port_points(self.ctx, attr, C.byref(n), ptr(None, C.c_double)), self.ctx, "port_points")
@@ -103,9 +115,18 @@ class Solver:
                                     C.byref(b)), self.ctx, "opem_spmv_bench")
         return t.value, b.value
 
+    def krylov_bench(self, precision="fp32", S=128, k=76, nrep=5):
+        """((mdot s, mupdate s), (mdot bytes, mupdate bytes)) of the batched Gram-Schmidt kernels."""
+        t = (C.c_double * 2)()
+        b = (C.c_double * 2)()
+        check(lib().opem_krylov_bench(self.ctx, 0 if precision == "fp64" else 1, S, k, nrep, t, b), self.ctx,
+              "opem_krylov_bench")
+        return (t[0], t[1]), (b[0], b[1])
+
     # ------------------------------------------------------------------ hot path
     def sweep(self, freqs, precision="fp64", batch=0, restart=150, maxit=20000, tol=1e-10, reciprocal=True,
-              h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch", coarse=0, orth_passes=2):
+              h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch", coarse=0, orth_passes=2,
+              precond_share=1):
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
         nf, np_ = freqs.size, len(self.ports)
         mode_fn = mode_fn or provider_for(self.mesh)
@@ -130,7 +151,8 @@ class Solver:
                            precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64,
                            batch=int(batch), restart=int(restart), maxit=int(maxit), tol=float(tol),
                            reciprocal=int(bool(reciprocal)), h_tol=float(h_tol),
-                           precond={"patch": 0, "element": 1}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes))
+                           precond={"patch": 0, "element": 1}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes),
+                           precond_share=int(precond_share))
         S = np.zeros((nf, np_, np_), np.complex128)
         Sr = np.zeros_like(S)
         Cp = np.zeros_like(S)

@@ -98,6 +98,31 @@ def test_system_matrix_microstrip(p):
     assert d <= 1e-11 * abs(ref).max(), d
 
 
+@pytest.mark.parametrize("nf", [3, 12, 70])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
+def test_sweep_operator(nf, prec, tol):
+    """The batched operator of the solve (one pass of the frequency-independent matrices for every frequency:
+    lossy substrate, impedance trace, radiation lid, two port ABCs) against A(f) x from the assembled values
+    (themselves pinned to the oracle above).  nf = 3 takes the narrow form, 12 and 70 the wide one; the
+    error is measured against sum_j |A_ij| |x_j| per row."""
+    m = microstrip_mesh(nx=6, ny=4, nz=4, n_sub=1, seed=2)
+    fr = np.linspace(2e9, 9e9, nf)
+    gams = [{2: 0.3 + 180j * f / 5e9, 3: 0.2 + 150j * f / 5e9} for f in fr]
+    rng = np.random.default_rng(nf)
+    s = _solver(m, 2)
+    try:
+        x = rng.standard_normal((nf, s.ndof)) + 1j * rng.standard_normal((nf, s.ndof))
+        y = s.sweep_apply(fr, gams, x, prec)
+        A = [_csr(s, s.system_matrix(f, g)) for f, g in zip(fr, gams)]
+    finally:
+        s.close()
+    for i in range(nf):
+        ref = A[i] @ x[i]
+        bound = abs(A[i]) @ np.abs(x[i])
+        err = (np.abs(y[i] - ref) / bound.max()).max()
+        assert err < tol, (i, err)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-14), ("fp32", 2e-6)])
 def test_spmv(prec, tol):
     from openparem_amd.solver import Device, spmv
@@ -216,15 +241,29 @@ def test_sweep_nd3():
     assert np.abs(r["S"][0] - o["S"]).max() < 1e-8
 
 
-def test_batched_multigroup_spmv_sweep():
-    """Nine frequencies in one batch (multi-group SpMV path for >= 8 matrix groups) against one-at-a-time FP64."""
+@pytest.fixture(scope="module")
+def nine_point_fp64():
+    """Nine frequencies solved one at a time in FP64 (the batched runs below are checked against it)."""
     m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
     f = np.linspace(8e9, 12e9, 9)
     s = _solver(m, 2)
     try:
-        rb = s.sweep(f, precision="mixed", tol=1e-10, batch=9, orth_passes=1)
         r1 = s.sweep(f, precision="fp64", tol=1e-10, batch=1)
     finally:
         s.close()
-    assert rb["converged"].all() and r1["converged"].all()
+    assert r1["converged"].all()
+    return m, f, r1
+
+
+@pytest.mark.parametrize("share,prec", [(1, "mixed"), (2, "mixed"), (4, "mixed"), (3, "fp64")])
+def test_batched_sweep_shared_patch_inverses(nine_point_fp64, share, prec):
+    """Nine frequencies in one batch with one set of patch inverses per `share` adjacent frequencies (the
+    last set covering fewer when 9 % share != 0): same S as one-at-a-time FP64."""
+    m, f, r1 = nine_point_fp64
+    s = _solver(m, 2)
+    try:
+        rb = s.sweep(f, precision=prec, tol=1e-10, batch=9, orth_passes=1, precond_share=share)
+    finally:
+        s.close()
+    assert rb["converged"].all()
     assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
