@@ -13,7 +13,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libopem.so")
-SOURCES = ["kernels.hip", "elemmat.hip", "opem.hip", "nd_tables.cpp", "space.cpp", "quadrature.cpp", "coarse.cpp"]
+SOURCES = ["kernels.hip", "elemmat.hip", "opem.hip", "nd_tables.cpp", "space.cpp", "quadrature.cpp", "coarse.cpp",
+           "poly.cpp"]
 ARCH = "gfx950"
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -63,6 +63,7 @@ struct SweepMatrix {
   const V *ov_val;
   int64_t nov;
   int nterm;
+  int blk16_max;  // largest number of entries in 16 consecutive rows (LDS staging bound)
 };
 // y[s*n + row] = rowmask[s%R][row] * (mode ? -(A_s x_s) : A_s x_s)[row] for active s; x system-interleaved
 // (x[j*S + s]); csys[s] = c_s, dsys[s][t] complex.  S <= 256.
@@ -80,8 +81,20 @@ void launch_transpose(hipStream_t st, int64_t n, int S, const X *in, X *out, boo
 template <typename X, typename Z>
 void launch_axpy_il(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32_t *active);
 
+// ---------------------------------------------------------------- polynomial preconditioner
+// One step of p(B) v for every system (vectors system-major, stride n): u += c_s prod, then (t != null)
+// prod -= c_s t, with c_s = coef[s*ldc], coef[s*ldc+1] (complex); first: u = c_s prod.
+template <typename X>
+void launch_poly_step(hipStream_t st, int S, int64_t n, const double *coef, int ldc, X *prod, const X *t, X *u,
+                      bool first, const int32_t *active);
+// out[s*ldout + i] = in[s*ldin + i], i < n, for active s
+template <typename X>
+void launch_copy_sys(hipStream_t st, int S, int64_t n, const X *in, int64_t ldin, X *out, int64_t ldout,
+                     const int32_t *active);
+
 // ---------------------------------------------------------------- Krylov helpers (batched, S systems)
-// partial[s][blk][i] = sum_{chunk} conj(Vb[s][i][n]) w[s][n], i < k;  vectors in basis stride ldv.
+// partial[s][blk][i] = sum_{chunk} conj(Vb[s][i][n]) w[s][n], i < k;  vectors in basis stride ldv;
+// nblk must be mdot_blocks(n): one partial per wavefront (4 per workgroup of 2048 elements).
 template <typename X>
 void launch_mdot(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *w,
                  int64_t ldw_sys, double *partial, int nblk, const int32_t *active);
@@ -162,11 +175,16 @@ void launch_axpy(hipStream_t st, int S, int64_t n, const Z *z, X *x, const int32
 template <typename X>
 void launch_norm2(hipStream_t st, int S, int64_t n, const X *x, double *part, int nblk, const int32_t *active);
 int reduce_blocks(int64_t n);
+int mdot_blocks(int64_t n);
 // OPEM_XCD_REMAP=1 remaps block ids so each XCD works on one contiguous range (measured slower on the
 // sweep: off by default).
 int xcd_remap_enabled();
 // OPEM_PATCH_MFMA=0 applies FP16 patch inverses with vector FMAs instead of MFMA (default 1).
 int patch_mfma_enabled();
+// OPEM_SPMV_WIDE=1 runs the batched operator one row per wavefront over all systems instead of 16-system tiles.
+int spmv_wide_enabled();
+// OPEM_SPMV_PAIR=0 disables the two-systems-per-lane form of the batched operator.
+int spmv_pair_enabled();
 
 // ---------------------------------------------------------------- Givens / Hessenberg (device)
 // Apply previous rotations to column j of H[s] ((m+1) x m, column-major, complex), new rotation,

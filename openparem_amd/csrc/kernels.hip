@@ -266,6 +266,24 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t *__restri
   }
 }
 
+int spmv_pair_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("OPEM_SPMV_PAIR");
+    v = e ? atoi(e) : 1;
+  }
+  return v;
+}
+
+int spmv_wide_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("OPEM_SPMV_WIDE");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
+}
+
 int patch_mfma_enabled() {
   static int v = -1;
   if (v < 0) {
@@ -428,6 +446,164 @@ __global__ __launch_bounds__(256) void k_spmv_sweep_w(SweepMatrix<V> M, int S, i
   }
 }
 
+// Tiled form (S > 8): a workgroup covers 16 rows x one tile of 16 systems, lane (row r, system s) walks row r
+// for system s, so each gathered column is one 128-byte segment of x.  Workgroups are numbered tile-fastest:
+// with 8 tiles (S = 128) workgroup b runs on XCD b % 8 = its tile, so each XCD streams one tile's rows in
+// order and keeps that tile's window of x in its own L2 (the matrix is shared through the MALL).
+template <typename V, typename X, bool LOSSY>
+__global__ __launch_bounds__(256) void k_spmv_sweep_t(SweepMatrix<V> M, int S, int R, const V *__restrict__ csys,
+                                                      const V *__restrict__ dsys, const uint8_t *__restrict__ rowmask,
+                                                      const X *__restrict__ x, X *__restrict__ y, int mode,
+                                                      const int32_t *__restrict__ active) {
+  const int ntile = (S + 15) >> 4;
+  const int64_t b = blockIdx.x;
+  const int tile = (int)(b % ntile);
+  const int64_t row = (b / ntile) * 16 + (threadIdx.x >> 4);
+  const int s = tile * 16 + (threadIdx.x & 15);
+  if (row >= M.n) return;
+  const bool on = s < S && (active == nullptr || active[s] != 0);
+  if (!on) return;
+  const V cs = csys[s];
+  V ax = (V)0, ay = (V)0;
+  const int beg = M.rowptr[row], end = M.rowptr[row + 1];
+#pragma unroll 4
+  for (int k = beg; k < end; k++) {
+    const int64_t j = M.colind[k];
+    const V ar = M.Kt[k] + cs * M.Mr[k];
+    const X xv = x[j * S + s];
+    if constexpr (LOSSY) {
+      const V ai = -cs * M.Mi[k];
+      ax += ar * xv.x - ai * xv.y;
+      ay += ar * xv.y + ai * xv.x;
+    } else {
+      ax += ar * xv.x;
+      ay += ar * xv.y;
+    }
+  }
+  if (M.nterm > 0) {
+    for (int e = M.ov_ptr[row]; e < M.ov_ptr[row + 1]; e++) {
+      const int64_t j = M.ov_col[e];
+      V cr = (V)0, ci = (V)0;
+      for (int t = 0; t < M.nterm; t++) {
+        const V tv = M.ov_val[(int64_t)t * M.nov + e];
+        cr += dsys[((int64_t)s * M.nterm + t) * 2] * tv;
+        ci += dsys[((int64_t)s * M.nterm + t) * 2 + 1] * tv;
+      }
+      const X xv = x[j * S + s];
+      ax += cr * xv.x - ci * xv.y;
+      ay += cr * xv.y + ci * xv.x;
+    }
+  }
+  const bool free_row = rowmask[(int64_t)(s % R) * M.n + row] != 0;
+  X o;
+  o.x = free_row ? (mode ? -ax : ax) : (V)0;
+  o.y = free_row ? (mode ? -ay : ay) : (V)0;
+  y[(int64_t)s * M.n + row] = o;
+}
+
+// Paired form (S > 8, S even): a workgroup covers 16 rows x one tile of 32 systems; lane (row r, l) owns
+// systems s, s+1 (s = 32*tile + 2l), so a gathered column is one 16-byte load per lane (FP32) and one
+// 256-byte segment per row.  The workgroup's matrix entries (column, Kt, Mr[, Mi]) are staged in LDS
+// first, so the row loop issues one global load per two systems.  Workgroups are numbered tile-fastest.
+template <typename X>
+struct Pair;
+template <>
+struct Pair<float2> {
+  using T = float4;
+};
+template <>
+struct Pair<double2> {
+  struct T {
+    double x, y, z, w;
+  };
+};
+
+template <typename V, typename X, bool LOSSY>
+__global__ __launch_bounds__(256) void k_spmv_sweep_p(SweepMatrix<V> M, int S, int R, const V *__restrict__ csys,
+                                                      const V *__restrict__ dsys, const uint8_t *__restrict__ rowmask,
+                                                      const X *__restrict__ x, X *__restrict__ y, int mode,
+                                                      const int32_t *__restrict__ active) {
+  using P2 = typename Pair<X>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ntile = (S + 31) >> 5;
+  const int64_t b = blockIdx.x;
+  const int tile = (int)(b % ntile);
+  const int64_t r0 = (b / ntile) * 16;
+  const int64_t rend = r0 + 16 < M.n ? r0 + 16 : M.n;
+  const int e0 = M.rowptr[r0], ne = M.rowptr[rend] - e0;
+  V *lk = (V *)smem;
+  V *lm = lk + M.blk16_max;
+  V *lmi = lm + M.blk16_max;
+  int32_t *lcol = (int32_t *)(lmi + (LOSSY ? M.blk16_max : 0));
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    lcol[e] = M.colind[e0 + e];
+    lk[e] = M.Kt[e0 + e];
+    lm[e] = M.Mr[e0 + e];
+    if constexpr (LOSSY) lmi[e] = M.Mi[e0 + e];
+  }
+  __syncthreads();
+  const int64_t row = r0 + (threadIdx.x >> 4);
+  const int s = tile * 32 + 2 * (threadIdx.x & 15);
+  if (row >= M.n || s >= S) return;
+  const bool on0 = active == nullptr || active[s] != 0, on1 = active == nullptr || active[s + 1] != 0;
+  if (!on0 && !on1) return;
+  const V c0 = csys[s], c1 = csys[s + 1];
+  V a0x = (V)0, a0y = (V)0, a1x = (V)0, a1y = (V)0;
+  const int kb = M.rowptr[row] - e0, ke = M.rowptr[row + 1] - e0;
+#pragma unroll 4
+  for (int k = kb; k < ke; k++) {
+    const int64_t j = lcol[k];
+    const V kv = lk[k], mr = lm[k];
+    const P2 xv = *(const P2 *)(x + j * S + s);
+    const V r0v = kv + c0 * mr, r1v = kv + c1 * mr;
+    if constexpr (LOSSY) {
+      const V mi = lmi[k];
+      const V i0 = -c0 * mi, i1 = -c1 * mi;
+      a0x += r0v * xv.x - i0 * xv.y;
+      a0y += r0v * xv.y + i0 * xv.x;
+      a1x += r1v * xv.z - i1 * xv.w;
+      a1y += r1v * xv.w + i1 * xv.z;
+    } else {
+      a0x += r0v * xv.x;
+      a0y += r0v * xv.y;
+      a1x += r1v * xv.z;
+      a1y += r1v * xv.w;
+    }
+  }
+  if (M.nterm > 0) {
+    for (int e = M.ov_ptr[row]; e < M.ov_ptr[row + 1]; e++) {
+      const int64_t j = M.ov_col[e];
+      V d0r = (V)0, d0i = (V)0, d1r = (V)0, d1i = (V)0;
+      for (int t = 0; t < M.nterm; t++) {
+        const V tv = M.ov_val[(int64_t)t * M.nov + e];
+        d0r += dsys[((int64_t)s * M.nterm + t) * 2] * tv;
+        d0i += dsys[((int64_t)s * M.nterm + t) * 2 + 1] * tv;
+        d1r += dsys[((int64_t)(s + 1) * M.nterm + t) * 2] * tv;
+        d1i += dsys[((int64_t)(s + 1) * M.nterm + t) * 2 + 1] * tv;
+      }
+      const P2 xv = *(const P2 *)(x + j * S + s);
+      a0x += d0r * xv.x - d0i * xv.y;
+      a0y += d0r * xv.y + d0i * xv.x;
+      a1x += d1r * xv.z - d1i * xv.w;
+      a1y += d1r * xv.w + d1i * xv.z;
+    }
+  }
+  if (on0) {
+    const bool fr = rowmask[(int64_t)(s % R) * M.n + row] != 0;
+    X o;
+    o.x = fr ? (mode ? -a0x : a0x) : (V)0;
+    o.y = fr ? (mode ? -a0y : a0y) : (V)0;
+    y[(int64_t)s * M.n + row] = o;
+  }
+  if (on1) {
+    const bool fr = rowmask[(int64_t)((s + 1) % R) * M.n + row] != 0;
+    X o;
+    o.x = fr ? (mode ? -a1x : a1x) : (V)0;
+    o.y = fr ? (mode ? -a1y : a1y) : (V)0;
+    y[(int64_t)(s + 1) * M.n + row] = o;
+  }
+}
+
 // Narrow form (S <= 8): LPR lanes split a row's entries, each lane carries all S systems.
 template <typename V, typename X, int LPR, int SP, bool LOSSY>
 __global__ __launch_bounds__(256) void k_spmv_sweep_n(SweepMatrix<V> M, int S, int R, const V *__restrict__ csys,
@@ -508,7 +684,20 @@ __global__ __launch_bounds__(256) void k_spmv_sweep_n(SweepMatrix<V> M, int S, i
 template <typename V, typename X, bool LOSSY>
 static void spmv_sweep_l(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
                          const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr) {
-  if (S > 8) {
+  if (S > 8 && S % 2 == 0 && spmv_pair_enabled()) {
+    if (S > 256) throw std::runtime_error("sweep spmv: at most 256 systems per batch");
+    if (M.blk16_max <= 0) throw std::runtime_error("sweep spmv: blk16_max not set");
+    const int64_t nb = ((M.n + 15) / 16) * ((S + 31) / 32);
+    const size_t lds = (size_t)M.blk16_max * ((LOSSY ? 3 : 2) * sizeof(V) + 4);
+    if (lds > 64 * 1024) throw std::runtime_error("sweep spmv: rows too long for the staged form");
+    hipLaunchKernelGGL((k_spmv_sweep_p<V, X, LOSSY>), dim3((unsigned)nb), dim3(256), lds, st, M, S, R, csys, dsys,
+                       rowmask, x, y, mode, active);
+  } else if (S > 8 && !spmv_wide_enabled()) {
+    const int64_t nb = ((M.n + 15) / 16) * ((S + 15) / 16);
+    if (S > 256) throw std::runtime_error("sweep spmv: at most 256 systems per batch");
+    hipLaunchKernelGGL((k_spmv_sweep_t<V, X, LOSSY>), dim3((unsigned)nb), dim3(256), 0, st, M, S, R, csys, dsys,
+                       rowmask, x, y, mode, active);
+  } else if (S > 8) {
     dim3 grid((unsigned)((M.n + 15) / 16));
     if (S <= 64)
       hipLaunchKernelGGL((k_spmv_sweep_w<V, X, 1, LOSSY>), grid, dim3(256), 0, st, M, S, R, csys, dsys, rowmask, x,
@@ -664,61 +853,149 @@ template void launch_axpy_il<double2, double2>(hipStream_t, int, int64_t, const 
                                                const int32_t *);
 template void launch_axpy_il<double2, float2>(hipStream_t, int, int64_t, const float2 *, double2 *, const int32_t *);
 
+// ------------------------------------------------------------------ polynomial preconditioner steps
+// Step i of p(B) v (Loe-Morgan): u += c prod; prod -= c t, with c = coef[s*ldc + 2i..] (complex, per system);
+// t == null: last step (u += c prod only); first: u = c prod.
+template <typename X>
+__global__ void k_poly_step(int64_t n, const double *__restrict__ coef, int ldc, X *__restrict__ prod,
+                            const X *__restrict__ t, X *__restrict__ u, int first, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const double cr = coef[(int64_t)s * ldc], ci = coef[(int64_t)s * ldc + 1];
+  X *ps = prod + (int64_t)s * n;
+  X *us = u + (int64_t)s * n;
+  const X *ts = t ? t + (int64_t)s * n : nullptr;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 p = to_d2(ps[i]);
+    double2 o = first ? make_double2(0.0, 0.0) : to_d2(us[i]);
+    o.x += cr * p.x - ci * p.y;
+    o.y += cr * p.y + ci * p.x;
+    us[i] = from_d2<X>(o);
+    if (ts) {
+      const double2 q = to_d2(ts[i]);
+      ps[i] = from_d2<X>(make_double2(p.x - (cr * q.x - ci * q.y), p.y - (cr * q.y + ci * q.x)));
+    }
+  }
+}
+
+template <typename X>
+void launch_poly_step(hipStream_t st, int S, int64_t n, const double *coef, int ldc, X *prod, const X *t, X *u,
+                      bool first, const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_poly_step<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, coef, ldc, prod, t, u,
+                     first ? 1 : 0, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_poly_step<double2>(hipStream_t, int, int64_t, const double *, int, double2 *, const double2 *,
+                                        double2 *, bool, const int32_t *);
+template void launch_poly_step<float2>(hipStream_t, int, int64_t, const double *, int, float2 *, const float2 *,
+                                       float2 *, bool, const int32_t *);
+
+// out[s][i] = in[s*ldin + i] for active s
+template <typename X>
+__global__ void k_copy_sys(int64_t n, const X *__restrict__ in, int64_t ldin, X *__restrict__ out, int64_t ldout,
+                           const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[(int64_t)s * ldout + i] = in[(int64_t)s * ldin + i];
+}
+
+template <typename X>
+void launch_copy_sys(hipStream_t st, int S, int64_t n, const X *in, int64_t ldin, X *out, int64_t ldout,
+                     const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_copy_sys<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, in, ldin, out, ldout, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_copy_sys<double2>(hipStream_t, int, int64_t, const double2 *, int64_t, double2 *, int64_t,
+                                       const int32_t *);
+template void launch_copy_sys<float2>(hipStream_t, int, int64_t, const float2 *, int64_t, float2 *, int64_t,
+                                      const int32_t *);
+
 // ------------------------------------------------------------------ Krylov helpers
 constexpr int kMaxK = kMaxBasis;  // basis vectors per reduction (restart + 1)
 
-template <typename X>
+template <typename X, int E>
 __global__ __launch_bounds__(kRedThreads) void k_mdot(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
                                                       const X *__restrict__ w, int64_t ldw_sys,
                                                       double *__restrict__ partial, int nblk,
                                                       const int32_t *__restrict__ active) {
   const int s = blockIdx.y;
   if (active && !active[s]) return;
-  __shared__ double2 red[kRedThreads / 64][kMaxK];
-  const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * (kRedThreads * E) + threadIdx.x;
   const X *ws = w + s * ldw_sys;
   const X *Vs = Vb + s * ldv_sys;
-  double2 wl[kRedElems];
+  double2 wl[E];
 #pragma unroll
-  for (int e = 0; e < kRedElems; e++) {
+  for (int e = 0; e < E; e++) {
     const int64_t i = base + e * kRedThreads;
     wl[e] = i < n ? to_d2(ws[i]) : make_double2(0.0, 0.0);
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int v = 0; v < k; v++) {
-    double sx = 0.0, sy = 0.0;
+  // four basis vectors per pass; their wave sums by a reduce-scatter butterfly: after the xor-32 and
+  // xor-16 steps each 16-lane quarter holds one vector's partial sums, finished by xor 8, 4, 2, 1
+  const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
+  for (int v0 = 0; v0 < k; v0 += 4) {
+    double ax[4], ay[4];
 #pragma unroll
-    for (int e = 0; e < kRedElems; e++) {
-      const int64_t i = base + e * kRedThreads;
-      if (i < n) {
-        const double2 a = to_d2(Vs[(int64_t)v * n + i]);
-        sx += a.x * wl[e].x + a.y * wl[e].y;
-        sy += a.x * wl[e].y - a.y * wl[e].x;
+    for (int q = 0; q < 4; q++) {
+      ax[q] = ay[q] = 0.0;
+      if (v0 + q < k) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int64_t i = base + e * kRedThreads;
+          if (i < n) {
+            const double2 a = to_d2(Vs[(int64_t)(v0 + q) * n + i]);
+            ax[q] += a.x * wl[e].x + a.y * wl[e].y;
+            ay[q] += a.x * wl[e].y - a.y * wl[e].x;
+          }
+        }
       }
     }
-    sx = wave_sum(sx);
-    sy = wave_sum(sy);
-    if (lane == 0) red[wave][v] = make_double2(sx, sy);
-  }
-  __syncthreads();
-  for (int v = threadIdx.x; v < k; v += kRedThreads) {
-    double2 t = red[0][v];
-    for (int q = 1; q < kRedThreads / 64; q++) {
-      t.x += red[q][v].x;
-      t.y += red[q][v].y;
+    double bx[2], by[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const double kx = h32 ? ax[t + 2] : ax[t], ky = h32 ? ay[t + 2] : ay[t];
+      const double sx = h32 ? ax[t] : ax[t + 2], sy = h32 ? ay[t] : ay[t + 2];
+      bx[t] = kx + __shfl_xor(sx, 32, 64);
+      by[t] = ky + __shfl_xor(sy, 32, 64);
     }
-    double *p = partial + (((int64_t)s * nblk + blockIdx.x) * kMaxK + v) * 2;
-    p[0] = t.x;
-    p[1] = t.y;
+    double cx = (h16 ? bx[1] : bx[0]) + __shfl_xor(h16 ? bx[0] : bx[1], 16, 64);
+    double cy = (h16 ? by[1] : by[0]) + __shfl_xor(h16 ? by[0] : by[1], 16, 64);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      cx += __shfl_xor(cx, o, 64);
+      cy += __shfl_xor(cy, o, 64);
+    }
+    const int q = (h32 ? 2 : 0) + (h16 ? 1 : 0);
+    if ((lane & 15) == 0 && v0 + q < k) {
+      double *p = partial + (((int64_t)s * nblk + blockIdx.x * (kRedThreads / 64) + wave) * kMaxK + v0 + q) * 2;
+      p[0] = cx;
+      p[1] = cy;
+    }
   }
 }
+
+constexpr int kMdotElems = 8;  // elements per thread (4 and 16 measured slower at 128 systems x 76 vectors)
+
+static int mdot_grid(int64_t n) {
+  const int64_t c = (int64_t)kRedThreads * kMdotElems;
+  const int64_t b = (n + c - 1) / c;
+  return (int)(b < 1 ? 1 : b);
+}
+
+int mdot_blocks(int64_t n) { return mdot_grid(n) * (kRedThreads / 64); }
 
 template <typename X>
 void launch_mdot(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *w,
                  int64_t ldw_sys, double *partial, int nblk, const int32_t *active) {
   if (k > kMaxK) throw std::runtime_error("restart too large");
-  hipLaunchKernelGGL((k_mdot<X>), dim3(nblk, S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, w, ldw_sys, partial,
-                     nblk, active);
+  if (nblk != mdot_blocks(n)) throw std::runtime_error("mdot: partial block count mismatch");
+  hipLaunchKernelGGL((k_mdot<X, kMdotElems>), dim3(mdot_grid(n), S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, w,
+                     ldw_sys, partial, nblk, active);
   HIP_LAUNCH_CHECK();
 }
 template void launch_mdot<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, const double2 *,
@@ -784,26 +1061,41 @@ template void launch_mupdate<double2>(hipStream_t, int, int64_t, int, const doub
 template void launch_mupdate<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, float2 *, int64_t,
                                      const double *, int, double *, int, const int32_t *);
 
-// partial layout for k>=1 complex: [s][blk][kMaxK] double2 ; for norms (k == 0): [s][blk] double
-__global__ void k_reduce(int nblk, int k, const double *__restrict__ partial, double *__restrict__ out, int ldo,
-                         double *__restrict__ Hcol, int ldH, const int32_t *__restrict__ active) {
+// partial layout for k>=1 complex: [s][blk][kMaxK] double2 ; for norms (k == 0): [s][blk] double.
+// Workgroup (s, chunk of 64 entries): lane (v, q) sums blocks q, q+4, ... for entry v; the four sums are then
+// added in a fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_reduce(int nblk, int k, const double *__restrict__ partial,
+                                                double *__restrict__ out, int ldo, double *__restrict__ Hcol, int ldH,
+                                                const int32_t *__restrict__ active) {
   const int s = blockIdx.x;
   if (active && !active[s]) return;
+  __shared__ double red[4][64][2];
+  const int vl = threadIdx.x & 63, q = threadIdx.x >> 6;
   if (k == 0) {
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int b = 0; b < nblk; b++) t += partial[(int64_t)s * nblk + b];
-      out[(int64_t)s * ldo] = t;
-    }
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) t += partial[(int64_t)s * nblk + b];
+    t = wave_sum(t);
+    if (vl == 0) red[q][0][0] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) out[(int64_t)s * ldo] = red[0][0][0] + red[1][0][0] + red[2][0][0] + red[3][0][0];
     return;
   }
-  for (int v = threadIdx.x; v < k; v += blockDim.x) {
-    double tx = 0.0, ty = 0.0;
-    for (int b = 0; b < nblk; b++) {
+  const int v = blockIdx.y * 64 + vl;
+  double tx = 0.0, ty = 0.0;
+  if (v < k) {
+#pragma unroll 4
+    for (int b = q; b < nblk; b += 4) {
       const double *p = partial + (((int64_t)s * nblk + b) * kMaxK + v) * 2;
       tx += p[0];
       ty += p[1];
     }
+  }
+  red[q][vl][0] = tx;
+  red[q][vl][1] = ty;
+  __syncthreads();
+  if (q == 0 && v < k) {
+    tx = ((red[0][vl][0] + red[1][vl][0]) + red[2][vl][0]) + red[3][vl][0];
+    ty = ((red[0][vl][1] + red[1][vl][1]) + red[2][vl][1]) + red[3][vl][1];
     out[((int64_t)s * ldo + v) * 2] = tx;
     out[((int64_t)s * ldo + v) * 2 + 1] = ty;
     if (Hcol) {
@@ -815,7 +1107,8 @@ __global__ void k_reduce(int nblk, int k, const double *__restrict__ partial, do
 
 void launch_reduce(hipStream_t st, int S, int nblk, int k, const double *partial, double *out, int ldo,
                    double *Hcol, int ldH, const int32_t *active) {
-  hipLaunchKernelGGL(k_reduce, dim3(S), dim3(128), 0, st, nblk, k, partial, out, ldo, Hcol, ldH, active);
+  const int nchunk = k == 0 ? 1 : (k + 63) / 64;
+  hipLaunchKernelGGL(k_reduce, dim3(S, nchunk), dim3(256), 0, st, nblk, k, partial, out, ldo, Hcol, ldH, active);
   HIP_LAUNCH_CHECK();
 }
 

@@ -25,6 +25,7 @@
 
 #include "../../include/opem.h"
 #include "coarse.h"
+#include "poly.h"
 #include "kernels.h"
 #include "nd_tables.h"
 #include "quadrature.h"
@@ -131,6 +132,7 @@ struct Ctx {
   int coarse_size = 96;                        // 0 disables the coarse level
   int precond = 0;  // 0 vertex patches (falls back to element blocks when a patch exceeds 100 DOFs), 1 element blocks
   int precond_share = 1;  // frequency groups per set of patch inverses
+  int poly_degree = 0;    // GMRES-polynomial preconditioner degree + 1 (0: off)
   int lpr = 16;
 };
 
@@ -539,6 +541,16 @@ static void frequency_coefs(const Ctx &c, const Overlay &ov, double freq, const 
 // Kt = sum_a K_a / mur_a, Mr - j Mi = sum_a er_a (1 - j tand_a) M_a (so c_s = -k0^2, matching
 // frequency_coefs), in FP64 and FP32, and the boundary terms as a CSR over their rows.  With
 // unit_materials every slot counts with mur = er = 1, tand = 0.
+// largest number of CSR entries in 16 consecutive rows (bound of the operator's LDS staging)
+static int blk16_max(const Space &s) {
+  int64_t m = 1;
+  for (int64_t r0 = 0; r0 < s.ndof; r0 += 16) {
+    const int64_t r1 = std::min<int64_t>(r0 + 16, s.ndof);
+    m = std::max<int64_t>(m, s.rowptr[r1] - s.rowptr[r0]);
+  }
+  return (int)m;
+}
+
 struct SweepOperator {
   DevBuf base64, base32, ovptr, ovcol, ovval64, ovval32;
   bool lossy = false;
@@ -549,14 +561,14 @@ struct SweepOperator {
     return SweepMatrix<double>{c.sp.ndof, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
                                base64.as<double>(), base64.as<double>() + nnz,
                                lossy ? base64.as<double>() + 2 * nnz : nullptr, ovptr.as<int32_t>(),
-                               ovcol.as<int32_t>(), ovval64.as<double>(), nov, nterm};
+                               ovcol.as<int32_t>(), ovval64.as<double>(), nov, nterm, blk16_max(c.sp)};
   }
   SweepMatrix<float> m32(const Ctx &c) const {
     const int64_t nnz = c.sp.nnz;
     return SweepMatrix<float>{c.sp.ndof, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
                               base32.as<float>(), base32.as<float>() + nnz,
                               lossy ? base32.as<float>() + 2 * nnz : nullptr, ovptr.as<int32_t>(),
-                              ovcol.as<int32_t>(), ovval32.as<float>(), nov, nterm};
+                              ovcol.as<int32_t>(), ovval32.as<float>(), nov, nterm, blk16_max(c.sp)};
   }
 };
 
@@ -713,7 +725,7 @@ static void device_space(Ctx &c) {
 struct Work {
   // sized per batch
   DevBuf A64, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
-      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch;
+      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch, pinv, pprod, pt, pu;
   // batched operator: frequency-independent matrices (per sweep) and per-system coefficients (per batch)
   SweepMatrix<double> M64{};
   SweepMatrix<float> M32{};
@@ -760,7 +772,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   hipStream_t st = c.st;
   const int64_t n = s.ndof, nnz = s.nnz;
   const int nd = s.nd;
-  const int nblk = reduce_blocks(n);
+  const int nblk = reduce_blocks(n), nbm = mdot_blocks(n);
   const bool mixed = std::is_same<X, float2>::value;
   // preconditioner: vertex-patch additive Schwarz (inverses shared by the excitations of a frequency
   // except on port patches; FP16 with per-block scales inside the FP32 solve), or element-block Jacobi
@@ -825,6 +837,32 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
                                c.dc_row_val.as<double>(), W.yc.as<double>(), z, act);
     }
   };
+  // polynomial preconditioner: the outer Krylov space is built with B p(B), B = A M, p of degree pd - 1 from
+  // the GMRES polynomial of a pd-step Arnoldi run on the first residual (roots per system, Leja order)
+  const int pd = c.poly_degree;
+  if (pd > 0 && pd + 1 > m) throw std::invalid_argument("poly_degree must be below the restart length");
+  if (pd > 0) {
+    W.pprod.alloc((size_t)S * n * sizeof(X));
+    W.pt.alloc((size_t)S * n * sizeof(X));
+    W.pu.alloc((size_t)S * n * sizeof(X));
+  }
+  bool roots_ready = false;
+  auto apply_B = [&](const X *v, int64_t ldv_sys, X *out, const int32_t *act) {
+    precond(v, ldv_sys, W.z.as<X>(), act);
+    sweep_spmv<X>(c, W, S, R, W.z.as<X>(), out, 0, act);
+  };
+  // u = p(B) v  (Loe-Morgan: prod = v; u = sum_i prod_i / theta_i with prod_{i+1} = prod_i - B prod_i / theta_i)
+  int pdeg = pd;  // steps of p(B) after root stabilisation (set by the harvest)
+  auto apply_poly = [&](const X *v, int64_t ldv_sys, X *u, const int32_t *act) {
+    launch_copy_sys<X>(st, S, n, v, ldv_sys, W.pprod.as<X>(), n, act);
+    for (int i = 0; i < pdeg - 1; i++) {
+      apply_B(W.pprod.as<X>(), n, W.pt.as<X>(), act);
+      launch_poly_step<X>(st, S, n, W.pinv.as<double>() + 2 * i, 2 * pdeg, W.pprod.as<X>(), W.pt.as<X>(), u, i == 0,
+                          act);
+    }
+    launch_poly_step<X>(st, S, n, W.pinv.as<double>() + 2 * (pdeg - 1), 2 * pdeg, W.pprod.as<X>(), nullptr, u,
+                        pdeg == 1, act);
+  };
   const int64_t ldv = (int64_t)(m + 1) * n;
   W.V.alloc((size_t)S * ldv * sizeof(X));
   W.w.alloc((size_t)S * n * sizeof(X));
@@ -841,7 +879,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   W.active.alloc((size_t)S * 4);
   W.cyc_active.alloc((size_t)S * 4);
   W.kfinal.alloc((size_t)S * 4);
-  W.partial.alloc((size_t)S * nblk * kMaxBasis * 16);
+  W.partial.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16);
   W.hbuf.alloc((size_t)S * kMaxBasis * 16);
   W.nrm2.alloc((size_t)S * 8);
   std::vector<double> rn2(S), thresh(S);
@@ -888,17 +926,82 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     } else {
       launch_scale<X>(st, S, n, (const X *)W.r.p, n, W.V.as<X>(), ldv, W.nrm2.as<double>(), W.active.as<int32_t>());
     }
+    if (pd > 0 && !roots_ready) {
+      // harvest: pd Arnoldi steps with B from V0, no rotations; raw H columns and subdiagonal per system
+      const int ldH = m * (m + 1);
+      std::vector<double> hsub((size_t)pd * S);
+      for (int jj = 0; jj < pd; jj++) {
+        apply_B(W.V.as<X>() + (int64_t)jj * n, ldv, W.w.as<X>(), W.active.as<int32_t>());
+        double *Hcol = W.Hh.as<double>() + (int64_t)jj * (m + 1) * 2;
+        launch_mdot<X>(st, S, n, jj + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.partial.as<double>(), nbm,
+                       W.active.as<int32_t>());
+        launch_reduce(st, S, nbm, jj + 1, W.partial.as<double>(), W.hbuf.as<double>(), kMaxBasis, Hcol, ldH,
+                      W.active.as<int32_t>());
+        launch_mupdate<X>(st, S, n, jj + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.hbuf.as<double>(), kMaxBasis,
+                          W.partial.as<double>(), nblk, W.active.as<int32_t>());
+        launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0,
+                      W.active.as<int32_t>());
+        HIPCHK(hipMemcpyAsync(hsub.data() + (size_t)jj * S, W.nrm2.p, S * 8, hipMemcpyDeviceToHost, st));
+        launch_scale<X>(st, S, n, W.w.as<X>(), n, W.V.as<X>() + (int64_t)(jj + 1) * n, ldv, W.nrm2.as<double>(),
+                        W.active.as<int32_t>());
+      }
+      std::vector<double> Hh((size_t)S * ldH * 2);
+      HIPCHK(hipMemcpyAsync(Hh.data(), W.Hh.p, Hh.size() * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      // roots per system (Leja order, extra copies of unstable ones), then one coefficient row of D steps per
+      // system: steps before a system's last root use 1/theta_i, the padding steps use 0 (no-op), the final
+      // step 1/theta_last; a system without usable roots gets p = 1
+      std::vector<std::vector<cplx>> roots(S);
+      for (int i = 0; i < S; i++) {
+        std::vector<cplx> Hd((size_t)pd * pd, 0.0), th;
+        for (int jj = 0; jj < pd; jj++)
+          for (int r = 0; r <= std::min(jj + 1, pd - 1); r++) {
+            const size_t e = ((size_t)i * ldH + (size_t)jj * (m + 1) + r) * 2;
+            Hd[(size_t)r * pd + jj] = cplx(Hh[e], Hh[e + 1]);
+          }
+        const double hn = std::sqrt(std::max(0.0, hsub[(size_t)(pd - 1) * S + i]));
+        if (act[i] && gmres_poly_roots(pd, Hd, hn, th)) {
+          poly_add_roots(th);
+          roots[i] = th;
+        }
+      }
+      pdeg = 1;
+      for (int i = 0; i < S; i++) pdeg = std::max(pdeg, (int)roots[i].size());
+      std::vector<double> pinv((size_t)S * pdeg * 2, 0.0);
+      for (int i = 0; i < S; i++) {
+        const int ds = (int)roots[i].size();
+        if (ds == 0) {
+          pinv[((size_t)i * pdeg + pdeg - 1) * 2] = 1.0;
+          continue;
+        }
+        for (int k = 0; k < ds - 1; k++) {
+          const cplx q = 1.0 / roots[i][k];
+          pinv[((size_t)i * pdeg + k) * 2] = q.real();
+          pinv[((size_t)i * pdeg + k) * 2 + 1] = q.imag();
+        }
+        const cplx q = 1.0 / roots[i][ds - 1];
+        pinv[((size_t)i * pdeg + pdeg - 1) * 2] = q.real();
+        pinv[((size_t)i * pdeg + pdeg - 1) * 2 + 1] = q.imag();
+      }
+      W.pinv.upload(pinv, st);
+      roots_ready = true;
+      HIPCHK(hipMemsetAsync(W.Hh.p, 0, (size_t)S * m * (m + 1) * 16, st));
+    }
     int j = 0;
     for (j = 0; j < m; j++) {
       X *Vj = W.V.as<X>() + (int64_t)j * n;
-      precond(Vj, ldv, W.z.as<X>(), W.active.as<int32_t>());
-      sweep_spmv<X>(c, W, S, R, W.z.as<X>(), W.w.as<X>(), 0, W.active.as<int32_t>());
+      if (pd > 0) {
+        apply_poly(Vj, ldv, W.pu.as<X>(), W.active.as<int32_t>());
+        apply_B(W.pu.as<X>(), n, W.w.as<X>(), W.active.as<int32_t>());
+      } else {
+        apply_B(Vj, ldv, W.w.as<X>(), W.active.as<int32_t>());
+      }
       double *Hcol = W.Hh.as<double>() + (int64_t)j * (m + 1) * 2;
       const int ldH = m * (m + 1);
       for (int pass = 0; pass < passes; pass++) {
-        launch_mdot<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.partial.as<double>(), nblk,
+        launch_mdot<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.partial.as<double>(), nbm,
                        W.active.as<int32_t>());
-        launch_reduce(st, S, nblk, j + 1, W.partial.as<double>(), W.hbuf.as<double>(), kMaxBasis, Hcol, ldH,
+        launch_reduce(st, S, nbm, j + 1, W.partial.as<double>(), W.hbuf.as<double>(), kMaxBasis, Hcol, ldH,
                       W.active.as<int32_t>());
         launch_mupdate<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), n, W.hbuf.as<double>(), kMaxBasis,
                           pass == passes - 1 ? W.partial.as<double>() : nullptr, nblk, W.active.as<int32_t>());
@@ -924,7 +1027,12 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
                      W.cyc_active.as<int32_t>());
     launch_combine_basis<X>(st, S, n, m, W.V.as<X>(), ldv, W.y.as<double>(), W.kfinal.as<int32_t>(), W.u.as<X>(),
                             W.cyc_active.as<int32_t>());
-    precond(W.u.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>());
+    if (pd > 0) {
+      apply_poly(W.u.as<X>(), n, W.pu.as<X>(), W.cyc_active.as<int32_t>());
+      precond(W.pu.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>());
+    } else {
+      precond(W.u.as<X>(), n, W.z.as<X>(), W.cyc_active.as<int32_t>());
+    }
     launch_axpy_il<double2, X>(st, S, n, W.z.as<X>(), W.xil.as<double2>(), W.cyc_active.as<int32_t>());
     HIPCHK(hipMemcpyAsync(kf.data(), W.kfinal.p, S * 4, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> ca(S);
@@ -942,7 +1050,7 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   hipStream_t st = c.st;
   const int64_t n = s.ndof, nnz = s.nnz;
   const int nd = s.nd;
-  const int nblk = reduce_blocks(n);
+  const int nblk = reduce_blocks(n), nbm = mdot_blocks(n);
   DevBuf b, r, z, p, q, rz, t, al, nal, act, part, part2;
   b.alloc((size_t)S * n * 16);
   r.alloc((size_t)S * n * 16);
@@ -955,8 +1063,26 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   al.alloc(S * 16);
   nal.alloc(S * 16);
   act.alloc(S * 4);
-  part.alloc((size_t)S * nblk * kMaxBasis * 16);
+  part.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16);
   part2.alloc((size_t)S * 16);
+  // P through the batched operator: Kt = sum of unit masses, zero second matrix and coefficients, no boundary
+  // terms, every row free; p is transposed to the system-interleaved layout before each product
+  DevBuf pbase, pcs, pil;
+  {
+    const int nslot = (int)c.slot_attr.size();
+    std::vector<double> wp((size_t)nslot * 3, 0.0);
+    for (int a = 0; a < nslot; a++) wp[a * 3] = 1.0;
+    DevBuf dwp;
+    dwp.upload(wp, st);
+    pbase.alloc((size_t)2 * nnz * 8);
+    launch_base_matrices(st, nnz, nslot, c.d_M.as<double>(), c.d_M.as<double>(), dwp.as<double>(),
+                         pbase.as<double>(), nullptr, 0);
+    pcs.upload(std::vector<double>((size_t)S * 2, 0.0), st);
+    pil.alloc((size_t)S * n * 16);
+  }
+  const SweepMatrix<double> MP{n, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), pbase.as<double>(),
+                               pbase.as<double>() + nnz, nullptr, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
+                               pbase.as<double>(), 0, 0, blk16_max(s)};
   // b = Qs x, then scale by j/w per system
   launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), c.d_Qs.as<double2>(),
                                 nnz, c.d_onesmask.as<uint8_t>(), W.x.as<double2>(), b.as<double2>(), nullptr, nullptr,
@@ -979,8 +1105,8 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(),
                      r.as<double2>(), z.as<double2>(), nullptr);
   HIPCHK(hipMemcpyAsync(p.p, z.p, (size_t)S * n * 16, hipMemcpyDeviceToDevice, st));
-  launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nblk, nullptr);
-  launch_reduce(st, S, nblk, 1, part.as<double>(), rz.as<double>(), 1, nullptr, 0, nullptr);
+  launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nbm, nullptr);
+  launch_reduce(st, S, nbm, 1, part.as<double>(), rz.as<double>(), 1, nullptr, 0, nullptr);
   // ||b||
   launch_norm2<double2>(st, S, n, b.as<double2>(), part.as<double>(), nblk, nullptr);
   launch_reduce(st, S, nblk, 0, part.as<double>(), part2.as<double>(), 1, nullptr, 0, nullptr);
@@ -991,12 +1117,13 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   for (int i = 0; i < S; i++) a[i] = bn2[i] > 0;
   HIPCHK(hipMemcpyAsync(act.p, a.data(), S * 4, hipMemcpyHostToDevice, st));
   for (int it = 0; it < 20000; it++) {
-    launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
-                                  c.d_P.as<double2>(), nnz, c.d_onesmask.as<uint8_t>(), p.as<double2>(),
-                                  q.as<double2>(), nullptr, act.as<int32_t>(), 0, c.lpr, 0);
-    launch_mdot<double2>(st, S, n, 1, p.as<double2>(), n, q.as<double2>(), n, part.as<double>(), nblk,
+    launch_transpose<double2>(st, n, S, p.as<double2>(), pil.as<double2>(), true);
+    launch_spmv_sweep<double, double2>(st, MP, S, 1, pcs.as<double>(), pcs.as<double>(),
+                                       c.d_onesmask.as<uint8_t>(), pil.as<double2>(), q.as<double2>(), 0,
+                                       act.as<int32_t>(), c.lpr);
+    launch_mdot<double2>(st, S, n, 1, p.as<double2>(), n, q.as<double2>(), n, part.as<double>(), nbm,
                          act.as<int32_t>());
-    launch_reduce(st, S, nblk, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
+    launch_reduce(st, S, nbm, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
     hipLaunchKernelGGL(k_cg_scalars, dim3((S + 63) / 64), dim3(64), 0, st, S, 0, rz.as<double2>(), t.as<double2>(),
                        al.as<double2>(), nal.as<double2>(), act.as<int32_t>());
     hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, al.as<double2>(), p.as<double2>(),
@@ -1018,9 +1145,9 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
     }
     hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(),
                        r.as<double2>(), z.as<double2>(), act.as<int32_t>());
-    launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nblk,
+    launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nbm,
                          act.as<int32_t>());
-    launch_reduce(st, S, nblk, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
+    launch_reduce(st, S, nbm, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
     hipLaunchKernelGGL(k_cg_scalars, dim3((S + 63) / 64), dim3(64), 0, st, S, 1, rz.as<double2>(), t.as<double2>(),
                        al.as<double2>(), nal.as<double2>(), act.as<int32_t>());
     hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, nullptr, z.as<double2>(),
@@ -1366,7 +1493,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     {
       const int nblk = reduce_blocks(n);
       W.r.alloc((size_t)S * n * 16);
-      W.partial.alloc((size_t)S * nblk * kMaxBasis * 16);
+      W.partial.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16);
       W.nrm2.alloc((size_t)S * 8);
       W.xil.alloc((size_t)S * n * 16);
       launch_transpose<double2>(st, n, S, W.x.as<double2>(), W.xil.as<double2>(), true);
@@ -1754,6 +1881,8 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
       throw std::invalid_argument("precond_share x nports must be <= 8");
     ctx->precond = in->precond;
     ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
+    if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");
+    ctx->poly_degree = in->poly_degree;
     if (ctx->coarse_size != in->coarse_size) ctx->have_coarse = false;
     ctx->coarse_size = in->coarse_size;
     sweep(*ctx, in, out);
@@ -1874,13 +2003,13 @@ int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *a
       throw std::invalid_argument("bad krylov bench arguments");
     hipStream_t st = ctx->st;
     const int64_t n = ctx->sp.ndof;
-    const int nblk = reduce_blocks(n);
+    const int nblk = reduce_blocks(n), nbm = mdot_blocks(n);
     const size_t sv = prec == 0 ? 16 : 8;
     const int64_t ldv = (int64_t)(k + 1) * n;
     DevBuf V, w, part, h;
     V.alloc((size_t)S * ldv * sv);
     w.alloc((size_t)S * n * sv);
-    part.alloc((size_t)S * nblk * kMaxBasis * 16);
+    part.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16);
     h.alloc((size_t)S * kMaxBasis * 16);
     if (prec == 0) {
       hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * ldv, V.as<double2>(), 1e-3);
@@ -1892,9 +2021,9 @@ int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *a
     HIPCHK(hipMemsetAsync(h.p, 0, (size_t)S * kMaxBasis * 16, st));
     auto mdot = [&]() {
       if (prec == 0)
-        launch_mdot<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, part.as<double>(), nblk, nullptr);
+        launch_mdot<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, part.as<double>(), nbm, nullptr);
       else
-        launch_mdot<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, part.as<double>(), nblk, nullptr);
+        launch_mdot<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, part.as<double>(), nbm, nullptr);
     };
     auto mupd = [&]() {
       if (prec == 0)
@@ -1927,7 +2056,7 @@ int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *a
       avg_seconds[1] = m2 * 1e-3 / nrep;
     }
     if (alg_bytes) {
-      alg_bytes[0] = (double)S * (k + 1) * n * sv + (double)S * nblk * k * 16;
+      alg_bytes[0] = (double)S * (k + 1) * n * sv + (double)S * nbm * k * 16;
       alg_bytes[1] = (double)S * (k + 2) * n * sv + (double)S * k * 16 + (double)S * nblk * 8;
     }
   });

@@ -98,13 +98,14 @@ def test_system_matrix_microstrip(p):
     assert d <= 1e-11 * abs(ref).max(), d
 
 
-@pytest.mark.parametrize("nf", [3, 12, 70])
+@pytest.mark.parametrize("nf", [3, 12, 13, 70])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
 def test_sweep_operator(nf, prec, tol):
     """The batched operator of the solve (one pass of the frequency-independent matrices for every frequency:
     lossy substrate, impedance trace, radiation lid, two port ABCs) against A(f) x from the assembled values
-    (themselves pinned to the oracle above).  nf = 3 takes the narrow form, 12 and 70 the wide one; the
-    error is measured against sum_j |A_ij| |x_j| per row."""
+    (themselves pinned to the oracle above).  nf = 3 takes the narrow form, 13 the one-system-per-lane tiles,
+    12 and 70 the paired tiles (70: a partial last tile); the error is measured against sum_j |A_ij| |x_j|
+    per row."""
     m = microstrip_mesh(nx=6, ny=4, nz=4, n_sub=1, seed=2)
     fr = np.linspace(2e9, 9e9, nf)
     gams = [{2: 0.3 + 180j * f / 5e9, 3: 0.2 + 150j * f / 5e9} for f in fr]
@@ -263,6 +264,20 @@ def test_batched_sweep_shared_patch_inverses(nine_point_fp64, share, prec):
     s = _solver(m, 2)
     try:
         rb = s.sweep(f, precision=prec, tol=1e-10, batch=9, orth_passes=1, precond_share=share)
+    finally:
+        s.close()
+    assert rb["converged"].all()
+    assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("deg,prec", [(3, "mixed"), (4, "mixed"), (5, "fp64")])
+def test_polynomial_preconditioner(nine_point_fp64, deg, prec):
+    """GMRES-polynomial preconditioning (Krylov space of B p(B), roots from a deg-step Arnoldi run per system):
+    the batched nine-frequency sweep reaches the same S as one-at-a-time FP64 without it."""
+    m, f, r1 = nine_point_fp64
+    s = _solver(m, 2)
+    try:
+        rb = s.sweep(f, precision=prec, tol=1e-10, batch=9, orth_passes=1, precond_share=2, poly_degree=deg)
     finally:
         s.close()
     assert rb["converged"].all()
