@@ -25,6 +25,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _pmc_traffic_ratio(kernel, shape):
+    """HBM traffic / algorithmic bytes of a kernel at a shape, as measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes and recorded in profiles/roofline_pmc.md ("traffic_ratio <kernel> <shape> <ratio>"); None if absent."""
+    path = os.path.join(ROOT, "profiles", "roofline_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_ratio" and f[1] == kernel and f[2] == shape:
+            return float(f[3])
+    return None
+
+
 def _cpu_baseline(work, its_per_system, restart):
     """Oracle (numpy/scipy FP64, one thread) on one frequency of the same workload, bounded in time.
 
@@ -120,6 +133,7 @@ def main():
     ap.add_argument("--coarse", type=int, default=0)
     ap.add_argument("--share", type=int, default=4, help="adjacent frequencies sharing one set of patch inverses")
     ap.add_argument("--poly", type=int, default=0, help="GMRES-polynomial preconditioner degree (0: off)")
+    ap.add_argument("--arnoldi-step", type=int, default=2, help="Krylov vectors per Gram-Schmidt pass (1 or 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the systems of a sweep are "
@@ -138,11 +152,11 @@ def main():
 
     for _ in range(args.warmup):
         s.sweep(freqs, precision="mixed", batch=len(freqs), orth_passes=1, precond_share=args.share,
-                poly_degree=args.poly, **opts)
+                poly_degree=args.poly, arnoldi_step=args.arnoldi_step, **opts)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = s.sweep(freqs, precision="mixed", batch=len(freqs), orth_passes=1, precond_share=args.share,
-                    poly_degree=args.poly, **opts)
+                    poly_degree=args.poly, arnoldi_step=args.arnoldi_step, **opts)
     t1 = time.perf_counter()
     elapsed = t1 - t0
     done = int(np.all(r["converged"].astype(bool), axis=1).sum())
@@ -163,6 +177,7 @@ def main():
     kmean = args.restart // 2 + 1
     (t_mdot, t_mupd), (b_mdot, b_mupd) = s.krylov_bench("fp32", S=S, k=kmean, nrep=5)
     t_spmv, b_spmv = s.spmv_bench("fp32", ngroups=len(freqs), R=nports, nrep=10)
+    ratio_mdot = _pmc_traffic_ratio("k_mdot", f"S={S},k={kmean},n={s.ndof}")
     its_mean = float(r["iters"].mean())
     s.close()
 
@@ -191,7 +206,8 @@ def main():
                         f"{s.ndof} DOF, nnz {s.nnz}",
             "order": work["order"], "ndof": s.ndof, "nnz": s.nnz, "systems": len(freqs) * nports,
             "precond": f"vertex-patch additive Schwarz (one inverse set per {args.share} adjacent frequencies)"
-                       + (f", GMRES polynomial degree {args.poly}" if args.poly else "") + f", GMRES({args.restart})",
+                       + (f", GMRES polynomial degree {args.poly}" if args.poly else "") + f", GMRES({args.restart})"
+                       + (", two Krylov vectors per Gram-Schmidt pass" if args.arnoldi_step == 2 else ""),
         },
         "points_converged": done,
         "iters_mean": its_mean,
@@ -203,7 +219,8 @@ def main():
         "roofline": {"kernel": f"k_mdot (Gram-Schmidt projection h = V^H w, FP32 basis, {S} systems x "
                                f"{kmean} vectors)", "bound": "hbm", "achieved": b_mdot / t_mdot / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_mdot / t_mdot / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "alg_bytes_per_launch": b_mdot, "launch_seconds": t_mdot},
+                     "traffic": (None if ratio_mdot is None else ratio_mdot * b_mdot),
+                     "alg_bytes_per_launch": b_mdot, "launch_seconds": t_mdot},
         "kernels": {
             "k_mupdate": {"GB/s": b_mupd / t_mupd / 1e9, "alg_bytes": b_mupd, "ms": t_mupd * 1e3},
             "k_spmv_sweep_w (batched operator, one pass of Kt/M for all systems)": {

@@ -139,6 +139,9 @@ typedef struct {
   int poly_degree;              /* d > 0: GMRES-polynomial preconditioning, the Krylov space built with  */
                                 /*    B p(B) (B = A M, deg p = d - 1, roots from a d-step Arnoldi run per */
                                 /*    system); each iteration applies B d times (0 = off, <= 32)       */
+  int arnoldi_step;             /* 2: two Krylov vectors per Gram-Schmidt pass (z1 = B v, z2 = B z1,    */
+                                /*    block projection, Cholesky QR of the pair); 0 or 1: one; with    */
+                                /*    poly_degree > 0 one is used                                      */
 } opem_sweep_in;
 
 typedef struct {

@@ -105,6 +105,29 @@ void launch_mupdate(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_
 // out[s][i] = sum_blk partial[s][blk][i]  (fixed order) ; if acc: Hcol[s][i] += out
 void launch_reduce(hipStream_t st, int S, int nblk, int k, const double *partial, double *out, int ldo,
                    double *Hcol, int ldH, const int32_t *active);
+// as launch_reduce with ldp complex entries per partial block (launch_reduce: ldp = kMaxBasis)
+void launch_reduce_ld(hipStream_t st, int S, int nblk, int k, int ldp, const double *partial, double *out, int ldo,
+                      double *Hcol, int ldH, const int32_t *active);
+
+// ---------------------------------------------------------------- two-step (block) Arnoldi
+// partial[s][blk][v][c] = conj(V[s][v]) . z_c[s] over a chunk, c = 0, 1 (one pass over V; nblk = mdot_blocks(n));
+// reduce with launch_reduce_ld(k = 2k, ldp = 2*kMaxBasis) into C[s][v][c].
+template <typename X>
+void launch_mdot2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *z1, const X *z2,
+                  int64_t ldz_sys, double *partial, int nblk, const int32_t *active);
+// z_c -= sum_v V[v] C[v][c] (C row stride ldc pairs), Gram partials gram[s][blk][4] of the result (nblk = reduce_blocks)
+template <typename X>
+void launch_mupdate2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *z1, X *z2,
+                     int64_t ldz_sys, const double *C, int ldc, double *gram, int nblk, const int32_t *active);
+void launch_reduce4(hipStream_t st, int S, int nblk, const double *gram, double *out, const int32_t *active);
+// Cholesky QR of the pair: q1, q2 (written with stride ldq_sys) and R[s] = (r11, Re r12, Im r12, r22)
+template <typename X>
+void launch_cholqr2(hipStream_t st, int S, int64_t n, const double *gram, const X *z1, const X *z2, int64_t ldz_sys,
+                    X *q1, X *q2, int64_t ldq_sys, double *R, const int32_t *active);
+// raw Hessenberg columns j, j+1 into Hraw and Hw, and the Givens subdiagonals nrm2a, nrm2b
+void launch_hcols2(hipStream_t st, int S, int m, int j, const double *C, int ldc, const double *R, double *Hraw,
+                   double *Hw, double *nrm2a, double *nrm2b, const int32_t *active);
+
 // v[s] = w[s] / nrm[s]  ; nrm from nrm2[s] (sqrt taken inside), also stored into H[s][(k+1)*ldH? ]
 template <typename X>
 void launch_scale(hipStream_t st, int S, int64_t n, const X *w, int64_t ldw_sys, X *v, int64_t ldv_sys,

@@ -1061,10 +1061,299 @@ template void launch_mupdate<double2>(hipStream_t, int, int64_t, int, const doub
 template void launch_mupdate<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, float2 *, int64_t,
                                      const double *, int, double *, int, const int32_t *);
 
+// ------------------------------------------------------------------ two-step (block) Arnoldi
+// Projection of two new vectors against the basis in one pass over V:
+//   partial[s][blk][v][c] = sum_{chunk} conj(V[s][v][i]) z_c[s][i], c = 0, 1 (complex), one partial per wavefront;
+// two basis vectors per pass, their four complex sums finished by the same reduce-scatter butterfly as k_mdot.
+template <typename X, int E>
+__global__ __launch_bounds__(kRedThreads) void k_mdot2(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
+                                                       const X *__restrict__ z1, const X *__restrict__ z2,
+                                                       int64_t ldz_sys, double *__restrict__ partial, int nblk,
+                                                       const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  const int64_t base = (int64_t)blockIdx.x * (kRedThreads * E) + threadIdx.x;
+  const X *Vs = Vb + s * ldv_sys;
+  X w1[E], w2[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int64_t i = base + e * kRedThreads;
+    X zero;
+    zero.x = 0;
+    zero.y = 0;
+    w1[e] = i < n ? z1[s * ldz_sys + i] : zero;
+    w2[e] = i < n ? z2[s * ldz_sys + i] : zero;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
+  for (int v0 = 0; v0 < k; v0 += 2) {
+    double ax[4], ay[4];  // q = 2*(v - v0) + c
+#pragma unroll
+    for (int q = 0; q < 4; q++) ax[q] = ay[q] = 0.0;
+#pragma unroll
+    for (int dv = 0; dv < 2; dv++) {
+      if (v0 + dv < k) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int64_t i = base + e * kRedThreads;
+          if (i < n) {
+            const double2 a = to_d2(Vs[(int64_t)(v0 + dv) * n + i]);
+            const double2 b1 = to_d2(w1[e]), b2 = to_d2(w2[e]);
+            ax[2 * dv] += a.x * b1.x + a.y * b1.y;
+            ay[2 * dv] += a.x * b1.y - a.y * b1.x;
+            ax[2 * dv + 1] += a.x * b2.x + a.y * b2.y;
+            ay[2 * dv + 1] += a.x * b2.y - a.y * b2.x;
+          }
+        }
+      }
+    }
+    double bx[2], by[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const double kx = h32 ? ax[t + 2] : ax[t], ky = h32 ? ay[t + 2] : ay[t];
+      const double sx = h32 ? ax[t] : ax[t + 2], sy = h32 ? ay[t] : ay[t + 2];
+      bx[t] = kx + __shfl_xor(sx, 32, 64);
+      by[t] = ky + __shfl_xor(sy, 32, 64);
+    }
+    double cx = (h16 ? bx[1] : bx[0]) + __shfl_xor(h16 ? bx[0] : bx[1], 16, 64);
+    double cy = (h16 ? by[1] : by[0]) + __shfl_xor(h16 ? by[0] : by[1], 16, 64);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      cx += __shfl_xor(cx, o, 64);
+      cy += __shfl_xor(cy, o, 64);
+    }
+    const int q = (h32 ? 2 : 0) + (h16 ? 1 : 0);
+    const int v = v0 + (q >> 1), c = q & 1;
+    if ((lane & 15) == 0 && v < k) {
+      double *p = partial + ((((int64_t)s * nblk + blockIdx.x * (kRedThreads / 64) + wave) * kMaxK + v) * 2 + c) * 2;
+      p[0] = cx;
+      p[1] = cy;
+    }
+  }
+}
+
+template <typename X>
+void launch_mdot2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *z1, const X *z2,
+                  int64_t ldz_sys, double *partial, int nblk, const int32_t *active) {
+  if (k > kMaxK) throw std::runtime_error("restart too large");
+  if (nblk != mdot_blocks(n)) throw std::runtime_error("mdot2: partial block count mismatch");
+  hipLaunchKernelGGL((k_mdot2<X, kMdotElems>), dim3(mdot_grid(n), S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys,
+                     z1, z2, ldz_sys, partial, nblk, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_mdot2<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, const double2 *,
+                                    const double2 *, int64_t, double *, int, const int32_t *);
+template void launch_mdot2<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, const float2 *,
+                                   const float2 *, int64_t, double *, int, const int32_t *);
+
+// z_c -= sum_v V[v] C[v][c] (c = 0, 1; C[s][v][c] complex with row stride ldc entries), then Gram partials
+// gram[s][blk][4] = (|z1|^2, |z2|^2, Re z1^H z2, Im z1^H z2) of the updated pair.
+template <typename X>
+__global__ __launch_bounds__(kRedThreads) void k_mupdate2(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
+                                                          X *__restrict__ z1, X *__restrict__ z2, int64_t ldz_sys,
+                                                          const double *__restrict__ C, int ldc,
+                                                          double *__restrict__ gram, int nblk,
+                                                          const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  __shared__ double2 hs[2][kMaxK];
+  __shared__ double red[4][kRedThreads / 64];
+  for (int v = threadIdx.x; v < k; v += kRedThreads) {
+    const double *cv = C + ((int64_t)s * ldc + v) * 4;
+    hs[0][v] = make_double2(cv[0], cv[1]);
+    hs[1][v] = make_double2(cv[2], cv[3]);
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  X *a1 = z1 + s * ldz_sys;
+  X *a2 = z2 + s * ldz_sys;
+  const X *Vs = Vb + s * ldv_sys;
+  double g11 = 0.0, g22 = 0.0, g12x = 0.0, g12y = 0.0;
+#pragma unroll
+  for (int e = 0; e < kRedElems; e++) {
+    const int64_t i = base + e * kRedThreads;
+    if (i < n) {
+      double2 p = to_d2(a1[i]), q = to_d2(a2[i]);
+      for (int v = 0; v < k; v++) {
+        const double2 a = to_d2(Vs[(int64_t)v * n + i]);
+        const double2 c1 = cmul(a, hs[0][v]), c2 = cmul(a, hs[1][v]);
+        p.x -= c1.x;
+        p.y -= c1.y;
+        q.x -= c2.x;
+        q.y -= c2.y;
+      }
+      const X po = from_d2<X>(p), qo = from_d2<X>(q);
+      a1[i] = po;
+      a2[i] = qo;
+      const double2 pp = to_d2(po), qq = to_d2(qo);
+      g11 += pp.x * pp.x + pp.y * pp.y;
+      g22 += qq.x * qq.x + qq.y * qq.y;
+      g12x += pp.x * qq.x + pp.y * qq.y;
+      g12y += pp.x * qq.y - pp.y * qq.x;
+    }
+  }
+  g11 = wave_sum(g11);
+  g22 = wave_sum(g22);
+  g12x = wave_sum(g12x);
+  g12y = wave_sum(g12y);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = g11;
+    red[1][threadIdx.x >> 6] = g22;
+    red[2][threadIdx.x >> 6] = g12x;
+    red[3][threadIdx.x >> 6] = g12y;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int q = 0; q < kRedThreads / 64; q++) t += red[threadIdx.x][q];
+    gram[((int64_t)s * nblk + blockIdx.x) * 4 + threadIdx.x] = t;
+  }
+}
+
+template <typename X>
+void launch_mupdate2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *z1, X *z2,
+                     int64_t ldz_sys, const double *C, int ldc, double *gram, int nblk, const int32_t *active) {
+  hipLaunchKernelGGL((k_mupdate2<X>), dim3(nblk, S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, z1, z2, ldz_sys, C,
+                     ldc, gram, nblk, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_mupdate2<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, double2 *,
+                                       double2 *, int64_t, const double *, int, double *, int, const int32_t *);
+template void launch_mupdate2<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, float2 *, float2 *,
+                                      int64_t, const double *, int, double *, int, const int32_t *);
+
+// out[s][0..3] = sum_blk gram[s][blk][0..3] (fixed order)
+__global__ void k_reduce4(int nblk, const double *__restrict__ gram, double *__restrict__ out,
+                          const int32_t *__restrict__ active) {
+  const int s = blockIdx.x;
+  if (active && !active[s]) return;
+  __shared__ double red[4][64];
+  const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double t = 0.0;
+  for (int b = l; b < nblk; b += 64) t += gram[((int64_t)s * nblk + b) * 4 + c];
+  t = wave_sum(t);
+  if (l == 0) out[(int64_t)s * 4 + c] = t;
+  (void)red;
+}
+
+void launch_reduce4(hipStream_t st, int S, int nblk, const double *gram, double *out, const int32_t *active) {
+  hipLaunchKernelGGL(k_reduce4, dim3(S), dim3(256), 0, st, nblk, gram, out, active);
+  HIP_LAUNCH_CHECK();
+}
+
+// Cholesky QR of the projected pair from its Gram matrix G = [z1 z2]^H [z1 z2]:
+//   r11 = sqrt(G11), r12 = G12 / r11, r22 = sqrt(G22 - |r12|^2);  q1 = z1 / r11 -> V[j+1], q2 = (z2 - q1 r12) / r22 -> V[j+2]
+// R[s] = (r11, Re r12, Im r12, r22).  A vanishing r11 or r22 (relative to the Gram entries) counts as zero, and the
+// matching q as the zero vector (the Givens step then ends the system).
+__device__ __forceinline__ void chol2(const double *g, double &r11, double2 &r12, double &r22) {
+  r11 = g[0] > 0.0 ? sqrt(g[0]) : 0.0;
+  r12 = r11 > 0.0 ? make_double2(g[2] / r11, g[3] / r11) : make_double2(0.0, 0.0);
+  const double d = g[1] - (r12.x * r12.x + r12.y * r12.y);
+  r22 = d > 1e-12 * g[1] ? sqrt(d) : 0.0;
+}
+
+template <typename X>
+__global__ void k_cholqr2(int64_t n, const double *__restrict__ gram, const X *__restrict__ z1,
+                          const X *__restrict__ z2, int64_t ldz_sys, X *__restrict__ q1, X *__restrict__ q2,
+                          int64_t ldq_sys, double *__restrict__ R, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y;
+  if (active && !active[s]) return;
+  double r11, r22;
+  double2 r12;
+  chol2(gram + (int64_t)s * 4, r11, r12, r22);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    R[(int64_t)s * 4] = r11;
+    R[(int64_t)s * 4 + 1] = r12.x;
+    R[(int64_t)s * 4 + 2] = r12.y;
+    R[(int64_t)s * 4 + 3] = r22;
+  }
+  const double i11 = r11 > 0.0 ? 1.0 / r11 : 0.0, i22 = r22 > 0.0 ? 1.0 / r22 : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = to_d2(z1[s * ldz_sys + i]), b = to_d2(z2[s * ldz_sys + i]);
+    const double2 u = make_double2(a.x * i11, a.y * i11);
+    const double2 ur = cmul(u, r12);
+    q1[s * ldq_sys + i] = from_d2<X>(u);
+    q2[s * ldq_sys + i] = from_d2<X>(make_double2((b.x - ur.x) * i22, (b.y - ur.y) * i22));
+  }
+}
+
+template <typename X>
+void launch_cholqr2(hipStream_t st, int S, int64_t n, const double *gram, const X *z1, const X *z2, int64_t ldz_sys,
+                    X *q1, X *q2, int64_t ldq_sys, double *R, const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  hipLaunchKernelGGL((k_cholqr2<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, gram, z1, z2, ldz_sys, q1, q2,
+                     ldq_sys, R, active);
+  HIP_LAUNCH_CHECK();
+}
+template void launch_cholqr2<double2>(hipStream_t, int, int64_t, const double *, const double2 *, const double2 *,
+                                      int64_t, double2 *, double2 *, int64_t, double *, const int32_t *);
+template void launch_cholqr2<float2>(hipStream_t, int, int64_t, const double *, const float2 *, const float2 *,
+                                     int64_t, float2 *, float2 *, int64_t, double *, const int32_t *);
+
+// Raw Hessenberg columns j and j+1 of the two-step relation (B v_j = z1, B z1 = z2):
+//   col j:   H[0..j] = C0, H[j+1] = r11
+//   col j+1: H[l] = (C1[l] - y[l] - C0[j] C0[l]) / r11 (l <= j), H[j+1] = r12 / r11 - C0[j], H[j+2] = r22 / r11,
+//   y = Hraw[0..j, 0..j-1] C0[0..j-1].  Both columns go to Hraw and to the working H (rotated later);
+//   nrm2a = r11^2, nrm2b = (r22 / r11)^2 feed the Givens steps.
+__global__ void k_hcols2(int m, int j, const double *__restrict__ C, int ldc, const double *__restrict__ R,
+                         double2 *__restrict__ Hraw, double2 *__restrict__ Hw, double *__restrict__ nrm2a,
+                         double *__restrict__ nrm2b, const int32_t *__restrict__ active) {
+  const int s = blockIdx.x;
+  if (active && !active[s]) return;
+  const double *Cs = C + (int64_t)s * ldc * 4;
+  auto c0 = [&](int v) { return make_double2(Cs[v * 4], Cs[v * 4 + 1]); };
+  auto c1 = [&](int v) { return make_double2(Cs[v * 4 + 2], Cs[v * 4 + 3]); };
+  double2 *Hr = Hraw + (int64_t)s * m * (m + 1);
+  double2 *Hs = Hw + (int64_t)s * m * (m + 1);
+  const double r11 = R[(int64_t)s * 4], r22 = R[(int64_t)s * 4 + 3];
+  const double2 r12 = make_double2(R[(int64_t)s * 4 + 1], R[(int64_t)s * 4 + 2]);
+  const double i11 = r11 > 0.0 ? 1.0 / r11 : 0.0;
+  const double2 cj = c0(j);
+  for (int l = threadIdx.x; l <= j + 2; l += blockDim.x) {
+    double2 hj = make_double2(0.0, 0.0), h1 = make_double2(0.0, 0.0);
+    if (l <= j) {
+      hj = c0(l);
+      double2 y = make_double2(0.0, 0.0);
+      for (int c = (l > 0 ? l - 1 : 0); c < j; c++) {
+        const double2 t = cmul(Hr[(int64_t)c * (m + 1) + l], c0(c));
+        y.x += t.x;
+        y.y += t.y;
+      }
+      const double2 cc = cmul(cj, c0(l)), d = c1(l);
+      h1 = make_double2((d.x - y.x - cc.x) * i11, (d.y - y.y - cc.y) * i11);
+    } else if (l == j + 1) {
+      hj = make_double2(r11, 0.0);
+      h1 = make_double2(r12.x * i11 - cj.x, r12.y * i11 - cj.y);
+    } else {
+      h1 = make_double2(r22 * i11, 0.0);
+    }
+    if (l <= j + 1) {
+      Hr[(int64_t)j * (m + 1) + l] = hj;
+      Hs[(int64_t)j * (m + 1) + l] = hj;
+    }
+    if (j + 1 < m) {
+      Hr[(int64_t)(j + 1) * (m + 1) + l] = h1;
+      Hs[(int64_t)(j + 1) * (m + 1) + l] = h1;
+    }
+  }
+  if (threadIdx.x == 0) {
+    nrm2a[s] = r11 * r11;
+    nrm2b[s] = r22 * i11 * r22 * i11;
+  }
+}
+
+void launch_hcols2(hipStream_t st, int S, int m, int j, const double *C, int ldc, const double *R, double *Hraw,
+                   double *Hw, double *nrm2a, double *nrm2b, const int32_t *active) {
+  hipLaunchKernelGGL(k_hcols2, dim3(S), dim3(256), 0, st, m, j, C, ldc, R, (double2 *)Hraw, (double2 *)Hw, nrm2a,
+                     nrm2b, active);
+  HIP_LAUNCH_CHECK();
+}
+
 // partial layout for k>=1 complex: [s][blk][kMaxK] double2 ; for norms (k == 0): [s][blk] double.
 // Workgroup (s, chunk of 64 entries): lane (v, q) sums blocks q, q+4, ... for entry v; the four sums are then
 // added in a fixed order, so the result does not depend on scheduling.
-__global__ __launch_bounds__(256) void k_reduce(int nblk, int k, const double *__restrict__ partial,
+__global__ __launch_bounds__(256) void k_reduce(int nblk, int k, int ldp, const double *__restrict__ partial,
                                                 double *__restrict__ out, int ldo, double *__restrict__ Hcol, int ldH,
                                                 const int32_t *__restrict__ active) {
   const int s = blockIdx.x;
@@ -1085,7 +1374,7 @@ __global__ __launch_bounds__(256) void k_reduce(int nblk, int k, const double *_
   if (v < k) {
 #pragma unroll 4
     for (int b = q; b < nblk; b += 4) {
-      const double *p = partial + (((int64_t)s * nblk + b) * kMaxK + v) * 2;
+      const double *p = partial + (((int64_t)s * nblk + b) * ldp + v) * 2;
       tx += p[0];
       ty += p[1];
     }
@@ -1105,11 +1394,17 @@ __global__ __launch_bounds__(256) void k_reduce(int nblk, int k, const double *_
   }
 }
 
+void launch_reduce_ld(hipStream_t st, int S, int nblk, int k, int ldp, const double *partial, double *out, int ldo,
+                      double *Hcol, int ldH, const int32_t *active) {
+  const int nchunk = k == 0 ? 1 : (k + 63) / 64;
+  hipLaunchKernelGGL(k_reduce, dim3(S, nchunk), dim3(256), 0, st, nblk, k, ldp, partial, out, ldo, Hcol, ldH,
+                     active);
+  HIP_LAUNCH_CHECK();
+}
+
 void launch_reduce(hipStream_t st, int S, int nblk, int k, const double *partial, double *out, int ldo,
                    double *Hcol, int ldH, const int32_t *active) {
-  const int nchunk = k == 0 ? 1 : (k + 63) / 64;
-  hipLaunchKernelGGL(k_reduce, dim3(S, nchunk), dim3(256), 0, st, nblk, k, partial, out, ldo, Hcol, ldH, active);
-  HIP_LAUNCH_CHECK();
+  launch_reduce_ld(st, S, nblk, k, kMaxK, partial, out, ldo, Hcol, ldH, active);
 }
 
 template <typename X>

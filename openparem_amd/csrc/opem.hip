@@ -133,6 +133,7 @@ struct Ctx {
   int precond = 0;  // 0 vertex patches (falls back to element blocks when a patch exceeds 100 DOFs), 1 element blocks
   int precond_share = 1;  // frequency groups per set of patch inverses
   int poly_degree = 0;    // GMRES-polynomial preconditioner degree + 1 (0: off)
+  int arnoldi_step = 1;   // 2: two new Krylov vectors per orthogonalisation pass
   int lpr = 16;
 };
 
@@ -725,7 +726,8 @@ static void device_space(Ctx &c) {
 struct Work {
   // sized per batch
   DevBuf A64, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
-      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch, pinv, pprod, pt, pu;
+      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch, pinv, pprod, pt, pu, w2, C2, gram, gram4, R2, Hraw,
+      nrm2b;
   // batched operator: frequency-independent matrices (per sweep) and per-system coefficients (per batch)
   SweepMatrix<double> M64{};
   SweepMatrix<float> M32{};
@@ -879,8 +881,18 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   W.active.alloc((size_t)S * 4);
   W.cyc_active.alloc((size_t)S * 4);
   W.kfinal.alloc((size_t)S * 4);
-  W.partial.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16);
+  const bool two_step = c.arnoldi_step == 2 && pd == 0;
+  W.partial.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16 * (two_step ? 2 : 1));
   W.hbuf.alloc((size_t)S * kMaxBasis * 16);
+  if (two_step) {
+    W.w2.alloc((size_t)S * n * sizeof(X));
+    W.C2.alloc((size_t)S * kMaxBasis * 32);
+    W.gram.alloc((size_t)S * nblk * 32);
+    W.gram4.alloc((size_t)S * 32);
+    W.R2.alloc((size_t)S * 32);
+    W.Hraw.alloc((size_t)S * m * (m + 1) * 16);
+    W.nrm2b.alloc((size_t)S * 8);
+  }
   W.nrm2.alloc((size_t)S * 8);
   std::vector<double> rn2(S), thresh(S);
   std::vector<int32_t> act(S), kf(S);
@@ -990,6 +1002,39 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     int j = 0;
     for (j = 0; j < m; j++) {
       X *Vj = W.V.as<X>() + (int64_t)j * n;
+      if (two_step && j + 1 < m) {
+        // two Krylov vectors per pass over V: z1 = B v_j, z2 = B z1, block projection and update, Cholesky QR of
+        // the pair into v_{j+1}, v_{j+2}, Hessenberg columns j and j+1 from the raw relation
+        const int32_t *a = W.active.as<int32_t>();
+        apply_B(Vj, ldv, W.w.as<X>(), a);
+        apply_B(W.w.as<X>(), n, W.w2.as<X>(), a);
+        launch_mdot2<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), W.w2.as<X>(), n, W.partial.as<double>(), nbm,
+                        a);
+        launch_reduce_ld(st, S, nbm, 2 * (j + 1), 2 * kMaxBasis, W.partial.as<double>(), W.C2.as<double>(),
+                         2 * kMaxBasis, nullptr, 0, a);
+        launch_mupdate2<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), W.w2.as<X>(), n, W.C2.as<double>(),
+                           kMaxBasis, W.gram.as<double>(), nblk, a);
+        launch_reduce4(st, S, nblk, W.gram.as<double>(), W.gram4.as<double>(), a);
+        launch_cholqr2<X>(st, S, n, W.gram4.as<double>(), W.w.as<X>(), W.w2.as<X>(), n, Vj + n, Vj + 2 * n, ldv,
+                          W.R2.as<double>(), a);
+        launch_hcols2(st, S, m, j, W.C2.as<double>(), kMaxBasis, W.R2.as<double>(), W.Hraw.as<double>(),
+                      W.Hh.as<double>(), W.nrm2.as<double>(), W.nrm2b.as<double>(), a);
+        launch_givens_n(st, S, m, j, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
+                        W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
+                        W.nrm2.as<double>());
+        launch_givens_n(st, S, m, j + 1, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
+                        W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
+                        W.nrm2b.as<double>());
+        j++;
+        if ((j & 7) == 7) {
+          HIPCHK(hipMemcpyAsync(act.data(), W.active.p, S * 4, hipMemcpyDeviceToHost, st));
+          HIPCHK(hipStreamSynchronize(st));
+          bool any_a = false;
+          for (int i = 0; i < S; i++) any_a |= act[i] != 0;
+          if (!any_a) break;
+        }
+        continue;
+      }
       if (pd > 0) {
         apply_poly(Vj, ldv, W.pu.as<X>(), W.active.as<int32_t>());
         apply_B(W.pu.as<X>(), n, W.w.as<X>(), W.active.as<int32_t>());
@@ -1883,6 +1928,8 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
     ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
     if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");
     ctx->poly_degree = in->poly_degree;
+    if (in->arnoldi_step < 0 || in->arnoldi_step > 2) throw std::invalid_argument("arnoldi_step must be 0, 1 or 2");
+    ctx->arnoldi_step = in->arnoldi_step == 2 ? 2 : 1;
     if (ctx->coarse_size != in->coarse_size) ctx->have_coarse = false;
     ctx->coarse_size = in->coarse_size;
     sweep(*ctx, in, out);

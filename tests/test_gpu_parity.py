@@ -282,3 +282,20 @@ def test_polynomial_preconditioner(nine_point_fp64, deg, prec):
         s.close()
     assert rb["converged"].all()
     assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("prec,restart", [("mixed", 150), ("mixed", 31), ("fp64", 150)])
+def test_two_step_arnoldi(nine_point_fp64, prec, restart):
+    """Two Krylov vectors per Gram-Schmidt pass (block projection, Cholesky QR of the pair, Hessenberg columns
+    from the raw two-step relation; restart 31 ends each cycle on a single step): same S as one-at-a-time
+    FP64 with the one-step Arnoldi, and a comparable iteration count."""
+    m, f, r1 = nine_point_fp64
+    s = _solver(m, 2)
+    try:
+        rb = s.sweep(f, precision=prec, tol=1e-10, batch=9, orth_passes=1, precond_share=2, arnoldi_step=2,
+                     restart=restart)
+    finally:
+        s.close()
+    assert rb["converged"].all()
+    assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
+    assert rb["iters"].max() <= 2 * r1["iters"].max() + 2 * restart
