@@ -107,11 +107,13 @@ int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const 
 int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, double *avg_seconds, double *alg_bytes);
 
 /* Times the Arnoldi orthogonalisation kernels of the batched GMRES (classical Gram-Schmidt inside
- * KSPGMRES, the VecMDot / VecMAXPY pair of solveComplexLinearSystem.c's KSPSolve) for S systems and a
- * basis of k vectors: avg_seconds[0] = projection h = V^H w, avg_seconds[1] = update w -= V h (+ norm);
- * alg_bytes[0] = S (k+1) N s_v + S nblk k 16, alg_bytes[1] = S (k+2) N s_v + S k 16 + S nblk 8
- * (nblk = ceil(N / 1024) partial blocks). */
-int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *avg_seconds, double *alg_bytes);
+ * KSPGMRES, the VecMDot / VecMAXPY pair of solveComplexLinearSystem.c's KSPSolve) for S systems, a basis
+ * of k vectors and ns new vectors per pass (1 = one-step Arnoldi, 2..4 = s-step): avg_seconds[0] =
+ * projection C = V^H [z_1..z_ns], avg_seconds[1] = update Z -= V C (+ norms / Gram matrix);
+ * alg_bytes[0] = S (k+ns) N s_v + S nblk k ns 16, alg_bytes[1] = S (k+2ns) N s_v + S k ns 16 + S nblk' g 8
+ * (nblk = 4 ceil(N / 2048) projection partials, nblk' = ceil(N / 1024) update partials of g doubles). */
+int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int ns, int nrep, double *avg_seconds,
+                      double *alg_bytes);
 
 typedef struct {
   int nfreq;
@@ -139,9 +141,9 @@ typedef struct {
   int poly_degree;              /* d > 0: GMRES-polynomial preconditioning, the Krylov space built with  */
                                 /*    B p(B) (B = A M, deg p = d - 1, roots from a d-step Arnoldi run per */
                                 /*    system); each iteration applies B d times (0 = off, <= 32)       */
-  int arnoldi_step;             /* 2: two Krylov vectors per Gram-Schmidt pass (z1 = B v, z2 = B z1,    */
-                                /*    block projection, Cholesky QR of the pair); 0 or 1: one; with    */
-                                /*    poly_degree > 0 one is used                                      */
+  int arnoldi_step;             /* s = 2..4: s Krylov vectors per Gram-Schmidt pass (z_1 = B v,        */
+                                /*    z_{c+1} = B z_c, block projection, Cholesky QR of the block);     */
+                                /*    0 or 1: one; with poly_degree > 0 one is used                    */
 } opem_sweep_in;
 
 typedef struct {

@@ -68,7 +68,7 @@ def lib():
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]
         L.opem_spmv_bench.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
-        L.opem_krylov_bench.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+        L.opem_krylov_bench.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
         L.opem_sweep_apply.argtypes = [P, C.c_int, C.c_int, f64p, f64p, f64p, f64p]
         L.opem_calculate_s.argtypes = [C.c_int, f64p, f64p, f64p, f64p, C.c_int, f64p]
         _lib = L

@@ -109,24 +109,28 @@ void launch_reduce(hipStream_t st, int S, int nblk, int k, const double *partial
 void launch_reduce_ld(hipStream_t st, int S, int nblk, int k, int ldp, const double *partial, double *out, int ldo,
                       double *Hcol, int ldH, const int32_t *active);
 
-// ---------------------------------------------------------------- two-step (block) Arnoldi
-// partial[s][blk][v][c] = conj(V[s][v]) . z_c[s] over a chunk, c = 0, 1 (one pass over V; nblk = mdot_blocks(n));
-// reduce with launch_reduce_ld(k = 2k, ldp = 2*kMaxBasis) into C[s][v][c].
+// ---------------------------------------------------------------- s-step (block) Arnoldi, ns = 2..4
+// Z holds the ns new vectors, z_c of system s at Z + (c*S + s)*n.
+// partial[s][blk][v][c] = conj(V[s][v]) . z_c[s] over a chunk (one pass over V; nblk = mdot_blocks(n));
+// reduce with launch_reduce_ld(k = ns*k, ldp = ns*kMaxBasis, ldo = ns*kMaxBasis) into C[s][v][c].
 template <typename X>
-void launch_mdot2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *z1, const X *z2,
-                  int64_t ldz_sys, double *partial, int nblk, const int32_t *active);
-// z_c -= sum_v V[v] C[v][c] (C row stride ldc pairs), Gram partials gram[s][blk][4] of the result (nblk = reduce_blocks)
+void launch_mdot_s(hipStream_t st, int ns, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *Z,
+                   double *partial, int nblk, const int32_t *active);
+// z_c -= sum_v V[v] C[v][c] (C row stride ldc = kMaxBasis), Gram partials gram[s][blk][ns][ns] (complex, upper
+// triangle) of the result (nblk = reduce_blocks(n))
 template <typename X>
-void launch_mupdate2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *z1, X *z2,
-                     int64_t ldz_sys, const double *C, int ldc, double *gram, int nblk, const int32_t *active);
-void launch_reduce4(hipStream_t st, int S, int nblk, const double *gram, double *out, const int32_t *active);
-// Cholesky QR of the pair: q1, q2 (written with stride ldq_sys) and R[s] = (r11, Re r12, Im r12, r22)
+void launch_mupdate_s(hipStream_t st, int ns, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *Z,
+                      const double *C, int ldc, double *gram, int nblk, const int32_t *active);
+// out[s][e] = sum_blk gram[s][blk][e], e < ne
+void launch_reduce_small(hipStream_t st, int S, int nblk, int ne, const double *gram, double *out,
+                         const int32_t *active);
+// Cholesky QR: R[s] (ns x ns complex, R^H R = Gram) and q_c = column c of Z R^{-1}, written at Q + s*ldq_sys + c*n
 template <typename X>
-void launch_cholqr2(hipStream_t st, int S, int64_t n, const double *gram, const X *z1, const X *z2, int64_t ldz_sys,
-                    X *q1, X *q2, int64_t ldq_sys, double *R, const int32_t *active);
-// raw Hessenberg columns j, j+1 into Hraw and Hw, and the Givens subdiagonals nrm2a, nrm2b
-void launch_hcols2(hipStream_t st, int S, int m, int j, const double *C, int ldc, const double *R, double *Hraw,
-                   double *Hw, double *nrm2a, double *nrm2b, const int32_t *active);
+void launch_cholqr_s(hipStream_t st, int ns, int S, int64_t n, const double *gram, const X *Z, X *Q, int64_t ldq_sys,
+                     double *R, const int32_t *active);
+// raw Hessenberg columns j .. j+ns-1 into Hraw and Hw; nrm2[t*S + s] = Givens subdiagonal of column j+t, squared
+void launch_hcols_s(hipStream_t st, int ns, int S, int m, int j, const double *C, int ldc, const double *R,
+                    double *Hraw, double *Hw, double *nrm2, const int32_t *active);
 
 // v[s] = w[s] / nrm[s]  ; nrm from nrm2[s] (sqrt taken inside), also stored into H[s][(k+1)*ldH? ]
 template <typename X>

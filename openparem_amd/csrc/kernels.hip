@@ -1061,48 +1061,75 @@ template void launch_mupdate<double2>(hipStream_t, int, int64_t, int, const doub
 template void launch_mupdate<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, float2 *, int64_t,
                                      const double *, int, double *, int, const int32_t *);
 
-// ------------------------------------------------------------------ two-step (block) Arnoldi
-// Projection of two new vectors against the basis in one pass over V:
-//   partial[s][blk][v][c] = sum_{chunk} conj(V[s][v][i]) z_c[s][i], c = 0, 1 (complex), one partial per wavefront;
-// two basis vectors per pass, their four complex sums finished by the same reduce-scatter butterfly as k_mdot.
-template <typename X, int E>
-__global__ __launch_bounds__(kRedThreads) void k_mdot2(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
-                                                       const X *__restrict__ z1, const X *__restrict__ z2,
-                                                       int64_t ldz_sys, double *__restrict__ partial, int nblk,
-                                                       const int32_t *__restrict__ active) {
-  const int s = blockIdx.y;
+// ------------------------------------------------------------------ s-step (block) Arnoldi, s = 2..4
+// The s new vectors z_c (c < NS) live in one buffer, z_c of system s at Z + (c*S + s)*n.
+// Projection in one pass over V: partial[s][blk][v][c] = sum_{chunk} conj(V[s][v][i]) z_c[s][i] (complex), one
+// partial per wavefront; each pass takes VB = 4/NS basis vectors (4 complex sums) and finishes them with the
+// reduce-scatter butterfly of k_mdot (NS = 3 leaves one of the four slots empty).
+template <typename X, int E, int NS>
+__global__ __launch_bounds__(kRedThreads) void k_mdot_s(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
+                                                        const X *__restrict__ Z, double *__restrict__ partial,
+                                                        int nblk, const int32_t *__restrict__ active) {
+  constexpr int VB = NS >= 3 ? 1 : 4 / NS;
+  const int s = blockIdx.y, S = gridDim.y;
   if (active && !active[s]) return;
   const int64_t base = (int64_t)blockIdx.x * (kRedThreads * E) + threadIdx.x;
   const X *Vs = Vb + s * ldv_sys;
-  X w1[E], w2[E];
+  X w[NS][E];
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int64_t i = base + e * kRedThreads;
-    X zero;
-    zero.x = 0;
-    zero.y = 0;
-    w1[e] = i < n ? z1[s * ldz_sys + i] : zero;
-    w2[e] = i < n ? z2[s * ldz_sys + i] : zero;
-  }
+  for (int c = 0; c < NS; c++)
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int64_t i = base + e * kRedThreads;
+      X zero;
+      zero.x = 0;
+      zero.y = 0;
+      w[c][e] = i < n ? Z[((int64_t)c * S + s) * n + i] : zero;
+    }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
-  for (int v0 = 0; v0 < k; v0 += 2) {
-    double ax[4], ay[4];  // q = 2*(v - v0) + c
+  for (int v0 = 0; v0 < k; v0 += VB) {
+    double ax[4], ay[4];  // slot q = (v - v0) * NS + c
 #pragma unroll
     for (int q = 0; q < 4; q++) ax[q] = ay[q] = 0.0;
 #pragma unroll
-    for (int dv = 0; dv < 2; dv++) {
+    for (int dv = 0; dv < VB; dv++) {
       if (v0 + dv < k) {
+        if constexpr (std::is_same<X, float2>::value) {
+          // FP32 basis: the E products of a lane are summed in FP32, the lane sums in FP64
+          float fx[NS], fy[NS];
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-          const int64_t i = base + e * kRedThreads;
-          if (i < n) {
-            const double2 a = to_d2(Vs[(int64_t)(v0 + dv) * n + i]);
-            const double2 b1 = to_d2(w1[e]), b2 = to_d2(w2[e]);
-            ax[2 * dv] += a.x * b1.x + a.y * b1.y;
-            ay[2 * dv] += a.x * b1.y - a.y * b1.x;
-            ax[2 * dv + 1] += a.x * b2.x + a.y * b2.y;
-            ay[2 * dv + 1] += a.x * b2.y - a.y * b2.x;
+          for (int c = 0; c < NS; c++) fx[c] = fy[c] = 0.f;
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int64_t i = base + e * kRedThreads;
+            if (i < n) {
+              const float2 a = Vs[(int64_t)(v0 + dv) * n + i];
+#pragma unroll
+              for (int c = 0; c < NS; c++) {
+                fx[c] += a.x * w[c][e].x + a.y * w[c][e].y;
+                fy[c] += a.x * w[c][e].y - a.y * w[c][e].x;
+              }
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < NS; c++) {
+            ax[dv * NS + c] += (double)fx[c];
+            ay[dv * NS + c] += (double)fy[c];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int64_t i = base + e * kRedThreads;
+            if (i < n) {
+              const double2 a = to_d2(Vs[(int64_t)(v0 + dv) * n + i]);
+#pragma unroll
+              for (int c = 0; c < NS; c++) {
+                const double2 b = to_d2(w[c][e]);
+                ax[dv * NS + c] += a.x * b.x + a.y * b.y;
+                ay[dv * NS + c] += a.x * b.y - a.y * b.x;
+              }
+            }
           }
         }
       }
@@ -1123,9 +1150,10 @@ __global__ __launch_bounds__(kRedThreads) void k_mdot2(int64_t n, int k, const X
       cy += __shfl_xor(cy, o, 64);
     }
     const int q = (h32 ? 2 : 0) + (h16 ? 1 : 0);
-    const int v = v0 + (q >> 1), c = q & 1;
-    if ((lane & 15) == 0 && v < k) {
-      double *p = partial + ((((int64_t)s * nblk + blockIdx.x * (kRedThreads / 64) + wave) * kMaxK + v) * 2 + c) * 2;
+    const int v = v0 + q / NS, c = q % NS;
+    if ((lane & 15) == 0 && q < VB * NS && v < k) {
+      double *p =
+          partial + ((((int64_t)s * nblk + blockIdx.x * (kRedThreads / 64) + wave) * kMaxK + v) * NS + c) * 2;
       p[0] = cx;
       p[1] = cy;
     }
@@ -1133,220 +1161,337 @@ __global__ __launch_bounds__(kRedThreads) void k_mdot2(int64_t n, int k, const X
 }
 
 template <typename X>
-void launch_mdot2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *z1, const X *z2,
-                  int64_t ldz_sys, double *partial, int nblk, const int32_t *active) {
+void launch_mdot_s(hipStream_t st, int ns, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, const X *Z,
+                   double *partial, int nblk, const int32_t *active) {
   if (k > kMaxK) throw std::runtime_error("restart too large");
-  if (nblk != mdot_blocks(n)) throw std::runtime_error("mdot2: partial block count mismatch");
-  hipLaunchKernelGGL((k_mdot2<X, kMdotElems>), dim3(mdot_grid(n), S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys,
-                     z1, z2, ldz_sys, partial, nblk, active);
+  if (nblk != mdot_blocks(n)) throw std::runtime_error("mdot_s: partial block count mismatch");
+  const dim3 grid(mdot_grid(n), S);
+  if (ns == 2)
+    hipLaunchKernelGGL((k_mdot_s<X, kMdotElems, 2>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, partial,
+                       nblk, active);
+  else if (ns == 3)
+    hipLaunchKernelGGL((k_mdot_s<X, kMdotElems, 3>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, partial,
+                       nblk, active);
+  else if (ns == 4)
+    hipLaunchKernelGGL((k_mdot_s<X, kMdotElems, 4>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, partial,
+                       nblk, active);
+  else
+    throw std::runtime_error("mdot_s: block size must be 2..4");
   HIP_LAUNCH_CHECK();
 }
-template void launch_mdot2<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, const double2 *,
-                                    const double2 *, int64_t, double *, int, const int32_t *);
-template void launch_mdot2<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, const float2 *,
-                                   const float2 *, int64_t, double *, int, const int32_t *);
+template void launch_mdot_s<double2>(hipStream_t, int, int, int64_t, int, const double2 *, int64_t, const double2 *,
+                                     double *, int, const int32_t *);
+template void launch_mdot_s<float2>(hipStream_t, int, int, int64_t, int, const float2 *, int64_t, const float2 *,
+                                    double *, int, const int32_t *);
 
-// z_c -= sum_v V[v] C[v][c] (c = 0, 1; C[s][v][c] complex with row stride ldc entries), then Gram partials
-// gram[s][blk][4] = (|z1|^2, |z2|^2, Re z1^H z2, Im z1^H z2) of the updated pair.
-template <typename X>
-__global__ __launch_bounds__(kRedThreads) void k_mupdate2(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
-                                                          X *__restrict__ z1, X *__restrict__ z2, int64_t ldz_sys,
-                                                          const double *__restrict__ C, int ldc,
-                                                          double *__restrict__ gram, int nblk,
-                                                          const int32_t *__restrict__ active) {
-  const int s = blockIdx.y;
+// z_c -= sum_v V[v] C[v][c] (C[s][v][c] complex, row stride ldc blocks of NS), then the Gram matrix of the result,
+// gram[s][blk][a][b] = z_a^H z_b (complex, NS x NS).
+template <typename X, int NS>
+__global__ __launch_bounds__(kRedThreads) void k_mupdate_s(int64_t n, int k, const X *__restrict__ Vb, int64_t ldv_sys,
+                                                           X *__restrict__ Z, const double *__restrict__ C, int ldc,
+                                                           double *__restrict__ gram, int nblk,
+                                                           const int32_t *__restrict__ active) {
+  const int s = blockIdx.y, S = gridDim.y;
   if (active && !active[s]) return;
-  __shared__ double2 hs[2][kMaxK];
-  __shared__ double red[4][kRedThreads / 64];
-  for (int v = threadIdx.x; v < k; v += kRedThreads) {
-    const double *cv = C + ((int64_t)s * ldc + v) * 4;
-    hs[0][v] = make_double2(cv[0], cv[1]);
-    hs[1][v] = make_double2(cv[2], cv[3]);
-  }
+  __shared__ double2 hs[NS][kMaxK];
+  __shared__ double red[NS * NS * 2][kRedThreads / 64];
+  for (int v = threadIdx.x; v < k; v += kRedThreads)
+    for (int c = 0; c < NS; c++) {
+      const double *cv = C + (((int64_t)s * ldc + v) * NS + c) * 2;
+      hs[c][v] = make_double2(cv[0], cv[1]);
+    }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
-  X *a1 = z1 + s * ldz_sys;
-  X *a2 = z2 + s * ldz_sys;
   const X *Vs = Vb + s * ldv_sys;
-  double g11 = 0.0, g22 = 0.0, g12x = 0.0, g12y = 0.0;
+  double g[NS][NS][2];
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+#pragma unroll
+    for (int b = 0; b < NS; b++) g[a][b][0] = g[a][b][1] = 0.0;
 #pragma unroll
   for (int e = 0; e < kRedElems; e++) {
     const int64_t i = base + e * kRedThreads;
     if (i < n) {
-      double2 p = to_d2(a1[i]), q = to_d2(a2[i]);
+      double2 p[NS];
+#pragma unroll
+      for (int c = 0; c < NS; c++) p[c] = to_d2(Z[((int64_t)c * S + s) * n + i]);
       for (int v = 0; v < k; v++) {
         const double2 a = to_d2(Vs[(int64_t)v * n + i]);
-        const double2 c1 = cmul(a, hs[0][v]), c2 = cmul(a, hs[1][v]);
-        p.x -= c1.x;
-        p.y -= c1.y;
-        q.x -= c2.x;
-        q.y -= c2.y;
+#pragma unroll
+        for (int c = 0; c < NS; c++) {
+          const double2 t = cmul(a, hs[c][v]);
+          p[c].x -= t.x;
+          p[c].y -= t.y;
+        }
       }
-      const X po = from_d2<X>(p), qo = from_d2<X>(q);
-      a1[i] = po;
-      a2[i] = qo;
-      const double2 pp = to_d2(po), qq = to_d2(qo);
-      g11 += pp.x * pp.x + pp.y * pp.y;
-      g22 += qq.x * qq.x + qq.y * qq.y;
-      g12x += pp.x * qq.x + pp.y * qq.y;
-      g12y += pp.x * qq.y - pp.y * qq.x;
+      double2 q[NS];
+#pragma unroll
+      for (int c = 0; c < NS; c++) {
+        const X o = from_d2<X>(p[c]);
+        Z[((int64_t)c * S + s) * n + i] = o;
+        q[c] = to_d2(o);
+      }
+#pragma unroll
+      for (int a = 0; a < NS; a++)
+#pragma unroll
+        for (int b = a; b < NS; b++) {
+          g[a][b][0] += q[a].x * q[b].x + q[a].y * q[b].y;
+          g[a][b][1] += q[a].x * q[b].y - q[a].y * q[b].x;
+        }
     }
   }
-  g11 = wave_sum(g11);
-  g22 = wave_sum(g22);
-  g12x = wave_sum(g12x);
-  g12y = wave_sum(g12y);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = g11;
-    red[1][threadIdx.x >> 6] = g22;
-    red[2][threadIdx.x >> 6] = g12x;
-    red[3][threadIdx.x >> 6] = g12y;
-  }
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+#pragma unroll
+    for (int b = a; b < NS; b++)
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        const double t = wave_sum(g[a][b][r]);
+        if ((threadIdx.x & 63) == 0) red[(a * NS + b) * 2 + r][wave] = t;
+      }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < NS * NS * 2) {
+    const int ab = threadIdx.x >> 1, a = ab / NS, b = ab % NS, r = threadIdx.x & 1;
     double t = 0.0;
-    for (int q = 0; q < kRedThreads / 64; q++) t += red[threadIdx.x][q];
-    gram[((int64_t)s * nblk + blockIdx.x) * 4 + threadIdx.x] = t;
+    if (b >= a)
+      for (int q = 0; q < kRedThreads / 64; q++) t += red[threadIdx.x][q];
+    gram[((int64_t)s * nblk + blockIdx.x) * NS * NS * 2 + threadIdx.x] = t;
   }
 }
 
 template <typename X>
-void launch_mupdate2(hipStream_t st, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *z1, X *z2,
-                     int64_t ldz_sys, const double *C, int ldc, double *gram, int nblk, const int32_t *active) {
-  hipLaunchKernelGGL((k_mupdate2<X>), dim3(nblk, S), dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, z1, z2, ldz_sys, C,
-                     ldc, gram, nblk, active);
+void launch_mupdate_s(hipStream_t st, int ns, int S, int64_t n, int k, const X *Vb, int64_t ldv_sys, X *Z,
+                      const double *C, int ldc, double *gram, int nblk, const int32_t *active) {
+  const dim3 grid(nblk, S);
+  if (ns == 2)
+    hipLaunchKernelGGL((k_mupdate_s<X, 2>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, C, ldc, gram, nblk,
+                       active);
+  else if (ns == 3)
+    hipLaunchKernelGGL((k_mupdate_s<X, 3>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, C, ldc, gram, nblk,
+                       active);
+  else if (ns == 4)
+    hipLaunchKernelGGL((k_mupdate_s<X, 4>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, C, ldc, gram, nblk,
+                       active);
+  else
+    throw std::runtime_error("mupdate_s: block size must be 2..4");
   HIP_LAUNCH_CHECK();
 }
-template void launch_mupdate2<double2>(hipStream_t, int, int64_t, int, const double2 *, int64_t, double2 *,
-                                       double2 *, int64_t, const double *, int, double *, int, const int32_t *);
-template void launch_mupdate2<float2>(hipStream_t, int, int64_t, int, const float2 *, int64_t, float2 *, float2 *,
-                                      int64_t, const double *, int, double *, int, const int32_t *);
+template void launch_mupdate_s<double2>(hipStream_t, int, int, int64_t, int, const double2 *, int64_t, double2 *,
+                                        const double *, int, double *, int, const int32_t *);
+template void launch_mupdate_s<float2>(hipStream_t, int, int, int64_t, int, const float2 *, int64_t, float2 *,
+                                       const double *, int, double *, int, const int32_t *);
 
-// out[s][0..3] = sum_blk gram[s][blk][0..3] (fixed order)
-__global__ void k_reduce4(int nblk, const double *__restrict__ gram, double *__restrict__ out,
+// out[s][e] = sum_blk gram[s][blk][e], e < ne (fixed order)
+__global__ void k_reduce_small(int nblk, int ne, const double *__restrict__ gram, double *__restrict__ out,
+                               const int32_t *__restrict__ active) {
+  const int s = blockIdx.x;
+  if (active && !active[s]) return;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int e = wave; e < ne; e += blockDim.x >> 6) {
+    double t = 0.0;
+    for (int b = l; b < nblk; b += 64) t += gram[((int64_t)s * nblk + b) * ne + e];
+    t = wave_sum(t);
+    if (l == 0) out[(int64_t)s * ne + e] = t;
+  }
+}
+
+void launch_reduce_small(hipStream_t st, int S, int nblk, int ne, const double *gram, double *out,
+                         const int32_t *active) {
+  hipLaunchKernelGGL(k_reduce_small, dim3(S), dim3(256), 0, st, nblk, ne, gram, out, active);
+  HIP_LAUNCH_CHECK();
+}
+
+// Cholesky factor R (upper, NS x NS, R^H R = G) of the Gram matrix G (upper triangle used); a pivot below 1e-12 of
+// its Gram diagonal counts as zero (its column of R is zero, and the matching q the zero vector).
+template <int NS>
+__device__ __forceinline__ void chol_s(const double *G, double2 R[NS][NS]) {
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+#pragma unroll
+    for (int b = 0; b < NS; b++) R[a][b] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int a = 0; a < NS; a++) {
+    double d = G[(a * NS + a) * 2];
+#pragma unroll
+    for (int i = 0; i < a; i++) d -= R[i][a].x * R[i][a].x + R[i][a].y * R[i][a].y;
+    const double gaa = G[(a * NS + a) * 2];
+    const double raa = d > 1e-12 * gaa && d > 0.0 ? sqrt(d) : 0.0;
+    R[a][a] = make_double2(raa, 0.0);
+#pragma unroll
+    for (int b = a + 1; b < NS; b++) {
+      double2 t = make_double2(G[(a * NS + b) * 2], G[(a * NS + b) * 2 + 1]);
+#pragma unroll
+      for (int i = 0; i < a; i++) {
+        const double2 u = cconjmul(R[i][a], R[i][b]);
+        t.x -= u.x;
+        t.y -= u.y;
+      }
+      R[a][b] = raa > 0.0 ? make_double2(t.x / raa, t.y / raa) : make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// Q = Z R^{-1} (q_c -> Q + c*ldq_sys... of each system), R[s] stored as NS x NS complex
+template <typename X, int NS>
+__global__ void k_cholqr_s(int64_t n, const double *__restrict__ gram, const X *__restrict__ Z, X *__restrict__ Q,
+                           int64_t ldq_sys, double *__restrict__ Rout, const int32_t *__restrict__ active) {
+  const int s = blockIdx.y, S = gridDim.y;
+  if (active && !active[s]) return;
+  double2 R[NS][NS];
+  chol_s<NS>(gram + (int64_t)s * NS * NS * 2, R);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int a = 0; a < NS; a++)
+      for (int b = 0; b < NS; b++) {
+        Rout[(((int64_t)s * NS + a) * NS + b) * 2] = R[a][b].x;
+        Rout[(((int64_t)s * NS + a) * NS + b) * 2 + 1] = R[a][b].y;
+      }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 q[NS];
+#pragma unroll
+    for (int c = 0; c < NS; c++) {
+      double2 t = to_d2(Z[((int64_t)c * S + s) * n + i]);
+#pragma unroll
+      for (int a = 0; a < c; a++) {
+        const double2 u = cmul(q[a], R[a][c]);
+        t.x -= u.x;
+        t.y -= u.y;
+      }
+      const double rc = R[c][c].x;
+      q[c] = rc > 0.0 ? make_double2(t.x / rc, t.y / rc) : make_double2(0.0, 0.0);
+      Q[s * ldq_sys + (int64_t)c * n + i] = from_d2<X>(q[c]);
+    }
+  }
+}
+
+template <typename X>
+void launch_cholqr_s(hipStream_t st, int ns, int S, int64_t n, const double *gram, const X *Z, X *Q, int64_t ldq_sys,
+                     double *R, const int32_t *active) {
+  int64_t bl = (n + 255) / 256;
+  if (bl > 4096) bl = 4096;
+  const dim3 grid((unsigned)bl, S);
+  if (ns == 2)
+    hipLaunchKernelGGL((k_cholqr_s<X, 2>), grid, dim3(256), 0, st, n, gram, Z, Q, ldq_sys, R, active);
+  else if (ns == 3)
+    hipLaunchKernelGGL((k_cholqr_s<X, 3>), grid, dim3(256), 0, st, n, gram, Z, Q, ldq_sys, R, active);
+  else if (ns == 4)
+    hipLaunchKernelGGL((k_cholqr_s<X, 4>), grid, dim3(256), 0, st, n, gram, Z, Q, ldq_sys, R, active);
+  else
+    throw std::runtime_error("cholqr_s: block size must be 2..4");
+  HIP_LAUNCH_CHECK();
+}
+template void launch_cholqr_s<double2>(hipStream_t, int, int, int64_t, const double *, const double2 *, double2 *,
+                                       int64_t, double *, const int32_t *);
+template void launch_cholqr_s<float2>(hipStream_t, int, int, int64_t, const double *, const float2 *, float2 *,
+                                      int64_t, double *, const int32_t *);
+
+// Raw Hessenberg columns j .. j+NS-1 of the s-step relation (z_1 = B v_j, z_{c+1} = B z_c; Z = V C + Q R):
+//   column j      = [C_0 ; R_{:,0}]
+//   columns j+1.. = [[C_{1:} - Y - C_0 c_j] ; [R_{:,1:} - R_{:,0} c_j]] R_{0:NS-1,0:NS-1}^{-1},
+//   c_j = C[j][0:NS-1] (row), Y = Hraw[0..j][0..j-1] C[0..j-1][0:NS-1].
+// Both go to Hraw and to the working H; nrm2[t][s] = (R_tt / R_{t-1,t-1})^2 (t >= 1), nrm2[0][s] = R_00^2 are the
+// Givens subdiagonals.
+template <int NS>
+__global__ void k_hcols_s(int m, int j, int S, const double *__restrict__ C, int ldc, const double *__restrict__ Rg,
+                          double2 *__restrict__ Hraw, double2 *__restrict__ Hw, double *__restrict__ nrm2,
                           const int32_t *__restrict__ active) {
   const int s = blockIdx.x;
   if (active && !active[s]) return;
-  __shared__ double red[4][64];
-  const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
-  double t = 0.0;
-  for (int b = l; b < nblk; b += 64) t += gram[((int64_t)s * nblk + b) * 4 + c];
-  t = wave_sum(t);
-  if (l == 0) out[(int64_t)s * 4 + c] = t;
-  (void)red;
-}
-
-void launch_reduce4(hipStream_t st, int S, int nblk, const double *gram, double *out, const int32_t *active) {
-  hipLaunchKernelGGL(k_reduce4, dim3(S), dim3(256), 0, st, nblk, gram, out, active);
-  HIP_LAUNCH_CHECK();
-}
-
-// Cholesky QR of the projected pair from its Gram matrix G = [z1 z2]^H [z1 z2]:
-//   r11 = sqrt(G11), r12 = G12 / r11, r22 = sqrt(G22 - |r12|^2);  q1 = z1 / r11 -> V[j+1], q2 = (z2 - q1 r12) / r22 -> V[j+2]
-// R[s] = (r11, Re r12, Im r12, r22).  A vanishing r11 or r22 (relative to the Gram entries) counts as zero, and the
-// matching q as the zero vector (the Givens step then ends the system).
-__device__ __forceinline__ void chol2(const double *g, double &r11, double2 &r12, double &r22) {
-  r11 = g[0] > 0.0 ? sqrt(g[0]) : 0.0;
-  r12 = r11 > 0.0 ? make_double2(g[2] / r11, g[3] / r11) : make_double2(0.0, 0.0);
-  const double d = g[1] - (r12.x * r12.x + r12.y * r12.y);
-  r22 = d > 1e-12 * g[1] ? sqrt(d) : 0.0;
-}
-
-template <typename X>
-__global__ void k_cholqr2(int64_t n, const double *__restrict__ gram, const X *__restrict__ z1,
-                          const X *__restrict__ z2, int64_t ldz_sys, X *__restrict__ q1, X *__restrict__ q2,
-                          int64_t ldq_sys, double *__restrict__ R, const int32_t *__restrict__ active) {
-  const int s = blockIdx.y;
-  if (active && !active[s]) return;
-  double r11, r22;
-  double2 r12;
-  chol2(gram + (int64_t)s * 4, r11, r12, r22);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    R[(int64_t)s * 4] = r11;
-    R[(int64_t)s * 4 + 1] = r12.x;
-    R[(int64_t)s * 4 + 2] = r12.y;
-    R[(int64_t)s * 4 + 3] = r22;
-  }
-  const double i11 = r11 > 0.0 ? 1.0 / r11 : 0.0, i22 = r22 > 0.0 ? 1.0 / r22 : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double2 a = to_d2(z1[s * ldz_sys + i]), b = to_d2(z2[s * ldz_sys + i]);
-    const double2 u = make_double2(a.x * i11, a.y * i11);
-    const double2 ur = cmul(u, r12);
-    q1[s * ldq_sys + i] = from_d2<X>(u);
-    q2[s * ldq_sys + i] = from_d2<X>(make_double2((b.x - ur.x) * i22, (b.y - ur.y) * i22));
-  }
-}
-
-template <typename X>
-void launch_cholqr2(hipStream_t st, int S, int64_t n, const double *gram, const X *z1, const X *z2, int64_t ldz_sys,
-                    X *q1, X *q2, int64_t ldq_sys, double *R, const int32_t *active) {
-  int64_t bl = (n + 255) / 256;
-  if (bl > 4096) bl = 4096;
-  hipLaunchKernelGGL((k_cholqr2<X>), dim3((unsigned)bl, S), dim3(256), 0, st, n, gram, z1, z2, ldz_sys, q1, q2,
-                     ldq_sys, R, active);
-  HIP_LAUNCH_CHECK();
-}
-template void launch_cholqr2<double2>(hipStream_t, int, int64_t, const double *, const double2 *, const double2 *,
-                                      int64_t, double2 *, double2 *, int64_t, double *, const int32_t *);
-template void launch_cholqr2<float2>(hipStream_t, int, int64_t, const double *, const float2 *, const float2 *,
-                                     int64_t, float2 *, float2 *, int64_t, double *, const int32_t *);
-
-// Raw Hessenberg columns j and j+1 of the two-step relation (B v_j = z1, B z1 = z2):
-//   col j:   H[0..j] = C0, H[j+1] = r11
-//   col j+1: H[l] = (C1[l] - y[l] - C0[j] C0[l]) / r11 (l <= j), H[j+1] = r12 / r11 - C0[j], H[j+2] = r22 / r11,
-//   y = Hraw[0..j, 0..j-1] C0[0..j-1].  Both columns go to Hraw and to the working H (rotated later);
-//   nrm2a = r11^2, nrm2b = (r22 / r11)^2 feed the Givens steps.
-__global__ void k_hcols2(int m, int j, const double *__restrict__ C, int ldc, const double *__restrict__ R,
-                         double2 *__restrict__ Hraw, double2 *__restrict__ Hw, double *__restrict__ nrm2a,
-                         double *__restrict__ nrm2b, const int32_t *__restrict__ active) {
-  const int s = blockIdx.x;
-  if (active && !active[s]) return;
-  const double *Cs = C + (int64_t)s * ldc * 4;
-  auto c0 = [&](int v) { return make_double2(Cs[v * 4], Cs[v * 4 + 1]); };
-  auto c1 = [&](int v) { return make_double2(Cs[v * 4 + 2], Cs[v * 4 + 3]); };
+  auto cc = [&](int v, int c) {
+    const double *p = C + (((int64_t)s * ldc + v) * NS + c) * 2;
+    return make_double2(p[0], p[1]);
+  };
+  double2 R[NS][NS];
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+      R[a][b] = make_double2(Rg[(((int64_t)s * NS + a) * NS + b) * 2], Rg[(((int64_t)s * NS + a) * NS + b) * 2 + 1]);
   double2 *Hr = Hraw + (int64_t)s * m * (m + 1);
   double2 *Hs = Hw + (int64_t)s * m * (m + 1);
-  const double r11 = R[(int64_t)s * 4], r22 = R[(int64_t)s * 4 + 3];
-  const double2 r12 = make_double2(R[(int64_t)s * 4 + 1], R[(int64_t)s * 4 + 2]);
-  const double i11 = r11 > 0.0 ? 1.0 / r11 : 0.0;
-  const double2 cj = c0(j);
-  for (int l = threadIdx.x; l <= j + 2; l += blockDim.x) {
-    double2 hj = make_double2(0.0, 0.0), h1 = make_double2(0.0, 0.0);
+  double2 cj[NS];
+#pragma unroll
+  for (int c = 0; c < NS - 1; c++) cj[c] = cc(j, c);
+  for (int l = threadIdx.x; l <= j + NS; l += blockDim.x) {
+    // column j
+    double2 h0 = make_double2(0.0, 0.0);
+    if (l <= j)
+      h0 = cc(l, 0);
+    else
+      h0 = R[l - j - 1][0];
+    // u = row l of the block before R^{-1}
+    double2 u[NS - 1];
     if (l <= j) {
-      hj = c0(l);
-      double2 y = make_double2(0.0, 0.0);
-      for (int c = (l > 0 ? l - 1 : 0); c < j; c++) {
-        const double2 t = cmul(Hr[(int64_t)c * (m + 1) + l], c0(c));
-        y.x += t.x;
-        y.y += t.y;
+      double2 c0l = cc(l, 0);
+#pragma unroll
+      for (int c = 0; c < NS - 1; c++) {
+        double2 y = make_double2(0.0, 0.0);
+        for (int i = (l > 0 ? l - 1 : 0); i < j; i++) {
+          const double2 t = cmul(Hr[(int64_t)i * (m + 1) + l], cc(i, c));
+          y.x += t.x;
+          y.y += t.y;
+        }
+        const double2 d = cc(l, c + 1), e = cmul(c0l, cj[c]);
+        u[c] = make_double2(d.x - y.x - e.x, d.y - y.y - e.y);
       }
-      const double2 cc = cmul(cj, c0(l)), d = c1(l);
-      h1 = make_double2((d.x - y.x - cc.x) * i11, (d.y - y.y - cc.y) * i11);
-    } else if (l == j + 1) {
-      hj = make_double2(r11, 0.0);
-      h1 = make_double2(r12.x * i11 - cj.x, r12.y * i11 - cj.y);
     } else {
-      h1 = make_double2(r22 * i11, 0.0);
+      const int r = l - j - 1;
+#pragma unroll
+      for (int c = 0; c < NS - 1; c++) {
+        const double2 e = cmul(R[r][0], cj[c]);
+        u[c] = make_double2(R[r][c + 1].x - e.x, R[r][c + 1].y - e.y);
+      }
+    }
+    // h = u R_{0:NS-1,0:NS-1}^{-1} (row vector, forward substitution)
+    double2 h[NS - 1];
+#pragma unroll
+    for (int c = 0; c < NS - 1; c++) {
+      double2 t = u[c];
+#pragma unroll
+      for (int a = 0; a < c; a++) {
+        const double2 w = cmul(h[a], R[a][c]);
+        t.x -= w.x;
+        t.y -= w.y;
+      }
+      const double rd = R[c][c].x;
+      h[c] = rd > 0.0 ? make_double2(t.x / rd, t.y / rd) : make_double2(0.0, 0.0);
     }
     if (l <= j + 1) {
-      Hr[(int64_t)j * (m + 1) + l] = hj;
-      Hs[(int64_t)j * (m + 1) + l] = hj;
+      Hr[(int64_t)j * (m + 1) + l] = h0;
+      Hs[(int64_t)j * (m + 1) + l] = h0;
     }
-    if (j + 1 < m) {
-      Hr[(int64_t)(j + 1) * (m + 1) + l] = h1;
-      Hs[(int64_t)(j + 1) * (m + 1) + l] = h1;
+#pragma unroll
+    for (int c = 0; c < NS - 1; c++) {
+      const int col = j + 1 + c;
+      if (col < m && l <= col + 1) {
+        Hr[(int64_t)col * (m + 1) + l] = h[c];
+        Hs[(int64_t)col * (m + 1) + l] = h[c];
+      }
     }
   }
   if (threadIdx.x == 0) {
-    nrm2a[s] = r11 * r11;
-    nrm2b[s] = r22 * i11 * r22 * i11;
+    nrm2[s] = R[0][0].x * R[0][0].x;
+#pragma unroll
+    for (int t = 1; t < NS; t++) {
+      const double q = R[t - 1][t - 1].x > 0.0 ? R[t][t].x / R[t - 1][t - 1].x : 0.0;
+      nrm2[(int64_t)t * S + s] = q * q;
+    }
   }
 }
 
-void launch_hcols2(hipStream_t st, int S, int m, int j, const double *C, int ldc, const double *R, double *Hraw,
-                   double *Hw, double *nrm2a, double *nrm2b, const int32_t *active) {
-  hipLaunchKernelGGL(k_hcols2, dim3(S), dim3(256), 0, st, m, j, C, ldc, R, (double2 *)Hraw, (double2 *)Hw, nrm2a,
-                     nrm2b, active);
+void launch_hcols_s(hipStream_t st, int ns, int S, int m, int j, const double *C, int ldc, const double *R,
+                    double *Hraw, double *Hw, double *nrm2, const int32_t *active) {
+  if (ns == 2)
+    hipLaunchKernelGGL((k_hcols_s<2>), dim3(S), dim3(256), 0, st, m, j, S, C, ldc, R, (double2 *)Hraw, (double2 *)Hw,
+                       nrm2, active);
+  else if (ns == 3)
+    hipLaunchKernelGGL((k_hcols_s<3>), dim3(S), dim3(256), 0, st, m, j, S, C, ldc, R, (double2 *)Hraw, (double2 *)Hw,
+                       nrm2, active);
+  else if (ns == 4)
+    hipLaunchKernelGGL((k_hcols_s<4>), dim3(S), dim3(256), 0, st, m, j, S, C, ldc, R, (double2 *)Hraw, (double2 *)Hw,
+                       nrm2, active);
+  else
+    throw std::runtime_error("hcols_s: block size must be 2..4");
   HIP_LAUNCH_CHECK();
 }
 
@@ -1600,62 +1745,86 @@ __device__ __forceinline__ int64_t patch_offset(const PatchLayout &L, int p, int
 }
 __device__ __forceinline__ int pad8(int n) { return (n + 7) & ~7; }
 
+// Gauss-Jordan with partial pivoting in LDS.  FP16 storage inverts in FP32 (the FP32 error, about cond * 6e-8,
+// stays far below the FP16 rounding of the stored inverse, and the half-size LDS block doubles residency);
+// FP64 storage inverts in FP64.
+template <typename T>
+struct Cx;
+template <>
+struct Cx<float> {
+  using V = float2;
+};
+template <>
+struct Cx<double> {
+  using V = double2;
+};
+
 template <typename P>
 __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R, const double2 *__restrict__ a64,
                                                      int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
                                                      P *__restrict__ inv, float *__restrict__ scale) {
   constexpr bool kPacked = std::is_same<P, __half2>::value;
-  extern __shared__ double2 lds[];
+  using T = typename std::conditional<kPacked, float, double>::type;
+  using C2 = typename Cx<T>::V;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C2 *lds = (C2 *)lds_raw;
   const int p = blockIdx.x;
   const int q = blockIdx.y / R, r = blockIdx.y % R;
   if (!L.port[p] && r != 0) return;  // shared inverse: built once per group
   const int g0 = q * L.pg, ng = min(L.pg, G - g0);
   const double wg = 1.0 / ng;
   const int np = L.idx_off[p + 1] - L.idx_off[p];
-  double2 *A = lds;                          // np*np
-  double2 *colk = lds + np * np;             // np
+  C2 *A = lds;                 // np*np
+  C2 *colk = lds + np * np;    // np
   __shared__ int perm[128];
-  __shared__ double2 pinv_s;
+  __shared__ C2 pinv_s;
   __shared__ double red[4];
   const int32_t *I = L.idx + L.idx_off[p];
   const int32_t *nz = L.pnz + L.nz_off[p];
   const uint8_t *mask = rowmask + (int64_t)r * n;
   const int tid = threadIdx.x;
   const int nn = np * np;
+  auto mk = [](T x, T y) {
+    C2 o;
+    o.x = x;
+    o.y = y;
+    return o;
+  };
+  auto mul = [&](C2 a, C2 b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
   // block of the mean matrix of the group's frequencies (the frequencies of a group are adjacent)
   for (int t = tid; t < nn; t += blockDim.x) {
     const int a = t / np, b = t - a * np;
     const bool fa = mask[I[a]] != 0, fb = mask[I[b]] != 0;
-    double2 v = make_double2(a == b ? 1.0 : 0.0, 0.0);
+    double vx = a == b ? 1.0 : 0.0, vy = 0.0;
     if (fa && fb) {
       const int32_t qz = nz[t];
-      v = make_double2(0.0, 0.0);
+      vx = 0.0;
       if (qz >= 0)
         for (int g = g0; g < g0 + ng; g++) {
           const double2 e = a64[(int64_t)g * nnz + qz];
-          v.x += e.x;
-          v.y += e.y;
+          vx += e.x;
+          vy += e.y;
         }
-      v.x *= wg;
-      v.y *= wg;
+      vx *= wg;
+      vy *= wg;
     }
-    A[t] = v;
+    A[t] = mk((T)vx, (T)vy);
   }
   __syncthreads();
   for (int k = 0; k < np; k++) {
     if (tid < 64) {
-      double best = -1.0;
+      T best = (T)-1;
       int bi = k;
       for (int i = k + tid; i < np; i += 64) {
-        const double2 v = A[i * np + k];
-        const double m = v.x * v.x + v.y * v.y;
+        const C2 v = A[i * np + k];
+        const T m = v.x * v.x + v.y * v.y;
         if (m > best) {
           best = m;
           bi = i;
         }
       }
       for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o, 64);
+        const T ob = __shfl_xor(best, o, 64);
         const int oi = __shfl_xor(bi, o, 64);
         if (ob > best || (ob == best && oi < bi)) {
           best = ob;
@@ -1668,30 +1837,30 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
     const int bi = perm[k];
     if (bi != k)
       for (int c = tid; c < np; c += blockDim.x) {
-        const double2 t0 = A[k * np + c];
+        const C2 t0 = A[k * np + c];
         A[k * np + c] = A[bi * np + c];
         A[bi * np + c] = t0;
       }
     __syncthreads();
     if (tid == 0) {
-      const double2 piv = A[k * np + k];
-      const double pm = piv.x * piv.x + piv.y * piv.y;
-      pinv_s = pm > 0.0 ? make_double2(piv.x / pm, -piv.y / pm) : make_double2(0.0, 0.0);
+      const C2 piv = A[k * np + k];
+      const T pm = piv.x * piv.x + piv.y * piv.y;
+      pinv_s = pm > (T)0 ? mk(piv.x / pm, -piv.y / pm) : mk((T)0, (T)0);
     }
     __syncthreads();
-    const double2 pinv = pinv_s;
-    for (int c = tid; c < np; c += blockDim.x) A[k * np + c] = (c == k) ? pinv : cmul(A[k * np + c], pinv);
+    const C2 pinv = pinv_s;
+    for (int c = tid; c < np; c += blockDim.x) A[k * np + c] = (c == k) ? pinv : mul(A[k * np + c], pinv);
     for (int i = tid; i < np; i += blockDim.x) colk[i] = A[i * np + k];
     __syncthreads();
     for (int t = tid; t < nn; t += blockDim.x) {
       const int i = t / np, c = t - i * np;
       if (i == k) continue;
-      const double2 f = colk[i];
+      const C2 f = colk[i];
       if (c == k) {
-        A[t] = make_double2(-(f.x * pinv.x - f.y * pinv.y), -(f.x * pinv.y + f.y * pinv.x));
+        A[t] = mk(-(f.x * pinv.x - f.y * pinv.y), -(f.x * pinv.y + f.y * pinv.x));
       } else {
-        const double2 u = cmul(f, A[k * np + c]);
-        A[t] = make_double2(A[t].x - u.x, A[t].y - u.y);
+        const C2 u = mul(f, A[k * np + c]);
+        A[t] = mk(A[t].x - u.x, A[t].y - u.y);
       }
     }
     __syncthreads();
@@ -1700,7 +1869,7 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
     const int pk = perm[k];
     if (pk != k)
       for (int i = tid; i < np; i += blockDim.x) {
-        const double2 t0 = A[i * np + k];
+        const C2 t0 = A[i * np + k];
         A[i * np + k] = A[i * np + pk];
         A[i * np + pk] = t0;
       }
@@ -1710,7 +1879,7 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
   if constexpr (kPacked) {
     // block scale: the largest |Re|,|Im| maps to 1 in the half-precision store
     double m = 0.0;
-    for (int t = tid; t < nn; t += blockDim.x) m = fmax(m, fmax(fabs(A[t].x), fabs(A[t].y)));
+    for (int t = tid; t < nn; t += blockDim.x) m = fmax(m, fmax(fabs((double)A[t].x), fabs((double)A[t].y)));
     m = wave_sum_max(m);
     if ((tid & 63) == 0) red[tid >> 6] = m;
     __syncthreads();
@@ -1724,15 +1893,15 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
       const int i = t / npk, j = t - i * npk;
       double re = 0.0, im = 0.0;
       if (j < np) {
-        const double2 u = A[i * np + j], w = A[j * np + i];
-        re = (u.x + w.x) * h;
-        im = (u.y + w.y) * h;
+        const C2 u = A[i * np + j], w = A[j * np + i];
+        re = ((double)u.x + (double)w.x) * h;
+        im = ((double)u.y + (double)w.y) * h;
       }
       Pr[t] = __float2half((float)re);
       Pi[t] = __float2half((float)im);
     }
   } else {
-    for (int t = tid; t < nn; t += blockDim.x) out[t] = A[t];
+    for (int t = tid; t < nn; t += blockDim.x) out[t] = make_double2((double)A[t].x, (double)A[t].y);
   }
 }
 
@@ -1741,7 +1910,7 @@ void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, cons
                         const uint8_t *rowmask, int64_t n, P *inv, float *scale) {
   if (L.nmax > 100) throw std::runtime_error("vertex patch larger than 100 DOFs");
   if (L.pg < 1) throw std::runtime_error("patch build: frequencies per preconditioner must be >= 1");
-  const size_t lds = (size_t)(L.nmax * L.nmax + L.nmax) * sizeof(double2);
+  const size_t lds = (size_t)(L.nmax * L.nmax + L.nmax) * (std::is_same<P, __half2>::value ? 8 : 16);
   static size_t attr_bytes = 0;
   if (lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute((const void *)k_patch_build<P>, hipFuncAttributeMaxDynamicSharedMemorySize,

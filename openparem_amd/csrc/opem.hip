@@ -881,17 +881,17 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   W.active.alloc((size_t)S * 4);
   W.cyc_active.alloc((size_t)S * 4);
   W.kfinal.alloc((size_t)S * 4);
-  const bool two_step = c.arnoldi_step == 2 && pd == 0;
-  W.partial.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16 * (two_step ? 2 : 1));
+  const int ns = pd == 0 ? c.arnoldi_step : 1;  // Krylov vectors per Gram-Schmidt pass
+  W.partial.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16 * ns);
   W.hbuf.alloc((size_t)S * kMaxBasis * 16);
-  if (two_step) {
-    W.w2.alloc((size_t)S * n * sizeof(X));
-    W.C2.alloc((size_t)S * kMaxBasis * 32);
-    W.gram.alloc((size_t)S * nblk * 32);
-    W.gram4.alloc((size_t)S * 32);
-    W.R2.alloc((size_t)S * 32);
+  if (ns > 1) {
+    W.w2.alloc((size_t)ns * S * n * sizeof(X));
+    W.C2.alloc((size_t)S * kMaxBasis * 16 * ns);
+    W.gram.alloc((size_t)S * nblk * ns * ns * 16);
+    W.gram4.alloc((size_t)S * ns * ns * 16);
+    W.R2.alloc((size_t)S * ns * ns * 16);
     W.Hraw.alloc((size_t)S * m * (m + 1) * 16);
-    W.nrm2b.alloc((size_t)S * 8);
+    W.nrm2b.alloc((size_t)ns * S * 8);
   }
   W.nrm2.alloc((size_t)S * 8);
   std::vector<double> rn2(S), thresh(S);
@@ -1002,31 +1002,31 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     int j = 0;
     for (j = 0; j < m; j++) {
       X *Vj = W.V.as<X>() + (int64_t)j * n;
-      if (two_step && j + 1 < m) {
-        // two Krylov vectors per pass over V: z1 = B v_j, z2 = B z1, block projection and update, Cholesky QR of
-        // the pair into v_{j+1}, v_{j+2}, Hessenberg columns j and j+1 from the raw relation
+      // s-step: ns Krylov vectors per pass over V (z_1 = B v_j, z_{c+1} = B z_c, block projection and update,
+      // Cholesky QR of the block, Hessenberg columns j .. j+ns-1 from the raw relation); the tail of a cycle
+      // shorter than ns takes single steps
+      const int nb = (ns > 1 && j + ns <= m) ? ns : (ns > 1 && j + 2 <= m ? 2 : 1);
+      if (nb > 1) {
         const int32_t *a = W.active.as<int32_t>();
-        apply_B(Vj, ldv, W.w.as<X>(), a);
-        apply_B(W.w.as<X>(), n, W.w2.as<X>(), a);
-        launch_mdot2<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), W.w2.as<X>(), n, W.partial.as<double>(), nbm,
-                        a);
-        launch_reduce_ld(st, S, nbm, 2 * (j + 1), 2 * kMaxBasis, W.partial.as<double>(), W.C2.as<double>(),
-                         2 * kMaxBasis, nullptr, 0, a);
-        launch_mupdate2<X>(st, S, n, j + 1, W.V.as<X>(), ldv, W.w.as<X>(), W.w2.as<X>(), n, W.C2.as<double>(),
-                           kMaxBasis, W.gram.as<double>(), nblk, a);
-        launch_reduce4(st, S, nblk, W.gram.as<double>(), W.gram4.as<double>(), a);
-        launch_cholqr2<X>(st, S, n, W.gram4.as<double>(), W.w.as<X>(), W.w2.as<X>(), n, Vj + n, Vj + 2 * n, ldv,
-                          W.R2.as<double>(), a);
-        launch_hcols2(st, S, m, j, W.C2.as<double>(), kMaxBasis, W.R2.as<double>(), W.Hraw.as<double>(),
-                      W.Hh.as<double>(), W.nrm2.as<double>(), W.nrm2b.as<double>(), a);
-        launch_givens_n(st, S, m, j, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
-                        W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
-                        W.nrm2.as<double>());
-        launch_givens_n(st, S, m, j + 1, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
-                        W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
-                        W.nrm2b.as<double>());
-        j++;
-        if ((j & 7) == 7) {
+        X *Z = W.w2.as<X>();
+        apply_B(Vj, ldv, Z, a);
+        for (int cb = 1; cb < nb; cb++) apply_B(Z + (int64_t)(cb - 1) * S * n, n, Z + (int64_t)cb * S * n, a);
+        launch_mdot_s<X>(st, nb, S, n, j + 1, W.V.as<X>(), ldv, Z, W.partial.as<double>(), nbm, a);
+        launch_reduce_ld(st, S, nbm, nb * (j + 1), nb * kMaxBasis, W.partial.as<double>(), W.C2.as<double>(),
+                         nb * kMaxBasis, nullptr, 0, a);
+        launch_mupdate_s<X>(st, nb, S, n, j + 1, W.V.as<X>(), ldv, Z, W.C2.as<double>(), kMaxBasis,
+                            W.gram.as<double>(), nblk, a);
+        launch_reduce_small(st, S, nblk, nb * nb * 2, W.gram.as<double>(), W.gram4.as<double>(), a);
+        launch_cholqr_s<X>(st, nb, S, n, W.gram4.as<double>(), Z, Vj + n, ldv, W.R2.as<double>(), a);
+        launch_hcols_s(st, nb, S, m, j, W.C2.as<double>(), kMaxBasis, W.R2.as<double>(), W.Hraw.as<double>(),
+                       W.Hh.as<double>(), W.nrm2b.as<double>(), a);
+        for (int t = 0; t < nb; t++)
+          launch_givens_n(st, S, m, j + t, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
+                          W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
+                          W.nrm2b.as<double>() + (int64_t)t * S);
+        const int j0 = j;
+        j += nb - 1;
+        if ((j0 >> 3) != ((j + 1) >> 3)) {
           HIPCHK(hipMemcpyAsync(act.data(), W.active.p, S * 4, hipMemcpyDeviceToHost, st));
           HIPCHK(hipStreamSynchronize(st));
           bool any_a = false;
@@ -1928,8 +1928,8 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
     ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
     if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");
     ctx->poly_degree = in->poly_degree;
-    if (in->arnoldi_step < 0 || in->arnoldi_step > 2) throw std::invalid_argument("arnoldi_step must be 0, 1 or 2");
-    ctx->arnoldi_step = in->arnoldi_step == 2 ? 2 : 1;
+    if (in->arnoldi_step < 0 || in->arnoldi_step > 4) throw std::invalid_argument("arnoldi_step must be 0..4");
+    ctx->arnoldi_step = in->arnoldi_step > 1 ? in->arnoldi_step : 1;
     if (ctx->coarse_size != in->coarse_size) ctx->have_coarse = false;
     ctx->coarse_size = in->coarse_size;
     sweep(*ctx, in, out);
@@ -2043,42 +2043,61 @@ int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, doubl
   });
 }
 
-int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *avg_seconds, double *alg_bytes) {
+int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int ns, int nrep, double *avg_seconds,
+                      double *alg_bytes) {
   return guarded(ctx, [&] {
     if (!ctx->have_space) throw std::invalid_argument("space not built");
-    if (S < 1 || k < 1 || k + 1 > kMaxBasis || nrep < 1 || (prec != 0 && prec != 1))
+    if (S < 1 || k < 1 || k + ns > kMaxBasis || nrep < 1 || (prec != 0 && prec != 1) || ns < 1 || ns > 4)
       throw std::invalid_argument("bad krylov bench arguments");
     hipStream_t st = ctx->st;
     const int64_t n = ctx->sp.ndof;
     const int nblk = reduce_blocks(n), nbm = mdot_blocks(n);
     const size_t sv = prec == 0 ? 16 : 8;
-    const int64_t ldv = (int64_t)(k + 1) * n;
-    DevBuf V, w, part, h;
+    const int64_t ldv = (int64_t)(k + ns) * n;
+    DevBuf V, w, part, h, gram;
     V.alloc((size_t)S * ldv * sv);
-    w.alloc((size_t)S * n * sv);
-    part.alloc((size_t)S * std::max(nblk, mdot_blocks(n)) * kMaxBasis * 16);
-    h.alloc((size_t)S * kMaxBasis * 16);
+    w.alloc((size_t)ns * S * n * sv);
+    part.alloc((size_t)S * std::max(nblk, nbm) * kMaxBasis * 16 * ns);
+    h.alloc((size_t)S * kMaxBasis * 16 * ns);
+    gram.alloc((size_t)S * nblk * ns * ns * 16 + 64);
     if (prec == 0) {
       hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * ldv, V.as<double2>(), 1e-3);
-      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, w.as<double2>(), 1e-3);
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, (int64_t)ns * S * n, w.as<double2>(), 1e-3);
     } else {
       hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * ldv, V.as<float2>(), 1e-3);
-      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)S * n, w.as<float2>(), 1e-3);
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, (int64_t)ns * S * n, w.as<float2>(), 1e-3);
     }
-    HIPCHK(hipMemsetAsync(h.p, 0, (size_t)S * kMaxBasis * 16, st));
+    HIPCHK(hipMemsetAsync(h.p, 0, (size_t)S * kMaxBasis * 16 * ns, st));
     auto mdot = [&]() {
-      if (prec == 0)
-        launch_mdot<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, part.as<double>(), nbm, nullptr);
-      else
-        launch_mdot<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, part.as<double>(), nbm, nullptr);
+      if (ns == 1) {
+        if (prec == 0)
+          launch_mdot<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, part.as<double>(), nbm, nullptr);
+        else
+          launch_mdot<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, part.as<double>(), nbm, nullptr);
+      } else {
+        if (prec == 0)
+          launch_mdot_s<double2>(st, ns, S, n, k, V.as<double2>(), ldv, w.as<double2>(), part.as<double>(), nbm,
+                                 nullptr);
+        else
+          launch_mdot_s<float2>(st, ns, S, n, k, V.as<float2>(), ldv, w.as<float2>(), part.as<double>(), nbm, nullptr);
+      }
     };
     auto mupd = [&]() {
-      if (prec == 0)
-        launch_mupdate<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, h.as<double>(), kMaxBasis,
-                                part.as<double>(), nblk, nullptr);
-      else
-        launch_mupdate<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, h.as<double>(), kMaxBasis,
-                               part.as<double>(), nblk, nullptr);
+      if (ns == 1) {
+        if (prec == 0)
+          launch_mupdate<double2>(st, S, n, k, V.as<double2>(), ldv, w.as<double2>(), n, h.as<double>(), kMaxBasis,
+                                  part.as<double>(), nblk, nullptr);
+        else
+          launch_mupdate<float2>(st, S, n, k, V.as<float2>(), ldv, w.as<float2>(), n, h.as<double>(), kMaxBasis,
+                                 part.as<double>(), nblk, nullptr);
+      } else {
+        if (prec == 0)
+          launch_mupdate_s<double2>(st, ns, S, n, k, V.as<double2>(), ldv, w.as<double2>(), h.as<double>(),
+                                    kMaxBasis, gram.as<double>(), nblk, nullptr);
+        else
+          launch_mupdate_s<float2>(st, ns, S, n, k, V.as<float2>(), ldv, w.as<float2>(), h.as<double>(), kMaxBasis,
+                                   gram.as<double>(), nblk, nullptr);
+      }
     };
     hipEvent_t e0, e1, e2;
     HIPCHK(hipEventCreate(&e0));
@@ -2103,8 +2122,9 @@ int opem_krylov_bench(opem_ctx *ctx, int prec, int S, int k, int nrep, double *a
       avg_seconds[1] = m2 * 1e-3 / nrep;
     }
     if (alg_bytes) {
-      alg_bytes[0] = (double)S * (k + 1) * n * sv + (double)S * nbm * k * 16;
-      alg_bytes[1] = (double)S * (k + 2) * n * sv + (double)S * k * 16 + (double)S * nblk * 8;
+      const int ngram = ns == 1 ? 1 : ns * ns * 2;
+      alg_bytes[0] = (double)S * (k + ns) * n * sv + (double)S * nbm * k * ns * 16;
+      alg_bytes[1] = (double)S * (k + 2 * ns) * n * sv + (double)S * k * ns * 16 + (double)S * nblk * ngram * 8;
     }
   });
 }

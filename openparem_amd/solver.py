@@ -115,11 +115,12 @@ class Solver:
                                     C.byref(b)), self.ctx, "opem_spmv_bench")
         return t.value, b.value
 
-    def krylov_bench(self, precision="fp32", S=128, k=76, nrep=5):
-        """((mdot s, mupdate s), (mdot bytes, mupdate bytes)) of the batched Gram-Schmidt kernels."""
+    def krylov_bench(self, precision="fp32", S=128, k=76, nrep=5, ns=1):
+        """((projection s, update s), (projection bytes, update bytes)) of the batched Gram-Schmidt kernels with
+        ns new vectors per pass."""
         t = (C.c_double * 2)()
         b = (C.c_double * 2)()
-        check(lib().opem_krylov_bench(self.ctx, 0 if precision == "fp64" else 1, S, k, nrep, t, b), self.ctx,
+        check(lib().opem_krylov_bench(self.ctx, 0 if precision == "fp64" else 1, S, k, ns, nrep, t, b), self.ctx,
               "opem_krylov_bench")
         return (t[0], t[1]), (b[0], b[1])
 

@@ -284,15 +284,17 @@ def test_polynomial_preconditioner(nine_point_fp64, deg, prec):
     assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
 
 
-@pytest.mark.parametrize("prec,restart", [("mixed", 150), ("mixed", 31), ("fp64", 150)])
-def test_two_step_arnoldi(nine_point_fp64, prec, restart):
-    """Two Krylov vectors per Gram-Schmidt pass (block projection, Cholesky QR of the pair, Hessenberg columns
-    from the raw two-step relation; restart 31 ends each cycle on a single step): same S as one-at-a-time
-    FP64 with the one-step Arnoldi, and a comparable iteration count."""
+@pytest.mark.parametrize("step,prec,restart", [(2, "mixed", 150), (2, "mixed", 31), (2, "fp64", 150),
+                                               (3, "mixed", 150), (3, "mixed", 31), (4, "mixed", 150),
+                                               (4, "mixed", 30), (4, "fp64", 150)])
+def test_block_arnoldi(nine_point_fp64, step, prec, restart):
+    """s Krylov vectors per Gram-Schmidt pass (block projection, Cholesky QR of the block, Hessenberg columns from
+    the raw s-step relation; restarts 30 and 31 end cycles on shorter blocks or single steps): same S as
+    one-at-a-time FP64 with the one-step Arnoldi, and a comparable iteration count."""
     m, f, r1 = nine_point_fp64
     s = _solver(m, 2)
     try:
-        rb = s.sweep(f, precision=prec, tol=1e-10, batch=9, orth_passes=1, precond_share=2, arnoldi_step=2,
+        rb = s.sweep(f, precision=prec, tol=1e-10, batch=9, orth_passes=1, precond_share=2, arnoldi_step=step,
                      restart=restart)
     finally:
         s.close()
