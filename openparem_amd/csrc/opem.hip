@@ -96,6 +96,18 @@ struct PortTrace {
   std::vector<double> phi;              // npts*ndf*3 basis values
 };
 
+// Device workspace of the batched solve.  The context keeps it between sweeps (buffers only grow), so repeated
+// sweeps make no device allocations for the Krylov basis and the other per-batch arrays; opem_destroy frees it.
+struct Work {
+  DevBuf A64, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
+      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch, pinv, pprod, pt, pu, w2, C2, gram, gram4, R2, Hraw,
+      nrm2b;
+  // batched operator: frequency-independent matrices (per sweep) and per-system coefficients (per batch)
+  SweepMatrix<double> M64{};
+  SweepMatrix<float> M32{};
+  DevBuf csys64, dsys64, csys32, dsys32;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -135,6 +147,7 @@ struct Ctx {
   int poly_degree = 0;    // GMRES-polynomial preconditioner degree + 1 (0: off)
   int arnoldi_step = 1;   // 2: two new Krylov vectors per orthogonalisation pass
   int lpr = 16;
+  Work work;  // solver workspace, kept between sweeps
 };
 
 // ------------------------------------------------------------------ small kernels
@@ -723,16 +736,6 @@ static void device_space(Ctx &c) {
 }
 
 // ------------------------------------------------------------------ batched solve
-struct Work {
-  // sized per batch
-  DevBuf A64, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
-      hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch, pinv, pprod, pt, pu, w2, C2, gram, gram4, R2, Hraw,
-      nrm2b;
-  // batched operator: frequency-independent matrices (per sweep) and per-system coefficients (per batch)
-  SweepMatrix<double> M64{};
-  SweepMatrix<float> M32{};
-  DevBuf csys64, dsys64, csys32, dsys32;
-};
 
 // y = mask .* (A_s x) (mode 0) or mask .* (-A_s x) (mode 1) for the batch, x system-interleaved
 template <typename X>
@@ -1399,7 +1402,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     for (int64_t i = 0; i < n; i++) masks[(size_t)r * n + i] = !pec[i];
     for (int32_t d : c.ports[port_of[r]].dofs) masks[(size_t)r * n + d] = 0;
   }
-  Work W;
+  Work &W = c.work;
   W.masks.upload(masks, st);
   W.M64 = sop.m64(c);
   W.M32 = sop.m32(c);
