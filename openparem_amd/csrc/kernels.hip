@@ -1942,7 +1942,8 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
                                                      const float *__restrict__ scale, const X *__restrict__ v,
                                                      int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
                                                      const int32_t *__restrict__ active) {
-  using T = typename std::conditional<std::is_same<P, __half2>::value, float, double>::type;
+  // FP32 arithmetic for FP32 vectors; FP64 arithmetic (a linear operator to FP64 rounding) for FP64 vectors
+  using T = typename std::conditional<std::is_same<X, float2>::value, float, double>::type;
   const int p = blockIdx.x, q = blockIdx.y;
   const int g0 = q * L.pg;
   const int ns = min(L.pg, G - g0) * R;
@@ -2141,7 +2142,7 @@ void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R
   const int S = ngroups * R;
   const unsigned gblocks = (unsigned)std::min<int64_t>(65536, (n * S + 255) / 256);
   if (nsm > 8) throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
-  if constexpr (std::is_same<P, __half2>::value) {
+  if constexpr (std::is_same<P, __half2>::value && std::is_same<X, float2>::value) {
     if (patch_mfma_enabled()) {
       hipLaunchKernelGGL((k_patch_local_mfma<X>), grid, dim3(64), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
                          (int64_t)S, active);
@@ -2172,6 +2173,9 @@ template void launch_patch_apply<double2, double2>(hipStream_t, const PatchLayou
 template void launch_patch_apply<__half2, float2>(hipStream_t, const PatchLayout &, int, int, int64_t,
                                                   const __half2 *, const float *, const float2 *, int64_t, float2 *,
                                                   const int32_t *, void *);
+template void launch_patch_apply<__half2, double2>(hipStream_t, const PatchLayout &, int, int, int64_t,
+                                                   const __half2 *, const float *, const double2 *, int64_t,
+                                                   double2 *, const int32_t *, void *);
 
 // ------------------------------------------------------------------ coarse space
 template <typename X>

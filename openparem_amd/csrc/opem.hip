@@ -147,6 +147,12 @@ struct Ctx {
   int poly_degree = 0;    // GMRES-polynomial preconditioner degree + 1 (0: off)
   int arnoldi_step = 1;   // 2: two new Krylov vectors per orthogonalisation pass
   int lpr = 16;
+  // vertex patches over all DOFs for the H-projection CG: FP16 inverses of the unit mass P (built once per space)
+  Patches hpatches;
+  int hpatch_state = 0;  // 0 not built, 1 built, -1 unavailable (a patch above 100 DOFs)
+  DevBuf dh_idx_off, dh_idx, dh_val_off, dh_nz_off, dh_port, dh_nz, dh_d2p_ptr, dh_d2p_patch, dh_d2p_row, dh_yoff,
+      dh_inv, dh_scale;
+  int64_t hpatch_size = 0;
   Work work;  // solver workspace, kept between sweeps
 };
 
@@ -1093,6 +1099,76 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
 }
 
 // H = P^{-1} (j/w) Qs x  by batched CG with point Jacobi on P (fem3D.cpp:610-648)
+static PatchLayout h_layout(const Ctx &c) {
+  PatchLayout L{};
+  L.idx_off = c.dh_idx_off.as<int32_t>();
+  L.idx = c.dh_idx.as<int32_t>();
+  L.val_off = c.dh_val_off.as<int64_t>();
+  L.nz_off = c.dh_nz_off.as<int64_t>();
+  L.port = c.dh_port.as<uint8_t>();
+  L.pnz = c.dh_nz.as<int32_t>();
+  L.d2p_ptr = c.dh_d2p_ptr.as<int32_t>();
+  L.d2p_patch = c.dh_d2p_patch.as<int32_t>();
+  L.d2p_row = c.dh_d2p_row.as<int32_t>();
+  L.yoff = c.dh_yoff.as<int64_t>();
+  L.group_stride = 0;
+  L.shared_size = c.hpatch_size;
+  L.exc_size = 0;
+  L.npatch = c.hpatches.npatch;
+  L.ytot = (int64_t)c.hpatches.idx.size();
+  L.nmax = c.hpatches.nmax;
+  L.pg = 8;
+  return L;
+}
+
+// Additive Schwarz over vertex patches of all DOFs for the mass P (SPD): FP16-stored symmetrised inverses applied
+// in FP64 arithmetic, so the preconditioner is one fixed symmetric positive definite operator, as CG requires.
+// All systems share the single inverse set (group stride 0); the block scales are replicated for the 32 groups of
+// up to 8 systems a batch of at most 256 systems has.
+static void build_h_patches(Ctx &c) {
+  if (c.hpatch_state != 0) return;
+  const Space &s = c.sp;
+  hipStream_t st = c.st;
+  int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  build_patches(s, std::vector<uint8_t>(s.ndof, 0), c.hpatches, nth);
+  if (c.hpatches.nmax > 100) {
+    c.hpatch_state = -1;
+    return;
+  }
+  const Patches &P = c.hpatches;
+  const int64_t npch = P.npatch;
+  std::vector<int64_t> voff(npch), yoff(npch);
+  int64_t sz = 0;
+  for (int64_t p = 0; p < npch; p++) {
+    const int64_t np_ = P.idx_off[p + 1] - P.idx_off[p];
+    voff[p] = sz;
+    sz += np_ * ((np_ + 7) / 8 * 8);
+    yoff[p] = P.idx_off[p];
+  }
+  c.hpatch_size = sz;
+  c.dh_idx_off.upload(P.idx_off, st);
+  c.dh_idx.upload(P.idx, st);
+  c.dh_val_off.upload(voff, st);
+  c.dh_nz_off.upload(P.val_off, st);
+  c.dh_port.upload(std::vector<uint8_t>(npch, 0), st);
+  c.dh_nz.upload(P.nz, st);
+  c.dh_d2p_ptr.upload(P.d2p_ptr, st);
+  c.dh_d2p_patch.upload(P.d2p_patch, st);
+  c.dh_d2p_row.upload(P.d2p_row, st);
+  c.dh_yoff.upload(yoff, st);
+  c.dh_inv.alloc((size_t)sz * sizeof(__half2));
+  constexpr int kGroups = 32;
+  c.dh_scale.alloc((size_t)kGroups * npch * sizeof(float));
+  PatchLayout HL = h_layout(c);
+  launch_patch_build<__half2>(st, HL, 1, 1, c.d_P.as<double>(), s.nnz, c.d_onesmask.as<uint8_t>(), s.ndof,
+                              c.dh_inv.as<__half2>(), c.dh_scale.as<float>());
+  for (int q = 1; q < kGroups; q++)
+    HIPCHK(hipMemcpyAsync(c.dh_scale.as<float>() + (int64_t)q * npch, c.dh_scale.p, npch * sizeof(float),
+                          hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  c.hpatch_state = 1;
+}
+
 static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, double htol, DevBuf &h) {
   const Space &s = c.sp;
   hipStream_t st = c.st;
@@ -1131,6 +1207,25 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   const SweepMatrix<double> MP{n, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), pbase.as<double>(),
                                pbase.as<double>() + nnz, nullptr, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
                                pbase.as<double>(), 0, 0, blk16_max(s)};
+  // preconditioner: vertex patches of P when every patch fits (S <= 256), else point Jacobi
+  build_h_patches(c);
+  const bool hp = c.hpatch_state == 1 && S <= 256;
+  const PatchLayout HL = hp ? h_layout(c) : PatchLayout{};
+  DevBuf hy, hzil;
+  if (hp) {
+    hy.alloc((size_t)S * HL.ytot * 16);
+    hzil.alloc((size_t)S * n * 16);
+  }
+  auto hprec = [&](const double2 *rr, double2 *zz, const int32_t *a) {
+    if (hp) {
+      launch_patch_apply<__half2, double2>(st, HL, S, 1, n, c.dh_inv.as<__half2>(), c.dh_scale.as<float>(), rr, n,
+                                           hzil.as<double2>(), a, (void *)hy.p);
+      launch_transpose<double2>(st, n, S, hzil.as<double2>(), zz, false);
+    } else {
+      hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(), rr, zz,
+                         a);
+    }
+  };
   // b = Qs x, then scale by j/w per system
   launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), c.d_Qs.as<double2>(),
                                 nnz, c.d_onesmask.as<uint8_t>(), W.x.as<double2>(), b.as<double2>(), nullptr, nullptr,
@@ -1150,8 +1245,7 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
   // h = 0, r = b, z = M r, p = z, rz = r.z
   HIPCHK(hipMemsetAsync(h.p, 0, (size_t)S * n * 16, st));
   HIPCHK(hipMemcpyAsync(r.p, b.p, (size_t)S * n * 16, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(),
-                     r.as<double2>(), z.as<double2>(), nullptr);
+  hprec(r.as<double2>(), z.as<double2>(), nullptr);
   HIPCHK(hipMemcpyAsync(p.p, z.p, (size_t)S * n * 16, hipMemcpyDeviceToDevice, st));
   launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nbm, nullptr);
   launch_reduce(st, S, nbm, 1, part.as<double>(), rz.as<double>(), 1, nullptr, 0, nullptr);
@@ -1191,8 +1285,7 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
       HIPCHK(hipMemcpyAsync(act.p, a.data(), S * 4, hipMemcpyHostToDevice, st));
       if (!any) break;
     }
-    hipLaunchKernelGGL(k_diag_apply, dim3(blocks_for(n), S), dim3(256), 0, st, n, c.d_Pdinv.as<double>(),
-                       r.as<double2>(), z.as<double2>(), act.as<int32_t>());
+    hprec(r.as<double2>(), z.as<double2>(), act.as<int32_t>());
     launch_mdot<double2>(st, S, n, 1, r.as<double2>(), n, z.as<double2>(), n, part.as<double>(), nbm,
                          act.as<int32_t>());
     launch_reduce(st, S, nbm, 1, part.as<double>(), t.as<double>(), 1, nullptr, 0, act.as<int32_t>());
@@ -1736,6 +1829,9 @@ int opem_space_build(opem_ctx *ctx, int order, int64_t *ndof, int64_t *nnz) {
     build_space(ctx->mesh, order, ctx->sp, nth);
     device_space(*ctx);
     ctx->have_space = true;
+    ctx->have_patches = false;  // patch and coarse structures belong to the previous space
+    ctx->have_coarse = false;
+    ctx->hpatch_state = 0;
     if (ndof) *ndof = ctx->sp.ndof;
     if (nnz) *nnz = ctx->sp.nnz;
   });
