@@ -1071,30 +1071,54 @@ __global__ __launch_bounds__(kRedThreads) void k_mdot_s(int64_t n, int k, const 
                                                         const X *__restrict__ Z, double *__restrict__ partial,
                                                         int nblk, const int32_t *__restrict__ active) {
   constexpr int VB = NS >= 3 ? 1 : 4 / NS;
+  // basis groups whose loads are issued together before any of them is used (FP32 basis: two groups, so a
+  // lane has 2*VB*E loads in flight; FP64 keeps one group for registers)
+  constexpr int U = std::is_same<X, float2>::value ? 2 : 1;
   const int s = blockIdx.y, S = gridDim.y;
   if (active && !active[s]) return;
   const int64_t base = (int64_t)blockIdx.x * (kRedThreads * E) + threadIdx.x;
   const X *Vs = Vb + s * ldv_sys;
+  // Element offsets clamped into [0, n): lanes past the end read a valid entry and multiply it by z = 0, so
+  // the loads carry no branch and are issued back to back.
+  uint32_t off[E];
   X w[NS][E];
 #pragma unroll
-  for (int c = 0; c < NS; c++)
+  for (int e = 0; e < E; e++) {
+    const int64_t i = base + e * kRedThreads;
+    off[e] = (uint32_t)(i < n ? i : n - 1);
+  }
+#pragma unroll
+  for (int c = 0; c < NS; c++) {
+    const X *zc = Z + ((int64_t)c * S + s) * n;
 #pragma unroll
     for (int e = 0; e < E; e++) {
-      const int64_t i = base + e * kRedThreads;
-      X zero;
-      zero.x = 0;
-      zero.y = 0;
-      w[c][e] = i < n ? Z[((int64_t)c * S + s) * n + i] : zero;
+      X t = zc[off[e]];
+      if (base + e * kRedThreads >= n) t.x = t.y = 0;
+      w[c][e] = t;
     }
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
-  for (int v0 = 0; v0 < k; v0 += VB) {
-    double ax[4], ay[4];  // slot q = (v - v0) * NS + c
+  for (int v0 = 0; v0 < k; v0 += U * VB) {
+    X a[U][VB][E];
 #pragma unroll
-    for (int q = 0; q < 4; q++) ax[q] = ay[q] = 0.0;
+    for (int u = 0; u < U; u++)
 #pragma unroll
-    for (int dv = 0; dv < VB; dv++) {
-      if (v0 + dv < k) {
+      for (int dv = 0; dv < VB; dv++) {
+        const int v = min(v0 + u * VB + dv, k - 1);  // the tail re-reads the last vector; its sums are dropped
+        const X *vp = Vs + (int64_t)v * n;
+#pragma unroll
+        for (int e = 0; e < E; e++) a[u][dv][e] = vp[off[e]];
+      }
+    if constexpr (U > 1) __builtin_amdgcn_sched_barrier(0);  // keep every group's loads ahead of the sums
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int vu = v0 + u * VB;
+      double ax[4], ay[4];  // slot q = (v - vu) * NS + c
+#pragma unroll
+      for (int q = 0; q < 4; q++) ax[q] = ay[q] = 0.0;
+#pragma unroll
+      for (int dv = 0; dv < VB; dv++) {
         if constexpr (std::is_same<X, float2>::value) {
           // FP32 basis: the E products of a lane are summed in FP32, the lane sums in FP64
           float fx[NS], fy[NS];
@@ -1102,60 +1126,54 @@ __global__ __launch_bounds__(kRedThreads) void k_mdot_s(int64_t n, int k, const 
           for (int c = 0; c < NS; c++) fx[c] = fy[c] = 0.f;
 #pragma unroll
           for (int e = 0; e < E; e++) {
-            const int64_t i = base + e * kRedThreads;
-            if (i < n) {
-              const float2 a = Vs[(int64_t)(v0 + dv) * n + i];
+            const float2 t = a[u][dv][e];
 #pragma unroll
-              for (int c = 0; c < NS; c++) {
-                fx[c] += a.x * w[c][e].x + a.y * w[c][e].y;
-                fy[c] += a.x * w[c][e].y - a.y * w[c][e].x;
-              }
+            for (int c = 0; c < NS; c++) {
+              fx[c] += t.x * w[c][e].x + t.y * w[c][e].y;
+              fy[c] += t.x * w[c][e].y - t.y * w[c][e].x;
             }
           }
 #pragma unroll
           for (int c = 0; c < NS; c++) {
-            ax[dv * NS + c] += (double)fx[c];
-            ay[dv * NS + c] += (double)fy[c];
+            ax[dv * NS + c] = (double)fx[c];
+            ay[dv * NS + c] = (double)fy[c];
           }
         } else {
 #pragma unroll
           for (int e = 0; e < E; e++) {
-            const int64_t i = base + e * kRedThreads;
-            if (i < n) {
-              const double2 a = to_d2(Vs[(int64_t)(v0 + dv) * n + i]);
+            const double2 t = to_d2(a[u][dv][e]);
 #pragma unroll
-              for (int c = 0; c < NS; c++) {
-                const double2 b = to_d2(w[c][e]);
-                ax[dv * NS + c] += a.x * b.x + a.y * b.y;
-                ay[dv * NS + c] += a.x * b.y - a.y * b.x;
-              }
+            for (int c = 0; c < NS; c++) {
+              const double2 b = to_d2(w[c][e]);
+              ax[dv * NS + c] += t.x * b.x + t.y * b.y;
+              ay[dv * NS + c] += t.x * b.y - t.y * b.x;
             }
           }
         }
       }
-    }
-    double bx[2], by[2];
+      double bx[2], by[2];
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const double kx = h32 ? ax[t + 2] : ax[t], ky = h32 ? ay[t + 2] : ay[t];
-      const double sx = h32 ? ax[t] : ax[t + 2], sy = h32 ? ay[t] : ay[t + 2];
-      bx[t] = kx + __shfl_xor(sx, 32, 64);
-      by[t] = ky + __shfl_xor(sy, 32, 64);
-    }
-    double cx = (h16 ? bx[1] : bx[0]) + __shfl_xor(h16 ? bx[0] : bx[1], 16, 64);
-    double cy = (h16 ? by[1] : by[0]) + __shfl_xor(h16 ? by[0] : by[1], 16, 64);
+      for (int t = 0; t < 2; t++) {
+        const double kx = h32 ? ax[t + 2] : ax[t], ky = h32 ? ay[t + 2] : ay[t];
+        const double sx = h32 ? ax[t] : ax[t + 2], sy = h32 ? ay[t] : ay[t + 2];
+        bx[t] = kx + __shfl_xor(sx, 32, 64);
+        by[t] = ky + __shfl_xor(sy, 32, 64);
+      }
+      double cx = (h16 ? bx[1] : bx[0]) + __shfl_xor(h16 ? bx[0] : bx[1], 16, 64);
+      double cy = (h16 ? by[1] : by[0]) + __shfl_xor(h16 ? by[0] : by[1], 16, 64);
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      cx += __shfl_xor(cx, o, 64);
-      cy += __shfl_xor(cy, o, 64);
-    }
-    const int q = (h32 ? 2 : 0) + (h16 ? 1 : 0);
-    const int v = v0 + q / NS, c = q % NS;
-    if ((lane & 15) == 0 && q < VB * NS && v < k) {
-      double *p =
-          partial + ((((int64_t)s * nblk + blockIdx.x * (kRedThreads / 64) + wave) * kMaxK + v) * NS + c) * 2;
-      p[0] = cx;
-      p[1] = cy;
+      for (int o = 8; o > 0; o >>= 1) {
+        cx += __shfl_xor(cx, o, 64);
+        cy += __shfl_xor(cy, o, 64);
+      }
+      const int q = (h32 ? 2 : 0) + (h16 ? 1 : 0);
+      const int v = vu + q / NS, c = q % NS;
+      if ((lane & 15) == 0 && q < VB * NS && v < k) {
+        double *p =
+            partial + ((((int64_t)s * nblk + blockIdx.x * (kRedThreads / 64) + wave) * kMaxK + v) * NS + c) * 2;
+        p[0] = cx;
+        p[1] = cy;
+      }
     }
   }
 }
@@ -1165,6 +1183,7 @@ void launch_mdot_s(hipStream_t st, int ns, int S, int64_t n, int k, const X *Vb,
                    double *partial, int nblk, const int32_t *active) {
   if (k > kMaxK) throw std::runtime_error("restart too large");
   if (nblk != mdot_blocks(n)) throw std::runtime_error("mdot_s: partial block count mismatch");
+  if (n < 1 || n > INT32_MAX) throw std::runtime_error("mdot_s: vector length out of range");
   const dim3 grid(mdot_grid(n), S);
   if (ns == 2)
     hipLaunchKernelGGL((k_mdot_s<X, kMdotElems, 2>), grid, dim3(kRedThreads), 0, st, n, k, Vb, ldv_sys, Z, partial,
@@ -1215,7 +1234,23 @@ __global__ __launch_bounds__(kRedThreads) void k_mupdate_s(int64_t n, int k, con
       double2 p[NS];
 #pragma unroll
       for (int c = 0; c < NS; c++) p[c] = to_d2(Z[((int64_t)c * S + s) * n + i]);
-      for (int v = 0; v < k; v++) {
+      // UV basis entries are loaded before any is used, so a lane keeps UV loads in flight
+      constexpr int UV = 8;
+      int v = 0;
+      for (; v + UV <= k; v += UV) {
+        X a[UV];
+#pragma unroll
+        for (int u = 0; u < UV; u++) a[u] = Vs[(int64_t)(v + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < UV; u++)
+#pragma unroll
+          for (int c = 0; c < NS; c++) {
+            const double2 t = cmul(to_d2(a[u]), hs[c][v + u]);
+            p[c].x -= t.x;
+            p[c].y -= t.y;
+          }
+      }
+      for (; v < k; v++) {
         const double2 a = to_d2(Vs[(int64_t)v * n + i]);
 #pragma unroll
         for (int c = 0; c < NS; c++) {
