@@ -131,7 +131,7 @@ def main():
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--restart", type=int, default=300)
     ap.add_argument("--coarse", type=int, default=0)
-    ap.add_argument("--share", type=int, default=4, help="adjacent frequencies sharing one set of patch inverses")
+    ap.add_argument("--share", type=int, default=16, help="adjacent frequencies sharing one set of patch inverses")
     ap.add_argument("--poly", type=int, default=0, help="GMRES-polynomial preconditioner degree (0: off)")
     ap.add_argument("--arnoldi-step", type=int, default=3, help="Krylov vectors per Gram-Schmidt pass (1..4)")
     args = ap.parse_args()

@@ -137,7 +137,8 @@ typedef struct {
   int coarse_size;              /* aggregates of the gradient coarse level (0 = one-level, <= 128) */
   int orth_passes;              /* classical Gram-Schmidt passes per Arnoldi step: 2 (default) or 1 */
   int precond_share;            /* vertex patches: adjacent frequencies of a batch sharing one set of */
-                                /*    patch inverses, built from their mean matrix (0 -> 1, <= 8 / nports) */
+                                /*    patch inverses, built from their mean matrix (0 -> 1; x nports <= 32 */
+                                /*    in mixed precision, <= 8 in FP64)                          */
   int poly_degree;              /* d > 0: GMRES-polynomial preconditioning, the Krylov space built with  */
                                 /*    B p(B) (B = A M, deg p = d - 1, roots from a d-step Arnoldi run per */
                                 /*    system); each iteration applies B d times (0 = off, <= 32)       */

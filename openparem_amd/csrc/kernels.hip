@@ -2044,35 +2044,37 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
   }
 }
 
-// FP16 storage on MFMA: one wavefront per (patch, preconditioner group).  The complex product is two real
-// products, [Yr | Yi] = Ar [Vr | Vi] + Ai [-Vi | Vr], an (n_p x n_p) x (n_p x 2ns) GEMM on
-// v_mfma_f32_16x16x32_f16 (FP32 accumulation): column j < ns is Re of system j, ns <= j < 2ns is Im of
-// system j - ns.  Each system's patch vector is scaled to max |Re|,|Im| = 1 and split into two halves,
-// hi = f16(v), lo = f16(v - hi), so the operand keeps about 22 bits; the inverse is exact in its FP16
-// store.  Operand layouts (16x16x32, 8 values per lane): A[i][k] in lane i%16 + 16*(k%32/8), B[k][j] in
-// lane j + 16*(k%32/8), D[i][j] in register i%4 of lane j + 16*(i/4).
+// FP16 storage on MFMA: one workgroup per (patch, preconditioner group), one wavefront per 8 systems of the
+// group (up to four wavefronts read the same inverse on one CU, so the block comes from HBM once).  The complex product is two real products, [Yr | Yi] = Ar [Vr | Vi] + Ai [-Vi | Vr], an
+// (n_p x n_p) x (n_p x 2ns) GEMM on v_mfma_f32_16x16x32_f16 (FP32 accumulation): column j < ns is Re of
+// system j, ns <= j < 2ns is Im of system j - ns (ns <= 8 systems of the wavefront).  Each system's patch
+// vector is scaled to max |Re|,|Im| = 1 and split into two halves, hi = f16(v), lo = f16(v - hi), so the
+// operand keeps about 22 bits; the inverse is exact in its FP16 store.  Operand layouts (16x16x32, 8 values
+// per lane): A[i][k] in lane i%16 + 16*(k%32/8), B[k][j] in lane j + 16*(k%32/8), D[i][j] in register i%4 of
+// lane j + 16*(i/4).
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef float f4v_t __attribute__((ext_vector_type(4)));
+constexpr int kPatchMfmaSys = 32;  // systems per (patch, group) on the MFMA path: up to four wavefronts
 
 template <typename X>
-__global__ __launch_bounds__(64) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
-                                                         const float *__restrict__ scale, const X *__restrict__ v,
-                                                         int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
-                                                         const int32_t *__restrict__ active) {
+__global__ __launch_bounds__(256) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
+                                                          const float *__restrict__ scale, const X *__restrict__ v,
+                                                          int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
+                                                          const int32_t *__restrict__ active) {
   const int p = blockIdx.x, q = blockIdx.y;
   const int g0 = q * L.pg;
-  const int ns = min(L.pg, G - g0) * R;  // <= 8
+  const int ns = min(L.pg, G - g0) * R;  // <= 8 * (blockDim.x / 64)
   const int64_t s0 = (int64_t)g0 * R;
   bool any = false;
   for (int k = 0; k < ns; k++) any |= active == nullptr || active[s0 + k] != 0;
-  if (!any) return;
-  __shared__ float2 vs[8][128];
-  __shared__ unsigned int mx[8];
-  const int lane = threadIdx.x;
+  if (!any) return;  // uniform over the workgroup
+  __shared__ float2 vs[kPatchMfmaSys][128];
+  __shared__ unsigned int mx[kPatchMfmaSys];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0, npk = pad8(np);
-  if (lane < 8) mx[lane] = 0u;
+  if (threadIdx.x < kPatchMfmaSys) mx[threadIdx.x] = 0u;
   __syncthreads();
-  for (int e = lane; e < ns * 128; e += 64) {
+  for (int e = threadIdx.x; e < ns * 128; e += blockDim.x) {
     const int sy = e >> 7, t = e & 127;
     float2 val = make_float2(0.f, 0.f);
     if (t < np) {
@@ -2083,9 +2085,12 @@ __global__ __launch_bounds__(64) void k_patch_local_mfma(PatchLayout L, int G, i
     atomicMax(&mx[sy], __float_as_uint(fmaxf(fabsf(val.x), fabsf(val.y))));
   }
   __syncthreads();
+  // this wavefront's systems: [8 wv, 8 wv + nw) of the group
+  const int nw = min(8, ns - 8 * wv);
   const int j = lane & 15, kq = lane >> 4;
-  const int sy = j < 2 * ns ? (j < ns ? j : j - ns) : -1;
-  const int comp = j < ns ? 0 : 1;
+  const int sl = j < 2 * nw ? (j < nw ? j : j - nw) : -1;
+  const int sy = sl >= 0 ? 8 * wv + sl : -1;
+  const int comp = j < nw ? 0 : 1;
   const float ms = sy >= 0 ? __uint_as_float(mx[sy]) : 0.f;
   const float ims = ms > 0.f ? 1.f / ms : 0.f;
   const int nkb = (npk + 31) >> 5;  // <= 4 for n_p <= 100
@@ -2108,6 +2113,7 @@ __global__ __launch_bounds__(64) void k_patch_local_mfma(PatchLayout L, int G, i
       b2l[kb][e] = (_Float16)(c2 - (float)h2);
     }
   }
+  if (nw <= 0) return;  // no barrier follows
   const int nreg = L.port[p] ? R : 1;
   for (int ri = 0; ri < nreg; ri++) {
     const _Float16 *Ar = (const _Float16 *)(inv + patch_offset(L, p, q, ri));
@@ -2116,21 +2122,25 @@ __global__ __launch_bounds__(64) void k_patch_local_mfma(PatchLayout L, int G, i
     const int64_t s = s0 + sy;
     const bool mine = sy >= 0 && (nreg == 1 || sy % R == ri) && (active == nullptr || active[s] != 0);
     for (int mt = 0; mt < (np + 15) >> 4; mt++) {
-      const int i = mt * 16 + j;
+      // rows past n_p read row n_p - 1 (their outputs are not stored); columns past the padded width are
+      // never read (kb < nkb), and padded columns are zero in the store
+      const int i = min(mt * 16 + j, np - 1);
+      h8_t ar[4], ai[4];
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++)
+        if (kb < nkb) {
+          const int k0 = min(kb * 32 + kq * 8, npk - 8);  // k0 >= npk meets zero operands in B
+          ar[kb] = *(const h8_t *)(Ar + (int64_t)i * npk + k0);
+          ai[kb] = *(const h8_t *)(Ai + (int64_t)i * npk + k0);
+        }
       f4v_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kb = 0; kb < 4; kb++) {
         if (kb < nkb) {
-          const int k0 = kb * 32 + kq * 8;
-          h8_t ar = {0, 0, 0, 0, 0, 0, 0, 0}, ai = {0, 0, 0, 0, 0, 0, 0, 0};
-          if (i < np && k0 < npk) {
-            ar = *(const h8_t *)(Ar + (int64_t)i * npk + k0);
-            ai = *(const h8_t *)(Ai + (int64_t)i * npk + k0);
-          }
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, b1h[kb], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, b1l[kb], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, b2h[kb], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, b2l[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[kb], b1h[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[kb], b1l[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[kb], b2h[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[kb], b2l[kb], acc, 0, 0, 0);
         }
       }
       if (mine) {
@@ -2176,17 +2186,20 @@ void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R
   X *y = (X *)ybuf;
   const int S = ngroups * R;
   const unsigned gblocks = (unsigned)std::min<int64_t>(65536, (n * S + 255) / 256);
-  if (nsm > 8) throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
   if constexpr (std::is_same<P, __half2>::value && std::is_same<X, float2>::value) {
     if (patch_mfma_enabled()) {
-      hipLaunchKernelGGL((k_patch_local_mfma<X>), grid, dim3(64), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
-                         (int64_t)S, active);
+      if (nsm > kPatchMfmaSys)
+        throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 32");
+      const unsigned threads = 64u * (unsigned)((nsm + 7) / 8);
+      hipLaunchKernelGGL((k_patch_local_mfma<X>), grid, dim3(threads), 0, st, L, ngroups, R, inv, scale, v, ldv_sys,
+                         y, (int64_t)S, active);
       HIP_LAUNCH_CHECK();
       hipLaunchKernelGGL((k_patch_gather<X>), dim3(gblocks), dim3(256), 0, st, L, n, S, y, z, active);
       HIP_LAUNCH_CHECK();
       return;
     }
   }
+  if (nsm > 8) throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
   if (nsm <= 2)
     hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
                        (int64_t)S, active);

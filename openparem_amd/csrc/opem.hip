@@ -810,7 +810,8 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     PL.npatch = c.patches.npatch;
     PL.nmax = c.patches.nmax;
     PL.pg = std::max(1, std::min(c.precond_share, G));
-    if (PL.pg * R > 8) throw std::invalid_argument("precond_share x nports must be <= 8");
+    if (PL.pg * R > (mixed ? 32 : 8))
+      throw std::invalid_argument(mixed ? "precond_share x nports must be <= 32" : "precond_share x nports must be <= 8");
     const int nq = (G + PL.pg - 1) / PL.pg;
     W.inv.alloc((size_t)nq * PL.group_stride * sizeof(PP));
     float *scale = nullptr;
@@ -2021,8 +2022,11 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
       throw std::invalid_argument("bad precision");
     if (in->precond < 0 || in->precond > 1) throw std::invalid_argument("bad preconditioner");
     if (in->coarse_size < 0 || in->coarse_size > 128) throw std::invalid_argument("coarse_size must be 0..128");
-    if (in->precond_share < 0 || in->precond_share * in->nports > 8)
-      throw std::invalid_argument("precond_share x nports must be <= 8");
+    // mixed precision applies the FP16 patch inverses on MFMA, up to 32 systems per inverse set
+    const int share_max = in->precision == OPEM_PREC_MIXED ? 32 : 8;
+    if (in->precond_share < 0 || in->precond_share * in->nports > share_max)
+      throw std::invalid_argument(share_max == 32 ? "precond_share x nports must be <= 32"
+                                                  : "precond_share x nports must be <= 8");
     ctx->precond = in->precond;
     ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
     if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");

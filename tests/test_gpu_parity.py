@@ -256,10 +256,13 @@ def nine_point_fp64():
     return m, f, r1
 
 
-@pytest.mark.parametrize("share,prec", [(1, "mixed"), (2, "mixed"), (4, "mixed"), (3, "fp64")])
+@pytest.mark.parametrize("share,prec", [(1, "mixed"), (2, "mixed"), (4, "mixed"), (5, "mixed"), (8, "mixed"),
+                                        (16, "mixed"), (3, "fp64")])
 def test_batched_sweep_shared_patch_inverses(nine_point_fp64, share, prec):
     """Nine frequencies in one batch with one set of patch inverses per `share` adjacent frequencies (the
-    last set covering fewer when 9 % share != 0): same S as one-at-a-time FP64."""
+    last set covering fewer when 9 % share != 0): same S as one-at-a-time FP64.  Shares 5, 8 and 16 put 10,
+    16 and 18 systems on one inverse set (one MFMA wavefront per 8 systems; the last wavefront of a set holds
+    8 systems, 2 systems, or none)."""
     m, f, r1 = nine_point_fp64
     s = _solver(m, 2)
     try:
@@ -268,6 +271,20 @@ def test_batched_sweep_shared_patch_inverses(nine_point_fp64, share, prec):
         s.close()
     assert rb["converged"].all()
     assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
+
+
+def test_shared_patch_inverse_limits(nine_point_fp64):
+    """An inverse set serves at most 32 systems in mixed precision (MFMA apply) and 8 in FP64."""
+    from openparem_amd._lib import OpemError
+    m, f, _ = nine_point_fp64
+    s = _solver(m, 2)
+    try:
+        with pytest.raises(OpemError, match="<= 8"):
+            s.sweep(f, precision="fp64", tol=1e-10, batch=9, precond_share=5)
+        with pytest.raises(OpemError, match="<= 32"):
+            s.sweep(f, precision="mixed", tol=1e-10, batch=9, precond_share=17)
+    finally:
+        s.close()
 
 
 @pytest.mark.parametrize("deg,prec", [(3, "mixed"), (4, "mixed"), (5, "fp64")])
