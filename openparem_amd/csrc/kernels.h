@@ -161,7 +161,7 @@ struct PatchLayout {
   const int64_t *nz_off;             // offset of the block's CSR-position map in pnz
   const uint8_t *port;               // per patch: excitation-dependent
   const int32_t *pnz;                // CSR position of (idx[i], idx[j]) or -1
-  const int32_t *d2p_ptr, *d2p_patch, *d2p_row;  // DOF -> (patch, local row)
+  const int32_t *d2p_ptr, *d2p_y;    // DOF -> its rows of the patch-output buffer (positions in idx)
   const int64_t *yoff;               // offset of patch p's rows in the per-system patch-output buffer
   int64_t group_stride, shared_size, exc_size, npatch, ytot;
   int nmax;

@@ -2011,8 +2011,7 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
     T ax[NSM], ay[NSM];
 #pragma unroll
     for (int k = 0; k < NSM; k++) ax[k] = ay[k] = (T)0;
-    for (int j = 0; j < np; j++) {
-      T a_x, a_y;
+    auto entry = [&](int j, T &a_x, T &a_y) {
       if constexpr (std::is_same<P, __half2>::value) {
         const __half *Pr = (const __half *)A;
         a_x = __half2float(Pr[(int64_t)j * npk + t]);
@@ -2022,6 +2021,8 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
         a_x = a.x;
         a_y = a.y;
       }
+    };
+    auto fma_row = [&](int j, T a_x, T a_y) {
 #pragma unroll
       for (int k = 0; k < NSM; k++) {
         if (k < ns) {
@@ -2030,6 +2031,20 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
           ay[k] += a_x * (T)xv.y + a_y * (T)xv.x;
         }
       }
+    };
+    // four rows of the inverse are loaded before any is used (four loads in flight per lane)
+    int j = 0;
+    for (; j + 4 <= np; j += 4) {
+      T a_x[4], a_y[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) entry(j + u, a_x[u], a_y[u]);
+#pragma unroll
+      for (int u = 0; u < 4; u++) fma_row(j + u, a_x[u], a_y[u]);
+    }
+    for (; j < np; j++) {
+      T a_x, a_y;
+      entry(j, a_x, a_y);
+      fma_row(j, a_x, a_y);
     }
 #pragma unroll
     for (int k = 0; k < NSM; k++) {
@@ -2045,7 +2060,8 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
 }
 
 // FP16 storage on MFMA: one workgroup per (patch, preconditioner group), one wavefront per 8 systems of the
-// group (up to four wavefronts read the same inverse on one CU, so the block comes from HBM once).  The complex product is two real products, [Yr | Yi] = Ar [Vr | Vi] + Ai [-Vi | Vr], an
+// group (up to four wavefronts read the same inverse on one CU, so the block comes from HBM once).  The
+// complex product is two real products, [Yr | Yi] = Ar [Vr | Vi] + Ai [-Vi | Vr], an
 // (n_p x n_p) x (n_p x 2ns) GEMM on v_mfma_f32_16x16x32_f16 (FP32 accumulation): column j < ns is Re of
 // system j, ns <= j < 2ns is Im of system j - ns (ns <= 8 systems of the wavefront).  Each system's patch
 // vector is scaled to max |Re|,|Im| = 1 and split into two halves, hi = f16(v), lo = f16(v - hi), so the
@@ -2057,7 +2073,7 @@ typedef float f4v_t __attribute__((ext_vector_type(4)));
 constexpr int kPatchMfmaSys = 32;  // systems per (patch, group) on the MFMA path: up to four wavefronts
 
 template <typename X>
-__global__ __launch_bounds__(256) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
                                                           const float *__restrict__ scale, const X *__restrict__ v,
                                                           int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
                                                           const int32_t *__restrict__ active) {
@@ -2069,20 +2085,26 @@ __global__ __launch_bounds__(256) void k_patch_local_mfma(PatchLayout L, int G, 
   for (int k = 0; k < ns; k++) any |= active == nullptr || active[s0 + k] != 0;
   if (!any) return;  // uniform over the workgroup
   __shared__ float2 vs[kPatchMfmaSys][128];
-  __shared__ unsigned int mx[kPatchMfmaSys];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int b0 = L.idx_off[p], np = L.idx_off[p + 1] - b0, npk = pad8(np);
-  if (threadIdx.x < kPatchMfmaSys) mx[threadIdx.x] = 0u;
-  __syncthreads();
-  for (int e = threadIdx.x; e < ns * 128; e += blockDim.x) {
-    const int sy = e >> 7, t = e & 127;
-    float2 val = make_float2(0.f, 0.f);
-    if (t < np) {
-      const X xv = v[(s0 + sy) * ldv_sys + L.idx[b0 + t]];
-      val = make_float2((float)xv.x, (float)xv.y);
+  {
+    // the group's patch vectors into LDS, entry e = 128 sy + t (zero past n_p); blockDim = 64 ceil(ns / 8),
+    // so 16 entries per thread cover the group.  Indices are clamped so every load is issued unconditionally.
+    const int ne = ns * 128;
+    int32_t col[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) col[m] = L.idx[b0 + min((int)((threadIdx.x + m * blockDim.x) & 127), np - 1)];
+    X val[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int e = min((int)(threadIdx.x + m * blockDim.x), ne - 1);
+      val[m] = v[(s0 + (e >> 7)) * ldv_sys + col[m]];
     }
-    vs[sy][t] = val;
-    atomicMax(&mx[sy], __float_as_uint(fmaxf(fabsf(val.x), fabsf(val.y))));
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int e = threadIdx.x + m * blockDim.x, t = e & 127;
+      if (e < ne) vs[e >> 7][t] = t < np ? make_float2((float)val[m].x, (float)val[m].y) : make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
   // this wavefront's systems: [8 wv, 8 wv + nw) of the group
@@ -2091,7 +2113,18 @@ __global__ __launch_bounds__(256) void k_patch_local_mfma(PatchLayout L, int G, 
   const int sl = j < 2 * nw ? (j < nw ? j : j - nw) : -1;
   const int sy = sl >= 0 ? 8 * wv + sl : -1;
   const int comp = j < nw ? 0 : 1;
-  const float ms = sy >= 0 ? __uint_as_float(mx[sy]) : 0.f;
+  // max |Re|,|Im| of the system's patch vector: the four lanes j, j+16, j+32, j+48 hold its 128 entries
+  float ms = 0.f;
+  if (sy >= 0)
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float2 x = vs[sy][kb * 32 + kq * 8 + e];
+        ms = fmaxf(ms, fmaxf(fabsf(x.x), fabsf(x.y)));
+      }
+  ms = fmaxf(ms, __shfl_xor(ms, 16, 64));
+  ms = fmaxf(ms, __shfl_xor(ms, 32, 64));
   const float ims = ms > 0.f ? 1.f / ms : 0.f;
   const int nkb = (npk + 31) >> 5;  // <= 4 for n_p <= 100
   h8_t b1h[4], b1l[4], b2h[4], b2l[4];
@@ -2168,7 +2201,7 @@ __global__ __launch_bounds__(256) void k_patch_gather(PatchLayout L, int64_t n, 
     const int64_t d = e / S;
     double2 acc = make_double2(0.0, 0.0);
     for (int k = L.d2p_ptr[d]; k < L.d2p_ptr[d + 1]; k++) {
-      const X t = y[(L.yoff[L.d2p_patch[k]] + L.d2p_row[k]) * S + s];
+      const X t = y[(int64_t)L.d2p_y[k] * S + s];
       acc.x += t.x;
       acc.y += t.y;
     }

@@ -129,8 +129,7 @@ struct Ctx {
   Patches patches;
   bool have_patches = false;
   std::vector<uint8_t> patch_excl;
-  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_val_off_h, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_patch,
-      dp_d2p_row, dp_yoff;
+  DevBuf dp_idx_off, dp_idx, dp_val_off, dp_val_off_h, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_y, dp_yoff;
   int64_t patch_shared_size = 0, patch_exc_size = 0;      // FP64 storage: n_p^2 per block
   int64_t patch_shared_size_h = 0, patch_exc_size_h = 0;  // FP16 storage: n_p * roundup8(n_p) per block
   std::vector<uint8_t> patch_port_dofs;  // port-DOF mask the patch regions were split with
@@ -150,7 +149,7 @@ struct Ctx {
   // vertex patches over all DOFs for the H-projection CG: FP16 inverses of the unit mass P (built once per space)
   Patches hpatches;
   int hpatch_state = 0;  // 0 not built, 1 built, -1 unavailable (a patch above 100 DOFs)
-  DevBuf dh_idx_off, dh_idx, dh_val_off, dh_nz_off, dh_port, dh_nz, dh_d2p_ptr, dh_d2p_patch, dh_d2p_row, dh_yoff,
+  DevBuf dh_idx_off, dh_idx, dh_val_off, dh_nz_off, dh_port, dh_nz, dh_d2p_ptr, dh_d2p_y, dh_yoff,
       dh_inv, dh_scale;
   int64_t hpatch_size = 0;
   Work work;  // solver workspace, kept between sweeps
@@ -799,8 +798,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
     PL.port = c.dp_port.as<uint8_t>();
     PL.pnz = c.dp_nz.as<int32_t>();
     PL.d2p_ptr = c.dp_d2p_ptr.as<int32_t>();
-    PL.d2p_patch = c.dp_d2p_patch.as<int32_t>();
-    PL.d2p_row = c.dp_d2p_row.as<int32_t>();
+    PL.d2p_y = c.dp_d2p_y.as<int32_t>();
     PL.yoff = c.dp_yoff.as<int64_t>();
     PL.ytot = (int64_t)c.patches.idx.size();
     W.ypatch.alloc((size_t)S * PL.ytot * 16);
@@ -1109,8 +1107,7 @@ static PatchLayout h_layout(const Ctx &c) {
   L.port = c.dh_port.as<uint8_t>();
   L.pnz = c.dh_nz.as<int32_t>();
   L.d2p_ptr = c.dh_d2p_ptr.as<int32_t>();
-  L.d2p_patch = c.dh_d2p_patch.as<int32_t>();
-  L.d2p_row = c.dh_d2p_row.as<int32_t>();
+  L.d2p_y = c.dh_d2p_y.as<int32_t>();
   L.yoff = c.dh_yoff.as<int64_t>();
   L.group_stride = 0;
   L.shared_size = c.hpatch_size;
@@ -1154,8 +1151,7 @@ static void build_h_patches(Ctx &c) {
   c.dh_port.upload(std::vector<uint8_t>(npch, 0), st);
   c.dh_nz.upload(P.nz, st);
   c.dh_d2p_ptr.upload(P.d2p_ptr, st);
-  c.dh_d2p_patch.upload(P.d2p_patch, st);
-  c.dh_d2p_row.upload(P.d2p_row, st);
+  c.dh_d2p_y.upload(P.d2p_y, st);
   c.dh_yoff.upload(yoff, st);
   c.dh_inv.alloc((size_t)sz * sizeof(__half2));
   constexpr int kGroups = 32;
@@ -1486,8 +1482,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       c.dp_yoff.upload(yoff, st);
       c.dp_nz.upload(c.patches.nz, st);
       c.dp_d2p_ptr.upload(c.patches.d2p_ptr, st);
-      c.dp_d2p_patch.upload(c.patches.d2p_patch, st);
-      c.dp_d2p_row.upload(c.patches.d2p_row, st);
+      c.dp_d2p_y.upload(c.patches.d2p_y, st);
     }
   }
   const int R = np;

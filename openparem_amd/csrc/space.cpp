@@ -279,16 +279,9 @@ void build_patches(const Space &s, const std::vector<uint8_t> &exclude, Patches 
   P.d2p_ptr.assign(s.ndof + 1, 0);
   for (int32_t d : P.idx) P.d2p_ptr[d + 1]++;
   for (int64_t i = 0; i < s.ndof; i++) P.d2p_ptr[i + 1] += P.d2p_ptr[i];
-  P.d2p_patch.resize(P.idx.size());
-  P.d2p_row.resize(P.idx.size());
+  P.d2p_y.resize(P.idx.size());
   std::vector<int32_t> fill(P.d2p_ptr.begin(), P.d2p_ptr.end() - 1);
-  for (int64_t p = 0; p < P.npatch; p++)
-    for (int32_t k = P.idx_off[p]; k < P.idx_off[p + 1]; k++) {
-      const int32_t d = P.idx[k];
-      P.d2p_patch[fill[d]] = (int32_t)p;
-      P.d2p_row[fill[d]] = k - P.idx_off[p];
-      fill[d]++;
-    }
+  for (int32_t k = 0; k < (int32_t)P.idx.size(); k++) P.d2p_y[fill[P.idx[k]]++] = k;
 }
 
 int64_t csr_find(const Space &s, int32_t row, int32_t col) {

@@ -54,8 +54,7 @@ struct Patches {
   std::vector<int64_t> val_off;   // npatch+1, into n_p*n_p blocks
   std::vector<int32_t> nz;        // CSR position of (idx[i], idx[j]) or -1, per block
   std::vector<int32_t> d2p_ptr;   // ndof+1
-  std::vector<int32_t> d2p_patch; // patch of each (DOF, patch-row) pair
-  std::vector<int32_t> d2p_row;   // local row
+  std::vector<int32_t> d2p_y;     // position in idx of each (DOF, patch-row) pair (its patch-output row)
 };
 void build_patches(const Space &s, const std::vector<uint8_t> &exclude, Patches &P, int nthreads);
 
