@@ -284,6 +284,16 @@ int spmv_wide_enabled() {
   return v;
 }
 
+float patch_split_kappa() {
+  static float v = -1.f;
+  if (v < 0.f) {
+    const char *e = getenv("OPEM_PATCH_SPLIT_KAPPA");
+    v = e ? (float)atof(e) : 20.f;
+    if (!(v >= 0.f)) v = 20.f;
+  }
+  return v;
+}
+
 int patch_mfma_enabled() {
   static int v = -1;
   if (v < 0) {
@@ -1780,9 +1790,13 @@ __device__ __forceinline__ int64_t patch_offset(const PatchLayout &L, int p, int
 }
 __device__ __forceinline__ int pad8(int n) { return (n + 7) & ~7; }
 
-// Gauss-Jordan with partial pivoting in LDS.  FP16 storage inverts in FP32 (the FP32 error, about cond * 6e-8,
-// stays far below the FP16 rounding of the stored inverse, and the half-size LDS block doubles residency);
-// FP64 storage inverts in FP64.
+// Gauss-Jordan with partial pivoting in LDS, in FP64 (n_p <= 100: 160 KB).
+// FP16 storage keeps a block as four planes of n_p x roundup8(n_p) halves, [Re hi | Im hi | Re lo | Im lo], of the
+// symmetrised inverse scaled so its largest |Re|,|Im| is 1: hi = f16(a), lo = f16(2^11 (a - hi)).  The hi planes
+// alone carry 11 bits, enough while the block's condition estimate kappa = |A_p|_F |A_p^-1|_F / n_p stays small;
+// above the threshold (low k0 h: the near-kernel gradient part dominates the inverse by orders of magnitude and
+// hi-only rounding destroys the rest) the block is marked by a negative scale and applied with both planes
+// (about 22 bits).  Blocks below it leave the lo planes unwritten and unread.
 template <typename T>
 struct Cx;
 template <>
@@ -1797,9 +1811,10 @@ struct Cx<double> {
 template <typename P>
 __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R, const double2 *__restrict__ a64,
                                                      int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
-                                                     P *__restrict__ inv, float *__restrict__ scale) {
+                                                     P *__restrict__ inv, float *__restrict__ scale,
+                                                     float split_kappa) {
   constexpr bool kPacked = std::is_same<P, __half2>::value;
-  using T = typename std::conditional<kPacked, float, double>::type;
+  using T = double;
   using C2 = typename Cx<T>::V;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C2 *lds = (C2 *)lds_raw;
@@ -1846,6 +1861,15 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
     A[t] = mk((T)vx, (T)vy);
   }
   __syncthreads();
+  double a2 = 0.0;  // |A_p|_F^2 (FP16 storage: condition estimate)
+  if constexpr (kPacked) {
+    for (int t = tid; t < nn; t += blockDim.x) a2 += A[t].x * A[t].x + A[t].y * A[t].y;
+    a2 = wave_sum(a2);
+    if ((tid & 63) == 0) red[tid >> 6] = a2;
+    __syncthreads();
+    a2 = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+  }
   for (int k = 0; k < np; k++) {
     if (tid < 64) {
       T best = (T)-1;
@@ -1912,28 +1936,45 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
   }
   P *out = inv + patch_offset(L, p, q, r);
   if constexpr (kPacked) {
-    // block scale: the largest |Re|,|Im| maps to 1 in the half-precision store
-    double m = 0.0;
-    for (int t = tid; t < nn; t += blockDim.x) m = fmax(m, fmax(fabs((double)A[t].x), fabs((double)A[t].y)));
+    // block scale: the largest |Re|,|Im| maps to 1 in the half-precision store; |A_p^-1|_F for the condition
+    // estimate
+    double m = 0.0, i2 = 0.0;
+    for (int t = tid; t < nn; t += blockDim.x) {
+      m = fmax(m, fmax(fabs(A[t].x), fabs(A[t].y)));
+      i2 += A[t].x * A[t].x + A[t].y * A[t].y;
+    }
     m = wave_sum_max(m);
-    if ((tid & 63) == 0) red[tid >> 6] = m;
+    i2 = wave_sum(i2);
+    __shared__ double red2[4];
+    if ((tid & 63) == 0) {
+      red[tid >> 6] = m;
+      red2[tid >> 6] = i2;
+    }
     __syncthreads();
     m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    i2 = red2[0] + red2[1] + red2[2] + red2[3];
     const double sc = m > 0.0 ? m : 1.0;
-    if (tid == 0) scale[((int64_t)q * R + (L.port[p] ? r : 0)) * L.npatch + p] = (float)sc;
+    const bool split = sqrt(a2 * i2) / np > (double)split_kappa;
+    if (tid == 0) scale[((int64_t)q * R + (L.port[p] ? r : 0)) * L.npatch + p] = split ? -(float)sc : (float)sc;
     const double h = 0.5 / sc;
     const int npk = pad8(np);
-    __half *Pr = (__half *)out, *Pi = Pr + (int64_t)np * npk;
+    const int64_t pl = (int64_t)np * npk;
+    __half *Pr = (__half *)out, *Pi = Pr + pl, *Prl = Pr + 2 * pl, *Pil = Pr + 3 * pl;
     for (int t = tid; t < np * npk; t += blockDim.x) {
       const int i = t / npk, j = t - i * npk;
       double re = 0.0, im = 0.0;
       if (j < np) {
         const C2 u = A[i * np + j], w = A[j * np + i];
-        re = ((double)u.x + (double)w.x) * h;
-        im = ((double)u.y + (double)w.y) * h;
+        re = (u.x + w.x) * h;
+        im = (u.y + w.y) * h;
       }
-      Pr[t] = __float2half((float)re);
-      Pi[t] = __float2half((float)im);
+      const __half hr = __float2half((float)re), hi = __float2half((float)im);
+      Pr[t] = hr;
+      Pi[t] = hi;
+      if (split) {
+        Prl[t] = __float2half((float)((re - (double)__half2float(hr)) * 2048.0));
+        Pil[t] = __float2half((float)((im - (double)__half2float(hi)) * 2048.0));
+      }
     }
   } else {
     for (int t = tid; t < nn; t += blockDim.x) out[t] = make_double2((double)A[t].x, (double)A[t].y);
@@ -1945,7 +1986,7 @@ void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, cons
                         const uint8_t *rowmask, int64_t n, P *inv, float *scale) {
   if (L.nmax > 100) throw std::runtime_error("vertex patch larger than 100 DOFs");
   if (L.pg < 1) throw std::runtime_error("patch build: frequencies per preconditioner must be >= 1");
-  const size_t lds = (size_t)(L.nmax * L.nmax + L.nmax) * (std::is_same<P, __half2>::value ? 8 : 16);
+  const size_t lds = (size_t)(L.nmax * L.nmax + L.nmax) * 16;
   static size_t attr_bytes = 0;
   if (lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute((const void *)k_patch_build<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1958,7 +1999,7 @@ void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, cons
   }
   const int nq = (G + L.pg - 1) / L.pg;
   hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)L.npatch, nq * R), dim3(256), lds, st, L, G, R,
-                     (const double2 *)a64, nnz, rowmask, n, inv, scale);
+                     (const double2 *)a64, nnz, rowmask, n, inv, scale, patch_split_kappa());
   HIP_LAUNCH_CHECK();
 }
 template void launch_patch_build<double2>(hipStream_t, const PatchLayout &, int, int, const double *, int64_t,
@@ -2007,15 +2048,22 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
   const int npk = pad8(np);
   for (int ri = 0; ri < nreg; ri++) {
     const P *A = inv + patch_offset(L, p, q, ri);
-    const T sc = scale ? (T)scale[((int64_t)q * R + ri) * L.npatch + p] : (T)1;
+    const float scv = scale ? scale[((int64_t)q * R + ri) * L.npatch + p] : 1.f;
+    const bool lo = scv < 0.f;  // FP16: the block carries its lo planes
+    const T sc = (T)fabsf(scv);
     T ax[NSM], ay[NSM];
 #pragma unroll
     for (int k = 0; k < NSM; k++) ax[k] = ay[k] = (T)0;
     auto entry = [&](int j, T &a_x, T &a_y) {
       if constexpr (std::is_same<P, __half2>::value) {
         const __half *Pr = (const __half *)A;
-        a_x = __half2float(Pr[(int64_t)j * npk + t]);
-        a_y = __half2float(Pr[(int64_t)np * npk + (int64_t)j * npk + t]);
+        const int64_t pl = (int64_t)np * npk, e = (int64_t)j * npk + t;
+        a_x = __half2float(Pr[e]);
+        a_y = __half2float(Pr[pl + e]);
+        if (lo) {
+          a_x += __half2float(Pr[2 * pl + e]) * (T)(1.0 / 2048.0);
+          a_y += __half2float(Pr[3 * pl + e]) * (T)(1.0 / 2048.0);
+        }
       } else {
         const double2 a = A[(int64_t)j * np + t];
         a_x = a.x;
@@ -2125,6 +2173,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
   ms = fmaxf(ms, __shfl_xor(ms, 16, 64));
   ms = fmaxf(ms, __shfl_xor(ms, 32, 64));
+  // below the smallest normal FP32 the reciprocal would overflow (0 * inf = NaN); such a patch vector is
+  // negligible against the unit-norm Krylov vector and contributes zero
+  if (ms < 1.17549435e-38f) ms = 0.f;
   const float ims = ms > 0.f ? 1.f / ms : 0.f;
   const int nkb = (npk + 31) >> 5;  // <= 4 for n_p <= 100
   h8_t b1h[4], b1l[4], b2h[4], b2l[4];
@@ -2151,7 +2202,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int ri = 0; ri < nreg; ri++) {
     const _Float16 *Ar = (const _Float16 *)(inv + patch_offset(L, p, q, ri));
     const _Float16 *Ai = Ar + (int64_t)np * npk;
-    const float sc = scale[((int64_t)q * R + ri) * L.npatch + p];
+    const float scv = scale[((int64_t)q * R + ri) * L.npatch + p];
+    const bool lo = scv < 0.f;  // the block carries lo planes (uniform over the workgroup)
+    const float sc = fabsf(scv);
     const int64_t s = s0 + sy;
     const bool mine = sy >= 0 && (nreg == 1 || sy % R == ri) && (active == nullptr || active[s] != 0);
     for (int mt = 0; mt < (np + 15) >> 4; mt++) {
@@ -2175,6 +2228,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[kb], b2h[kb], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[kb], b2l[kb], acc, 0, 0, 0);
         }
+      }
+      if (lo) {
+        // lo planes (2^11 (a - hi)) times the hi halves of the vectors; the lo x lo term is below 2^-22
+        const int64_t pl = (int64_t)np * npk;
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+          if (kb < nkb) {
+            const int k0 = min(kb * 32 + kq * 8, npk - 8);
+            ar[kb] = *(const h8_t *)(Ar + 2 * pl + (int64_t)i * npk + k0);
+            ai[kb] = *(const h8_t *)(Ar + 3 * pl + (int64_t)i * npk + k0);
+          }
+        f4v_t acc2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+          if (kb < nkb) {
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[kb], b1h[kb], acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[kb], b2h[kb], acc2, 0, 0, 0);
+          }
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] += acc2[r] * (1.f / 2048.f);
       }
       if (mine) {
         const float f = sc * ms;

@@ -131,7 +131,7 @@ struct Ctx {
   std::vector<uint8_t> patch_excl;
   DevBuf dp_idx_off, dp_idx, dp_val_off, dp_val_off_h, dp_nz_off, dp_port, dp_nz, dp_d2p_ptr, dp_d2p_y, dp_yoff;
   int64_t patch_shared_size = 0, patch_exc_size = 0;      // FP64 storage: n_p^2 per block
-  int64_t patch_shared_size_h = 0, patch_exc_size_h = 0;  // FP16 storage: n_p * roundup8(n_p) per block
+  int64_t patch_shared_size_h = 0, patch_exc_size_h = 0;  // FP16 storage: 2 n_p roundup8(n_p) half2 per block
   std::vector<uint8_t> patch_port_dofs;  // port-DOF mask the patch regions were split with
   // gradient coarse space (second level)
   Coarse coarse;
@@ -818,6 +818,15 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
       scale = W.pscale.as<float>();
     }
     launch_patch_build<PP>(st, PL, G, R, W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(), scale);
+    if (mixed && getenv("OPEM_PATCH_STATS")) {  // diagnostic: blocks that carry lo planes
+      std::vector<float> sv((size_t)nq * R * PL.npatch);
+      HIPCHK(hipMemcpyAsync(sv.data(), scale, sv.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      size_t neg = 0, tot = 0;  // the excitation-0 blocks of every group
+      for (int q = 0; q < nq; q++)
+        for (int64_t p = 0; p < PL.npatch; p++, tot++) neg += sv[((size_t)q * R) * PL.npatch + p] < 0.f;
+      fprintf(stderr, "opem: %zu of %zu FP16 patch blocks carry lo planes\n", neg, tot);
+    }
   } else {
     W.inv.alloc((size_t)S * nd * n * sizeof(P));
     launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
@@ -1140,7 +1149,7 @@ static void build_h_patches(Ctx &c) {
   for (int64_t p = 0; p < npch; p++) {
     const int64_t np_ = P.idx_off[p + 1] - P.idx_off[p];
     voff[p] = sz;
-    sz += np_ * ((np_ + 7) / 8 * 8);
+    sz += 2 * np_ * ((np_ + 7) / 8 * 8);  // hi and lo planes of Re and Im
     yoff[p] = P.idx_off[p];
   }
   c.hpatch_size = sz;
@@ -1461,7 +1470,8 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
         for (int32_t k = c.patches.idx_off[p]; k < c.patches.idx_off[p + 1]; k++)
           if (portdof[c.patches.idx[k]]) pport[p] = 1;
         const int64_t np_ = c.patches.idx_off[p + 1] - c.patches.idx_off[p];
-        const int64_t sz = np_ * np_, szh = np_ * ((np_ + 7) / 8 * 8);
+        // FP16 blocks: hi and lo planes of Re and Im, n_p x roundup8(n_p) halves each
+        const int64_t sz = np_ * np_, szh = 2 * np_ * ((np_ + 7) / 8 * 8);
         voff[p] = pport[p] ? ex : sh;
         voff_h[p] = pport[p] ? exh : shh;
         (pport[p] ? ex : sh) += sz;
