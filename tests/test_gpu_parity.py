@@ -227,6 +227,27 @@ def test_sweep_microstrip(coarse, passes):
         assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
 
 
+def test_sweep_microstrip_low_k0h():
+    """Low k0 h (0.3 and 0.6 GHz on a few-mm grid): the patch inverses are dominated by their near-kernel
+    gradient part, which FP16 hi planes alone round away; with the lo planes the mixed solve must converge to
+    the FP64 S without a blow-up.  The FP32 inner vectors themselves are dominated by gradient components here
+    (about 1e4 x the rest at 0.3 GHz), so the inner solve needs several times the FP64 iterations (measured
+    1641 vs 320 at 0.3 GHz, 433 vs 299 at 0.6 GHz); the bound only excludes a stall."""
+    m = microstrip_mesh(nx=10, ny=6, nz=12, n_sub=2, seed=8)
+    f = np.array([0.3e9, 0.6e9])
+    s = _solver(m, 2)
+    try:
+        rm = s.sweep(f, precision="mixed", tol=1e-10, batch=2, orth_passes=1, arnoldi_step=3, restart=100,
+                     maxit=4000)
+        r64 = s.sweep(f, precision="fp64", tol=1e-10, batch=1, restart=100, maxit=4000)
+    finally:
+        s.close()
+    assert r64["converged"].all() and rm["converged"].all()
+    assert np.isfinite(rm["S"]).all()
+    assert np.abs(rm["S"] - r64["S"]).max() < 1e-7
+    assert rm["iters"].max() <= 8 * r64["iters"].max()
+
+
 def test_sweep_nd3():
     """ND3 (vertex patches exceed 100 DOFs: element-block Jacobi is used)."""
     from openparem_amd.ports import te10
