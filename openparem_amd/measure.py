@@ -1,0 +1,82 @@
+"""Measurements on the shielded-microstrip workloads (one JSON line per step, printed as each step ends).
+
+    python -m openparem_amd.measure c3   # 1.84 M ND2 DOF at 5 GHz: mixed (FP32 inner + FP64 refinement) vs plain FP64
+    python -m openparem_amd.measure c4   # 0.48 M ND2 DOF, 64 frequencies 1-20 GHz x 2 ports, in four batches of 16,
+                                         # then plain FP64 at four of the frequencies
+
+Both use vertex-patch additive Schwarz, GMRES(300) to a true residual of 1e-10 and at most 8000 iterations per
+system.  The mixed solves use three Krylov vectors per Gram-Schmidt pass; the plain FP64 solves take one frequency
+at a time with two classical Gram-Schmidt passes (the comparison path of bench.py).
+"""
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import workloads
+from .solver import Solver
+
+OPTS = dict(restart=300, tol=1e-10, maxit=8000)
+
+
+def _emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def c3():
+    w = workloads.make("microstrip-c3")
+    t = time.perf_counter()
+    s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+    _emit({"workload": "microstrip-c3", "setup_s": round(time.perf_counter() - t, 2), "ndof": s.ndof, "nnz": s.nnz})
+    f = w["freqs"]
+    try:
+        for rep in range(2):  # the second solve reuses the solver workspace (the first allocates it)
+            t = time.perf_counter()
+            r = s.sweep(f, precision="mixed", batch=1, orth_passes=1, arnoldi_step=3, **OPTS)
+            _emit({"precision": "mixed", "rep": rep, "s": round(time.perf_counter() - t, 3),
+                   "iters": r["iters"].tolist(), "converged": r["converged"].tolist()})
+        t = time.perf_counter()
+        r64 = s.sweep(f, precision="fp64", batch=1, orth_passes=2, **OPTS)
+        _emit({"precision": "fp64", "s": round(time.perf_counter() - t, 3), "iters": r64["iters"].tolist(),
+               "converged": r64["converged"].tolist(), "max_abs_dS": float(np.abs(r["S"] - r64["S"]).max()),
+               "S21_dB": float(20 * np.log10(abs(r64["S"][0, 1, 0])))})
+    finally:
+        s.close()
+
+
+def c4():
+    w = workloads.make("microstrip-c4")
+    t = time.perf_counter()
+    s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+    _emit({"workload": "microstrip-c4", "setup_s": round(time.perf_counter() - t, 2), "ndof": s.ndof, "nnz": s.nnz})
+    f = w["freqs"]
+    try:
+        S, tot, done = [], 0.0, 0
+        for c in range(4):
+            fc = f[c * 16:(c + 1) * 16]
+            t = time.perf_counter()
+            # 300 MHz spacing: one inverse set per 4 frequencies spans 0.9 GHz
+            r = s.sweep(fc, precision="mixed", batch=16, orth_passes=1, precond_share=4, arnoldi_step=3, **OPTS)
+            dt = time.perf_counter() - t
+            tot += dt
+            done += int(np.all(r["converged"].astype(bool), axis=1).sum())
+            S.append(r["S"])
+            _emit({"batch": c, "f_GHz": [round(fc[0] / 1e9, 2), round(fc[-1] / 1e9, 2)], "s": round(dt, 2),
+                   "iters_mean": float(r["iters"].mean()), "iters_max": int(r["iters"].max()),
+                   "points_converged": int(np.all(r["converged"].astype(bool), axis=1).sum())})
+        _emit({"mixed_total_s": round(tot, 2), "points_converged": done, "freq_points_per_s": round(done / tot, 3)})
+        S = np.concatenate(S)
+        for i in (0, 21, 42, 63):
+            t = time.perf_counter()
+            r64 = s.sweep(f[i:i + 1], precision="fp64", batch=1, orth_passes=2, **OPTS)
+            _emit({"fp64_f_GHz": round(f[i] / 1e9, 2), "s": round(time.perf_counter() - t, 2),
+                   "iters": r64["iters"].tolist(), "converged": r64["converged"].tolist(),
+                   "max_abs_dS": float(np.abs(S[i] - r64["S"][0]).max())})
+    finally:
+        s.close()
+
+
+if __name__ == "__main__":
+    which = sys.argv[1] if len(sys.argv) > 1 else "c3"
+    {"c3": c3, "c4": c4}[which]()
