@@ -169,6 +169,8 @@ struct PatchLayout {
 };
 // inv: per-preconditioner-group storage (group_stride elements each); FP16 storage takes per-block
 // scales.  G = number of frequency groups in the batch (systems s = g*R + r).
+// out[q] = mean of a64[g] over the pg frequencies g of group q (nnz complex values each)
+void launch_group_mean(hipStream_t st, int G, int pg, int64_t nnz, const double *a64, double *out);
 template <typename P>
 void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, const double *a64, int64_t nnz,
                         const uint8_t *rowmask, int64_t n, P *inv, float *scale);

@@ -1911,15 +1911,18 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
     for (int c = tid; c < np; c += blockDim.x) A[k * np + c] = (c == k) ? pinv : mul(A[k * np + c], pinv);
     for (int i = tid; i < np; i += blockDim.x) colk[i] = A[i * np + k];
     __syncthreads();
-    for (int t = tid; t < nn; t += blockDim.x) {
-      const int i = t / np, c = t - i * np;
+    // rows over wavefronts, columns over lanes (np <= 100: two columns per lane)
+    for (int i = tid >> 6; i < np; i += blockDim.x >> 6) {
       if (i == k) continue;
       const C2 f = colk[i];
-      if (c == k) {
-        A[t] = mk(-(f.x * pinv.x - f.y * pinv.y), -(f.x * pinv.y + f.y * pinv.x));
-      } else {
-        const C2 u = mul(f, A[k * np + c]);
-        A[t] = mk(A[t].x - u.x, A[t].y - u.y);
+      for (int c = tid & 63; c < np; c += 64) {
+        const int t = i * np + c;
+        if (c == k) {
+          A[t] = mk(-(f.x * pinv.x - f.y * pinv.y), -(f.x * pinv.y + f.y * pinv.x));
+        } else {
+          const C2 u = mul(f, A[k * np + c]);
+          A[t] = mk(A[t].x - u.x, A[t].y - u.y);
+        }
       }
     }
     __syncthreads();
@@ -1979,6 +1982,31 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
   } else {
     for (int t = tid; t < nn; t += blockDim.x) out[t] = make_double2((double)A[t].x, (double)A[t].y);
   }
+}
+
+// out[q][e] = mean of a64[g][e] over the frequencies g of group q (pg per group, the last may hold fewer):
+// the matrix a shared set of patch inverses is built from, read once as a stream
+__global__ __launch_bounds__(256) void k_group_mean(int G, int pg, int64_t nnz, const double2 *__restrict__ a64,
+                                                    double2 *__restrict__ out) {
+  const int q = blockIdx.y;
+  const int g0 = q * pg, ng = min(pg, G - g0);
+  const double w = 1.0 / ng;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int g = g0; g < g0 + ng; g++) {
+      const double2 a = a64[(int64_t)g * nnz + e];
+      acc.x += a.x;
+      acc.y += a.y;
+    }
+    out[(int64_t)q * nnz + e] = make_double2(acc.x * w, acc.y * w);
+  }
+}
+
+void launch_group_mean(hipStream_t st, int G, int pg, int64_t nnz, const double *a64, double *out) {
+  const int nq = (G + pg - 1) / pg;
+  const unsigned bx = (unsigned)std::min<int64_t>(4096, (nnz + 255) / 256);
+  hipLaunchKernelGGL(k_group_mean, dim3(bx, nq), dim3(256), 0, st, G, pg, nnz, (const double2 *)a64, (double2 *)out);
+  HIP_LAUNCH_CHECK();
 }
 
 template <typename P>

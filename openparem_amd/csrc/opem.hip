@@ -99,7 +99,7 @@ struct PortTrace {
 // Device workspace of the batched solve.  The context keeps it between sweeps (buffers only grow), so repeated
 // sweeps make no device allocations for the Krylov basis and the other per-batch arrays; opem_destroy frees it.
 struct Work {
-  DevBuf A64, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
+  DevBuf A64, Amean, masks, x, xil, r, V, w, z, u, Hh, cs, sn, g, y, est, thresh, active, cyc_active, kfinal, partial,
       hbuf, nrm2, inv, pdv, pd, bn, cinv, rc, yc, t, pscale, ypatch, pinv, pprod, pt, pu, w2, C2, gram, gram4, R2, Hraw,
       nrm2b;
   // batched operator: frequency-independent matrices (per sweep) and per-system coefficients (per batch)
@@ -817,7 +817,19 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
       W.pscale.alloc((size_t)nq * R * PL.npatch * sizeof(float));
       scale = W.pscale.as<float>();
     }
-    launch_patch_build<PP>(st, PL, G, R, W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(), scale);
+    if (PL.pg > 1) {
+      // shared sets are built from the mean matrix of their frequencies: averaged once as a stream, then each
+      // block reads one value per entry
+      W.Amean.alloc((size_t)nq * nnz * 16);
+      launch_group_mean(st, G, PL.pg, nnz, W.A64.as<double>(), W.Amean.as<double>());
+      PatchLayout PM = PL;
+      PM.pg = 1;
+      launch_patch_build<PP>(st, PM, nq, R, W.Amean.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(),
+                             scale);
+    } else {
+      launch_patch_build<PP>(st, PL, G, R, W.A64.as<double>(), nnz, W.masks.as<uint8_t>(), n, W.inv.as<PP>(),
+                             scale);
+    }
     if (mixed && getenv("OPEM_PATCH_STATS")) {  // diagnostic: blocks that carry lo planes
       std::vector<float> sv((size_t)nq * R * PL.npatch);
       HIPCHK(hipMemcpyAsync(sv.data(), scale, sv.size() * sizeof(float), hipMemcpyDeviceToHost, st));
