@@ -2149,7 +2149,7 @@ typedef float f4v_t __attribute__((ext_vector_type(4)));
 constexpr int kPatchMfmaSys = 32;  // systems per (patch, group) on the MFMA path: up to four wavefronts
 
 template <typename X>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
+__global__ __launch_bounds__(256) void k_patch_local_mfma(PatchLayout L, int G, int R, const __half2 *__restrict__ inv,
                                                           const float *__restrict__ scale, const X *__restrict__ v,
                                                           int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
                                                           const int32_t *__restrict__ active) {
@@ -2239,13 +2239,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       // rows past n_p read row n_p - 1 (their outputs are not stored); columns past the padded width are
       // never read (kb < nkb), and padded columns are zero in the store
       const int i = min(mt * 16 + j, np - 1);
-      h8_t ar[4], ai[4];
+      // the tile's hi and (on marked blocks) lo planes are loaded before any is used: one HBM round trip per
+      // row tile
+      const int64_t pl = (int64_t)np * npk;
+      h8_t ar[4], ai[4], arl[4], ail[4];
 #pragma unroll
       for (int kb = 0; kb < 4; kb++)
         if (kb < nkb) {
-          const int k0 = min(kb * 32 + kq * 8, npk - 8);  // k0 >= npk meets zero operands in B
-          ar[kb] = *(const h8_t *)(Ar + (int64_t)i * npk + k0);
-          ai[kb] = *(const h8_t *)(Ai + (int64_t)i * npk + k0);
+          const int64_t o = (int64_t)i * npk + min(kb * 32 + kq * 8, npk - 8);  // k0 >= npk meets zero B rows
+          ar[kb] = *(const h8_t *)(Ar + o);
+          ai[kb] = *(const h8_t *)(Ai + o);
+          if (lo) {
+            arl[kb] = *(const h8_t *)(Ar + 2 * pl + o);
+            ail[kb] = *(const h8_t *)(Ar + 3 * pl + o);
+          }
         }
       f4v_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -2259,20 +2266,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
       if (lo) {
         // lo planes (2^11 (a - hi)) times the hi halves of the vectors; the lo x lo term is below 2^-22
-        const int64_t pl = (int64_t)np * npk;
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++)
-          if (kb < nkb) {
-            const int k0 = min(kb * 32 + kq * 8, npk - 8);
-            ar[kb] = *(const h8_t *)(Ar + 2 * pl + (int64_t)i * npk + k0);
-            ai[kb] = *(const h8_t *)(Ar + 3 * pl + (int64_t)i * npk + k0);
-          }
         f4v_t acc2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 4; kb++)
           if (kb < nkb) {
-            acc2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[kb], b1h[kb], acc2, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[kb], b2h[kb], acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[kb], b1h[kb], acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[kb], b2h[kb], acc2, 0, 0, 0);
           }
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[r] += acc2[r] * (1.f / 2048.f);
