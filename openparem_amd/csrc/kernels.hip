@@ -1808,8 +1808,13 @@ struct Cx<double> {
   using V = double2;
 };
 
+// One workgroup of kBuildThreads threads per block: the FP64 block fills the LDS of a CU, so the workgroup
+// itself must carry the CU's parallelism (16 wavefronts, four per SIMD).
+constexpr int kBuildThreads = 1024;
+constexpr int kBuildWaves = kBuildThreads / 64;
+
 template <typename P>
-__global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R, const double2 *__restrict__ a64,
+__global__ __launch_bounds__(kBuildThreads) void k_patch_build(PatchLayout L, int G, int R, const double2 *__restrict__ a64,
                                                      int64_t nnz, const uint8_t *__restrict__ rowmask, int64_t n,
                                                      P *__restrict__ inv, float *__restrict__ scale,
                                                      float split_kappa) {
@@ -1828,7 +1833,7 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
   C2 *colk = lds + np * np;    // np
   __shared__ int perm[128];
   __shared__ C2 pinv_s;
-  __shared__ double red[4];
+  __shared__ double red[kBuildWaves];
   const int32_t *I = L.idx + L.idx_off[p];
   const int32_t *nz = L.pnz + L.nz_off[p];
   const uint8_t *mask = rowmask + (int64_t)r * n;
@@ -1867,7 +1872,8 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
     a2 = wave_sum(a2);
     if ((tid & 63) == 0) red[tid >> 6] = a2;
     __syncthreads();
-    a2 = red[0] + red[1] + red[2] + red[3];
+    a2 = 0.0;
+    for (int w = 0; w < kBuildWaves; w++) a2 += red[w];
     __syncthreads();
   }
   for (int k = 0; k < np; k++) {
@@ -1948,14 +1954,18 @@ __global__ __launch_bounds__(256) void k_patch_build(PatchLayout L, int G, int R
     }
     m = wave_sum_max(m);
     i2 = wave_sum(i2);
-    __shared__ double red2[4];
+    __shared__ double red2[kBuildWaves];
     if ((tid & 63) == 0) {
       red[tid >> 6] = m;
       red2[tid >> 6] = i2;
     }
     __syncthreads();
-    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    i2 = red2[0] + red2[1] + red2[2] + red2[3];
+    m = 0.0;
+    i2 = 0.0;
+    for (int w = 0; w < kBuildWaves; w++) {
+      m = fmax(m, red[w]);
+      i2 += red2[w];
+    }
     const double sc = m > 0.0 ? m : 1.0;
     const bool split = sqrt(a2 * i2) / np > (double)split_kappa;
     if (tid == 0) scale[((int64_t)q * R + (L.port[p] ? r : 0)) * L.npatch + p] = split ? -(float)sc : (float)sc;
@@ -2026,7 +2036,7 @@ void launch_patch_build(hipStream_t st, const PatchLayout &L, int G, int R, cons
     attr_bytes = lds;
   }
   const int nq = (G + L.pg - 1) / L.pg;
-  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)L.npatch, nq * R), dim3(256), lds, st, L, G, R,
+  hipLaunchKernelGGL((k_patch_build<P>), dim3((unsigned)L.npatch, nq * R), dim3(kBuildThreads), lds, st, L, G, R,
                      (const double2 *)a64, nnz, rowmask, n, inv, scale, patch_split_kappa());
   HIP_LAUNCH_CHECK();
 }
