@@ -1,12 +1,15 @@
-"""Measurements on the shielded-microstrip workloads (one JSON line per step, printed as each step ends).
+"""Measurements on the microstrip configurations and the mixed set (one JSON line per step, printed as it ends).
 
     python -m openparem_amd.measure c3   # 1.84 M ND2 DOF at 5 GHz: mixed (FP32 inner + FP64 refinement) vs plain FP64
     python -m openparem_amd.measure c4   # 0.48 M ND2 DOF, 64 frequencies 1-20 GHz x 2 ports, in four batches of 16,
                                          # then plain FP64 at four of the frequencies
 
-Both use vertex-patch additive Schwarz, GMRES(300) to a true residual of 1e-10 and at most 8000 iterations per
-system.  The mixed solves use three Krylov vectors per Gram-Schmidt pass; the plain FP64 solves take one frequency
-at a time with two classical Gram-Schmidt passes (the comparison path of bench.py).
+    python -m openparem_amd.measure c5   # the seeded mixed set of workloads.c5_instances(): one batched mixed
+                                         # sweep per instance, plain FP64 at its middle frequency
+
+All use vertex-patch additive Schwarz (element-block Jacobi at ND3), GMRES(300) to a true residual of 1e-10 and at
+most 8000 iterations per system.  The mixed solves use three Krylov vectors per Gram-Schmidt pass; the plain FP64
+solves take one frequency at a time with two classical Gram-Schmidt passes (the comparison path of bench.py).
 """
 import json
 import sys
@@ -77,6 +80,38 @@ def c4():
         s.close()
 
 
+def c5():
+    tot_t, tot_pts, tot_f = 0.0, 0, 0
+    for i, w in enumerate(workloads.c5_instances()):
+        s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+        f = w["freqs"]
+        try:
+            # one inverse set per adjacent frequencies spanning at most about 1 GHz
+            share = int(max(1, min(16, np.floor(1e9 / (f[1] - f[0]) + 1e-9)))) if len(f) > 1 else 1
+            t = time.perf_counter()
+            r = s.sweep(f, precision="mixed", batch=len(f), orth_passes=1, precond_share=share, arnoldi_step=3,
+                        **OPTS)
+            dt = time.perf_counter() - t
+            done = int(np.all(r["converged"].astype(bool), axis=1).sum())
+            mid = len(f) // 2
+            t = time.perf_counter()
+            r64 = s.sweep(f[mid:mid + 1], precision="fp64", batch=1, orth_passes=2, **OPTS)
+            t64 = time.perf_counter() - t
+            tot_t += dt
+            tot_pts += done
+            tot_f += len(f)
+            _emit({"instance": i, "family": w["family"], "order": w["order"], "ndof": s.ndof, "freqs": len(f),
+                   "share": share, "s": round(dt, 2), "points_converged": done,
+                   "freq_points_per_s": round(done / dt, 3), "iters_mean": float(r["iters"].mean()),
+                   "iters_max": int(r["iters"].max()), "fp64_mid_s": round(t64, 2),
+                   "fp64_mid_iters": r64["iters"].tolist(), "fp64_mid_converged": r64["converged"].tolist(),
+                   "max_abs_dS_mid": float(np.abs(r["S"][mid] - r64["S"][0]).max())})
+        finally:
+            s.close()
+    _emit({"instances": 12, "points": tot_f, "points_converged": tot_pts, "mixed_s": round(tot_t, 2),
+           "aggregate_freq_points_per_s": round(tot_pts / tot_t, 3)})
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c3"
-    {"c3": c3, "c4": c4}[which]()
+    {"c3": c3, "c4": c4, "c5": c5}[which]()

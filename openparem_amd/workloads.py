@@ -6,6 +6,10 @@
 * ``microstrip-c4``  same geometry on a 22x11x50 grid (~0.48 M DOF), 64 frequencies 1-20 GHz x 2 ports.
 * ``waveguide-sweep`` the c1 guide on a 24x12x36 grid (~0.43 M ND2 DOF), 64 frequencies over the TE10-only
                      band 7-12.5 GHz x 2 ports = 128 systems on one pattern.
+
+``c5_instances()`` is a seeded mixed set (12 instances): guide or microstrip family, ND1/ND2 (one small ND3),
+target DOF count log-uniform in 50 k - 1.2 M, 8-32 frequencies, 2 ports.  Guides sweep the TE10 band
+7-12.5 GHz; microstrips sweep 2-10 GHz, below the box's multimode band.
 """
 from __future__ import annotations
 
@@ -32,3 +36,32 @@ def make(name):
         m = microstrip_mesh(nx=8, ny=5, nz=8, n_sub=2, seed=6)
         return dict(mesh=m, order=2, freqs=np.linspace(2e9, 10e9, 4), boundaries={4: COPPER})
     raise ValueError(name)
+
+
+# ND DOFs per hexahedral grid cell (6 tetrahedra) at orders 1, 2, 3 on these box meshes
+_DOF_PER_CELL = {1: 8.0, 2: 41.0, 3: 117.0}
+
+
+def c5_instances(seed=5):
+    """The mixed set: list of dicts with family, order, mesh, freqs, boundaries (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(12):
+        family = "waveguide" if i % 2 == 0 else "microstrip"
+        order = 3 if i == 10 else (1 if i % 3 == 0 else 2)
+        target = 6e4 if order == 3 else float(np.exp(rng.uniform(np.log(5e4), np.log(1.2e6))))
+        nf = int(rng.choice([8, 16, 24, 32]))
+        cells = target / _DOF_PER_CELL[order]
+        if family == "waveguide":
+            f = (cells / (12 * 6 * 18)) ** (1.0 / 3.0)
+            m = waveguide_mesh(nx=max(4, round(12 * f)), ny=max(2, round(6 * f)), nz=max(4, round(18 * f)),
+                               seed=100 + i)
+            out.append(dict(family=family, order=order, mesh=m, freqs=np.linspace(7e9, 12.5e9, nf),
+                            boundaries={}))
+        else:
+            f = (cells / (22 * 11 * 50)) ** (1.0 / 3.0)
+            m = microstrip_mesh(nx=max(8, round(22 * f)), ny=max(5, round(11 * f)), nz=max(6, round(50 * f)),
+                                seed=100 + i)
+            out.append(dict(family=family, order=order, mesh=m, freqs=np.linspace(2e9, 10e9, nf),
+                            boundaries={4: COPPER}))
+    return out
