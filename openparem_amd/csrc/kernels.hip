@@ -2055,10 +2055,12 @@ template <typename P, typename X, int NSM>
 __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R, const P *__restrict__ inv,
                                                      const float *__restrict__ scale, const X *__restrict__ v,
                                                      int64_t ldv_sys, X *__restrict__ y, int64_t ldy_sys,
-                                                     const int32_t *__restrict__ active) {
+                                                     const int32_t *__restrict__ active, int group_fast) {
   // FP32 arithmetic for FP32 vectors; FP64 arithmetic (a linear operator to FP64 rounding) for FP64 vectors
   using T = typename std::conditional<std::is_same<X, float2>::value, float, double>::type;
-  const int p = blockIdx.x, q = blockIdx.y;
+  // group_fast: the groups of a patch are consecutive workgroups (grid (groups, patches)), so groups that share
+  // one inverse (group stride 0) read it from L2 after the first
+  const int p = group_fast ? blockIdx.y : blockIdx.x, q = group_fast ? blockIdx.x : blockIdx.y;
   const int g0 = q * L.pg;
   const int ns = min(L.pg, G - g0) * R;
   const int64_t s0 = (int64_t)g0 * R;
@@ -2343,15 +2345,17 @@ void launch_patch_apply(hipStream_t st, const PatchLayout &L, int ngroups, int R
     }
   }
   if (nsm > 8) throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
+  const int gf = L.group_stride == 0 && nq > 1 && L.npatch <= 65535;
+  const dim3 gl = gf ? dim3((unsigned)nq, (unsigned)L.npatch) : grid;
   if (nsm <= 2)
-    hipLaunchKernelGGL((k_patch_local<P, X, 2>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
-                       (int64_t)S, active);
+    hipLaunchKernelGGL((k_patch_local<P, X, 2>), gl, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                       (int64_t)S, active, gf);
   else if (nsm <= 4)
-    hipLaunchKernelGGL((k_patch_local<P, X, 4>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
-                       (int64_t)S, active);
+    hipLaunchKernelGGL((k_patch_local<P, X, 4>), gl, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                       (int64_t)S, active, gf);
   else if (nsm <= 8)
-    hipLaunchKernelGGL((k_patch_local<P, X, 8>), grid, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
-                       (int64_t)S, active);
+    hipLaunchKernelGGL((k_patch_local<P, X, 8>), gl, dim3(128), 0, st, L, ngroups, R, inv, scale, v, ldv_sys, y,
+                       (int64_t)S, active, gf);
   else
     throw std::runtime_error("patch apply: frequencies per preconditioner x excitations must be <= 8");
   HIP_LAUNCH_CHECK();
