@@ -166,6 +166,7 @@ struct PatchLayout {
   int64_t group_stride, shared_size, exc_size, npatch, ytot;
   int nmax;
   int pg;                            // frequency groups per preconditioner group
+  int real_inv;                      // FP16 store of a real matrix's inverse: the Im planes are zero, not read
 };
 // inv: per-preconditioner-group storage (group_stride elements each); FP16 storage takes per-block
 // scales.  G = number of frequency groups in the batch (systems s = g*R + r).

@@ -2090,6 +2090,7 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
     const P *A = inv + patch_offset(L, p, q, ri);
     const float scv = scale ? scale[((int64_t)q * R + ri) * L.npatch + p] : 1.f;
     const bool lo = scv < 0.f;  // FP16: the block carries its lo planes
+    const bool real = std::is_same<P, __half2>::value && L.real_inv != 0;  // uniform
     const T sc = (T)fabsf(scv);
     T ax[NSM], ay[NSM];
 #pragma unroll
@@ -2099,10 +2100,10 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
         const __half *Pr = (const __half *)A;
         const int64_t pl = (int64_t)np * npk, e = (int64_t)j * npk + t;
         a_x = __half2float(Pr[e]);
-        a_y = __half2float(Pr[pl + e]);
+        a_y = real ? (T)0 : (T)__half2float(Pr[pl + e]);
         if (lo) {
           a_x += __half2float(Pr[2 * pl + e]) * (T)(1.0 / 2048.0);
-          a_y += __half2float(Pr[3 * pl + e]) * (T)(1.0 / 2048.0);
+          if (!real) a_y += __half2float(Pr[3 * pl + e]) * (T)(1.0 / 2048.0);
         }
       } else {
         const double2 a = A[(int64_t)j * np + t];
@@ -2120,14 +2121,29 @@ __global__ __launch_bounds__(128) void k_patch_local(PatchLayout L, int G, int R
         }
       }
     };
+    auto fma_row_real = [&](int j, T a_x) {
+#pragma unroll
+      for (int k = 0; k < NSM; k++) {
+        if (k < ns) {
+          const X xv = vs[k][j];
+          ax[k] += a_x * (T)xv.x;
+          ay[k] += a_x * (T)xv.y;
+        }
+      }
+    };
     // four rows of the inverse are loaded before any is used (four loads in flight per lane)
     int j = 0;
     for (; j + 4 <= np; j += 4) {
       T a_x[4], a_y[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) entry(j + u, a_x[u], a_y[u]);
+      if (real) {
 #pragma unroll
-      for (int u = 0; u < 4; u++) fma_row(j + u, a_x[u], a_y[u]);
+        for (int u = 0; u < 4; u++) fma_row_real(j + u, a_x[u]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) fma_row(j + u, a_x[u], a_y[u]);
+      }
     }
     for (; j < np; j++) {
       T a_x, a_y;

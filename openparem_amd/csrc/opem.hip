@@ -1137,6 +1137,7 @@ static PatchLayout h_layout(const Ctx &c) {
   L.ytot = (int64_t)c.hpatches.idx.size();
   L.nmax = c.hpatches.nmax;
   L.pg = 8;
+  L.real_inv = 1;  // P is real
   return L;
 }
 
