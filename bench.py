@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--share", type=int, default=16, help="adjacent frequencies sharing one set of patch inverses")
     ap.add_argument("--poly", type=int, default=0, help="GMRES-polynomial preconditioner degree (0: off)")
     ap.add_argument("--arnoldi-step", type=int, default=3, help="Krylov vectors per Gram-Schmidt pass (1..4)")
+    ap.add_argument("--warm-start", type=int, default=1,
+                    help="1: even-indexed frequencies first, odd ones from their neighbours' interpolated solutions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the systems of a sweep are "
@@ -152,11 +154,13 @@ def main():
 
     for _ in range(args.warmup):
         s.sweep(freqs, precision="mixed", batch=len(freqs), orth_passes=1, precond_share=args.share,
-                poly_degree=args.poly, arnoldi_step=args.arnoldi_step, **opts)
+                poly_degree=args.poly, arnoldi_step=args.arnoldi_step, warm_start=args.warm_start,
+                **opts)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = s.sweep(freqs, precision="mixed", batch=len(freqs), orth_passes=1, precond_share=args.share,
-                    poly_degree=args.poly, arnoldi_step=args.arnoldi_step, **opts)
+                    poly_degree=args.poly, arnoldi_step=args.arnoldi_step, warm_start=args.warm_start,
+                    **opts)
     t1 = time.perf_counter()
     elapsed = t1 - t0
     done = int(np.all(r["converged"].astype(bool), axis=1).sum())
@@ -208,7 +212,9 @@ def main():
             "order": work["order"], "ndof": s.ndof, "nnz": s.nnz, "systems": len(freqs) * nports,
             "precond": f"vertex-patch additive Schwarz (one inverse set per {args.share} adjacent frequencies)"
                        + (f", GMRES polynomial degree {args.poly}" if args.poly else "") + f", GMRES({args.restart})"
-                       + (f", {args.arnoldi_step} Krylov vectors per Gram-Schmidt pass" if args.arnoldi_step > 1 else ""),
+                       + (f", {args.arnoldi_step} Krylov vectors per Gram-Schmidt pass" if args.arnoldi_step > 1 else "")
+                       + (", even-indexed frequencies first, odd ones from their neighbours' interpolated solutions"
+                          if args.warm_start else ""),
         },
         "points_converged": done,
         "iters_mean": its_mean,

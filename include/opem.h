@@ -145,6 +145,10 @@ typedef struct {
   int arnoldi_step;             /* s = 2..4: s Krylov vectors per Gram-Schmidt pass (z_1 = B v,        */
                                 /*    z_{c+1} = B z_c, block projection, Cholesky QR of the block);     */
                                 /*    0 or 1: one; with poly_degree > 0 one is used                    */
+  int warm_start;               /* L = 1..4 (with batch >= nfreq >= 3): bisection over the frequency    */
+                                /*    index, every 2^L-th frequency first, then the midpoints, each     */
+                                /*    started from the linear interpolation in frequency of its solved  */
+                                /*    neighbours (free rows); 0: off                                   */
 } opem_sweep_in;
 
 typedef struct {

@@ -34,7 +34,8 @@ class SweepIn(C.Structure):
                 ("tol", C.c_double), ("reciprocal", C.c_int), ("h_tol", C.c_double), ("precond", C.c_int),
                 ("coarse_size", C.c_int), ("orth_passes", C.c_int),
                 ("precond_share", C.c_int), ("poly_degree", C.c_int),
-                ("arnoldi_step", C.c_int)]
+                ("arnoldi_step", C.c_int),
+                ("warm_start", C.c_int)]
 
 
 class SweepOut(C.Structure):

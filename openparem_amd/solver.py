@@ -127,7 +127,7 @@ class Solver:
     # ------------------------------------------------------------------ hot path
     def sweep(self, freqs, precision="fp64", batch=0, restart=150, maxit=20000, tol=1e-10, reciprocal=True,
               h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch", coarse=0, orth_passes=2,
-              precond_share=1, poly_degree=0, arnoldi_step=1):
+              precond_share=1, poly_degree=0, arnoldi_step=1, warm_start=0):
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
         nf, np_ = freqs.size, len(self.ports)
         mode_fn = mode_fn or provider_for(self.mesh)
@@ -154,7 +154,7 @@ class Solver:
                            reciprocal=int(bool(reciprocal)), h_tol=float(h_tol),
                            precond={"patch": 0, "element": 1}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes),
                            precond_share=int(precond_share), poly_degree=int(poly_degree),
-                           arnoldi_step=int(arnoldi_step))
+                           arnoldi_step=int(arnoldi_step), warm_start=int(warm_start))
         S = np.zeros((nf, np_, np_), np.complex128)
         Sr = np.zeros_like(S)
         Cp = np.zeros_like(S)

@@ -294,6 +294,29 @@ def test_batched_sweep_shared_patch_inverses(nine_point_fp64, share, prec):
     assert np.abs(rb["S"] - r1["S"]).max() < 1e-6
 
 
+@pytest.mark.parametrize("prec,levels", [("mixed", 1), ("mixed", 2), ("fp64", 1)])
+def test_warm_start(nine_point_fp64, prec, levels):
+    """Bisection over the frequency index: every 2^L-th frequency from the Dirichlet lift, then the midpoints,
+    each started from the interpolation of its solved neighbours (the last by extrapolation): same S as
+    one-at-a-time FP64.  In FP64 (one cycle to the outer tolerance) the warm-started frequencies need fewer
+    iterations; mixed precision trades iterations for shorter inner cycles on warm starts (1e-3 per cycle), which
+    pays on long cycles only, so no iteration bound is asserted there."""
+    m, f, r1 = nine_point_fp64
+    s = _solver(m, 2)
+    try:
+        kw = dict(precision=prec, tol=1e-10, batch=9, orth_passes=1 if prec == "mixed" else 2, precond_share=1)
+        rc = s.sweep(f, warm_start=0, **kw)
+        rw = s.sweep(f, warm_start=levels, **kw)
+        r8 = s.sweep(f[:8], warm_start=levels, **kw)  # even count: the last frequency is extrapolated
+    finally:
+        s.close()
+    assert rw["converged"].all() and r8["converged"].all()
+    assert np.abs(rw["S"] - r1["S"]).max() < 1e-6
+    assert np.abs(r8["S"] - r1["S"][:8]).max() < 1e-6
+    if prec == "fp64":
+        assert rw["iters"][1::2].sum() < rc["iters"][1::2].sum()
+
+
 def test_shared_patch_inverse_limits(nine_point_fp64):
     """An inverse set serves at most 32 systems in mixed precision (MFMA apply) and 8 in FP64."""
     from openparem_amd._lib import OpemError
