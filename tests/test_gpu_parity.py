@@ -318,7 +318,7 @@ def test_warm_start(nine_point_fp64, prec, levels):
 
 
 def test_shared_patch_inverse_limits(nine_point_fp64):
-    """An inverse set serves at most 32 systems in mixed precision (MFMA apply) and 8 in FP64."""
+    """An inverse set serves at most 32 systems in mixed precision (MFMA apply) and 8 in FP64; warm_start is 0..4."""
     from openparem_amd._lib import OpemError
     m, f, _ = nine_point_fp64
     s = _solver(m, 2)
@@ -327,6 +327,8 @@ def test_shared_patch_inverse_limits(nine_point_fp64):
             s.sweep(f, precision="fp64", tol=1e-10, batch=9, precond_share=5)
         with pytest.raises(OpemError, match="<= 32"):
             s.sweep(f, precision="mixed", tol=1e-10, batch=9, precond_share=17)
+        with pytest.raises(OpemError, match="warm_start"):
+            s.sweep(f, precision="mixed", tol=1e-10, batch=9, warm_start=5)
     finally:
         s.close()
 
