@@ -9,6 +9,7 @@ every workload is reproducible without files:
   ``regression/OpenParEM3D/WR90/straight*``).
 * ``microstrip_mesh`` - shielded microstrip box: substrate, zero-thickness
   surface-impedance trace, PEC ground/walls, 1st-order radiation lid, two ports.
+* ``tee_mesh``        - H-plane tee junction of rectangular guides, three wave ports.
 * ``random_tets``     - independent random affine tetrahedra (assembly-only workload).
 
 Structured grids are split into 6 Kuhn tetrahedra per hexahedral cell (all cells
@@ -138,6 +139,53 @@ def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jit
                 materials={1: dict(er=er, tand=tand, mur=1.0)},
                 name=f"waveguide_{nx}x{ny}x{nz}")
     m.geometry = dict(a=a, b=b, length=length)
+    return m
+
+
+def tee_mesh(a=22.86e-3, b=10.16e-3, length=60e-3, arm=25e-3, z0=None, h=2.5e-3, ny=3):
+    """H-plane tee: main guide x in [0,a], y in [0,b], z in [0,L] with ports at z=0 (attr 2) and z=L (attr 3);
+    side arm x in [a, a+arm], z in [z0, z0+a] with its port at x=a+arm (attr 4).  A structured Kuhn grid of cell
+    size about h (no jitter: the junction's reentrant edges stay on grid lines).  Each port's TE10 mode is
+    described in ``geometry["ports"]`` by the corner of its cross-section, the unit width axis and a, b."""
+    z0 = (length - a) / 2 if z0 is None else z0
+
+    def seg(lo, hi):
+        return max(1, int(round((hi - lo) / h)))
+
+    xs, _ = _graded([(0.0, a, seg(0, a)), (a, a + arm, seg(a, a + arm))])
+    zs, _ = _graded([(0.0, z0, seg(0, z0)), (z0, z0 + a, seg(z0, z0 + a)), (z0 + a, length, seg(z0 + a, length))])
+    ys = _grid_axis(0.0, b, ny)
+    nx, nz = len(xs) - 1, len(zs) - 1
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    xyz_all = np.stack([X.transpose(2, 1, 0).ravel(), Y.transpose(2, 1, 0).ravel(), Z.transpose(2, 1, 0).ravel()], 1)
+    tets, cell, (ci, cj, ck) = _box_tets(nx, ny, nz)
+    xc = 0.5 * (xs[ci] + xs[ci + 1])
+    zc = 0.5 * (zs[ck] + zs[ck + 1])
+    keep_cell = (xc < a) | ((zc > z0) & (zc < z0 + a))
+    tets = tets[keep_cell[cell]]
+    used = np.unique(tets)
+    remap = -np.ones(xyz_all.shape[0], np.int64)
+    remap[used] = np.arange(used.size)
+    tets = remap[tets]
+    xyz = xyz_all[used]
+    bf, _ = _boundary_faces(tets)
+    c = xyz[bf].mean(axis=1)
+    tol = 1e-9
+    attr = np.ones(bf.shape[0], dtype=np.int32)          # PEC walls
+    attr[np.abs(c[:, 2]) < tol] = 2
+    attr[np.abs(c[:, 2] - length) < tol] = 3
+    attr[np.abs(c[:, 0] - (a + arm)) < tol] = 4
+    m = TetMesh(xyz=xyz, tets=tets.astype(np.int32), tet_attr=np.ones(tets.shape[0], np.int32),
+                bfaces=bf.astype(np.int32), bface_attr=attr,
+                bdr_kind={1: "pec", 2: "port", 3: "port", 4: "port"},
+                port_normal={2: np.array([0.0, 0.0, -1.0]), 3: np.array([0.0, 0.0, 1.0]),
+                             4: np.array([1.0, 0.0, 0.0])},
+                materials={1: dict(er=1.0, tand=0.0, mur=1.0)},
+                name=f"tee_{nx}x{ny}x{nz}")
+    m.geometry = dict(a=a, b=b, length=length, arm=arm, z0=z0, ports={
+        2: dict(origin=np.zeros(3), u=np.array([1.0, 0.0, 0.0])),
+        3: dict(origin=np.zeros(3), u=np.array([1.0, 0.0, 0.0])),
+        4: dict(origin=np.array([0.0, 0.0, z0]), u=np.array([0.0, 0.0, 1.0]))})
     return m
 
 

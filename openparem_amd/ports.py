@@ -36,9 +36,14 @@ def te10(mesh, attr, freq):
     n = np.asarray(mesh.port_normal[attr], dtype=np.float64)
     y_over = gamma / (1j * omega * MU0 * mur)
 
+    # cross-section corner and width axis of this port (x from 0 on the straight guide); E_t along y
+    pg = mesh.geometry.get("ports", {}).get(attr)
+    origin = np.zeros(3) if pg is None else np.asarray(pg["origin"], dtype=np.float64)
+    u = np.array([1.0, 0.0, 0.0]) if pg is None else np.asarray(pg["u"], dtype=np.float64)
+
     def E(pts):
         e = np.zeros(pts.shape, np.complex128)
-        e[..., 1] = np.sin(kc * pts[..., 0])
+        e[..., 1] = np.sin(kc * ((pts - origin) @ u))
         return e
 
     def H(pts):
@@ -51,7 +56,7 @@ def te10(mesh, attr, freq):
 
 
 def provider_for(mesh):
-    if mesh.name.startswith("waveguide"):
+    if mesh.name.startswith("waveguide") or mesh.name.startswith("tee"):
         return te10
     if mesh.name.startswith("microstrip"):
         from .microstrip_mode import quasi_tem

@@ -239,6 +239,25 @@ def test_lossy_guide_is_passive():
     assert abs(S[1, 0]) < 1.0
 
 
+def test_tee_is_lossless_reciprocal_and_symmetric():
+    """Three-port H-plane tee (air, PEC walls, TE10 at every port): the unforced S of a lossless reciprocal
+    junction is unitary and symmetric, and the two main-guide ports are mirror images, so S11 = S22 and
+    S13 = S23 -- up to discretization error, which falls with the cell size."""
+    from openparem_amd.mesh import tee_mesh
+    from openparem_amd.ports import te10
+    errs = []
+    for h in (4e-3, 3e-3):
+        m = tee_mesh(h=h, ny=2, length=50e-3, arm=20e-3)
+        pr = Problem(m, 2)
+        S = pr.solve_point(9e9, {a: te10(m, a, 9e9) for a in pr.ports})["S_raw"]
+        e = max(np.abs(S.conj().T @ S - np.eye(3)).max(), np.abs(S - S.T).max(),
+                abs(S[0, 0] - S[1, 1]), abs(S[0, 2] - S[1, 2]))
+        errs.append(e)
+        assert e < 1e-2
+        assert 0.3 < abs(S[2, 0]) < 0.9  # power does split into the side arm
+    assert errs[1] < errs[0]
+
+
 # ----------------------------------------------------------------------------- meshes / providers
 def test_meshes_are_valid():
     m = waveguide_mesh()
