@@ -9,7 +9,7 @@ every workload is reproducible without files:
   ``regression/OpenParEM3D/WR90/straight*``).
 * ``microstrip_mesh`` - shielded microstrip box: substrate, zero-thickness
   surface-impedance trace, PEC ground/walls, 1st-order radiation lid, two ports.
-* ``tee_mesh``        - H-plane tee junction of rectangular guides, three wave ports.
+* ``tee_mesh``        - H-plane tee (three wave ports) or cross (four) junction of rectangular guides.
 * ``random_tets``     - independent random affine tetrahedra (assembly-only workload).
 
 Structured grids are split into 6 Kuhn tetrahedra per hexahedral cell (all cells
@@ -142,17 +142,19 @@ def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jit
     return m
 
 
-def tee_mesh(a=22.86e-3, b=10.16e-3, length=60e-3, arm=25e-3, z0=None, h=2.5e-3, ny=3):
+def tee_mesh(a=22.86e-3, b=10.16e-3, length=60e-3, arm=25e-3, z0=None, h=2.5e-3, ny=3, cross=False):
     """H-plane tee: main guide x in [0,a], y in [0,b], z in [0,L] with ports at z=0 (attr 2) and z=L (attr 3);
-    side arm x in [a, a+arm], z in [z0, z0+a] with its port at x=a+arm (attr 4).  A structured Kuhn grid of cell
-    size about h (no jitter: the junction's reentrant edges stay on grid lines).  Each port's TE10 mode is
-    described in ``geometry["ports"]`` by the corner of its cross-section, the unit width axis and a, b."""
+    side arm x in [a, a+arm], z in [z0, z0+a] with its port at x=a+arm (attr 4); with ``cross`` a second arm
+    x in [-arm, 0] with its port at x=-arm (attr 5).  A structured Kuhn grid of cell size about h (no jitter:
+    the junction's reentrant edges stay on grid lines).  Each port's TE10 mode is described in
+    ``geometry["ports"]`` by the corner of its cross-section, the unit width axis and a, b."""
     z0 = (length - a) / 2 if z0 is None else z0
 
     def seg(lo, hi):
         return max(1, int(round((hi - lo) / h)))
 
-    xs, _ = _graded([(0.0, a, seg(0, a)), (a, a + arm, seg(a, a + arm))])
+    xseg = ([(-arm, 0.0, seg(-arm, 0.0))] if cross else []) + [(0.0, a, seg(0, a)), (a, a + arm, seg(a, a + arm))]
+    xs, _ = _graded(xseg)
     zs, _ = _graded([(0.0, z0, seg(0, z0)), (z0, z0 + a, seg(z0, z0 + a)), (z0 + a, length, seg(z0 + a, length))])
     ys = _grid_axis(0.0, b, ny)
     nx, nz = len(xs) - 1, len(zs) - 1
@@ -161,7 +163,7 @@ def tee_mesh(a=22.86e-3, b=10.16e-3, length=60e-3, arm=25e-3, z0=None, h=2.5e-3,
     tets, cell, (ci, cj, ck) = _box_tets(nx, ny, nz)
     xc = 0.5 * (xs[ci] + xs[ci + 1])
     zc = 0.5 * (zs[ck] + zs[ck + 1])
-    keep_cell = (xc < a) | ((zc > z0) & (zc < z0 + a))
+    keep_cell = ((xc > 0) & (xc < a)) | ((zc > z0) & (zc < z0 + a))
     tets = tets[keep_cell[cell]]
     used = np.unique(tets)
     remap = -np.ones(xyz_all.shape[0], np.int64)
@@ -175,17 +177,21 @@ def tee_mesh(a=22.86e-3, b=10.16e-3, length=60e-3, arm=25e-3, z0=None, h=2.5e-3,
     attr[np.abs(c[:, 2]) < tol] = 2
     attr[np.abs(c[:, 2] - length) < tol] = 3
     attr[np.abs(c[:, 0] - (a + arm)) < tol] = 4
+    kinds = {1: "pec", 2: "port", 3: "port", 4: "port"}
+    normals = {2: np.array([0.0, 0.0, -1.0]), 3: np.array([0.0, 0.0, 1.0]), 4: np.array([1.0, 0.0, 0.0])}
+    ports = {2: dict(origin=np.zeros(3), u=np.array([1.0, 0.0, 0.0])),
+             3: dict(origin=np.zeros(3), u=np.array([1.0, 0.0, 0.0])),
+             4: dict(origin=np.array([0.0, 0.0, z0]), u=np.array([0.0, 0.0, 1.0]))}
+    if cross:
+        attr[np.abs(c[:, 0] + arm) < tol] = 5
+        kinds[5] = "port"
+        normals[5] = np.array([-1.0, 0.0, 0.0])
+        ports[5] = dict(origin=np.array([0.0, 0.0, z0]), u=np.array([0.0, 0.0, 1.0]))
     m = TetMesh(xyz=xyz, tets=tets.astype(np.int32), tet_attr=np.ones(tets.shape[0], np.int32),
-                bfaces=bf.astype(np.int32), bface_attr=attr,
-                bdr_kind={1: "pec", 2: "port", 3: "port", 4: "port"},
-                port_normal={2: np.array([0.0, 0.0, -1.0]), 3: np.array([0.0, 0.0, 1.0]),
-                             4: np.array([1.0, 0.0, 0.0])},
+                bfaces=bf.astype(np.int32), bface_attr=attr, bdr_kind=kinds, port_normal=normals,
                 materials={1: dict(er=1.0, tand=0.0, mur=1.0)},
                 name=f"tee_{nx}x{ny}x{nz}")
-    m.geometry = dict(a=a, b=b, length=length, arm=arm, z0=z0, ports={
-        2: dict(origin=np.zeros(3), u=np.array([1.0, 0.0, 0.0])),
-        3: dict(origin=np.zeros(3), u=np.array([1.0, 0.0, 0.0])),
-        4: dict(origin=np.array([0.0, 0.0, z0]), u=np.array([0.0, 0.0, 1.0]))})
+    m.geometry = dict(a=a, b=b, length=length, arm=arm, z0=z0, ports=ports)
     return m
 
 

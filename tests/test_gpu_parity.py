@@ -248,20 +248,22 @@ def test_sweep_microstrip_low_k0h():
     assert rm["iters"].max() <= 8 * r64["iters"].max()
 
 
-@pytest.mark.parametrize("prec", ["mixed", "fp64"])
-def test_sweep_three_port_tee(prec):
-    """Three ports (H-plane tee): the batched sweep's 3 x 3 S against the oracle's direct solve."""
+@pytest.mark.parametrize("cross,prec", [(False, "mixed"), (False, "fp64"), (True, "mixed"), (True, "fp64")])
+def test_sweep_tee_junctions(cross, prec):
+    """Three ports (H-plane tee) and four (cross): the batched sweep's N x N S against the oracle's direct
+    solve."""
     from openparem_amd.mesh import tee_mesh
     from openparem_amd.ports import te10
-    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3)
+    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=cross)
     f = np.array([9e9, 10e9])
+    n = 4 if cross else 3
     s = _solver(m, 2)
     try:
         r = s.sweep(f, precision=prec, tol=1e-11, batch=2, orth_passes=1 if prec == "mixed" else 2,
                     precond_share=2, arnoldi_step=3 if prec == "mixed" else 1)
     finally:
         s.close()
-    assert r["converged"].all() and r["S"].shape == (2, 3, 3)
+    assert r["converged"].all() and r["S"].shape == (2, n, n)
     pr = Problem(m, 2)
     for i, fi in enumerate(f):
         o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
