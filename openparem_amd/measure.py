@@ -100,7 +100,8 @@ def c5():
             tot_t += dt
             tot_pts += done
             tot_f += len(f)
-            _emit({"instance": i, "family": w["family"], "order": w["order"], "ndof": s.ndof, "freqs": len(f),
+            _emit({"instance": i, "family": w["family"], "ports": len(s.ports), "order": w["order"], "ndof": s.ndof,
+                   "freqs": len(f),
                    "share": share, "s": round(dt, 2), "points_converged": done,
                    "freq_points_per_s": round(done / dt, 3), "iters_mean": float(r["iters"].mean()),
                    "iters_max": int(r["iters"].max()), "fp64_mid_s": round(t64, 2),
@@ -108,7 +109,7 @@ def c5():
                    "max_abs_dS_mid": float(np.abs(r["S"][mid] - r64["S"][0]).max())})
         finally:
             s.close()
-    _emit({"instances": 12, "points": tot_f, "points_converged": tot_pts, "mixed_s": round(tot_t, 2),
+    _emit({"instances": i + 1, "points": tot_f, "points_converged": tot_pts, "mixed_s": round(tot_t, 2),
            "aggregate_freq_points_per_s": round(tot_pts / tot_t, 3)})
 
 

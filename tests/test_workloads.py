@@ -15,16 +15,18 @@ def test_named_workloads():
 
 
 def test_c5_set():
-    """Twelve seeded instances: alternating families, ND1/ND2 and one ND3, 8-32 frequencies in each family's band,
-    valid tetrahedra (positive volume, bounded aspect ratio)."""
+    """Fourteen seeded instances: alternating guide/microstrip families, ND1/ND2 and one ND3, then a 3-port tee and
+    a 4-port cross; 8-32 frequencies in each family's band; valid tetrahedra (positive volume, bounded aspect
+    ratio)."""
     inst = workloads.c5_instances()
-    assert len(inst) == 12
-    assert [w["family"] for w in inst] == ["waveguide", "microstrip"] * 6
+    assert len(inst) == 14
+    assert [w["family"] for w in inst] == ["waveguide", "microstrip"] * 6 + ["tee", "cross"]
     assert sorted({w["order"] for w in inst}) == [1, 2, 3] and sum(w["order"] == 3 for w in inst) == 1
+    assert [len([k for k, v in w["mesh"].bdr_kind.items() if v == "port"]) for w in inst[12:]] == [3, 4]
     for w in inst:
         f = w["freqs"]
         assert len(f) in (8, 16, 24, 32)
-        lo, hi = (7e9, 12.5e9) if w["family"] == "waveguide" else (2e9, 10e9)
+        lo, hi = (2e9, 10e9) if w["family"] == "microstrip" else (7e9, 12.5e9)
         assert np.isclose(f[0], lo) and np.isclose(f[-1], hi)
         m = w["mesh"]
         a, b, c, d = (m.xyz[m.tets[:, k]] for k in range(4))
