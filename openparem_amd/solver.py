@@ -128,6 +128,14 @@ class Solver:
     def sweep(self, freqs, precision="fp64", batch=0, restart=150, maxit=20000, tol=1e-10, reciprocal=True,
               h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch", coarse=0, orth_passes=2,
               precond_share=1, poly_degree=0, arnoldi_step=1, warm_start=0):
+        """One ``opem_sweep``: S (forced reciprocal unless ``reciprocal=False``) for every frequency, all port
+        excitations.  ``precision`` "fp64" or "mixed" (FP32 inner GMRES, FP64 refinement); ``batch`` frequencies per
+        batched solve (0: all); GMRES(``restart``) to the true residual ``tol``; ``precond`` "patch" (vertex
+        patches) or "element" (element blocks); ``precond_share`` adjacent frequencies per set of patch inverses;
+        ``arnoldi_step`` Krylov vectors per Gram-Schmidt pass; ``warm_start`` bisection levels over the frequency
+        index (interpolated starting iterates).  ``mode_fn(mesh, attr, f)`` supplies each port's modal E, H, gamma,
+        V, Z (default: the mesh family's provider).  Returns S, S_raw, Cp, Cm, iters, true_res, converged and,
+        with ``want_fields``, the E and H DOFs x, h [freq][excitation][ndof]."""
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
         nf, np_ = freqs.size, len(self.ports)
         mode_fn = mode_fn or provider_for(self.mesh)

@@ -327,13 +327,17 @@ def test_warm_start(nine_point_fp64, prec, levels):
     s = _solver(m, 2)
     try:
         kw = dict(precision=prec, tol=1e-10, batch=9, orth_passes=1 if prec == "mixed" else 2, precond_share=1)
-        rc = s.sweep(f, warm_start=0, **kw)
-        rw = s.sweep(f, warm_start=levels, **kw)
+        rc = s.sweep(f, warm_start=0, want_fields=True, **kw)
+        rw = s.sweep(f, warm_start=levels, want_fields=True, **kw)
         r8 = s.sweep(f[:8], warm_start=levels, **kw)  # even count: the last frequency is extrapolated
     finally:
         s.close()
     assert rw["converged"].all() and r8["converged"].all()
     assert np.abs(rw["S"] - r1["S"]).max() < 1e-6
+    # fields come back in the caller's frequency order although the batches solve them out of order
+    for key in ("x", "h"):
+        d = np.abs(rw[key] - rc[key]).max(axis=2) / np.abs(rc[key]).max(axis=2)
+        assert d.max() < 1e-6, key
     assert np.abs(r8["S"] - r1["S"][:8]).max() < 1e-6
     if prec == "fp64":
         assert rw["iters"][1::2].sum() < rc["iters"][1::2].sum()
