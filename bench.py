@@ -233,7 +233,7 @@ def main():
         "kernels": {
             ("k_mupdate_s" if nsb > 1 else "k_mupdate"): {"GB/s": b_mupd / t_mupd / 1e9, "alg_bytes": b_mupd,
                                                           "ms": t_mupd * 1e3},
-            "k_spmv_sweep_w (batched operator, one pass of Kt/M for all systems)": {
+            "k_spmv_sweep_p (batched operator, one pass of Kt/Mr over each 32-system tile)": {
                 "GB/s": b_spmv / t_spmv / 1e9, "alg_bytes": b_spmv, "ms": t_spmv * 1e3},
         },
         "cpu_baseline": cpu,
