@@ -270,6 +270,25 @@ def test_sweep_tee_junctions(cross, prec):
         assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
 
 
+def test_warm_start_three_ports():
+    """Warm start with three excitations per frequency (H-plane tee, five frequencies): S against the oracle."""
+    from openparem_amd.mesh import tee_mesh
+    from openparem_amd.ports import te10
+    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3)
+    f = np.linspace(7.5e9, 11.5e9, 5)
+    s = _solver(m, 2)
+    try:
+        r = s.sweep(f, precision="mixed", tol=1e-11, batch=5, orth_passes=1, precond_share=2, arnoldi_step=3,
+                    warm_start=1)
+    finally:
+        s.close()
+    assert r["converged"].all()
+    pr = Problem(m, 2)
+    for i, fi in enumerate(f):
+        o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+        assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
+
+
 def test_sweep_nd3():
     """ND3 (vertex patches exceed 100 DOFs: element-block Jacobi is used)."""
     from openparem_amd.ports import te10
