@@ -2129,12 +2129,15 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
     if (in->precond_share < 0 || in->precond_share * in->nports > share_max)
       throw std::invalid_argument(share_max == 32 ? "precond_share x nports must be <= 32"
                                                   : "precond_share x nports must be <= 8");
-    ctx->precond = in->precond;
-    ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
     if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");
-    ctx->poly_degree = in->poly_degree;
     if (in->arnoldi_step < 0 || in->arnoldi_step > 4) throw std::invalid_argument("arnoldi_step must be 0..4");
     if (in->warm_start < 0 || in->warm_start > 4) throw std::invalid_argument("warm_start must be 0..4");
+    for (int f = 0; f < in->nfreq; f++)
+      if (!(in->freqs[f] > 0)) throw std::invalid_argument("frequencies must be positive");
+    // the context keeps the options only once every argument has passed
+    ctx->precond = in->precond;
+    ctx->precond_share = in->precond_share > 0 ? in->precond_share : 1;
+    ctx->poly_degree = in->poly_degree;
     ctx->arnoldi_step = in->arnoldi_step > 1 ? in->arnoldi_step : 1;
     if (ctx->coarse_size != in->coarse_size) ctx->have_coarse = false;
     ctx->coarse_size = in->coarse_size;
