@@ -119,8 +119,9 @@ def _boundary_faces(tets):
 
 
 def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jitter=0.10, seed=1,
-                   er=1.0, tand=0.0):
-    """Rectangular guide x in [0,a], y in [0,b], z in [0,L]; ports at z=0 (attr 2) and z=L (attr 3)."""
+                   er=1.0, tand=0.0, short=False):
+    """Rectangular guide x in [0,a], y in [0,b], z in [0,L]; ports at z=0 (attr 2) and z=L (attr 3), or with
+    ``short`` one port at z=0 and a PEC wall at z=L (attr 3)."""
     rng = np.random.default_rng(seed)
     xs, ys, zs = _grid_axis(0, a, nx), _grid_axis(0, b, ny), _grid_axis(0, length, nz)
     feat = [{0, nx}, {0, ny}, {0, nz}]
@@ -134,10 +135,11 @@ def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jit
     attr[np.abs(c[:, 2] - length) < tol] = 3
     m = TetMesh(xyz=xyz, tets=tets.astype(np.int32), tet_attr=np.ones(tets.shape[0], np.int32),
                 bfaces=bf.astype(np.int32), bface_attr=attr,
-                bdr_kind={1: "pec", 2: "port", 3: "port"},
-                port_normal={2: np.array([0.0, 0.0, -1.0]), 3: np.array([0.0, 0.0, 1.0])},
+                bdr_kind={1: "pec", 2: "port", 3: "pec" if short else "port"},
+                port_normal={2: np.array([0.0, 0.0, -1.0])} if short else
+                {2: np.array([0.0, 0.0, -1.0]), 3: np.array([0.0, 0.0, 1.0])},
                 materials={1: dict(er=er, tand=tand, mur=1.0)},
-                name=f"waveguide_{nx}x{ny}x{nz}")
+                name=f"waveguide_{nx}x{ny}x{nz}" + ("_short" if short else ""))
     m.geometry = dict(a=a, b=b, length=length)
     return m
 

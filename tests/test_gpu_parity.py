@@ -289,6 +289,30 @@ def test_warm_start_three_ports():
         assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_sweep_one_port(prec):
+    """One excitation per frequency (guide shorted at z = L): S11 against the oracle and |S11| = 1 (the lossless
+    one-port); the mixed sweep batches nine frequencies on one inverse set with a warm start."""
+    from openparem_amd.ports import te10
+    m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4, short=True)
+    f = np.linspace(8e9, 12e9, 9)
+    s = _solver(m, 2)
+    try:
+        if prec == "fp64":
+            r = s.sweep(f, precision="fp64", tol=1e-11, batch=1, restart=60)
+        else:
+            r = s.sweep(f, precision="mixed", tol=1e-11, batch=9, orth_passes=1, precond_share=9, arnoldi_step=3,
+                        warm_start=1)
+    finally:
+        s.close()
+    assert r["converged"].all() and r["S"].shape == (9, 1, 1)
+    assert np.abs(np.abs(r["S"][:, 0, 0]) - 1).max() < 1e-8
+    pr = Problem(m, 2)
+    for i, fi in enumerate(f):
+        o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+        assert abs(r["S"][i, 0, 0] - o["S"][0, 0]) < 1e-8, (fi, r["S"][i], o["S"])
+
+
 def test_sweep_nd3():
     """ND3 (vertex patches exceed 100 DOFs: element-block Jacobi is used)."""
     from openparem_amd.ports import te10

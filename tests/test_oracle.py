@@ -211,6 +211,25 @@ def test_wr90_analytic_s21():
     assert abs(r["S_raw"][0, 1] - r["S_raw"][1, 0]) < 1e-3
 
 
+def test_shorted_guide_one_port():
+    """One port, PEC wall at z = L: the discrete system is lossless apart from the port, so the one-port reflects
+    everything, |S11| = 1 to rounding, whatever the mesh.  The phase follows S11 = -exp(-2 j beta L) at the port
+    plane up to discretization error, which falls from ND1 to ND2 (measured 9.8 -> 0.39 degrees at 10 GHz)."""
+    m = waveguide_mesh(nx=8, ny=4, nz=12, seed=1, short=True)
+    f = 10e9
+    k0 = 2 * np.pi * f * np.sqrt(oasm.MU0 * oasm.EPS0)
+    beta = np.sqrt(k0 ** 2 - (np.pi / m.geometry["a"]) ** 2)
+    exact = -np.exp(-2j * beta * m.geometry["length"])
+    err = []
+    for p in (1, 2):
+        pr = Problem(m, p)
+        assert pr.ports == [2]
+        S = pr.solve_point(f, {a: te10_mode(m, a, f) for a in pr.ports})["S_raw"]
+        assert S.shape == (1, 1) and abs(abs(S[0, 0]) - 1) < 1e-12
+        err.append(abs(np.angle(S[0, 0] / exact, deg=True)))
+    assert err[1] < 0.6 and err[1] < err[0] / 10
+
+
 @pytest.mark.parametrize("p,tol", [(2, 0.03), (3, 0.005)])
 def test_reference_s_on_reference_mesh(p, tol):
     """S on the reference's 30-tet WR90 mesh against the reference's own results (results.ods).
