@@ -126,7 +126,9 @@ typedef struct {
   const double *V;              /* complex [nfreq][nports] modal voltage                          */
   const double *Z;              /* complex [nfreq][nports] modal impedance                        */
   int precision;                /* OPEM_PREC_FP64 or OPEM_PREC_MIXED                              */
-  int batch;                    /* frequencies per batched launch; 1 = one system group at a time */
+  int batch;                    /* frequencies per batched launch; 1 = one system group at a time, */
+                                /*    0 = all; lowered to at most 256 systems (batch x nports) and */
+                                /*    to what three quarters of the device memory holds           */
   int restart;                  /* GMRES(m), m <= 400 (0 -> 150)                                  */
   int maxit;                    /* Krylov iteration limit per system                              */
   double tol;                   /* true relative residual ||b - A x|| / ||b|| target              */

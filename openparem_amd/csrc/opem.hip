@@ -1546,8 +1546,23 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   d_pdoff.upload(pd_off, st);
   d_pd.upload(pd, st);
   const int nf = in->nfreq;
-  const int batch = std::max(1, std::min(in->batch > 0 ? in->batch : nf, nf));
   const int m = std::max(2, std::min(in->restart > 0 ? in->restart : 150, kMaxBasis - 2));
+  // frequencies per batch: the caller's choice (0: all), then at most 256 systems (the batched operator's limit)
+  // and at most what three quarters of the device memory holds by a conservative per-system estimate (Krylov
+  // basis and work vectors, FP64 iterate, residual and H-projection vectors, the frequency's FP64 matrix values,
+  // patch outputs or element-block inverses)
+  int batch = std::max(1, std::min(in->batch > 0 ? in->batch : nf, nf));
+  batch = std::min(batch, std::max(1, 256 / R));
+  {
+    size_t mfree = 0, mtotal = 0;
+    HIPCHK(hipMemGetInfo(&mfree, &mtotal));
+    const double vec = in->precision == OPEM_PREC_MIXED ? 8.0 : 16.0;
+    const bool patch = c.precond == 0 && c.have_patches;
+    const double per_sys = (double)(m + 4 + c.arnoldi_step) * n * vec + 10.0 * n * 16 + (double)s.nnz * 16 / R +
+                           (patch ? (double)c.patches.idx.size() * 16 : (double)s.nd * n * 16);
+    const double fit = std::floor(0.75 * (double)mtotal / (per_sys * R));
+    if (fit < batch) batch = (int)std::max(1.0, fit);
+  }
   const int passes = in->orth_passes == 1 ? 1 : 2;
   const double tol = in->tol > 0 ? in->tol : 1e-10;
   const double htol = in->h_tol > 0 ? in->h_tol : 1e-13;

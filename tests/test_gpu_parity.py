@@ -402,6 +402,21 @@ def test_shared_patch_inverse_limits(nine_point_fp64):
         s.close()
 
 
+def test_sweep_splits_large_batches():
+    """130 frequencies x 2 ports with batch = 0 (all) exceed the 256 systems of one batch: the library splits the
+    sweep (128 + 2 frequencies) instead of refusing it, with the S of batches of 13."""
+    m = waveguide_mesh(nx=3, ny=2, nz=4, seed=5)
+    f = np.linspace(8e9, 12e9, 130)
+    s = _solver(m, 1)
+    try:
+        ra = s.sweep(f, precision="fp64", batch=0, restart=60, tol=1e-11)
+        rb = s.sweep(f, precision="fp64", batch=13, restart=60, tol=1e-11)
+    finally:
+        s.close()
+    assert ra["converged"].all() and rb["converged"].all()
+    assert np.abs(ra["S"] - rb["S"]).max() < 1e-9
+
+
 @pytest.mark.parametrize("deg,prec", [(3, "mixed"), (4, "mixed"), (5, "fp64")])
 def test_polynomial_preconditioner(nine_point_fp64, deg, prec):
     """GMRES-polynomial preconditioning (Krylov space of B p(B), roots from a deg-step Arnoldi run per system):
