@@ -7,10 +7,11 @@
 * ``waveguide-sweep`` the c1 guide on a 24x12x36 grid (~0.43 M ND2 DOF), 64 frequencies over the TE10-only
                      band 7-12.5 GHz x 2 ports = 128 systems on one pattern.
 
-``c5_instances()`` is a seeded mixed set (14 instances): twelve guide or microstrip instances, ND1/ND2 (one small
-ND3), target DOF count log-uniform in 50 k - 1.2 M, 8-32 frequencies, 2 ports; then an H-plane tee (3 ports) and
-a cross (4 ports) of guides at ND2.  Guides, tees and crosses sweep the TE10 band 7-12.5 GHz; microstrips sweep
-2-10 GHz, below the box's multimode band.
+``c5_instances()`` is a seeded mixed set (16 instances): twelve guide or microstrip instances, ND1/ND2 (one small
+ND3), target DOF count log-uniform in 50 k - 1.2 M, 8-32 frequencies, 2 ports; then four junctions of guides: an
+H-plane tee (3 ports) and a cross (4 ports) at ND2, a finer tee at ND1 (148 k DOF, 24 frequencies) and a coarser
+cross at ND2 (77 k DOF, 32 frequencies).  Guides, tees and crosses sweep the TE10 band 7-12.5 GHz; microstrips
+sweep 2-10 GHz, below the box's multimode band.
 """
 from __future__ import annotations
 
@@ -65,8 +66,9 @@ def c5_instances(seed=5):
                                 seed=100 + i)
             out.append(dict(family=family, order=order, mesh=m, freqs=np.linspace(2e9, 10e9, nf),
                             boundaries={4: COPPER}))
-    for cross, h, nf in ((False, 1.6e-3, 16), (True, 1.8e-3, 8)):
+    for cross, h, nf, order in ((False, 1.6e-3, 16, 2), (True, 1.8e-3, 8, 2), (False, 1.0e-3, 24, 1),
+                                (True, 2.4e-3, 32, 2)):
         m = tee_mesh(h=h, ny=max(2, round(10.16e-3 / h)), cross=cross)
-        out.append(dict(family="cross" if cross else "tee", order=2, mesh=m, freqs=np.linspace(7e9, 12.5e9, nf),
-                        boundaries={}))
+        out.append(dict(family="cross" if cross else "tee", order=order, mesh=m,
+                        freqs=np.linspace(7e9, 12.5e9, nf), boundaries={}))
     return out

@@ -15,14 +15,14 @@ def test_named_workloads():
 
 
 def test_c5_set():
-    """Fourteen seeded instances: alternating guide/microstrip families, ND1/ND2 and one ND3, then a 3-port tee and
-    a 4-port cross; 8-32 frequencies in each family's band; valid tetrahedra (positive volume, bounded aspect
+    """Sixteen seeded instances: alternating guide/microstrip families, ND1/ND2 and one ND3, then 3-port tees and
+    4-port crosses; 8-32 frequencies in each family's band; valid tetrahedra (positive volume, bounded aspect
     ratio)."""
     inst = workloads.c5_instances()
-    assert len(inst) == 14
-    assert [w["family"] for w in inst] == ["waveguide", "microstrip"] * 6 + ["tee", "cross"]
+    assert len(inst) == 16
+    assert [w["family"] for w in inst] == ["waveguide", "microstrip"] * 6 + ["tee", "cross"] * 2
     assert sorted({w["order"] for w in inst}) == [1, 2, 3] and sum(w["order"] == 3 for w in inst) == 1
-    assert [len([k for k, v in w["mesh"].bdr_kind.items() if v == "port"]) for w in inst[12:]] == [3, 4]
+    assert [len([k for k, v in w["mesh"].bdr_kind.items() if v == "port"]) for w in inst[12:]] == [3, 4, 3, 4]
     for w in inst:
         f = w["freqs"]
         assert len(f) in (8, 16, 24, 32)
