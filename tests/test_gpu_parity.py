@@ -227,6 +227,32 @@ def test_sweep_microstrip(coarse, passes):
         assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
 
 
+def test_c1_guide_against_oracle_fixture():
+    """The 12x6x18 WR90 guide at 10 GHz, ND2, 53 280 DOF: S against the oracle's direct solve (the fixture
+    tests/golden/c1_guide_10ghz.json, written by tests/golden/make_c1.py), plain FP64 to 1e-10 and mixed to 1e-8,
+    and against the closed form: angle S21 = -beta L = 88.0085 degrees, |S11| = 0 up to discretization."""
+    import json
+    import os
+    from openparem_amd import workloads
+    d = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_guide_10ghz.json")))
+    w = workloads.make("waveguide-c1")
+    s = _solver(w["mesh"], w["order"])
+    try:
+        assert s.ndof == d["ndof"] == 53280
+        r64 = s.sweep(w["freqs"], precision="fp64", tol=1e-12, restart=150)
+        rmx = s.sweep(w["freqs"], precision="mixed", tol=1e-10, restart=150, orth_passes=1, arnoldi_step=3)
+    finally:
+        s.close()
+    so = np.array([complex(*v) for v in d["S"]]).reshape(2, 2)
+    assert r64["converged"].all() and rmx["converged"].all()
+    e64, emx = np.abs(r64["S"][0] - so).max(), np.abs(rmx["S"][0] - so).max()
+    print(f"C1: |dS| fp64 {e64:.2e} mixed {emx:.2e}, iterations {r64['iters'].tolist()} {rmx['iters'].tolist()}",
+          flush=True)
+    assert e64 < 1e-10 and emx < 1e-8
+    assert abs(np.angle(r64["S"][0, 1, 0], deg=True) - 88.0085) < 0.1
+    assert abs(r64["S"][0, 0, 0]) < 5e-3
+
+
 def test_sweep_microstrip_low_k0h():
     """Low k0 h (0.3 and 0.6 GHz on a few-mm grid): the patch inverses are dominated by their near-kernel
     gradient part, which FP16 hi planes alone round away; with the lo planes the mixed solve must converge to
