@@ -182,6 +182,11 @@ def main():
     nsb = max(1, args.arnoldi_step) if not args.poly else 1
     (t_mdot, t_mupd), (b_mdot, b_mupd) = s.krylov_bench("fp32", S=S, k=kmean, nrep=5, ns=nsb)
     t_spmv, b_spmv = s.spmv_bench("fp32", ngroups=len(freqs), R=nports, nrep=10)
+    # The single-matrix complex CSR SpMV (explicit values, one vector) on the same pattern: the MatMult of a
+    # one-system solve, HBM-bound, each launch on its own copy of the matrix.  Rows grouped as the pattern allows
+    # (ND2: pairs of rows with one column list) and one row at a time.
+    csr = {(p_, g_): s.csr_spmv_bench(p_, row_group=g_, nrep=50) for p_ in ("fp64", "fp32") for g_ in (0, 1)}
+    csr_n, csr_nnz = s.ndof, s.nnz
     ratio_mdot = _pmc_traffic_ratio("k_mdot_s" if nsb > 1 else "k_mdot", f"S={S},k={kmean},ns={nsb},n={s.ndof}")
     its_mean = float(r["iters"].mean())
     s.close()
@@ -236,6 +241,16 @@ def main():
             "k_spmv_sweep_p (batched operator, one pass of Kt/Mr over each 32-system tile)": {
                 "GB/s": b_spmv / t_spmv / 1e9, "alg_bytes": b_spmv, "ms": t_spmv * 1e3},
         },
+        "spmv_csr": {
+            "kernel": "k_csr_spmv (explicit complex CSR y = A x, one matrix and one vector, int32 indices)",
+            "note": "frac: the bytes the kernel must move / time / 8 TB/s; canonical_frac: the plain CSR byte count "
+                    "nnz (s_v + 4) + 4 (N + 1) + 2 N s_v / time / 8 TB/s (the grouped kernel reads each column "
+                    "index once per row pair, so its own count is lower)",
+            **{f"{p_}_{'grouped' if g_ == 0 else 'rows'}": {
+                "GB/s": b_ / t_ / 1e9, "frac": b_ / t_ / 1e9 / HBM_PEAK_GBS, "alg_bytes": b_, "us": t_ * 1e6,
+                "canonical_frac": (csr_nnz * ((16 if p_ == "fp64" else 8) + 4) + 4 * (csr_n + 1)
+                                   + 2 * csr_n * (16 if p_ == "fp64" else 8)) / t_ / 1e9 / HBM_PEAK_GBS}
+               for (p_, g_), (t_, b_) in csr.items()}},
         "cpu_baseline": cpu,
         "assembly_c2": asm,
     }

@@ -106,6 +106,21 @@ int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const 
  * (s_r = 8 FP64 / 4 FP32 real, s_v = 16 / 8 complex; nov, nterm = boundary-term entries and terms). */
 int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, double *avg_seconds, double *alg_bytes);
 
+/* Times the single-matrix complex CSR SpMV y = A x (MatMult of KSPSolve on the explicit complex AIJ matrix,
+ * solveComplexLinearSystem.c:100-190, stored as hypre_ParCSRMatrixToMat builds it, csr.c:171-223) on the
+ * context's pattern: explicit complex values, one vector, device-resident synthetic inputs.  Each launch reads
+ * its own copy of the column indices and values (copies in rotation, at least 2 GiB in all) so no launch is
+ * served from an earlier one's cache lines.  prec 0 FP64, 1 FP32.
+ * row_group: 0 = the largest G in {3, 2, 1} such that every G consecutive rows share one column list (ND2: 2,
+ * ND3: 3, ND1: 1), 1 = one row at a time, 2 or 3 = that G (misuse if the pattern does not admit it); the kernel
+ * reads each column index and gathers each x value once per group.  lanes_per_row 0 = default (16 for groups,
+ * else the context's choice from the mean row length), else 4/8/16/32/64 (8/16/32 for groups).
+ * *avg_seconds = mean kernel duration over nrep launches (HIP events on the context stream); *alg_bytes = the
+ * bytes the kernel must move, nnz s_v + 4 nnz/G + 4 (N+1) + 2 N s_v (s_v = 16 FP64 / 8 FP32 complex); with G = 1
+ * this is the plain CSR count nnz (s_v + 4) + 4 (N+1) + 2 N s_v. */
+int opem_csr_spmv_bench(opem_ctx *ctx, int prec, int row_group, int lanes_per_row, int nrep, double *avg_seconds,
+                        double *alg_bytes);
+
 /* Times the Arnoldi orthogonalisation kernels of the batched GMRES (classical Gram-Schmidt inside
  * KSPGMRES, the VecMDot / VecMAXPY pair of solveComplexLinearSystem.c's KSPSolve) for S systems, a basis
  * of k vectors and ns new vectors per pass (1 = one-step Arnoldi, 2..4 = s-step): avg_seconds[0] =

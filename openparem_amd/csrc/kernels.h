@@ -49,6 +49,14 @@ template <typename V, typename X>
 void launch_spmv(hipStream_t st, int ngroups, int R, int64_t n, const int32_t *rowptr, const int32_t *colind,
                  const V *vals, int64_t nnz, const uint8_t *rowmask, const X *x, X *y, const X *b,
                  const int32_t *active, int mode, int lanes_per_row, int64_t vstride);
+// One explicit complex CSR matrix and one vector: y = rowmask .* (A x), rowmask null = every row free.
+// Accumulates in FP64 for FP32 storage too.  row_group G in {1, 2, 3}: G consecutive rows share one column list
+// (csr_row_group must have returned G for this pattern).  lanes_per_row: 4..64 for G = 1, 8/16/32 for G > 1.
+template <typename V>
+void launch_csr_spmv(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *colind, const V *vals,
+                     const uint8_t *rowmask, const V *x, V *y, int row_group, int lanes_per_row);
+// Largest G in {3, 2} for which every G consecutive rows (from row 0) share one column list, else 1.
+int csr_row_group(int64_t n, const int32_t *rowptr, const int32_t *colind);
 
 // ---------------------------------------------------------------- batched sweep SpMV
 // The S systems of a batch on one pattern: A_s = Kt + c_s (Mr - j Mi) + sum_t d_{s,t} T_t, with Kt, Mr, Mi

@@ -372,6 +372,140 @@ template void launch_spmv<float2, float2>(hipStream_t, int, int, int64_t, const 
                                           const float2 *, int64_t, const uint8_t *, const float2 *, float2 *,
                                           const float2 *, const int32_t *, int, int, int64_t);
 
+// ------------------------------------------------------------------ single-matrix complex CSR SpMV
+// One explicit complex matrix, one vector.  A row group is G consecutive rows with one column list (ND_p
+// numbers the DOFs of an edge, face or interior consecutively and they share their elements: G = 2 for ND2,
+// 3 for ND3, checked on the host); row g of the group keeps its values at beg + g*len.  LPR lanes share a
+// group, so each column index and each gathered x value serves G rows.  Per pass a lane issues U column
+// loads, G*U value loads and then U gathers of x before any arithmetic; positions past the row end are
+// clamped to its last entry and their products dropped by a select, so no load is conditional.
+template <typename V, int G, int LPR, int U>
+__global__ __launch_bounds__(256) void k_csr_spmv(int64_t n, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ colind, const V *__restrict__ vals,
+                                                  const uint8_t *__restrict__ rowmask, const V *__restrict__ x,
+                                                  V *__restrict__ y, int xcd_remap) {
+  const int sub = threadIdx.x % LPR;
+  const int64_t bid = xcd_remap ? xcd_logical(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t r0 = (bid * blockDim.x + threadIdx.x) / LPR * G;
+  if (r0 >= n) return;  // 256 % LPR == 0: a row group leaves together
+  const int beg = rowptr[r0], len = rowptr[r0 + 1] - beg;
+  double re[G], im[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) re[g] = im[g] = 0.0;
+  for (int j0 = 0; j0 < len; j0 += LPR * U) {
+    int c[U];
+    V a[G][U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + sub + u * LPR;
+      const int k = beg + (j < len ? j : len - 1);
+      c[u] = colind[k];
+#pragma unroll
+      for (int g = 0; g < G; g++) a[g][u] = vals[k + g * len];
+    }
+    V xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool ok = j0 + sub + u * LPR < len;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const double pr = (double)a[g][u].x * (double)xv[u].x - (double)a[g][u].y * (double)xv[u].y;
+        const double pi = (double)a[g][u].x * (double)xv[u].y + (double)a[g][u].y * (double)xv[u].x;
+        re[g] += ok ? pr : 0.0;
+        im[g] += ok ? pi : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+      re[g] += __shfl_xor(re[g], o, LPR);
+      im[g] += __shfl_xor(im[g], o, LPR);
+    }
+  }
+  if (sub < G) {  // after the butterfly every lane holds every row's sum; lane g writes row r0 + g
+    double rr = re[0], ii = im[0];
+#pragma unroll
+    for (int g = 1; g < G; g++)
+      if (sub == g) {
+        rr = re[g];
+        ii = im[g];
+      }
+    const bool free_row = rowmask == nullptr || rowmask[r0 + sub] != 0;
+    V out;
+    out.x = (decltype(out.x))(free_row ? rr : 0.0);
+    out.y = (decltype(out.y))(free_row ? ii : 0.0);
+    y[r0 + sub] = out;
+  }
+}
+
+template <typename V, int G, int LPR, int U>
+static void csr_spmv_go(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *colind, const V *vals,
+                        const uint8_t *rowmask, const V *x, V *y) {
+  const int64_t groups = n / G;
+  hipLaunchKernelGGL((k_csr_spmv<V, G, LPR, U>), dim3((unsigned)((groups * LPR + 255) / 256)), dim3(256), 0, st, n,
+                     rowptr, colind, vals, rowmask, x, y, xcd_remap_enabled());
+}
+
+template <typename V>
+void launch_csr_spmv(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *colind, const V *vals,
+                     const uint8_t *rowmask, const V *x, V *y, int row_group, int lanes_per_row) {
+  if (n <= 0) return;
+  if (n > (int64_t)(1u << 31) / 64) throw std::runtime_error("csr spmv: too many rows for one launch");
+  if (row_group < 1 || row_group > 3 || n % row_group)
+    throw std::runtime_error("csr spmv: row group must be 1, 2 or 3 and divide the row count");
+  const int key = row_group * 100 + lanes_per_row;
+  switch (key) {
+    case 104: csr_spmv_go<V, 1, 4, 4>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 108: csr_spmv_go<V, 1, 8, 4>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 116: csr_spmv_go<V, 1, 16, 4>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 132: csr_spmv_go<V, 1, 32, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 164: csr_spmv_go<V, 1, 64, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 208: csr_spmv_go<V, 2, 8, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 216: csr_spmv_go<V, 2, 16, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 232: csr_spmv_go<V, 2, 32, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 308: csr_spmv_go<V, 3, 8, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 316: csr_spmv_go<V, 3, 16, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    case 332: csr_spmv_go<V, 3, 32, 2>(st, n, rowptr, colind, vals, rowmask, x, y); break;
+    default:
+      throw std::runtime_error("csr spmv: lanes_per_row must be 4..64 for single rows, 8, 16 or 32 for row groups");
+  }
+  HIP_LAUNCH_CHECK();
+}
+template void launch_csr_spmv<double2>(hipStream_t, int64_t, const int32_t *, const int32_t *, const double2 *,
+                                       const uint8_t *, const double2 *, double2 *, int, int);
+template void launch_csr_spmv<float2>(hipStream_t, int64_t, const int32_t *, const int32_t *, const float2 *,
+                                      const uint8_t *, const float2 *, float2 *, int, int);
+
+// Largest G in {3, 2} such that G divides n and every G consecutive rows starting at a multiple of G have one
+// column list; 1 otherwise.
+int csr_row_group(int64_t n, const int32_t *rowptr, const int32_t *colind) {
+  for (int G = 3; G >= 2; G--) {
+    if (n % G) continue;
+    bool ok = true;
+    for (int64_t r0 = 0; ok && r0 < n; r0 += G) {
+      const int32_t beg = rowptr[r0], len = rowptr[r0 + 1] - beg;
+      for (int g = 1; ok && g < G; g++) {
+        const int32_t bg = rowptr[r0 + g];
+        if (rowptr[r0 + g + 1] - bg != len || bg != beg + g * len) {
+          ok = false;
+          break;
+        }
+        for (int32_t j = 0; j < len; j++)
+          if (colind[bg + j] != colind[beg + j]) {
+            ok = false;
+            break;
+          }
+      }
+    }
+    if (ok) return G;
+  }
+  return 1;
+}
+
 // ------------------------------------------------------------------ batched sweep SpMV
 // y[s][row] = rowmask[s % R][row] * (+/-) sum_j (A_s)_{row j} x[j][s]
 //   A_s = Kt + c_s (Mr - j Mi) + sum_t d_{s,t} T_t       (T_t on the boundary overlay rows only)

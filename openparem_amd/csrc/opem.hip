@@ -2100,18 +2100,17 @@ int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const i
     for (int64_t k = 0; k < nnz; k++)
       if (colind[k] < 0 || colind[k] >= n) throw std::invalid_argument("colind out of range");
     hipStream_t st = ctx->st;
-    DevBuf drp, dci, dv, dx, dy, dm;
+    DevBuf drp, dci, dv, dx, dy;
     drp.upload(std::vector<int32_t>(rowptr, rowptr + n + 1), st);
     dci.upload(std::vector<int32_t>(colind, colind + nnz), st);
-    dm.upload(std::vector<uint8_t>(n, 1), st);
-    const int lpr = nnz / n <= 12 ? 8 : nnz / n <= 48 ? 16 : 32;
+    const int group = csr_row_group(n, rowptr, colind);
+    const int lpr = group > 1 ? 16 : nnz / n <= 12 ? 8 : nnz / n <= 48 ? 16 : 32;
     if (prec == OPEM_PREC_FP64) {
       dv.upload(std::vector<double>(vals, vals + 2 * nnz), st);
       dx.upload(std::vector<double>(x, x + 2 * n), st);
       dy.alloc(n * 16);
-      launch_spmv<double2, double2>(st, 1, 1, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<double2>(), nnz,
-                                    dm.as<uint8_t>(), dx.as<double2>(), dy.as<double2>(), nullptr, nullptr, 0, lpr,
-                                    nnz);
+      launch_csr_spmv<double2>(st, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<double2>(), nullptr,
+                               dx.as<double2>(), dy.as<double2>(), group, lpr);
       HIPCHK(hipMemcpyAsync(y, dy.p, n * 16, hipMemcpyDeviceToHost, st));
     } else {
       std::vector<float> v32(2 * nnz), x32(2 * n), y32(2 * n);
@@ -2120,8 +2119,8 @@ int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const i
       dv.upload(v32, st);
       dx.upload(x32, st);
       dy.alloc(n * 8);
-      launch_spmv<float2, float2>(st, 1, 1, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<float2>(), nnz,
-                                  dm.as<uint8_t>(), dx.as<float2>(), dy.as<float2>(), nullptr, nullptr, 0, lpr, nnz);
+      launch_csr_spmv<float2>(st, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<float2>(), nullptr,
+                              dx.as<float2>(), dy.as<float2>(), group, lpr);
       HIPCHK(hipMemcpyAsync(y32.data(), dy.p, n * 8, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       for (int64_t i = 0; i < 2 * n; i++) y[i] = y32[i];
@@ -2264,6 +2263,70 @@ int opem_spmv_bench(opem_ctx *ctx, int prec, int ngroups, int R, int nrep, doubl
     if (alg_bytes)
       *alg_bytes = 4.0 * (n + 1) + 4.0 * nnz + 2.0 * sr * nnz + 4.0 * (n + 1) + 4.0 * nov + sr * nterm * nov +
                    (double)R * n + 2.0 * S * n * sv;
+  });
+}
+
+int opem_csr_spmv_bench(opem_ctx *ctx, int prec, int row_group, int lanes_per_row, int nrep, double *avg_seconds,
+                        double *alg_bytes) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if ((prec != 0 && prec != 1) || nrep < 1 || row_group < 0 || row_group > 3)
+      throw std::invalid_argument("bad csr spmv bench arguments");
+    Ctx &c = *ctx;
+    hipStream_t st = c.st;
+    const int64_t n = c.sp.ndof, nnz = c.sp.nnz;
+    const int detected = csr_row_group(n, c.sp.rowptr.data(), c.sp.colind.data());
+    const int group = row_group == 0 ? detected : row_group;
+    if (group > 1 && group != detected)
+      throw std::invalid_argument("the pattern's rows do not come in groups of that size");
+    const int lpr = lanes_per_row ? lanes_per_row : group > 1 ? 16 : c.lpr;
+    if (group == 1 ? (lpr != 4 && lpr != 8 && lpr != 16 && lpr != 32 && lpr != 64)
+                   : (lpr != 8 && lpr != 16 && lpr != 32))
+      throw std::invalid_argument("bad lanes per row for this row group");
+    const size_t sv = prec == 0 ? 16 : 8;
+    // Enough copies of (column indices, values) that consecutive launches stream at least 2 GiB of distinct
+    // matrix data: the device's last-level cache cannot serve a repeated launch from an earlier one.
+    const size_t per_copy = (size_t)nnz * (4 + sv);
+    const int ncopy = (int)std::min<size_t>(16, std::max<size_t>(2, ((size_t)2 << 30) / std::max<size_t>(1, per_copy) + 1));
+    std::vector<DevBuf> ci(ncopy), va(ncopy);
+    for (int k = 0; k < ncopy; k++) {
+      ci[k].alloc((size_t)nnz * 4);
+      HIPCHK(hipMemcpyAsync(ci[k].p, c.d_colind.p, (size_t)nnz * 4, hipMemcpyDeviceToDevice, st));
+      va[k].alloc((size_t)nnz * sv);
+      if (prec == 0)
+        hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, nnz, va[k].as<double2>(), 1.0);
+      else
+        hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, nnz, va[k].as<float2>(), 1.0);
+    }
+    DevBuf x, y;
+    x.alloc((size_t)n * sv);
+    y.alloc((size_t)n * sv);
+    if (prec == 0)
+      hipLaunchKernelGGL(k_fill<double2>, dim3(4096), dim3(256), 0, st, n, x.as<double2>(), 1.0);
+    else
+      hipLaunchKernelGGL(k_fill<float2>, dim3(4096), dim3(256), 0, st, n, x.as<float2>(), 1.0);
+    auto run = [&](int k) {
+      if (prec == 0)
+        launch_csr_spmv<double2>(st, n, c.d_rowptr.as<int32_t>(), ci[k].as<int32_t>(), va[k].as<double2>(), nullptr,
+                                 x.as<double2>(), y.as<double2>(), group, lpr);
+      else
+        launch_csr_spmv<float2>(st, n, c.d_rowptr.as<int32_t>(), ci[k].as<int32_t>(), va[k].as<float2>(), nullptr,
+                                x.as<float2>(), y.as<float2>(), group, lpr);
+    };
+    for (int k = 0; k < ncopy; k++) run(k);
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, st));
+    for (int r = 0; r < nrep; r++) run(r % ncopy);
+    HIPCHK(hipEventRecord(b, st));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (avg_seconds) *avg_seconds = ms * 1e-3 / nrep;
+    if (alg_bytes) *alg_bytes = (double)nnz * sv + 4.0 * (double)(nnz / group) + 4.0 * (n + 1) + 2.0 * n * sv;
   });
 }
 

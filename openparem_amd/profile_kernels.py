@@ -6,7 +6,8 @@
 Kernels, in dispatch order after the setup: the s-step projection / update k_mdot_s / k_mupdate_s (FP32 basis,
 128 systems x (restart/2 + 1) vectors, the mean basis of a GMRES(restart) cycle, ns = 3 new vectors per pass:
 one warm-up and two timed launches each), the batched
-operator (FP32, 64 frequencies x 2 excitations: one warm-up and two timed launches), then two 1 GiB
+operator (FP32, 64 frequencies x 2 excitations: one warm-up and two timed launches), the single-matrix complex
+CSR SpMV k_csr_spmv in FP64 and then FP32 (one launch per copy of the matrix, then two timed launches), then two 1 GiB
 device-to-device copies whose FETCH_SIZE / WRITE_SIZE calibrate the counters (a copy reads and writes
 exactly 1 GiB).
 """
@@ -29,8 +30,12 @@ def main():
     nf, R = len(w["freqs"]), len(s.ports)
     (tm, tu), (bm, bu) = s.krylov_bench("fp32", S=nf * R, k=args.restart // 2 + 1, nrep=2, ns=args.ns)
     ts, bs = s.spmv_bench("fp32", ngroups=nf, R=R, nrep=2)
+    tc64, bc64 = s.csr_spmv_bench("fp64", nrep=2)
+    tc32, bc32 = s.csr_spmv_bench("fp32", nrep=2)
     print(json.dumps({"k_mdot_s": {"ms": tm * 1e3, "alg_bytes": bm}, "k_mupdate_s": {"ms": tu * 1e3, "alg_bytes": bu},
-                      "spmv_sweep": {"ms": ts * 1e3, "alg_bytes": bs}}), flush=True)
+                      "spmv_sweep": {"ms": ts * 1e3, "alg_bytes": bs},
+                      "csr_spmv_fp64": {"ms": tc64 * 1e3, "alg_bytes": bc64},
+                      "csr_spmv_fp32": {"ms": tc32 * 1e3, "alg_bytes": bc32}}), flush=True)
     hip = C.CDLL("libamdhip64.so")
     n = 1 << 30
     a, d = C.c_void_p(), C.c_void_p()

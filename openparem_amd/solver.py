@@ -115,6 +115,15 @@ class Solver:
                                     C.byref(b)), self.ctx, "opem_spmv_bench")
         return t.value, b.value
 
+    def csr_spmv_bench(self, precision="fp64", row_group=0, lanes_per_row=0, nrep=50):
+        """(mean kernel seconds, bytes the kernel must move) of the single-matrix complex CSR SpMV with explicit
+        values on this pattern (each launch on its own copy of the matrix).  row_group 0 = rows grouped as the
+        pattern allows (ND2 pairs, ND3 triples), 1 = one row at a time."""
+        t, b = C.c_double(), C.c_double()
+        check(lib().opem_csr_spmv_bench(self.ctx, 0 if precision == "fp64" else 1, row_group, lanes_per_row, nrep,
+                                        C.byref(t), C.byref(b)), self.ctx, "opem_csr_spmv_bench")
+        return t.value, b.value
+
     def krylov_bench(self, precision="fp32", S=128, k=76, nrep=5, ns=1):
         """((projection s, update s), (projection bytes, update bytes)) of the batched Gram-Schmidt kernels with
         ns new vectors per pass."""

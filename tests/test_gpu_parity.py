@@ -142,6 +142,109 @@ def test_spmv(prec, tol):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("mean_len", [3, 30, 90])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_spmv_ragged(mean_len, prec, tol):
+    """Empty rows, one-entry rows and rows several passes long (up to 300 entries), at mean row lengths that
+    select 8, 16 and 32 lanes per row; checked against scipy's CSR product, entry by entry."""
+    from openparem_amd.solver import Device, spmv
+    print(f"spmv ragged mean_len={mean_len} {prec}", flush=True)
+    rng = np.random.default_rng(mean_len)
+    n = 3000
+    lens = rng.poisson(mean_len, n)
+    lens[rng.choice(n, 200, replace=False)] = 0
+    lens[rng.choice(n, 50, replace=False)] = 1
+    lens[rng.choice(n, 20, replace=False)] = 300
+    rows = np.repeat(np.arange(n), lens)
+    cols = rng.integers(0, n, rows.size)
+    vals = rng.standard_normal(rows.size) + 1j * rng.standard_normal(rows.size)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dev = Device()
+    try:
+        y = spmv(dev, A.indptr, A.indices, A.data, x, prec)
+    finally:
+        dev.close()
+    ref = A @ x
+    bound = abs(A) @ np.abs(x)
+    assert np.all(y[np.diff(A.indptr) == 0] == 0)
+    err = (np.abs(y - ref) / np.maximum(bound, 1e-300)).max()
+    assert err < tol, err
+
+
+def _grouped_csr(rng, ngroups, G, mean_len, n_cols=None, break_one=False):
+    """Random complex CSR whose rows come in groups of G with one column list (as ND2/ND3 patterns do), with
+    empty groups and groups several passes long."""
+    n = ngroups * G
+    n_cols = n_cols or n
+    lens = rng.poisson(mean_len, ngroups)
+    lens[rng.choice(ngroups, ngroups // 15, replace=False)] = 0
+    lens[rng.choice(ngroups, 10, replace=False)] = 200
+    rowptr = [0]
+    cols = []
+    for L in lens:
+        c = np.sort(rng.choice(n_cols, L, replace=False)).astype(np.int32)
+        for _ in range(G):
+            cols.append(c)
+            rowptr.append(rowptr[-1] + L)
+    colind = np.concatenate(cols)
+    if break_one:  # one row of one group gets a different first column: the grouping no longer applies
+        k = int(np.argmax(lens > 2)) * G + G - 1
+        row = colind[rowptr[k]:rowptr[k + 1]]
+        row[0] = np.setdiff1d(np.arange(n_cols), row)[0]
+        row.sort()
+    vals = rng.standard_normal(colind.size) + 1j * rng.standard_normal(colind.size)
+    return sp.csr_matrix((vals, colind, np.array(rowptr)), shape=(n, n_cols))
+
+
+@pytest.mark.parametrize("G,mean_len,break_one", [(2, 40, False), (3, 60, False), (2, 40, True), (3, 5, False)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_spmv_row_groups(G, mean_len, break_one, prec, tol):
+    """Rows in groups of 2 or 3 sharing one column list (the grouped kernel), and the same with one row changed
+    (the one-row kernel), against scipy's CSR product entry by entry; empty groups give zero rows."""
+    from openparem_amd.solver import Device, spmv
+    print(f"spmv row groups G={G} mean_len={mean_len} break={break_one} {prec}", flush=True)
+    rng = np.random.default_rng(100 * G + mean_len + break_one)
+    A = _grouped_csr(rng, 1500, G, mean_len, break_one=break_one)
+    x = rng.standard_normal(A.shape[1]) + 1j * rng.standard_normal(A.shape[1])
+    dev = Device()
+    try:
+        y = spmv(dev, A.indptr, A.indices, A.data, x, prec)
+    finally:
+        dev.close()
+    ref = A @ x
+    bound = abs(A) @ np.abs(x)
+    assert np.all(y[np.diff(A.indptr) == 0] == 0)
+    err = (np.abs(y - ref) / np.maximum(bound, 1e-300)).max()
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("p,G", [(1, 1), (2, 2), (3, 3)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_csr_spmv_bench_bytes(p, G, prec):
+    """The timing entry runs on the context's pattern, groups its rows as ND_p allows (1, 2, 3) and reports the
+    bytes of one launch; misuse (a group size the pattern does not admit, bad lanes) is an error."""
+    from openparem_amd._lib import OpemError
+    from openparem_amd.solver import Solver
+    m = waveguide_mesh(nx=4, ny=3, nz=5, seed=9)
+    s = Solver(m, p)
+    try:
+        sv = 16 if prec == "fp64" else 8
+        t, b = s.csr_spmv_bench(prec, nrep=4)
+        assert t > 0
+        assert b == s.nnz * sv + 4 * (s.nnz // G) + 4 * (s.ndof + 1) + 2 * s.ndof * sv
+        t1, b1 = s.csr_spmv_bench(prec, row_group=1, nrep=4)
+        assert t1 > 0 and b1 == s.nnz * (sv + 4) + 4 * (s.ndof + 1) + 2 * s.ndof * sv
+        with pytest.raises(OpemError, match="rc=-2"):
+            s.csr_spmv_bench(prec, row_group=1, lanes_per_row=12)
+        with pytest.raises(OpemError, match="rc=-2"):
+            s.csr_spmv_bench(prec, row_group=3 if G != 3 else 2)
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize("precond", ["patch", "element"])
 @pytest.mark.parametrize("p", [1, 2])
 def test_sweep_waveguide_fp64(p, precond):
