@@ -90,34 +90,8 @@ def _cpu_baseline(work, its_per_system, restart):
                         f"{its_per_system:.0f} iterations (the plain FP64 GPU path's mean)"))
 
 
-def _assembly_c2(order, n_elem, nrep=5):
-    """Assembly kernel alone (config 2): element matrices of n_elem random tets on MFMA in FP64, FP32 and
-    BF16-scaled; error against the FP64 oracle on the first 2000 elements."""
-    from openparem_amd.mesh import random_tets
-    from openparem_amd.solver import Device
-    from oracle import nd as ond
-
-    v, er, td = random_tets(n_elem, seed=7)
-    k0 = 2 * np.pi * 5e9 / 299792458.0 * 1e-2
-    nd = order * (order + 2) * (order + 3) // 2
-    K, M = ond.element_matrices(v[:2000], order)
-    ref = np.stack([K - (k0 * k0 * er[:2000])[:, None, None] * M, (k0 * k0 * er[:2000] * td[:2000])[:, None, None] * M], 1)
-    rn = np.linalg.norm(ref.reshape(2000, -1), axis=1)
-    out = {}
-    dev = Device()
-    try:
-        for prec, name, osz in ((0, "fp64", 8), (1, "fp32", 4), (2, "bf16", 4)):
-            _, t = dev.element_matrices(order, prec, v, er, td, k0, nrep=nrep, want=False)
-            got, _ = dev.element_matrices(order, prec, v[:2000], er[:2000], td[:2000], k0, nrep=1)
-            err = np.linalg.norm((got - ref).reshape(2000, -1), axis=1) / rn
-            b = n_elem * (12 * 8 + 2 * 8 + 2 * nd * nd * osz)
-            f = n_elem * (2 * 16 + 2 * 8) * nd * nd
-            out[name] = {"ms": t * 1e3, "GB/s": b / t / 1e9, "TFLOP/s": f / t / 1e12, "frac_hbm": b / t / 1e9 / HBM_PEAK_GBS,
-                         "rel_err_max": float(err.max()), "rel_err_p99": float(np.quantile(err, 0.99))}
-    finally:
-        dev.close()
-    return {"elements": n_elem, "order": order, "nd": nd, "bytes_per_element": "12*8 + 2*8 + 2*nd^2*s_out",
-            **out}
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -144,6 +118,7 @@ def main():
         return 2
 
     from openparem_amd import workloads
+    from openparem_amd.measure import assembly_c2
     from openparem_amd.solver import Solver
 
     work = workloads.make(args.workload)
@@ -156,6 +131,7 @@ def main():
         s.sweep(freqs, precision="mixed", batch=len(freqs), orth_passes=1, precond_share=args.share,
                 poly_degree=args.poly, arnoldi_step=args.arnoldi_step, warm_start=args.warm_start,
                 **opts)
+    _progress(f"{args.warmup} warm-up sweep(s) done ({s.ndof} DOF, {len(freqs)} frequencies)")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = s.sweep(freqs, precision="mixed", batch=len(freqs), orth_passes=1, precond_share=args.share,
@@ -166,6 +142,7 @@ def main():
     done = int(np.all(r["converged"].astype(bool), axis=1).sum())
     value = args.steps * done / elapsed
 
+    _progress(f"{args.steps} timed sweep(s): {elapsed:.2f} s")
     # plain FP64 GPU path on an evenly spaced subset
     idx = np.unique(np.linspace(0, len(freqs) - 1, args.fp64_points).round().astype(int))
     t2 = time.perf_counter()
@@ -173,6 +150,7 @@ def main():
     t3 = time.perf_counter()
     fp64_value = len(idx) / (t3 - t2)
     dS = float(np.abs(r["S"][idx] - r64["S"]).max())
+    _progress(f"plain FP64 subset: {len(idx)} points in {t3 - t2:.2f} s")
 
     # Kernel rooflines at the batched shape of the sweep (all frequencies x excitations, FP32 inner solve):
     # the Gram-Schmidt projection (the largest share of the solve) with a basis of m/2 + 1 vectors (the mean
@@ -190,11 +168,14 @@ def main():
     ratio_mdot = _pmc_traffic_ratio("k_mdot_s" if nsb > 1 else "k_mdot", f"S={S},k={kmean},ns={nsb},n={s.ndof}")
     its_mean = float(r["iters"].mean())
     s.close()
+    _progress("kernel timings done")
 
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(work, float(r64["iters"].mean()), args.restart)
-    asm = None if args.no_assembly else _assembly_c2(2, 1_000_000)
+    if cpu is not None:
+        _progress("CPU baseline done")
+    asm = None if args.no_assembly else assembly_c2(2, 1_000_000, HBM_PEAK_GBS)
 
     line = {
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",

@@ -81,16 +81,33 @@ struct ElemT<2> {
 };
 
 // tab layout: Tre[16][NP] then Tim[8][NP] (K rows, padded N); bf16: Tre[32][NP], Tim[32][NP]
+// Output staging: a wave's 16 elements own one contiguous output range (16 x 2 nd^2 values), but an MFMA tile
+// holds only 16 columns of each element, i.e. 64-128 byte pieces that straddle cache lines whenever nd^2 is not a
+// multiple of 16.  The wave therefore computes CW columns (CW/16 tiles) of both planes into its LDS stage
+// Stage[wave][2][16][CW] and then stores each element's CW-long runs with consecutive lanes on consecutive addresses.
 template <int PREC>
+struct ElemCW {
+  static constexpr int CW = PREC == 0 ? 32 : 64;  // 8 KB of stage per wave at every precision
+};
+
+// NT: non-temporal output stores.  They pay when every element block (nd^2 values) is a whole number of 64-byte
+// segments, so each run covers whole segments (ND2: +5-40% measured); where runs straddle segments (ND1 FP64, ND3)
+// the L2 must merge the partial lines and plain stores are faster.
+template <int PREC, bool NT>
 __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const double *__restrict__ verts,
                                               const double *__restrict__ er, const double *__restrict__ tand,
                                               double k0, const typename ElemT<PREC>::T *__restrict__ tab,
                                               typename ElemT<PREC>::O *__restrict__ out) {
   using T = typename ElemT<PREC>::T;
   using O = typename ElemT<PREC>::O;
-  __shared__ double X[4][16][24];  // per wave: 16 elements x (16 re + 8 im) factors
-  __shared__ double sc[4][16];
+  constexpr int CW = ElemCW<PREC>::CW;
+  // Per wave one 8 KB region: first the 16 elements' factors X[16][24] (16 re + 8 im) and BF16 scales sc[16],
+  // read into registers before the column loop, then the output stage [2][16][CW].
+  static_assert(2 * 16 * CW * sizeof(O) >= 16 * 24 * 8 + 16 * 8, "stage too small for the factors");
+  __shared__ __align__(16) O Stage[4][2][16][CW];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double(*X1)[24] = (double(*)[24])(void *)Stage[wave];
+  double *sc1 = (double *)(void *)Stage[wave] + 16 * 24;
   const int64_t e0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
   if (lane < 16) {
     const int64_t e = e0 + lane;
@@ -115,88 +132,117 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
       const double si = mi > 0.0 ? mi : 1.0;
       for (int k = 0; k < 16; k++) row[k] /= s;
       for (int k = 16; k < 24; k++) row[k] /= si;
-      sc[wave][lane] = s;
-      X[wave][lane][23] = si;  // stash imag scale in the pad slot (row[22..23] are zero padding)
-      row[23] = si;
+      sc1[lane] = s;
+      row[23] = si;  // imag scale in the pad slot (row[22..23] are zero padding)
     }
-    for (int k = 0; k < 24; k++) X[wave][lane][k] = row[k];
+    for (int k = 0; k < 24; k++) X1[lane][k] = row[k];
   }
-  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (e0 >= ne) return;
   const int i16 = lane & 15, kq = lane >> 4;
   const int ndd = nd * nd;
+  const int nel = ne - e0 < 16 ? (int)(ne - e0) : 16;
   O *ob = out + e0 * (int64_t)(2 * ndd);
   const int ntile = NP / 16;
+  O(*st)[16][CW] = Stage[wave];
+  // A operands, fixed over the column tiles
+  T are[4], aim[2];
+  bf8_t bre_a, bim_a;
   if constexpr (PREC == 0 || PREC == 1) {
-    using V4 = typename std::conditional<PREC == 0, d4_t, f4_t>::type;
-    // A operands (fixed over N tiles): Re K=16 -> 4 steps, Im K=8 -> 2 steps
-    T are[4], aim[2];
-    for (int kb = 0; kb < 4; kb++) are[kb] = (T)X[wave][i16][kb * 4 + kq];
-    for (int kb = 0; kb < 2; kb++) aim[kb] = (T)X[wave][i16][16 + kb * 4 + kq];
-    const T *Tre = tab;
-    const T *Tim = tab + 16 * NP;
-    for (int t = 0; t < ntile; t++) {
-      const int n = t * 16 + i16;
-      V4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
-#pragma unroll
-      for (int kb = 0; kb < 4; kb++) {
-        const T b = Tre[(kb * 4 + kq) * NP + n];
-        if constexpr (PREC == 0)
-          cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kb], b, cre, 0, 0, 0);
-        else
-          cre = __builtin_amdgcn_mfma_f32_16x16x4f32(are[kb], b, cre, 0, 0, 0);
-      }
-#pragma unroll
-      for (int kb = 0; kb < 2; kb++) {
-        const T b = Tim[(kb * 4 + kq) * NP + n];
-        if constexpr (PREC == 0)
-          cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kb], b, cim, 0, 0, 0);
-        else
-          cim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[kb], b, cim, 0, 0, 0);
-      }
-      if (n < ndd) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          // f64 16x16x4: D row i in lane group i%4, register i/4; f32 16x16x4: group i/4, register i%4
-          const int el = PREC == 0 ? r * 4 + kq : kq * 4 + r;
-          if (e0 + el < ne) {
-            ob[(int64_t)el * 2 * ndd + n] = (O)cre[r];
-            ob[(int64_t)el * 2 * ndd + ndd + n] = (O)cim[r];
-          }
-        }
-      }
-    }
+    for (int kb = 0; kb < 4; kb++) are[kb] = (T)X1[i16][kb * 4 + kq];
+    for (int kb = 0; kb < 2; kb++) aim[kb] = (T)X1[i16][16 + kb * 4 + kq];
   } else {
     // bf16: one K=32 step each; lanes kq = 0,1 carry K rows 0..15, kq = 2,3 zeros
-    bf8_t are, aim;
     for (int t = 0; t < 8; t++) {
       const int k = kq * 8 + t;
-      are[t] = (__bf16)(k < 16 ? (float)X[wave][i16][k] : 0.0f);
-      aim[t] = (__bf16)(k < 6 ? (float)X[wave][i16][16 + k] : 0.0f);
+      bre_a[t] = (__bf16)(k < 16 ? (float)X1[i16][k] : 0.0f);
+      bim_a[t] = (__bf16)(k < 6 ? (float)X1[i16][16 + k] : 0.0f);
     }
-    const __bf16 *Tre = tab;
-    const __bf16 *Tim = tab + 32 * NP;
-    for (int tt = 0; tt < ntile; tt++) {
-      const int n = tt * 16 + i16;
-      bf8_t bre, bim;
-      for (int t = 0; t < 8; t++) {
-        bre[t] = Tre[(kq * 8 + t) * NP + n];
-        bim[t] = Tim[(kq * 8 + t) * NP + n];
-      }
-      f4_t cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
-      cre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(are, bre, cre, 0, 0, 0);
-      cim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim, bim, cim, 0, 0, 0);
-      if (n < ndd) {
+  }
+  double scr[4] = {1.0, 1.0, 1.0, 1.0}, sci[4] = {1.0, 1.0, 1.0, 1.0};
+  if constexpr (PREC == 2) {
+    for (int r = 0; r < 4; r++) {
+      scr[r] = sc1[kq * 4 + r];
+      sci[r] = X1[kq * 4 + r][23];
+    }
+  }
+  // the factors are in registers: the region becomes the output stage
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int c0 = 0; c0 < ndd; c0 += CW) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int el = kq * 4 + r;
-          if (e0 + el < ne) {
-            ob[(int64_t)el * 2 * ndd + n] = (O)(cre[r] * sc[wave][el]);
-            ob[(int64_t)el * 2 * ndd + ndd + n] = (O)(cim[r] * X[wave][el][23]);
-          }
+    for (int tt = 0; tt < CW / 16; tt++) {
+      const int t = c0 / 16 + tt;
+      if (t >= ntile) break;
+      const int n = t * 16 + i16;
+      f4_t fre = {0, 0, 0, 0}, fim = {0, 0, 0, 0};
+      d4_t dre = {0, 0, 0, 0}, dim = {0, 0, 0, 0};
+      if constexpr (PREC == 0 || PREC == 1) {
+        const T *Tre = tab;
+        const T *Tim = tab + 16 * NP;
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+          const T b = Tre[(kb * 4 + kq) * NP + n];
+          if constexpr (PREC == 0)
+            dre = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kb], b, dre, 0, 0, 0);
+          else
+            fre = __builtin_amdgcn_mfma_f32_16x16x4f32(are[kb], b, fre, 0, 0, 0);
+        }
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) {
+          const T b = Tim[(kb * 4 + kq) * NP + n];
+          if constexpr (PREC == 0)
+            dim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kb], b, dim, 0, 0, 0);
+          else
+            fim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[kb], b, fim, 0, 0, 0);
+        }
+      } else {
+        const __bf16 *Tre = tab;
+        const __bf16 *Tim = tab + 32 * NP;
+        bf8_t bre, bim;
+        for (int q = 0; q < 8; q++) {
+          bre[q] = Tre[(kq * 8 + q) * NP + n];
+          bim[q] = Tim[(kq * 8 + q) * NP + n];
+        }
+        fre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bre_a, bre, fre, 0, 0, 0);
+        fim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bim_a, bim, fim, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        // f64 16x16x4: D row i in lane group i%4, register i/4; f32 16x16x4: group i/4, register i%4
+        const int el = PREC == 0 ? r * 4 + kq : kq * 4 + r;
+        if constexpr (PREC == 0) {
+          st[0][el][tt * 16 + i16] = dre[r];
+          st[1][el][tt * 16 + i16] = dim[r];
+        } else {
+          st[0][el][tt * 16 + i16] = (O)(fre[r] * scr[r]);
+          st[1][el][tt * 16 + i16] = (O)(fim[r] * sci[r]);
         }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int len = ndd - c0 < CW ? ndd - c0 : CW;
+#pragma unroll
+    for (int it = 0; it < 2 * 16 * CW / 64; it++) {
+      const int idx = it * 64 + lane;
+      const int col = idx % CW, rest = idx / CW;
+      const int el = rest & 15, part = rest >> 4;
+      if (col < len && el < nel) {
+        O *dst = ob + (int64_t)el * 2 * ndd + part * ndd + c0 + col;
+        if constexpr (NT)
+          __builtin_nontemporal_store(st[part][el][col], dst);
+        else
+          *dst = st[part][el][col];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -254,16 +300,29 @@ void launch_element_matrices(hipStream_t st, int prec, int p, int64_t ne, const 
                              const double *tand, double k0, const void *tab, void *, void *out) {
   const int nd = nd_tet_ndof(p), NP = padded(nd);
   const unsigned blocks = (unsigned)((ne + 63) / 64);
-  if (prec == 0)
-    hipLaunchKernelGGL((k_elem<0>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                       (const double *)tab, (double *)out);
-  else if (prec == 1)
-    hipLaunchKernelGGL((k_elem<1>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                       (const float *)tab, (float *)out);
-  else if (prec == 2)
-    hipLaunchKernelGGL((k_elem<2>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                       (const __bf16 *)tab, (float *)out);
-  else
+  const bool nt = (nd * nd * (prec == 0 ? 8 : 4)) % 64 == 0;
+  if (prec == 0) {
+    if (nt)
+      hipLaunchKernelGGL((k_elem<0, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                         (const double *)tab, (double *)out);
+    else
+      hipLaunchKernelGGL((k_elem<0, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                         (const double *)tab, (double *)out);
+  } else if (prec == 1) {
+    if (nt)
+      hipLaunchKernelGGL((k_elem<1, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                         (const float *)tab, (float *)out);
+    else
+      hipLaunchKernelGGL((k_elem<1, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                         (const float *)tab, (float *)out);
+  } else if (prec == 2) {
+    if (nt)
+      hipLaunchKernelGGL((k_elem<2, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                         (const __bf16 *)tab, (float *)out);
+    else
+      hipLaunchKernelGGL((k_elem<2, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
+                         (const __bf16 *)tab, (float *)out);
+  } else
     throw std::runtime_error("element matrices: bad precision");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("element kernel launch: ") + hipGetErrorString(e));

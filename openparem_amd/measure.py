@@ -4,6 +4,9 @@
     python -m openparem_amd.measure c4   # 0.48 M ND2 DOF, 64 frequencies 1-20 GHz x 2 ports, in four batches of 16,
                                          # then plain FP64 at four of the frequencies
 
+    python -m openparem_amd.measure c2   # assembly kernel alone: element matrices of 1M random tets, ND1-ND3,
+                                         # FP64 / FP32 / BF16-scaled MFMA
+
     python -m openparem_amd.measure c5   # the seeded mixed set of workloads.c5_instances(): one batched mixed
                                          # sweep per instance, plain FP64 at its middle frequency
 
@@ -25,6 +28,48 @@ OPTS = dict(restart=300, tol=1e-10, maxit=8000)
 
 def _emit(d):
     print(json.dumps(d), flush=True)
+
+
+def assembly_c2(order, n_elem, hbm_peak_gbs=8000.0, nrep=5, n_err=20000):
+    """Assembly kernel alone (config 2): element matrices of n_elem random tets on MFMA in FP64, FP32 and
+    BF16-scaled.  Errors of FP32 and BF16 are against the FP64 kernel's matrices on the first n_err elements (the
+    FP64 kernel matches the FP64 oracle to 1e-12, tests/test_gpu_parity.py::test_element_matrices).  The kernel
+    writes 2 nd^2 values per element and is bound by those HBM writes at every precision."""
+    from .mesh import random_tets
+    from .solver import Device
+
+    v, er, td = random_tets(n_elem, seed=7)
+    k0 = 2 * np.pi * 5e9 / 299792458.0 * 1e-2
+    nd = order * (order + 2) * (order + 3) // 2
+    out = {}
+    dev = Device()
+    try:
+        ref = None
+        # dense MFMA peaks of MI355X, TFLOP/s
+        for prec, name, osz, peak in ((0, "fp64", 8, 78.6), (1, "fp32", 4, 157.3), (2, "bf16", 4, 2500.0)):
+            _, t = dev.element_matrices(order, prec, v, er, td, k0, nrep=nrep, want=False)
+            got, _ = dev.element_matrices(order, prec, v[:n_err], er[:n_err], td[:n_err], k0, nrep=1)
+            if ref is None:
+                ref = got
+                rn = np.linalg.norm(ref.reshape(n_err, -1), axis=1)
+            err = np.linalg.norm((got - ref).reshape(n_err, -1), axis=1) / rn
+            b = n_elem * (12 * 8 + 2 * 8 + 2 * nd * nd * osz)
+            f = n_elem * (2 * 16 + 2 * 8) * nd * nd
+            out[name] = {"ms": t * 1e3, "GB/s": b / t / 1e9, "TFLOP/s": f / t / 1e12, "frac_hbm": b / t / 1e9 / hbm_peak_gbs,
+                         "frac_mfma": f / t / 1e12 / peak,
+                         "rel_err_max_vs_fp64": float(err.max()), "rel_err_p99_vs_fp64": float(np.quantile(err, 0.99))}
+    finally:
+        dev.close()
+    return {"elements": n_elem, "order": order, "nd": nd, "bytes_per_element": "12*8 + 2*8 + 2*nd^2*s_out",
+            "flops_per_element": "2*(16 + 8)*nd^2 (MFMA contraction of 16 + 8 padded geometric factors)", **out}
+
+
+def c2():
+    for order in (1, 2, 3):
+        t = time.perf_counter()
+        r = assembly_c2(order, 1_000_000)
+        r["wall_s"] = round(time.perf_counter() - t, 1)
+        _emit(r)
 
 
 def c3():
@@ -115,4 +160,4 @@ def c5():
 
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c3"
-    {"c3": c3, "c4": c4, "c5": c5}[which]()
+    {"c2": c2, "c3": c3, "c4": c4, "c5": c5}[which]()
