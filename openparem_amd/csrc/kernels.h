@@ -75,9 +75,28 @@ struct SweepMatrix {
 };
 // y[s*n + row] = rowmask[s%R][row] * (mode ? -(A_s x_s) : A_s x_s)[row] for active s; x system-interleaved
 // (x[j*S + s]); csys[s] = c_s, dsys[s][t] complex.  S <= 256.
+// form (S > 8 only; S <= 8 always runs the narrow form): 0 = the fixed rule (paired tiles when S is even and
+// OPEM_SPMV_PAIR allows, else one-system-per-lane tiles, or the wide form under OPEM_SPMV_WIDE), 1 = paired tiles,
+// 2 = one-system-per-lane tiles, 3 = one row per wavefront.  Every form walks a row's entries in the same order.
 template <typename V, typename X>
 void launch_spmv_sweep(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
-                       const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr);
+                       const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr,
+                       int form = 0);
+// Whether form 1..3 can run this batch (the paired form needs S even and its staged rows within 64 KB of LDS).
+template <typename V>
+inline bool spmv_sweep_form_ok(const SweepMatrix<V> &M, int S, int form) {
+  if (S <= 8 || S > 256) return false;
+  if (form == 1)
+    return S % 2 == 0 && M.blk16_max > 0 &&
+           (size_t)M.blk16_max * ((M.Mi ? 3 : 2) * sizeof(V) + 4) <= 64 * 1024;
+  return form == 2 || form == 3;
+}
+// OPEM_SPMV_TUNE: 1 (default) times the forms of the batched operator once per batch shape and context and keeps
+// the fastest whose output is bit-identical to the fixed rule's; 0 keeps the fixed rule; 2 as 1, printing timings.
+// When OPEM_SPMV_PAIR or OPEM_SPMV_WIDE is set and OPEM_SPMV_TUNE is not, tuning is off (the forced form runs).
+int spmv_tune_level();
+// out += sum_i (2 i + 1) a[i] (mod 2^64) over count 64-bit words: a position-weighted checksum.
+void launch_hash64(hipStream_t st, int64_t count, const uint64_t *a, unsigned long long *out);
 // out64 = [Kt | Mr | Mi] (nnz each; Mi only if lossy), Kt = sum_a w[a][0] K_a, Mr = sum_a w[a][1] M_a,
 // Mi = sum_a w[a][2] M_a; out32 (optional) the same in FP32.
 void launch_base_matrices(hipStream_t st, int64_t nnz, int nslot, const double *K, const double *M, const double *w,

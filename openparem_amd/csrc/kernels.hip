@@ -827,8 +827,9 @@ __global__ __launch_bounds__(256) void k_spmv_sweep_n(SweepMatrix<V> M, int S, i
 
 template <typename V, typename X, bool LOSSY>
 static void spmv_sweep_l(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
-                         const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr) {
-  if (S > 8 && S % 2 == 0 && spmv_pair_enabled()) {
+                         const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr, int form) {
+  if (S > 8 && form == 0) form = S % 2 == 0 && spmv_pair_enabled() ? 1 : spmv_wide_enabled() ? 3 : 2;
+  if (S > 8 && form == 1) {
     if (S > 256) throw std::runtime_error("sweep spmv: at most 256 systems per batch");
     if (M.blk16_max <= 0) throw std::runtime_error("sweep spmv: blk16_max not set");
     const int64_t nb = ((M.n + 15) / 16) * ((S + 31) / 32);
@@ -836,7 +837,7 @@ static void spmv_sweep_l(hipStream_t st, const SweepMatrix<V> &M, int S, int R, 
     if (lds > 64 * 1024) throw std::runtime_error("sweep spmv: rows too long for the staged form");
     hipLaunchKernelGGL((k_spmv_sweep_p<V, X, LOSSY>), dim3((unsigned)nb), dim3(256), lds, st, M, S, R, csys, dsys,
                        rowmask, x, y, mode, active);
-  } else if (S > 8 && !spmv_wide_enabled()) {
+  } else if (S > 8 && form == 2) {
     const int64_t nb = ((M.n + 15) / 16) * ((S + 15) / 16);
     if (S > 256) throw std::runtime_error("sweep spmv: at most 256 systems per batch");
     hipLaunchKernelGGL((k_spmv_sweep_t<V, X, LOSSY>), dim3((unsigned)nb), dim3(256), 0, st, M, S, R, csys, dsys,
@@ -884,18 +885,49 @@ static void spmv_sweep_l(hipStream_t st, const SweepMatrix<V> &M, int S, int R, 
 
 template <typename V, typename X>
 void launch_spmv_sweep(hipStream_t st, const SweepMatrix<V> &M, int S, int R, const V *csys, const V *dsys,
-                       const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr) {
+                       const uint8_t *rowmask, const X *x, X *y, int mode, const int32_t *active, int lpr, int form) {
+  if (form < 0 || form > 3) throw std::runtime_error("sweep spmv: form must be 0..3");
   if (M.Mi)
-    spmv_sweep_l<V, X, true>(st, M, S, R, csys, dsys, rowmask, x, y, mode, active, lpr);
+    spmv_sweep_l<V, X, true>(st, M, S, R, csys, dsys, rowmask, x, y, mode, active, lpr, form);
   else
-    spmv_sweep_l<V, X, false>(st, M, S, R, csys, dsys, rowmask, x, y, mode, active, lpr);
+    spmv_sweep_l<V, X, false>(st, M, S, R, csys, dsys, rowmask, x, y, mode, active, lpr, form);
 }
 template void launch_spmv_sweep<double, double2>(hipStream_t, const SweepMatrix<double> &, int, int, const double *,
                                                  const double *, const uint8_t *, const double2 *, double2 *, int,
-                                                 const int32_t *, int);
+                                                 const int32_t *, int, int);
 template void launch_spmv_sweep<float, float2>(hipStream_t, const SweepMatrix<float> &, int, int, const float *,
                                                const float *, const uint8_t *, const float2 *, float2 *, int,
-                                               const int32_t *, int);
+                                               const int32_t *, int, int);
+
+int spmv_tune_level() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("OPEM_SPMV_TUNE");
+    // a form forced by OPEM_SPMV_PAIR / OPEM_SPMV_WIDE stays forced unless tuning is asked for explicitly
+    const bool forced = getenv("OPEM_SPMV_PAIR") != nullptr || getenv("OPEM_SPMV_WIDE") != nullptr;
+    v = e ? atoi(e) : forced ? 0 : 1;
+    if (v < 0 || v > 2) v = 1;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_hash64(int64_t count, const uint64_t *__restrict__ a,
+                                                unsigned long long *__restrict__ out) {
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (unsigned long long)a[i] * (2ull * (unsigned long long)i + 1ull);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+void launch_hash64(hipStream_t st, int64_t count, const uint64_t *a, unsigned long long *out) {
+  if (count <= 0) return;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_hash64, dim3((unsigned)blocks), dim3(256), 0, st, count, a, out);
+  HIP_LAUNCH_CHECK();
+}
 
 // Kt = sum_a wk[a] K[a], Mr = sum_a wr[a] M[a], Mi = sum_a wi[a] M[a] (FP64, plus FP32 copies if f32 != null)
 __global__ void k_base_matrices(int64_t nnz, int nslot, const double *__restrict__ K, const double *__restrict__ M,

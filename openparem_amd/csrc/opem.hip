@@ -21,6 +21,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/opem.h"
@@ -146,6 +147,8 @@ struct Ctx {
   int poly_degree = 0;    // GMRES-polynomial preconditioner degree + 1 (0: off)
   int arnoldi_step = 1;   // 2: two new Krylov vectors per orthogonalisation pass
   int lpr = 16;
+  // batched-operator form chosen by timing per (S, R, vector bytes, lossy, boundary terms) on this context
+  std::map<std::tuple<int, int, int, int, int>, int> spmv_form;
   // vertex patches over all DOFs for the H-projection CG: FP16 inverses of the unit mass P (built once per space)
   Patches hpatches;
   int hpatch_state = 0;  // 0 not built, 1 built, -1 unavailable (a patch above 100 DOFs)
@@ -755,18 +758,74 @@ static void device_space(Ctx &c) {
 
 // ------------------------------------------------------------------ batched solve
 
+// Times the forms of the batched operator that can run this batch on the caller's vectors and returns the fastest
+// whose output hashes equal to the fixed rule's (form 0), so the choice never changes a result.  Runs once per batch
+// shape and context; the caller then launches the chosen form, which rewrites y.
+template <typename V, typename X>
+static int tune_spmv_form(Ctx &c, const SweepMatrix<V> &M, int S, int R, const V *cs, const V *ds, const uint8_t *mask,
+                          const X *x, X *y, int mode, const int32_t *act) {
+  hipStream_t st = c.st;
+  const int64_t words = (int64_t)S * M.n * (int64_t)sizeof(X) / 8;
+  DevBuf hb;
+  hb.alloc(4 * sizeof(unsigned long long));
+  HIPCHK(hipMemsetAsync(hb.p, 0, 4 * sizeof(unsigned long long), st));
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  float ms[4] = {-1.f, -1.f, -1.f, -1.f};
+  for (int f = 0; f <= 3; f++) {
+    if (f > 0 && !spmv_sweep_form_ok(M, S, f)) continue;
+    launch_spmv_sweep<V, X>(st, M, S, R, cs, ds, mask, x, y, mode, act, c.lpr, f);
+    launch_hash64(st, words, (const uint64_t *)y, hb.as<unsigned long long>() + f);
+    HIPCHK(hipEventRecord(a, st));
+    for (int r = 0; r < 3; r++) launch_spmv_sweep<V, X>(st, M, S, R, cs, ds, mask, x, y, mode, act, c.lpr, f);
+    HIPCHK(hipEventRecord(b, st));
+    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventElapsedTime(&ms[f], a, b));
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  unsigned long long h[4];
+  HIPCHK(hipMemcpyAsync(h, hb.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int best = 0;
+  for (int f = 1; f <= 3; f++)
+    if (ms[f] >= 0.f && h[f] == h[0] && ms[f] < ms[best]) best = f;
+  if (spmv_tune_level() >= 2)
+    fprintf(stderr, "[opem] batched operator S=%d R=%d %zu-byte vectors: rule %.3f ms, paired %.3f%s, lanes %.3f%s, "
+                    "wide %.3f%s ms -> form %d\n",
+            S, R, sizeof(X), ms[0] / 3, ms[1] / 3, ms[1] >= 0.f && h[1] != h[0] ? " (differs)" : "", ms[2] / 3,
+            ms[2] >= 0.f && h[2] != h[0] ? " (differs)" : "", ms[3] / 3, ms[3] >= 0.f && h[3] != h[0] ? " (differs)" : "",
+            best);
+  return best;
+}
+
 // y = mask .* (A_s x) (mode 0) or mask .* (-A_s x) (mode 1) for the batch, x system-interleaved
+template <typename V, typename X>
+static void sweep_spmv_m(Ctx &c, const SweepMatrix<V> &M, const V *cs, const V *ds, const uint8_t *mask, int S, int R,
+                         const X *x, X *y, int mode, const int32_t *act) {
+  int form = 0;
+  if (S > 8 && S <= 256 && spmv_tune_level() > 0) {
+    const auto key = std::make_tuple(S, R, (int)sizeof(X), M.Mi ? 1 : 0, M.nterm);
+    auto it = c.spmv_form.find(key);
+    if (it != c.spmv_form.end())
+      form = it->second;
+    else
+      form = c.spmv_form[key] = tune_spmv_form<V, X>(c, M, S, R, cs, ds, mask, x, y, mode, act);
+  }
+  launch_spmv_sweep<V, X>(c.st, M, S, R, cs, ds, mask, x, y, mode, act, c.lpr, form);
+}
 template <typename X>
 static void sweep_spmv(Ctx &c, Work &W, int S, int R, const X *x, X *y, int mode, const int32_t *act);
 template <>
 void sweep_spmv<double2>(Ctx &c, Work &W, int S, int R, const double2 *x, double2 *y, int mode, const int32_t *act) {
-  launch_spmv_sweep<double, double2>(c.st, W.M64, S, R, W.csys64.as<double>(), W.dsys64.as<double>(),
-                                     W.masks.as<uint8_t>(), x, y, mode, act, c.lpr);
+  sweep_spmv_m<double, double2>(c, W.M64, W.csys64.as<double>(), W.dsys64.as<double>(), W.masks.as<uint8_t>(), S, R,
+                                x, y, mode, act);
 }
 template <>
 void sweep_spmv<float2>(Ctx &c, Work &W, int S, int R, const float2 *x, float2 *y, int mode, const int32_t *act) {
-  launch_spmv_sweep<float, float2>(c.st, W.M32, S, R, W.csys32.as<float>(), W.dsys32.as<float>(),
-                                   W.masks.as<uint8_t>(), x, y, mode, act, c.lpr);
+  sweep_spmv_m<float, float2>(c, W.M32, W.csys32.as<float>(), W.dsys32.as<float>(), W.masks.as<uint8_t>(), S, R, x,
+                              y, mode, act);
 }
 
 template <typename X>

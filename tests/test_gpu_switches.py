@@ -48,6 +48,7 @@ def reference_s():
     {"OPEM_PATCH_SPLIT_KAPPA": "0"},                 # every FP16 block carries lo planes
     {"OPEM_PATCH_SPLIT_KAPPA": "1e30"},              # no block carries lo planes
     {"OPEM_XCD_REMAP": "1"},                         # XCD-contiguous workgroup order of the single-matrix SpMV
+    {"OPEM_SPMV_TUNE": "0"},                         # the fixed rule for the batched operator's form
 ], ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()) or "defaults")
 def test_switch_gives_the_same_s(reference_s, env):
     penv = dict(os.environ)
@@ -62,3 +63,27 @@ def test_switch_gives_the_same_s(reference_s, env):
     err = np.abs(S - reference_s).max()
     print(f"  |dS| {err:.2e}, iterations {d['iters']}", flush=True)
     assert d["converged"] == 1 and err < 1e-6
+
+
+def _child(env):
+    penv = dict(os.environ)
+    penv["PYTHONPATH"] = ROOT + (os.pathsep + penv["PYTHONPATH"] if penv.get("PYTHONPATH") else "")
+    penv.update(env)
+    p = subprocess.run([sys.executable, "-c", CHILD], cwd=ROOT, env=penv, capture_output=True, text=True,
+                       timeout=150)
+    assert p.returncode == 0, p.stderr[-2000:]
+    return json.loads([ln for ln in p.stdout.splitlines() if ln.startswith("{")][-1]), p.stderr
+
+
+def test_tuned_operator_is_bit_identical():
+    """The timed choice of the batched operator's form changes no bit: the sweep with tuning (which prints its
+    timings and choice) and the sweep with the fixed rule return identical S and iteration counts."""
+    print("tuned vs fixed-rule operator", flush=True)
+    tuned, err = _child({"OPEM_SPMV_TUNE": "2"})
+    fixed, _ = _child({"OPEM_SPMV_TUNE": "0"})
+    lines = [ln for ln in err.splitlines() if "batched operator" in ln]
+    for ln in lines:
+        print("  " + ln, flush=True)
+    assert lines, "tuning did not run"
+    assert tuned["iters"] == fixed["iters"]
+    assert tuned["S"] == fixed["S"]
