@@ -121,8 +121,8 @@ def _boundary_faces(tets):
 def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jitter=0.10, seed=1,
                    er=1.0, tand=0.0, short=False, slab=None):
     """Rectangular guide x in [0,a], y in [0,b], z in [0,L]; ports at z=0 (attr 2) and z=L (attr 3), or with
-    ``short`` one port at z=0 and a PEC wall at z=L (attr 3).  ``slab = (k1, k2, er, tand)`` fills the whole
-    cross-section between the grid planes k1 < k2 (cell indices along z, kept planar) with a second material
+    ``short`` one port at z=0 and a PEC wall at z=L (attr 3).  ``slab = (k1, k2, er, tand[, mur])`` fills the
+    whole cross-section between the grid planes k1 < k2 (cell indices along z, kept planar) with a second material
     (volume attribute 2)."""
     rng = np.random.default_rng(seed)
     xs, ys, zs = _grid_axis(0, a, nx), _grid_axis(0, b, ny), _grid_axis(0, length, nz)
@@ -138,12 +138,13 @@ def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jit
     tet_attr = np.ones(tets.shape[0], np.int32)
     materials = {1: dict(er=er, tand=tand, mur=1.0)}
     if slab:
-        k1, k2, er2, tand2 = slab
+        k1, k2, er2, tand2 = slab[:4]
+        mur2 = slab[4] if len(slab) > 4 else 1.0
         if not 0 < k1 < k2 < nz:
             raise ValueError("slab planes must satisfy 0 < k1 < k2 < nz")
         zc = xyz[tets][:, :, 2].mean(axis=1)
         tet_attr[(zc > zs[k1]) & (zc < zs[k2])] = 2
-        materials[2] = dict(er=er2, tand=tand2, mur=1.0)
+        materials[2] = dict(er=er2, tand=tand2, mur=mur2)
     m = TetMesh(xyz=xyz, tets=tets.astype(np.int32), tet_attr=tet_attr,
                 bfaces=bf.astype(np.int32), bface_attr=attr,
                 bdr_kind={1: "pec", 2: "port", 3: "pec" if short else "port"},
