@@ -55,7 +55,37 @@ def te10(mesh, attr, freq):
     return dict(E=E, H=H, gamma=complex(gamma), V=V, Z=complex(Z))
 
 
+def tem_pmc_sides(mesh, attr, freq):
+    """TEM mode of a guide with PEC walls at y = 0, b and magnetic walls at x = 0, a: uniform E_y (peak 1),
+    gamma = j k0 sqrt(er mur), same conventions as te10 (V across the guide, Z = V V* / (2 conj(P)))."""
+    a, b = mesh.geometry["a"], mesh.geometry["b"]
+    mat = mesh.materials[1]
+    er = mat["er"] * (1 - 1j * mat["tand"])
+    mur = mat["mur"]
+    gamma = np.sqrt(-k0_of(freq) ** 2 * er * mur + 0j)
+    if gamma.real < 0 or (gamma.real == 0 and gamma.imag < 0):
+        gamma = -gamma
+    omega = 2 * np.pi * freq
+    n = np.asarray(mesh.port_normal[attr], dtype=np.float64)
+    y_over = gamma / (1j * omega * MU0 * mur)
+
+    def E(pts):
+        e = np.zeros(pts.shape, np.complex128)
+        e[..., 1] = 1.0
+        return e
+
+    def H(pts):
+        return y_over * np.cross(n, E(pts))
+
+    V = complex(b)
+    P = 0.5 * np.conj(y_over) * (a * b)
+    Z = 0.5 * V * np.conj(V) / np.conj(P)
+    return dict(E=E, H=H, gamma=complex(gamma), V=V, Z=complex(Z))
+
+
 def provider_for(mesh):
+    if mesh.name.startswith("waveguide") and mesh.name.endswith("_pmc"):
+        return tem_pmc_sides
     if mesh.name.startswith("waveguide") or mesh.name.startswith("tee"):
         return te10
     if mesh.name.startswith("microstrip"):

@@ -287,6 +287,19 @@ def test_dielectric_slab_against_transmission_line(er2, tand2, mur2):
     assert errs[1] < 3e-3 and errs[0] > 5 * errs[1], errs
 
 
+def test_pmc_side_walls_carry_tem():
+    """PEC at y = 0, b and magnetic (natural) walls at x = 0, a: the guide carries a TEM mode (uniform E_y) at every
+    frequency, so S21 = exp(-j k0 L) and S11 = 0 up to discretization (ND2: 2.8e-4 at 10 GHz).  Were the magnetic
+    walls held at E_t = 0 like PEC, E_y would vanish on them and the guide would be cut off below 6.56 GHz."""
+    from openparem_amd.ports import k0_of, tem_pmc_sides
+    m = waveguide_mesh(nx=6, ny=3, nz=18, seed=3, pmc_sides=True)
+    pr = Problem(m, 2)
+    for f in (5e9, 10e9):
+        S = pr.solve_point(f, {q: tem_pmc_sides(m, q, f) for q in pr.ports})["S"]
+        s21 = np.exp(-1j * k0_of(f) * 30e-3)
+        assert abs(S[1, 0] - s21) < 1e-3 and abs(S[0, 0]) < 1e-3, (f, S)
+
+
 def test_lossy_guide_is_passive():
     m = waveguide_mesh(nx=6, ny=3, nz=8, seed=2, er=2.0, tand=0.01)
     pr = Problem(m, 2)
