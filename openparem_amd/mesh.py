@@ -119,12 +119,17 @@ def _boundary_faces(tets):
 
 
 def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jitter=0.10, seed=1,
-                   er=1.0, tand=0.0, short=False, slab=None, pmc_sides=False):
+                   er=1.0, tand=0.0, short=False, slab=None, pmc_sides=False, end=None):
     """Rectangular guide x in [0,a], y in [0,b], z in [0,L]; ports at z=0 (attr 2) and z=L (attr 3), or with
     ``short`` one port at z=0 and a PEC wall at z=L (attr 3).  ``slab = (k1, k2, er, tand[, mur])`` fills the
     whole cross-section between the grid planes k1 < k2 (cell indices along z, kept planar) with a second material
-    (volume attribute 2).  ``pmc_sides`` makes the side walls x = 0, a magnetic walls (attribute 4): with PEC at
+    (volume attribute 2).  ``end`` = "radiation" or "zs" closes z = L (attr 3) with that boundary instead of a port
+    or PEC (one port at z = 0).  ``pmc_sides`` makes the side walls x = 0, a magnetic walls (attribute 4): with PEC at
     y = 0, b the guide then carries a TEM mode (uniform E_y, ``ports.tem_pmc_sides``)."""
+    if end is not None and (short or end not in ("radiation", "zs")):
+        raise ValueError('end must be "radiation" or "zs", without short')
+    one_port = short or end is not None
+    end_kind = "pec" if short else end if end else "port"
     rng = np.random.default_rng(seed)
     xs, ys, zs = _grid_axis(0, a, nx), _grid_axis(0, b, ny), _grid_axis(0, length, nz)
     feat = [{0, nx}, {0, ny}, {0, nz} | ({slab[0], slab[1]} if slab else set())]
@@ -150,11 +155,11 @@ def waveguide_mesh(a=22.86e-3, b=10.16e-3, length=30e-3, nx=12, ny=6, nz=18, jit
         materials[2] = dict(er=er2, tand=tand2, mur=mur2)
     m = TetMesh(xyz=xyz, tets=tets.astype(np.int32), tet_attr=tet_attr,
                 bfaces=bf.astype(np.int32), bface_attr=attr,
-                bdr_kind={1: "pec", 2: "port", 3: "pec" if short else "port", **({4: "pmc"} if pmc_sides else {})},
-                port_normal={2: np.array([0.0, 0.0, -1.0])} if short else
+                bdr_kind={1: "pec", 2: "port", 3: end_kind, **({4: "pmc"} if pmc_sides else {})},
+                port_normal={2: np.array([0.0, 0.0, -1.0])} if one_port else
                 {2: np.array([0.0, 0.0, -1.0]), 3: np.array([0.0, 0.0, 1.0])},
                 materials=materials,
-                name=f"waveguide_{nx}x{ny}x{nz}" + ("_short" if short else "") + ("_slab" if slab else "") + ("_pmc" if pmc_sides else ""))
+                name=f"waveguide_{nx}x{ny}x{nz}" + ("_short" if short else "") + (f"_{end}" if end else "") + ("_slab" if slab else "") + ("_pmc" if pmc_sides else ""))
     m.geometry = dict(a=a, b=b, length=length)
     return m
 

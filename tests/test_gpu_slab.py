@@ -86,3 +86,45 @@ def test_pmc_side_walls_sweep(precision, tol):
         e = max(abs(S[1, 0] - np.exp(-1j * k0_of(f) * L)), abs(S[0, 0]))
         print(f"  {f / 1e9:.0f} GHz: |S21 - exp(-j k0 L)|, |S11| <= {e:.2e}", flush=True)
         assert e < 1e-2
+
+
+@pytest.mark.parametrize("end", ["radiation", "zs"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 1e-8)])
+def test_impedance_termination_sweep(precision, tol, end):
+    """A guide closed at z = L by the first-order radiation boundary (eta0) or by a resistive sheet (conductivity
+    chosen so R_s = 50 ohm at 10 GHz, R_s = sqrt(pi f mu0 / sigma) at each frequency): the one-port GPU sweep against
+    the oracle at 10 GHz and against the normal-incidence reflection closed form (oracle's distance at ND2 on this
+    mesh: 0.7e-3 to 3.9e-3 over 8-12 GHz)."""
+    from openparem_amd.mesh import waveguide_mesh
+    from openparem_amd.ports import te10
+    from openparem_amd.solver import Solver
+    from oracle.pipeline import Problem
+    from test_oracle import termination_closed_form
+
+    print(f"termination {end} {precision}", flush=True)
+    mu0 = 4e-7 * np.pi
+    sigma = np.pi * 10e9 * mu0 / 50.0 ** 2
+    m = waveguide_mesh(nx=NX, ny=NY, nz=NZ, seed=3, end=end)
+    freqs = np.array([8e9, 10e9, 12e9])
+    s = Solver(m, 2, boundaries={3: sigma} if end == "zs" else None)
+    try:
+        r = s.sweep(freqs, precision=precision, tol=1e-12 if precision == "fp64" else 1e-10, restart=150,
+                    batch=len(freqs))
+    finally:
+        s.close()
+    assert r["converged"].all(), (r["converged"], r["iters"])
+
+    def zs_of(f):
+        return np.sqrt(mu0 / 8.8541878176e-12) if end == "radiation" else np.sqrt(np.pi * f * mu0 / sigma)
+
+    pr = Problem(m, 2)
+    o = pr.solve_point(10e9, {q: te10(m, q, 10e9) for q in pr.ports}, rs=None if end == "radiation" else {3: zs_of(10e9)})
+    dS = np.abs(r["S"][1] - o["S"]).max()
+    print(f"|dS| vs oracle at 10 GHz {dS:.2e}", flush=True)
+    assert dS < tol
+    for i, f in enumerate(freqs):
+        s11 = termination_closed_form(f, A, L, zs_of(f))
+        e = abs(r["S"][i][0, 0] - s11)
+        print(f"  {f / 1e9:.0f} GHz: |S11| {abs(r['S'][i][0, 0]):.4f} (closed form {abs(s11):.4f}), |dS| {e:.2e}",
+              flush=True)
+        assert e < 5e-3

@@ -300,6 +300,33 @@ def test_pmc_side_walls_carry_tem():
         assert abs(S[1, 0] - s21) < 1e-3 and abs(S[0, 0]) < 1e-3, (f, S)
 
 
+def termination_closed_form(f, a, length, zs):
+    """One-port S11 of a guide closed at z = length by a surface impedance zs: a TE10 wave at normal incidence
+    reflects with (zs - Z_TE) / (zs + Z_TE), Z_TE = omega mu0 / beta, seen through the guide's round trip."""
+    mu0 = 4e-7 * np.pi
+    k0 = 2 * np.pi * f * np.sqrt(mu0 * 8.8541878176e-12)
+    beta = np.sqrt(k0 ** 2 - (np.pi / a) ** 2)
+    zte = 2 * np.pi * f * mu0 / beta
+    return (zs - zte) / (zs + zte) * np.exp(-2j * beta * length)
+
+
+@pytest.mark.parametrize("end,zs", [("radiation", None), ("zs", 50.0)])
+def test_impedance_termination(end, zs):
+    """A guide closed by the first-order radiation boundary (Z = eta0) or by a 50-ohm surface impedance at 10 GHz:
+    S11 against the normal-incidence reflection closed form, ND1 -> ND2 (measured 3.4e-2 -> 1.8e-3 and
+    5.1e-2 -> 3.9e-3).  A wrong sign or factor of the boundary term j omega mu0 / Z would miss by O(0.1-1)."""
+    from openparem_amd.ports import te10
+    eta0 = np.sqrt(4e-7 * np.pi / 8.8541878176e-12)
+    s11 = termination_closed_form(10e9, 22.86e-3, 30e-3, eta0 if zs is None else zs)
+    errs = []
+    for p in (1, 2):
+        m = waveguide_mesh(nx=6, ny=3, nz=18, seed=3, end=end)
+        pr = Problem(m, p)
+        S = pr.solve_point(10e9, {q: te10(m, q, 10e9) for q in pr.ports}, rs=None if zs is None else {3: zs})["S"]
+        errs.append(abs(S[0, 0] - s11))
+    assert errs[1] < 5e-3 and errs[0] > 5 * errs[1], errs
+
+
 def test_lossy_guide_is_passive():
     m = waveguide_mesh(nx=6, ny=3, nz=8, seed=2, er=2.0, tand=0.01)
     pr = Problem(m, 2)
