@@ -1,10 +1,11 @@
-"""Two materials in one guide: a slab across the whole WR90 cross-section between air sections (teflon, er 2.2;
-lossy teflon, tan delta 0.05; a magnetic slab, mu_r 2).
+"""The GPU sweep against closed forms, on guides where the continuum answer is known (each also against the oracle):
 
-The GPU sweep (four frequencies in one batch, FP64 and mixed) is checked against the oracle's direct solve at
-9 GHz and against the transmission-line closed form at every frequency (tests/test_oracle.py pins the oracle
-to that closed form; the remaining difference is discretization error, 2.1e-3 to 2.6e-3 at ND2 on this mesh over
-8-11 GHz for teflon, where |S11| falls from 0.60 to the slab's half-wave transparency near 11 GHz).
+* a slab across the whole WR90 cross-section between air sections (teflon, er 2.2; lossy teflon, tan delta 0.05;
+  a magnetic slab, mu_r 2): the transmission-line cascade.  tests/test_oracle.py pins the oracle to it; the
+  remaining difference is discretization error, 2.1e-3 to 2.6e-3 at ND2 on this mesh over 8-11 GHz for teflon,
+  where |S11| falls from 0.60 to the slab's half-wave transparency near 11 GHz;
+* magnetic side walls: a TEM mode, S21 = exp(-j k0 L);
+* radiation and resistive-sheet terminations: the normal-incidence reflection.
 """
 import os
 import sys
