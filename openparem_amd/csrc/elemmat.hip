@@ -90,10 +90,14 @@ struct ElemCW {
   static constexpr int CW = PREC == 0 ? 32 : 64;  // 8 KB of stage per wave at every precision
 };
 
-// NT: non-temporal output stores.  They pay when every element block (nd^2 values) is a whole number of 64-byte
-// segments, so each run covers whole segments (ND2: +5-40% measured); where runs straddle segments (ND1 FP64, ND3)
-// the L2 must merge the partial lines and plain stores are faster.
-template <int PREC, bool NT>
+// NT: non-temporal output stores.  They pay when every run covers whole 64-byte segments: element blocks (nd^2
+// values) that are whole numbers of segments (ND2: +5-40% measured) and the contiguous WHOLE range; where runs
+// straddle segments (ND3) the L2 must merge the partial lines and plain stores are faster.
+// WHOLE (ND1 only, nd^2 = 36): a wave's 16 element blocks (16 x 2 x 36 values, one contiguous output range) fit in
+// its stage, so the tiles are staged in output order and the range is streamed out with consecutive lanes, with
+// non-temporal stores (ND1 FP64 0.17 -> 0.12 ms per 1M elements measured against plain stores).
+constexpr int kWholeNdd = 36;
+template <int PREC, bool NT, bool WHOLE>
 __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const double *__restrict__ verts,
                                               const double *__restrict__ er, const double *__restrict__ tand,
                                               double k0, const typename ElemT<PREC>::T *__restrict__ tab,
@@ -103,8 +107,10 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
   constexpr int CW = ElemCW<PREC>::CW;
   // Per wave one 8 KB region: first the 16 elements' factors X[16][24] (16 re + 8 im) and BF16 scales sc[16],
   // read into registers before the column loop, then the output stage [2][16][CW].
-  static_assert(2 * 16 * CW * sizeof(O) >= 16 * 24 * 8 + 16 * 8, "stage too small for the factors");
-  __shared__ __align__(16) O Stage[4][2][16][CW];
+  constexpr int SW = WHOLE ? 16 * 2 * kWholeNdd : 2 * 16 * CW;  // stage values per wave
+  constexpr int TPC = WHOLE ? (kWholeNdd + 15) / 16 : CW / 16;  // column tiles per staged chunk
+  static_assert(SW * sizeof(O) >= 16 * 24 * 8 + 16 * 8, "stage too small for the factors");
+  __shared__ __align__(16) O Stage[4][SW];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double(*X1)[24] = (double(*)[24])(void *)Stage[wave];
   double *sc1 = (double *)(void *)Stage[wave] + 16 * 24;
@@ -146,7 +152,8 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
   const int nel = ne - e0 < 16 ? (int)(ne - e0) : 16;
   O *ob = out + e0 * (int64_t)(2 * ndd);
   const int ntile = NP / 16;
-  O(*st)[16][CW] = Stage[wave];
+  O(*st)[16][CW] = (O(*)[16][CW])(void *)Stage[wave];
+  O *wst = Stage[wave];
   // A operands, fixed over the column tiles
   T are[4], aim[2];
   bf8_t bre_a, bim_a;
@@ -172,9 +179,9 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int c0 = 0; c0 < ndd; c0 += CW) {
+  for (int c0 = 0; c0 < ndd; c0 += WHOLE ? ndd : CW) {
 #pragma unroll
-    for (int tt = 0; tt < CW / 16; tt++) {
+    for (int tt = 0; tt < TPC; tt++) {
       const int t = c0 / 16 + tt;
       if (t >= ntile) break;
       const int n = t * 16 + i16;
@@ -214,18 +221,38 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
       for (int r = 0; r < 4; r++) {
         // f64 16x16x4: D row i in lane group i%4, register i/4; f32 16x16x4: group i/4, register i%4
         const int el = PREC == 0 ? r * 4 + kq : kq * 4 + r;
+        O vre, vim;
         if constexpr (PREC == 0) {
-          st[0][el][tt * 16 + i16] = dre[r];
-          st[1][el][tt * 16 + i16] = dim[r];
+          vre = dre[r];
+          vim = dim[r];
         } else {
-          st[0][el][tt * 16 + i16] = (O)(fre[r] * scr[r]);
-          st[1][el][tt * 16 + i16] = (O)(fim[r] * sci[r]);
+          vre = (O)(fre[r] * scr[r]);
+          vim = (O)(fim[r] * sci[r]);
+        }
+        if constexpr (WHOLE) {
+          if (n < ndd) {  // output order: element, plane, column
+            wst[el * 2 * ndd + n] = vre;
+            wst[el * 2 * ndd + ndd + n] = vim;
+          }
+        } else {
+          st[0][el][tt * 16 + i16] = vre;
+          st[1][el][tt * 16 + i16] = vim;
         }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (WHOLE) {
+      const int total = nel * 2 * ndd;  // the wave's contiguous output range
+      for (int i = lane; i < total; i += 64) {
+        if constexpr (NT)
+          __builtin_nontemporal_store(wst[i], ob + i);
+        else
+          ob[i] = wst[i];
+      }
+      continue;  // one pass covers every column
+    }
     const int len = ndd - c0 < CW ? ndd - c0 : CW;
 #pragma unroll
     for (int it = 0; it < 2 * 16 * CW / 64; it++) {
@@ -300,30 +327,29 @@ void launch_element_matrices(hipStream_t st, int prec, int p, int64_t ne, const 
                              const double *tand, double k0, const void *tab, void *, void *out) {
   const int nd = nd_tet_ndof(p), NP = padded(nd);
   const unsigned blocks = (unsigned)((ne + 63) / 64);
-  const bool nt = (nd * nd * (prec == 0 ? 8 : 4)) % 64 == 0;
+  const bool whole = nd * nd <= kWholeNdd;
+  const bool nt = !whole && (nd * nd * (prec == 0 ? 8 : 4)) % 64 == 0;
+#define OPEM_ELEM(P, TT, OO)                                                                                       \
+  {                                                                                                              \
+    if (whole)                                                                                                   \
+      hipLaunchKernelGGL((k_elem<P, true, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0, \
+                         (const TT *)tab, (OO *)out);                                                            \
+    else if (nt)                                                                                                 \
+      hipLaunchKernelGGL((k_elem<P, true, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0, \
+                         (const TT *)tab, (OO *)out);                                                            \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_elem<P, false, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand,    \
+                         k0, (const TT *)tab, (OO *)out);                                                        \
+  }
   if (prec == 0) {
-    if (nt)
-      hipLaunchKernelGGL((k_elem<0, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                         (const double *)tab, (double *)out);
-    else
-      hipLaunchKernelGGL((k_elem<0, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                         (const double *)tab, (double *)out);
+    OPEM_ELEM(0, double, double)
   } else if (prec == 1) {
-    if (nt)
-      hipLaunchKernelGGL((k_elem<1, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                         (const float *)tab, (float *)out);
-    else
-      hipLaunchKernelGGL((k_elem<1, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                         (const float *)tab, (float *)out);
+    OPEM_ELEM(1, float, float)
   } else if (prec == 2) {
-    if (nt)
-      hipLaunchKernelGGL((k_elem<2, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                         (const __bf16 *)tab, (float *)out);
-    else
-      hipLaunchKernelGGL((k_elem<2, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,
-                         (const __bf16 *)tab, (float *)out);
+    OPEM_ELEM(2, __bf16, float)
   } else
     throw std::runtime_error("element matrices: bad precision");
+#undef OPEM_ELEM
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("element kernel launch: ") + hipGetErrorString(e));
 }
