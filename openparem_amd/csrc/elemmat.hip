@@ -97,7 +97,9 @@ struct ElemCW {
 // its stage, so the tiles are staged in output order and the range is streamed out with consecutive lanes, with
 // non-temporal stores (ND1 FP64 0.17 -> 0.12 ms per 1M elements measured against plain stores).
 constexpr int kWholeNdd = 36;
-template <int PREC, bool NT, bool WHOLE>
+// PLANE (ND3): one plane (Re, then Im) of 16 elements at a time, in runs twice as long as the two-plane chunks, for
+// the long misaligned element blocks of ND3 (nd^2 = 2025).
+template <int PREC, bool NT, int MODE>  // MODE 0: two-plane chunks, 1: WHOLE, 2: PLANE
 __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const double *__restrict__ verts,
                                               const double *__restrict__ er, const double *__restrict__ tand,
                                               double k0, const typename ElemT<PREC>::T *__restrict__ tab,
@@ -107,8 +109,10 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
   constexpr int CW = ElemCW<PREC>::CW;
   // Per wave one 8 KB region: first the 16 elements' factors X[16][24] (16 re + 8 im) and BF16 scales sc[16],
   // read into registers before the column loop, then the output stage [2][16][CW].
-  constexpr int SW = WHOLE ? 16 * 2 * kWholeNdd : 2 * 16 * CW;  // stage values per wave
-  constexpr int TPC = WHOLE ? (kWholeNdd + 15) / 16 : CW / 16;  // column tiles per staged chunk
+  constexpr bool WHOLE = MODE == 1, PLANE = MODE == 2;
+  constexpr int CWS = PLANE ? 2 * CW : CW;                          // staged columns per chunk
+  constexpr int SW = WHOLE ? 16 * 2 * kWholeNdd : 2 * 16 * CW;      // stage values per wave
+  constexpr int TPC = WHOLE ? (kWholeNdd + 15) / 16 : CWS / 16;     // column tiles per staged chunk
   static_assert(SW * sizeof(O) >= 16 * 24 * 8 + 16 * 8, "stage too small for the factors");
   __shared__ __align__(16) O Stage[4][SW];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -179,7 +183,8 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int c0 = 0; c0 < ndd; c0 += WHOLE ? ndd : CW) {
+  for (int part = 0; part < (PLANE ? 2 : 1); part++)
+  for (int c0 = 0; c0 < ndd; c0 += WHOLE ? ndd : CWS) {
 #pragma unroll
     for (int tt = 0; tt < TPC; tt++) {
       const int t = c0 / 16 + tt;
@@ -190,21 +195,25 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
       if constexpr (PREC == 0 || PREC == 1) {
         const T *Tre = tab;
         const T *Tim = tab + 16 * NP;
+        if (!PLANE || part == 0) {
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-          const T b = Tre[(kb * 4 + kq) * NP + n];
-          if constexpr (PREC == 0)
-            dre = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kb], b, dre, 0, 0, 0);
-          else
-            fre = __builtin_amdgcn_mfma_f32_16x16x4f32(are[kb], b, fre, 0, 0, 0);
+          for (int kb = 0; kb < 4; kb++) {
+            const T b = Tre[(kb * 4 + kq) * NP + n];
+            if constexpr (PREC == 0)
+              dre = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kb], b, dre, 0, 0, 0);
+            else
+              fre = __builtin_amdgcn_mfma_f32_16x16x4f32(are[kb], b, fre, 0, 0, 0);
+          }
         }
+        if (!PLANE || part == 1) {
 #pragma unroll
-        for (int kb = 0; kb < 2; kb++) {
-          const T b = Tim[(kb * 4 + kq) * NP + n];
-          if constexpr (PREC == 0)
-            dim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kb], b, dim, 0, 0, 0);
-          else
-            fim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[kb], b, fim, 0, 0, 0);
+          for (int kb = 0; kb < 2; kb++) {
+            const T b = Tim[(kb * 4 + kq) * NP + n];
+            if constexpr (PREC == 0)
+              dim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kb], b, dim, 0, 0, 0);
+            else
+              fim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[kb], b, fim, 0, 0, 0);
+          }
         }
       } else {
         const __bf16 *Tre = tab;
@@ -234,6 +243,8 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
             wst[el * 2 * ndd + n] = vre;
             wst[el * 2 * ndd + ndd + n] = vim;
           }
+        } else if constexpr (PLANE) {
+          wst[el * CWS + tt * 16 + i16] = part == 0 ? vre : vim;
         } else {
           st[0][el][tt * 16 + i16] = vre;
           st[1][el][tt * 16 + i16] = vim;
@@ -252,6 +263,25 @@ __global__ __launch_bounds__(256) void k_elem(int64_t ne, int nd, int NP, const 
           ob[i] = wst[i];
       }
       continue;  // one pass covers every column
+    }
+    if constexpr (PLANE) {
+      const int len = ndd - c0 < CWS ? ndd - c0 : CWS;
+#pragma unroll
+      for (int it = 0; it < 16 * CWS / 64; it++) {
+        const int idx = it * 64 + lane;
+        const int col = idx % CWS, el = idx / CWS;
+        if (col < len && el < nel) {
+          O *dst = ob + (int64_t)el * 2 * ndd + part * ndd + c0 + col;
+          if constexpr (NT)
+            __builtin_nontemporal_store(wst[el * CWS + col], dst);
+          else
+            *dst = wst[el * CWS + col];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      continue;
     }
     const int len = ndd - c0 < CW ? ndd - c0 : CW;
 #pragma unroll
@@ -328,18 +358,28 @@ void launch_element_matrices(hipStream_t st, int prec, int p, int64_t ne, const 
   const int nd = nd_tet_ndof(p), NP = padded(nd);
   const unsigned blocks = (unsigned)((ne + 63) / 64);
   const bool whole = nd * nd <= kWholeNdd;
+  // ND3: one plane at a time (measured per 1M elements against two-plane chunks: FP64 13.37 -> 11.63 ms, FP32
+  // 7.52 -> 6.95 ms with non-temporal stores; BF16 6.85 -> 6.44 ms with plain stores, 6.69 ms non-temporal)
+  const bool plane = nd * nd > 1000;
+  const bool plane_nt = prec != 2;
   const bool nt = !whole && (nd * nd * (prec == 0 ? 8 : 4)) % 64 == 0;
 #define OPEM_ELEM(P, TT, OO)                                                                                       \
   {                                                                                                              \
     if (whole)                                                                                                   \
-      hipLaunchKernelGGL((k_elem<P, true, true>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0, \
+      hipLaunchKernelGGL((k_elem<P, true, 1>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,    \
+                         (const TT *)tab, (OO *)out);                                                            \
+    else if (plane && plane_nt)                                                                                  \
+      hipLaunchKernelGGL((k_elem<P, true, 2>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,    \
+                         (const TT *)tab, (OO *)out);                                                            \
+    else if (plane)                                                                                              \
+      hipLaunchKernelGGL((k_elem<P, false, 2>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,   \
                          (const TT *)tab, (OO *)out);                                                            \
     else if (nt)                                                                                                 \
-      hipLaunchKernelGGL((k_elem<P, true, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0, \
+      hipLaunchKernelGGL((k_elem<P, true, 0>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,    \
                          (const TT *)tab, (OO *)out);                                                            \
     else                                                                                                         \
-      hipLaunchKernelGGL((k_elem<P, false, false>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand,    \
-                         k0, (const TT *)tab, (OO *)out);                                                        \
+      hipLaunchKernelGGL((k_elem<P, false, 0>), dim3(blocks), dim3(256), 0, st, ne, nd, NP, verts, er, tand, k0,   \
+                         (const TT *)tab, (OO *)out);                                                            \
   }
   if (prec == 0) {
     OPEM_ELEM(0, double, double)
