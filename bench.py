@@ -175,7 +175,7 @@ def main():
         cpu = _cpu_baseline(work, float(r64["iters"].mean()), args.restart)
     if cpu is not None:
         _progress("CPU baseline done")
-    asm = None if args.no_assembly else assembly_c2(2, 1_000_000, HBM_PEAK_GBS)
+    asm = None if args.no_assembly else {f"nd{p_}": assembly_c2(p_, 1_000_000, HBM_PEAK_GBS) for p_ in (1, 2, 3)}
 
     line = {
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",
