@@ -150,12 +150,16 @@ typedef struct {
   int reciprocal;               /* 1: S <- (S + S^T)/2 (results.cpp:401-419)                      */
   double h_tol;                 /* H-projection CG relative tolerance (0 -> 1e-13)                */
   int precond;                  /* 0: vertex-patch additive Schwarz (element blocks if a patch has */
-                                /*    more than 100 DOFs), 1: element-block Jacobi                */
+                                /*    more than 100 DOFs), 1: element-block Jacobi, 2: sparse LU   */
+                                /*    (nested-dissection multifrontal factor of each frequency's  */
+                                /*    matrix, FP32 fronts in mixed precision: the PCCHOLESKY/PCLU */
+                                /*    of solveComplexLinearSystem.c:133-168)                      */
   int coarse_size;              /* aggregates of the gradient coarse level (0 = one-level, <= 128) */
   int orth_passes;              /* classical Gram-Schmidt passes per Arnoldi step: 2 (default) or 1 */
-  int precond_share;            /* vertex patches: adjacent frequencies of a batch sharing one set of */
-                                /*    patch inverses, built from their mean matrix (0 -> 1; x nports <= 32 */
-                                /*    in mixed precision, <= 8 in FP64)                          */
+  int precond_share;            /* adjacent frequencies of a batch sharing one set of patch inverses */
+                                /*    or one LU factor, built from their mean matrix (0 -> 1; x nports */
+                                /*    <= 32 for patches in mixed precision, <= 8 in FP64, <= 64 for LU; */
+                                /*    LU uses the largest divisor of the batch's frequency count)   */
   int poly_degree;              /* d > 0: GMRES-polynomial preconditioning, the Krylov space built with  */
                                 /*    B p(B) (B = A M, deg p = d - 1, roots from a d-step Arnoldi run per */
                                 /*    system); each iteration applies B d times (0 = off, <= 32)       */

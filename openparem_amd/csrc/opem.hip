@@ -15,9 +15,11 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -26,8 +28,11 @@
 
 #include "../../include/opem.h"
 #include "coarse.h"
+#include "devbuf.h"
 #include "poly.h"
 #include "kernels.h"
+#include "lu.h"
+#include "lu_symbolic.h"
 #include "nd_tables.h"
 #include "quadrature.h"
 #include "space.h"
@@ -36,42 +41,6 @@ namespace opem {
 
 static const double kMu0 = 4e-7 * M_PI;
 static const double kEps0 = 8.8541878176e-12;  // OpenParEM3D.cpp:216
-
-#define HIPCHK(x)                                                                                 \
-  do {                                                                                            \
-    hipError_t e_ = (x);                                                                          \
-    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void alloc(size_t b) {
-    if (b <= bytes && p) return;
-    release();
-    if (b == 0) b = 16;
-    HIPCHK(hipMalloc(&p, b));
-    bytes = b;
-  }
-  template <typename T>
-  T *as() const {
-    return (T *)p;
-  }
-  template <typename T>
-  void upload(const std::vector<T> &v, hipStream_t st) {
-    alloc(v.size() * sizeof(T));
-    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
-  }
-};
 
 struct Material {
   double er = 1.0, tand = 0.0, mur = 1.0;
@@ -156,6 +125,15 @@ struct Ctx {
       dh_inv, dh_scale;
   int64_t hpatch_size = 0;
   Work work;  // solver workspace, kept between sweeps
+  // sparse LU (precond 2): nested-dissection tree of the free DOFs with the port DOFs as the last root, and the
+  // numeric factors of the current batch
+  LuTree lu_tree;
+  bool have_lu_tree = false;
+  std::vector<uint8_t> lu_excl, lu_last;
+  LuFactor lu;
+  LuFactor *lu_pc = nullptr;  // set while a Krylov solve is preconditioned by lu
+  int lu_nrhs = 0;            // systems per factor
+  int lu_leaf = 8;            // dissection leaves: at most this many mesh vertices
 };
 
 // ------------------------------------------------------------------ small kernels
@@ -845,21 +823,27 @@ struct BatchResult {
 
 // GMRES(m) with element-block-Jacobi right preconditioning, restarted with the true residual in
 // FP64 at every cycle (for X = float2 this is FP32 GMRES inside FP64 iterative refinement).
+// The operator of a batched Krylov solve: y = mask .* (A x) (mode 0) or y = -mask .* (A x) (mode 1) for every
+// system, x system-interleaved, y system-major; fp64 selects double2 vectors, else float2.
+using BatchOp = std::function<void(bool fp64, const void *x, void *y, int mode, const int32_t *act)>;
+
 template <typename X>
-static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double tol, int passes, double inner_red,
-                        std::vector<int> &iters, std::vector<double> &relres, std::vector<int> &conv,
-                        const std::vector<double> &bnorm) {
+static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, int R, int m, int maxit, double tol,
+                        int passes, double inner_red, std::vector<int> &iters, std::vector<double> &relres,
+                        std::vector<int> &conv, const std::vector<double> &bnorm) {
   using P = typename VecT<X>::P;
   const Space &s = c.sp;
   hipStream_t st = c.st;
-  const int64_t n = s.ndof, nnz = s.nnz;
+  const int64_t nnz = s.nnz;
   const int nd = s.nd;
+  const bool is64 = std::is_same<X, double2>::value;
   const int nblk = reduce_blocks(n), nbm = mdot_blocks(n);
   const bool mixed = std::is_same<X, float2>::value;
   // preconditioner: vertex-patch additive Schwarz (inverses shared by the excitations of a frequency
   // except on port patches; FP16 with per-block scales inside the FP32 solve), or element-block Jacobi
   using PP = typename std::conditional<std::is_same<X, float2>::value, __half2, double2>::type;
-  const bool use_patch = c.have_patches && c.precond == 0;
+  const bool use_lu = c.lu_pc != nullptr;
+  const bool use_patch = !use_lu && c.have_patches && c.precond == 0;
   PatchLayout PL{};
   const int G = S / R;
   if (use_patch) {
@@ -911,19 +895,23 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
         for (int64_t p = 0; p < PL.npatch; p++, tot++) neg += sv[((size_t)q * R) * PL.npatch + p] < 0.f;
       fprintf(stderr, "opem: %zu of %zu FP16 patch blocks carry lo planes\n", neg, tot);
     }
-  } else {
+  } else if (!use_lu) {
     W.inv.alloc((size_t)S * nd * n * sizeof(P));
     launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
                        nnz, W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                        c.d_blocks.as<int32_t>(), c.nblocks);
   }
-  const bool use_coarse = c.have_coarse && c.coarse.nc > 0;
+  const bool use_coarse = !use_lu && c.have_coarse && c.coarse.nc > 0;
   if (use_coarse) {
     W.t.alloc((size_t)S * n * sizeof(X));
     W.rc.alloc((size_t)S * c.coarse.nc * 16);
     W.yc.alloc((size_t)S * c.coarse.nc * 16);
   }
   auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
+    if (use_lu) {
+      c.lu_pc->solve<X>(c.lu_nrhs, v, ldv_sys, z);
+      return;
+    }
     if (use_patch)
       launch_patch_apply<PP, X>(st, PL, G, R, n, W.inv.as<PP>(), mixed ? W.pscale.as<float>() : nullptr, v, ldv_sys,
                                 z, act, (void *)W.ypatch.p);
@@ -932,7 +920,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
                             v, ldv_sys, z, act, (int64_t)nd * n);
     if (use_coarse) {  // multiplicative second level: z += C Ac^-1 C^T (v - A z)
       const int nc = c.coarse.nc;
-      sweep_spmv<X>(c, W, S, R, z, W.t.as<X>(), 0, act);
+      op(is64, z, W.t.as<X>(), 0, act);
       launch_coarse_restrict<X>(st, S, nc, c.dc_col_ptr.as<int32_t>(), c.dc_col_row.as<int32_t>(),
                                 c.dc_col_val.as<double>(), v, ldv_sys, W.t.as<X>(), n, W.rc.as<double>(), act);
       launch_coarse_solve(st, S, R, nc, W.cinv.as<double>(), W.rc.as<double>(), W.yc.as<double>(), act);
@@ -952,7 +940,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   bool roots_ready = false;
   auto apply_B = [&](const X *v, int64_t ldv_sys, X *out, const int32_t *act) {
     precond(v, ldv_sys, W.z.as<X>(), act);
-    sweep_spmv<X>(c, W, S, R, W.z.as<X>(), out, 0, act);
+    op(is64, W.z.as<X>(), out, 0, act);
   };
   // u = p(B) v  (Loe-Morgan: prod = v; u = sum_i prod_i / theta_i with prod_{i+1} = prod_i - B prod_i / theta_i)
   int pdeg = pd;  // steps of p(B) after root stabilisation (set by the harvest)
@@ -1006,7 +994,7 @@ static void gmres_batch(Ctx &c, Work &W, int S, int R, int m, int maxit, double 
   const double2 *x64 = W.xil.as<double2>();
   while (true) {
     // true residual in FP64: r = mask .* (-A x)
-    sweep_spmv<double2>(c, W, S, R, x64, W.r.as<double2>(), 1, nullptr);
+    op(true, x64, W.r.as<double2>(), 1, nullptr);
     launch_norm2<double2>(st, S, n, W.r.as<double2>(), W.partial.as<double>(), nblk, nullptr);
     launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0, nullptr);
     HIPCHK(hipMemcpyAsync(rn2.data(), W.nrm2.p, S * 8, hipMemcpyDeviceToHost, st));
@@ -1473,6 +1461,8 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     for (size_t i = 0; i < c.ports.size(); i++)
       if (c.ports[i].attr == in->port_attr[k]) port_of[k] = (int)i;
     if (port_of[k] < 0) throw std::invalid_argument("port_attr is not an OPEM_BDR_PORT boundary");
+    for (int k2 = 0; k2 < k; k2++)
+      if (in->port_attr[k2] == in->port_attr[k]) throw std::invalid_argument("port_attr has a duplicate entry");
   }
   if ((int)c.ports.size() != np) throw std::invalid_argument("every port boundary must be an S-port");
   Overlay ov;
@@ -1539,6 +1529,36 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   std::vector<uint8_t> portdof(n, 0);
   for (auto &P : c.ports)
     for (int32_t d : P.dofs) portdof[d] = 1;
+  // sparse LU: PEC DOFs are left out, the port DOFs form the last root (each excitation factors only the ports
+  // it does not drive); a DOF on two ports is Dirichlet in every excitation and is left out too
+  const bool use_lu = c.precond == 2;
+  if (use_lu) {
+    std::vector<uint8_t> excl(pec), last(n, 0), nport(n, 0);
+    for (auto &P : c.ports)
+      for (int32_t d : P.dofs) nport[d]++;
+    for (int64_t i = 0; i < n; i++) {
+      if (nport[i] > 1) excl[i] = 1;
+      last[i] = nport[i] == 1 && !excl[i];
+    }
+    std::vector<int32_t> group_of(n, -1);
+    for (int k = 0; k < np; k++)
+      for (int32_t d : c.ports[port_of[k]].dofs) group_of[d] = k;
+    const bool tree_new = !c.have_lu_tree || c.lu_excl != excl || c.lu_last != last;
+    if (tree_new) {
+      c.have_lu_tree = false;
+      lu_analyse_mesh(c.mesh, s, excl, last, c.lu_leaf, c.lu_tree);
+      c.lu_excl = excl;
+      c.lu_last = last;
+      c.have_lu_tree = true;
+    }
+    std::vector<int32_t> rgroup;
+    if (c.lu_tree.last_node >= 0) {
+      const int r = c.lu_tree.last_node;
+      for (int64_t k = c.lu_tree.dbeg[r]; k < c.lu_tree.dbeg[r + 1]; k++)
+        rgroup.push_back(group_of[c.lu_tree.perm[k]]);
+    }
+    c.lu.setup(c.lu_tree, np, rgroup, st);
+  }
   if (c.precond == 0 && (!c.have_patches || c.patch_excl != pec || c.patch_port_dofs != portdof)) {
     int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     build_patches(s, pec, c.patches, nth);
@@ -1617,8 +1637,12 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     HIPCHK(hipMemGetInfo(&mfree, &mtotal));
     const double vec = in->precision == OPEM_PREC_MIXED ? 8.0 : 16.0;
     const bool patch = c.precond == 0 && c.have_patches;
-    const double per_sys = (double)(m + 4 + c.arnoldi_step) * n * vec + 10.0 * n * 16 + (double)s.nnz * 16 / R +
-                           (patch ? (double)c.patches.idx.size() * 16 : (double)s.nd * n * 16);
+    double per_sys = (double)(m + 4 + c.arnoldi_step) * n * vec + 10.0 * n * 16 + (double)s.nnz * 16 / R;
+    if (use_lu)  // fronts of one factor per frequency group, apply vectors
+      per_sys += ((double)c.lu_tree.foff[c.lu_tree.nnodes] * vec + 6.0 * n * 16 * R) /
+                     (R * (double)std::max(1, in->precond_share)) + 6.0 * n * 16;
+    else
+      per_sys += patch ? (double)c.patches.idx.size() * 16 : (double)s.nd * n * 16;
     const double fit = std::floor(0.75 * (double)mtotal / (per_sys * R));
     if (fit < batch) batch = (int)std::max(1.0, fit);
   }
@@ -1651,6 +1675,8 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     for (int f0 = 0; f0 < nf; f0 += batch) batches.push_back({f0, std::min(batch, nf - f0)});
   }
   DevBuf xstash;  // warm start: solutions of the solved frequencies, [frequency][excitation][ndof]
+  int64_t lu_stats_zero = 0;
+  double lu_seconds = 0;
   HIPCHK(hipStreamSynchronize(st));
   const double t0 = now_s();
   double t_setup = 0, t_solve = 0, t_post = 0;
@@ -1819,12 +1845,44 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     // long cycle would cost more basis passes (quadratic in its length) than it saves; cycles of 1e-3 are used
     // (64-frequency, 410k-DOF guide: 4.87 s per sweep against 5.51 s at 1e-5 and 5.52 s without warm start).
     const double inner_red = (warm && f0 > 0) ? 1e-3 : 1e-5;
+    int lu_share = 1;
+    for (int g = std::max(1, std::min(c.precond_share, F)); g >= 1; g--)
+      if (F % g == 0) {
+        lu_share = g;
+        break;
+      }
+    const BatchOp op = [&](bool fp64, const void *x, void *y, int mode, const int32_t *act) {
+      if (fp64)
+        sweep_spmv<double2>(c, W, S, R, (const double2 *)x, (double2 *)y, mode, act);
+      else
+        sweep_spmv<float2>(c, W, S, R, (const float2 *)x, (float2 *)y, mode, act);
+    };
+    if (use_lu) {
+      // one factor per group of lu_share adjacent frequencies (their mean matrix when lu_share > 1)
+      const int nq = F / lu_share;
+      const double *vals = W.A64.as<double>();
+      if (lu_share > 1) {
+        W.Amean.alloc((size_t)nq * nnz * 16);
+        launch_group_mean(st, F, lu_share, nnz, W.A64.as<double>(), W.Amean.as<double>());
+        vals = W.Amean.as<double>();
+      }
+      LuFactorStats lst;
+      c.lu.factor(mixed ? 1 : 0, nq, vals, nnz, &lst);
+      lu_stats_zero += lst.zero_pivots;
+      lu_seconds += lst.seconds_factor;
+      c.lu_pc = &c.lu;
+      c.lu_nrhs = lu_share * R;
+    }
+    struct LuReset {
+      Ctx &c;
+      ~LuReset() { c.lu_pc = nullptr; }
+    } lu_reset{c};
     if (mixed)
-      gmres_batch<float2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red, iters, relres,
-                          conv, bn);
+      gmres_batch<float2>(c, W, op, n, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red, iters,
+                          relres, conv, bn);
     else
-      gmres_batch<double2>(c, W, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red, iters, relres,
-                           conv, bn);
+      gmres_batch<double2>(c, W, op, n, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red, iters,
+                           relres, conv, bn);
     if (warm && f0 + F < nf) {  // keep this stage's solutions for the later ones
       xstash.alloc((size_t)nf * R * n * 16);
       for (int fi = 0; fi < F; fi++)
@@ -1918,10 +1976,51 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     HIPCHK(hipStreamSynchronize(st));
     t_post += now_s() - tc;
   }
+  if (use_lu && getenv("OPEM_LU_STATS"))
+    fprintf(stderr, "opem: sparse LU %d nodes, max front %lld, %.3g GFLOP per factor, factor time %.3f s, %lld zero "
+            "pivots\n", c.lu_tree.nnodes, (long long)c.lu_tree.max_front, c.lu_tree.flops / 1e9, lu_seconds,
+            (long long)lu_stats_zero);
   out->seconds_total = now_s() - t0;
   out->seconds_setup = t_setup;
   out->seconds_solve = t_solve;
   out->seconds_post = t_post;
+}
+
+// Checks a caller's complex CSR matrix before anything is uploaded: rowptr starts at 0 and is monotone, every
+// column index lies in [0, n).  Returns nnz.
+static int64_t check_caller_csr(int64_t n, const int32_t *rowptr, const int32_t *colind) {
+  if (rowptr[0] != 0) throw std::invalid_argument("rowptr[0] must be 0");
+  for (int64_t i = 0; i < n; i++)
+    if (rowptr[i + 1] < rowptr[i]) throw std::invalid_argument("rowptr not monotone");
+  const int64_t nnz = rowptr[n];
+  for (int64_t k = 0; k < nnz; k++)
+    if (colind[k] < 0 || colind[k] >= n) throw std::invalid_argument("colind out of range");
+  return nnz;
+}
+
+static int csr_lanes_per_row(int group, int64_t n, int64_t nnz) {
+  return group > 1 ? 16 : nnz / n <= 12 ? 8 : nnz / n <= 48 ? 16 : 32;
+}
+
+// ------------------------------------------------------------------ allocation-free staging area
+// opem_spmv_noalloc stages the caller's arrays here instead of allocating.  Each GPU the library runs on holds
+// its own copy for the life of the process; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device (the array's name is a host-side stand-in, not a device address).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_addr;  // device -> address of its copy of scratch
+
+// Address of `scratch` on `device`, looked up once per device; `device` must be current.
+static void *scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) return it->second;
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+  g_scratch_addr[device] = p;
+  return p;
 }
 
 }  // namespace opem
@@ -1929,7 +2028,9 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
 // ====================================================================== C ABI
 using namespace opem;
 
-struct opem_ctx : public Ctx {};
+struct opem_ctx : public Ctx {
+  void *scratch = nullptr;  // this device's copy of the staging area of opem_spmv_noalloc
+};
 
 template <class F>
 static int guarded(opem_ctx *ctx, F f) {
@@ -1963,6 +2064,13 @@ int opem_create(opem_ctx **ctx, int device) {
     delete c;
     return 4;
   }
+  try {
+    c->scratch = scratch_address(device);
+  } catch (const std::exception &) {
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return 5;
+  }
   *ctx = c;
   return 0;
 }
@@ -1995,6 +2103,7 @@ int opem_mesh_set(opem_ctx *ctx, int64_t nv, const double *xyz, int64_t nt, cons
     m.bface_attr.assign(bface_attr, bface_attr + nbf);
     ctx->have_mesh = true;
     ctx->have_space = false;
+    ctx->spmv_form.clear();  // operator forms were chosen and checked on the previous pattern
   });
 }
 
@@ -2003,6 +2112,13 @@ int opem_space_build(opem_ctx *ctx, int order, int64_t *ndof, int64_t *nnz) {
     if (!ctx->have_mesh) throw std::invalid_argument("mesh not set");
     if (order < 1 || order > 3) throw std::invalid_argument("order must be 1..3");
     int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    // everything derived from the previous space is invalid from here on, also if this build fails halfway
+    ctx->have_space = false;
+    ctx->have_patches = false;
+    ctx->have_coarse = false;
+    ctx->hpatch_state = 0;
+    ctx->spmv_form.clear();
+    ctx->have_lu_tree = false;
     build_space(ctx->mesh, order, ctx->sp, nth);
     device_space(*ctx);
     ctx->have_space = true;
@@ -2153,17 +2269,14 @@ int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const i
               const double *x, double *y) {
   return guarded(ctx, [&] {
     if (n <= 0 || !rowptr || !colind || !vals || !x || !y) throw std::invalid_argument("bad arguments");
-    const int64_t nnz = rowptr[n];
-    for (int64_t i = 0; i < n; i++)
-      if (rowptr[i + 1] < rowptr[i]) throw std::invalid_argument("rowptr not monotone");
-    for (int64_t k = 0; k < nnz; k++)
-      if (colind[k] < 0 || colind[k] >= n) throw std::invalid_argument("colind out of range");
+    if (prec != OPEM_PREC_FP64 && prec != OPEM_PREC_MIXED) throw std::invalid_argument("bad precision");
+    const int64_t nnz = check_caller_csr(n, rowptr, colind);
     hipStream_t st = ctx->st;
     DevBuf drp, dci, dv, dx, dy;
     drp.upload(std::vector<int32_t>(rowptr, rowptr + n + 1), st);
     dci.upload(std::vector<int32_t>(colind, colind + nnz), st);
     const int group = csr_row_group(n, rowptr, colind);
-    const int lpr = group > 1 ? 16 : nnz / n <= 12 ? 8 : nnz / n <= 48 ? 16 : 32;
+    const int lpr = csr_lanes_per_row(group, n, nnz);
     if (prec == OPEM_PREC_FP64) {
       dv.upload(std::vector<double>(vals, vals + 2 * nnz), st);
       dx.upload(std::vector<double>(x, x + 2 * n), st);
@@ -2188,6 +2301,46 @@ int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const i
   });
 }
 
+int opem_spmv_noalloc(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind,
+                      const double *vals, const double *x, double *y) {
+  return guarded(ctx, [&] {
+    if (n <= 0 || !rowptr || !colind || !vals || !x || !y) throw std::invalid_argument("bad arguments");
+    if (prec != OPEM_PREC_FP64 && prec != OPEM_PREC_MIXED) throw std::invalid_argument("bad precision");
+    const int64_t nnz = check_caller_csr(n, rowptr, colind);
+    // staging layout (256-byte aligned pieces): rowptr, colind, FP64 values, x and y; FP32 adds converted copies
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const bool f32 = prec == OPEM_PREC_MIXED;
+    const size_t o_rp = 0, o_ci = o_rp + up((size_t)(n + 1) * 4), o_v = o_ci + up((size_t)nnz * 4),
+                 o_x = o_v + up((size_t)nnz * 16), o_y = o_x + up((size_t)n * 16), o_v32 = o_y + up((size_t)n * 16),
+                 o_x32 = o_v32 + (f32 ? up((size_t)nnz * 8) : 0), o_y32 = o_x32 + (f32 ? up((size_t)n * 8) : 0),
+                 total = o_y32 + (f32 ? up((size_t)n * 8) : 0);
+    if (total > kScratchBytes)
+      throw std::invalid_argument("matrix and vectors need " + std::to_string(total) +
+                                  " staging bytes; opem_spmv_noalloc holds at most " + std::to_string(kScratchBytes));
+    char *base = (char *)ctx->scratch;
+    hipStream_t st = ctx->st;
+    HIPCHK(hipMemcpyAsync(base + o_rp, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_ci, colind, (size_t)nnz * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_v, vals, (size_t)nnz * 16, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_x, x, (size_t)n * 16, hipMemcpyHostToDevice, st));
+    const int group = csr_row_group(n, rowptr, colind);
+    const int lpr = csr_lanes_per_row(group, n, nnz);
+    const int32_t *rp = (const int32_t *)(base + o_rp), *ci = (const int32_t *)(base + o_ci);
+    if (!f32) {
+      launch_csr_spmv<double2>(st, n, rp, ci, (const double2 *)(base + o_v), nullptr, (const double2 *)(base + o_x),
+                               (double2 *)(base + o_y), group, lpr);
+    } else {
+      launch_convert<double2, float2>(st, nnz, (const double2 *)(base + o_v), (float2 *)(base + o_v32));
+      launch_convert<double2, float2>(st, n, (const double2 *)(base + o_x), (float2 *)(base + o_x32));
+      launch_csr_spmv<float2>(st, n, rp, ci, (const float2 *)(base + o_v32), nullptr, (const float2 *)(base + o_x32),
+                              (float2 *)(base + o_y32), group, lpr);
+      launch_convert<float2, double2>(st, n, (const float2 *)(base + o_y32), (double2 *)(base + o_y));
+    }
+    HIPCHK(hipMemcpyAsync(y, base + o_y, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
 int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
   return guarded(ctx, [&] {
     if (!in || !out || !out->S || in->nfreq < 1 || !in->freqs || !in->port_attr || !in->mode_E || !in->mode_H ||
@@ -2195,13 +2348,12 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
       throw std::invalid_argument("bad sweep arguments");
     if (in->precision != OPEM_PREC_FP64 && in->precision != OPEM_PREC_MIXED)
       throw std::invalid_argument("bad precision");
-    if (in->precond < 0 || in->precond > 1) throw std::invalid_argument("bad preconditioner");
+    if (in->precond < 0 || in->precond > 2) throw std::invalid_argument("bad preconditioner");
     if (in->coarse_size < 0 || in->coarse_size > 128) throw std::invalid_argument("coarse_size must be 0..128");
     // mixed precision applies the FP16 patch inverses on MFMA, up to 32 systems per inverse set
-    const int share_max = in->precision == OPEM_PREC_MIXED ? 32 : 8;
+    const int share_max = in->precond == 2 ? 64 : in->precision == OPEM_PREC_MIXED ? 32 : 8;
     if (in->precond_share < 0 || in->precond_share * in->nports > share_max)
-      throw std::invalid_argument(share_max == 32 ? "precond_share x nports must be <= 32"
-                                                  : "precond_share x nports must be <= 8");
+      throw std::invalid_argument("precond_share x nports must be <= " + std::to_string(share_max));
     if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");
     if (in->arnoldi_step < 0 || in->arnoldi_step > 4) throw std::invalid_argument("arnoldi_step must be 0..4");
     if (in->warm_start < 0 || in->warm_start > 4) throw std::invalid_argument("warm_start must be 0..4");

@@ -7,6 +7,8 @@
     python -m openparem_amd.measure c2   # assembly kernel alone: element matrices of 1M random tets, ND1-ND3,
                                          # FP64 / FP32 / BF16-scaled MFMA
 
+    python -m openparem_amd.measure kernels  # the roofline kernels alone at the guide benchmark's shape (profiling)
+
     python -m openparem_amd.measure c5   # the seeded mixed set of workloads.c5_instances(): one batched mixed
                                          # sweep per instance, plain FP64 at its middle frequency
 
@@ -158,6 +160,38 @@ def c5():
            "aggregate_freq_points_per_s": round(tot_pts / tot_t, 3)})
 
 
+def kernels(restart=300, ns=3):
+    """The roofline kernels alone at the guide benchmark's shape, for rocprofv3 --pmc and --kernel-trace runs.
+
+    In dispatch order after the set-up: the s-step projection / update k_mdot_s / k_mupdate_s (FP32 basis, 128
+    systems x (restart/2 + 1) vectors, the mean basis of a GMRES(restart) cycle, ns new vectors per pass: one
+    warm-up and two timed launches each), the batched operator (FP32, 64 frequencies x 2 excitations: one warm-up
+    and two timed launches), the single-matrix complex CSR SpMV k_csr_spmv in FP64 and then FP32 (one launch per
+    copy of the matrix, then two timed launches), then two 1 GiB device-to-device copies whose FETCH_SIZE /
+    WRITE_SIZE calibrate the counters (a copy reads and writes exactly 1 GiB)."""
+    import torch
+
+    w = workloads.make("waveguide-sweep")
+    s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+    try:
+        nf, R = len(w["freqs"]), len(s.ports)
+        (tm, tu), (bm, bu) = s.krylov_bench("fp32", S=nf * R, k=restart // 2 + 1, nrep=2, ns=ns)
+        ts, bs = s.spmv_bench("fp32", ngroups=nf, R=R, nrep=2)
+        tc64, bc64 = s.csr_spmv_bench("fp64", nrep=2)
+        tc32, bc32 = s.csr_spmv_bench("fp32", nrep=2)
+        _emit({"k_mdot_s": {"ms": tm * 1e3, "alg_bytes": bm}, "k_mupdate_s": {"ms": tu * 1e3, "alg_bytes": bu},
+               "spmv_sweep": {"ms": ts * 1e3, "alg_bytes": bs},
+               "csr_spmv_fp64": {"ms": tc64 * 1e3, "alg_bytes": bc64},
+               "csr_spmv_fp32": {"ms": tc32 * 1e3, "alg_bytes": bc32}})
+    finally:
+        s.close()
+    a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda:0")
+    d = torch.empty_like(a)
+    for _ in range(2):
+        d.copy_(a)  # device to device: 1 GiB read + 1 GiB write
+    torch.cuda.synchronize()
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c3"
-    {"c2": c2, "c3": c3, "c4": c4, "c5": c5}[which]()
+    {"c2": c2, "c3": c3, "c4": c4, "c5": c5, "kernels": kernels}[which]()

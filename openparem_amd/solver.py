@@ -140,7 +140,9 @@ class Solver:
         """One ``opem_sweep``: S (forced reciprocal unless ``reciprocal=False``) for every frequency, all port
         excitations.  ``precision`` "fp64" or "mixed" (FP32 inner GMRES, FP64 refinement); ``batch`` frequencies per
         batched solve (0: all); GMRES(``restart``) to the true residual ``tol``; ``precond`` "patch" (vertex
-        patches) or "element" (element blocks); ``precond_share`` adjacent frequencies per set of patch inverses;
+        patches), "element" (element blocks) or "lu" (sparse LU of each frequency's matrix, FP32 factors in mixed
+        precision); ``precond_share`` adjacent frequencies per set of patch inverses or per LU factor (their mean
+        matrix);
         ``arnoldi_step`` Krylov vectors per Gram-Schmidt pass; ``warm_start`` bisection levels over the frequency
         index (interpolated starting iterates).  ``mode_fn(mesh, attr, f)`` supplies each port's modal E, H, gamma,
         V, Z (default: the mesh family's provider).  Returns S, S_raw, Cp, Cm, iters, true_res, converged and,
@@ -169,7 +171,7 @@ class Solver:
                            precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64,
                            batch=int(batch), restart=int(restart), maxit=int(maxit), tol=float(tol),
                            reciprocal=int(bool(reciprocal)), h_tol=float(h_tol),
-                           precond={"patch": 0, "element": 1}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes),
+                           precond={"patch": 0, "element": 1, "lu": 2}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes),
                            precond_share=int(precond_share), poly_degree=int(poly_degree),
                            arnoldi_step=int(arnoldi_step), warm_start=int(warm_start))
         S = np.zeros((nf, np_, np_), np.complex128)

@@ -80,10 +80,11 @@ def test_misuse_is_reported_and_leaves_the_context_usable():
             (dict(nports=1, port_attr=[1]), "not an OPEM_BDR_PORT"),
             (dict(nports=1, port_attr=[2]), "every port boundary must be an S-port"),
             (dict(nports=0, port_attr=[2]), "nports must be 1..4"),
+            (dict(nports=2, port_attr=[2, 2]), "duplicate"),
             (dict(freqs=[9e9, -1e9]), "frequencies must be positive"),
             (dict(freqs=[float("nan")]), "frequencies must be positive"),
             (dict(precision=5), "bad precision"),
-            (dict(precond=2), "bad preconditioner"),
+            (dict(precond=3), "bad preconditioner"),
             (dict(coarse_size=129), "coarse_size"),
             (dict(poly_degree=33), "poly_degree"),
             (dict(arnoldi_step=5), "arnoldi_step"),
@@ -120,3 +121,38 @@ def test_solver_raises_and_recovers():
         np.testing.assert_array_equal(r1["iters"], r0["iters"])
         r2 = s.sweep(f, precision="fp64", restart=4, maxit=2)  # too few iterations: a flag, not an error
         assert not r2["converged"].any() and (r2["iters"] <= 4).all()
+
+
+def test_spmv_rejects_bad_csr_before_upload():
+    """opem_spmv on a caller's CSR: rowptr[0] != 0, a negative rowptr[n], a column out of range and an unknown
+    precision are rejected on the host (-2) before anything reaches the device; y is left untouched."""
+    from openparem_amd.solver import Device
+    L = lib()
+    dev = Device()
+    try:
+        n = 6
+        ci = np.arange(n, dtype=np.int32)
+        v = np.zeros(2 * n, np.float64)
+        v[0::2] = 1.0  # identity (1 + 0j on the diagonal)
+        x = np.ones(2 * n, np.float64)  # 1 + 1j
+        for rp, prec, msg in ((np.arange(1, n + 2, dtype=np.int32), 0, "rowptr[0]"),
+                              (np.array([0, 1, 2, 3, 4, 5, -6], np.int32), 0, "monotone"),
+                              (np.arange(n + 1, dtype=np.int32), 2, "bad precision")):
+            y = np.full(2 * n, 5.0)
+            rc = L.opem_spmv(dev.ctx, prec, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
+                             ptr(x, C.c_double), ptr(y, C.c_double))
+            print(f"spmv misuse {msg}: rc={rc} {_err(L, dev.ctx)!r}", flush=True)
+            assert rc == -2 and msg in _err(L, dev.ctx)
+            assert (y == 5.0).all()
+        bad = ci.copy()
+        bad[3] = n
+        y = np.full(2 * n, 5.0)
+        rp = np.arange(n + 1, dtype=np.int32)
+        assert L.opem_spmv(dev.ctx, 0, n, ptr(rp, C.c_int32), ptr(bad, C.c_int32), ptr(v, C.c_double),
+                           ptr(x, C.c_double), ptr(y, C.c_double)) == -2
+        assert "colind" in _err(L, dev.ctx) and (y == 5.0).all()
+        assert L.opem_spmv(dev.ctx, 0, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
+                           ptr(x, C.c_double), ptr(y, C.c_double)) == 0
+        assert np.array_equal(y, x)
+    finally:
+        dev.close()

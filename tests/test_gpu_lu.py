@@ -1,0 +1,117 @@
+"""Sparse-LU-preconditioned sweeps (precond "lu": nested-dissection multifrontal factor on the GPU, the counterpart of
+the PCCHOLESKY/PCLU preconditioner of `solveComplexLinearSystem.c:133-168`) against the FP64 oracle's direct solve.
+
+A factor of the system's own matrix makes GMRES converge in a few iterations, so these tests also bound the
+iteration counts: a wrong factor (a dropped update, a misplaced pivot, a wrong root block) shows up as many
+iterations long before it shows up as a wrong S.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from openparem_amd.mesh import microstrip_mesh, tee_mesh, waveguide_mesh
+from openparem_amd.ports import te10
+from oracle import assemble as oasm
+from oracle.pipeline import Problem
+
+pytestmark = pytest.mark.gpu
+
+COPPER = 5.813e7
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _solver(mesh, p):
+    from openparem_amd.solver import Solver
+    bnd = {a: COPPER for a, k in mesh.bdr_kind.items() if k == "zs"}
+    return Solver(mesh, p, boundaries=bnd)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_lu_sweep_guide_against_oracle(p):
+    """Two-port guide at ND1-ND3, two frequencies: S and the E and H DOFs against the oracle; FP64 factors need one
+    GMRES cycle of at most a few iterations, FP32 factors a few refinement cycles."""
+    m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
+    f = np.array([9e9, 11e9])
+    print(f"lu guide p={p}", flush=True)
+    s = _solver(m, p)
+    try:
+        r64 = s.sweep(f, precision="fp64", tol=1e-12, batch=2, restart=30, precond="lu", want_fields=True)
+        rmx = s.sweep(f, precision="mixed", tol=1e-11, batch=2, restart=30, precond="lu", orth_passes=1)
+    finally:
+        s.close()
+    print(f"iterations fp64 {r64['iters'].tolist()} mixed {rmx['iters'].tolist()}", flush=True)
+    assert r64["converged"].all() and rmx["converged"].all()
+    assert r64["iters"].max() <= 6 and rmx["iters"].max() <= 20
+    pr = Problem(m, p)
+    for i, fi in enumerate(f):
+        o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+        assert np.abs(r64["S"][i] - o["S"]).max() < 1e-9
+        assert np.abs(rmx["S"][i] - o["S"]).max() < 1e-8
+        for k in range(len(pr.ports)):
+            xo, ho = o["info"][k]["x"], o["info"][k]["h"]
+            assert np.abs(r64["x"][i, k] - xo).max() < 1e-10 * np.abs(xo).max()
+            assert np.abs(r64["h"][i, k] - ho).max() < 1e-9 * np.abs(ho).max()
+
+
+def test_lu_one_port_short():
+    """One port (guide shorted at z = L): the port root has no free rows in the only excitation."""
+    m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4, short=True)
+    f = np.linspace(8e9, 12e9, 4)
+    s = _solver(m, 2)
+    try:
+        r = s.sweep(f, precision="mixed", tol=1e-11, batch=4, restart=30, precond="lu", precond_share=2)
+    finally:
+        s.close()
+    print(f"one-port iterations {r['iters'].ravel().tolist()}", flush=True)
+    assert r["converged"].all() and np.abs(np.abs(r["S"][:, 0, 0]) - 1).max() < 1e-8
+    pr = Problem(m, 2)
+    for i, fi in enumerate(f):
+        o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+        assert abs(r["S"][i, 0, 0] - o["S"][0, 0]) < 1e-8
+
+
+@pytest.mark.parametrize("cross", [False, True])
+def test_lu_tee_and_cross(cross):
+    """Three and four ports: every excitation factors the port rows of the other ports."""
+    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=cross)
+    f = np.array([9e9, 10e9])
+    s = _solver(m, 2)
+    try:
+        r = s.sweep(f, precision="fp64", tol=1e-12, batch=2, restart=30, precond="lu")
+    finally:
+        s.close()
+    print(f"{'cross' if cross else 'tee'} iterations {r['iters'].tolist()}", flush=True)
+    assert r["converged"].all() and r["iters"].max() <= 6
+    pr = Problem(m, 2)
+    for i, fi in enumerate(f):
+        o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+        assert np.abs(r["S"][i] - o["S"]).max() < 1e-9
+
+
+@pytest.fixture(scope="module")
+def band():
+    d = json.load(open(os.path.join(HERE, "golden", "microstrip_band.json")))
+    return d, np.array(d["freqs"]), np.array([[complex(*v) for v in S] for S in d["S"]]).reshape(-1, 2, 2)
+
+
+@pytest.mark.parametrize("prec,share", [("fp64", 1), ("mixed", 1), ("mixed", 2)])
+def test_lu_microstrip_band(band, prec, share):
+    """The reduced microstrip box over 1-20 GHz (multimode upper band included) against the oracle's S fixture."""
+    from openparem_amd.solver import Solver
+    d, f, So = band
+    m = microstrip_mesh(nx=8, ny=5, nz=8, n_sub=2, seed=6)
+    s = Solver(m, 2, boundaries={4: d["conductivity"]})
+    try:
+        assert s.ndof == d["ndof"]
+        r = s.sweep(f, precision=prec, tol=1e-11, batch=len(f), restart=40, precond="lu", precond_share=share,
+                    orth_passes=1 if prec == "mixed" else 2)
+    finally:
+        s.close()
+    dS = np.abs(r["S"] - So).max(axis=(1, 2))
+    print(f"band {prec} share {share}: iterations {r['iters'].ravel().tolist()} |dS| {dS.max():.2e}", flush=True)
+    assert r["converged"].all()
+    assert dS.max() < (1e-9 if prec == "fp64" else 1e-8)
+    if share == 1:
+        assert r["iters"].max() <= (6 if prec == "fp64" else 30)
