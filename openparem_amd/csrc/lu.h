@@ -24,7 +24,6 @@ namespace opem {
 struct LuFactorStats {
   double seconds_factor = 0;
   int64_t zero_pivots = 0;  // pivots that were exactly zero (replaced by a tiny value; the factor is then inexact)
-  int64_t lib_info = 0;     // nonzero info values returned by the dense library factorizations
 };
 
 class LuFactor {

@@ -3,17 +3,17 @@
 // Factorization, level by level of the assembly tree (all fronts of a level are independent):
 //   1. every front is cleared and receives its original matrix entries (one scatter over the pattern);
 //   2. children's update blocks are added into their parents (one launch per child slot, so no two writers meet);
-//   3. small fronts: one workgroup per (front, matrix) runs a right-looking LU in panels of 16 columns with row
-//      pivoting inside the pivot block, then inverts the unit-lower and upper factors of the pivot block in place;
-//      large fronts: the dense library (rocSOLVER getrf on the pivot rows, rocBLAS trsm + gemm for the Schur
-//      complement, rocSOLVER trtri for the inverses), batched over the matrices.
+//   3. blocked right-looking LU of every front of the level at once, kNB pivot columns per step: panel
+//      factorization with row pivoting inside the pivot block, the panel's interchanges on the other columns,
+//      the block row of U, and the tiled trailing update (which also forms the Schur complement of the update
+//      rows); then the unit-lower and upper factors of each pivot block are inverted (the same triangular
+//      block-row and tiled update kernels on scratch) and written back into the pivot block.
+// Each kernel launch serves every front of a level and every matrix of the batch (blockIdx.y), so a level costs a
+// few launches per kNB columns of its largest pivot block however many fronts it has.
 // Apply (multi right-hand side, all levels): forward through the tree with front vectors (children's update
 // parts added into the parent's), backward with the solved ancestors' values gathered through the update lists.
 // Every dense step is y = M x with M a block of a factored front, so all threads of a level work at once.
 #include "lu.h"
-
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -25,13 +25,6 @@
 
 namespace opem {
 
-#define ROCBLAS_CHK(x)                                                                          \
-  do {                                                                                          \
-    rocblas_status s_ = (x);                                                                    \
-    if (s_ != rocblas_status_success)                                                           \
-      throw std::runtime_error(std::string(#x) + ": " + rocblas_status_to_string(s_));          \
-  } while (0)
-
 #define LU_LAUNCH_CHECK()                                                                        \
   do {                                                                                           \
     hipError_t e_ = hipGetLastError();                                                           \
@@ -40,8 +33,7 @@ namespace opem {
 
 namespace {
 
-constexpr int kNB = 16;          // panel width of the small-front LU
-constexpr int kSmallFront = 1536;  // fronts up to this order use the one-workgroup kernel
+constexpr int kNB = 32;            // pivot columns per step of the blocked LU
 constexpr int kMaxRhsThread = 16;  // right-hand sides per thread in the apply kernels
 constexpr int kKC = 64;            // input rows staged in LDS per pass of the apply kernels
 
@@ -97,7 +89,7 @@ __device__ __forceinline__ double cabs1(T a) {
 struct LuDev {
   const int64_t *dbeg, *uoff, *foff, *fvoff;
   const int32_t *uidx, *umap, *parent, *kid_off, *kids;
-  int64_t n, fstride, fvstride;
+  int64_t n, ppqs, fstride, fvstride;
 };
 
 // ---------------------------------------------------------------- factorization kernels
@@ -148,67 +140,115 @@ __global__ __launch_bounds__(256) void k_lu_extend_add(const int2 *__restrict__ 
   }
 }
 
-// Partial LU of one front per workgroup (see the file comment).  pp[k] (k < nd): the front row that ended at pivot
-// position k.  Exactly zero pivots are replaced by a tiny value and counted.
+// ---------------------------------------------------------------- blocked dense LU of many fronts at once
+// A dense block is described by where it lives: selector (0 = fronts, 1 = port blocks, 2 = L-inverse scratch,
+// 3 = U-inverse scratch), offset, leading dimension.  Each launch below serves every front of a tree level (and
+// every matrix, blockIdx.y); per step of kNB pivot columns: panel LU, row interchanges of the other columns,
+// block row of U, trailing update.
+constexpr int kPanelThreads = 1024;
+constexpr int kTile = 64;
+
+struct DenseFront {
+  int sel;          // where the front lives (0 fronts, 1 port blocks)
+  int nf, nd;       // order, pivot rows
+  int pad;
+  int64_t off;      // offset of the front (column-major, ld = nf)
+  int64_t pp_off;   // offset of its pivot permutation / interchange record (int32 arrays, per-matrix stride)
+  int64_t x_off;    // offset of its nd x nd inverse scratch (selectors 2 and 3)
+};
+
+struct DenseBase {
+  void *p[4];
+  int64_t qs[4];  // per-matrix strides (elements)
+  int32_t *pp;    // pivot permutations: fronts [matrix][n], port blocks after them
+  int32_t *sw;    // interchange record, same layout
+  int64_t ppqs;   // per-matrix stride of pp / sw
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_front_small(const int32_t *__restrict__ nodes, LuDev D,
-                                                        T *__restrict__ Fall, int32_t *__restrict__ ppall,
-                                                        unsigned long long *__restrict__ zero_pivots) {
+__device__ __forceinline__ T *dense_ptr(const DenseBase &B, int sel, int64_t off, int q) {
+  return (T *)B.p[sel] + q * B.qs[sel] + off;
+}
+
+// Panel: columns k0 .. k0+nb-1 over the pivot rows k0 .. nd-1, row pivoting among them.  Columns are taken in
+// groups of kGrp: inside a group rank-1 updates, then the group's block row of the later panel columns and one
+// rank-kGrp update of them.  Interchanges are applied inside the panel only and recorded in sw; pp follows them.
+// Exactly zero pivots are replaced by a tiny value and counted.  The update rows (nd .. nf-1) of the panel are
+// solved afterwards by k_dense_lrows.
+constexpr int kGrp = 8;
+
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void k_dense_panel(const int32_t *__restrict__ list,
+                                                               const DenseFront *__restrict__ fr, DenseBase B, int k0,
+                                                               unsigned long long *__restrict__ zero_pivots) {
   using R = decltype(T().x);
-  const int t = nodes[blockIdx.x];
+  const DenseFront f = fr[list[blockIdx.x]];
   const int q = blockIdx.y;
-  const int nd = (int)(D.dbeg[t + 1] - D.dbeg[t]);
-  const int nf = nd + (int)(D.uoff[t + 1] - D.uoff[t]);
-  T *F = Fall + q * D.fstride + D.foff[t];
-  int32_t *pp = ppall + q * D.n + D.dbeg[t];
-  const int tid = threadIdx.x;
-  __shared__ double red_v[256];
-  __shared__ int red_i[256];
-  __shared__ T Lkk[kNB * kNB];
-  __shared__ T colbuf[kSmallFront];
-  for (int k = tid; k < nd; k += 256) pp[k] = k;
+  const int nf = f.nf, nd = f.nd;
+  const int nb = min(kNB, nd - k0);
+  T *F = dense_ptr<T>(B, f.sel, f.off, q);
+  int32_t *pp = B.pp + q * B.ppqs + f.pp_off;
+  int32_t *sw = B.sw + q * B.ppqs + f.pp_off;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  __shared__ double rv[kPanelThreads / 64];
+  __shared__ int ri[kPanelThreads / 64];
+  __shared__ int piv_s;
+  __shared__ T Lg[kGrp * kGrp];
+  if (k0 == 0)
+    for (int k = tid; k < nd; k += kPanelThreads) pp[k] = k;
   __syncthreads();
-  for (int k0 = 0; k0 < nd; k0 += kNB) {
-    const int nb = min(kNB, nd - k0);
-    // ---- panel: columns k0 .. k0+nb-1
-    for (int j = k0; j < k0 + nb; j++) {
+  for (int g0 = k0; g0 < k0 + nb; g0 += kGrp) {
+    const int g1 = min(g0 + kGrp, k0 + nb);
+    for (int j = g0; j < g1; j++) {
       double best = -1.0;
       int bi = j;
-      for (int i = j + tid; i < nd; i += 256) {
+      for (int i = j + tid; i < nd; i += kPanelThreads) {
         const double v = cabs1(F[i + (int64_t)nf * j]);
         if (v > best) {
           best = v;
           bi = i;
         }
       }
-      red_v[tid] = best;
-      red_i[tid] = bi;
-      __syncthreads();
-      for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) {
-          const double v2 = red_v[tid + s];
-          const int i2 = red_i[tid + s];
-          if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) {
-            red_v[tid] = v2;
-            red_i[tid] = i2;
-          }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(best, o, 64);
+        const int i2 = __shfl_xor(bi, o, 64);
+        if (v2 > best || (v2 == best && i2 < bi)) {
+          best = v2;
+          bi = i2;
         }
-        __syncthreads();
       }
-      const int piv = red_i[0];
-      const bool zero = !(red_v[0] > 0.0);
+      if (lane == 0) {
+        rv[wid] = best;
+        ri[wid] = bi;
+      }
       __syncthreads();
-      if (piv != j) {
-        for (int c = tid; c < nf; c += 256) {
-          const T a = F[j + (int64_t)nf * c];
-          F[j + (int64_t)nf * c] = F[piv + (int64_t)nf * c];
-          F[piv + (int64_t)nf * c] = a;
-        }
-        if (tid == 0) {
+      if (tid == 0) {
+        double bb = rv[0];
+        int p = ri[0];
+        for (int w = 1; w < kPanelThreads / 64; w++)
+          if (rv[w] > bb || (rv[w] == bb && ri[w] < p)) {
+            bb = rv[w];
+            p = ri[w];
+          }
+        sw[j] = p;
+        if (p != j) {
           const int32_t a = pp[j];
-          pp[j] = pp[piv];
-          pp[piv] = a;
+          pp[j] = pp[p];
+          pp[p] = a;
         }
+        piv_s = bb > 0.0 ? p : -1 - p;
+        if (!(bb > 0.0)) atomicAdd(zero_pivots, 1ull);
+      }
+      __syncthreads();
+      int p = piv_s;
+      const bool zero = p < 0;
+      if (zero) p = -1 - p;
+      if (p != j && tid < nb) {
+        const int c = k0 + tid;
+        const T a = F[j + (int64_t)nf * c];
+        F[j + (int64_t)nf * c] = F[p + (int64_t)nf * c];
+        F[p + (int64_t)nf * c] = a;
       }
       __syncthreads();
       if (zero && tid == 0) {
@@ -216,130 +256,279 @@ __global__ __launch_bounds__(256) void k_lu_front_small(const int32_t *__restric
         tiny.x = std::is_same<R, float>::value ? (R)1e-30 : (R)1e-290;
         tiny.y = 0;
         F[j + (int64_t)nf * j] = tiny;
-        atomicAdd(zero_pivots, 1ull);
       }
       __syncthreads();
       const T inv = cinv_t(F[j + (int64_t)nf * j]);
-      for (int i = j + 1 + tid; i < nf; i += 256) F[i + (int64_t)nf * j] = cmul_t(F[i + (int64_t)nf * j], inv);
-      __syncthreads();
-      // rank-1 update of the remaining panel columns
-      const int ncol = k0 + nb - (j + 1);
-      if (ncol > 0) {
-        for (int i = j + 1 + tid; i < nf; i += 256) {
-          const T l = F[i + (int64_t)nf * j];
-          for (int c = j + 1; c < k0 + nb; c++) {
-            const T u = F[j + (int64_t)nf * c];
-            F[i + (int64_t)nf * c] = csub_t(F[i + (int64_t)nf * c], cmul_t(l, u));
-          }
-        }
+      for (int i = j + 1 + tid; i < nd; i += kPanelThreads) {
+        const T l = cmul_t(F[i + (int64_t)nf * j], inv);
+        F[i + (int64_t)nf * j] = l;
+        for (int c = j + 1; c < g1; c++) F[i + (int64_t)nf * c] = csub_t(F[i + (int64_t)nf * c], cmul_t(l, F[j + (int64_t)nf * c]));
       }
       __syncthreads();
     }
-    // ---- block row of U: rows k0 .. k0+nb-1, columns k0+nb .. nf-1  (unit lower L_kk solve)
-    for (int e = tid; e < nb * nb; e += 256) Lkk[e] = F[(k0 + e % nb) + (int64_t)nf * (k0 + e / nb)];
-    __syncthreads();
-    for (int c = k0 + nb + tid; c < nf; c += 256) {
-      T x[kNB];
+    if (g1 < k0 + nb) {
+      // block row of the later panel columns: rows g0 .. g1-1 = Lgg^-1 (unit lower) rows g0 .. g1-1
+      const int gn = g1 - g0;
+      for (int e = tid; e < gn * gn; e += kPanelThreads) Lg[e] = F[(g0 + e % gn) + (int64_t)nf * (g0 + e / gn)];
+      __syncthreads();
+      if (tid < k0 + nb - g1) {
+        const int c = g1 + tid;
+        T x[kGrp];
 #pragma unroll
-      for (int r = 0; r < kNB; r++)
-        if (r < nb) x[r] = F[k0 + r + (int64_t)nf * c];
+        for (int r = 0; r < kGrp; r++)
+          if (r < gn) x[r] = F[g0 + r + (int64_t)nf * c];
 #pragma unroll
-      for (int r = 1; r < kNB; r++)
-        if (r < nb) {
+        for (int r = 1; r < kGrp; r++)
+          if (r < gn) {
 #pragma unroll
-          for (int s = 0; s < kNB; s++)
-            if (s < r) x[r] = csub_t(x[r], cmul_t(Lkk[r + nb * s], x[s]));
-        }
-#pragma unroll
-      for (int r = 1; r < kNB; r++)
-        if (r < nb) F[k0 + r + (int64_t)nf * c] = x[r];
-    }
-    __syncthreads();
-    // ---- trailing update: rows and columns k0+nb .. nf-1
-    const int r1 = k0 + nb;
-    if (r1 < nf) {
-      const int tx = tid & 63, ty = tid >> 6;
-      for (int i0 = r1; i0 < nf; i0 += 64) {
-        const int i = i0 + tx;
-        T l[kNB];
-        if (i < nf) {
-#pragma unroll
-          for (int s = 0; s < kNB; s++)
-            if (s < nb) l[s] = F[i + (int64_t)nf * (k0 + s)];
-        }
-        for (int c = r1 + ty; c < nf; c += 4) {
-          if (i < nf) {
-            T acc = F[i + (int64_t)nf * c];
-            const T *u = F + k0 + (int64_t)nf * c;
-#pragma unroll
-            for (int s = 0; s < kNB; s++)
-              if (s < nb) acc = csub_t(acc, cmul_t(l[s], u[s]));
-            F[i + (int64_t)nf * c] = acc;
+            for (int s2 = 0; s2 < kGrp; s2++)
+              if (s2 < r) x[r] = csub_t(x[r], cmul_t(Lg[r + gn * s2], x[s2]));
+            F[g0 + r + (int64_t)nf * c] = x[r];
           }
+      }
+      __syncthreads();
+      // rank-gn update of the later panel columns on the pivot rows below the group
+      for (int i = g1 + tid; i < nd; i += kPanelThreads) {
+        T l[kGrp];
+#pragma unroll
+        for (int s2 = 0; s2 < kGrp; s2++)
+          if (s2 < gn) l[s2] = F[i + (int64_t)nf * (g0 + s2)];
+        for (int c = g1; c < k0 + nb; c++) {
+          T acc = F[i + (int64_t)nf * c];
+#pragma unroll
+          for (int s2 = 0; s2 < kGrp; s2++)
+            if (s2 < gn) acc = csub_t(acc, cmul_t(l[s2], F[g0 + s2 + (int64_t)nf * c]));
+          F[i + (int64_t)nf * c] = acc;
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  // ---- inverse of U11 in place (upper, with diagonal)
-  for (int j = 0; j < nd; j++) {
-    for (int k = tid; k <= j; k += 256) colbuf[k] = F[k + (int64_t)nf * j];
-    __syncthreads();
-    const T d = cinv_t(colbuf[j]);
-    T nd_;
-    nd_.x = -d.x;
-    nd_.y = -d.y;
-    for (int i = tid; i < j; i += 256) {
-      double2 acc = make_double2(0.0, 0.0);
-      for (int k = i; k < j; k++) {
-        const double2 a = d2(F[i + (int64_t)nf * k]), b = d2(colbuf[k]);
-        acc.x += a.x * b.x - a.y * b.y;
-        acc.y += a.x * b.y + a.y * b.x;
-      }
-      F[i + (int64_t)nf * j] = cmul_t(nd_, cvt<T>(acc));
-    }
-    if (tid == 0) F[j + (int64_t)nf * j] = d;
-    __syncthreads();
-  }
-  // ---- inverse of the unit-lower L11 in place (strictly lower part)
-  for (int j = nd - 2; j >= 0; j--) {
-    for (int k = j + 1 + tid; k < nd; k += 256) colbuf[k] = F[k + (int64_t)nf * j];
-    __syncthreads();
-    for (int i = j + 1 + tid; i < nd; i += 256) {
-      double2 acc = d2(colbuf[i]);
-      for (int k = j + 1; k < i; k++) {
-        const double2 a = d2(F[i + (int64_t)nf * k]), b = d2(colbuf[k]);
-        acc.x += a.x * b.x - a.y * b.y;
-        acc.y += a.x * b.y + a.y * b.x;
-      }
-      F[i + (int64_t)nf * j] = cvt<T>(make_double2(-acc.x, -acc.y));
-    }
-    __syncthreads();
   }
 }
 
-// LAPACK-style row interchanges (1-based, relative to the block) -> the permutation pp[k] = block row at pivot k.
-// One workgroup per (block, matrix); the swaps run in LDS on one thread.
-__global__ void k_lu_ipiv_to_perm(const int64_t *__restrict__ off, const int32_t *__restrict__ len,
-                                  const int32_t *__restrict__ ipiv, int32_t *__restrict__ pp, int64_t qstride) {
-  extern __shared__ int32_t sp[];
-  const int b = blockIdx.x, q = blockIdx.y;
-  const int n = len[b];
-  const int32_t *ip = ipiv + q * qstride + off[b];
-  int32_t *out = pp + q * qstride + off[b];
-  for (int k = threadIdx.x; k < n; k += blockDim.x) sp[k] = k;
+// Update rows of a factored panel: rows nd .. nf-1 of columns k0 .. k0+nb-1 times U_kk^-1 (from the right).
+// Task = (front, first row after nd); one thread per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_lrows(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr,
+                                                     DenseBase B, int k0) {
+  const int2 tk = tasks[blockIdx.x];
+  const DenseFront f = fr[tk.x];
+  const int q = blockIdx.y;
+  const int nf = f.nf, nd = f.nd;
+  const int nb = min(kNB, nd - k0);
+  T *F = dense_ptr<T>(B, f.sel, f.off, q);
+  __shared__ T Us[kNB * kNB];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) Us[e] = F[(k0 + e % nb) + (int64_t)nf * (k0 + e / nb)];
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 0; k < n; k++) {
-      const int s = ip[k] - 1;
-      if (s != k && s >= 0 && s < n) {
-        const int32_t a = sp[k];
-        sp[k] = sp[s];
-        sp[s] = a;
-      }
+  const int i = nd + tk.y + threadIdx.x;
+  if (i >= nf) return;
+  T x[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; c++)
+    if (c < nb) x[c] = F[i + (int64_t)nf * (k0 + c)];
+#pragma unroll
+  for (int c = 0; c < kNB; c++)
+    if (c < nb) {
+#pragma unroll
+      for (int s2 = 0; s2 < kNB; s2++)
+        if (s2 < c) x[c] = csub_t(x[c], cmul_t(x[s2], Us[s2 + nb * c]));
+      x[c] = cmul_t(x[c], cinv_t(Us[c + nb * c]));
+      F[i + (int64_t)nf * (k0 + c)] = x[c];
     }
+}
+
+// Row interchanges of a panel applied to every other column of the front.  Task = (front, first column); one
+// thread per column, the nb interchanges in order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_laswp(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr,
+                                                     DenseBase B, int k0) {
+  const int2 tk = tasks[blockIdx.x];
+  const DenseFront f = fr[tk.x];
+  const int q = blockIdx.y;
+  const int c = tk.y + threadIdx.x;
+  const int nb = min(kNB, f.nd - k0);
+  if (c >= f.nf || (c >= k0 && c < k0 + nb)) return;
+  T *F = dense_ptr<T>(B, f.sel, f.off, q) + (int64_t)f.nf * c;
+  const int32_t *sw = B.sw + q * B.ppqs + f.pp_off;
+  for (int j = k0; j < k0 + nb; j++) {
+    const int p = sw[j];
+    if (p != j) {
+      const T a = F[j];
+      F[j] = F[p];
+      F[p] = a;
+    }
+  }
+}
+
+// Triangular block-row solve: X (nb x ncols, rows of one block) = T^-1 X, T the nb x nb diagonal block, unit lower
+// (upper == 0) or upper with its diagonal (upper == 1).  Task = (job, first column); one thread per column.
+struct TrsmJob {
+  int tsel, xsel;
+  int nb, ncols;
+  int64_t toff, xoff;
+  int ldt, ldx;
+  int upper, pad;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_trsm(const int2 *__restrict__ tasks, const TrsmJob *__restrict__ jobs,
+                                                    DenseBase B) {
+  const int2 tk = tasks[blockIdx.x];
+  const TrsmJob J = jobs[tk.x];
+  const int q = blockIdx.y;
+  __shared__ T Ts[kNB * kNB];
+  const T *Tm = dense_ptr<T>(B, J.tsel, J.toff, q);
+  for (int e = threadIdx.x; e < J.nb * J.nb; e += 256) Ts[e] = Tm[(e % J.nb) + (int64_t)J.ldt * (e / J.nb)];
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += blockDim.x) out[k] = sp[k];
+  const int c = tk.y + threadIdx.x;
+  if (c >= J.ncols) return;
+  T *X = dense_ptr<T>(B, J.xsel, J.xoff, q) + (int64_t)J.ldx * c;
+  const int nb = J.nb;
+  T x[kNB];
+#pragma unroll
+  for (int r = 0; r < kNB; r++)
+    if (r < nb) x[r] = X[r];
+  if (!J.upper) {
+#pragma unroll
+    for (int r = 1; r < kNB; r++)
+      if (r < nb) {
+#pragma unroll
+        for (int s = 0; s < kNB; s++)
+          if (s < r) x[r] = csub_t(x[r], cmul_t(Ts[r + nb * s], x[s]));
+      }
+  } else {
+#pragma unroll
+    for (int r = kNB - 1; r >= 0; r--)
+      if (r < nb) {
+#pragma unroll
+        for (int s = 0; s < kNB; s++)
+          if (s > r && s < nb) x[r] = csub_t(x[r], cmul_t(Ts[r + nb * s], x[s]));
+        x[r] = cmul_t(x[r], cinv_t(Ts[r + nb * r]));
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < kNB; r++)
+    if (r < nb) X[r] = x[r];
+}
+
+// Tiled C -= A B, A m x k, B k x n (k <= kNB), C m x n, for a list of jobs; blocks are numbered job after job
+// (prefix of tile counts), each block one kTile x kTile tile of its job.  256 threads as 16 x 16, 4 x 4
+// complex values per thread; the A and B slices are staged in LDS.
+struct GemmJob {
+  int asel, bsel, csel, pad;
+  int m, n, k, lda;
+  int ldb, ldc, pad2, pad3;
+  int64_t aoff, boff, coff;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_gemm(const GemmJob *__restrict__ jobs, const int64_t *__restrict__ prefix,
+                                                    int njobs, DenseBase B) {
+  __shared__ T As[kNB][kTile];
+  __shared__ T Bs[kNB][kTile];
+  const int64_t b = blockIdx.x;
+  const int q = blockIdx.y;
+  int lo = 0, hi = njobs - 1;  // last job with prefix <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const GemmJob J = jobs[lo];
+  const int64_t local = b - prefix[lo];
+  const int ntc = (J.n + kTile - 1) / kTile;
+  const int r0 = (int)(local / ntc) * kTile, c0 = (int)(local % ntc) * kTile;
+  const T *A = dense_ptr<T>(B, J.asel, J.aoff, q);
+  const T *Bm = dense_ptr<T>(B, J.bsel, J.boff, q);
+  T *C = dense_ptr<T>(B, J.csel, J.coff, q);
+  const int tid = threadIdx.x;
+  const int K = J.k;
+  for (int e = tid; e < kTile * kNB; e += 256) {
+    const int rr = e % kTile, s = e / kTile;
+    T v;
+    v.x = 0;
+    v.y = 0;
+    if (s < K && r0 + rr < J.m) v = A[(r0 + rr) + (int64_t)J.lda * s];
+    As[s][rr] = v;
+  }
+  for (int e = tid; e < kTile * kNB; e += 256) {
+    const int s = e % kNB, cc = e / kNB;
+    T v;
+    v.x = 0;
+    v.y = 0;
+    if (s < K && c0 + cc < J.n) v = Bm[s + (int64_t)J.ldb * (c0 + cc)];
+    Bs[s][cc] = v;
+  }
+  __syncthreads();
+  const int tr = tid & 15, tc = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      acc[i][jj].x = 0;
+      acc[i][jj].y = 0;
+    }
+  for (int s = 0; s < K; s++) {
+    T a[4], bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) a[i] = As[s][tr + 16 * i];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) bv[jj] = Bs[s][tc + 16 * jj];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        acc[i][jj].x += a[i].x * bv[jj].x - a[i].y * bv[jj].y;
+        acc[i][jj].y += a[i].x * bv[jj].y + a[i].y * bv[jj].x;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int r = r0 + tr + 16 * i;
+    if (r >= J.m) continue;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const int c = c0 + tc + 16 * jj;
+      if (c >= J.n) continue;
+      T *p = C + r + (int64_t)J.ldc * c;
+      *p = csub_t(*p, acc[i][jj]);
+    }
+  }
+}
+
+// Scratch X = I and Y = I (nd x nd) for every listed front.  Task = (front, first column of a 16-column chunk).
+template <typename T>
+__global__ void k_dense_eye(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr, DenseBase B) {
+  const int2 tk = tasks[blockIdx.x];
+  const DenseFront f = fr[tk.x];
+  const int q = blockIdx.y;
+  T *X = dense_ptr<T>(B, 2, f.x_off, q), *Y = dense_ptr<T>(B, 3, f.x_off, q);
+  const int c1 = min(tk.y + 16, f.nd);
+  for (int c = tk.y; c < c1; c++)
+    for (int i = threadIdx.x; i < f.nd; i += blockDim.x) {
+      T v;
+      v.x = i == c ? 1 : 0;
+      v.y = 0;
+      X[i + (int64_t)f.nd * c] = v;
+      Y[i + (int64_t)f.nd * c] = v;
+    }
+}
+
+// Pivot block <- strictly lower part of L^-1 (scratch 2) + upper part of U^-1 (scratch 3), same tasks.
+template <typename T>
+__global__ void k_dense_put_inverses(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr, DenseBase B) {
+  const int2 tk = tasks[blockIdx.x];
+  const DenseFront f = fr[tk.x];
+  const int q = blockIdx.y;
+  const T *X = dense_ptr<T>(B, 2, f.x_off, q), *Y = dense_ptr<T>(B, 3, f.x_off, q);
+  T *F = dense_ptr<T>(B, f.sel, f.off, q);
+  const int c1 = min(tk.y + 16, f.nd);
+  for (int c = tk.y; c < c1; c++)
+    for (int i = threadIdx.x; i < f.nd; i += blockDim.x)
+      F[i + (int64_t)f.nf * c] = i > c ? X[i + (int64_t)f.nd * c] : Y[i + (int64_t)f.nd * c];
 }
 
 // Excitation e's port block: E[i + no*k] = S[O[i] + nfr * O[k]], S the root front (the ports' Schur complement).
@@ -417,82 +606,95 @@ struct RootDev {
   const int32_t *no;     // [nexc]
   int no_max;
   int64_t estride;       // elements per (matrix, excitation) block: no_max^2
-  const int32_t *rpp;    // [matrix][nexc][no_max] pivot permutation of the excitation blocks
+  const int32_t *rpp;    // pivot permutations of the excitation blocks: pp + n, [matrix stride ppqs][nexc][no_max]
 };
 
-// One task = (node or excitation, first row).  Each thread owns one row of the task and up to kMaxRhsThread
-// right-hand sides (rhs j = j0 + rstep * m); the inputs are staged kKC rows at a time in LDS.
-template <typename T, int KIND>
-__global__ __launch_bounds__(256) void k_lu_apply(const int2 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
-                                                  int ty_n, const T *__restrict__ Fall, const T *__restrict__ Eall,
-                                                  const int32_t *__restrict__ ppall, double2 *__restrict__ fvall,
-                                                  double2 *__restrict__ yall, double2 *__restrict__ tall,
-                                                  double2 *__restrict__ xall, double2 *__restrict__ yrall) {
-  __shared__ double2 sin[kKC * kMaxRhsThread * 4];
-  const int2 tk = tasks[blockIdx.x];
-  const int q = blockIdx.y;
-  const int rows_wg = 256 / ty_n;
-  const int tx = threadIdx.x % rows_wg, ty = threadIdx.x / rows_wg;
-  // rhs subset of this task
-  int j0 = 0, rstep = 1;
-  if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
-    j0 = tk.x;
-    rstep = Rt.nexc;
-  }
-  const int nsub = (nrhs - j0 + rstep - 1) / rstep;  // rhs in the subset
-  // this thread's rhs: subset indices ty, ty + ty_n, ...
-  const int my_n = nsub > ty ? (nsub - ty + ty_n - 1) / ty_n : 0;
-  const int64_t nrows_all = D.n * nrhs;
-  double2 *fv = fvall + q * D.fvstride;
-  double2 *y = yall + q * nrows_all, *tt = tall + q * nrows_all, *x = xall + q * nrows_all;
-  int t = -1, e = -1;
-  int64_t nd = 0, nu = 0, nf = 0, dbeg = 0;
-  const T *M = nullptr;
-  int64_t ld = 0, rows = 0, cols = 0;
+// Apply as a split-K product, deterministic: phase 1 (task = node or excitation, first row, first column, slot)
+// writes the partial sums of its rows over one chunk of kKCH columns to its slot; phase 2 (task = node or
+// excitation, first row, chunk count, first slot) adds a row tile's slots in chunk order and applies the output
+// rule.  Each thread owns one row and up to kMaxRhsThread right-hand sides (rhs j = j0 + rstep * m); matrix loads
+// are issued kUnroll at a time with clamped indices, so none is conditional; the inputs are staged kKC rows at a
+// time in LDS.
+constexpr int kKCH = 256;
+constexpr int kUnroll = 8;
+
+template <int KIND>
+__device__ __forceinline__ void apply_geom(const int4 tk, const LuDev &D, const RootDev &Rt, int64_t &rows,
+                                           int64_t &cols, int64_t &ld, int &t, int &e, int64_t &dbeg, int64_t &nd,
+                                           int64_t &nf) {
   if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
     e = tk.x;
     t = Rt.node;
-    nd = D.dbeg[t + 1] - D.dbeg[t];
-    dbeg = D.dbeg[t];
-    rows = cols = Rt.no[e];
-    ld = rows;
-    M = Eall + ((int64_t)q * Rt.nexc + e) * Rt.estride;
-  } else {
-    t = tk.x;
     dbeg = D.dbeg[t];
     nd = D.dbeg[t + 1] - dbeg;
-    nu = D.uoff[t + 1] - D.uoff[t];
-    nf = nd + nu;
+    nf = nd;
+    rows = cols = Rt.no[e];
+    ld = rows;
+  } else {
+    t = tk.x;
+    e = 0;
+    dbeg = D.dbeg[t];
+    nd = D.dbeg[t + 1] - dbeg;
+    nf = nd + (D.uoff[t + 1] - D.uoff[t]);
     ld = nf;
-    const T *F = Fall + q * D.fstride + D.foff[t];
-    if (KIND == K_FWD_L || KIND == K_BWD_U) {
-      M = F;
-      rows = cols = nd;
-    } else if (KIND == K_FWD_L21) {
-      M = F + nd;
-      rows = nu;
+    if (KIND == K_FWD_L21) {
+      rows = nf - nd;
       cols = nd;
-    } else {
-      M = F + nf * nd;
+    } else if (KIND == K_BWD_U12) {
       rows = nd;
-      cols = nu;
+      cols = nf - nd;
+    } else {
+      rows = cols = nd;
     }
   }
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
+                                                       int ty_n, const T *__restrict__ Fall, const T *__restrict__ Eall,
+                                                       const int32_t *__restrict__ ppall, const double2 *__restrict__ fvall,
+                                                       const double2 *__restrict__ yall,
+                                                       const double2 *__restrict__ tall,
+                                                       const double2 *__restrict__ xall,
+                                                       const double2 *__restrict__ yrall, double2 *__restrict__ part,
+                                                       int64_t part_qstride) {
+  __shared__ double2 sin[kKC * kMaxRhsThread * 4];
+  const int4 tk = tasks[blockIdx.x];
+  const int q = blockIdx.y;
+  const int rows_wg = 256 / ty_n;
+  const int tx = threadIdx.x % rows_wg, ty = threadIdx.x / rows_wg;
+  int t, e;
+  int64_t rows, cols, ld, dbeg, nd, nf;
+  apply_geom<KIND>(tk, D, Rt, rows, cols, ld, t, e, dbeg, nd, nf);
+  int j0 = 0, rstep = 1;
+  if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
+    j0 = e;
+    rstep = Rt.nexc;
+  }
+  const int nsub = (nrhs - j0 + rstep - 1) / rstep;
+  const int my_n = nsub > ty ? (nsub - ty + ty_n - 1) / ty_n : 0;
+  const T *M;
+  if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
+    M = Eall + ((int64_t)q * Rt.nexc + e) * Rt.estride;
+  } else {
+    const T *F = Fall + q * D.fstride + D.foff[t];
+    M = KIND == K_FWD_L21 ? F + nd : KIND == K_BWD_U12 ? F + nf * nd : F;
+  }
+  const int64_t nrows_all = D.n * nrhs;
+  const double2 *fv = fvall + q * D.fvstride;
+  const double2 *y = yall + q * nrows_all, *tt = tall + q * nrows_all, *x = xall + q * nrows_all;
+  const int32_t *pp = ppall + q * D.ppqs + dbeg;
+  const int32_t *rpp = (KIND == K_ROOT_L) ? Rt.rpp + q * D.ppqs + (int64_t)e * Rt.no_max : nullptr;
+  const int32_t *O = (KIND == K_ROOT_L || KIND == K_ROOT_U) ? Rt.O + (int64_t)e * Rt.no_max : nullptr;
+  const double2 *yr = yrall + (int64_t)q * Rt.no_max * nrhs;
   const int64_t i = tk.y + tx;
+  const int64_t ic = i < rows ? i : rows - 1;  // clamped row: loads stay inside the block
+  const int64_t c_beg = tk.z, c_end = min((int64_t)tk.z + kKCH, cols);
   double2 acc[kMaxRhsThread];
 #pragma unroll
   for (int m = 0; m < kMaxRhsThread; m++) acc[m] = make_double2(0.0, 0.0);
-  // column range touched by this task's rows
-  int64_t k_lo = 0, k_hi = cols;
-  if (KIND == K_FWD_L || KIND == K_ROOT_L) k_hi = min((int64_t)tk.y + rows_wg, rows);  // k <= i
-  if (KIND == K_BWD_U || KIND == K_ROOT_U) k_lo = tk.y;                                 // k >= i
-  const int32_t *pp = ppall + q * D.n + dbeg;
-  const int32_t *rpp = (KIND == K_ROOT_L) ? Rt.rpp + ((int64_t)q * Rt.nexc + e) * Rt.no_max : nullptr;
-  const int32_t *O = (KIND == K_ROOT_L || KIND == K_ROOT_U) ? Rt.O + (int64_t)e * Rt.no_max : nullptr;
-  const double2 *yr = yrall + (int64_t)q * Rt.no_max * nrhs;
-  for (int64_t kc = k_lo; kc < k_hi; kc += kKC) {
-    const int kn = (int)min((int64_t)kKC, k_hi - kc);
-    // stage inputs in[k][m] for the subset rhs (m < nsub)
+  for (int64_t kc = c_beg; kc < c_end; kc += kKC) {
+    const int kn = (int)min((int64_t)kKC, c_end - kc);
     for (int s = threadIdx.x; s < kn * nsub; s += 256) {
       const int kk = s / nsub, m = s % nsub;
       const int j = j0 + m * rstep;
@@ -513,50 +715,94 @@ __global__ __launch_bounds__(256) void k_lu_apply(const int2 *__restrict__ tasks
       sin[kk * nsub + m] = v;
     }
     __syncthreads();
-    if (i < rows) {
-      for (int kk = 0; kk < kn; kk++) {
-        const int64_t k = kc + kk;
-        double2 a;
+    for (int kk0 = 0; kk0 < kn; kk0 += kUnroll) {
+      double2 a[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int kk = min(kk0 + u, kn - 1);
+        a[u] = d2(M[ic + ld * (kc + kk)]);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int64_t k = kc + kk0 + u;
+        bool use = kk0 + u < kn;
         if (KIND == K_FWD_L || KIND == K_ROOT_L) {
-          if (k > i) break;
-          a = k == i ? make_double2(1.0, 0.0) : d2(M[i + ld * k]);
+          if (k == i) a[u] = make_double2(1.0, 0.0);
+          use = use && k <= i;
         } else if (KIND == K_BWD_U || KIND == K_ROOT_U) {
-          if (k < i) continue;
-          a = d2(M[i + ld * k]);
-        } else {
-          a = d2(M[i + ld * k]);
+          use = use && k >= i;
         }
+        if (!use) a[u] = make_double2(0.0, 0.0);
+        const int kk = min(kk0 + u, kn - 1);
 #pragma unroll
         for (int mm = 0; mm < kMaxRhsThread; mm++) {
           if (mm >= my_n) break;
           const double2 b = sin[kk * nsub + ty + mm * ty_n];
-          acc[mm].x += a.x * b.x - a.y * b.y;
-          acc[mm].y += a.x * b.y + a.y * b.x;
+          acc[mm].x += a[u].x * b.x - a[u].y * b.y;
+          acc[mm].y += a[u].x * b.y + a[u].y * b.x;
         }
       }
     }
     __syncthreads();
   }
   if (i >= rows) return;
+  double2 *pq = part + q * part_qstride + ((int64_t)tk.w * rows_wg + tx) * nrhs;
 #pragma unroll
   for (int mm = 0; mm < kMaxRhsThread; mm++) {
     if (mm >= my_n) break;
-    const int j = j0 + (ty + mm * ty_n) * rstep;
+    pq[j0 + (ty + mm * ty_n) * rstep] = acc[mm];
+  }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
+                                                      int ty_n, double2 *__restrict__ fvall,
+                                                      double2 *__restrict__ yall, double2 *__restrict__ tall,
+                                                      double2 *__restrict__ xall, double2 *__restrict__ yrall,
+                                                      const double2 *__restrict__ part, int64_t part_qstride) {
+  const int4 tk = tasks[blockIdx.x];
+  const int q = blockIdx.y;
+  const int rows_wg = 256 / ty_n;
+  const int tx = threadIdx.x % rows_wg, ty = threadIdx.x / rows_wg;
+  int t, e;
+  int64_t rows, cols, ld, dbeg, nd, nf;
+  apply_geom<KIND>(tk, D, Rt, rows, cols, ld, t, e, dbeg, nd, nf);
+  const int64_t i = tk.y + tx;
+  if (i >= rows) return;
+  int j0 = 0, rstep = 1;
+  if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
+    j0 = e;
+    rstep = Rt.nexc;
+  }
+  const int nsub = (nrhs - j0 + rstep - 1) / rstep;
+  const int64_t nrows_all = D.n * nrhs;
+  double2 *fv = fvall + q * D.fvstride;
+  double2 *y = yall + q * nrows_all, *tt = tall + q * nrows_all, *x = xall + q * nrows_all;
+  const int32_t *O = (KIND == K_ROOT_L || KIND == K_ROOT_U) ? Rt.O + (int64_t)e * Rt.no_max : nullptr;
+  const double2 *pq = part + q * part_qstride;
+  for (int m = ty; m < nsub; m += ty_n) {
+    const int j = j0 + m * rstep;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int c = 0; c < tk.z; c++) {
+      const double2 v = pq[((int64_t)(tk.w + c) * rows_wg + tx) * nrhs + j];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
     if (KIND == K_FWD_L) {
-      y[(dbeg + i) * nrhs + j] = acc[mm];
+      y[(dbeg + i) * nrhs + j] = acc;
     } else if (KIND == K_FWD_L21) {
       double2 &d = fv[(D.fvoff[t] + nd + i) * nrhs + j];
-      d.x -= acc[mm].x;
-      d.y -= acc[mm].y;
+      d.x -= acc.x;
+      d.y -= acc.y;
     } else if (KIND == K_BWD_U12) {
       const double2 yv = y[(dbeg + i) * nrhs + j];
-      tt[(dbeg + i) * nrhs + j] = make_double2(yv.x - acc[mm].x, yv.y - acc[mm].y);
+      tt[(dbeg + i) * nrhs + j] = make_double2(yv.x - acc.x, yv.y - acc.y);
     } else if (KIND == K_BWD_U) {
-      x[(dbeg + i) * nrhs + j] = acc[mm];
+      x[(dbeg + i) * nrhs + j] = acc;
     } else if (KIND == K_ROOT_L) {
-      yrall[(int64_t)q * Rt.no_max * nrhs + i * nrhs + j] = acc[mm];
+      yrall[(int64_t)q * Rt.no_max * nrhs + i * nrhs + j] = acc;
     } else {
-      x[(dbeg + O[i]) * nrhs + j] = acc[mm];
+      x[(dbeg + O[i]) * nrhs + j] = acc;
     }
   }
 }
@@ -573,98 +819,59 @@ inline int grid_for(int64_t n, int cap = 8192) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, cap));
 }
 
-// ---------------------------------------------------------------- dense library dispatch
-template <typename T>
-struct Lib;
-template <>
-struct Lib<double2> {
-  using C = rocblas_double_complex;
-  static rocblas_status getrf(rocblas_handle h, int m, int n, double2 *A, int lda, rocblas_stride sA, int *ip,
-                              rocblas_stride sP, int *info, int bc) {
-    return rocsolver_zgetrf_strided_batched(h, m, n, (C *)A, lda, sA, ip, sP, info, bc);
-  }
-  static rocblas_status trsm_ru(rocblas_handle h, int m, int n, const double2 *A, int lda, rocblas_stride sA,
-                                double2 *B, int ldb, rocblas_stride sB, int bc) {
-    const C one(1.0, 0.0);
-    return rocblas_ztrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                         rocblas_diagonal_non_unit, m, n, &one, (const C *)A, lda, sA, (C *)B, ldb,
-                                         sB, bc);
-  }
-  static rocblas_status gemm_sub(rocblas_handle h, int m, int n, int k, const double2 *A, int lda, rocblas_stride sA,
-                                 const double2 *B, int ldb, rocblas_stride sB, double2 *Cm, int ldc,
-                                 rocblas_stride sC, int bc) {
-    const C mone(-1.0, 0.0), one(1.0, 0.0);
-    return rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &mone,
-                                         (const C *)A, lda, sA, (const C *)B, ldb, sB, &one, (C *)Cm, ldc, sC, bc);
-  }
-  static rocblas_status trtri(rocblas_handle h, rocblas_fill uplo, rocblas_diagonal diag, int n, double2 *A, int lda,
-                              rocblas_stride sA, int *info, int bc) {
-    return rocsolver_ztrtri_strided_batched(h, uplo, diag, n, (C *)A, lda, sA, info, bc);
-  }
-};
-template <>
-struct Lib<float2> {
-  using C = rocblas_float_complex;
-  static rocblas_status getrf(rocblas_handle h, int m, int n, float2 *A, int lda, rocblas_stride sA, int *ip,
-                              rocblas_stride sP, int *info, int bc) {
-    return rocsolver_cgetrf_strided_batched(h, m, n, (C *)A, lda, sA, ip, sP, info, bc);
-  }
-  static rocblas_status trsm_ru(rocblas_handle h, int m, int n, const float2 *A, int lda, rocblas_stride sA, float2 *B,
-                                int ldb, rocblas_stride sB, int bc) {
-    const C one(1.0f, 0.0f);
-    return rocblas_ctrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                         rocblas_diagonal_non_unit, m, n, &one, (const C *)A, lda, sA, (C *)B, ldb,
-                                         sB, bc);
-  }
-  static rocblas_status gemm_sub(rocblas_handle h, int m, int n, int k, const float2 *A, int lda, rocblas_stride sA,
-                                 const float2 *B, int ldb, rocblas_stride sB, float2 *Cm, int ldc, rocblas_stride sC,
-                                 int bc) {
-    const C mone(-1.0f, 0.0f), one(1.0f, 0.0f);
-    return rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &mone,
-                                         (const C *)A, lda, sA, (const C *)B, ldb, sB, &one, (C *)Cm, ldc, sC, bc);
-  }
-  static rocblas_status trtri(rocblas_handle h, rocblas_fill uplo, rocblas_diagonal diag, int n, float2 *A, int lda,
-                              rocblas_stride sA, int *info, int bc) {
-    return rocsolver_ctrtri_strided_batched(h, uplo, diag, n, (C *)A, lda, sA, info, bc);
-  }
-};
-
 }  // namespace
 
 // ---------------------------------------------------------------- host side
 struct LuFactor::Impl {
   hipStream_t st = nullptr;
-  rocblas_handle blas = nullptr;
   // tree on the device
   DevBuf dbeg, uoff, foff, fvoff, uidx, umap, parent, kid_off, kids, perm, asm_pos;
   int64_t fstride = 0, fvtot = 0;
   // schedule
-  std::vector<std::vector<int32_t>> small_nodes, big_nodes, level_nodes;  // per level (root excluded)
-  DevBuf d_small, d_levels;                                             // node lists, concatenated
-  std::vector<int64_t> small_off, level_off;
-  std::vector<std::vector<int2>> ea_tasks;  // per (level * max_children + slot)
+  std::vector<std::vector<int32_t>> level_nodes;  // per level
+  DevBuf d_levels;                                // node lists, concatenated
+  std::vector<int64_t> level_off;
   DevBuf d_ea;
   std::vector<int64_t> ea_off;
   int ea_cb = 16;
-  DevBuf d_bigoff, d_biglen;  // per big node: dbeg, nd (for the pivot conversion)
-  std::vector<int32_t> big_all;
   // root
   int root = -1, nexc = 1, no_max = 0;
   std::vector<int32_t> no;
-  DevBuf d_O, d_no, d_root_off, d_root_len;
+  DevBuf d_O, d_no;
+  // blocked dense LU: fronts (tree nodes, then the port blocks), per-level step lists
+  struct Step {
+    int k0;
+    int64_t panel_off, panel_n, lrows_off, lrows_n, laswp_off, laswp_n, trsm_off, trsm_n, gemm_off, gemm_n,
+        gemm_tiles;
+  };
+  struct DLevel {
+    std::vector<Step> lu, inv;
+    int64_t list_off = 0, list_n = 0;  // fronts of the level
+    int64_t cols_off = 0, cols_n = 0;  // (front, 16-column chunk) tasks of the eye / put kernels
+    int64_t xsize = 0;                 // inverse scratch elements per matrix
+  };
+  std::vector<DenseFront> dfr;
+  std::vector<DLevel> dlev;    // tree levels, then one level of port blocks
+  std::vector<int32_t> h_list;
+  std::vector<int2> h_laswp, h_trsm_tasks, h_lrows, h_cols;
+  std::vector<TrsmJob> h_trsm;
+  std::vector<GemmJob> h_gemm;
+  std::vector<int64_t> h_gprefix;
+  DevBuf d_dfr, d_list, d_laswp, d_trsm_tasks, d_trsm, d_gemm, d_gprefix, d_lrows, d_cols;
+  int64_t xmax = 1, ppqs = 1;
   // numeric storage
-  DevBuf fronts, pp, ipiv, info, zp, E, rpp, ripiv;
+  DevBuf fronts, pp, sw, zp, E, X, Y;
   // apply storage and per-ty_n task lists
   DevBuf b, y, t, x, fv, yr;
   struct ApplyTasks {
-    DevBuf d;
-    std::vector<int64_t> off[6];  // per kind: offsets per level (nlevels + 1)
-    std::vector<int2> all;
+    DevBuf d1, d2;
+    std::vector<int64_t> off1[6], off2[6];  // per kind: task offsets per level (nlevels + 1)
+    std::vector<int4> all1, all2;           // phase-1 (id, r0, c0, slot) and phase-2 (id, r0, chunks, slot0) tasks
+    int64_t max_slots = 1;                  // partial-sum slots of the largest launch
   };
+  DevBuf part;
   std::map<int, ApplyTasks> atasks;  // keyed by ty_n
-  ~Impl() {
-    if (blas) rocblas_destroy_handle(blas);
-  }
+  void schedule_level(const std::vector<int32_t> &ids, DLevel &L);
   LuDev dev(const LuTree &T, int nrhs) const {
     LuDev D;
     D.dbeg = dbeg.as<int64_t>();
@@ -677,6 +884,7 @@ struct LuFactor::Impl {
     D.kid_off = kid_off.as<int32_t>();
     D.kids = kids.as<int32_t>();
     D.n = T.n;
+    D.ppqs = ppqs;
     D.fstride = fstride;
     D.fvstride = fvtot * nrhs;
     return D;
@@ -691,12 +899,162 @@ size_t LuFactor::device_bytes_per_instance() const {
   return (size_t)d_->fstride * (prec_ == 0 ? 16 : 8);
 }
 
+// Step lists of the blocked LU and of the two triangular inverses for the fronts `ids` (indices into dfr).
+void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) {
+  if (ids.empty()) return;
+  int maxnd = 0;
+  L.list_off = (int64_t)h_list.size();
+  L.xsize = 0;
+  for (int id : ids) {
+    maxnd = std::max(maxnd, dfr[id].nd);
+    dfr[id].x_off = L.xsize;
+    L.xsize += (int64_t)dfr[id].nd * dfr[id].nd;
+    h_list.push_back(id);
+  }
+  L.list_n = (int64_t)ids.size();
+  xmax = std::max(xmax, L.xsize);
+  L.cols_off = (int64_t)h_cols.size();
+  for (int id : ids)
+    for (int c0 = 0; c0 < dfr[id].nd; c0 += 16) h_cols.push_back(make_int2(id, c0));
+  L.cols_n = (int64_t)h_cols.size() - L.cols_off;
+  auto gemm_push = [&](const GemmJob &J, int64_t &tiles) {
+    h_gprefix.push_back(tiles);
+    h_gemm.push_back(J);
+    tiles += (int64_t)((J.m + kTile - 1) / kTile) * ((J.n + kTile - 1) / kTile);
+  };
+  auto trsm_push = [&](const TrsmJob &J) {
+    const int jid = (int)h_trsm.size();
+    h_trsm.push_back(J);
+    for (int c0 = 0; c0 < J.ncols; c0 += 256) h_trsm_tasks.push_back(make_int2(jid, c0));
+  };
+  for (int k0 = 0; k0 < maxnd; k0 += kNB) {
+    Step S{};
+    S.k0 = k0;
+    S.panel_off = (int64_t)h_list.size();
+    for (int id : ids)
+      if (dfr[id].nd > k0) h_list.push_back(id);
+    S.panel_n = (int64_t)h_list.size() - S.panel_off;
+    S.lrows_off = (int64_t)h_lrows.size();
+    for (int id : ids)
+      if (dfr[id].nd > k0)
+        for (int r0 = 0; r0 < dfr[id].nf - dfr[id].nd; r0 += 256) h_lrows.push_back(make_int2(id, r0));
+    S.lrows_n = (int64_t)h_lrows.size() - S.lrows_off;
+    S.laswp_off = (int64_t)h_laswp.size();
+    for (int id : ids)
+      if (dfr[id].nd > k0)
+        for (int c0 = 0; c0 < dfr[id].nf; c0 += 256) h_laswp.push_back(make_int2(id, c0));
+    S.laswp_n = (int64_t)h_laswp.size() - S.laswp_off;
+    S.trsm_off = (int64_t)h_trsm_tasks.size();
+    S.gemm_off = (int64_t)h_gemm.size();
+    S.gemm_tiles = 0;
+    for (int id : ids) {
+      const DenseFront &f = dfr[id];
+      if (f.nd <= k0) continue;
+      const int nb = std::min(kNB, f.nd - k0), r = f.nf - k0 - nb;
+      if (r <= 0) continue;
+      TrsmJob T{};
+      T.tsel = T.xsel = f.sel;
+      T.nb = nb;
+      T.ncols = r;
+      T.toff = f.off + k0 + (int64_t)f.nf * k0;
+      T.xoff = f.off + k0 + (int64_t)f.nf * (k0 + nb);
+      T.ldt = T.ldx = f.nf;
+      T.upper = 0;
+      trsm_push(T);
+      GemmJob G{};
+      G.asel = G.bsel = G.csel = f.sel;
+      G.m = G.n = r;
+      G.k = nb;
+      G.lda = G.ldb = G.ldc = f.nf;
+      G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
+      G.boff = T.xoff;
+      G.coff = f.off + (k0 + nb) + (int64_t)f.nf * (k0 + nb);
+      gemm_push(G, S.gemm_tiles);
+    }
+    S.trsm_n = (int64_t)h_trsm_tasks.size() - S.trsm_off;
+    S.gemm_n = (int64_t)h_gemm.size() - S.gemm_off;
+    L.lu.push_back(S);
+  }
+  // inverses: L^-1 block rows in ascending order, U^-1 block rows in descending order, in the same launches
+  const int nsteps = (maxnd + kNB - 1) / kNB;
+  for (int st_ = 0; st_ < nsteps; st_++) {
+    Step S{};
+    S.trsm_off = (int64_t)h_trsm_tasks.size();
+    S.gemm_off = (int64_t)h_gemm.size();
+    S.gemm_tiles = 0;
+    std::vector<GemmJob> gj;
+    for (int id : ids) {
+      const DenseFront &f = dfr[id];
+      const int nblk = (f.nd + kNB - 1) / kNB;
+      if (st_ >= nblk) continue;
+      {  // L: block st_
+        const int k0 = st_ * kNB, nb = std::min(kNB, f.nd - k0);
+        TrsmJob T{};
+        T.tsel = f.sel;
+        T.xsel = 2;
+        T.nb = nb;
+        T.ncols = k0 + nb;
+        T.toff = f.off + k0 + (int64_t)f.nf * k0;
+        T.ldt = f.nf;
+        T.xoff = f.x_off + k0;
+        T.ldx = f.nd;
+        T.upper = 0;
+        trsm_push(T);
+        if (f.nd - k0 - nb > 0) {
+          GemmJob G{};
+          G.asel = f.sel;
+          G.bsel = G.csel = 2;
+          G.m = f.nd - k0 - nb;
+          G.n = k0 + nb;
+          G.k = nb;
+          G.lda = f.nf;
+          G.ldb = G.ldc = f.nd;
+          G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
+          G.boff = f.x_off + k0;
+          G.coff = f.x_off + k0 + nb;
+          gj.push_back(G);
+        }
+      }
+      {  // U: block nblk - 1 - st_
+        const int k0 = (nblk - 1 - st_) * kNB, nb = std::min(kNB, f.nd - k0);
+        TrsmJob T{};
+        T.tsel = f.sel;
+        T.xsel = 3;
+        T.nb = nb;
+        T.ncols = f.nd - k0;
+        T.toff = f.off + k0 + (int64_t)f.nf * k0;
+        T.ldt = f.nf;
+        T.xoff = f.x_off + k0 + (int64_t)f.nd * k0;
+        T.ldx = f.nd;
+        T.upper = 1;
+        trsm_push(T);
+        if (k0 > 0) {
+          GemmJob G{};
+          G.asel = f.sel;
+          G.bsel = G.csel = 3;
+          G.m = k0;
+          G.n = f.nd - k0;
+          G.k = nb;
+          G.lda = f.nf;
+          G.ldb = G.ldc = f.nd;
+          G.aoff = f.off + (int64_t)f.nf * k0;
+          G.boff = f.x_off + k0 + (int64_t)f.nd * k0;
+          G.coff = f.x_off + (int64_t)f.nd * k0;
+          gj.push_back(G);
+        }
+      }
+    }
+    for (auto &G : gj) gemm_push(G, S.gemm_tiles);
+    S.trsm_n = (int64_t)h_trsm_tasks.size() - S.trsm_off;
+    S.gemm_n = (int64_t)h_gemm.size() - S.gemm_off;
+    L.inv.push_back(S);
+  }
+}
+
 void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root_group, hipStream_t st) {
   Impl &I = *d_;
   T_ = &T;
   I.st = st;
-  if (!I.blas) ROCBLAS_CHK(rocblas_create_handle(&I.blas));
-  ROCBLAS_CHK(rocblas_set_stream(I.blas, st));
   const int nn = T.nnodes;
   I.dbeg.upload(T.dbeg, st);
   I.uoff.upload(T.uoff, st);
@@ -726,6 +1084,8 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   // root
   I.root = T.last_node;
   I.nexc = std::max(1, nexc);
+  I.no.assign(I.nexc, 0);
+  I.no_max = 1;
   if (I.root >= 0) {
     const int64_t ndr = T.nd(I.root);
     if ((int64_t)root_group.size() != ndr) throw std::invalid_argument("lu: root_group must cover the port rows");
@@ -735,8 +1095,6 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
       for (int e = 0; e < I.nexc; e++)
         if (root_group[r] != e) O[e].push_back((int32_t)r);
     }
-    I.no.assign(I.nexc, 0);
-    I.no_max = 1;
     for (int e = 0; e < I.nexc; e++) {
       I.no[e] = (int32_t)O[e].size();
       I.no_max = std::max(I.no_max, I.no[e]);
@@ -745,32 +1103,13 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
     for (int e = 0; e < I.nexc; e++) std::copy(O[e].begin(), O[e].end(), Of.begin() + (size_t)e * I.no_max);
     I.d_O.upload(Of, st);
     I.d_no.upload(I.no, st);
-    std::vector<int64_t> roff(I.nexc);
-    for (int e = 0; e < I.nexc; e++) roff[e] = (int64_t)e * I.no_max;
-    I.d_root_off.upload(roff, st);
-    I.d_root_len.upload(I.no, st);
   }
+  I.ppqs = T.n + (int64_t)I.nexc * I.no_max;
   // schedule per level
   const int L = T.nlevels;
-  I.small_nodes.assign(L, {});
-  I.big_nodes.assign(L, {});
   I.level_nodes.assign(L, {});
-  for (int t = 0; t < nn; t++) {
-    I.level_nodes[T.level[t]].push_back(t);
-    if (t == I.root) continue;
-    if (T.nf(t) <= kSmallFront)
-      I.small_nodes[T.level[t]].push_back(t);
-    else
-      I.big_nodes[T.level[t]].push_back(t);
-  }
+  for (int t = 0; t < nn; t++) I.level_nodes[T.level[t]].push_back(t);
   std::vector<int32_t> cat;
-  I.small_off.assign(L + 1, 0);
-  for (int l = 0; l < L; l++) {
-    cat.insert(cat.end(), I.small_nodes[l].begin(), I.small_nodes[l].end());
-    I.small_off[l + 1] = (int64_t)cat.size();
-  }
-  I.d_small.upload(cat, st);
-  cat.clear();
   I.level_off.assign(L + 1, 0);
   for (int l = 0; l < L; l++) {
     cat.insert(cat.end(), I.level_nodes[l].begin(), I.level_nodes[l].end());
@@ -792,28 +1131,126 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
       I.ea_off[(size_t)l * MC + s + 1] = (int64_t)ea.size();
     }
   I.d_ea.upload(ea, st);
-  // big nodes: pivot conversion lists
-  I.big_all.clear();
-  for (int l = 0; l < L; l++) I.big_all.insert(I.big_all.end(), I.big_nodes[l].begin(), I.big_nodes[l].end());
+  // dense fronts: tree nodes (the port root is only gathered), then one block per excitation
+  I.dfr.assign(nn + I.nexc, DenseFront{});
+  for (int t = 0; t < nn; t++) {
+    DenseFront &f = I.dfr[t];
+    f.sel = 0;
+    f.nf = (int)T.nf(t);
+    f.nd = (int)T.nd(t);
+    f.off = T.foff[t];
+    f.pp_off = T.dbeg[t];
+  }
+  const int64_t es = (int64_t)I.no_max * I.no_max;
+  for (int e = 0; e < I.nexc; e++) {
+    DenseFront &f = I.dfr[nn + e];
+    f.sel = 1;
+    f.nf = f.nd = I.root >= 0 ? I.no[e] : 0;
+    f.off = (int64_t)e * es;
+    f.pp_off = T.n + (int64_t)e * I.no_max;
+  }
+  I.h_list.clear();
+  I.h_laswp.clear();
+  I.h_trsm_tasks.clear();
+  I.h_trsm.clear();
+  I.h_gemm.clear();
+  I.h_gprefix.clear();
+  I.h_lrows.clear();
+  I.h_cols.clear();
+  I.dlev.assign(L + 1, Impl::DLevel{});
+  I.xmax = 1;
+  for (int l = 0; l < L; l++) {
+    std::vector<int32_t> ids;
+    for (int t : I.level_nodes[l])
+      if (t != I.root && T.nd(t) > 0) ids.push_back(t);
+    I.schedule_level(ids, I.dlev[l]);
+  }
+  if (I.root >= 0) {
+    std::vector<int32_t> ids;
+    for (int e = 0; e < I.nexc; e++)
+      if (I.no[e] > 0) ids.push_back(nn + e);
+    I.schedule_level(ids, I.dlev[L]);
+  }
+  I.d_dfr.upload(I.dfr, st);
+  I.d_list.upload(I.h_list, st);
+  I.d_laswp.upload(I.h_laswp, st);
+  I.d_trsm_tasks.upload(I.h_trsm_tasks, st);
+  I.d_trsm.upload(I.h_trsm, st);
+  I.d_gemm.upload(I.h_gemm, st);
+  I.d_gprefix.upload(I.h_gprefix, st);
+  I.d_lrows.upload(I.h_lrows, st);
+  I.d_cols.upload(I.h_cols, st);
   I.atasks.clear();
   HIPCHK(hipStreamSynchronize(st));
+}
+
+template <typename T>
+static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int ninst, const DenseBase &B) {
+  hipStream_t st = I.st;
+  if (L.list_n == 0) return;
+  const DenseFront *fr = I.d_dfr.as<DenseFront>();
+  for (const auto &S : L.lu) {
+    hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
+                       I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
+    if (S.lrows_n > 0)
+      hipLaunchKernelGGL((k_dense_lrows<T>), dim3((unsigned)S.lrows_n, ninst), dim3(256), 0, st,
+                         I.d_lrows.as<int2>() + S.lrows_off, fr, B, S.k0);
+    if (S.laswp_n > 0)
+      hipLaunchKernelGGL((k_dense_laswp<T>), dim3((unsigned)S.laswp_n, ninst), dim3(256), 0, st,
+                         I.d_laswp.as<int2>() + S.laswp_off, fr, B, S.k0);
+    if (S.trsm_n > 0)
+      hipLaunchKernelGGL((k_dense_trsm<T>), dim3((unsigned)S.trsm_n, ninst), dim3(256), 0, st,
+                         I.d_trsm_tasks.as<int2>() + S.trsm_off, I.d_trsm.as<TrsmJob>(), B);
+    if (S.gemm_tiles > 0)
+      hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
+                         I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
+                         (int)S.gemm_n, B);
+    LU_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((k_dense_eye<T>), dim3((unsigned)L.cols_n, ninst), dim3(256), 0, st,
+                     I.d_cols.as<int2>() + L.cols_off, fr, B);
+  for (const auto &S : L.inv) {
+    if (S.trsm_n > 0)
+      hipLaunchKernelGGL((k_dense_trsm<T>), dim3((unsigned)S.trsm_n, ninst), dim3(256), 0, st,
+                         I.d_trsm_tasks.as<int2>() + S.trsm_off, I.d_trsm.as<TrsmJob>(), B);
+    if (S.gemm_tiles > 0)
+      hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
+                         I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
+                         (int)S.gemm_n, B);
+    LU_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((k_dense_put_inverses<T>), dim3((unsigned)L.cols_n, ninst), dim3(256), 0, st,
+                     I.d_cols.as<int2>() + L.cols_off, fr, B);
+  LU_LAUNCH_CHECK();
 }
 
 template <typename T>
 static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const double *vals, int64_t vstride,
                         LuFactorStats *stats) {
   hipStream_t st = I.st;
-  const int nn = Tr.nnodes;
   const int64_t fs = I.fstride;
+  const int64_t es = (int64_t)I.no_max * I.no_max;
   I.fronts.alloc((size_t)ninst * fs * sizeof(T));
-  I.pp.alloc((size_t)ninst * std::max<int64_t>(Tr.n, 1) * 4);
-  I.ipiv.alloc((size_t)ninst * std::max<int64_t>(Tr.n, 1) * 4);
-  const int nbig = (int)I.big_all.size();
-  I.info.alloc((size_t)ninst * (2 * nbig + 3 * I.nexc + 1) * 4);
+  I.pp.alloc((size_t)ninst * I.ppqs * 4);
+  I.sw.alloc((size_t)ninst * I.ppqs * 4);
+  I.E.alloc((size_t)ninst * I.nexc * es * sizeof(T));
+  I.X.alloc((size_t)ninst * I.xmax * sizeof(T));
+  I.Y.alloc((size_t)ninst * I.xmax * sizeof(T));
   I.zp.alloc(8);
-  HIPCHK(hipMemsetAsync(I.info.p, 0, (size_t)ninst * (2 * nbig + 3 * I.nexc + 1) * 4, st));
   HIPCHK(hipMemsetAsync(I.zp.p, 0, 8, st));
   T *F = I.fronts.as<T>();
+  DenseBase B{};
+  B.p[0] = F;
+  B.qs[0] = fs;
+  B.p[1] = I.E.p;
+  B.qs[1] = (int64_t)I.nexc * es;
+  B.p[2] = I.X.p;
+  B.qs[2] = I.xmax;
+  B.p[3] = I.Y.p;
+  B.qs[3] = I.xmax;
+  B.pp = I.pp.as<int32_t>();
+  B.sw = I.sw.as<int32_t>();
+  B.ppqs = I.ppqs;
   LuDev D = I.dev(Tr, 1);
   hipLaunchKernelGGL((k_lu_zero<T>), dim3(grid_for((int64_t)ninst * fs, 65536)), dim3(256), 0, st,
                      (int64_t)ninst * fs, F);
@@ -821,8 +1258,6 @@ static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const do
                      (int64_t)Tr.asm_pos.size(), I.asm_pos.as<int64_t>(), (const double2 *)vals, vstride, F, fs);
   LU_LAUNCH_CHECK();
   const int MC = std::max(1, Tr.max_children);
-  int info_k = 0;
-  int *info = I.info.as<int>();
   for (int l = 0; l < Tr.nlevels; l++) {
     for (int s = 0; s < MC; s++) {
       const int64_t a = I.ea_off[(size_t)l * MC + s], b = I.ea_off[(size_t)l * MC + s + 1];
@@ -831,91 +1266,28 @@ static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const do
                            I.d_ea.as<int2>() + a, D, F, I.ea_cb);
     }
     LU_LAUNCH_CHECK();
-    const int64_t s0 = I.small_off[l], s1 = I.small_off[l + 1];
-    if (s1 > s0)
-      hipLaunchKernelGGL((k_lu_front_small<T>), dim3((unsigned)(s1 - s0), ninst), dim3(256), 0, st,
-                         I.d_small.as<int32_t>() + s0, D, F, I.pp.as<int32_t>(),
-                         I.zp.as<unsigned long long>());
-    LU_LAUNCH_CHECK();
-    for (int t : I.big_nodes[l]) {
-      const int nd = (int)Tr.nd(t), nu = (int)Tr.nu(t), nf = nd + nu;
-      T *Ft = F + Tr.foff[t];
-      int *ip = I.ipiv.as<int>() + Tr.dbeg[t];
-      ROCBLAS_CHK(Lib<T>::getrf(I.blas, nd, nf, Ft, nf, fs, ip, Tr.n, info + (size_t)info_k * ninst, ninst));
-      info_k++;
-      if (nu > 0) {
-        ROCBLAS_CHK(Lib<T>::trsm_ru(I.blas, nu, nd, Ft, nf, fs, Ft + nd, nf, fs, ninst));
-        ROCBLAS_CHK(Lib<T>::gemm_sub(I.blas, nu, nu, nd, Ft + nd, nf, fs, Ft + (int64_t)nf * nd, nf, fs,
-                                     Ft + nd + (int64_t)nf * nd, nf, fs, ninst));
-      }
-      ROCBLAS_CHK(Lib<T>::trtri(I.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, nd, Ft, nf, fs,
-                                info + (size_t)info_k * ninst, ninst));
-      info_k++;
-      ROCBLAS_CHK(Lib<T>::trtri(I.blas, rocblas_fill_lower, rocblas_diagonal_unit, nd, Ft, nf, fs,
-                                info + (size_t)info_k * ninst, ninst));
-      // the unit-lower trtri reports no singularity; its slot is reused
-    }
-    if (!I.big_nodes[l].empty()) {
-      std::vector<int64_t> off;
-      std::vector<int32_t> len;
-      int mx = 0;
-      for (int t : I.big_nodes[l]) {
-        off.push_back(Tr.dbeg[t]);
-        len.push_back((int32_t)Tr.nd(t));
-        mx = std::max(mx, (int)Tr.nd(t));
-      }
-      DevBuf doff, dlen;
-      doff.upload(off, st);
-      dlen.upload(len, st);
-      hipLaunchKernelGGL(k_lu_ipiv_to_perm, dim3((unsigned)off.size(), ninst), dim3(256), (size_t)mx * 4, st,
-                         doff.as<int64_t>(), dlen.as<int32_t>(), I.ipiv.as<int32_t>(), I.pp.as<int32_t>(), Tr.n);
-      LU_LAUNCH_CHECK();
-      HIPCHK(hipStreamSynchronize(st));  // doff/dlen go out of scope
-    }
+    dense_level<T>(I, I.dlev[l], ninst, B);
   }
   if (I.root >= 0) {
     const int r = I.root;
     const int nfr = (int)Tr.nd(r);
-    const int64_t es = (int64_t)I.no_max * I.no_max;
-    I.E.alloc((size_t)ninst * I.nexc * es * sizeof(T));
-    I.rpp.alloc((size_t)ninst * I.nexc * I.no_max * 4);
-    I.ripiv.alloc((size_t)ninst * I.nexc * I.no_max * 4);
     T *E = I.E.as<T>();
     for (int e = 0; e < I.nexc; e++) {
       const int no = I.no[e];
       if (no == 0) continue;
-      T *Ee = E + (int64_t)e * es;
       hipLaunchKernelGGL((k_lu_root_gather<T>), dim3(grid_for((int64_t)no * no, 16384), ninst), dim3(256), 0, st,
-                         F + Tr.foff[r], fs, nfr, I.d_O.as<int32_t>() + (int64_t)e * I.no_max, no, Ee,
+                         F + Tr.foff[r], fs, nfr, I.d_O.as<int32_t>() + (int64_t)e * I.no_max, no, E + (int64_t)e * es,
                          (int64_t)I.nexc * es);
       LU_LAUNCH_CHECK();
-      int *rip = I.ripiv.as<int>() + (int64_t)e * I.no_max;
-      ROCBLAS_CHK(Lib<T>::getrf(I.blas, no, no, Ee, no, (int64_t)I.nexc * es, rip, (int64_t)I.nexc * I.no_max,
-                                info + (size_t)info_k * ninst, ninst));
-      info_k++;
-      ROCBLAS_CHK(Lib<T>::trtri(I.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, no, Ee, no,
-                                (int64_t)I.nexc * es, info + (size_t)info_k * ninst, ninst));
-      info_k++;
-      ROCBLAS_CHK(Lib<T>::trtri(I.blas, rocblas_fill_lower, rocblas_diagonal_unit, no, Ee, no, (int64_t)I.nexc * es,
-                                info + (size_t)info_k * ninst, ninst));
     }
-    // pivot permutations of all excitation blocks (matrix stride nexc * no_max)
-    hipLaunchKernelGGL(k_lu_ipiv_to_perm, dim3(I.nexc, ninst), dim3(256), (size_t)I.no_max * 4, st,
-                       I.d_root_off.as<int64_t>(), I.d_root_len.as<int32_t>(), I.ripiv.as<int32_t>(),
-                       I.rpp.as<int32_t>(), (int64_t)I.nexc * I.no_max);
-    LU_LAUNCH_CHECK();
+    dense_level<T>(I, I.dlev[Tr.nlevels], ninst, B);
   }
-  std::vector<int> hinfo((size_t)info_k * ninst);
-  if (!hinfo.empty()) HIPCHK(hipMemcpyAsync(hinfo.data(), info, hinfo.size() * 4, hipMemcpyDeviceToHost, st));
   unsigned long long zp = 0;
   HIPCHK(hipMemcpyAsync(&zp, I.zp.p, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (stats) {
     stats->zero_pivots = (int64_t)zp;
-    stats->lib_info = 0;
-    for (int v : hinfo) stats->lib_info += v != 0;
   }
-  (void)nn;
 }
 
 void LuFactor::factor(int prec, int ninst, const double *vals, int64_t vals_stride, LuFactorStats *stats) {
@@ -948,32 +1320,55 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   // task lists for this row tiling
   auto it = I.atasks.find(ty_n);
   if (it == I.atasks.end()) {
-    LuFactor::Impl::ApplyTasks A;
+    it = I.atasks.emplace(std::piecewise_construct, std::forward_as_tuple(ty_n), std::forward_as_tuple()).first;
+    auto &A = it->second;
     const int L = Tr.nlevels;
-    for (int kind = 0; kind < 6; kind++) A.off[kind].assign(L + 1, 0);
-    // layout: kind-major, then level
+    for (int kind = 0; kind < 6; kind++) {
+      A.off1[kind].assign(L + 1, 0);
+      A.off2[kind].assign(L + 1, 0);
+    }
+    // rows x cols of each kind's block and its triangle: 1 = lower with unit diagonal (k <= i), 2 = upper (k >= i)
+    auto add_block = [&](int id, int64_t rows, int64_t cols, int tri, int &slot) {
+      for (int64_t r0 = 0; r0 < rows; r0 += rows_wg) {
+        const int64_t r1 = std::min(rows, r0 + rows_wg);
+        int64_t k0 = 0, k1 = cols;
+        if (tri == 1) k1 = std::min(cols, r1);
+        if (tri == 2) k0 = r0;
+        const int slot0 = slot;
+        int nch = 0;
+        for (int64_t c0 = k0; c0 < k1; c0 += kKCH, nch++) A.all1.push_back(make_int4(id, (int)r0, (int)c0, slot++));
+        A.all2.push_back(make_int4(id, (int)r0, nch, slot0));
+      }
+    };
     for (int kind = 0; kind < 6; kind++) {
       for (int l = 0; l < L; l++) {
-        A.off[kind][l] = (int64_t)A.all.size();
+        A.off1[kind][l] = (int64_t)A.all1.size();
+        A.off2[kind][l] = (int64_t)A.all2.size();
+        int slot = 0;
         if (kind <= K_BWD_U) {
           for (int t : I.level_nodes[l]) {
             if (t == I.root) continue;
-            const int64_t rows = kind == K_FWD_L21 ? Tr.nu(t) : Tr.nd(t);
-            for (int64_t r0 = 0; r0 < rows; r0 += rows_wg) A.all.push_back(make_int2(t, (int)r0));
+            const int64_t nd = Tr.nd(t), nu = Tr.nu(t);
+            if (kind == K_FWD_L) add_block(t, nd, nd, 1, slot);
+            if (kind == K_FWD_L21) add_block(t, nu, nd, 0, slot);
+            if (kind == K_BWD_U12) add_block(t, nd, nu, 0, slot);
+            if (kind == K_BWD_U) add_block(t, nd, nd, 2, slot);
           }
         } else if (I.root >= 0 && Tr.level[I.root] == l) {
-          for (int e = 0; e < I.nexc; e++)
-            for (int64_t r0 = 0; r0 < I.no[e]; r0 += rows_wg) A.all.push_back(make_int2(e, (int)r0));
+          for (int e = 0; e < I.nexc; e++) add_block(e, I.no[e], I.no[e], kind == K_ROOT_L ? 1 : 2, slot);
         }
+        A.max_slots = std::max<int64_t>(A.max_slots, slot);
       }
-      A.off[kind][L] = (int64_t)A.all.size();
+      A.off1[kind][L] = (int64_t)A.all1.size();
+      A.off2[kind][L] = (int64_t)A.all2.size();
     }
-    it = I.atasks.emplace(std::piecewise_construct, std::forward_as_tuple(ty_n), std::forward_as_tuple()).first;
-    it->second.all = std::move(A.all);
-    for (int kind = 0; kind < 6; kind++) it->second.off[kind] = std::move(A.off[kind]);
-    it->second.d.upload(it->second.all, st);
+    A.d1.upload(A.all1, st);
+    A.d2.upload(A.all2, st);
   }
   auto &A = it->second;
+  const int64_t part_qstride = A.max_slots * rows_wg * nrhs;
+  I.part.alloc((size_t)ninst * part_qstride * 16);
+  double2 *part = I.part.as<double2>();
   LuDev D = I.dev(Tr, nrhs);
   RootDev Rt{};
   Rt.node = I.root;
@@ -982,7 +1377,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   Rt.no = I.d_no.as<int32_t>();
   Rt.no_max = std::max(1, I.no_max);
   Rt.estride = (int64_t)Rt.no_max * Rt.no_max;
-  Rt.rpp = I.rpp.as<int32_t>();
+  Rt.rpp = I.pp.as<int32_t>() + Tr.n;
   const T *F = I.fronts.as<T>();
   const T *E = I.E.as<T>();
   const int32_t *pp = I.pp.as<int32_t>();
@@ -991,39 +1386,24 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   hipLaunchKernelGGL((k_lu_gather_rhs<X>), dim3(grid_for(n * nrhs), ninst), dim3(256), 0, st, n, nrhs,
                      I.perm.as<int32_t>(), v, ldv, b);
   LU_LAUNCH_CHECK();
-  auto launch = [&](int kind, int l) {
-    const int64_t a0 = A.off[kind][l], a1 = A.off[kind][l + 1];
-    if (a1 <= a0) return;
-    const int2 *tk = A.d.as<int2>() + a0;
-    const dim3 g((unsigned)(a1 - a0), ninst);
-    switch (kind) {
-      case K_FWD_L:
-        hipLaunchKernelGGL((k_lu_apply<T, K_FWD_L>), g, dim3(256), 0, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv, y, tt,
-                           x, yr);
-        break;
-      case K_FWD_L21:
-        hipLaunchKernelGGL((k_lu_apply<T, K_FWD_L21>), g, dim3(256), 0, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv, y,
-                           tt, x, yr);
-        break;
-      case K_BWD_U12:
-        hipLaunchKernelGGL((k_lu_apply<T, K_BWD_U12>), g, dim3(256), 0, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv, y,
-                           tt, x, yr);
-        break;
-      case K_BWD_U:
-        hipLaunchKernelGGL((k_lu_apply<T, K_BWD_U>), g, dim3(256), 0, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv, y, tt,
-                           x, yr);
-        break;
-      case K_ROOT_L:
-        hipLaunchKernelGGL((k_lu_apply<T, K_ROOT_L>), g, dim3(256), 0, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv, y,
-                           tt, x, yr);
-        break;
-      default:
-        hipLaunchKernelGGL((k_lu_apply<T, K_ROOT_U>), g, dim3(256), 0, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv, y,
-                           tt, x, yr);
-        break;
-    }
+  auto launch = [&](auto kind_c, int l) {
+    constexpr int KIND = decltype(kind_c)::value;
+    const int64_t a0 = A.off1[KIND][l], a1 = A.off1[KIND][l + 1];
+    const int64_t b0 = A.off2[KIND][l], b1 = A.off2[KIND][l + 1];
+    if (a1 > a0)
+      hipLaunchKernelGGL((k_lu_apply_part<T, KIND>), dim3((unsigned)(a1 - a0), ninst), dim3(256), 0, st,
+                         A.d1.as<int4>() + a0, D, Rt, nrhs, ty_n, F, E, pp, fv, y, tt, x, yr, part, part_qstride);
+    if (b1 > b0)
+      hipLaunchKernelGGL((k_lu_apply_sum<KIND>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
+                         A.d2.as<int4>() + b0, D, Rt, nrhs, ty_n, fv, y, tt, x, yr, part, part_qstride);
     LU_LAUNCH_CHECK();
   };
+  using KL = std::integral_constant<int, K_FWD_L>;
+  using KL21 = std::integral_constant<int, K_FWD_L21>;
+  using KU12 = std::integral_constant<int, K_BWD_U12>;
+  using KU = std::integral_constant<int, K_BWD_U>;
+  using KRL = std::integral_constant<int, K_ROOT_L>;
+  using KRU = std::integral_constant<int, K_ROOT_U>;
   const int L = Tr.nlevels;
   for (int l = 0; l < L; l++) {
     const int64_t a0 = I.level_off[l], a1 = I.level_off[l + 1];
@@ -1031,9 +1411,9 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
       hipLaunchKernelGGL(k_lu_fwd_gather, dim3((unsigned)(a1 - a0), ninst), dim3(256), 0, st,
                          I.d_levels.as<int32_t>() + a0, D, nrhs, b, fv);
     LU_LAUNCH_CHECK();
-    launch(K_FWD_L, l);
-    launch(K_FWD_L21, l);
-    launch(K_ROOT_L, l);
+    launch(KL(), l);
+    launch(KL21(), l);
+    launch(KRL(), l);
   }
   if (I.root >= 0) {
     hipLaunchKernelGGL(k_lu_zero_rows, dim3(grid_for(Tr.nd(I.root) * nrhs), ninst), dim3(256), 0, st,
@@ -1041,9 +1421,9 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
     LU_LAUNCH_CHECK();
   }
   for (int l = L - 1; l >= 0; l--) {
-    launch(K_ROOT_U, l);
-    launch(K_BWD_U12, l);
-    launch(K_BWD_U, l);
+    launch(KRU(), l);
+    launch(KU12(), l);
+    launch(KU(), l);
   }
   const int64_t ndof = Tr.ndof;
   hipLaunchKernelGGL((k_lu_zero<X>), dim3(grid_for(ndof * S, 65536)), dim3(256), 0, st, ndof * S, z);

@@ -127,6 +127,38 @@ def c4():
         s.close()
 
 
+def c4lu(share=1, fp64_points=(0, 21, 42, 63)):
+    """Config 4 with the sparse-LU preconditioner: the 64-frequency x 2-port microstrip sweep (0.48 M ND2 DOF,
+    1-20 GHz) in mixed precision (FP32 factors, FP32 GMRES inside FP64 refinement), batched as memory allows,
+    `share` adjacent frequencies per factor; then plain FP64 (FP64 factor of each frequency, one at a time) at
+    `fp64_points`."""
+    import os
+    os.environ.setdefault("OPEM_LU_STATS", "1")
+    w = workloads.make("microstrip-c4")
+    t = time.perf_counter()
+    s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+    _emit({"workload": "microstrip-c4", "setup_s": round(time.perf_counter() - t, 2), "ndof": s.ndof, "nnz": s.nnz})
+    f = w["freqs"]
+    try:
+        for rep in range(2):
+            t = time.perf_counter()
+            r = s.sweep(f, precision="mixed", restart=30, tol=1e-10, maxit=600, precond="lu", precond_share=share,
+                        orth_passes=1)
+            dt = time.perf_counter() - t
+            done = int(np.all(r["converged"].astype(bool), axis=1).sum())
+            _emit({"rep": rep, "share": share, "s": round(dt, 2), "points_converged": done,
+                   "freq_points_per_s": round(done / dt, 3), "iters_mean": float(r["iters"].mean()),
+                   "iters_max": int(r["iters"].max()), "seconds": r["seconds"]})
+        for i in fp64_points:
+            t = time.perf_counter()
+            r64 = s.sweep(f[i:i + 1], precision="fp64", batch=1, restart=30, tol=1e-10, maxit=600, precond="lu")
+            _emit({"fp64_f_GHz": round(f[i] / 1e9, 2), "s": round(time.perf_counter() - t, 2),
+                   "iters": r64["iters"].tolist(), "converged": r64["converged"].tolist(),
+                   "max_abs_dS": float(np.abs(r["S"][i] - r64["S"][0]).max())})
+    finally:
+        s.close()
+
+
 def c5():
     tot_t, tot_pts, tot_f = 0.0, 0, 0
     for i, w in enumerate(workloads.c5_instances()):
@@ -194,4 +226,7 @@ def kernels(restart=300, ns=3):
 
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c3"
-    {"c2": c2, "c3": c3, "c4": c4, "c5": c5, "kernels": kernels}[which]()
+    if which == "c4lu":
+        c4lu(share=int(sys.argv[2]) if len(sys.argv) > 2 else 1)
+    else:
+        {"c2": c2, "c3": c3, "c4": c4, "c5": c5, "kernels": kernels}[which]()
