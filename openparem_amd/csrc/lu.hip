@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) void k_lu_extend_add(const int2 *__restrict__ 
 // block row of U, trailing update.
 constexpr int kPanelThreads = 1024;
 constexpr int kTile = 64;
+constexpr int kOuter = 128;  // pivot columns per trailing update
 
 struct DenseFront {
   int sel;          // where the front lives (0 fronts, 1 port blocks)
@@ -411,9 +412,14 @@ __global__ __launch_bounds__(256) void k_dense_trsm(const int2 *__restrict__ tas
     if (r < nb) X[r] = x[r];
 }
 
-// Tiled C -= A B, A m x k, B k x n (k <= kNB), C m x n, for a list of jobs; blocks are numbered job after job
-// (prefix of tile counts), each block one kTile x kTile tile of its job.  256 threads as 16 x 16, 4 x 4
-// complex values per thread; the A and B slices are staged in LDS.
+// Tiled C -= A B, A m x k, B k x n (any k, taken in slices of kNB), C m x n, for a list of jobs; blocks are numbered job after job
+// (prefix of tile counts), each block one kTile x kTile tile of its job, on MFMA (v_mfma_f32_16x16x4f32 or
+// v_mfma_f64_16x16x4f64).  Wave w owns columns 16w .. 16w+15 of the tile and all four 16-row blocks.  The MFMA
+// computes the transposed product (first operand B^T, second A^T), so D[i][j] = C[row j][col i]: consecutive
+// lanes own consecutive rows of the column-major C and its stores are 128-byte runs.  A complex product is four
+// real ones: Re += Br.Ar - Bi.Ai, Im += Bi.Ar + Br.Ai.  Operand layouts of the 16x16x4 MFMA: first operand
+// P[i][k] in lane i + 16 k, second Q[k][j] in lane j + 16 k; D[i][j] in register i % 4 of lane j + 16 (i / 4)
+// (f32) or register i / 4 of lane j + 16 (i % 4) (f64).
 struct GemmJob {
   int asel, bsel, csel, pad;
   int m, n, k, lda;
@@ -421,9 +427,33 @@ struct GemmJob {
   int64_t aoff, boff, coff;
 };
 
+typedef float lu_f4 __attribute__((ext_vector_type(4)));
+typedef double lu_d4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<float2> {
+  using V = lu_f4;
+  static __device__ __forceinline__ V step(float p, float q, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(p, q, c, 0, 0, 0);
+  }
+  static constexpr bool f64 = false;
+};
+template <>
+struct MfmaAcc<double2> {
+  using V = lu_d4;
+  static __device__ __forceinline__ V step(double p, double q, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(p, q, c, 0, 0, 0);
+  }
+  static constexpr bool f64 = true;
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_gemm(const GemmJob *__restrict__ jobs, const int64_t *__restrict__ prefix,
                                                     int njobs, DenseBase B) {
+  using M = MfmaAcc<T>;
+  using V = typename M::V;
   __shared__ T As[kNB][kTile];
   __shared__ T Bs[kNB][kTile];
   const int64_t b = blockIdx.x;
@@ -444,57 +474,59 @@ __global__ __launch_bounds__(256) void k_dense_gemm(const GemmJob *__restrict__ 
   const T *Bm = dense_ptr<T>(B, J.bsel, J.boff, q);
   T *C = dense_ptr<T>(B, J.csel, J.coff, q);
   const int tid = threadIdx.x;
-  const int K = J.k;
-  for (int e = tid; e < kTile * kNB; e += 256) {
-    const int rr = e % kTile, s = e / kTile;
-    T v;
-    v.x = 0;
-    v.y = 0;
-    if (s < K && r0 + rr < J.m) v = A[(r0 + rr) + (int64_t)J.lda * s];
-    As[s][rr] = v;
-  }
-  for (int e = tid; e < kTile * kNB; e += 256) {
-    const int s = e % kNB, cc = e / kNB;
-    T v;
-    v.x = 0;
-    v.y = 0;
-    if (s < K && c0 + cc < J.n) v = Bm[s + (int64_t)J.ldb * (c0 + cc)];
-    Bs[s][cc] = v;
-  }
-  __syncthreads();
-  const int tr = tid & 15, tc = tid >> 4;
-  T acc[4][4];
+  const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  V cr[4], ci[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-      acc[i][jj].x = 0;
-      acc[i][jj].y = 0;
+  for (int rb = 0; rb < 4; rb++) {
+    cr[rb] = V{0, 0, 0, 0};
+    ci[rb] = V{0, 0, 0, 0};
+  }
+  for (int kc = 0; kc < J.k; kc += kNB) {  // K in slices of kNB columns; C is read and written once
+    const int K = min(kNB, J.k - kc);
+    if (kc > 0) __syncthreads();
+    for (int e = tid; e < kTile * kNB; e += 256) {
+      const int rr = e % kTile, s = e / kTile;
+      T v;
+      v.x = 0;
+      v.y = 0;
+      if (s < K && r0 + rr < J.m) v = A[(r0 + rr) + (int64_t)J.lda * (kc + s)];
+      As[s][rr] = v;
     }
-  for (int s = 0; s < K; s++) {
-    T a[4], bv[4];
+    for (int e = tid; e < kTile * kNB; e += 256) {
+      const int s = e % kNB, cc = e / kNB;
+      T v;
+      v.x = 0;
+      v.y = 0;
+      if (s < K && c0 + cc < J.n) v = Bm[kc + s + (int64_t)J.ldb * (c0 + cc)];
+      Bs[s][cc] = v;
+    }
+    __syncthreads();
+    const int nk = (K + 3) >> 2;
+    for (int kb = 0; kb < nk; kb++) {
+      const T bv = Bs[4 * kb + lk][16 * w + li];
 #pragma unroll
-    for (int i = 0; i < 4; i++) a[i] = As[s][tr + 16 * i];
-#pragma unroll
-    for (int jj = 0; jj < 4; jj++) bv[jj] = Bs[s][tc + 16 * jj];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) {
-        acc[i][jj].x += a[i].x * bv[jj].x - a[i].y * bv[jj].y;
-        acc[i][jj].y += a[i].x * bv[jj].y + a[i].y * bv[jj].x;
+      for (int rb = 0; rb < 4; rb++) {
+        const T av = As[4 * kb + lk][16 * rb + li];
+        cr[rb] = M::step(bv.x, av.x, cr[rb]);
+        cr[rb] = M::step(bv.y, -av.y, cr[rb]);
+        ci[rb] = M::step(bv.y, av.x, ci[rb]);
+        ci[rb] = M::step(bv.x, av.y, ci[rb]);
       }
+    }
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int r = r0 + tr + 16 * i;
-    if (r >= J.m) continue;
+  for (int rb = 0; rb < 4; rb++) {
+    const int row = r0 + 16 * rb + li;
 #pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-      const int c = c0 + tc + 16 * jj;
-      if (c >= J.n) continue;
-      T *p = C + r + (int64_t)J.ldc * c;
-      *p = csub_t(*p, acc[i][jj]);
+    for (int r = 0; r < 4; r++) {
+      const int col = c0 + 16 * w + (M::f64 ? 4 * r + lk : 4 * lk + r);
+      if (row < J.m && col < J.n) {
+        T *p = C + row + (int64_t)J.ldc * col;
+        T v = *p;
+        v.x -= cr[rb][r];
+        v.y -= ci[rb][r];
+        *p = v;
+      }
     }
   }
 }
@@ -615,8 +647,8 @@ struct RootDev {
 // rule.  Each thread owns one row and up to kMaxRhsThread right-hand sides (rhs j = j0 + rstep * m); matrix loads
 // are issued kUnroll at a time with clamped indices, so none is conditional; the inputs are staged kKC rows at a
 // time in LDS.
-constexpr int kKCH = 256;
-constexpr int kUnroll = 8;
+constexpr int kKCH = 64;
+constexpr int kUnroll = 16;
 
 template <int KIND>
 __device__ __forceinline__ void apply_geom(const int4 tk, const LuDev &D, const RootDev &Rt, int64_t &rows,
@@ -649,7 +681,7 @@ __device__ __forceinline__ void apply_geom(const int4 tk, const LuDev &D, const 
   }
 }
 
-template <typename T, int KIND>
+template <typename T, int KIND, int MAXR>
 __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
                                                        int ty_n, const T *__restrict__ Fall, const T *__restrict__ Eall,
                                                        const int32_t *__restrict__ ppall, const double2 *__restrict__ fvall,
@@ -658,7 +690,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
                                                        const double2 *__restrict__ xall,
                                                        const double2 *__restrict__ yrall, double2 *__restrict__ part,
                                                        int64_t part_qstride) {
-  __shared__ double2 sin[kKC * kMaxRhsThread * 4];
+  extern __shared__ double2 sin[];  // kKC x nsub staged inputs
   const int4 tk = tasks[blockIdx.x];
   const int q = blockIdx.y;
   const int rows_wg = 256 / ty_n;
@@ -690,9 +722,9 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
   const int64_t i = tk.y + tx;
   const int64_t ic = i < rows ? i : rows - 1;  // clamped row: loads stay inside the block
   const int64_t c_beg = tk.z, c_end = min((int64_t)tk.z + kKCH, cols);
-  double2 acc[kMaxRhsThread];
+  double2 acc[MAXR];
 #pragma unroll
-  for (int m = 0; m < kMaxRhsThread; m++) acc[m] = make_double2(0.0, 0.0);
+  for (int m = 0; m < MAXR; m++) acc[m] = make_double2(0.0, 0.0);
   for (int64_t kc = c_beg; kc < c_end; kc += kKC) {
     const int kn = (int)min((int64_t)kKC, c_end - kc);
     for (int s = threadIdx.x; s < kn * nsub; s += 256) {
@@ -715,31 +747,30 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
       sin[kk * nsub + m] = v;
     }
     __syncthreads();
+    const T *Mc = M + ic + ld * kc;
     for (int kk0 = 0; kk0 < kn; kk0 += kUnroll) {
-      double2 a[kUnroll];
+      T araw[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const int kk = min(kk0 + u, kn - 1);
-        a[u] = d2(M[ic + ld * (kc + kk)]);
-      }
+      for (int u = 0; u < kUnroll; u++) araw[u] = Mc[ld * min(kk0 + u, kn - 1)];
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         const int64_t k = kc + kk0 + u;
+        double2 a = d2(araw[u]);
         bool use = kk0 + u < kn;
         if (KIND == K_FWD_L || KIND == K_ROOT_L) {
-          if (k == i) a[u] = make_double2(1.0, 0.0);
+          if (k == i) a = make_double2(1.0, 0.0);
           use = use && k <= i;
         } else if (KIND == K_BWD_U || KIND == K_ROOT_U) {
           use = use && k >= i;
         }
-        if (!use) a[u] = make_double2(0.0, 0.0);
+        if (!use) a = make_double2(0.0, 0.0);
         const int kk = min(kk0 + u, kn - 1);
 #pragma unroll
-        for (int mm = 0; mm < kMaxRhsThread; mm++) {
+        for (int mm = 0; mm < MAXR; mm++) {
           if (mm >= my_n) break;
           const double2 b = sin[kk * nsub + ty + mm * ty_n];
-          acc[mm].x += a[u].x * b.x - a[u].y * b.y;
-          acc[mm].y += a[u].x * b.y + a[u].y * b.x;
+          acc[mm].x += a.x * b.x - a.y * b.y;
+          acc[mm].y += a.x * b.y + a.y * b.x;
         }
       }
     }
@@ -748,7 +779,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
   if (i >= rows) return;
   double2 *pq = part + q * part_qstride + ((int64_t)tk.w * rows_wg + tx) * nrhs;
 #pragma unroll
-  for (int mm = 0; mm < kMaxRhsThread; mm++) {
+  for (int mm = 0; mm < MAXR; mm++) {
     if (mm >= my_n) break;
     pq[j0 + (ty + mm * ty_n) * rstep] = acc[mm];
   }
@@ -927,53 +958,124 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     h_trsm.push_back(J);
     for (int c0 = 0; c0 < J.ncols; c0 += 256) h_trsm_tasks.push_back(make_int2(jid, c0));
   };
-  for (int k0 = 0; k0 < maxnd; k0 += kNB) {
+  // blocked LU in outer blocks of kOuter pivot columns: inner steps of kNB columns update only the rest of their
+  // outer block; then the outer block's rows of U over the trailing columns (by inner blocks) and one trailing
+  // update of depth kOuter, so the trailing matrix is read and written once per outer block
+  auto empty_step = [&](int k0) {
     Step S{};
     S.k0 = k0;
     S.panel_off = (int64_t)h_list.size();
-    for (int id : ids)
-      if (dfr[id].nd > k0) h_list.push_back(id);
-    S.panel_n = (int64_t)h_list.size() - S.panel_off;
     S.lrows_off = (int64_t)h_lrows.size();
-    for (int id : ids)
-      if (dfr[id].nd > k0)
-        for (int r0 = 0; r0 < dfr[id].nf - dfr[id].nd; r0 += 256) h_lrows.push_back(make_int2(id, r0));
-    S.lrows_n = (int64_t)h_lrows.size() - S.lrows_off;
     S.laswp_off = (int64_t)h_laswp.size();
-    for (int id : ids)
-      if (dfr[id].nd > k0)
-        for (int c0 = 0; c0 < dfr[id].nf; c0 += 256) h_laswp.push_back(make_int2(id, c0));
-    S.laswp_n = (int64_t)h_laswp.size() - S.laswp_off;
     S.trsm_off = (int64_t)h_trsm_tasks.size();
     S.gemm_off = (int64_t)h_gemm.size();
-    S.gemm_tiles = 0;
-    for (int id : ids) {
-      const DenseFront &f = dfr[id];
-      if (f.nd <= k0) continue;
-      const int nb = std::min(kNB, f.nd - k0), r = f.nf - k0 - nb;
-      if (r <= 0) continue;
-      TrsmJob T{};
-      T.tsel = T.xsel = f.sel;
-      T.nb = nb;
-      T.ncols = r;
-      T.toff = f.off + k0 + (int64_t)f.nf * k0;
-      T.xoff = f.off + k0 + (int64_t)f.nf * (k0 + nb);
-      T.ldt = T.ldx = f.nf;
-      T.upper = 0;
-      trsm_push(T);
-      GemmJob G{};
-      G.asel = G.bsel = G.csel = f.sel;
-      G.m = G.n = r;
-      G.k = nb;
-      G.lda = G.ldb = G.ldc = f.nf;
-      G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
-      G.boff = T.xoff;
-      G.coff = f.off + (k0 + nb) + (int64_t)f.nf * (k0 + nb);
-      gemm_push(G, S.gemm_tiles);
-    }
+    return S;
+  };
+  auto close_step = [&](Step &S) {
+    S.panel_n = (int64_t)h_list.size() - S.panel_off;
+    S.lrows_n = (int64_t)h_lrows.size() - S.lrows_off;
+    S.laswp_n = (int64_t)h_laswp.size() - S.laswp_off;
     S.trsm_n = (int64_t)h_trsm_tasks.size() - S.trsm_off;
     S.gemm_n = (int64_t)h_gemm.size() - S.gemm_off;
     L.lu.push_back(S);
+  };
+  for (int ob0 = 0; ob0 < maxnd; ob0 += kOuter) {
+    for (int k0 = ob0; k0 < ob0 + kOuter && k0 < maxnd; k0 += kNB) {
+      Step S = empty_step(k0);
+      for (int id : ids)
+        if (dfr[id].nd > k0) h_list.push_back(id);
+      S.lrows_off = (int64_t)h_lrows.size();
+      for (int id : ids)
+        if (dfr[id].nd > k0)
+          for (int r0 = 0; r0 < dfr[id].nf - dfr[id].nd; r0 += 256) h_lrows.push_back(make_int2(id, r0));
+      S.laswp_off = (int64_t)h_laswp.size();
+      for (int id : ids)
+        if (dfr[id].nd > k0)
+          for (int c0 = 0; c0 < dfr[id].nf; c0 += 256) h_laswp.push_back(make_int2(id, c0));
+      S.trsm_off = (int64_t)h_trsm_tasks.size();
+      S.gemm_off = (int64_t)h_gemm.size();
+      S.gemm_tiles = 0;
+      for (int id : ids) {
+        const DenseFront &f = dfr[id];
+        if (f.nd <= k0) continue;
+        const int nb = std::min(kNB, f.nd - k0), ob1 = std::min(ob0 + kOuter, f.nd), r = ob1 - k0 - nb;
+        if (r <= 0) continue;
+        TrsmJob T{};
+        T.tsel = T.xsel = f.sel;
+        T.nb = nb;
+        T.ncols = r;
+        T.toff = f.off + k0 + (int64_t)f.nf * k0;
+        T.xoff = f.off + k0 + (int64_t)f.nf * (k0 + nb);
+        T.ldt = T.ldx = f.nf;
+        T.upper = 0;
+        trsm_push(T);
+        GemmJob G{};
+        G.asel = G.bsel = G.csel = f.sel;
+        G.m = f.nf - k0 - nb;
+        G.n = r;
+        G.k = nb;
+        G.lda = G.ldb = G.ldc = f.nf;
+        G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
+        G.boff = T.xoff;
+        G.coff = f.off + (k0 + nb) + (int64_t)f.nf * (k0 + nb);
+        gemm_push(G, S.gemm_tiles);
+      }
+      close_step(S);
+    }
+    // rows ob0 .. ob1 of U over the trailing columns ob1 .. nf: block forward substitution by inner blocks
+    for (int k0 = ob0; k0 < ob0 + kOuter && k0 < maxnd; k0 += kNB) {
+      Step S = empty_step(k0);
+      S.gemm_tiles = 0;
+      for (int id : ids) {
+        const DenseFront &f = dfr[id];
+        if (f.nd <= k0) continue;
+        const int nb = std::min(kNB, f.nd - k0), ob1 = std::min(ob0 + kOuter, f.nd), tc = f.nf - ob1;
+        if (tc <= 0) continue;
+        TrsmJob T{};
+        T.tsel = T.xsel = f.sel;
+        T.nb = nb;
+        T.ncols = tc;
+        T.toff = f.off + k0 + (int64_t)f.nf * k0;
+        T.xoff = f.off + k0 + (int64_t)f.nf * ob1;
+        T.ldt = T.ldx = f.nf;
+        T.upper = 0;
+        trsm_push(T);
+        if (ob1 - k0 - nb > 0) {
+          GemmJob G{};
+          G.asel = G.bsel = G.csel = f.sel;
+          G.m = ob1 - k0 - nb;
+          G.n = tc;
+          G.k = nb;
+          G.lda = G.ldb = G.ldc = f.nf;
+          G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
+          G.boff = T.xoff;
+          G.coff = f.off + (k0 + nb) + (int64_t)f.nf * ob1;
+          gemm_push(G, S.gemm_tiles);
+        }
+      }
+      close_step(S);
+    }
+    // trailing update of depth ob1 - ob0
+    {
+      Step S = empty_step(ob0);
+      S.gemm_tiles = 0;
+      for (int id : ids) {
+        const DenseFront &f = dfr[id];
+        if (f.nd <= ob0) continue;
+        const int ob1 = std::min(ob0 + kOuter, f.nd), mrem = f.nf - ob1;
+        if (mrem <= 0) continue;
+        GemmJob G{};
+        G.asel = G.bsel = G.csel = f.sel;
+        G.m = G.n = mrem;
+        G.k = ob1 - ob0;
+        G.lda = G.ldb = G.ldc = f.nf;
+        G.aoff = f.off + ob1 + (int64_t)f.nf * ob0;
+        G.boff = f.off + ob0 + (int64_t)f.nf * ob1;
+        G.coff = f.off + ob1 + (int64_t)f.nf * ob1;
+        gemm_push(G, S.gemm_tiles);
+      }
+      close_step(S);
+    }
   }
   // inverses: L^-1 block rows in ascending order, U^-1 block rows in descending order, in the same launches
   const int nsteps = (maxnd + kNB - 1) / kNB;
@@ -1190,8 +1292,9 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
   if (L.list_n == 0) return;
   const DenseFront *fr = I.d_dfr.as<DenseFront>();
   for (const auto &S : L.lu) {
-    hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
-                       I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
+    if (S.panel_n > 0)
+      hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
+                         I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
     if (S.lrows_n > 0)
       hipLaunchKernelGGL((k_dense_lrows<T>), dim3((unsigned)S.lrows_n, ninst), dim3(256), 0, st,
                          I.d_lrows.as<int2>() + S.lrows_off, fr, B, S.k0);
@@ -1308,7 +1411,9 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   if (nrhs > 4 * kMaxRhsThread) throw std::invalid_argument("lu: at most 64 right-hand sides per matrix");
   hipStream_t st = I.st;
   const int64_t n = Tr.n, S = (int64_t)ninst * nrhs;
-  const int ty_n = (nrhs + kMaxRhsThread - 1) / kMaxRhsThread;
+  // right-hand sides per thread: 2, 4 or 16 (the template instance), rows per workgroup 256 / ty_n
+  const int maxr = nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : kMaxRhsThread;
+  const int ty_n = (nrhs + maxr - 1) / maxr;
   const int rows_wg = 256 / ty_n;
   const size_t vec = (size_t)ninst * std::max<int64_t>(n, 1) * nrhs * 16;
   I.b.alloc(vec);
@@ -1390,9 +1495,20 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
     constexpr int KIND = decltype(kind_c)::value;
     const int64_t a0 = A.off1[KIND][l], a1 = A.off1[KIND][l + 1];
     const int64_t b0 = A.off2[KIND][l], b1 = A.off2[KIND][l + 1];
-    if (a1 > a0)
-      hipLaunchKernelGGL((k_lu_apply_part<T, KIND>), dim3((unsigned)(a1 - a0), ninst), dim3(256), 0, st,
-                         A.d1.as<int4>() + a0, D, Rt, nrhs, ty_n, F, E, pp, fv, y, tt, x, yr, part, part_qstride);
+    const size_t lds = (size_t)kKC * nrhs * sizeof(double2);
+    if (a1 > a0) {
+      const dim3 g((unsigned)(a1 - a0), ninst);
+      const int4 *tk = A.d1.as<int4>() + a0;
+      if (maxr == 2)
+        hipLaunchKernelGGL((k_lu_apply_part<T, KIND, 2>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv,
+                           y, tt, x, yr, part, part_qstride);
+      else if (maxr == 4)
+        hipLaunchKernelGGL((k_lu_apply_part<T, KIND, 4>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv,
+                           y, tt, x, yr, part, part_qstride);
+      else
+        hipLaunchKernelGGL((k_lu_apply_part<T, KIND, kMaxRhsThread>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F,
+                           E, pp, fv, y, tt, x, yr, part, part_qstride);
+    }
     if (b1 > b0)
       hipLaunchKernelGGL((k_lu_apply_sum<KIND>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
                          A.d2.as<int4>() + b0, D, Rt, nrhs, ty_n, fv, y, tt, x, yr, part, part_qstride);
