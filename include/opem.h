@@ -86,6 +86,16 @@ int opem_sweep_apply(opem_ctx *ctx, int prec, int nfreq, const double *freqs, co
 int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind,
               const double *vals, const double *x, double *y);
 
+/* Allocation-free variant of opem_spmv (same arguments, same kernel, identical y).  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The caller's rowptr, colind, vals, x and y are staged in a 64 MiB
+ * device array that the library holds on each GPU it runs on until the process ends; no hipMalloc, hipFree or
+ * allocating library routine is called.  Problems whose staged bytes exceed 64 MiB (FP64: 4 (n+1) + 20 nnz +
+ * 32 n bytes, each piece rounded up to 256; FP32 adds 8 nnz + 16 n) are rejected with -2.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two contexts, or two host threads) race. */
+int opem_spmv_noalloc(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind,
+                      const double *vals, const double *x, double *y);
+
 /* Element matrices of ne independent tets (the assembly kernel alone):
  * A_e = K_e - k0^2 er (1 - j tand) M_e, written as {Re A_e (nd*nd row-major), Im A_e} per element.
  * verts: ne*12 (4 vertices per tet, element-local order as given), er/tand: ne.

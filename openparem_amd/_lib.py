@@ -13,7 +13,7 @@ SYMBOLS = [
     "opem_create", "opem_destroy", "opem_last_error", "opem_mesh_set", "opem_space_build", "opem_space_get",
     "opem_base_matrix", "opem_material_set", "opem_boundary_set", "opem_port_points", "opem_system_matrix",
     "opem_spmv", "opem_element_matrices", "opem_sweep", "opem_calculate_s", "opem_spmv_bench",
-    "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench",
+    "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench", "opem_spmv_noalloc",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -65,6 +65,7 @@ def lib():
         L.opem_port_points.argtypes = [P, C.c_int, i64p, f64p]
         L.opem_system_matrix.argtypes = [P, C.c_double, f64p, f64p]
         L.opem_spmv.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
+        L.opem_spmv_noalloc.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
         L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]

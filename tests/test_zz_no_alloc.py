@@ -22,7 +22,6 @@ SCRATCH_BYTES = 64 << 20
 
 def _noalloc(dev, A, x, precision):
     L = lib()
-    L.opem_spmv_noalloc.argtypes = L.opem_spmv.argtypes
     rp = np.ascontiguousarray(A.indptr, np.int32)
     ci = np.ascontiguousarray(A.indices, np.int32)
     v = np.ascontiguousarray(A.data, np.complex128)
