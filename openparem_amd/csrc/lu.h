@@ -50,7 +50,9 @@ class LuFactor {
   int prec() const { return prec_; }
   int ninst() const { return ninst_; }
   const LuTree *tree() const { return T_; }
-  size_t device_bytes_per_instance() const;
+  // Device bytes one factored matrix needs (fronts, port blocks, inverse scratch, pivots, and the apply's vectors
+  // and partial sums for nrhs right-hand sides) at precision prec; valid after setup.
+  size_t device_bytes_per_instance(int prec, int nrhs) const;
 
   struct Impl;  // device-side state (lu.hip)
 

@@ -925,9 +925,22 @@ struct LuFactor::Impl {
 LuFactor::LuFactor() : d_(new Impl) {}
 LuFactor::~LuFactor() { delete d_; }
 
-size_t LuFactor::device_bytes_per_instance() const {
+size_t LuFactor::device_bytes_per_instance(int prec, int nrhs) const {
   if (!T_) return 0;
-  return (size_t)d_->fstride * (prec_ == 0 ? 16 : 8);
+  const Impl &I = *d_;
+  const LuTree &T = *T_;
+  const double v = prec == 0 ? 16.0 : 8.0;
+  double b = (double)I.fstride * v + (double)I.nexc * I.no_max * I.no_max * v + 2.0 * I.xmax * v + 2.0 * I.ppqs * 4;
+  // apply: b, y, t, x, front vectors, root vectors, partial sums (slots of the largest launch, 64-row tiles bound)
+  b += (4.0 * T.n + (double)I.fvtot + I.no_max) * nrhs * 16;
+  double slots = 0;
+  for (int l = 0; l < T.nlevels; l++) {
+    double s2 = 0;
+    for (int t : I.level_nodes[l]) s2 += (double)(T.nf(t) / 64 + 1) * (T.nf(t) / kKCH + 1);
+    slots = std::max(slots, s2);
+  }
+  b += slots * 256 * nrhs * 16;
+  return (size_t)b;
 }
 
 // Step lists of the blocked LU and of the two triangular inverses for the fronts `ids` (indices into dfr).

@@ -1638,13 +1638,17 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     const double vec = in->precision == OPEM_PREC_MIXED ? 8.0 : 16.0;
     const bool patch = c.precond == 0 && c.have_patches;
     double per_sys = (double)(m + 4 + c.arnoldi_step) * n * vec + 10.0 * n * 16 + (double)s.nnz * 16 / R;
-    if (use_lu)  // fronts of one factor per frequency group, apply vectors
-      per_sys += ((double)c.lu_tree.foff[c.lu_tree.nnodes] * vec + 6.0 * n * 16 * R) /
-                     (R * (double)std::max(1, in->precond_share)) + 6.0 * n * 16;
-    else
+    if (use_lu) {  // one factor per group of precond_share frequencies, plus each frequency's FP64 values
+      const int g = std::max(1, in->precond_share);
+      per_sys += (double)c.lu.device_bytes_per_instance(in->precision == OPEM_PREC_MIXED ? 1 : 0, g * R) / (g * R) +
+                 (double)s.nnz * 16 / (g * R);
+    } else
       per_sys += patch ? (double)c.patches.idx.size() * 16 : (double)s.nd * n * 16;
     const double fit = std::floor(0.75 * (double)mtotal / (per_sys * R));
     if (fit < batch) batch = (int)std::max(1.0, fit);
+    // an LU batch holds whole groups of precond_share frequencies
+    const int g = std::max(1, in->precond_share);
+    if (use_lu && g > 1 && batch > g) batch -= batch % g;
   }
   const int passes = in->orth_passes == 1 ? 1 : 2;
   const double tol = in->tol > 0 ? in->tol : 1e-10;
@@ -1672,7 +1676,15 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     }
   } else {
     for (int f = 0; f < nf; f++) ford.push_back(f);
-    for (int f0 = 0; f0 < nf; f0 += batch) batches.push_back({f0, std::min(batch, nf - f0)});
+    // with one LU factor per group of g frequencies, a last batch that is not a multiple of g is split into the
+    // whole groups and the remainder, so no batch needs more factors than the memory estimate allowed for
+    const int g = use_lu ? std::max(1, in->precond_share) : 1;
+    for (int f0 = 0; f0 < nf;) {
+      int cnt = std::min(batch, nf - f0);
+      if (g > 1 && cnt > g && cnt % g) cnt -= cnt % g;
+      batches.push_back({f0, cnt});
+      f0 += cnt;
+    }
   }
   DevBuf xstash;  // warm start: solutions of the solved frequencies, [frequency][excitation][ndof]
   int64_t lu_stats_zero = 0;
