@@ -204,6 +204,16 @@ typedef struct {
  * BoundaryDatabase::calculateS (port.cpp:7499-7606). */
 int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out);
 
+/* Kernel timings of the sparse-LU preconditioner (precond 2) on this context, accumulated since the last reset:
+ * the factorization's trailing updates, panels and the apply's matrix products, each timed with HIP events on the
+ * context stream while the environment variable OPEM_LU_TIMING is set (counts only otherwise).  out[12]:
+ *   [0] trailing-update seconds  [1] their real flops (8 per complex multiply-add)  [2] launches
+ *   [3] panel seconds  [4] launches  [5] apply matrix-product seconds  [6] matrix bytes they read  [7] launches
+ *   [8] factorization wall seconds  [9] matrices factored  [10] matrix-applies  [11] 1 when event timing is on.
+ * reset != 0 clears the counters after reading.  Counterpart: the PCFactor/MUMPS timing a PETSc -log_view
+ * reports for the PCCHOLESKY/PCLU of solveComplexLinearSystem.c:133-168. */
+int opem_lu_stats(opem_ctx *ctx, int reset, double *out);
+
 /* S from splits (BoundaryDatabase::calculateS, port.cpp:7499-7606, with Zsolve.c LUP inverse):
  * Cp/Cm complex [n drive][n], V/Z complex [n]; S complex [n][n]. Host-only, no context. */
 int opem_calculate_s(int n, const double *Cp, const double *Cm, const double *V, const double *Z,

@@ -14,6 +14,7 @@ SYMBOLS = [
     "opem_base_matrix", "opem_material_set", "opem_boundary_set", "opem_port_points", "opem_system_matrix",
     "opem_spmv", "opem_element_matrices", "opem_sweep", "opem_calculate_s", "opem_spmv_bench",
     "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench", "opem_spmv_noalloc",
+    "opem_lu_stats",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -66,6 +67,7 @@ def lib():
         L.opem_system_matrix.argtypes = [P, C.c_double, f64p, f64p]
         L.opem_spmv.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
         L.opem_spmv_noalloc.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
+        L.opem_lu_stats.argtypes = [P, C.c_int, f64p]
         L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]

@@ -47,6 +47,12 @@ class LuFactor {
   template <typename X>
   void solve(int nrhs, const X *v, int64_t ldv, X *z);
 
+  // Accumulated kernel timings (HIP events, only while OPEM_LU_TIMING is set in the environment) and counts:
+  // out[0..2] trailing-update seconds, real flops, launches; out[3..4] panel seconds, launches; out[5..7] apply
+  // matrix-product seconds, matrix bytes read, launches; out[8] factorization wall seconds; out[9] matrices
+  // factored; out[10] matrix-applies; out[11] 1 when event timing is on.  reset clears them afterwards.
+  void kernel_stats(double *out, bool reset);
+
   int prec() const { return prec_; }
   int ninst() const { return ninst_; }
   const LuTree *tree() const { return T_; }

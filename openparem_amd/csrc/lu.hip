@@ -874,6 +874,7 @@ struct LuFactor::Impl {
     int k0;
     int64_t panel_off, panel_n, lrows_off, lrows_n, laswp_off, laswp_n, trsm_off, trsm_n, gemm_off, gemm_n,
         gemm_tiles;
+    double gemm_flops;  // real flops of the step's trailing updates, per matrix (8 per complex multiply-add)
   };
   struct DLevel {
     std::vector<Step> lu, inv;
@@ -898,11 +899,60 @@ struct LuFactor::Impl {
     DevBuf d1, d2;
     std::vector<int64_t> off1[6], off2[6];  // per kind: task offsets per level (nlevels + 1)
     std::vector<int4> all1, all2;           // phase-1 (id, r0, c0, slot) and phase-2 (id, r0, chunks, slot0) tasks
+    std::vector<double> elems[6];           // per kind and level: matrix entries one apply must read, per matrix
     int64_t max_slots = 1;                  // partial-sum slots of the largest launch
   };
   DevBuf part;
   std::map<int, ApplyTasks> atasks;  // keyed by ty_n
+  // kernel timing (OPEM_LU_TIMING set): HIP events on the stream around each trailing update, panel and apply
+  // launch, drained when the stream is synchronised; class 0 trailing updates (work = flops), 1 panels,
+  // 2 apply matrix products (work = matrix bytes read)
+  bool timing = getenv("OPEM_LU_TIMING") != nullptr;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Pending {
+    int cls;
+    hipEvent_t a, b;
+    double work;
+  };
+  std::vector<Pending> pending;
+  double tsec[3] = {0, 0, 0}, twork[3] = {0, 0, 0}, tcount[3] = {0, 0, 0};
+  double factor_wall = 0, factors = 0, solves = 0;
+  hipEvent_t ev() {
+    if (evused == evpool.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    return evpool[evused++];
+  }
+  template <class F>
+  void timed(int cls, double work, F launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    hipEvent_t a = ev(), b = ev();
+    HIPCHK(hipEventRecord(a, st));
+    launch();
+    HIPCHK(hipEventRecord(b, st));
+    pending.push_back({cls, a, b, work});
+  }
+  void drain() {  // the stream must be synchronised
+    for (auto &p : pending) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+      tsec[p.cls] += ms * 1e-3;
+      twork[p.cls] += p.work;
+      tcount[p.cls] += 1;
+    }
+    pending.clear();
+    evused = 0;
+  }
   void schedule_level(const std::vector<int32_t> &ids, DLevel &L);
+  ~Impl() {
+    for (auto e : evpool) (void)hipEventDestroy(e);
+  }
   LuDev dev(const LuTree &T, int nrhs) const {
     LuDev D;
     D.dbeg = dbeg.as<int64_t>();
@@ -961,10 +1011,12 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
   for (int id : ids)
     for (int c0 = 0; c0 < dfr[id].nd; c0 += 16) h_cols.push_back(make_int2(id, c0));
   L.cols_n = (int64_t)h_cols.size() - L.cols_off;
+  double *cur_flops = nullptr;  // the step being filled
   auto gemm_push = [&](const GemmJob &J, int64_t &tiles) {
     h_gprefix.push_back(tiles);
     h_gemm.push_back(J);
     tiles += (int64_t)((J.m + kTile - 1) / kTile) * ((J.n + kTile - 1) / kTile);
+    if (cur_flops) *cur_flops += 8.0 * J.m * J.n * J.k;
   };
   auto trsm_push = [&](const TrsmJob &J) {
     const int jid = (int)h_trsm.size();
@@ -1008,6 +1060,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
       S.trsm_off = (int64_t)h_trsm_tasks.size();
       S.gemm_off = (int64_t)h_gemm.size();
       S.gemm_tiles = 0;
+      S.gemm_flops = 0;
+      cur_flops = &S.gemm_flops;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
         if (f.nd <= k0) continue;
@@ -1039,6 +1093,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     for (int k0 = ob0; k0 < ob0 + kOuter && k0 < maxnd; k0 += kNB) {
       Step S = empty_step(k0);
       S.gemm_tiles = 0;
+      S.gemm_flops = 0;
+      cur_flops = &S.gemm_flops;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
         if (f.nd <= k0) continue;
@@ -1072,6 +1128,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     {
       Step S = empty_step(ob0);
       S.gemm_tiles = 0;
+      S.gemm_flops = 0;
+      cur_flops = &S.gemm_flops;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
         if (f.nd <= ob0) continue;
@@ -1097,6 +1155,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     S.trsm_off = (int64_t)h_trsm_tasks.size();
     S.gemm_off = (int64_t)h_gemm.size();
     S.gemm_tiles = 0;
+    S.gemm_flops = 0;
+    cur_flops = &S.gemm_flops;
     std::vector<GemmJob> gj;
     for (int id : ids) {
       const DenseFront &f = dfr[id];
@@ -1306,8 +1366,10 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
   const DenseFront *fr = I.d_dfr.as<DenseFront>();
   for (const auto &S : L.lu) {
     if (S.panel_n > 0)
-      hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
-                         I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
+      I.timed(1, 0.0, [&] {
+        hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
+                           I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
+      });
     if (S.lrows_n > 0)
       hipLaunchKernelGGL((k_dense_lrows<T>), dim3((unsigned)S.lrows_n, ninst), dim3(256), 0, st,
                          I.d_lrows.as<int2>() + S.lrows_off, fr, B, S.k0);
@@ -1318,9 +1380,11 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
       hipLaunchKernelGGL((k_dense_trsm<T>), dim3((unsigned)S.trsm_n, ninst), dim3(256), 0, st,
                          I.d_trsm_tasks.as<int2>() + S.trsm_off, I.d_trsm.as<TrsmJob>(), B);
     if (S.gemm_tiles > 0)
-      hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
-                         I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
-                         (int)S.gemm_n, B);
+      I.timed(0, S.gemm_flops * ninst, [&] {
+        hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
+                           I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
+                           (int)S.gemm_n, B);
+      });
     LU_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL((k_dense_eye<T>), dim3((unsigned)L.cols_n, ninst), dim3(256), 0, st,
@@ -1330,9 +1394,11 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
       hipLaunchKernelGGL((k_dense_trsm<T>), dim3((unsigned)S.trsm_n, ninst), dim3(256), 0, st,
                          I.d_trsm_tasks.as<int2>() + S.trsm_off, I.d_trsm.as<TrsmJob>(), B);
     if (S.gemm_tiles > 0)
-      hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
-                         I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
-                         (int)S.gemm_n, B);
+      I.timed(0, S.gemm_flops * ninst, [&] {
+        hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
+                           I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
+                           (int)S.gemm_n, B);
+      });
     LU_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL((k_dense_put_inverses<T>), dim3((unsigned)L.cols_n, ninst), dim3(256), 0, st,
@@ -1416,7 +1482,11 @@ void LuFactor::factor(int prec, int ninst, const double *vals, int64_t vals_stri
     factor_impl<double2>(*d_, *T_, ninst, vals, vals_stride, stats);
   else
     factor_impl<float2>(*d_, *T_, ninst, vals, vals_stride, stats);
-  if (stats) stats->seconds_factor = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) stats->seconds_factor = dt;
+  d_->factor_wall += dt;
+  d_->factors += ninst;
+  if (d_->timing) d_->drain();  // factor_impl ends with a stream synchronisation
 }
 
 template <typename T, typename X>
@@ -1446,7 +1516,10 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
       A.off2[kind].assign(L + 1, 0);
     }
     // rows x cols of each kind's block and its triangle: 1 = lower with unit diagonal (k <= i), 2 = upper (k >= i)
+    for (int kind = 0; kind < 6; kind++) A.elems[kind].assign(L, 0.0);
+    double *cur_elems = nullptr;
     auto add_block = [&](int id, int64_t rows, int64_t cols, int tri, int &slot) {
+      *cur_elems += tri == 0 ? (double)rows * cols : tri == 1 ? 0.5 * rows * (rows - 1) : 0.5 * rows * (rows + 1);
       for (int64_t r0 = 0; r0 < rows; r0 += rows_wg) {
         const int64_t r1 = std::min(rows, r0 + rows_wg);
         int64_t k0 = 0, k1 = cols;
@@ -1462,6 +1535,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
       for (int l = 0; l < L; l++) {
         A.off1[kind][l] = (int64_t)A.all1.size();
         A.off2[kind][l] = (int64_t)A.all2.size();
+        cur_elems = &A.elems[kind][l];
         int slot = 0;
         if (kind <= K_BWD_U) {
           for (int t : I.level_nodes[l]) {
@@ -1509,7 +1583,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
     const int64_t a0 = A.off1[KIND][l], a1 = A.off1[KIND][l + 1];
     const int64_t b0 = A.off2[KIND][l], b1 = A.off2[KIND][l + 1];
     const size_t lds = (size_t)kKC * nrhs * sizeof(double2);
-    if (a1 > a0) {
+    if (a1 > a0) I.timed(2, A.elems[KIND][l] * sizeof(T) * ninst, [&] {
       const dim3 g((unsigned)(a1 - a0), ninst);
       const int4 *tk = A.d1.as<int4>() + a0;
       if (maxr == 2)
@@ -1521,7 +1595,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
       else
         hipLaunchKernelGGL((k_lu_apply_part<T, KIND, kMaxRhsThread>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F,
                            E, pp, fv, y, tt, x, yr, part, part_qstride);
-    }
+    });
     if (b1 > b0)
       hipLaunchKernelGGL((k_lu_apply_sum<KIND>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
                          A.d2.as<int4>() + b0, D, Rt, nrhs, ty_n, fv, y, tt, x, yr, part, part_qstride);
@@ -1568,7 +1642,25 @@ void LuFactor::solve(int nrhs, const X *v, int64_t ldv, X *z) {
     solve_impl<double2, X>(*d_, *T_, ninst_, nrhs, v, ldv, z);
   else
     solve_impl<float2, X>(*d_, *T_, ninst_, nrhs, v, ldv, z);
+  d_->solves += ninst_;
 }
+void LuFactor::kernel_stats(double *out, bool reset) {
+  Impl &I = *d_;
+  if (I.timing) {
+    HIPCHK(hipStreamSynchronize(I.st));
+    I.drain();
+  }
+  const double v[12] = {I.tsec[0], I.twork[0], I.tcount[0], I.tsec[1], I.tcount[1], I.tsec[2], I.twork[2], I.tcount[2],
+                        I.factor_wall, I.factors, I.solves, I.timing ? 1.0 : 0.0};
+  if (out)
+    for (int k = 0; k < 12; k++) out[k] = v[k];
+  if (!T_) return;
+  if (reset) {
+    for (int c = 0; c < 3; c++) I.tsec[c] = I.twork[c] = I.tcount[c] = 0;
+    I.factor_wall = I.factors = I.solves = 0;
+  }
+}
+
 template void LuFactor::solve<double2>(int, const double2 *, int64_t, double2 *);
 template void LuFactor::solve<float2>(int, const float2 *, int64_t, float2 *);
 

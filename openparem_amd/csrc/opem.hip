@@ -2382,6 +2382,19 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
   });
 }
 
+int opem_lu_stats(opem_ctx *ctx, int reset, double *out) {
+  return guarded(ctx, [&] {
+    if (!out && !reset) throw std::invalid_argument("bad arguments");
+    if (!ctx->have_lu_tree) {  // nothing factored on this context yet: every counter is zero
+      if (out)
+        for (int k = 0; k < 12; k++) out[k] = 0.0;
+      if (out) out[11] = getenv("OPEM_LU_TIMING") ? 1.0 : 0.0;
+      return;
+    }
+    ctx->lu.kernel_stats(out, reset != 0);
+  });
+}
+
 int opem_element_matrices(opem_ctx *ctx, int order, int prec, int64_t ne, const double *verts, const double *er,
                           const double *tand, double k0, int nrep, double *out, double *avg_seconds) {
   return guarded(ctx, [&] {

@@ -192,6 +192,17 @@ class Solver:
                                  post=sout.seconds_post))
 
 
+LU_STAT_KEYS = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",
+                "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies", "event_timing")
+
+
+def lu_stats(solver, reset=False):
+    """opem_lu_stats: sparse-LU kernel timings accumulated on the solver's context (dict)."""
+    out = np.zeros(12, np.float64)
+    check(lib().opem_lu_stats(solver.ctx, int(bool(reset)), ptr(out, C.c_double)), solver.ctx, "opem_lu_stats")
+    return dict(zip(LU_STAT_KEYS, out.tolist()))
+
+
 def spmv(ctx_solver, rowptr, colind, vals, x, precision="fp64"):
     """y = A x on the device (complex CSR parity helper)."""
     n = rowptr.size - 1
