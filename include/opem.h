@@ -86,6 +86,47 @@ int opem_sweep_apply(opem_ctx *ctx, int prec, int nfreq, const double *freqs, co
 int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind,
               const double *vals, const double *x, double *y);
 
+/* Options of a caller-matrix Krylov solve (zero fields take the default). */
+typedef struct {
+  int precision;   /* OPEM_PREC_FP64: FP64 factor and GMRES; OPEM_PREC_MIXED: FP32 factor and inner GMRES cycles   */
+                   /*    inside FP64 iterative refinement                                                           */
+  int restart;     /* GMRES(m), m <= 400 (0 -> 30)                                                                  */
+  int maxit;       /* iteration limit (0 -> 10000, PETSc's default)                                                 */
+  double tol;      /* true relative residual ||b - A x|| / ||b|| (0 -> 1e-10)                                       */
+  int first_rung;  /* preconditioner rung to start the retry ladder at: 0 sparse LU, 1 point Jacobi, 2 none        */
+} opem_solve_opts;
+
+/* solveComplexLinearSystem (src/OpenParEM3D/solveComplexLinearSystem.c:100-209) on a caller's complex CSR matrix
+ * (e.g. the AIJ that hypre_ParCSRMatrixToMat builds, csr.c:171-223, after eliminatePEC): b = -A x, b[PortDof] =
+ * x[PortDof], the PortDof rows and columns eliminated (MatZeroRowsColumns, diagonal 1), then GMRES preconditioned
+ * by a sparse LU of the eliminated matrix (nested dissection of its graph, multifrontal factor on the GPU: the
+ * reference's PCCHOLESKY/PCLU), with the reference's retry ladder: when a rung breaks down (non-finite residual)
+ * the next one is tried: point Jacobi (the reference's PCBJACOBI rung), then no preconditioner; a rung that runs
+ * to maxit ends the ladder (KSP_DIVERGED_ITS).  The pattern must be structurally symmetric, as MFEM assembles it.
+ * x: in: the Dirichlet values on port_dof (other entries ignored); out: the solution.  rnorm: ||b - A x||_2 in
+ * FP64 (the reference returns KSP's residual norm).  iters: GMRES iterations; converged: 1 when rnorm <= tol ||b||
+ * (nonconvergence is a flag, not an error); rung: the ladder rung that ran last.  Any of the outputs but x may be
+ * NULL. */
+int opem_solve(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
+               int64_t nport, const int32_t *port_dof, const opem_solve_opts *opts, double *x, double *rnorm,
+               int32_t *iters, int32_t *converged, int32_t *rung);
+
+/* solveHfield (solveComplexLinearSystem.c:235-315): P h = b for a caller's complex CSR P (the ND mass matrix of
+ * build_P, fem3D.cpp:192-207; no boundary conditions) and b (buildHb: b = j Q x/(w mu), solveComplexLinearSystem.c
+ * :211-221, which a caller forms with opem_spmv), by the same preconditioned GMRES and retry ladder as opem_solve.
+ * h: the solution (n complex). */
+int opem_hfield(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
+                const double *b, const opem_solve_opts *opts, double *h, double *rnorm, int32_t *iters,
+                int32_t *converged, int32_t *rung);
+
+/* Mode::calculateSplits (src/OpenParEM3D/port.cpp:3524-3568) for one port of the context's mesh: the modal E_t and
+ * H_t sampled at the port's opem_port_points (complex npts*3 each) are projected onto the port's trace space, and
+ * with the 3D E and H DOF vectors x and h (complex ndof each): e0 = m_E^H M_p x, e2 = m_E^H M_p m_E, h0, h2 likewise
+ * (M_p the exact trace mass matrix, tangentialInnerProduct port.cpp:3463-3489), C+ = (e0/e2 + h0/h2)/2,
+ * C- = (e0/e2 - h0/h2)/2.  splits: complex [6] = {C+, C-, e0, e2, h0, h2}. */
+int opem_port_project(opem_ctx *ctx, int attr, const double *mode_E, const double *mode_H, const double *x,
+                      const double *h, double *splits);
+
 /* Allocation-free variant of opem_spmv (same arguments, same kernel, identical y).  It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The caller's rowptr, colind, vals, x and y are staged in a 64 MiB

@@ -14,7 +14,7 @@ SYMBOLS = [
     "opem_base_matrix", "opem_material_set", "opem_boundary_set", "opem_port_points", "opem_system_matrix",
     "opem_spmv", "opem_element_matrices", "opem_sweep", "opem_calculate_s", "opem_spmv_bench",
     "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench", "opem_spmv_noalloc",
-    "opem_lu_stats",
+    "opem_lu_stats", "opem_solve", "opem_hfield", "opem_port_project",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -37,6 +37,11 @@ class SweepIn(C.Structure):
                 ("precond_share", C.c_int), ("poly_degree", C.c_int),
                 ("arnoldi_step", C.c_int),
                 ("warm_start", C.c_int)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("restart", C.c_int), ("maxit", C.c_int), ("tol", C.c_double),
+                ("first_rung", C.c_int)]
 
 
 class SweepOut(C.Structure):
@@ -68,6 +73,11 @@ def lib():
         L.opem_spmv.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
         L.opem_spmv_noalloc.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
         L.opem_lu_stats.argtypes = [P, C.c_int, f64p]
+        L.opem_solve.argtypes = [P, C.c_int64, i32p, i32p, f64p, C.c_int64, i32p, C.POINTER(SolveOpts), f64p, f64p,
+                                 i32p, i32p, i32p]
+        L.opem_hfield.argtypes = [P, C.c_int64, i32p, i32p, f64p, f64p, C.POINTER(SolveOpts), f64p, f64p, i32p, i32p,
+                                  i32p]
+        L.opem_port_project.argtypes = [P, C.c_int, f64p, f64p, f64p, f64p, f64p]
         L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]

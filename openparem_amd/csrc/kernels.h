@@ -245,9 +245,10 @@ int spmv_pair_enabled();
 
 // ---------------------------------------------------------------- Givens / Hessenberg (device)
 // Apply previous rotations to column j of H[s] ((m+1) x m, column-major, complex), new rotation,
-// update g; est[s] = |g[j+1]|; deactivate systems whose est <= thresh[s] (records kfinal[s] = j+1).
+// update g; est[s] = |g[j+1]|; deactivate systems whose est <= thresh[s] (records kfinal[s] = j+1), and every
+// active system at jlast, the cycle's last step (jlast <= m - 1).
 // nrm2[s] = |w|^2 of the new Arnoldi vector (becomes H[j+1][j]).
-void launch_givens_n(hipStream_t st, int S, int m, int j, double *H, double *cs, double *sn, double *g,
+void launch_givens_n(hipStream_t st, int S, int m, int j, int jlast, double *H, double *cs, double *sn, double *g,
                      double *est, const double *thresh, int32_t *active, int32_t *kfinal, const double *nrm2);
 // y[s] = R^{-1} g  (k = kfinal[s] or j+1 for still active)
 void launch_backsolve(hipStream_t st, int S, int m, const double *H, const double *g, const int32_t *kfinal,

@@ -2734,7 +2734,7 @@ template void launch_norm2<double2>(hipStream_t, int, int64_t, const double2 *, 
 template void launch_norm2<float2>(hipStream_t, int, int64_t, const float2 *, double *, int, const int32_t *);
 
 // ------------------------------------------------------------------ Givens / Hessenberg
-__global__ void k_givens(int S, int m, int j, double2 *__restrict__ H, double *__restrict__ cs,
+__global__ void k_givens(int S, int m, int j, int jlast, double2 *__restrict__ H, double *__restrict__ cs,
                          double2 *__restrict__ sn, double2 *__restrict__ g, double *__restrict__ est,
                          const double *__restrict__ thresh, int32_t *__restrict__ active,
                          int32_t *__restrict__ kfinal, const double *__restrict__ nrm2) {
@@ -2783,16 +2783,16 @@ __global__ void k_givens(int S, int m, int j, double2 *__restrict__ H, double *_
   const double e = sqrt(gs[j + 1].x * gs[j + 1].x + gs[j + 1].y * gs[j + 1].y);
   est[s] = e;
   const bool breakdown = !(nrm2[s] > 0.0);
-  if (e <= thresh[s] || breakdown || j == m - 1) {
+  if (e <= thresh[s] || breakdown || j >= jlast) {
     kfinal[s] = j + 1;
     active[s] = 0;
   }
 }
 
-void launch_givens_n(hipStream_t st, int S, int m, int j, double *H, double *cs, double *sn, double *g,
+void launch_givens_n(hipStream_t st, int S, int m, int j, int jlast, double *H, double *cs, double *sn, double *g,
                      double *est, const double *thresh, int32_t *active, int32_t *kfinal, const double *nrm2) {
-  hipLaunchKernelGGL(k_givens, dim3((S + 63) / 64), dim3(64), 0, st, S, m, j, (double2 *)H, cs, (double2 *)sn,
-                     (double2 *)g, est, thresh, active, kfinal, nrm2);
+  hipLaunchKernelGGL(k_givens, dim3((S + 63) / 64), dim3(64), 0, st, S, m, j, jlast, (double2 *)H, cs,
+                     (double2 *)sn, (double2 *)g, est, thresh, active, kfinal, nrm2);
   HIP_LAUNCH_CHECK();
 }
 

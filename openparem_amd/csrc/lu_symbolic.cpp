@@ -507,7 +507,8 @@ void lu_analyse_graph(int64_t n, const int32_t *rowptr, const int32_t *colind, c
       }
     }
   };
-  int tagc = 0;
+  int tagc = 0, cstamp = 0;
+  std::vector<int32_t> cvis(n, -1);
   auto split = [&](const std::vector<int32_t> &V, int, std::vector<int32_t> &A, std::vector<int32_t> &B,
                    std::vector<int32_t> &S) -> bool {
     // mark the set, BFS from V[0] to a far vertex, BFS again from it; cut at the level holding the median
@@ -515,8 +516,32 @@ void lu_analyse_graph(int64_t n, const int32_t *rowptr, const int32_t *colind, c
     for (int32_t v : V) mark[v] = t0 - 1;
     std::vector<int32_t> order;
     bfs(V[0], t0, order);
-    if (order.size() < V.size()) {  // disconnected: V[0]'s component against the rest, no separator needed
-      for (int32_t v : V) (mark[v] == t0 ? A : B).push_back(v);
+    if (order.size() < V.size()) {
+      // disconnected: whole components to either side (largest first, to the lighter side), no separator needed
+      const int cs = 2 * (++cstamp);
+      for (int32_t v : V) cvis[v] = cs;  // in the set, not visited
+      std::vector<std::vector<int32_t>> comps;
+      for (int32_t v0 : V) {
+        if (cvis[v0] != cs) continue;
+        std::vector<int32_t> comp{v0};
+        cvis[v0] = cs + 1;
+        for (size_t h = 0; h < comp.size(); h++) {
+          const int32_t v = comp[h];
+          for (int64_t k = gp[v]; k < gp[v + 1]; k++)
+            if (cvis[gc[k]] == cs) {
+              cvis[gc[k]] = cs + 1;
+              comp.push_back(gc[k]);
+            }
+        }
+        comps.push_back(std::move(comp));
+      }
+      std::stable_sort(comps.begin(), comps.end(), [](const std::vector<int32_t> &a, const std::vector<int32_t> &b) {
+        return a.size() > b.size();
+      });
+      for (auto &c : comps) {
+        std::vector<int32_t> &side = A.size() <= B.size() ? A : B;
+        side.insert(side.end(), c.begin(), c.end());
+      }
       for (int32_t v : V) mark[v] = -1;
       return true;
     }

@@ -134,6 +134,12 @@ struct Ctx {
   LuFactor *lu_pc = nullptr;  // set while a Krylov solve is preconditioned by lu
   int lu_nrhs = 0;            // systems per factor
   int lu_leaf = 8;            // dissection leaves: at most this many mesh vertices
+  // preconditioner rung of a caller-matrix solve: 0 the configured one, 2 point Jacobi (d_jacinv), 3 none
+  int pc_override = 0;
+  DevBuf d_jacinv;
+  // caller-matrix solves (opem_solve, opem_hfield): their own tree and factor
+  LuTree solve_tree;
+  LuFactor solve_lu;
 };
 
 // ------------------------------------------------------------------ small kernels
@@ -200,6 +206,54 @@ __global__ void k_diag_apply(int64_t n, const double *__restrict__ dinv, const d
     const double d = dinv[i];
     const double2 v = r[(int64_t)s * n + i];
     z[(int64_t)s * n + i] = make_double2(d * v.x, d * v.y);
+  }
+}
+
+// z[i*S + s] = d[i] v[s*ldv + i] (d = null: identity) -- point Jacobi / no preconditioner, system-interleaved out
+template <typename X>
+__global__ void k_pc_diag(int64_t n, int S, const double2 *__restrict__ d, const X *__restrict__ v, int64_t ldv,
+                          X *__restrict__ z) {
+  const int s = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const X a = v[(int64_t)s * ldv + i];
+    X o;
+    if (d) {
+      const double2 w = d[i];
+      o.x = (decltype(o.x))(w.x * (double)a.x - w.y * (double)a.y);
+      o.y = (decltype(o.y))(w.x * (double)a.y + w.y * (double)a.x);
+    } else {
+      o = a;
+    }
+    z[i * S + s] = o;
+  }
+}
+
+// y = mask .* (b - y) (b = null: b = 0), for the true residual of a caller-matrix solve
+template <typename X>
+__global__ void k_resid(int64_t n, const double2 *__restrict__ b, const uint8_t *__restrict__ mask, X *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    X o;
+    if (mask[i]) {
+      const double bx = b ? b[i].x : 0.0, by = b ? b[i].y : 0.0;
+      o.x = (decltype(o.x))(bx - (double)y[i].x);
+      o.y = (decltype(o.y))(by - (double)y[i].y);
+    } else {
+      o.x = 0;
+      o.y = 0;
+    }
+    y[i] = o;
+  }
+}
+
+// inverse of the diagonal of a complex CSR matrix (1 where the diagonal is missing or zero)
+__global__ void k_diag_inverse(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const double2 *__restrict__ v, double2 *__restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 a = make_double2(0.0, 0.0);
+    for (int32_t k = rp[i]; k < rp[i + 1]; k++)
+      if (ci[k] == i) a = v[k];
+    const double m = a.x * a.x + a.y * a.y;
+    d[i] = m > 0 ? make_double2(a.x / m, -a.y / m) : make_double2(1.0, 0.0);
   }
 }
 
@@ -842,8 +896,9 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
   // preconditioner: vertex-patch additive Schwarz (inverses shared by the excitations of a frequency
   // except on port patches; FP16 with per-block scales inside the FP32 solve), or element-block Jacobi
   using PP = typename std::conditional<std::is_same<X, float2>::value, __half2, double2>::type;
-  const bool use_lu = c.lu_pc != nullptr;
-  const bool use_patch = !use_lu && c.have_patches && c.precond == 0;
+  const bool rung = c.pc_override == 2 || c.pc_override == 3;
+  const bool use_lu = !rung && c.lu_pc != nullptr;
+  const bool use_patch = !rung && !use_lu && c.have_patches && c.precond == 0;
   PatchLayout PL{};
   const int G = S / R;
   if (use_patch) {
@@ -895,19 +950,24 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
         for (int64_t p = 0; p < PL.npatch; p++, tot++) neg += sv[((size_t)q * R) * PL.npatch + p] < 0.f;
       fprintf(stderr, "opem: %zu of %zu FP16 patch blocks carry lo planes\n", neg, tot);
     }
-  } else if (!use_lu) {
+  } else if (!use_lu && !rung) {
     W.inv.alloc((size_t)S * nd * n * sizeof(P));
     launch_bj_build<P>(st, S, R, nd, c.d_elem_dofs.as<int32_t>(), c.d_elem_nz.as<int32_t>(), W.A64.as<double>(),
                        nnz, W.masks.as<uint8_t>(), n, c.d_owner_elem.as<int32_t>(), W.inv.as<P>(),
                        c.d_blocks.as<int32_t>(), c.nblocks);
   }
-  const bool use_coarse = !use_lu && c.have_coarse && c.coarse.nc > 0;
+  const bool use_coarse = !use_lu && !rung && c.have_coarse && c.coarse.nc > 0;
   if (use_coarse) {
     W.t.alloc((size_t)S * n * sizeof(X));
     W.rc.alloc((size_t)S * c.coarse.nc * 16);
     W.yc.alloc((size_t)S * c.coarse.nc * 16);
   }
   auto precond = [&](const X *v, int64_t ldv_sys, X *z, const int32_t *act) {
+    if (c.pc_override == 2 || c.pc_override == 3) {
+      hipLaunchKernelGGL((k_pc_diag<X>), dim3(blocks_for(n, 4096), S), dim3(256), 0, st, n, S,
+                         c.pc_override == 2 ? c.d_jacinv.as<double2>() : nullptr, v, ldv_sys, z);
+      return;
+    }
     if (use_lu) {
       c.lu_pc->solve<X>(c.lu_nrhs, v, ldv_sys, z);
       return;
@@ -1011,6 +1071,10 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
       thresh[i] = std::max(tol * bnorm[i], (mixed ? inner_red : 0.0) * rn) * (mixed ? 1.0 : 0.5);
     }
     if (!any) break;
+    // a cycle never takes a system past maxit (PETSc counts every inner iteration against it)
+    int mc = m;
+    for (int i = 0; i < S; i++)
+      if (act[i]) mc = std::min(mc, maxit - iters[i]);
     HIPCHK(hipMemcpyAsync(W.active.p, act.data(), S * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(W.cyc_active.p, act.data(), S * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(W.thresh.p, thresh.data(), S * 8, hipMemcpyHostToDevice, st));
@@ -1089,12 +1153,12 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
       HIPCHK(hipMemsetAsync(W.Hh.p, 0, (size_t)S * m * (m + 1) * 16, st));
     }
     int j = 0;
-    for (j = 0; j < m; j++) {
+    for (j = 0; j < mc; j++) {
       X *Vj = W.V.as<X>() + (int64_t)j * n;
       // s-step: ns Krylov vectors per pass over V (z_1 = B v_j, z_{c+1} = B z_c, block projection and update,
       // Cholesky QR of the block, Hessenberg columns j .. j+ns-1 from the raw relation); the tail of a cycle
       // shorter than ns takes single steps
-      const int nb = (ns > 1 && j + ns <= m) ? ns : (ns > 1 && j + 2 <= m ? 2 : 1);
+      const int nb = (ns > 1 && j + ns <= mc) ? ns : (ns > 1 && j + 2 <= mc ? 2 : 1);
       if (nb > 1) {
         const int32_t *a = W.active.as<int32_t>();
         X *Z = W.w2.as<X>();
@@ -1110,8 +1174,8 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
         launch_hcols_s(st, nb, S, m, j, W.C2.as<double>(), kMaxBasis, W.R2.as<double>(), W.Hraw.as<double>(),
                        W.Hh.as<double>(), W.nrm2b.as<double>(), a);
         for (int t = 0; t < nb; t++)
-          launch_givens_n(st, S, m, j + t, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
-                          W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
+          launch_givens_n(st, S, m, j + t, mc - 1, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(),
+                          W.g.as<double>(), W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
                           W.nrm2b.as<double>() + (int64_t)t * S);
         const int j0 = j;
         j += nb - 1;
@@ -1142,8 +1206,8 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
       }
       launch_reduce(st, S, nblk, 0, W.partial.as<double>(), W.nrm2.as<double>(), 1, nullptr, 0,
                     W.active.as<int32_t>());
-      launch_givens_n(st, S, m, j, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(), W.g.as<double>(),
-                      W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
+      launch_givens_n(st, S, m, j, mc - 1, W.Hh.as<double>(), W.cs.as<double>(), W.sn.as<double>(),
+                      W.g.as<double>(), W.est.as<double>(), W.thresh.as<double>(), W.active.as<int32_t>(), W.kfinal.as<int32_t>(),
                       W.nrm2.as<double>());
       if (j + 1 < m)
         launch_scale<X>(st, S, n, W.w.as<X>(), n, W.V.as<X>() + (int64_t)(j + 1) * n, ldv, W.nrm2.as<double>(),
@@ -2014,6 +2078,126 @@ static int csr_lanes_per_row(int group, int64_t n, int64_t nnz) {
   return group > 1 ? 16 : nnz / n <= 12 ? 8 : nnz / n <= 48 ? 16 : 32;
 }
 
+// Krylov solve of a caller's complex CSR system, the counterpart of solveComplexLinearSystem and solveHfield
+// (solveComplexLinearSystem.c:100-209, 235-315).  Rows in `dir` are Dirichlet: their values are x's on entry, their
+// rows and columns are eliminated (MatZeroRowsColumns with diagonal 1) and the right-hand side is lifted,
+// b_free = b - A x_D (b = null: 0, the E-field case), b[dir] = x[dir].  GMRES (right-preconditioned, restarted,
+// FP64 true residual at every restart; in mixed precision FP32 inner cycles) with the retry ladder of the
+// reference: rung 0 a sparse LU of the eliminated matrix (nested dissection on its graph; FP32 factor in mixed
+// precision), rung 1 point Jacobi, rung 2 none.  As in the reference the ladder stops at the first rung that
+// converges or runs to the iteration limit; it moves on only when a rung breaks down (a non-finite residual).
+struct CallerSolve {
+  int precision = OPEM_PREC_FP64;
+  int restart = 30;
+  int maxit = 10000;
+  double tol = 1e-10;
+  int first_rung = 0;
+};
+
+static void solve_caller(Ctx &c, int64_t n, const int32_t *rp, const int32_t *ci, const double *vals, int64_t ndir,
+                         const int32_t *dir, const double *b, const CallerSolve &o, double *x, double *rnorm,
+                         int32_t *iters, int32_t *converged, int32_t *rung_used) {
+  hipStream_t st = c.st;
+  const int64_t nnz = rp[n];
+  const bool mixed = o.precision == OPEM_PREC_MIXED;
+  std::vector<uint8_t> mask(n, 1), excl(n, 0), last(n, 0);
+  for (int64_t k = 0; k < ndir; k++) {
+    mask[dir[k]] = 0;
+    excl[dir[k]] = 1;
+  }
+  DevBuf drp, dci, dv, dv32, db;
+  drp.upload(std::vector<int32_t>(rp, rp + n + 1), st);
+  dci.upload(std::vector<int32_t>(ci, ci + nnz), st);
+  dv.upload(std::vector<double>(vals, vals + 2 * nnz), st);
+  if (mixed) {
+    dv32.alloc((size_t)nnz * 8);
+    launch_convert<double2, float2>(st, nnz, dv.as<double2>(), dv32.as<float2>());
+  }
+  if (b) db.upload(std::vector<double>(b, b + 2 * n), st);
+  const int group = csr_row_group(n, rp, ci);
+  const int lpr = csr_lanes_per_row(group, n, nnz);
+  Work &W = c.work;
+  W.masks.upload(mask, st);
+  std::vector<double> xd((size_t)2 * n, 0.0);
+  double xd2 = 0;
+  for (int64_t k = 0; k < ndir; k++) {
+    const int32_t d = dir[k];
+    xd[2 * d] = x[2 * d];
+    xd[2 * d + 1] = x[2 * d + 1];
+    xd2 += x[2 * d] * x[2 * d] + x[2 * d + 1] * x[2 * d + 1];
+  }
+  const BatchOp op = [&](bool fp64, const void *xin, void *y, int mode, const int32_t *) {
+    if (fp64) {
+      launch_csr_spmv<double2>(st, n, drp.as<int32_t>(), dci.as<int32_t>(), dv.as<double2>(), W.masks.as<uint8_t>(),
+                               (const double2 *)xin, (double2 *)y, group, lpr);
+      if (mode == 1)
+        hipLaunchKernelGGL((k_resid<double2>), dim3(blocks_for(n, 4096)), dim3(256), 0, st, n,
+                           b ? db.as<double2>() : nullptr, W.masks.as<uint8_t>(), (double2 *)y);
+    } else {
+      launch_csr_spmv<float2>(st, n, drp.as<int32_t>(), dci.as<int32_t>(), dv32.as<float2>(), W.masks.as<uint8_t>(),
+                              (const float2 *)xin, (float2 *)y, group, lpr);
+      if (mode == 1)
+        hipLaunchKernelGGL((k_resid<float2>), dim3(blocks_for(n, 4096)), dim3(256), 0, st, n,
+                           b ? db.as<double2>() : nullptr, W.masks.as<uint8_t>(), (float2 *)y);
+    }
+  };
+  // ||b|| of the lifted system: free rows b - A x_D, Dirichlet rows x_D
+  W.x.upload(xd, st);
+  W.r.alloc((size_t)n * 16);
+  op(true, W.x.p, W.r.p, 1, nullptr);
+  std::vector<double> r0((size_t)2 * n);
+  HIPCHK(hipMemcpyAsync(r0.data(), W.r.p, r0.size() * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double b2 = xd2;
+  for (double v : r0) b2 += v * v;
+  std::vector<double> bn{std::sqrt(b2)};
+  if (b2 == 0.0) {  // b = 0 and no Dirichlet values: x = 0 solves it exactly
+    std::memset(x, 0, (size_t)n * 16);
+    if (rnorm) *rnorm = 0.0;
+    if (iters) *iters = 0;
+    if (converged) *converged = 1;
+    if (rung_used) *rung_used = std::max(0, std::min(2, o.first_rung));
+    return;
+  }
+  std::vector<int> it, cv;
+  std::vector<double> rel;
+  int rung = std::max(0, std::min(2, o.first_rung));
+  for (; rung <= 2; rung++) {
+    c.pc_override = rung == 0 ? 0 : rung == 1 ? 2 : 3;
+    if (rung == 0) {
+      lu_analyse_graph(n, rp, ci, excl, last, 64, c.solve_tree);
+      c.solve_lu.setup(c.solve_tree, 1, {}, st);
+      LuFactorStats fs;
+      c.solve_lu.factor(mixed ? 1 : 0, 1, dv.as<double>(), nnz, &fs);
+      c.lu_pc = &c.solve_lu;
+      c.lu_nrhs = 1;
+    } else if (rung == 1) {
+      c.d_jacinv.alloc((size_t)n * 16);
+      hipLaunchKernelGGL(k_diag_inverse, dim3(blocks_for(n, 4096)), dim3(256), 0, st, n, drp.as<int32_t>(),
+                         dci.as<int32_t>(), dv.as<double2>(), c.d_jacinv.as<double2>());
+    }
+    W.x.upload(xd, st);
+    struct Reset {
+      Ctx &c;
+      ~Reset() {
+        c.lu_pc = nullptr;
+        c.pc_override = 0;
+      }
+    } reset{c};
+    if (mixed)
+      gmres_batch<float2>(c, W, op, n, 1, 1, o.restart, o.maxit, o.tol, 1, 1e-5, it, rel, cv, bn);
+    else
+      gmres_batch<double2>(c, W, op, n, 1, 1, o.restart, o.maxit, o.tol, 2, 1e-5, it, rel, cv, bn);
+    if (cv[0] || std::isfinite(rel[0])) break;  // converged, or the rung ran to the iteration limit
+  }
+  HIPCHK(hipMemcpyAsync(x, W.x.p, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (rnorm) *rnorm = rel[0] * bn[0];
+  if (iters) *iters = it[0];
+  if (converged) *converged = cv[0];
+  if (rung_used) *rung_used = std::min(rung, 2);
+}
+
 // ------------------------------------------------------------------ allocation-free staging area
 // opem_spmv_noalloc stages the caller's arrays here instead of allocating.  Each GPU the library runs on holds
 // its own copy for the life of the process; host code reaches it only through the address hipGetSymbolAddress
@@ -2350,6 +2534,83 @@ int opem_spmv_noalloc(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr,
     }
     HIPCHK(hipMemcpyAsync(y, base + o_y, (size_t)n * 16, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+static void check_solve_opts(const opem_solve_opts *o, CallerSolve &cs) {
+  if (o) {
+    if (o->precision != OPEM_PREC_FP64 && o->precision != OPEM_PREC_MIXED) throw std::invalid_argument("bad precision");
+    if (o->restart < 0 || o->restart > kMaxBasis - 2) throw std::invalid_argument("restart must be 0..400");
+    if (o->maxit < 0 || !(o->tol >= 0) || o->first_rung < 0 || o->first_rung > 2)
+      throw std::invalid_argument("bad solve options");
+    cs.precision = o->precision;
+    if (o->restart > 0) cs.restart = o->restart;
+    if (o->maxit > 0) cs.maxit = o->maxit;
+    if (o->tol > 0) cs.tol = o->tol;
+    cs.first_rung = o->first_rung;
+  }
+}
+
+int opem_solve(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
+               int64_t nport, const int32_t *port_dof, const opem_solve_opts *opts, double *x, double *rnorm,
+               int32_t *iters, int32_t *converged, int32_t *rung) {
+  return guarded(ctx, [&] {
+    if (n <= 0 || !rowptr || !colind || !vals || !x || nport < 0 || (nport > 0 && !port_dof))
+      throw std::invalid_argument("bad arguments");
+    check_caller_csr(n, rowptr, colind);
+    for (int64_t k = 0; k < nport; k++)
+      if (port_dof[k] < 0 || port_dof[k] >= n) throw std::invalid_argument("port DOF out of range");
+    CallerSolve cs;
+    check_solve_opts(opts, cs);
+    solve_caller(*ctx, n, rowptr, colind, vals, nport, port_dof, nullptr, cs, x, rnorm, iters, converged, rung);
+  });
+}
+
+int opem_hfield(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
+                const double *b, const opem_solve_opts *opts, double *h, double *rnorm, int32_t *iters,
+                int32_t *converged, int32_t *rung) {
+  return guarded(ctx, [&] {
+    if (n <= 0 || !rowptr || !colind || !vals || !b || !h) throw std::invalid_argument("bad arguments");
+    check_caller_csr(n, rowptr, colind);
+    CallerSolve cs;
+    check_solve_opts(opts, cs);
+    std::vector<double> hx((size_t)2 * n, 0.0);
+    solve_caller(*ctx, n, rowptr, colind, vals, 0, nullptr, b, cs, hx.data(), rnorm, iters, converged, rung);
+    std::memcpy(h, hx.data(), hx.size() * 8);
+  });
+}
+
+int opem_port_project(opem_ctx *ctx, int attr, const double *mode_E, const double *mode_H, const double *x,
+                      const double *h, double *splits) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (!mode_E || !mode_H || !x || !h || !splits) throw std::invalid_argument("bad arguments");
+    build_ports(*ctx);
+    const PortTrace *P = nullptr;
+    for (auto &pt : ctx->ports)
+      if (pt.attr == attr) P = &pt;
+    if (!P) throw std::invalid_argument("attribute is not a port");
+    const int ndf = ctx->sp.ndf;
+    std::vector<cplx> me, mh, ue, uh;
+    port_project(*P, ndf, mode_E, true, me);
+    port_project(*P, ndf, mode_H, false, mh);
+    port_mass_apply(*P, me, ue);
+    port_mass_apply(*P, mh, uh);
+    cplx e0 = 0, e2 = 0, h0 = 0, h2 = 0;
+    for (size_t i = 0; i < P->dofs.size(); i++) {
+      const int32_t d = P->dofs[i];
+      const cplx xv(x[2 * d], x[2 * d + 1]), hv(h[2 * d], h[2 * d + 1]);
+      e0 += std::conj(ue[i]) * xv;
+      h0 += std::conj(uh[i]) * hv;
+      e2 += std::conj(me[i]) * ue[i];
+      h2 += std::conj(mh[i]) * uh[i];
+    }
+    const cplx re = e0 / e2, rh = h0 / h2;
+    const cplx out[6] = {0.5 * (re + rh), 0.5 * (re - rh), e0, e2, h0, h2};  // port.cpp:3564-3565
+    for (int k = 0; k < 6; k++) {
+      splits[2 * k] = out[k].real();
+      splits[2 * k + 1] = out[k].imag();
+    }
   });
 }
 

@@ -108,6 +108,18 @@ class Solver:
         check(lib().opem_port_points(self.ctx, attr, C.byref(n), ptr(pts, C.c_double)), self.ctx, "port_points")
         return pts
 
+    def port_project(self, attr, mode, x, h):
+        """opem_port_project: {C+, C-, e0, e2, h0, h2} of port `attr` for the E and H DOF vectors x, h, with the
+        modal fields of `mode` (a provider dict with callables E, H) sampled at the port's points."""
+        pts = self.port_points(attr)
+        Ev = cplx_view(mode["E"](pts).ravel())
+        Hv = cplx_view(mode["H"](pts).ravel())
+        out = np.zeros(6, np.complex128)
+        check(lib().opem_port_project(self.ctx, int(attr), ptr(Ev, C.c_double), ptr(Hv, C.c_double),
+                                      ptr(cplx_view(x), C.c_double), ptr(cplx_view(h), C.c_double),
+                                      ptr(out.view(np.float64), C.c_double)), self.ctx, "opem_port_project")
+        return dict(zip(("Cp", "Cm", "e0", "e2", "h0", "h2"), out.tolist()))
+
     def spmv_bench(self, precision="fp32", ngroups=1, R=2, nrep=20):
         """(mean kernel seconds, algorithmic bytes) of the batched SpMV on this pattern."""
         t, b = C.c_double(), C.c_double()
@@ -190,6 +202,45 @@ class Solver:
         return dict(S=S, S_raw=Sr, Cp=Cp, Cm=Cm, iters=it, true_res=res, converged=cv, x=x, h=h,
                     seconds=dict(total=sout.seconds_total, setup=sout.seconds_setup, solve=sout.seconds_solve,
                                  post=sout.seconds_post))
+
+
+def solve(ctx_holder, A, port_dof, x, precision="fp64", restart=30, maxit=10000, tol=1e-10, first_rung=0):
+    """opem_solve on a scipy CSR matrix A (complex): x carries the Dirichlet values on port_dof.  Returns the
+    solution and (rnorm, iters, converged, rung)."""
+    A = A.tocsr()
+    A.sort_indices()
+    n = A.shape[0]
+    rp = np.ascontiguousarray(A.indptr, np.int32)
+    ci = np.ascontiguousarray(A.indices, np.int32)
+    v = cplx_view(A.data)
+    pdof = np.ascontiguousarray(port_dof, np.int32)
+    xx = np.ascontiguousarray(x, np.complex128).copy()
+    o = _lib.SolveOpts(precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64, restart=int(restart),
+                       maxit=int(maxit), tol=float(tol), first_rung=int(first_rung))
+    rn, it, cv, rg = C.c_double(), C.c_int32(), C.c_int32(), C.c_int32()
+    check(lib().opem_solve(ctx_holder.ctx, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
+                           pdof.size, ptr(pdof, C.c_int32), C.byref(o), ptr(xx.view(np.float64), C.c_double),
+                           C.byref(rn), C.byref(it), C.byref(cv), C.byref(rg)), ctx_holder.ctx, "opem_solve")
+    return xx, (rn.value, it.value, cv.value, rg.value)
+
+
+def hfield(ctx_holder, P, b, precision="fp64", restart=30, maxit=10000, tol=1e-10, first_rung=0):
+    """opem_hfield: P h = b for a scipy CSR P.  Returns h and (rnorm, iters, converged, rung)."""
+    P = P.tocsr().astype(np.complex128)
+    P.sort_indices()
+    n = P.shape[0]
+    rp = np.ascontiguousarray(P.indptr, np.int32)
+    ci = np.ascontiguousarray(P.indices, np.int32)
+    v = cplx_view(P.data)
+    bb = cplx_view(b)
+    h = np.zeros(n, np.complex128)
+    o = _lib.SolveOpts(precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64, restart=int(restart),
+                       maxit=int(maxit), tol=float(tol), first_rung=int(first_rung))
+    rn, it, cv, rg = C.c_double(), C.c_int32(), C.c_int32(), C.c_int32()
+    check(lib().opem_hfield(ctx_holder.ctx, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
+                            ptr(bb, C.c_double), C.byref(o), ptr(h.view(np.float64), C.c_double), C.byref(rn),
+                            C.byref(it), C.byref(cv), C.byref(rg)), ctx_holder.ctx, "opem_hfield")
+    return h, (rn.value, it.value, cv.value, rg.value)
 
 
 LU_STAT_KEYS = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",

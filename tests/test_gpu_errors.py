@@ -156,3 +156,48 @@ def test_spmv_rejects_bad_csr_before_upload():
         assert np.array_equal(y, x)
     finally:
         dev.close()
+
+
+def test_solve_entries_reject_misuse():
+    """opem_solve / opem_hfield / opem_port_project: out-of-range port DOFs, bad options and a context without a
+    mesh are rejected (-2) before any device work; x and h keep their contents; the context stays usable."""
+    from openparem_amd.solver import Device
+    L = lib()
+    dev = Device()
+    try:
+        n = 6
+        rp = np.arange(n + 1, dtype=np.int32)
+        ci = np.arange(n, dtype=np.int32)
+        v = np.zeros(2 * n, np.float64)
+        v[0::2] = 2.0  # 2 I
+        x = np.full(2 * n, 5.0)
+        outs = (C.c_double(), C.c_int32(), C.c_int32(), C.c_int32())
+        refs = [C.byref(o) for o in outs]
+
+        def solve(port, opts):
+            return L.opem_solve(dev.ctx, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double), port.size,
+                                ptr(port, C.c_int32), C.byref(opts) if opts is not None else None,
+                                ptr(x, C.c_double), *refs)
+
+        ok = _lib.SolveOpts(precision=_lib.PREC_FP64)
+        for port, opts, msg in ((np.array([n], np.int32), ok, "port DOF out of range"),
+                                (np.array([1], np.int32), _lib.SolveOpts(precision=2), "bad precision"),
+                                (np.array([1], np.int32), _lib.SolveOpts(restart=401), "restart"),
+                                (np.array([1], np.int32), _lib.SolveOpts(first_rung=3), "bad solve options"),
+                                (np.array([1], np.int32), _lib.SolveOpts(tol=-1.0), "bad solve options")):
+            rc = solve(port, opts)
+            print(f"solve misuse {msg}: rc={rc} {_err(L, dev.ctx)!r}", flush=True)
+            assert rc == -2 and msg in _err(L, dev.ctx) and (x == 5.0).all()
+        h = np.full(2 * n, 5.0)
+        assert L.opem_hfield(dev.ctx, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double), None, None,
+                             ptr(h, C.c_double), *refs) == -2 and (h == 5.0).all()
+        sp6 = np.zeros(12)
+        z = np.zeros(2 * n)
+        assert L.opem_port_project(dev.ctx, 2, ptr(z, C.c_double), ptr(z, C.c_double), ptr(z, C.c_double),
+                                   ptr(z, C.c_double), ptr(sp6, C.c_double)) == -2
+        assert "space not built" in _err(L, dev.ctx)
+        # the context still solves: 2 I x = b with the Dirichlet row 1 = 5 + 5j, every other row b = 0
+        assert solve(np.array([1], np.int32), None) == 0
+        assert outs[2].value == 1 and x[2] == 5.0 and x[3] == 5.0 and (np.delete(x, [2, 3]) == 0.0).all()
+    finally:
+        dev.close()
