@@ -9,6 +9,9 @@ relative residual of 1e-10 with FP32 GMRES inside FP64 iterative refinement, pre
 FP64 GPU path (FP64 LU factor of every frequency, one frequency at a time, FP64 GMRES) then solves every frequency
 of the sweep; a frequency point counts towards the metric only when both paths converged and the mixed S is within
 1e-6 of the FP64 S.  One JSON line is printed.
+
+``--workload uneven-c5`` runs the uneven set instead (16 instances, ND1-3, 2-4 ports, 50 k - 1.5 M DOF, one after
+another; see ``_uneven``).
 """
 from __future__ import annotations
 
@@ -52,6 +55,104 @@ MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_FP64_PEAK_TFLOPS = 78.6
 
 
+def _lu_roofline(st, workload):
+    """Rooflines of the sparse-LU kernels over a timed region (HIP events on the context stream, from opem_lu_stats):
+    the trailing-update GEMM against the FP32 MFMA peak and the apply's factor products against HBM; the one that
+    took longer is the roofline kernel."""
+    gemm = {"kernel": "k_dense_gemm (trailing updates of the FP32 fronts on v_mfma_f32_16x16x4f32)", "bound": "mfma",
+            "achieved": st["gemm_flops"] / st["gemm_s"] / 1e12 if st["gemm_s"] else None,
+            "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "seconds": st["gemm_s"],
+            "launches": st["gemm_launches"], "flops": st["gemm_flops"]}
+    gemm["frac"] = gemm["achieved"] / MFMA_FP32_PEAK_TFLOPS if gemm["achieved"] else None
+    apply_ = {"kernel": "k_lu_apply_part (the preconditioner apply's products with the FP32 factor blocks)",
+              "bound": "hbm", "achieved": st["apply_bytes"] / st["apply_s"] / 1e9 if st["apply_s"] else None,
+              "peak": HBM_PEAK_GBS, "unit": "GB/s", "seconds": st["apply_s"], "launches": st["apply_launches"],
+              "bytes": st["apply_bytes"]}
+    apply_["frac"] = apply_["achieved"] / HBM_PEAK_GBS if apply_["achieved"] else None
+    dom = gemm if st["gemm_s"] >= st["apply_s"] else apply_
+    kname = "k_dense_gemm" if dom is gemm else "k_lu_apply_part"
+    ratio = _pmc_traffic_ratio(kname, workload)
+    work_per_launch = (dom["flops"] if dom is gemm else dom["bytes"]) / max(1.0, dom["launches"])
+    roof = {"kernel": dom["kernel"], "bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
+            "unit": dom["unit"], "frac": dom["frac"],
+            "traffic": None if ratio is None or dom is gemm else ratio * work_per_launch,
+            "per_launch": work_per_launch, "launch_seconds": dom["seconds"] / max(1.0, dom["launches"]),
+            "launches": dom["launches"]}
+    return roof, gemm, apply_
+
+
+def _uneven(args):
+    """The uneven set: 16 seeded guide, microstrip, tee and cross instances (ND1-3, 2-4 ports, 8-32 frequencies,
+    50 k - 1.5 M DOF), one after another.  Per instance: a one-frequency warm-up sweep (nested-dissection analysis,
+    kernel loading), the timed mixed sweep of all its frequencies, then the plain FP64 path on every frequency.  A
+    point counts when both converged and |dS| < 1e-6; value = counted points / summed mixed seconds."""
+    from openparem_amd import workloads
+    from openparem_amd.solver import Solver, lu_stats
+
+    keys = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",
+            "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies")
+    agg = {k: 0.0 for k in keys}
+    per, tot_t, tot64_t, tot_pass, tot_pts, tot_conv = [], 0.0, 0.0, 0, 0, 0
+    for i, w in enumerate(workloads.c5_instances()):
+        f = w["freqs"]
+        s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+        try:
+            share = 2 if len(f) > 1 and f[1] - f[0] <= 0.5e9 else 1
+            opts = dict(restart=args.restart, tol=args.tol, maxit=600, precond="lu")
+            mixed = dict(precision="mixed", precond_share=share, orth_passes=1, **opts)
+            s.sweep(f[:1], **mixed)
+            lu_stats(s, reset=True)
+            t0 = time.perf_counter()
+            r = s.sweep(f, **mixed)
+            dt = time.perf_counter() - t0
+            st = lu_stats(s, reset=True)
+            for k in keys:
+                agg[k] += st[k]
+            _progress(f"instance {i}: mixed sweep {dt:.2f} s")
+            t0 = time.perf_counter()
+            r64 = s.sweep(f, precision="fp64", batch=1, orth_passes=2, **opts)
+            dt64 = time.perf_counter() - t0
+            dS = np.abs(r["S"] - r64["S"]).max(axis=(1, 2))
+            conv = np.all(r["converged"].astype(bool), axis=1)
+            ok = conv & np.all(r64["converged"].astype(bool), axis=1) & (dS < 1e-6)
+            per.append({"instance": i, "family": w["family"], "order": w["order"], "ndof": s.ndof,
+                        "ports": len(s.ports), "freqs": len(f), "band_GHz": [f[0] / 1e9, f[-1] / 1e9],
+                        "share": share, "mixed_s": round(dt, 3), "fp64_s": round(dt64, 3),
+                        "points_converged": int(conv.sum()), "points_passing": int(ok.sum()),
+                        "iters_mean": float(r["iters"].mean()), "max_abs_dS_vs_fp64": float(dS.max())})
+            tot_t += dt
+            tot64_t += dt64
+            tot_pass += int(ok.sum())
+            tot_conv += int(conv.sum())
+            tot_pts += len(f)
+            _progress(f"instance {i} ({w['family']} ND{w['order']}, {s.ndof} DOF, {len(f)} freqs): mixed {dt:.2f} s, "
+                      f"fp64 {dt64:.2f} s, {int(ok.sum())}/{len(f)} pass, max |dS| {dS.max():.2e}")
+        finally:
+            s.close()
+    roof, gemm, apply_ = _lu_roofline(agg, "uneven-c5")
+    value = tot_pass / tot_t
+    fp64_value = tot_pts / tot64_t
+    line = {
+        "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",
+        "value": value, "unit": "freq-points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * tot_t, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c-fp32 factor and inner GMRES / c-fp64 refinement", "data": "synthetic",
+        "config": {"workload": f"uneven-c5: {len(per)} instances one after another, ND1-3, 2-4 ports, "
+                               f"{tot_pts} frequency points, {min(p['ndof'] for p in per)}-"
+                               f"{max(p['ndof'] for p in per)} DOF",
+                   "precond": "sparse LU per instance (FP32 factor, shared by 2 adjacent frequencies where they are "
+                              f"at most 0.5 GHz apart), GMRES({args.restart}) inside FP64 refinement"},
+        "points": tot_pts, "points_converged": tot_conv, "points_passing": tot_pass,
+        "fp64_plain": {"value": fp64_value, "unit": "freq-points/s", "points": tot_pts},
+        "speedup_vs_fp64_plain": value / fp64_value,
+        "max_abs_dS_vs_fp64": max(p["max_abs_dS_vs_fp64"] for p in per),
+        "roofline": roof, "kernels": {"k_dense_gemm": gemm, "k_lu_apply_part": apply_},
+        "instances": per, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +172,8 @@ def main():
         return 2
     # HIP-event timing of the sparse-LU kernels over the timed region (read by the library at context creation)
     os.environ["OPEM_LU_TIMING"] = "1"
+    if args.workload == "uneven-c5":
+        return _uneven(args)
 
     from openparem_amd import workloads
     from openparem_amd.measure import assembly_c2
@@ -123,26 +226,7 @@ def main():
         _progress("CPU baseline done")
     asm = None if args.no_assembly else {f"nd{p_}": assembly_c2(p_, 1_000_000, HBM_PEAK_GBS) for p_ in (1, 2, 3)}
 
-    # rooflines of the sparse-LU kernels over the timed region (HIP events on the context stream)
-    gemm = {"kernel": "k_dense_gemm (trailing updates of the FP32 fronts on v_mfma_f32_16x16x4f32)", "bound": "mfma",
-            "achieved": st["gemm_flops"] / st["gemm_s"] / 1e12 if st["gemm_s"] else None,
-            "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "seconds": st["gemm_s"],
-            "launches": st["gemm_launches"], "flops": st["gemm_flops"]}
-    gemm["frac"] = gemm["achieved"] / MFMA_FP32_PEAK_TFLOPS if gemm["achieved"] else None
-    apply_ = {"kernel": "k_lu_apply_part (the preconditioner apply's products with the FP32 factor blocks)",
-              "bound": "hbm", "achieved": st["apply_bytes"] / st["apply_s"] / 1e9 if st["apply_s"] else None,
-              "peak": HBM_PEAK_GBS, "unit": "GB/s", "seconds": st["apply_s"], "launches": st["apply_launches"],
-              "bytes": st["apply_bytes"]}
-    apply_["frac"] = apply_["achieved"] / HBM_PEAK_GBS if apply_["achieved"] else None
-    dom, other = (gemm, apply_) if st["gemm_s"] >= st["apply_s"] else (apply_, gemm)
-    kname = "k_dense_gemm" if dom is gemm else "k_lu_apply_part"
-    ratio = _pmc_traffic_ratio(kname, args.workload)
-    work_per_launch = (dom["flops"] if dom is gemm else dom["bytes"]) / max(1.0, dom["launches"])
-    roof = {"kernel": dom["kernel"], "bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
-            "unit": dom["unit"], "frac": dom["frac"],
-            "traffic": None if ratio is None or dom is gemm else ratio * work_per_launch,
-            "per_launch": work_per_launch, "launch_seconds": dom["seconds"] / max(1.0, dom["launches"]),
-            "launches": dom["launches"]}
+    roof, gemm, apply_ = _lu_roofline(st, args.workload)
 
     line = {
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",

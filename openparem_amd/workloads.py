@@ -8,10 +8,10 @@
                      band 7-12.5 GHz x 2 ports = 128 systems on one pattern.
 
 ``c5_instances()`` is a seeded mixed set (16 instances): twelve guide or microstrip instances, ND1/ND2 (one small
-ND3), target DOF count log-uniform in 50 k - 1.2 M, 8-32 frequencies, 2 ports; then four junctions of guides: an
+ND3), target DOF count log-uniform in 50 k - 1.5 M, 8-32 frequencies, 2 ports; then four junctions of guides: an
 H-plane tee (3 ports) and a cross (4 ports) at ND2, a finer tee at ND1 (148 k DOF, 24 frequencies) and a coarser
 cross at ND2 (77 k DOF, 32 frequencies).  Guides, tees and crosses sweep the TE10 band 7-12.5 GHz; microstrips
-sweep 2-10 GHz, below the box's multimode band.
+sweep 1-20 GHz, the band of the single-pattern sweep, multimode upper band included.
 """
 from __future__ import annotations
 
@@ -51,7 +51,7 @@ def c5_instances(seed=5):
     for i in range(12):
         family = "waveguide" if i % 2 == 0 else "microstrip"
         order = 3 if i == 10 else (1 if i % 3 == 0 else 2)
-        target = 6e4 if order == 3 else float(np.exp(rng.uniform(np.log(5e4), np.log(1.2e6))))
+        target = 6e4 if order == 3 else float(np.exp(rng.uniform(np.log(5e4), np.log(1.5e6))))
         nf = int(rng.choice([8, 16, 24, 32]))
         cells = target / _DOF_PER_CELL[order]
         if family == "waveguide":
@@ -64,7 +64,7 @@ def c5_instances(seed=5):
             f = (cells / (22 * 11 * 50)) ** (1.0 / 3.0)
             m = microstrip_mesh(nx=max(8, round(22 * f)), ny=max(5, round(11 * f)), nz=max(6, round(50 * f)),
                                 seed=100 + i)
-            out.append(dict(family=family, order=order, mesh=m, freqs=np.linspace(2e9, 10e9, nf),
+            out.append(dict(family=family, order=order, mesh=m, freqs=np.linspace(1e9, 20e9, nf),
                             boundaries={4: COPPER}))
     for cross, h, nf, order in ((False, 1.6e-3, 16, 2), (True, 1.8e-3, 8, 2), (False, 1.0e-3, 24, 1),
                                 (True, 2.4e-3, 32, 2)):
