@@ -260,6 +260,46 @@ int opem_lu_stats(opem_ctx *ctx, int reset, double *out);
 int opem_calculate_s(int n, const double *Cp, const double *Cm, const double *V, const double *Z,
                      int reciprocal, double *S);
 
+/* ---- S-parameter post-processing (host only, no context; src/OpenParEM3D/results.cpp, process3D.cpp) ----------
+ * S matrices are complex [nports][nports] row-major per frequency, the layout of opem_sweep_out.S (the reference's
+ * Result::S Mat).  These calls return 0, -2 for misuse, 1 for a failure (singular matrix, unwritable file);
+ * opem_results_error() gives the message of the calling thread's last failed call. */
+const char *opem_results_error(void);
+
+/* Result::S2Z (results.cpp:455-511): Z = k (I + S)(I - S)^-1 k, k = diag(sqrt(Zo)); Zo complex [nports]. */
+int opem_s2z(int nports, const double *S, const double *Zo, double *Z);
+/* Result::Z2S (results.cpp:513-560): S = (Z k^-1 + k)^-1 (Z k^-1 - k). */
+int opem_z2s(int nports, const double *Z, const double *Zo, double *S);
+/* Result::renormalize (results.cpp:562-578): S2Z with the ports' Zo, then Z2S with every port at Zref. */
+int opem_renormalize(int nports, const double *S, const double *Zo, double zref_re, double zref_im, double *S_out);
+
+/* What the S-parameter files say besides the numbers (the reference takes them from its project data). */
+typedef struct {
+  const char *version;          /* "major.minor.patch" for the first line; NULL or "" omits the line            */
+  const char *format;           /* "RI", "MA" or "DB" (touchstone_format)                                      */
+  const char *unit;             /* "Hz", "kHz", "MHz" or "GHz" (touchstone_frequency_unit)                     */
+  double reference_impedance;   /* ohm after renormalisation; 0: not renormalized                              */
+  const char *const *nets;      /* [nports] net names; NULL (or a NULL entry): net1, net2, ...                 */
+} opem_sparam_file;
+
+/* ResultDatabase::saveCSV (results.cpp:940-1046, rows Result::saveCSV :208-232): header lines, then one line per
+ * frequency with the entries (i, j) in the order i outer, j inner; freqs in Hz. */
+int opem_write_csv(const char *path, const opem_sparam_file *meta, int nports, int64_t nfreq, const double *freqs,
+                   const double *S);
+/* ResultDatabase::saveTouchstone, version 1.1 (results.cpp:1048-1190): rejects reference_impedance == 0, as the
+ * reference skips the file for data that is not renormalized.  Mixed-mode (Touchstone 2.0) output is not built. */
+int opem_write_touchstone(const char *path, const opem_sparam_file *meta, int nports, int64_t nfreq,
+                          const double *freqs, const double *S);
+/* ResultDatabase::save_as_test (results.cpp:276-341, 1540-1580): appends prototype regression cases for the
+ * results (magS(dB) and argS(deg), "equal" to 1e-12 or "lessthan" below -150 dB / 0.1 deg); *casenumber counts on. */
+int opem_save_as_test(const char *path, const char *casename, int nports, int64_t nfreq, const double *freqs,
+                      const double *S, int *casenumber);
+/* process3D (process3D.cpp:481-559): evaluates the S-parameter test cases in cases_path against a results CSV in
+ * DB format (as opem_write_csv writes it) and writes the sorted report to report_path.  The counts come back in
+ * the outputs: all cases passed when *npassed == *ncases (process3D's exit status 0). */
+int opem_process_tests(const char *cases_path, const char *results_csv, const char *report_path, int *ncases,
+                       int *nevaluated, int *npassed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,9 @@ SYMBOLS = [
     "opem_base_matrix", "opem_material_set", "opem_boundary_set", "opem_port_points", "opem_system_matrix",
     "opem_spmv", "opem_element_matrices", "opem_sweep", "opem_calculate_s", "opem_spmv_bench",
     "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench", "opem_spmv_noalloc",
-    "opem_lu_stats", "opem_solve", "opem_hfield", "opem_port_project",
+    "opem_lu_stats", "opem_solve", "opem_hfield", "opem_port_project", "opem_results_error", "opem_s2z",
+    "opem_z2s", "opem_renormalize", "opem_write_csv", "opem_write_touchstone", "opem_save_as_test",
+    "opem_process_tests",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -42,6 +44,11 @@ class SweepIn(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("restart", C.c_int), ("maxit", C.c_int), ("tol", C.c_double),
                 ("first_rung", C.c_int)]
+
+
+class SparamFile(C.Structure):
+    _fields_ = [("version", C.c_char_p), ("format", C.c_char_p), ("unit", C.c_char_p),
+                ("reference_impedance", C.c_double), ("nets", C.POINTER(C.c_char_p))]
 
 
 class SweepOut(C.Structure):
@@ -86,6 +93,16 @@ def lib():
         L.opem_krylov_bench.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
         L.opem_sweep_apply.argtypes = [P, C.c_int, C.c_int, f64p, f64p, f64p, f64p]
         L.opem_calculate_s.argtypes = [C.c_int, f64p, f64p, f64p, f64p, C.c_int, f64p]
+        L.opem_results_error.argtypes = []
+        L.opem_results_error.restype = C.c_char_p
+        L.opem_s2z.argtypes = [C.c_int, f64p, f64p, f64p]
+        L.opem_z2s.argtypes = [C.c_int, f64p, f64p, f64p]
+        L.opem_renormalize.argtypes = [C.c_int, f64p, f64p, C.c_double, C.c_double, f64p]
+        L.opem_write_csv.argtypes = [C.c_char_p, C.POINTER(SparamFile), C.c_int, C.c_int64, f64p, f64p]
+        L.opem_write_touchstone.argtypes = [C.c_char_p, C.POINTER(SparamFile), C.c_int, C.c_int64, f64p, f64p]
+        L.opem_save_as_test.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int64, f64p, f64p, C.POINTER(C.c_int)]
+        L.opem_process_tests.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
