@@ -23,7 +23,7 @@ import time
 import numpy as np
 
 from . import workloads
-from .solver import Solver
+from .solver import Solver, lu_stats
 
 OPTS = dict(restart=300, tol=1e-10, maxit=8000)
 
@@ -159,6 +159,35 @@ def c4lu(share=1, fp64_points=(0, 21, 42, 63)):
         s.close()
 
 
+def lu_pmc(nfreq=4, share=2):
+    """The sparse-LU kernels of the config-4 sweep at a counter-friendly size, for rocprofv3 --pmc passes: one mixed
+    sweep of the first `nfreq` frequencies of the 64-point plan (FP32 factors of the mean matrix of each `share`
+    adjacent frequencies) on a fresh context, then its opem_lu_stats totals (trailing-update flops, apply matrix
+    bytes, launch counts) as one JSON line, then two 1 GiB device-to-device copies whose FETCH_SIZE / WRITE_SIZE
+    calibrate the counters."""
+    import os
+
+    import torch
+
+    os.environ["OPEM_LU_TIMING"] = "1"  # launch counts and flops are kept only with event timing on
+    w = workloads.make("microstrip-c4")
+    s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+    try:
+        r = s.sweep(w["freqs"][:nfreq], precision="mixed", restart=30, tol=1e-10, maxit=600, precond="lu",
+                    precond_share=share, orth_passes=1)
+        st = lu_stats(s)
+        _emit({"lu_pmc": {"nfreq": nfreq, "share": share, "converged": int(r["converged"].sum()),
+                          "gemm_flops": st["gemm_flops"], "gemm_launches": st["gemm_launches"],
+                          "apply_bytes": st["apply_bytes"], "apply_launches": st["apply_launches"]}})
+    finally:
+        s.close()
+    a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda:0")
+    d = torch.empty_like(a)
+    for _ in range(2):
+        d.copy_(a)
+    torch.cuda.synchronize()
+
+
 def c5():
     tot_t, tot_pts, tot_f = 0.0, 0, 0
     for i, w in enumerate(workloads.c5_instances()):
@@ -226,7 +255,9 @@ def kernels(restart=300, ns=3):
 
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c3"
-    if which == "c4lu":
+    if which == "lu_pmc":
+        lu_pmc()
+    elif which == "c4lu":
         c4lu(share=int(sys.argv[2]) if len(sys.argv) > 2 else 1)
     else:
         {"c2": c2, "c3": c3, "c4": c4, "c5": c5, "kernels": kernels}[which]()
