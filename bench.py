@@ -5,7 +5,7 @@
 One step = one batched frequency sweep of the workload (default: the shielded-microstrip box at 0.48 M ND2 DOF,
 64 frequencies over 1-20 GHz x 2 port excitations = 128 systems on one pattern): every system solved to a true
 relative residual of 1e-10 with FP32 GMRES inside FP64 iterative refinement, preconditioned by a sparse LU factor
-(FP32, one per two adjacent frequencies), then the H projection, port projections and S extraction.  The plain
+(FP32, one per four adjacent frequencies), then the H projection, port projections and S extraction.  The plain
 FP64 GPU path (FP64 LU factor of every frequency, one frequency at a time, FP64 GMRES) then solves every frequency
 of the sweep; a frequency point counts towards the metric only when both paths converged and the mixed S is within
 1e-6 of the FP64 S.  One JSON line is printed.
@@ -42,9 +42,52 @@ def _pmc_traffic_ratio(kernel, shape):
     return None
 
 
-def _cpu_baseline(work):
-    """CPU baseline: not measured by this build of bench.py (see --no-cpu)."""
-    return None
+def _cpu_baseline(s, work, fi):
+    """CPU baseline: one frequency point of the workload (frequency index ``fi``, both excitations) solved on the host
+    cores by oracle/cpu_lu.cpp, the same algorithm in C++ with std::thread (nested-dissection multifrontal LU in
+    FP64, GMRES on the lifted, port-eliminated system) on the same matrix, with the Dirichlet values of the GPU's
+    FP64 solution of that point; the CPU solution is compared with the GPU's.  value = 1 / (numeric factorization +
+    port-root factorizations + solves) in freq-points/s; the per-mesh analysis (ordering, tree) is reported apart,
+    as the GPU's timed sweeps exclude it too."""
+    from openparem_amd.ports import provider_for
+    from oracle import cpu_baseline as cb
+
+    m = work["mesh"]
+    f = float(work["freqs"][fi])
+    mode_fn = provider_for(m)
+    gam = {a: mode_fn(m, a, f)["gamma"] for a in s.ports}
+    vals = s.system_matrix(f, gam)
+    r = s.sweep(np.array([f]), precision="fp64", batch=1, restart=30, tol=1e-12, maxit=600, precond="lu",
+                orth_passes=2, want_fields=True)
+    xg = r["x"][0]
+    sp_ = s.space()
+    bd = sp_["bface_dofs"]
+    pec_attrs = [a for a, k in m.bdr_kind.items() if k == "pec"]
+    n = s.ndof
+    excl = np.zeros(n, bool)
+    excl[np.unique(bd[np.isin(m.bface_attr, pec_attrs)])] = True
+    nport = np.zeros(n, np.int32)
+    group = np.full(n, -1, np.int32)
+    for e, a in enumerate(s.ports):
+        d = np.unique(bd[m.bface_attr == a])
+        nport[d] += 1
+        group[d] = e
+    excl |= nport > 1
+    group[excl] = -1
+    x, info = cb.port_solve(m.xyz, m.tets, work["order"], sp_["rowptr"], sp_["colind"], vals, excl, group, xg,
+                            tol=1e-10, restart=30, maxit=200)
+    dx = float(np.abs(x - xg).max() / np.abs(xg).max())
+    t = info["factor_s"] + info["root_factor_s"] + info["solve_s"]
+    return {"value": 1.0 / t, "unit": "freq-points/s", "cores": info["threads"], "kind": "port",
+            "sample": f"one frequency point of the workload ({f / 1e9:.4g} GHz, {len(s.ports)} port excitations, "
+                      f"{n} DOF): FP64 multifrontal LU factorization of A(w) (nested dissection, {info['flops']:.3g} "
+                      f"real flops), the port-root factor of each excitation and FP64 GMRES(30) to 1e-10 with that "
+                      f"factor, in C++ on {info['threads']} host threads (oracle/cpu_lu.cpp)",
+            "cpu_model": cb.cpu_model(), "nproc": os.cpu_count(),
+            "seconds": {"analysis": info["analysis_s"], "factor": info["factor_s"],
+                        "port_root_factors": info["root_factor_s"], "solve": info["solve_s"]},
+            "factor_gflops": info["flops"] / info["factor_s"] / 1e9, "gmres_iters": info["iters"],
+            "rel_res": info["rel_res"], "max_rel_dx_vs_gpu_fp64": dx}
 
 
 def _progress(msg):
@@ -57,28 +100,35 @@ MFMA_FP64_PEAK_TFLOPS = 78.6
 
 def _lu_roofline(st, workload):
     """Rooflines of the sparse-LU kernels over a timed region (HIP events on the context stream, from opem_lu_stats):
-    the trailing-update GEMM against the FP32 MFMA peak and the apply's factor products against HBM; the one that
-    took longer is the roofline kernel."""
-    gemm = {"kernel": "k_dense_gemm (trailing updates of the FP32 fronts on v_mfma_f32_16x16x4f32)", "bound": "mfma",
-            "achieved": st["gemm_flops"] / st["gemm_s"] / 1e12 if st["gemm_s"] else None,
-            "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "seconds": st["gemm_s"],
-            "launches": st["gemm_launches"], "flops": st["gemm_flops"]}
-    gemm["frac"] = gemm["achieved"] / MFMA_FP32_PEAK_TFLOPS if gemm["achieved"] else None
+    the trailing-update GEMMs (the 128-tile kernel of the deep FP32 updates and the 64-tile kernel of the rest)
+    against the FP32 MFMA peak and the apply's factor products against HBM; the one that took longest is the
+    roofline kernel.  Algorithmic flops are 8 m n k per complex update of an m x k by k x n block, whatever number of
+    real MFMA products the kernel spends on it."""
+    def mfma(name, key):
+        d = {"kernel": name, "bound": "mfma", "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+             "seconds": st[key + "_s"], "launches": st[key + "_launches"], "flops": st[key + "_flops"],
+             "achieved": st[key + "_flops"] / st[key + "_s"] / 1e12 if st[key + "_s"] else None}
+        d["frac"] = d["achieved"] / MFMA_FP32_PEAK_TFLOPS if d["achieved"] else None
+        return d
+    deep = mfma("k_dense_gemm_t128 (deep trailing updates of the FP32 fronts: 128x128 tiles, three real "
+                "v_mfma_f32_16x16x4f32 products per complex update)", "gemm128")
+    gemm = mfma("k_dense_gemm (the other trailing updates: 64x64 tiles, four real MFMA products per complex "
+                "update)", "gemm")
     apply_ = {"kernel": "k_lu_apply_part (the preconditioner apply's products with the FP32 factor blocks)",
               "bound": "hbm", "achieved": st["apply_bytes"] / st["apply_s"] / 1e9 if st["apply_s"] else None,
               "peak": HBM_PEAK_GBS, "unit": "GB/s", "seconds": st["apply_s"], "launches": st["apply_launches"],
               "bytes": st["apply_bytes"]}
     apply_["frac"] = apply_["achieved"] / HBM_PEAK_GBS if apply_["achieved"] else None
-    dom = gemm if st["gemm_s"] >= st["apply_s"] else apply_
-    kname = "k_dense_gemm" if dom is gemm else "k_lu_apply_part"
+    dom = max((deep, gemm, apply_), key=lambda d: d["seconds"])
+    kname = dom["kernel"].split()[0]
     ratio = _pmc_traffic_ratio(kname, workload)
-    work_per_launch = (dom["flops"] if dom is gemm else dom["bytes"]) / max(1.0, dom["launches"])
+    work_per_launch = (dom["bytes"] if dom is apply_ else dom["flops"]) / max(1.0, dom["launches"])
     roof = {"kernel": dom["kernel"], "bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
             "unit": dom["unit"], "frac": dom["frac"],
-            "traffic": None if ratio is None or dom is gemm else ratio * work_per_launch,
+            "traffic": None if ratio is None or dom is not apply_ else ratio * work_per_launch,
             "per_launch": work_per_launch, "launch_seconds": dom["seconds"] / max(1.0, dom["launches"]),
             "launches": dom["launches"]}
-    return roof, gemm, apply_
+    return roof, {"k_dense_gemm_t128": deep, "k_dense_gemm": gemm, "k_lu_apply_part": apply_}
 
 
 def _uneven(args):
@@ -90,7 +140,8 @@ def _uneven(args):
     from openparem_amd.solver import Solver, lu_stats
 
     keys = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",
-            "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies")
+            "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies", "gemm128_s", "gemm128_flops",
+            "gemm128_launches")
     agg = {k: 0.0 for k in keys}
     per, tot_t, tot64_t, tot_pass, tot_pts, tot_conv = [], 0.0, 0.0, 0, 0, 0
     for i, w in enumerate(workloads.c5_instances()):
@@ -129,7 +180,7 @@ def _uneven(args):
                       f"fp64 {dt64:.2f} s, {int(ok.sum())}/{len(f)} pass, max |dS| {dS.max():.2e}")
         finally:
             s.close()
-    roof, gemm, apply_ = _lu_roofline(agg, "uneven-c5")
+    roof, kern = _lu_roofline(agg, "uneven-c5")
     value = tot_pass / tot_t
     fp64_value = tot_pts / tot64_t
     line = {
@@ -146,7 +197,7 @@ def _uneven(args):
         "fp64_plain": {"value": fp64_value, "unit": "freq-points/s", "points": tot_pts},
         "speedup_vs_fp64_plain": value / fp64_value,
         "max_abs_dS_vs_fp64": max(p["max_abs_dS_vs_fp64"] for p in per),
-        "roofline": roof, "kernels": {"k_dense_gemm": gemm, "k_lu_apply_part": apply_},
+        "roofline": roof, "kernels": kern,
         "instances": per, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -162,7 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--restart", type=int, default=30)
-    ap.add_argument("--share", type=int, default=2, help="adjacent frequencies sharing one FP32 LU factor")
+    ap.add_argument("--share", type=int, default=4, help="adjacent frequencies sharing one FP32 LU factor")
     ap.add_argument("--tol", type=float, default=1e-10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -217,16 +268,17 @@ def main():
     csr = {(p_, g_): s.csr_spmv_bench(p_, row_group=g_, nrep=50) for p_ in ("fp64", "fp32") for g_ in (0, 1)}
     csr_n, csr_nnz = s.ndof, s.nnz
     its_mean = float(r["iters"].mean())
-    s.close()
     _progress("kernel timings done")
 
     cpu = None
     if not args.no_cpu:
-        cpu = _cpu_baseline(work)
-        _progress("CPU baseline done")
+        cpu = _cpu_baseline(s, work, len(freqs) // 2)
+        _progress(f"CPU baseline done: {1.0 / cpu['value']:.1f} s per point on {cpu['cores']} threads, "
+                  f"max |dx| vs GPU {cpu['max_rel_dx_vs_gpu_fp64']:.1e}")
+    s.close()
     asm = None if args.no_assembly else {f"nd{p_}": assembly_c2(p_, 1_000_000, HBM_PEAK_GBS) for p_ in (1, 2, 3)}
 
-    roof, gemm, apply_ = _lu_roofline(st, args.workload)
+    roof, kern = _lu_roofline(st, args.workload)
 
     line = {
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",
@@ -259,7 +311,7 @@ def main():
         "speedup_vs_fp64_plain": value / fp64_value if fp64_value else None,
         "max_abs_dS_vs_fp64": float(dS.max()),
         "roofline": roof,
-        "kernels": {"k_dense_gemm": gemm, "k_lu_apply_part": apply_,
+        "kernels": {**kern,
                     "k_dense_panel": {"seconds": st["panel_s"], "launches": st["panel_launches"]},
                     "factorizations": {"matrices": st["matrices_factored"], "wall_seconds": st["factor_wall_s"]},
                     "matrix_applies": st["matrix_applies"]},

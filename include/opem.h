@@ -247,10 +247,11 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out);
 
 /* Kernel timings of the sparse-LU preconditioner (precond 2) on this context, accumulated since the last reset:
  * the factorization's trailing updates, panels and the apply's matrix products, each timed with HIP events on the
- * context stream while the environment variable OPEM_LU_TIMING is set (counts only otherwise).  out[12]:
- *   [0] trailing-update seconds  [1] their real flops (8 per complex multiply-add)  [2] launches
- *   [3] panel seconds  [4] launches  [5] apply matrix-product seconds  [6] matrix bytes they read  [7] launches
- *   [8] factorization wall seconds  [9] matrices factored  [10] matrix-applies  [11] 1 when event timing is on.
+ * context stream while the environment variable OPEM_LU_TIMING is set (counts only otherwise).  out[15]:
+ *   [0] trailing-update seconds on the 64-tile kernel  [1] their real flops (8 per complex multiply-add)
+ *   [2] launches  [3] panel seconds  [4] launches  [5] apply matrix-product seconds  [6] matrix bytes they read
+ *   [7] launches  [8] factorization wall seconds  [9] matrices factored  [10] matrix-applies  [11] 1 when event
+ *   timing is on  [12] deep FP32 trailing-update seconds on the 128-tile kernel  [13] their real flops  [14] launches.
  * reset != 0 clears the counters after reading.  Counterpart: the PCFactor/MUMPS timing a PETSc -log_view
  * reports for the PCCHOLESKY/PCLU of solveComplexLinearSystem.c:133-168. */
 int opem_lu_stats(opem_ctx *ctx, int reset, double *out);

@@ -50,7 +50,9 @@ class LuFactor {
   // Accumulated kernel timings (HIP events, only while OPEM_LU_TIMING is set in the environment) and counts:
   // out[0..2] trailing-update seconds, real flops, launches; out[3..4] panel seconds, launches; out[5..7] apply
   // matrix-product seconds, matrix bytes read, launches; out[8] factorization wall seconds; out[9] matrices
-  // factored; out[10] matrix-applies; out[11] 1 when event timing is on.  reset clears them afterwards.
+  // factored; out[10] matrix-applies; out[11] 1 when event timing is on; out[12..14] seconds, real flops and
+  // launches of the deep FP32 trailing updates on the 128-tile kernel (out[0..2] count the 64-tile kernel only).
+  // reset clears them afterwards.
   void kernel_stats(double *out, bool reset);
 
   int prec() const { return prec_; }

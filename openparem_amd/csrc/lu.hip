@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <map>
 #include <tuple>
@@ -34,7 +35,7 @@ namespace opem {
 namespace {
 
 constexpr int kNB = 32;            // pivot columns per step of the blocked LU
-constexpr int kMaxRhsThread = 16;  // right-hand sides per thread in the apply kernels
+constexpr int kMaxRhsThread = 8;   // right-hand sides per thread in the apply kernels
 constexpr int kKC = 64;            // input rows staged in LDS per pass of the apply kernels
 
 // ---------------------------------------------------------------- complex helpers
@@ -416,8 +417,10 @@ __global__ __launch_bounds__(256) void k_dense_trsm(const int2 *__restrict__ tas
 // (prefix of tile counts), each block one kTile x kTile tile of its job, on MFMA (v_mfma_f32_16x16x4f32 or
 // v_mfma_f64_16x16x4f64).  Wave w owns columns 16w .. 16w+15 of the tile and all four 16-row blocks.  The MFMA
 // computes the transposed product (first operand B^T, second A^T), so D[i][j] = C[row j][col i]: consecutive
-// lanes own consecutive rows of the column-major C and its stores are 128-byte runs.  A complex product is four
-// real ones: Re += Br.Ar - Bi.Ai, Im += Bi.Ar + Br.Ai.  Operand layouts of the 16x16x4 MFMA: first operand
+// lanes own consecutive rows of the column-major C and its stores are 128-byte runs.  In FP64 a complex product is
+// four real ones: Re += Br.Ar - Bi.Ai, Im += Bi.Ar + Br.Ai; in FP32 three: P1 = Br.Ar, P2 = Bi.Ai and
+// P3 = (Br + Bi).(Ar + Ai) are accumulated and Re = P1 - P2, Im = P3 - P1 - P2 (the FP32 fronts only feed a
+// preconditioner).  Operand layouts of the 16x16x4 MFMA: first operand
 // P[i][k] in lane i + 16 k, second Q[k][j] in lane j + 16 k; D[i][j] in register i % 4 of lane j + 16 (i / 4)
 // (f32) or register i / 4 of lane j + 16 (i % 4) (f64).
 struct GemmJob {
@@ -475,59 +478,272 @@ __global__ __launch_bounds__(256) void k_dense_gemm(const GemmJob *__restrict__ 
   T *C = dense_ptr<T>(B, J.csel, J.coff, q);
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
-  V cr[4], ci[4];
+  V cr[4], ci[4], c3[4];
 #pragma unroll
   for (int rb = 0; rb < 4; rb++) {
     cr[rb] = V{0, 0, 0, 0};
     ci[rb] = V{0, 0, 0, 0};
+    c3[rb] = V{0, 0, 0, 0};
   }
-  for (int kc = 0; kc < J.k; kc += kNB) {  // K in slices of kNB columns; C is read and written once
+  // K in slices of kNB columns staged through registers: every load of the next slice is issued (addresses
+  // clamped into the blocks) before the current slice is multiplied, and the padding is masked at the LDS store
+  constexpr int kLd = kTile * kNB / 256;
+  T ra[kLd], rb_[kLd];
+  auto gload = [&](int kc) {
+    const int K = min(kNB, J.k - kc);
+#pragma unroll
+    for (int t = 0; t < kLd; t++) {
+      const int e = tid + 256 * t, rr = e % kTile, s = e / kTile;
+      ra[t] = A[min(r0 + rr, J.m - 1) + (int64_t)J.lda * (kc + min(s, K - 1))];
+    }
+#pragma unroll
+    for (int t = 0; t < kLd; t++) {
+      const int e = tid + 256 * t, s = e % kNB, cc = e / kNB;
+      rb_[t] = Bm[kc + min(s, K - 1) + (int64_t)J.ldb * min(c0 + cc, J.n - 1)];
+    }
+  };
+  // FP64 slices are loaded where they are stored (holding the next FP64 slice across the MFMAs spills registers)
+  constexpr bool prefetch = !M::f64;
+  if (prefetch) gload(0);
+  for (int kc = 0; kc < J.k; kc += kNB) {  // C is read and written once
     const int K = min(kNB, J.k - kc);
     if (kc > 0) __syncthreads();
-    for (int e = tid; e < kTile * kNB; e += 256) {
-      const int rr = e % kTile, s = e / kTile;
-      T v;
-      v.x = 0;
-      v.y = 0;
-      if (s < K && r0 + rr < J.m) v = A[(r0 + rr) + (int64_t)J.lda * (kc + s)];
-      As[s][rr] = v;
-    }
-    for (int e = tid; e < kTile * kNB; e += 256) {
-      const int s = e % kNB, cc = e / kNB;
-      T v;
-      v.x = 0;
-      v.y = 0;
-      if (s < K && c0 + cc < J.n) v = Bm[kc + s + (int64_t)J.ldb * (c0 + cc)];
-      Bs[s][cc] = v;
+    T zero;
+    zero.x = 0;
+    zero.y = 0;
+    if (prefetch) {
+#pragma unroll
+      for (int t = 0; t < kLd; t++) {
+        const int e = tid + 256 * t, rr = e % kTile, s = e / kTile;
+        As[s][rr] = (s < K && r0 + rr < J.m) ? ra[t] : zero;
+      }
+#pragma unroll
+      for (int t = 0; t < kLd; t++) {
+        const int e = tid + 256 * t, s = e % kNB, cc = e / kNB;
+        Bs[s][cc] = (s < K && c0 + cc < J.n) ? rb_[t] : zero;
+      }
+    } else {
+      for (int e = tid; e < kTile * kNB; e += 256) {
+        const int rr = e % kTile, s = e / kTile;
+        T v = zero;
+        if (s < K && r0 + rr < J.m) v = A[(r0 + rr) + (int64_t)J.lda * (kc + s)];
+        As[s][rr] = v;
+      }
+      for (int e = tid; e < kTile * kNB; e += 256) {
+        const int s = e % kNB, cc = e / kNB;
+        T v = zero;
+        if (s < K && c0 + cc < J.n) v = Bm[kc + s + (int64_t)J.ldb * (c0 + cc)];
+        Bs[s][cc] = v;
+      }
     }
     __syncthreads();
+    if (prefetch && kc + kNB < J.k) gload(kc + kNB);
     const int nk = (K + 3) >> 2;
     for (int kb = 0; kb < nk; kb++) {
       const T bv = Bs[4 * kb + lk][16 * w + li];
+      if constexpr (M::f64) {
 #pragma unroll
-      for (int rb = 0; rb < 4; rb++) {
-        const T av = As[4 * kb + lk][16 * rb + li];
-        cr[rb] = M::step(bv.x, av.x, cr[rb]);
-        cr[rb] = M::step(bv.y, -av.y, cr[rb]);
-        ci[rb] = M::step(bv.y, av.x, ci[rb]);
-        ci[rb] = M::step(bv.x, av.y, ci[rb]);
+        for (int rb = 0; rb < 4; rb++) {
+          const T av = As[4 * kb + lk][16 * rb + li];
+          cr[rb] = M::step(bv.x, av.x, cr[rb]);
+          cr[rb] = M::step(bv.y, -av.y, cr[rb]);
+          ci[rb] = M::step(bv.y, av.x, ci[rb]);
+          ci[rb] = M::step(bv.x, av.y, ci[rb]);
+        }
+      } else {  // three products: cr = Br.Ar, ci = Bi.Ai, c3 = (Br + Bi).(Ar + Ai)
+        const float bs = bv.x + bv.y;
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+          const T av = As[4 * kb + lk][16 * rb + li];
+          cr[rb] = M::step(bv.x, av.x, cr[rb]);
+          ci[rb] = M::step(bv.y, av.y, ci[rb]);
+          c3[rb] = M::step(bs, av.x + av.y, c3[rb]);
+        }
       }
     }
   }
+  if constexpr (!M::f64) {  // Re = P1 - P2, Im = P3 - P1 - P2
 #pragma unroll
-  for (int rb = 0; rb < 4; rb++) {
-    const int row = r0 + 16 * rb + li;
+    for (int rb = 0; rb < 4; rb++) {
+      const V p1 = cr[rb], p2 = ci[rb];
+      cr[rb] = p1 - p2;
+      ci[rb] = c3[rb] - p1 - p2;
+    }
+  }
+  // the values of C a lane updates are loaded together (clamped addresses; all 16 in FP32, 8 at a time in FP64),
+  // then the ones inside the job stored
+  constexpr int kHalf = M::f64 ? 2 : 4;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int col = c0 + 16 * w + (M::f64 ? 4 * r + lk : 4 * lk + r);
-      if (row < J.m && col < J.n) {
-        T *p = C + row + (int64_t)J.ldc * col;
-        T v = *p;
-        v.x -= cr[rb][r];
-        v.y -= ci[rb][r];
-        *p = v;
+  for (int h0 = 0; h0 < 4; h0 += kHalf) {
+    T cv[kHalf][4];
+#pragma unroll
+    for (int rb = 0; rb < kHalf; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = min(r0 + 16 * (h0 + rb) + li, J.m - 1);
+        const int col = min(c0 + 16 * w + (M::f64 ? 4 * r + lk : 4 * lk + r), J.n - 1);
+        cv[rb][r] = C[row + (int64_t)J.ldc * col];
+      }
+#pragma unroll
+    for (int rb = 0; rb < kHalf; rb++) {
+      const int row = r0 + 16 * (h0 + rb) + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = c0 + 16 * w + (M::f64 ? 4 * r + lk : 4 * lk + r);
+        if (row < J.m && col < J.n) {
+          T v = cv[rb][r];
+          v.x -= cr[h0 + rb][r];
+          v.y -= ci[h0 + rb][r];
+          C[row + (int64_t)J.ldc * col] = v;
+        }
       }
     }
+  }
+}
+
+// Deep trailing updates of FP32 fronts (C -= A B with k = the outer block): 128 x 128 tiles of C per 256-thread
+// block.  K is taken in slices of kBK staged through registers, so the next slice's global loads are in flight
+// while the current one is multiplied out of LDS.  Complex products use three real products instead of four:
+// with P1 = Br.Ar, P2 = Bi.Ai and P3 = (Br + Bi).(Ar + Ai) accumulated on v_mfma_f32_16x16x4f32, the product is
+// Re = P1 - P2, Im = P3 - P1 - P2.  As in k_dense_gemm the MFMA forms the transposed product (first operand B^T,
+// second A^T), so each store is a 128-byte run of a column of C.  Wave w owns rows 64 (w & 1) .. +63 and columns
+// 64 (w >> 1) .. +63 of the tile: 4 x 4 blocks of 16 x 16 with three accumulators each.  LDS rows are XOR-swizzled
+// (columns within and across 16-entry groups) so that the slice stores and the four k-rows one MFMA operand
+// load spans fall on different banks.  The flat block index (tiles x matrices) is remapped so that each of the
+// 8 XCDs receives a contiguous run of tiles, which share A row panels and B column panels in that XCD's L2.
+constexpr int kBT = 128;  // tile edge
+constexpr int kBK = 32;   // K slice
+
+__device__ __forceinline__ int swz_a(int s, int r) { return r ^ ((s & 3) << 4); }
+__device__ __forceinline__ int swz_b(int s, int c) { return c ^ (s & 15) ^ ((s & 3) << 4); }
+
+__global__ __launch_bounds__(256) void k_dense_gemm_t128(const GemmJob *__restrict__ jobs,
+                                                         const int64_t *__restrict__ prefix, int njobs,
+                                                         int64_t ntiles, int ninst, DenseBase B) {
+  __shared__ float2 As[kBK][kBT];
+  __shared__ float2 Bs[kBK][kBT];
+  // XCD-contiguous order of the flat (matrix, tile) index: hardware block b runs on XCD b % 8
+  const int64_t total = ntiles * ninst, b = blockIdx.x;
+  const int64_t qx = total / 8, rx = total % 8, xcd = b % 8;
+  const int64_t flat = xcd * qx + min(xcd, rx) + b / 8;
+  const int q = (int)(flat / ntiles);
+  const int64_t tile = flat % ntiles;
+  int lo = 0, hi = njobs - 1;  // last job with prefix <= tile
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= tile)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const GemmJob J = jobs[lo];
+  const int64_t local = tile - prefix[lo];
+  const int ntc = (J.n + kBT - 1) / kBT;
+  const int r0 = (int)(local / ntc) * kBT, c0 = (int)(local % ntc) * kBT;
+  const float2 *A = dense_ptr<float2>(B, J.asel, J.aoff, q);
+  const float2 *Bm = dense_ptr<float2>(B, J.bsel, J.boff, q);
+  float2 *C = dense_ptr<float2>(B, J.csel, J.coff, q);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+  lu_f4 p1[4][4], p2[4][4], p3[4][4];  // [column block][row block]
+#pragma unroll
+  for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+      p1[cb][rb] = lu_f4{0, 0, 0, 0};
+      p2[cb][rb] = lu_f4{0, 0, 0, 0};
+      p3[cb][rb] = lu_f4{0, 0, 0, 0};
+    }
+  float2 ra[16], rb_[16];
+  auto gload = [&](int kc) {
+    const int K = min(kBK, J.k - kc);
+    // every load is issued (addresses clamped into the blocks) and the padding is masked when the slice is stored
+    // to LDS, so the 32 loads of a slice stay in flight during the previous slice's MFMAs
+#pragma unroll
+    for (int t = 0; t < 16; t++) {  // A: 128 rows x kBK, consecutive threads on consecutive rows
+      const int e = tid + 256 * t, rr = e & (kBT - 1), s = e >> 7;
+      ra[t] = A[min(r0 + rr, J.m - 1) + (int64_t)J.lda * (kc + min(s, K - 1))];
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++) {  // B: kBK x 128 columns, consecutive threads on consecutive k
+      const int e = tid + 256 * t, s = e & (kBK - 1), cc = e >> 5;
+      rb_[t] = Bm[kc + min(s, K - 1) + (int64_t)J.ldb * min(c0 + cc, J.n - 1)];
+    }
+  };
+  gload(0);
+  for (int kc = 0; kc < J.k; kc += kBK) {
+    const int K = min(kBK, J.k - kc);
+    __syncthreads();  // the previous slice has been read
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int e = tid + 256 * t, rr = e & (kBT - 1), s = e >> 7;
+      const bool ok = s < K && r0 + rr < J.m;
+      As[s][swz_a(s, rr)] = ok ? ra[t] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int e = tid + 256 * t, s = e & (kBK - 1), cc = e >> 5;
+      const bool ok = s < K && c0 + cc < J.n;
+      Bs[s][swz_b(s, cc)] = ok ? rb_[t] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (kc + kBK < J.k) gload(kc + kBK);
+#pragma unroll 2
+    for (int kb = 0; kb < kBK / 4; kb++) {
+      const int s = 4 * kb + lk;
+      float2 av[4], bv[4];
+      float as[4], bs[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) {
+        av[rb] = As[s][swz_a(s, wr + 16 * rb + li)];
+        as[rb] = av[rb].x + av[rb].y;
+      }
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++) {
+        bv[cb] = Bs[s][swz_b(s, wc + 16 * cb + li)];
+        bs[cb] = bv[cb].x + bv[cb].y;
+      }
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+          p1[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[cb].x, av[rb].x, p1[cb][rb], 0, 0, 0);
+          p2[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[cb].y, av[rb].y, p2[cb][rb], 0, 0, 0);
+          p3[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bs[cb], as[rb], p3[cb][rb], 0, 0, 0);
+        }
+    }
+  }
+  // D[i][j] of block (cb, rb) = C[row r0 + wr + 16 rb + j][col c0 + wc + 16 cb + i]; lane l, register r: i = 4 (l / 16)
+  // + r, j = l % 16.  All 64 values of C a lane updates are loaded at once (clamped addresses), then the ones inside
+  // the job are stored.
+  float2 cv[4][4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = min(r0 + wr + 16 * rb + li, J.m - 1), col = min(c0 + wc + 16 * cb + 4 * lk + r, J.n - 1);
+        cv[rb][cb][r] = C[row + (int64_t)J.ldc * col];
+      }
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    const int row = r0 + wr + 16 * rb + li;
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = c0 + wc + 16 * cb + 4 * lk + r;
+        if (row < J.m && col < J.n) {
+          float2 v = cv[rb][cb][r];
+          const float re = p1[cb][rb][r] - p2[cb][rb][r];
+          const float im = p3[cb][rb][r] - p1[cb][rb][r] - p2[cb][rb][r];
+          v.x -= re;
+          v.y -= im;
+          C[row + (int64_t)J.ldc * col] = v;
+        }
+      }
   }
 }
 
@@ -643,10 +859,10 @@ struct RootDev {
 
 // Apply as a split-K product, deterministic: phase 1 (task = node or excitation, first row, first column, slot)
 // writes the partial sums of its rows over one chunk of kKCH columns to its slot; phase 2 (task = node or
-// excitation, first row, chunk count, first slot) adds a row tile's slots in chunk order and applies the output
-// rule.  Each thread owns one row and up to kMaxRhsThread right-hand sides (rhs j = j0 + rstep * m); matrix loads
-// are issued kUnroll at a time with clamped indices, so none is conditional; the inputs are staged kKC rows at a
-// time in LDS.
+// excitation, first row, chunk count, first slot) adds a row tile's slots (four interleaved running sums over the
+// chunks, in a fixed order) and applies the output rule.  In phase 1 each thread owns one row and up to
+// kMaxRhsThread right-hand sides (rhs j = j0 + rstep * m); matrix loads are issued kUnroll at a time with clamped
+// indices, so none is conditional; the inputs are staged kKC rows at a time in LDS.
 constexpr int kKCH = 64;
 constexpr int kUnroll = 16;
 
@@ -794,12 +1010,9 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
   const int4 tk = tasks[blockIdx.x];
   const int q = blockIdx.y;
   const int rows_wg = 256 / ty_n;
-  const int tx = threadIdx.x % rows_wg, ty = threadIdx.x / rows_wg;
   int t, e;
   int64_t rows, cols, ld, dbeg, nd, nf;
   apply_geom<KIND>(tk, D, Rt, rows, cols, ld, t, e, dbeg, nd, nf);
-  const int64_t i = tk.y + tx;
-  if (i >= rows) return;
   int j0 = 0, rstep = 1;
   if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
     j0 = e;
@@ -811,14 +1024,35 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
   double2 *y = yall + q * nrows_all, *tt = tall + q * nrows_all, *x = xall + q * nrows_all;
   const int32_t *O = (KIND == K_ROOT_L || KIND == K_ROOT_U) ? Rt.O + (int64_t)e * Rt.no_max : nullptr;
   const double2 *pq = part + q * part_qstride;
-  for (int m = ty; m < nsub; m += ty_n) {
+  const int nrow = (int)min((int64_t)rows_wg, rows - tk.y);
+  // one (row, right-hand side) pair per thread and pass, right-hand sides fastest, so consecutive threads read
+  // consecutive partial sums of a slot; the slots are summed four at a time with independent loads
+  for (int pi = threadIdx.x; pi < nrow * nsub; pi += 256) {
+    const int tx = pi / nsub, m = pi % nsub;
+    const int64_t i = tk.y + tx;
     const int j = j0 + m * rstep;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int c = 0; c < tk.z; c++) {
-      const double2 v = pq[((int64_t)(tk.w + c) * rows_wg + tx) * nrhs + j];
-      acc.x += v.x;
-      acc.y += v.y;
+    const double2 *src = pq + ((int64_t)tk.w * rows_wg + tx) * nrhs + j;
+    const int64_t cstride = (int64_t)rows_wg * nrhs;
+    double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+    int c = 0;
+    for (; c + 4 <= tk.z; c += 4) {
+      const double2 v0 = src[(c + 0) * cstride], v1 = src[(c + 1) * cstride];
+      const double2 v2 = src[(c + 2) * cstride], v3 = src[(c + 3) * cstride];
+      a0.x += v0.x;
+      a0.y += v0.y;
+      a1.x += v1.x;
+      a1.y += v1.y;
+      a2.x += v2.x;
+      a2.y += v2.y;
+      a3.x += v3.x;
+      a3.y += v3.y;
     }
+    for (; c < tk.z; c++) {
+      const double2 v = src[c * cstride];
+      a0.x += v.x;
+      a0.y += v.y;
+    }
+    const double2 acc = make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
     if (KIND == K_FWD_L) {
       y[(dbeg + i) * nrhs + j] = acc;
     } else if (KIND == K_FWD_L21) {
@@ -875,6 +1109,9 @@ struct LuFactor::Impl {
     int64_t panel_off, panel_n, lrows_off, lrows_n, laswp_off, laswp_n, trsm_off, trsm_n, gemm_off, gemm_n,
         gemm_tiles;
     double gemm_flops;  // real flops of the step's trailing updates, per matrix (8 per complex multiply-add)
+    int64_t gemm_tiles128;  // 128 x 128 tiles of the same jobs (prefix in h_gprefix128)
+    bool deep;              // the trailing update of an outer block (FP32 fronts may take the 128-tile kernel)
+    bool deep_fit;          // at least 80% of the 128-tile area is matrix (the default rule for taking it)
   };
   struct DLevel {
     std::vector<Step> lu, inv;
@@ -888,8 +1125,8 @@ struct LuFactor::Impl {
   std::vector<int2> h_laswp, h_trsm_tasks, h_lrows, h_cols;
   std::vector<TrsmJob> h_trsm;
   std::vector<GemmJob> h_gemm;
-  std::vector<int64_t> h_gprefix;
-  DevBuf d_dfr, d_list, d_laswp, d_trsm_tasks, d_trsm, d_gemm, d_gprefix, d_lrows, d_cols;
+  std::vector<int64_t> h_gprefix, h_gprefix128;
+  DevBuf d_dfr, d_list, d_laswp, d_trsm_tasks, d_trsm, d_gemm, d_gprefix, d_gprefix128, d_lrows, d_cols;
   int64_t xmax = 1, ppqs = 1;
   // numeric storage
   DevBuf fronts, pp, sw, zp, E, X, Y;
@@ -905,9 +1142,12 @@ struct LuFactor::Impl {
   DevBuf part;
   std::map<int, ApplyTasks> atasks;  // keyed by ty_n
   // kernel timing (OPEM_LU_TIMING set): HIP events on the stream around each trailing update, panel and apply
-  // launch, drained when the stream is synchronised; class 0 trailing updates (work = flops), 1 panels,
-  // 2 apply matrix products (work = matrix bytes read)
+  // launch, drained when the stream is synchronised; class 0 trailing updates on the 64-tile kernel (work =
+  // flops), 1 panels, 2 apply matrix products (work = matrix bytes read), 3 trailing updates on the 128-tile kernel
   bool timing = getenv("OPEM_LU_TIMING") != nullptr;
+  // deep trailing updates of FP32 fronts: OPEM_LU_DEEP_GEMM=0 keeps them on the 64-tile kernel, 2 puts every one
+  // on the 128-tile kernel, 1 (default) those whose jobs fill at least 85% of the 128-tile area
+  int deep_gemm = getenv("OPEM_LU_DEEP_GEMM") ? std::max(0, std::min(2, atoi(getenv("OPEM_LU_DEEP_GEMM")))) : 1;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   struct Pending {
@@ -916,7 +1156,7 @@ struct LuFactor::Impl {
     double work;
   };
   std::vector<Pending> pending;
-  double tsec[3] = {0, 0, 0}, twork[3] = {0, 0, 0}, tcount[3] = {0, 0, 0};
+  double tsec[4] = {0, 0, 0, 0}, twork[4] = {0, 0, 0, 0}, tcount[4] = {0, 0, 0, 0};
   double factor_wall = 0, factors = 0, solves = 0;
   hipEvent_t ev() {
     if (evused == evpool.size()) {
@@ -1012,10 +1252,13 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     for (int c0 = 0; c0 < dfr[id].nd; c0 += 16) h_cols.push_back(make_int2(id, c0));
   L.cols_n = (int64_t)h_cols.size() - L.cols_off;
   double *cur_flops = nullptr;  // the step being filled
+  int64_t *cur_tiles128 = nullptr;
   auto gemm_push = [&](const GemmJob &J, int64_t &tiles) {
     h_gprefix.push_back(tiles);
+    h_gprefix128.push_back(cur_tiles128 ? *cur_tiles128 : 0);
     h_gemm.push_back(J);
     tiles += (int64_t)((J.m + kTile - 1) / kTile) * ((J.n + kTile - 1) / kTile);
+    if (cur_tiles128) *cur_tiles128 += (int64_t)((J.m + kBT - 1) / kBT) * ((J.n + kBT - 1) / kBT);
     if (cur_flops) *cur_flops += 8.0 * J.m * J.n * J.k;
   };
   auto trsm_push = [&](const TrsmJob &J) {
@@ -1130,6 +1373,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
       S.gemm_tiles = 0;
       S.gemm_flops = 0;
       cur_flops = &S.gemm_flops;
+      cur_tiles128 = &S.gemm_tiles128;
+      S.deep = true;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
         if (f.nd <= ob0) continue;
@@ -1145,6 +1390,15 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
         G.coff = f.off + ob1 + (int64_t)f.nf * ob1;
         gemm_push(G, S.gemm_tiles);
       }
+      cur_tiles128 = nullptr;
+      // the 128-tile kernel where its padding wastes little: at least 85% of its tile area is matrix
+      double area = 0, area128 = 0;
+      for (int64_t g = S.gemm_off; g < (int64_t)h_gemm.size(); g++) {
+        const GemmJob &G = h_gemm[g];
+        area += (double)G.m * G.n;
+        area128 += (double)((G.m + kBT - 1) / kBT * kBT) * ((G.n + kBT - 1) / kBT * kBT);
+      }
+      S.deep_fit = area128 > 0 && area >= 0.85 * area128;
       close_step(S);
     }
   }
@@ -1330,6 +1584,7 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   I.h_trsm.clear();
   I.h_gemm.clear();
   I.h_gprefix.clear();
+  I.h_gprefix128.clear();
   I.h_lrows.clear();
   I.h_cols.clear();
   I.dlev.assign(L + 1, Impl::DLevel{});
@@ -1353,6 +1608,7 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   I.d_trsm.upload(I.h_trsm, st);
   I.d_gemm.upload(I.h_gemm, st);
   I.d_gprefix.upload(I.h_gprefix, st);
+  I.d_gprefix128.upload(I.h_gprefix128, st);
   I.d_lrows.upload(I.h_lrows, st);
   I.d_cols.upload(I.h_cols, st);
   I.atasks.clear();
@@ -1379,12 +1635,20 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
     if (S.trsm_n > 0)
       hipLaunchKernelGGL((k_dense_trsm<T>), dim3((unsigned)S.trsm_n, ninst), dim3(256), 0, st,
                          I.d_trsm_tasks.as<int2>() + S.trsm_off, I.d_trsm.as<TrsmJob>(), B);
-    if (S.gemm_tiles > 0)
-      I.timed(0, S.gemm_flops * ninst, [&] {
-        hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
-                           I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
-                           (int)S.gemm_n, B);
-      });
+    if (S.gemm_tiles > 0) {
+      if (std::is_same<T, float2>::value && S.deep && (I.deep_gemm == 2 || (I.deep_gemm == 1 && S.deep_fit)))
+        I.timed(3, S.gemm_flops * ninst, [&] {
+          hipLaunchKernelGGL(k_dense_gemm_t128, dim3((unsigned)(S.gemm_tiles128 * ninst)), dim3(256), 0, st,
+                             I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix128.as<int64_t>() + S.gemm_off,
+                             (int)S.gemm_n, S.gemm_tiles128, ninst, B);
+        });
+      else
+        I.timed(0, S.gemm_flops * ninst, [&] {
+          hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
+                             I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
+                             (int)S.gemm_n, B);
+        });
+    }
     LU_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL((k_dense_eye<T>), dim3((unsigned)L.cols_n, ninst), dim3(256), 0, st,
@@ -1491,12 +1755,15 @@ void LuFactor::factor(int prec, int ninst, const double *vals, int64_t vals_stri
 
 template <typename T, typename X>
 static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, const X *v, int64_t ldv, X *z) {
-  if (nrhs > 4 * kMaxRhsThread) throw std::invalid_argument("lu: at most 64 right-hand sides per matrix");
+  if (nrhs > 8 * kMaxRhsThread) throw std::invalid_argument("lu: at most 64 right-hand sides per matrix");
   hipStream_t st = I.st;
   const int64_t n = Tr.n, S = (int64_t)ninst * nrhs;
-  // right-hand sides per thread: 2, 4 or 16 (the template instance), rows per workgroup 256 / ty_n
+  // right-hand sides per thread: 2, 4 or 8 (the template instance), rows per workgroup 256 / ty_n
+  // (more right-hand sides per thread than 8 spill the accumulators to scratch); ty_n a power of two, so that row
+  // groups tile the workgroup
   const int maxr = nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : kMaxRhsThread;
-  const int ty_n = (nrhs + maxr - 1) / maxr;
+  int ty_n = 1;
+  while (ty_n * maxr < nrhs) ty_n *= 2;
   const int rows_wg = 256 / ty_n;
   const size_t vec = (size_t)ninst * std::max<int64_t>(n, 1) * nrhs * 16;
   I.b.alloc(vec);
@@ -1650,13 +1917,13 @@ void LuFactor::kernel_stats(double *out, bool reset) {
     HIPCHK(hipStreamSynchronize(I.st));
     I.drain();
   }
-  const double v[12] = {I.tsec[0], I.twork[0], I.tcount[0], I.tsec[1], I.tcount[1], I.tsec[2], I.twork[2], I.tcount[2],
-                        I.factor_wall, I.factors, I.solves, I.timing ? 1.0 : 0.0};
+  const double v[15] = {I.tsec[0], I.twork[0], I.tcount[0], I.tsec[1], I.tcount[1], I.tsec[2], I.twork[2], I.tcount[2],
+                        I.factor_wall, I.factors, I.solves, I.timing ? 1.0 : 0.0, I.tsec[3], I.twork[3], I.tcount[3]};
   if (out)
-    for (int k = 0; k < 12; k++) out[k] = v[k];
+    for (int k = 0; k < 15; k++) out[k] = v[k];
   if (!T_) return;
   if (reset) {
-    for (int c = 0; c < 3; c++) I.tsec[c] = I.twork[c] = I.tcount[c] = 0;
+    for (int c = 0; c < 4; c++) I.tsec[c] = I.twork[c] = I.tcount[c] = 0;
     I.factor_wall = I.factors = I.solves = 0;
   }
 }

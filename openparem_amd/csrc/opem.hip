@@ -2648,7 +2648,7 @@ int opem_lu_stats(opem_ctx *ctx, int reset, double *out) {
     if (!out && !reset) throw std::invalid_argument("bad arguments");
     if (!ctx->have_lu_tree) {  // nothing factored on this context yet: every counter is zero
       if (out)
-        for (int k = 0; k < 12; k++) out[k] = 0.0;
+        for (int k = 0; k < 15; k++) out[k] = 0.0;
       if (out) out[11] = getenv("OPEM_LU_TIMING") ? 1.0 : 0.0;
       return;
     }

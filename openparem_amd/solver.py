@@ -244,12 +244,13 @@ def hfield(ctx_holder, P, b, precision="fp64", restart=30, maxit=10000, tol=1e-1
 
 
 LU_STAT_KEYS = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",
-                "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies", "event_timing")
+                "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies", "event_timing",
+                "gemm128_s", "gemm128_flops", "gemm128_launches")
 
 
 def lu_stats(solver, reset=False):
     """opem_lu_stats: sparse-LU kernel timings accumulated on the solver's context (dict)."""
-    out = np.zeros(12, np.float64)
+    out = np.zeros(15, np.float64)
     check(lib().opem_lu_stats(solver.ctx, int(bool(reset)), ptr(out, C.c_double)), solver.ctx, "opem_lu_stats")
     return dict(zip(LU_STAT_KEYS, out.tolist()))
 

@@ -90,13 +90,33 @@ def test_lu_tee_and_cross(cross):
         assert np.abs(r["S"][i] - o["S"]).max() < 1e-9
 
 
+@pytest.mark.parametrize("share", [4, 8])
+def test_lu_cross_many_right_hand_sides(share):
+    """Four ports and FP32 factors shared by 4 or 8 close frequencies: 16 or 32 right-hand sides per factor, so the
+    apply runs with 2 or 4 row groups of 8 right-hand sides per thread.  S against the oracle's direct solve."""
+    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=True)
+    f = np.linspace(9.0e9, 9.35e9, 8)
+    s = _solver(m, 2)
+    try:
+        r = s.sweep(f, precision="mixed", tol=1e-11, batch=8, restart=30, precond="lu", orth_passes=1,
+                    precond_share=share)
+    finally:
+        s.close()
+    print(f"cross share {share}: iterations {r['iters'].tolist()}", flush=True)
+    assert r["converged"].all() and r["iters"].max() <= 40
+    pr = Problem(m, 2)
+    for i in (0, 3, 7):
+        o = pr.solve_point(f[i], {a: te10(m, a, f[i]) for a in pr.ports})
+        assert np.abs(r["S"][i] - o["S"]).max() < 1e-8
+
+
 @pytest.fixture(scope="module")
 def band():
     d = json.load(open(os.path.join(HERE, "golden", "microstrip_band.json")))
     return d, np.array(d["freqs"]), np.array([[complex(*v) for v in S] for S in d["S"]]).reshape(-1, 2, 2)
 
 
-@pytest.mark.parametrize("prec,share", [("fp64", 1), ("mixed", 1), ("mixed", 2)])
+@pytest.mark.parametrize("prec,share", [("fp64", 1), ("mixed", 1), ("mixed", 2), ("mixed", 4)])
 def test_lu_microstrip_band(band, prec, share):
     """The reduced microstrip box over 1-20 GHz (multimode upper band included) against the oracle's S fixture."""
     from openparem_amd.solver import Solver
@@ -115,3 +135,57 @@ def test_lu_microstrip_band(band, prec, share):
     assert dS.max() < (1e-9 if prec == "fp64" else 1e-8)
     if share == 1:
         assert r["iters"].max() <= (6 if prec == "fp64" else 30)
+
+
+DEEP_CHILD = r"""
+import json
+import numpy as np
+from openparem_amd.mesh import waveguide_mesh
+from openparem_amd.solver import Solver, lu_stats
+m = waveguide_mesh(nx=6, ny=3, nz=12, seed=7)
+f = np.array([9e9, 10e9, 11e9, 12e9])
+with Solver(m, 2) as s:
+    r = s.sweep(f, precision="mixed", tol=1e-11, batch=4, restart=30, precond="lu", orth_passes=1, precond_share=1)
+    st = lu_stats(s)
+print(json.dumps({"S": [[float(z.real), float(z.imag)] for z in r["S"].ravel()], "ndof": s.ndof,
+                  "converged": int(r["converged"].all()), "iters": r["iters"].tolist(),
+                  "deep_launches": st["gemm128_launches"], "small_launches": st["gemm_launches"]}), flush=True)
+"""
+
+
+def test_lu_deep_trailing_updates_128_tile_kernel():
+    """The 128 x 128 three-multiplication kernel of the deep FP32 trailing updates (k_dense_gemm_t128), forced onto
+    every deep update (OPEM_LU_DEEP_GEMM=2, edge tiles included) and switched off (=0), in child processes: each FP32
+    factor of a frequency's own matrix must bring GMRES to 1e-11 in a few iterations (a wrong Schur complement
+    would not), the iteration counts of the two kernels must agree, and S must equal the FP64 path's."""
+    import subprocess
+    import sys
+    root = os.path.dirname(HERE)
+    m = waveguide_mesh(nx=6, ny=3, nz=12, seed=7)
+    f = np.array([9e9, 10e9, 11e9, 12e9])
+    s = _solver(m, 2)
+    try:
+        r64 = s.sweep(f, precision="fp64", tol=1e-12, batch=1, restart=30, precond="lu")
+    finally:
+        s.close()
+    assert r64["converged"].all()
+    out = {}
+    for mode in ("0", "2"):
+        env = dict(os.environ, OPEM_LU_DEEP_GEMM=mode, OPEM_LU_TIMING="1")
+        env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        print(f"deep gemm mode {mode}", flush=True)
+        p = subprocess.run([sys.executable, "-c", DEEP_CHILD], cwd=root, env=env, capture_output=True, text=True,
+                           timeout=150)
+        assert p.returncode == 0, p.stderr[-2000:]
+        d = json.loads(p.stdout.strip().splitlines()[-1])
+        print(f"mode {mode}: ndof {d['ndof']} iters {d['iters']} deep launches {d['deep_launches']} "
+              f"64-tile launches {d['small_launches']}", flush=True)
+        S = np.array([complex(a, b) for a, b in d["S"]]).reshape(r64["S"].shape)
+        assert d["converged"] == 1
+        assert np.abs(S - r64["S"]).max() < 1e-8
+        assert max(max(it) for it in d["iters"]) <= 12
+        out[mode] = d
+    assert out["0"]["deep_launches"] == 0 and out["2"]["deep_launches"] > 0
+    it0 = np.array(out["0"]["iters"])
+    it2 = np.array(out["2"]["iters"])
+    assert np.abs(it0 - it2).max() <= 2
