@@ -863,7 +863,7 @@ struct RootDev {
 // chunks, in a fixed order) and applies the output rule.  In phase 1 each thread owns one row and up to
 // kMaxRhsThread right-hand sides (rhs j = j0 + rstep * m); matrix loads are issued kUnroll at a time with clamped
 // indices, so none is conditional; the inputs are staged kKC rows at a time in LDS.
-constexpr int kKCH = 64;
+constexpr int kKCH = 128;  // default columns per phase-1 chunk (OPEM_LU_KCH overrides: 32, 64, 256 or 512)
 constexpr int kUnroll = 16;
 
 template <int KIND>
@@ -905,7 +905,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
                                                        const double2 *__restrict__ tall,
                                                        const double2 *__restrict__ xall,
                                                        const double2 *__restrict__ yrall, double2 *__restrict__ part,
-                                                       int64_t part_qstride) {
+                                                       int64_t part_qstride, int kch) {
   extern __shared__ double2 sin[];  // kKC x nsub staged inputs
   const int4 tk = tasks[blockIdx.x];
   const int q = blockIdx.y;
@@ -937,7 +937,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
   const double2 *yr = yrall + (int64_t)q * Rt.no_max * nrhs;
   const int64_t i = tk.y + tx;
   const int64_t ic = i < rows ? i : rows - 1;  // clamped row: loads stay inside the block
-  const int64_t c_beg = tk.z, c_end = min((int64_t)tk.z + kKCH, cols);
+  const int64_t c_beg = tk.z, c_end = min((int64_t)tk.z + kch, cols);
   double2 acc[MAXR];
 #pragma unroll
   for (int m = 0; m < MAXR; m++) acc[m] = make_double2(0.0, 0.0);
@@ -1145,6 +1145,12 @@ struct LuFactor::Impl {
   // launch, drained when the stream is synchronised; class 0 trailing updates on the 64-tile kernel (work =
   // flops), 1 panels, 2 apply matrix products (work = matrix bytes read), 3 trailing updates on the 128-tile kernel
   bool timing = getenv("OPEM_LU_TIMING") != nullptr;
+  // columns per phase-1 chunk of the apply (the split-K depth; fewer chunks, fewer partial sums)
+  int kch = [] {
+    const char *e = getenv("OPEM_LU_KCH");
+    const int v = e ? atoi(e) : kKCH;
+    return v == 32 || v == 64 || v == 256 || v == 512 ? v : kKCH;
+  }();
   // deep trailing updates of FP32 fronts: OPEM_LU_DEEP_GEMM=0 keeps them on the 64-tile kernel, 2 puts every one
   // on the 128-tile kernel, 1 (default) those whose jobs fill at least 85% of the 128-tile area
   int deep_gemm = getenv("OPEM_LU_DEEP_GEMM") ? std::max(0, std::min(2, atoi(getenv("OPEM_LU_DEEP_GEMM")))) : 1;
@@ -1226,7 +1232,7 @@ size_t LuFactor::device_bytes_per_instance(int prec, int nrhs) const {
   double slots = 0;
   for (int l = 0; l < T.nlevels; l++) {
     double s2 = 0;
-    for (int t : I.level_nodes[l]) s2 += (double)(T.nf(t) / 64 + 1) * (T.nf(t) / kKCH + 1);
+    for (int t : I.level_nodes[l]) s2 += (double)(T.nf(t) / 64 + 1) * (T.nf(t) / I.kch + 1);
     slots = std::max(slots, s2);
   }
   b += slots * 256 * nrhs * 16;
@@ -1794,7 +1800,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
         if (tri == 2) k0 = r0;
         const int slot0 = slot;
         int nch = 0;
-        for (int64_t c0 = k0; c0 < k1; c0 += kKCH, nch++) A.all1.push_back(make_int4(id, (int)r0, (int)c0, slot++));
+        for (int64_t c0 = k0; c0 < k1; c0 += I.kch, nch++) A.all1.push_back(make_int4(id, (int)r0, (int)c0, slot++));
         A.all2.push_back(make_int4(id, (int)r0, nch, slot0));
       }
     };
@@ -1855,13 +1861,13 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
       const int4 *tk = A.d1.as<int4>() + a0;
       if (maxr == 2)
         hipLaunchKernelGGL((k_lu_apply_part<T, KIND, 2>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv,
-                           y, tt, x, yr, part, part_qstride);
+                           y, tt, x, yr, part, part_qstride, I.kch);
       else if (maxr == 4)
         hipLaunchKernelGGL((k_lu_apply_part<T, KIND, 4>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv,
-                           y, tt, x, yr, part, part_qstride);
+                           y, tt, x, yr, part, part_qstride, I.kch);
       else
         hipLaunchKernelGGL((k_lu_apply_part<T, KIND, kMaxRhsThread>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F,
-                           E, pp, fv, y, tt, x, yr, part, part_qstride);
+                           E, pp, fv, y, tt, x, yr, part, part_qstride, I.kch);
     });
     if (b1 > b0)
       hipLaunchKernelGGL((k_lu_apply_sum<KIND>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
