@@ -86,6 +86,12 @@ int opem_sweep_apply(opem_ctx *ctx, int prec, int nfreq, const double *freqs, co
 int opem_spmv(opem_ctx *ctx, int prec, int64_t n, const int32_t *rowptr, const int32_t *colind,
               const double *vals, const double *x, double *y);
 
+/* Bits of the flags output of opem_solve / opem_hfield. */
+#define OPEM_SOLVE_FP64_FALLBACK 1 /* mixed precision: the FP32 refinement stagnated (true residual down by less */
+                                   /* than 10% in two consecutive cycles) and FP64 GMRES finished the solve     */
+#define OPEM_SOLVE_SINGULAR 2      /* the sparse-LU factorization met exactly zero pivots (replaced by a tiny    */
+                                   /* value): the matrix is singular or nearly so; GMRES still decides           */
+
 /* Options of a caller-matrix Krylov solve (zero fields take the default). */
 typedef struct {
   int precision;   /* OPEM_PREC_FP64: FP64 factor and GMRES; OPEM_PREC_MIXED: FP32 factor and inner GMRES cycles   */
@@ -105,11 +111,11 @@ typedef struct {
  * to maxit ends the ladder (KSP_DIVERGED_ITS).  The pattern must be structurally symmetric, as MFEM assembles it.
  * x: in: the Dirichlet values on port_dof (other entries ignored); out: the solution.  rnorm: ||b - A x||_2 in
  * FP64 (the reference returns KSP's residual norm).  iters: GMRES iterations; converged: 1 when rnorm <= tol ||b||
- * (nonconvergence is a flag, not an error); rung: the ladder rung that ran last.  Any of the outputs but x may be
- * NULL. */
+ * (nonconvergence is a flag, not an error); rung: the ladder rung that ran last; flags: OPEM_SOLVE_* bits (FP64
+ * continuation after FP32 stagnation, zero pivots met).  Any of the outputs but x may be NULL. */
 int opem_solve(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
                int64_t nport, const int32_t *port_dof, const opem_solve_opts *opts, double *x, double *rnorm,
-               int32_t *iters, int32_t *converged, int32_t *rung);
+               int32_t *iters, int32_t *converged, int32_t *rung, int32_t *flags);
 
 /* solveHfield (solveComplexLinearSystem.c:235-315): P h = b for a caller's complex CSR P (the ND mass matrix of
  * build_P, fem3D.cpp:192-207; no boundary conditions) and b (buildHb: b = j Q x/(w mu), solveComplexLinearSystem.c
@@ -117,7 +123,7 @@ int opem_solve(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *c
  * h: the solution (n complex). */
 int opem_hfield(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
                 const double *b, const opem_solve_opts *opts, double *h, double *rnorm, int32_t *iters,
-                int32_t *converged, int32_t *rung);
+                int32_t *converged, int32_t *rung, int32_t *flags);
 
 /* Mode::calculateSplits (src/OpenParEM3D/port.cpp:3524-3568) for one port of the context's mesh: the modal E_t and
  * H_t sampled at the port's opem_port_points (complex npts*3 each) are projected onto the port's trace space, and
@@ -236,6 +242,14 @@ typedef struct {
   double seconds_setup;         /* coefficient combine + preconditioner build                     */
   double seconds_solve;         /* E solves                                                       */
   double seconds_post;          /* H projection + port projections + S                            */
+  int32_t *fallback;            /* optional [nfreq][nports]: 1 when the FP32 inner solves of a    */
+                                /*    mixed-precision system stagnated (its true residual fell by */
+                                /*    less than 10% in two consecutive refinement cycles) and FP64 */
+                                /*    GMRES finished it from its iterate (iters counts both)      */
+  int32_t *singular;            /* optional [nfreq][nports]: 1 when the sparse-LU factorization  */
+                                /*    of the system's batch met exactly zero pivots (replaced by a */
+                                /*    tiny value; the factor is then only a preconditioner of a   */
+                                /*    singular or near-singular matrix and GMRES decides)         */
 } opem_sweep_out;
 
 /* The per-frequency hot path for a list of frequencies and all port excitations:

@@ -55,7 +55,8 @@ class SweepOut(C.Structure):
     _fields_ = [("S", f64p), ("S_raw", f64p), ("Cp", f64p), ("Cm", f64p), ("iters", i32p),
                 ("true_res", f64p), ("converged", i32p), ("x_out", f64p), ("h_out", f64p),
                 ("seconds_total", C.c_double), ("seconds_setup", C.c_double),
-                ("seconds_solve", C.c_double), ("seconds_post", C.c_double)]
+                ("seconds_solve", C.c_double), ("seconds_post", C.c_double), ("fallback", i32p),
+                ("singular", i32p)]
 
 
 def lib():
@@ -81,9 +82,9 @@ def lib():
         L.opem_spmv_noalloc.argtypes = [P, C.c_int, C.c_int64, i32p, i32p, f64p, f64p, f64p]
         L.opem_lu_stats.argtypes = [P, C.c_int, f64p]
         L.opem_solve.argtypes = [P, C.c_int64, i32p, i32p, f64p, C.c_int64, i32p, C.POINTER(SolveOpts), f64p, f64p,
-                                 i32p, i32p, i32p]
+                                 i32p, i32p, i32p, i32p]
         L.opem_hfield.argtypes = [P, C.c_int64, i32p, i32p, f64p, f64p, C.POINTER(SolveOpts), f64p, f64p, i32p, i32p,
-                                  i32p]
+                                  i32p, i32p]
         L.opem_port_project.argtypes = [P, C.c_int, f64p, f64p, f64p, f64p, f64p]
         L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
                                             C.c_int, f64p, f64p]

@@ -881,10 +881,14 @@ struct BatchResult {
 // system, x system-interleaved, y system-major; fp64 selects double2 vectors, else float2.
 using BatchOp = std::function<void(bool fp64, const void *x, void *y, int mode, const int32_t *act)>;
 
+// only: systems to solve (nullptr: all; the others stay as they are and their outputs are not meaningful).
+// stagnant (FP32 inner solves only): systems whose true residual fell by less than 10% in each of two consecutive
+// refinement cycles are stopped and marked, for the caller to finish in FP64 (gmres_continue_fp64).
 template <typename X>
 static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, int R, int m, int maxit, double tol,
                         int passes, double inner_red, std::vector<int> &iters, std::vector<double> &relres,
-                        std::vector<int> &conv, const std::vector<double> &bnorm) {
+                        std::vector<int> &conv, const std::vector<double> &bnorm,
+                        const std::vector<uint8_t> *only = nullptr, std::vector<uint8_t> *stagnant = nullptr) {
   using P = typename VecT<X>::P;
   const Space &s = c.sp;
   hipStream_t st = c.st;
@@ -1052,6 +1056,12 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
   W.xil.alloc((size_t)S * n * 16);
   launch_transpose<double2>(st, n, S, W.x.as<double2>(), W.xil.as<double2>(), true);
   const double2 *x64 = W.xil.as<double2>();
+  // stagnation of the FP32 inner solves; OPEM_FORCE_STAGNATION marks every system after its first cycle (a host-
+  // forced condition for testing the FP64 continuation)
+  const bool force_stag = mixed && stagnant && getenv("OPEM_FORCE_STAGNATION") != nullptr;
+  std::vector<double> rn_prev(S, -1.0);
+  std::vector<int> slow(S, 0);
+  if (stagnant) stagnant->assign(S, 0);
   while (true) {
     // true residual in FP64: r = mask .* (-A x)
     op(true, x64, W.r.as<double2>(), 1, nullptr);
@@ -1063,8 +1073,16 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
     for (int i = 0; i < S; i++) {
       const double rn = std::sqrt(rn2[i]);
       relres[i] = bnorm[i] > 0 ? rn / bnorm[i] : 0.0;
-      const bool done = rn <= tol * bnorm[i] || iters[i] >= maxit;
+      bool done = rn <= tol * bnorm[i] || iters[i] >= maxit || (only && !(*only)[i]);
       conv[i] = rn <= tol * bnorm[i];
+      if (mixed && stagnant && !done && rn_prev[i] >= 0.0) {
+        slow[i] = rn > 0.9 * rn_prev[i] ? slow[i] + 1 : 0;
+        if (slow[i] >= 2 || force_stag) {
+          (*stagnant)[i] = 1;
+          done = true;
+        }
+      }
+      rn_prev[i] = rn;
       act[i] = !done;
       any |= !done;
       // inner target: the outer tolerance, or inner_red of the current residual in FP32
@@ -1510,6 +1528,38 @@ static void calculate_s_host(int N, const cplx *Cp, const cplx *Cm, const cplx *
   }
 }
 
+// Mixed-precision batched solve with the FP64 continuation: FP32 inner cycles inside FP64 refinement; systems whose
+// refinement stagnates are then finished by FP64 GMRES from their current iterate (the same preconditioner applied
+// to FP64 vectors), with the iteration budget that is left.  fallback[i] = 1 for those systems; their iteration
+// count includes both phases.
+static void gmres_mixed_with_fallback(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, int R, int m, int maxit,
+                                      double tol, int passes, double inner_red, std::vector<int> &iters,
+                                      std::vector<double> &relres, std::vector<int> &conv,
+                                      const std::vector<double> &bnorm, std::vector<uint8_t> &fallback) {
+  std::vector<uint8_t> stag;
+  gmres_batch<float2>(c, W, op, n, S, R, m, maxit, tol, passes, inner_red, iters, relres, conv, bnorm, nullptr, &stag);
+  fallback.assign(S, 0);
+  int used = 0;
+  bool any = false;
+  for (int i = 0; i < S; i++)
+    if (stag[i]) {
+      any = true;
+      used = std::max(used, iters[i]);
+    }
+  if (!any) return;
+  std::vector<int> it2, cv2;
+  std::vector<double> rel2;
+  gmres_batch<double2>(c, W, op, n, S, R, m, std::max(1, maxit - used), tol, 2, inner_red, it2, rel2, cv2, bnorm,
+                       &stag, nullptr);
+  for (int i = 0; i < S; i++)
+    if (stag[i]) {
+      iters[i] += it2[i];
+      relres[i] = rel2[i];
+      conv[i] = cv2[i];
+      fallback[i] = 1;
+    }
+}
+
 static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   if (!c.have_space) throw std::invalid_argument("space not built");
   const Space &s = c.sp;
@@ -1933,6 +1983,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       else
         sweep_spmv<float2>(c, W, S, R, (const float2 *)x, (float2 *)y, mode, act);
     };
+    int64_t lu_zero_batch = 0;  // exactly zero pivots the factorization of this batch replaced
     if (use_lu) {
       // one factor per group of lu_share adjacent frequencies (their mean matrix when lu_share > 1)
       const int nq = F / lu_share;
@@ -1945,6 +1996,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       LuFactorStats lst;
       c.lu.factor(mixed ? 1 : 0, nq, vals, nnz, &lst);
       lu_stats_zero += lst.zero_pivots;
+      lu_zero_batch = lst.zero_pivots;
       lu_seconds += lst.seconds_factor;
       c.lu_pc = &c.lu;
       c.lu_nrhs = lu_share * R;
@@ -1953,12 +2005,14 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
       Ctx &c;
       ~LuReset() { c.lu_pc = nullptr; }
     } lu_reset{c};
+    std::vector<uint8_t> fallback(S, 0);
     if (mixed)
-      gmres_batch<float2>(c, W, op, n, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red, iters,
-                          relres, conv, bn);
+      gmres_mixed_with_fallback(c, W, op, n, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red,
+                                iters, relres, conv, bn, fallback);
     else
       gmres_batch<double2>(c, W, op, n, S, R, m, in->maxit > 0 ? in->maxit : 20000, tol, passes, inner_red, iters,
                            relres, conv, bn);
+    const int batch_singular = use_lu && lu_zero_batch > 0;
     if (warm && f0 + F < nf) {  // keep this stage's solutions for the later ones
       xstash.alloc((size_t)nf * R * n * 16);
       for (int fi = 0; fi < F; fi++)
@@ -2042,6 +2096,8 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
         if (out->iters) out->iters[(size_t)fg * np + r] = iters[sidx];
         if (out->true_res) out->true_res[(size_t)fg * np + r] = relres[sidx];
         if (out->converged) out->converged[(size_t)fg * np + r] = conv[sidx];
+        if (out->fallback) out->fallback[(size_t)fg * np + r] = fallback[sidx];
+        if (out->singular) out->singular[(size_t)fg * np + r] = batch_singular;
       }
     }
     for (int fi = 0; fi < F; fi++) {
@@ -2096,7 +2152,8 @@ struct CallerSolve {
 
 static void solve_caller(Ctx &c, int64_t n, const int32_t *rp, const int32_t *ci, const double *vals, int64_t ndir,
                          const int32_t *dir, const double *b, const CallerSolve &o, double *x, double *rnorm,
-                         int32_t *iters, int32_t *converged, int32_t *rung_used) {
+                         int32_t *iters, int32_t *converged, int32_t *rung_used, int32_t *flags) {
+  if (flags) *flags = 0;
   hipStream_t st = c.st;
   const int64_t nnz = rp[n];
   const bool mixed = o.precision == OPEM_PREC_MIXED;
@@ -2171,6 +2228,7 @@ static void solve_caller(Ctx &c, int64_t n, const int32_t *rp, const int32_t *ci
       c.solve_lu.factor(mixed ? 1 : 0, 1, dv.as<double>(), nnz, &fs);
       c.lu_pc = &c.solve_lu;
       c.lu_nrhs = 1;
+      if (flags && fs.zero_pivots > 0) *flags |= OPEM_SOLVE_SINGULAR;
     } else if (rung == 1) {
       c.d_jacinv.alloc((size_t)n * 16);
       hipLaunchKernelGGL(k_diag_inverse, dim3(blocks_for(n, 4096)), dim3(256), 0, st, n, drp.as<int32_t>(),
@@ -2184,10 +2242,13 @@ static void solve_caller(Ctx &c, int64_t n, const int32_t *rp, const int32_t *ci
         c.pc_override = 0;
       }
     } reset{c};
-    if (mixed)
-      gmres_batch<float2>(c, W, op, n, 1, 1, o.restart, o.maxit, o.tol, 1, 1e-5, it, rel, cv, bn);
-    else
+    if (mixed) {
+      std::vector<uint8_t> fb;
+      gmres_mixed_with_fallback(c, W, op, n, 1, 1, o.restart, o.maxit, o.tol, 1, 1e-5, it, rel, cv, bn, fb);
+      if (flags && fb[0]) *flags |= OPEM_SOLVE_FP64_FALLBACK;
+    } else {
       gmres_batch<double2>(c, W, op, n, 1, 1, o.restart, o.maxit, o.tol, 2, 1e-5, it, rel, cv, bn);
+    }
     if (cv[0] || std::isfinite(rel[0])) break;  // converged, or the rung ran to the iteration limit
   }
   HIPCHK(hipMemcpyAsync(x, W.x.p, (size_t)n * 16, hipMemcpyDeviceToHost, st));
@@ -2553,7 +2614,7 @@ static void check_solve_opts(const opem_solve_opts *o, CallerSolve &cs) {
 
 int opem_solve(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
                int64_t nport, const int32_t *port_dof, const opem_solve_opts *opts, double *x, double *rnorm,
-               int32_t *iters, int32_t *converged, int32_t *rung) {
+               int32_t *iters, int32_t *converged, int32_t *rung, int32_t *flags) {
   return guarded(ctx, [&] {
     if (n <= 0 || !rowptr || !colind || !vals || !x || nport < 0 || (nport > 0 && !port_dof))
       throw std::invalid_argument("bad arguments");
@@ -2562,20 +2623,21 @@ int opem_solve(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *c
       if (port_dof[k] < 0 || port_dof[k] >= n) throw std::invalid_argument("port DOF out of range");
     CallerSolve cs;
     check_solve_opts(opts, cs);
-    solve_caller(*ctx, n, rowptr, colind, vals, nport, port_dof, nullptr, cs, x, rnorm, iters, converged, rung);
+    solve_caller(*ctx, n, rowptr, colind, vals, nport, port_dof, nullptr, cs, x, rnorm, iters, converged, rung,
+                 flags);
   });
 }
 
 int opem_hfield(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
                 const double *b, const opem_solve_opts *opts, double *h, double *rnorm, int32_t *iters,
-                int32_t *converged, int32_t *rung) {
+                int32_t *converged, int32_t *rung, int32_t *flags) {
   return guarded(ctx, [&] {
     if (n <= 0 || !rowptr || !colind || !vals || !b || !h) throw std::invalid_argument("bad arguments");
     check_caller_csr(n, rowptr, colind);
     CallerSolve cs;
     check_solve_opts(opts, cs);
     std::vector<double> hx((size_t)2 * n, 0.0);
-    solve_caller(*ctx, n, rowptr, colind, vals, 0, nullptr, b, cs, hx.data(), rnorm, iters, converged, rung);
+    solve_caller(*ctx, n, rowptr, colind, vals, 0, nullptr, b, cs, hx.data(), rnorm, iters, converged, rung, flags);
     std::memcpy(h, hx.data(), hx.size() * 8);
   });
 }

@@ -157,8 +157,9 @@ class Solver:
         matrix);
         ``arnoldi_step`` Krylov vectors per Gram-Schmidt pass; ``warm_start`` bisection levels over the frequency
         index (interpolated starting iterates).  ``mode_fn(mesh, attr, f)`` supplies each port's modal E, H, gamma,
-        V, Z (default: the mesh family's provider).  Returns S, S_raw, Cp, Cm, iters, true_res, converged and,
-        with ``want_fields``, the E and H DOFs x, h [freq][excitation][ndof]."""
+        V, Z (default: the mesh family's provider).  Returns S, S_raw, Cp, Cm, iters, true_res, converged,
+        fallback (FP64 GMRES finished a system whose FP32 refinement stagnated), singular (the system's LU batch met
+        exactly zero pivots) and, with ``want_fields``, the E and H DOFs x, h [freq][excitation][ndof]."""
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
         nf, np_ = freqs.size, len(self.ports)
         mode_fn = mode_fn or provider_for(self.mesh)
@@ -193,20 +194,25 @@ class Solver:
         it = np.zeros((nf, np_), np.int32)
         res = np.zeros((nf, np_), np.float64)
         cv = np.zeros((nf, np_), np.int32)
+        fb = np.zeros((nf, np_), np.int32)
+        sg = np.zeros((nf, np_), np.int32)
         x = np.zeros((nf, np_, self.ndof), np.complex128) if want_fields else None
         h = np.zeros((nf, np_, self.ndof), np.complex128) if want_fields else None
         fv = lambda a: ptr(a.view(np.float64), C.c_double) if a is not None else ptr(None, C.c_double)
         sout = _lib.SweepOut(S=fv(S), S_raw=fv(Sr), Cp=fv(Cp), Cm=fv(Cm), iters=ptr(it, C.c_int32),
-                             true_res=ptr(res, C.c_double), converged=ptr(cv, C.c_int32), x_out=fv(x), h_out=fv(h))
+                             true_res=ptr(res, C.c_double), converged=ptr(cv, C.c_int32), x_out=fv(x), h_out=fv(h),
+                             fallback=ptr(fb, C.c_int32), singular=ptr(sg, C.c_int32))
         check(lib().opem_sweep(self.ctx, C.byref(sin), C.byref(sout)), self.ctx, "opem_sweep")
-        return dict(S=S, S_raw=Sr, Cp=Cp, Cm=Cm, iters=it, true_res=res, converged=cv, x=x, h=h,
+        return dict(S=S, S_raw=Sr, Cp=Cp, Cm=Cm, iters=it, true_res=res, converged=cv, x=x, h=h, fallback=fb,
+                    singular=sg,
                     seconds=dict(total=sout.seconds_total, setup=sout.seconds_setup, solve=sout.seconds_solve,
                                  post=sout.seconds_post))
 
 
 def solve(ctx_holder, A, port_dof, x, precision="fp64", restart=30, maxit=10000, tol=1e-10, first_rung=0):
     """opem_solve on a scipy CSR matrix A (complex): x carries the Dirichlet values on port_dof.  Returns the
-    solution and (rnorm, iters, converged, rung)."""
+    solution and (rnorm, iters, converged, rung, flags) with flags the OPEM_SOLVE_* bits (1: FP64 continuation after
+    FP32 stagnation, 2: zero pivots met)."""
     A = A.tocsr()
     A.sort_indices()
     n = A.shape[0]
@@ -217,15 +223,16 @@ def solve(ctx_holder, A, port_dof, x, precision="fp64", restart=30, maxit=10000,
     xx = np.ascontiguousarray(x, np.complex128).copy()
     o = _lib.SolveOpts(precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64, restart=int(restart),
                        maxit=int(maxit), tol=float(tol), first_rung=int(first_rung))
-    rn, it, cv, rg = C.c_double(), C.c_int32(), C.c_int32(), C.c_int32()
+    rn, it, cv, rg, fl = C.c_double(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     check(lib().opem_solve(ctx_holder.ctx, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
                            pdof.size, ptr(pdof, C.c_int32), C.byref(o), ptr(xx.view(np.float64), C.c_double),
-                           C.byref(rn), C.byref(it), C.byref(cv), C.byref(rg)), ctx_holder.ctx, "opem_solve")
-    return xx, (rn.value, it.value, cv.value, rg.value)
+                           C.byref(rn), C.byref(it), C.byref(cv), C.byref(rg), C.byref(fl)), ctx_holder.ctx,
+          "opem_solve")
+    return xx, (rn.value, it.value, cv.value, rg.value, fl.value)
 
 
 def hfield(ctx_holder, P, b, precision="fp64", restart=30, maxit=10000, tol=1e-10, first_rung=0):
-    """opem_hfield: P h = b for a scipy CSR P.  Returns h and (rnorm, iters, converged, rung)."""
+    """opem_hfield: P h = b for a scipy CSR P.  Returns h and (rnorm, iters, converged, rung, flags)."""
     P = P.tocsr().astype(np.complex128)
     P.sort_indices()
     n = P.shape[0]
@@ -236,11 +243,11 @@ def hfield(ctx_holder, P, b, precision="fp64", restart=30, maxit=10000, tol=1e-1
     h = np.zeros(n, np.complex128)
     o = _lib.SolveOpts(precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64, restart=int(restart),
                        maxit=int(maxit), tol=float(tol), first_rung=int(first_rung))
-    rn, it, cv, rg = C.c_double(), C.c_int32(), C.c_int32(), C.c_int32()
+    rn, it, cv, rg, fl = C.c_double(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     check(lib().opem_hfield(ctx_holder.ctx, n, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
                             ptr(bb, C.c_double), C.byref(o), ptr(h.view(np.float64), C.c_double), C.byref(rn),
-                            C.byref(it), C.byref(cv), C.byref(rg)), ctx_holder.ctx, "opem_hfield")
-    return h, (rn.value, it.value, cv.value, rg.value)
+                            C.byref(it), C.byref(cv), C.byref(rg), C.byref(fl)), ctx_holder.ctx, "opem_hfield")
+    return h, (rn.value, it.value, cv.value, rg.value, fl.value)
 
 
 LU_STAT_KEYS = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",

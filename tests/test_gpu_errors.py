@@ -171,7 +171,7 @@ def test_solve_entries_reject_misuse():
         v = np.zeros(2 * n, np.float64)
         v[0::2] = 2.0  # 2 I
         x = np.full(2 * n, 5.0)
-        outs = (C.c_double(), C.c_int32(), C.c_int32(), C.c_int32())
+        outs = (C.c_double(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32())
         refs = [C.byref(o) for o in outs]
 
         def solve(port, opts):
