@@ -1001,6 +1001,137 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
   }
 }
 
+// Phase 1 on MFMA for FP32 factor blocks and at most 16 right-hand sides per launch row group (ty_n == 1): the
+// same tasks, partial-sum slots and output as k_lu_apply_part.  Wave w takes rows 64 w .. 64 w + 63 of the task's
+// 256 rows as four 16-row blocks; each 16 x 4 block of the factor is the first operand of v_mfma_f32_16x16x4f32
+// (row i in lane i + 16 k), the staged inputs (4 columns x 16 right-hand sides, FP32, zero past nsub) the second;
+// complex products by the three-product form (P1 = Ar.Xr, P2 = Ai.Xi, P3 = (Ar + Ai)(Xr + Xi); Re = P1 - P2,
+// Im = P3 - P1 - P2).  Sums over a chunk are FP32 (the factor and the inner Krylov vectors are FP32); the slots are
+// added in FP64 by k_lu_apply_sum.  The triangular blocks are masked per element (unit lower: k < i, diagonal 1;
+// upper: k >= i).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
+                                                       const float2 *__restrict__ Fall,
+                                                       const float2 *__restrict__ Eall,
+                                                       const int32_t *__restrict__ ppall,
+                                                       const double2 *__restrict__ fvall,
+                                                       const double2 *__restrict__ yall,
+                                                       const double2 *__restrict__ tall,
+                                                       const double2 *__restrict__ xall,
+                                                       const double2 *__restrict__ yrall, double2 *__restrict__ part,
+                                                       int64_t part_qstride, int kch) {
+  __shared__ float2 sX[kKC][16];  // staged inputs: column k of the chunk, right-hand side m
+  const int4 tk = tasks[blockIdx.x];
+  const int q = blockIdx.y;
+  int t, e;
+  int64_t rows, cols, ld, dbeg, nd, nf;
+  apply_geom<KIND>(tk, D, Rt, rows, cols, ld, t, e, dbeg, nd, nf);
+  int j0 = 0, rstep = 1;
+  if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
+    j0 = e;
+    rstep = Rt.nexc;
+  }
+  const int nsub = (nrhs - j0 + rstep - 1) / rstep;
+  const float2 *M;
+  if (KIND == K_ROOT_L || KIND == K_ROOT_U) {
+    M = Eall + ((int64_t)q * Rt.nexc + e) * Rt.estride;
+  } else {
+    const float2 *F = Fall + q * D.fstride + D.foff[t];
+    M = KIND == K_FWD_L21 ? F + nd : KIND == K_BWD_U12 ? F + nf * nd : F;
+  }
+  const int64_t nrows_all = D.n * nrhs;
+  const double2 *fv = fvall + q * D.fvstride;
+  const double2 *y = yall + q * nrows_all, *tt = tall + q * nrows_all, *x = xall + q * nrows_all;
+  const int32_t *pp = ppall + q * D.ppqs + dbeg;
+  const int32_t *rpp = (KIND == K_ROOT_L) ? Rt.rpp + q * D.ppqs + (int64_t)e * Rt.no_max : nullptr;
+  const int32_t *O = (KIND == K_ROOT_L || KIND == K_ROOT_U) ? Rt.O + (int64_t)e * Rt.no_max : nullptr;
+  const double2 *yr = yrall + (int64_t)q * Rt.no_max * nrhs;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int64_t rbase = tk.y + 64 * w;  // first row of this wave
+  const int64_t c_beg = tk.z, c_end = min((int64_t)tk.z + kch, cols);
+  lu_f4 p1[4], p2[4], p3[4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    p1[rb] = lu_f4{0, 0, 0, 0};
+    p2[rb] = lu_f4{0, 0, 0, 0};
+    p3[rb] = lu_f4{0, 0, 0, 0};
+  }
+  // the rows this lane's first-operand values come from (clamped, so every load stays inside the block)
+  int64_t arow[4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) arow[rb] = min(rbase + 16 * rb + li, rows - 1);
+  for (int64_t kc = c_beg; kc < c_end; kc += kKC) {
+    const int kn = (int)min((int64_t)kKC, c_end - kc);
+    for (int s = threadIdx.x; s < kKC * 16; s += 256) {
+      const int kk = s >> 4, m = s & 15;
+      float2 v = make_float2(0.f, 0.f);
+      if (kk < kn && m < nsub) {
+        const int j = j0 + m * rstep;
+        const int64_t k = kc + kk;
+        double2 dv;
+        if (KIND == K_FWD_L)
+          dv = fv[(D.fvoff[t] + pp[k]) * nrhs + j];
+        else if (KIND == K_FWD_L21)
+          dv = y[(dbeg + k) * nrhs + j];
+        else if (KIND == K_BWD_U12)
+          dv = x[(int64_t)D.uidx[D.uoff[t] + k] * nrhs + j];
+        else if (KIND == K_BWD_U)
+          dv = tt[(dbeg + k) * nrhs + j];
+        else if (KIND == K_ROOT_L)
+          dv = fv[(D.fvoff[t] + O[rpp[k]]) * nrhs + j];
+        else
+          dv = yr[k * nrhs + j];
+        v = make_float2((float)dv.x, (float)dv.y);
+      }
+      sX[kk][m] = v;
+    }
+    __syncthreads();
+    const int nk4 = (kn + 3) >> 2;
+#pragma unroll 4
+    for (int kb = 0; kb < nk4; kb++) {
+      const int kk = 4 * kb + lk;  // this lane's column of the 16 x 4 block (and row of the 4 x 16 one)
+      const int64_t kg = kc + min(kk, kn - 1);
+      const float2 xv = sX[kk][li];
+      const float xs = xv.x + xv.y;
+      float2 av[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) av[rb] = M[arow[rb] + ld * kg];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) {
+        const int64_t ig = rbase + 16 * rb + li;
+        float2 a = av[rb];
+        bool use = kk < kn;
+        if (KIND == K_FWD_L || KIND == K_ROOT_L) {
+          if (kg == ig) a = make_float2(1.f, 0.f);
+          use = use && kg <= ig;
+        } else if (KIND == K_BWD_U || KIND == K_ROOT_U) {
+          use = use && kg >= ig;
+        }
+        if (!use) a = make_float2(0.f, 0.f);
+        p1[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, xv.x, p1[rb], 0, 0, 0);
+        p2[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, xv.y, p2[rb], 0, 0, 0);
+        p3[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x + a.y, xs, p3[rb], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D[i][m] of row block rb: lane l, register r: i = 4 (l / 16) + r, m = l % 16
+  double2 *pq = part + q * part_qstride + (int64_t)tk.w * 256 * nrhs;
+  if (li < nsub) {
+    const int j = j0 + li * rstep;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t ig = rbase + 16 * rb + 4 * lk + r;
+        if (ig < rows) {
+          const float re = p1[rb][r] - p2[rb][r], im = p3[rb][r] - p1[rb][r] - p2[rb][r];
+          pq[(ig - tk.y) * nrhs + j] = make_double2((double)re, (double)im);
+        }
+      }
+  }
+}
+
 template <int KIND>
 __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
                                                       int ty_n, double2 *__restrict__ fvall,
@@ -1145,6 +1276,8 @@ struct LuFactor::Impl {
   // launch, drained when the stream is synchronised; class 0 trailing updates on the 64-tile kernel (work =
   // flops), 1 panels, 2 apply matrix products (work = matrix bytes read), 3 trailing updates on the 128-tile kernel
   bool timing = getenv("OPEM_LU_TIMING") != nullptr;
+  // OPEM_LU_APPLY_MFMA=0 keeps the FP32-factor apply on the vector kernel
+  bool apply_mfma = !(getenv("OPEM_LU_APPLY_MFMA") && getenv("OPEM_LU_APPLY_MFMA")[0] == '0');
   // columns per phase-1 chunk of the apply (the split-K depth; fewer chunks, fewer partial sums)
   int kch = [] {
     const char *e = getenv("OPEM_LU_KCH");
@@ -1859,6 +1992,13 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
     if (a1 > a0) I.timed(2, A.elems[KIND][l] * sizeof(T) * ninst, [&] {
       const dim3 g((unsigned)(a1 - a0), ninst);
       const int4 *tk = A.d1.as<int4>() + a0;
+      if constexpr (std::is_same<T, float2>::value) {
+        if (ty_n == 1 && I.apply_mfma) {  // FP32 factor, every right-hand side of the task in one MFMA tile
+          hipLaunchKernelGGL((k_lu_apply_mfma<KIND>), g, dim3(256), 0, st, tk, D, Rt, nrhs, F, E, pp, fv, y, tt, x,
+                             yr, part, part_qstride, I.kch);
+          return;
+        }
+      }
       if (maxr == 2)
         hipLaunchKernelGGL((k_lu_apply_part<T, KIND, 2>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F, E, pp, fv,
                            y, tt, x, yr, part, part_qstride, I.kch);

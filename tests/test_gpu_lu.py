@@ -189,3 +189,48 @@ def test_lu_deep_trailing_updates_128_tile_kernel():
     it0 = np.array(out["0"]["iters"])
     it2 = np.array(out["2"]["iters"])
     assert np.abs(it0 - it2).max() <= 2
+
+
+APPLY_CHILD = r"""
+import json
+import numpy as np
+from openparem_amd.mesh import tee_mesh
+from openparem_amd.solver import Solver
+m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=False)
+f = np.linspace(9.0e9, 9.3e9, 4)
+with Solver(m, 2) as s:
+    r = s.sweep(f, precision="mixed", tol=1e-11, batch=4, restart=30, precond="lu", orth_passes=1, precond_share=2)
+print(json.dumps({"S": [[float(z.real), float(z.imag)] for z in r["S"].ravel()],
+                  "converged": int(r["converged"].all()), "iters": r["iters"].tolist()}), flush=True)
+"""
+
+
+def test_lu_apply_mfma_matches_vector_kernel():
+    """The MFMA apply of FP32 factors (k_lu_apply_mfma) against the vector kernel (OPEM_LU_APPLY_MFMA=0), in child
+    processes, on a three-port tee with factors shared by 2 frequencies (6 right-hand sides per factor: the root
+    kinds take every third): the same iteration counts within one, and S within 1e-9 of each other and of the
+    oracle's direct solve."""
+    import subprocess
+    import sys
+    root = os.path.dirname(HERE)
+    out = {}
+    for mode in ("0", "1"):
+        env = dict(os.environ, OPEM_LU_APPLY_MFMA=mode)
+        env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        print(f"apply mfma {mode}", flush=True)
+        p = subprocess.run([sys.executable, "-c", APPLY_CHILD], cwd=root, env=env, capture_output=True, text=True,
+                           timeout=150)
+        assert p.returncode == 0, p.stderr[-2000:]
+        d = json.loads(p.stdout.strip().splitlines()[-1])
+        print(f"mode {mode}: iters {d['iters']}", flush=True)
+        assert d["converged"] == 1
+        out[mode] = d
+    S0 = np.array([complex(a, b) for a, b in out["0"]["S"]]).reshape(4, 3, 3)
+    S1 = np.array([complex(a, b) for a, b in out["1"]["S"]]).reshape(4, 3, 3)
+    assert np.abs(S0 - S1).max() < 1e-9
+    assert np.abs(np.array(out["0"]["iters"]) - np.array(out["1"]["iters"])).max() <= 1
+    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=False)
+    pr = Problem(m, 2)
+    f = np.linspace(9.0e9, 9.3e9, 4)
+    o = pr.solve_point(f[2], {a: te10(m, a, f[2]) for a in pr.ports})
+    assert np.abs(S1[2] - o["S"]).max() < 1e-8
