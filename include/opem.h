@@ -133,6 +133,20 @@ int opem_hfield(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *
 int opem_port_project(opem_ctx *ctx, int attr, const double *mode_E, const double *mode_H, const double *x,
                       const double *h, double *splits);
 
+/* The dominant mode of port attr at freq from the 2D port eigenproblem on the port's triangles: OpenParEM2D's
+ * ND_p x H1_p generalized eigenproblem (src/OpenParEM2D/fem2D.cpp:2221-2343, eigensolve.c:629-1017: A x = gamma^2 B x
+ * with x = (gamma e_t, e_z), the eigenvalue of smallest real part) and its H field (Hsolve.c:454), on the context's
+ * mesh, space, materials (of the port faces' owner tets) and boundaries (PEC and surface-impedance conductors are
+ * Dirichlet).  e_t is computed in the 3D space's own trace on the port, so the mode drives opem_sweep without a field
+ * transfer.  Outputs: gamma = alpha + j beta (complex), E_t and H_t (H_t of the wave leaving the domain) sampled at
+ * the port's opem_port_points (complex npts*3 each: the mode_E / mode_H layout of opem_sweep), normalised to a
+ * peak |E_t| of 1; with vpath (6 doubles: start and end point of a voltage path in the port) V = int E_t . dl along
+ * it (made real and positive), else V = 1; Z = V V* / (2 conj(P)), P = 1/2 int (E_t x H_t*) . n (Z_PV,
+ * fem2D.cpp:1819).  The eigenproblem is solved by shift-invert inverse iteration and Rayleigh-quotient steps with
+ * the sparse LU on the GPU. */
+int opem_port_mode(opem_ctx *ctx, int attr, double freq, const double *vpath, double *gamma, double *mode_E,
+                   double *mode_H, double *V, double *Z);
+
 /* Allocation-free variant of opem_spmv (same arguments, same kernel, identical y).  It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The caller's rowptr, colind, vals, x and y are staged in a 64 MiB

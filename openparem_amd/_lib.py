@@ -16,7 +16,7 @@ SYMBOLS = [
     "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench", "opem_spmv_noalloc",
     "opem_lu_stats", "opem_solve", "opem_hfield", "opem_port_project", "opem_results_error", "opem_s2z",
     "opem_z2s", "opem_renormalize", "opem_write_csv", "opem_write_touchstone", "opem_save_as_test",
-    "opem_process_tests",
+    "opem_process_tests", "opem_port_mode",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -86,6 +86,7 @@ def lib():
         L.opem_hfield.argtypes = [P, C.c_int64, i32p, i32p, f64p, f64p, C.POINTER(SolveOpts), f64p, f64p, i32p, i32p,
                                   i32p, i32p]
         L.opem_port_project.argtypes = [P, C.c_int, f64p, f64p, f64p, f64p, f64p]
+        L.opem_port_mode.argtypes = [P, C.c_int, C.c_double, f64p, f64p, f64p, f64p, f64p, f64p]
         L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]

@@ -2306,6 +2306,481 @@ static int guarded(opem_ctx *ctx, F f) {
   }
 }
 
+// ------------------------------------------------------------------ 2D port modes (OpenParEM2D's eigen-solve)
+// The dominant mode of a wave port from the 2D ND_p x H1_p eigenproblem on the port's triangles
+// (src/OpenParEM2D/fem2D.cpp:2221-2343 matrices, eigensolve.c:629-1017 eigenproblem and gamma, Hsolve.c:454 H):
+//   A = [St/mu - k0^2 eps Tt, 0; 0, 0],  B = [Tt/mu, G/mu; G^T/mu, Sz/mu - k0^2 eps Tz],  A x = gamma^2 B x,
+// x = (gamma e_t, e_z) for fields (e_t + n e_z) e^{-gamma s} along the port's outward normal n.  e_t lives in the
+// 3D space's trace on the port (its boundary-face DOFs), so the mode drives the 3D solve without a transfer.
+// PEC and surface-impedance conductors (the walls, a trace crossing the port) are Dirichlet for e_t and e_z.
+// The mode kept is the eigenvalue nearest below the shift -1.05 k0^2 max(eps_r mu_r) (the smallest real part of
+// the non-null spectrum, EPS_SMALLEST_REAL in eigensolve.c:852): inverse iteration with the sparse LU of A - s B
+// on the GPU (two refinement steps per solve), then Rayleigh-quotient steps with refactored shifts.
+namespace {
+
+struct H1Fn {
+  int kind;  // 0 vertex a, 1 edge (a < b) power k of (l_a - l_b), 2 bubble l0 l1 l2 l1^a l2^b
+  int a, b, k;
+};
+
+std::vector<H1Fn> h1_tri_basis(int p) {
+  std::vector<H1Fn> B = {{0, 0, 0, 0}, {0, 1, 0, 0}, {0, 2, 0, 0}};
+  const int ed[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (auto &e : ed)
+    for (int k = 0; k < p - 1; k++) B.push_back({1, e[0], e[1], k});
+  for (int s = 0; s < p - 2; s++)
+    for (int i = 0; i <= s; i++) B.push_back({2, s - i, i, 0});
+  return B;
+}
+
+// H1 values and barycentric derivatives d/dl_m at barycentrics L
+void h1_eval(const std::vector<H1Fn> &B, const double *L, double *val, double (*dl)[3]) {
+  for (size_t a = 0; a < B.size(); a++) {
+    const H1Fn &f = B[a];
+    double d[3] = {0, 0, 0};
+    if (f.kind == 0) {
+      val[a] = L[f.a];
+      d[f.a] = 1.0;
+    } else if (f.kind == 1) {
+      const double t = L[f.a] - L[f.b];
+      const double tk = std::pow(t, f.k), dtk = f.k > 0 ? f.k * std::pow(t, f.k - 1) : 0.0;
+      val[a] = L[f.a] * L[f.b] * tk;
+      d[f.a] = L[f.b] * tk + L[f.a] * L[f.b] * dtk;
+      d[f.b] = L[f.a] * tk - L[f.a] * L[f.b] * dtk;
+    } else {
+      const double c = L[0] * L[1] * L[2], m = std::pow(L[1], f.a) * std::pow(L[2], f.b);
+      val[a] = c * m;
+      const double dc[3] = {L[1] * L[2], L[0] * L[2], L[0] * L[1]};
+      double dm[3] = {0, 0, 0};
+      if (f.a > 0) dm[1] = f.a * std::pow(L[1], f.a - 1) * std::pow(L[2], f.b);
+      if (f.b > 0) dm[2] = f.b * std::pow(L[1], f.a) * std::pow(L[2], f.b - 1);
+      for (int q = 0; q < 3; q++) d[q] = dc[q] * m + c * dm[q];
+    }
+    for (int q = 0; q < 3; q++) dl[a][q] = d[q];
+  }
+}
+
+struct TriGeo {
+  double v[3][3], g[3][3], n[3], a2;  // vertices (sorted), in-plane barycentric gradients, unit normal, 2 x area
+};
+
+TriGeo tri_geo(const Mesh &m, const Space &s, int64_t f) {
+  TriGeo T;
+  for (int k = 0; k < 3; k++)
+    for (int d = 0; d < 3; d++) T.v[k][d] = m.xyz[(int64_t)s.bf_sorted[f * 3 + k] * 3 + d];
+  double e1[3], e2[3];
+  for (int d = 0; d < 3; d++) {
+    e1[d] = T.v[1][d] - T.v[0][d];
+    e2[d] = T.v[2][d] - T.v[0][d];
+  }
+  double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  T.a2 = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  for (int d = 0; d < 3; d++) T.n[d] = c[d] / T.a2;
+  auto cross = [](const double *x, const double *y, double *z) {
+    z[0] = x[1] * y[2] - x[2] * y[1];
+    z[1] = x[2] * y[0] - x[0] * y[2];
+    z[2] = x[0] * y[1] - x[1] * y[0];
+  };
+  cross(e2, T.n, T.g[1]);
+  cross(T.n, e1, T.g[2]);
+  for (int d = 0; d < 3; d++) {
+    T.g[1][d] /= T.a2;
+    T.g[2][d] /= T.a2;
+    T.g[0][d] = -(T.g[1][d] + T.g[2][d]);
+  }
+  return T;
+}
+
+// ND_p (AFW) values and scalar curls (about T.n) at barycentrics L
+void nd_eval_curl(const std::vector<BasisFn> &basis, const TriGeo &T, const double *L, double *phi, double *curl) {
+  for (size_t a = 0; a < basis.size(); a++) {
+    const BasisFn &f = basis[a];
+    double la = 1.0, gla[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++)
+      for (int q = 0; q < f.alpha[k]; q++) la *= L[k];
+    for (int mm = 0; mm < 3; mm++) {
+      if (f.alpha[mm] == 0) continue;
+      double c = f.alpha[mm];
+      for (int k = 0; k < 3; k++)
+        for (int q = 0; q < f.alpha[k] - (k == mm ? 1 : 0); q++) c *= L[k];
+      for (int d = 0; d < 3; d++) gla[d] += c * T.g[mm][d];
+    }
+    double w[3];
+    for (int d = 0; d < 3; d++) {
+      w[d] = L[f.i] * T.g[f.j][d] - L[f.j] * T.g[f.i][d];
+      phi[a * 3 + d] = la * w[d];
+    }
+    // curl = grad(la) x w + 2 la (g_i x g_j), dotted with n
+    const double *gi = T.g[f.i], *gj = T.g[f.j];
+    const double cw[3] = {gla[1] * w[2] - gla[2] * w[1], gla[2] * w[0] - gla[0] * w[2], gla[0] * w[1] - gla[1] * w[0]};
+    const double gg[3] = {gi[1] * gj[2] - gi[2] * gj[1], gi[2] * gj[0] - gi[0] * gj[2], gi[0] * gj[1] - gi[1] * gj[0]};
+    curl[a] = 0;
+    for (int d = 0; d < 3; d++) curl[a] += (cw[d] + 2.0 * la * gg[d]) * T.n[d];
+  }
+}
+
+// complex CSR with a shared pattern for A and B
+struct CsrAB {
+  int64_t n = 0;
+  std::vector<int32_t> rp, ci;
+  std::vector<cplx> a, b;
+};
+
+}  // namespace
+
+struct PortModeResult {
+  cplx gamma;
+  std::vector<cplx> et, ez, ht;  // port-local ND DOFs (PortTrace order), H1 DOFs, port-local ND DOFs
+  int iterations = 0;
+};
+
+static void port_mode_solve(Ctx &c, const PortTrace &P, double freq, PortModeResult &R) {
+  const Space &s = c.sp;
+  const Mesh &m = c.mesh;
+  const int p = s.p, ndf = s.ndf;
+  const int nf = (int)P.faces.size();
+  const int np = (int)P.dofs.size();
+  const double k0 = 2 * M_PI * freq * std::sqrt(kMu0 * kEps0), omega = 2 * M_PI * freq;
+  // conductors: PEC and surface-impedance faces
+  std::vector<uint8_t> cond_dof(s.ndof, 0);
+  std::map<std::pair<int32_t, int32_t>, int> cond_edge;
+  for (int64_t f = 0; f < m.nbf; f++) {
+    auto it = c.bdr.find(m.bface_attr[f]);
+    if (it == c.bdr.end() || (it->second.kind != OPEM_BDR_PEC && it->second.kind != OPEM_BDR_IMPEDANCE)) continue;
+    for (int k = 0; k < ndf; k++) cond_dof[s.bface_dofs[f * ndf + k]] = 1;
+    const int32_t *v = &s.bf_sorted[f * 3];
+    cond_edge[{v[0], v[1]}] = 1;
+    cond_edge[{v[0], v[2]}] = 1;
+    cond_edge[{v[1], v[2]}] = 1;
+  }
+  // H1 numbering: vertices, edges (sorted global pair, power), bubbles (face, exponents)
+  const std::vector<H1Fn> hb = h1_tri_basis(p);
+  const int nhf = (int)hb.size();
+  std::map<std::tuple<int, int64_t, int64_t, int>, int> hkey;
+  std::vector<int> hloc((size_t)nf * nhf);
+  std::vector<uint8_t> hdir;
+  for (int fi = 0; fi < nf; fi++) {
+    const int64_t f = P.faces[fi];
+    const int32_t *v = &s.bf_sorted[f * 3];
+    for (int a = 0; a < nhf; a++) {
+      const H1Fn &h = hb[a];
+      std::tuple<int, int64_t, int64_t, int> key;
+      bool dir = false;
+      if (h.kind == 0) {
+        key = std::make_tuple(0, (int64_t)v[h.a], (int64_t)0, 0);
+      } else if (h.kind == 1) {
+        key = std::make_tuple(1, (int64_t)v[h.a], (int64_t)v[h.b], h.k);
+        dir = cond_edge.count({v[h.a], v[h.b]}) > 0;
+      } else {
+        key = std::make_tuple(2, f, (int64_t)h.a, h.b);
+      }
+      auto it = hkey.find(key);
+      int id;
+      if (it == hkey.end()) {
+        id = (int)hkey.size();
+        hkey.emplace(key, id);
+        hdir.push_back(dir);
+      } else {
+        id = it->second;
+      }
+      hloc[(size_t)fi * nhf + a] = id;
+    }
+  }
+  const int nh = (int)hkey.size();
+  // a vertex is Dirichlet when it ends a conducting edge of the port
+  for (int fi = 0; fi < nf; fi++) {
+    const int32_t *v = &s.bf_sorted[P.faces[fi] * 3];
+    const int ed[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (auto &e : ed)
+      if (cond_edge.count({v[e[0]], v[e[1]]}))
+        for (int q = 0; q < 2; q++) hdir[hloc[(size_t)fi * nhf + e[q]]] = 1;
+  }
+  // free unknowns: [ND free..., H1 free...]
+  std::vector<int> nidx(np, -1), hidx(nh, -1);
+  int nfree = 0;
+  for (int i = 0; i < np; i++)
+    if (!cond_dof[P.dofs[i]]) nidx[i] = nfree++;
+  const int ntf = nfree;
+  for (int i = 0; i < nh; i++)
+    if (!hdir[i]) hidx[i] = nfree++;
+  if (ntf == 0) throw std::runtime_error("port mode: every tangential DOF of the port lies on a conductor");
+  // outward normal of the port (away from the owner tet of its first face)
+  double nout[3];
+  {
+    const TriGeo T = tri_geo(m, s, P.faces[0]);
+    const int32_t *tv = &m.tets[(int64_t)s.bface_owner[P.faces[0]] * 4];
+    double cen[3] = {0, 0, 0};
+    for (int k = 0; k < 4; k++)
+      for (int d = 0; d < 3; d++) cen[d] += 0.25 * m.xyz[(int64_t)tv[k] * 3 + d];
+    double dot = 0;
+    for (int d = 0; d < 3; d++) dot += (T.v[0][d] - cen[d]) * T.n[d];
+    for (int d = 0; d < 3; d++) nout[d] = dot >= 0 ? T.n[d] : -T.n[d];
+  }
+  // element matrices by quadrature (exact for these polynomial degrees), accumulated into A and B
+  std::vector<double> Lq, wq;
+  tri_rule(p + 2, Lq, wq);
+  const int nq = (int)wq.size();
+  const std::vector<BasisFn> nb = nd_tri_basis(p);
+  std::map<std::pair<int, int>, std::pair<cplx, cplx>> ent;  // (row, col) -> (A, B)
+  auto add = [&](int r, int cc, cplx av, cplx bv) {
+    if (r < 0 || cc < 0) return;
+    auto &e = ent[{r, cc}];
+    e.first += av;
+    e.second += bv;
+  };
+  std::vector<double> phi(ndf * 3), curl(ndf), hv(nhf);
+  std::vector<double> hdl((size_t)nhf * 3);
+  std::vector<double> hg((size_t)nhf * 3);
+  double epsmu_max = 0;
+  std::vector<cplx> face_eps(nf);
+  std::vector<double> face_mur(nf);
+  for (int fi = 0; fi < nf; fi++) {
+    const int64_t f = P.faces[fi];
+    const Material &mt = c.mat.at(m.tet_attr[s.bface_owner[f]]);
+    face_eps[fi] = cplx(mt.er, -mt.er * mt.tand);
+    face_mur[fi] = mt.mur;
+    epsmu_max = std::max(epsmu_max, mt.er * mt.mur);
+  }
+  for (int fi = 0; fi < nf; fi++) {
+    const int64_t f = P.faces[fi];
+    const TriGeo T = tri_geo(m, s, f);
+    const cplx keps = k0 * k0 * face_eps[fi];
+    const double imu = 1.0 / face_mur[fi];
+    std::vector<cplx> Ae((size_t)(ndf + nhf) * (ndf + nhf), 0.0), Be(Ae.size(), 0.0);
+    const int ne = ndf + nhf;
+    for (int q = 0; q < nq; q++) {
+      const double *L = &Lq[q * 3];
+      const double w = wq[q] * T.a2;
+      nd_eval_curl(nb, T, L, phi.data(), curl.data());
+      h1_eval(hb, L, hv.data(), reinterpret_cast<double(*)[3]>(hdl.data()));
+      for (int a = 0; a < nhf; a++)
+        for (int d = 0; d < 3; d++)
+          hg[a * 3 + d] = hdl[a * 3] * T.g[0][d] + hdl[a * 3 + 1] * T.g[1][d] + hdl[a * 3 + 2] * T.g[2][d];
+      for (int a = 0; a < ndf; a++)
+        for (int b = 0; b < ndf; b++) {
+          const double tt = phi[a * 3] * phi[b * 3] + phi[a * 3 + 1] * phi[b * 3 + 1] + phi[a * 3 + 2] * phi[b * 3 + 2];
+          Ae[a * ne + b] += w * (imu * curl[a] * curl[b] - keps * tt);
+          Be[a * ne + b] += w * imu * tt;
+        }
+      for (int a = 0; a < ndf; a++)
+        for (int b = 0; b < nhf; b++) {
+          const double gg = phi[a * 3] * hg[b * 3] + phi[a * 3 + 1] * hg[b * 3 + 1] + phi[a * 3 + 2] * hg[b * 3 + 2];
+          Be[a * ne + ndf + b] += w * imu * gg;
+          Be[(ndf + b) * ne + a] += w * imu * gg;
+        }
+      for (int a = 0; a < nhf; a++)
+        for (int b = 0; b < nhf; b++) {
+          const double ss = hg[a * 3] * hg[b * 3] + hg[a * 3 + 1] * hg[b * 3 + 1] + hg[a * 3 + 2] * hg[b * 3 + 2];
+          Be[(ndf + a) * ne + ndf + b] += w * (imu * ss - keps * hv[a] * hv[b]);
+        }
+    }
+    std::vector<int> gi(ne);
+    for (int a = 0; a < ndf; a++) gi[a] = nidx[P.loc[(size_t)fi * ndf + a]];
+    for (int a = 0; a < nhf; a++) gi[ndf + a] = hidx[hloc[(size_t)fi * nhf + a]];
+    for (int a = 0; a < ne; a++)
+      for (int b = 0; b < ne; b++)
+        if (Ae[a * ne + b] != 0.0 || Be[a * ne + b] != 0.0) add(gi[a], gi[b], Ae[a * ne + b], Be[a * ne + b]);
+  }
+  CsrAB M;
+  M.n = nfree;
+  M.rp.assign(nfree + 1, 0);
+  for (auto &e : ent) M.rp[e.first.first + 1]++;
+  for (int i = 0; i < nfree; i++) M.rp[i + 1] += M.rp[i];
+  for (auto &e : ent) {  // map order is row-major, so this fills rows in order
+    M.ci.push_back(e.first.second);
+    M.a.push_back(e.second.first);
+    M.b.push_back(e.second.second);
+  }
+  const int64_t nnz = (int64_t)M.ci.size();
+  auto matvec = [&](const std::vector<cplx> &vals, const std::vector<cplx> &x, std::vector<cplx> &y) {
+    y.assign(nfree, 0.0);
+    for (int i = 0; i < nfree; i++)
+      for (int k = M.rp[i]; k < M.rp[i + 1]; k++) y[i] += vals[k] * x[M.ci[k]];
+  };
+  // shift-invert inverse iteration with the GPU sparse LU
+  hipStream_t st = c.st;
+  LuTree T;
+  std::vector<uint8_t> none(nfree, 0);
+  lu_analyse_graph(nfree, M.rp.data(), M.ci.data(), none, none, 64, T);
+  LuFactor F;
+  F.setup(T, 1, {}, st);
+  DevBuf dk, dy, dz;
+  std::vector<cplx> kv(nnz);
+  cplx sigma = -1.05 * k0 * k0 * epsmu_max;
+  auto factor = [&](cplx sg) {
+    for (int64_t k = 0; k < nnz; k++) kv[k] = M.a[k] - sg * M.b[k];
+    std::vector<double> kd(2 * nnz);
+    std::memcpy(kd.data(), kv.data(), nnz * 16);
+    dk.upload(kd, st);
+    LuFactorStats fs;
+    F.factor(0, 1, dk.as<double>(), nnz, &fs);
+  };
+  dz.alloc((size_t)nfree * 16);
+  auto lu_solve = [&](const std::vector<cplx> &y, std::vector<cplx> &z) {
+    std::vector<double> yd(2 * (size_t)nfree);
+    std::memcpy(yd.data(), y.data(), (size_t)nfree * 16);
+    dy.upload(yd, st);
+    F.solve<double2>(1, dy.as<double2>(), nfree, dz.as<double2>());
+    z.resize(nfree);
+    HIPCHK(hipMemcpyAsync(z.data(), dz.p, (size_t)nfree * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  };
+  auto solve = [&](const std::vector<cplx> &y, std::vector<cplx> &z) {  // (A - sigma B) z = y, refined twice
+    lu_solve(y, z);
+    std::vector<cplx> r, dzv;
+    for (int it = 0; it < 2; it++) {
+      matvec(kv, z, r);
+      for (int i = 0; i < nfree; i++) r[i] = y[i] - r[i];
+      lu_solve(r, dzv);
+      for (int i = 0; i < nfree; i++) z[i] += dzv[i];
+    }
+  };
+  auto bilin = [&](const std::vector<cplx> &u, const std::vector<cplx> &v) {
+    cplx t = 0;
+    for (int i = 0; i < nfree; i++) t += u[i] * v[i];
+    return t;
+  };
+  std::vector<cplx> x(nfree), y, z, ax, bx;
+  for (int i = 0; i < nfree; i++) x[i] = cplx(1.0 + 0.1 * std::sin(1.0 + i), 0.05 * std::cos(2.0 * i));
+  factor(sigma);
+  cplx lam = sigma, lam_prev = 0;
+  int its = 0;
+  auto normalize = [&](std::vector<cplx> &v) {
+    double nn = 0;
+    for (auto &e : v) nn += std::norm(e);
+    nn = std::sqrt(nn);
+    for (auto &e : v) e /= nn;
+  };
+  for (; its < 300; its++) {  // inverse iteration at the fixed shift
+    matvec(M.b, x, y);
+    solve(y, z);
+    matvec(M.a, z, ax);
+    matvec(M.b, z, bx);
+    lam_prev = lam;
+    lam = bilin(z, ax) / bilin(z, bx);
+    x = z;
+    normalize(x);
+    if (its > 0 && std::abs(lam - lam_prev) <= 1e-9 * std::abs(lam)) break;
+  }
+  for (int rq = 0; rq < 3; rq++) {  // Rayleigh-quotient steps: shift at the current estimate
+    factor(lam);
+    sigma = lam;
+    matvec(M.b, x, y);
+    solve(y, z);
+    matvec(M.a, z, ax);
+    matvec(M.b, z, bx);
+    lam_prev = lam;
+    lam = bilin(z, ax) / bilin(z, bx);
+    x = z;
+    normalize(x);
+    its++;
+    if (std::abs(lam - lam_prev) <= 1e-14 * std::abs(lam)) break;
+  }
+  // gamma with alpha, beta >= 0 (eigensolve.c:446-466)
+  cplx g = std::sqrt(lam);
+  if (std::abs(g.imag()) > std::abs(g.real())) {
+    if (g.imag() < 0) g = -g;
+  } else if (g.real() < 0) {
+    g = -g;
+  }
+  R.gamma = g;
+  R.iterations = its;
+  R.et.assign(np, 0.0);
+  R.ez.assign(nh, 0.0);
+  for (int i = 0; i < np; i++)
+    if (nidx[i] >= 0) R.et[i] = x[nidx[i]] / g;
+  for (int i = 0; i < nh; i++)
+    if (hidx[i] >= 0) R.ez[i] = x[hidx[i]];
+  // H_t = n x (gamma e_t + grad e_z) / (j omega mu), L2-projected onto the port's ND_p (Hsolve.c)
+  std::vector<cplx> rhs(np, 0.0);
+  for (int fi = 0; fi < nf; fi++) {
+    const int64_t f = P.faces[fi];
+    const TriGeo T = tri_geo(m, s, f);
+    const cplx coef = 1.0 / (cplx(0, 1) * omega * kMu0 * face_mur[fi]);
+    for (int q = 0; q < nq; q++) {
+      const double *L = &Lq[q * 3];
+      const double w = wq[q] * T.a2;
+      nd_eval_curl(nb, T, L, phi.data(), curl.data());
+      h1_eval(hb, L, hv.data(), reinterpret_cast<double(*)[3]>(hdl.data()));
+      cplx u[3] = {0, 0, 0};
+      for (int a = 0; a < ndf; a++)
+        for (int d = 0; d < 3; d++) u[d] += g * phi[a * 3 + d] * R.et[P.loc[(size_t)fi * ndf + a]];
+      for (int a = 0; a < nhf; a++) {
+        const cplx ez = R.ez[hloc[(size_t)fi * nhf + a]];
+        for (int d = 0; d < 3; d++)
+          u[d] += ez * (hdl[a * 3] * T.g[0][d] + hdl[a * 3 + 1] * T.g[1][d] + hdl[a * 3 + 2] * T.g[2][d]);
+      }
+      const cplx nu[3] = {nout[1] * u[2] - nout[2] * u[1], nout[2] * u[0] - nout[0] * u[2],
+                          nout[0] * u[1] - nout[1] * u[0]};
+      for (int a = 0; a < ndf; a++)
+        rhs[P.loc[(size_t)fi * ndf + a]] +=
+            w * coef * (phi[a * 3] * nu[0] + phi[a * 3 + 1] * nu[1] + phi[a * 3 + 2] * nu[2]);
+    }
+  }
+  port_mass_solve(P, rhs, R.ht, false);
+}
+
+// Tangential field of port-local ND DOFs at the port's quadrature points (the opem_port_points layout)
+static void port_field_at_points(const PortTrace &P, int ndf, const std::vector<cplx> &v, std::vector<cplx> &out) {
+  const int64_t nq = P.npts / (int64_t)P.faces.size();
+  out.assign((size_t)P.npts * 3, 0.0);
+  for (int64_t pi = 0; pi < P.npts; pi++) {
+    const int64_t fi = pi / nq;
+    for (int a = 0; a < ndf; a++) {
+      const cplx c = v[P.loc[(size_t)fi * ndf + a]];
+      for (int d = 0; d < 3; d++) out[pi * 3 + d] += c * P.phi[(pi * ndf + a) * 3 + d];
+    }
+  }
+}
+
+// V = int E_t . dl along the segment a -> b (Gauss-Legendre on 64 pieces; points located in the port's triangles)
+static cplx port_line_voltage(const Ctx &c, const PortTrace &P, const std::vector<cplx> &et, const double *a,
+                              const double *b) {
+  static const double xg[6] = {-0.9324695142031521, -0.6612093864662645, -0.2386191860831969,
+                               0.2386191860831969,  0.6612093864662645,  0.9324695142031521};
+  static const double wg[6] = {0.1713244923791704, 0.3607615730481386, 0.4679139345726910,
+                               0.4679139345726910, 0.3607615730481386, 0.1713244923791704};
+  const Space &s = c.sp;
+  const int ndf = s.ndf, npc = 64;
+  double dl[3];
+  for (int d = 0; d < 3; d++) dl[d] = b[d] - a[d];
+  std::vector<double> out(ndf * 3);
+  cplx V = 0;
+  for (int pc = 0; pc < npc; pc++)
+    for (int k = 0; k < 6; k++) {
+      const double t = (pc + 0.5 * (xg[k] + 1.0)) / npc, w = 0.5 * wg[k] / npc;
+      double x[3];
+      for (int d = 0; d < 3; d++) x[d] = a[d] + t * dl[d];
+      int best = -1;
+      double bmin = -1e300, bL[3] = {0, 0, 0};
+      for (size_t fi = 0; fi < P.faces.size(); fi++) {
+        const TriGeo T = tri_geo(c.mesh, s, P.faces[fi]);
+        double L[3];
+        for (int q = 1; q < 3; q++) {
+          L[q] = 0;
+          for (int d = 0; d < 3; d++) L[q] += T.g[q][d] * (x[d] - T.v[0][d]);
+        }
+        L[0] = 1.0 - L[1] - L[2];
+        const double mn = std::min(L[0], std::min(L[1], L[2]));
+        if (mn > bmin) {
+          bmin = mn;
+          best = (int)fi;
+          std::memcpy(bL, L, sizeof bL);
+        }
+      }
+      double v[3][3];
+      const int64_t f = P.faces[best];
+      for (int q = 0; q < 3; q++)
+        for (int d = 0; d < 3; d++) v[q][d] = c.mesh.xyz[(int64_t)s.bf_sorted[f * 3 + q] * 3 + d];
+      tri_basis_eval(s.p, v, bL, out.data());
+      for (int aa = 0; aa < ndf; aa++) {
+        const cplx cc = et[P.loc[(size_t)best * ndf + aa]];
+        V += w * cc * (out[aa * 3] * dl[0] + out[aa * 3 + 1] * dl[1] + out[aa * 3 + 2] * dl[2]);
+      }
+    }
+  return V;
+}
+
 extern "C" {
 
 int opem_create(opem_ctx **ctx, int device) {
@@ -2639,6 +3114,80 @@ int opem_hfield(opem_ctx *ctx, int64_t n, const int32_t *rowptr, const int32_t *
     std::vector<double> hx((size_t)2 * n, 0.0);
     solve_caller(*ctx, n, rowptr, colind, vals, 0, nullptr, b, cs, hx.data(), rnorm, iters, converged, rung, flags);
     std::memcpy(h, hx.data(), hx.size() * 8);
+  });
+}
+
+/* see include/opem.h */
+int opem_port_mode(opem_ctx *ctx, int attr, double freq, const double *vpath, double *gamma, double *mode_E,
+                   double *mode_H, double *V, double *Z) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (!(freq > 0) || !gamma || !mode_E || !mode_H || !V || !Z) throw std::invalid_argument("bad arguments");
+    for (int sl = 0; sl < (int)ctx->slot_attr.size(); sl++)
+      if (!ctx->mat.count(ctx->slot_attr[sl])) throw std::invalid_argument("material not set");
+    build_ports(*ctx);
+    const PortTrace *P = nullptr;
+    for (auto &pt : ctx->ports)
+      if (pt.attr == attr) P = &pt;
+    if (!P) throw std::invalid_argument("attribute is not a port");
+    const int ndf = ctx->sp.ndf;
+    PortModeResult R;
+    port_mode_solve(*ctx, *P, freq, R);
+    std::vector<cplx> E, H;
+    port_field_at_points(*P, ndf, R.et, E);
+    port_field_at_points(*P, ndf, R.ht, H);
+    // normalisation: V real and positive when a path is given, else the largest E_t DOF real and positive; then
+    // the peak of |E_t| at the points is 1 (FieldSet::scaleSolution, port.cpp:2291-2335)
+    cplx Vc = 1.0;
+    cplx ph = 1.0;
+    if (vpath) {
+      Vc = port_line_voltage(*ctx, *P, R.et, vpath, vpath + 3);
+      if (std::abs(Vc) > 0) ph = std::abs(Vc) / Vc;
+    } else {
+      cplx big = 0;
+      for (auto &e : R.et)
+        if (std::abs(e) > std::abs(big)) big = e;
+      if (std::abs(big) > 0) ph = std::abs(big) / big;
+    }
+    double emax = 0;
+    for (int64_t i = 0; i < P->npts; i++) {
+      double e2 = 0;
+      for (int d = 0; d < 3; d++) e2 += std::norm(E[i * 3 + d]);
+      emax = std::max(emax, std::sqrt(e2));
+    }
+    if (!(emax > 0)) throw std::runtime_error("port mode: zero field");
+    const cplx sc = ph / emax;
+    for (auto &e : E) e *= sc;
+    for (auto &e : H) e *= sc;
+    if (vpath) Vc *= sc;
+    // P = 1/2 int (E_t x H_t*) . n over the port's quadrature points; Z = V V* / (2 conj(P)) (fem2D.cpp:1819)
+    double nout[3];
+    {
+      const TriGeo T = tri_geo(ctx->mesh, ctx->sp, P->faces[0]);
+      const int32_t *tv = &ctx->mesh.tets[(int64_t)ctx->sp.bface_owner[P->faces[0]] * 4];
+      double cen[3] = {0, 0, 0};
+      for (int k = 0; k < 4; k++)
+        for (int d = 0; d < 3; d++) cen[d] += 0.25 * ctx->mesh.xyz[(int64_t)tv[k] * 3 + d];
+      double dot = 0;
+      for (int d = 0; d < 3; d++) dot += (T.v[0][d] - cen[d]) * T.n[d];
+      for (int d = 0; d < 3; d++) nout[d] = dot >= 0 ? T.n[d] : -T.n[d];
+    }
+    cplx Pw = 0;
+    for (int64_t i = 0; i < P->npts; i++) {
+      const cplx *e = &E[i * 3], *h = &H[i * 3];
+      const cplx cx[3] = {e[1] * std::conj(h[2]) - e[2] * std::conj(h[1]), e[2] * std::conj(h[0]) - e[0] * std::conj(h[2]),
+                          e[0] * std::conj(h[1]) - e[1] * std::conj(h[0])};
+      Pw += 0.5 * P->wq[i] * (cx[0] * nout[0] + cx[1] * nout[1] + cx[2] * nout[2]);
+    }
+    const cplx Zc = 0.5 * Vc * std::conj(Vc) / std::conj(Pw);
+    gamma[0] = R.gamma.real();
+    gamma[1] = R.gamma.imag();
+    std::memcpy(mode_E, E.data(), E.size() * 16);
+    std::memcpy(mode_H, H.data(), H.size() * 16);
+    V[0] = Vc.real();
+    V[1] = Vc.imag();
+    Z[0] = Zc.real();
+    Z[1] = Zc.imag();
   });
 }
 

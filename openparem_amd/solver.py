@@ -120,6 +120,39 @@ class Solver:
                                       ptr(out.view(np.float64), C.c_double)), self.ctx, "opem_port_project")
         return dict(zip(("Cp", "Cm", "e0", "e2", "h0", "h2"), out.tolist()))
 
+    def port_mode(self, attr, freq, vpath=None):
+        """opem_port_mode: the dominant mode of port ``attr`` at ``freq`` from the 2D port eigenproblem.  Returns
+        gamma, E and H (complex (npts, 3) at the port's points), V and Z; ``vpath`` = (start, end) of a voltage path
+        in the port (V = 1 without one)."""
+        n = C.c_int64()
+        check(lib().opem_port_points(self.ctx, attr, C.byref(n), ptr(None, C.c_double)), self.ctx, "port_points")
+        E = np.zeros((n.value, 3), np.complex128)
+        H = np.zeros_like(E)
+        g, V, Z = (np.zeros(2) for _ in range(3))
+        vp = None if vpath is None else np.ascontiguousarray(np.concatenate([np.ravel(vpath[0]), np.ravel(vpath[1])]),
+                                                              np.float64)
+        check(lib().opem_port_mode(self.ctx, int(attr), float(freq), ptr(vp, C.c_double), ptr(g, C.c_double),
+                                   ptr(E.view(np.float64), C.c_double), ptr(H.view(np.float64), C.c_double),
+                                   ptr(V, C.c_double), ptr(Z, C.c_double)), self.ctx, "opem_port_mode")
+        return dict(gamma=complex(g[0], g[1]), E=E, H=H, V=complex(V[0], V[1]), Z=complex(Z[0], Z[1]))
+
+    def port_mode_provider(self, vpaths=None):
+        """A ``mode_fn`` for ``sweep`` that takes every port's mode from the 2D port eigenproblem (opem_port_mode);
+        ``vpaths``: {attr: (start, end)} voltage paths."""
+        pts = {a: self.port_points(a) for a in self.ports}
+
+        def mode_fn(mesh, attr, freq):
+            md = self.port_mode(attr, freq, None if vpaths is None else vpaths.get(attr))
+
+            def sampled(values):
+                def at(p):
+                    if p.shape != pts[attr].shape or not np.array_equal(p, pts[attr]):
+                        raise ValueError("2D port modes are sampled at the port's own points only")
+                    return values
+                return at
+            return dict(E=sampled(md["E"]), H=sampled(md["H"]), gamma=md["gamma"], V=md["V"], Z=md["Z"])
+        return mode_fn
+
     def spmv_bench(self, precision="fp32", ngroups=1, R=2, nrep=20):
         """(mean kernel seconds, algorithmic bytes) of the batched SpMV on this pattern."""
         t, b = C.c_double(), C.c_double()
