@@ -114,7 +114,8 @@ def _lu_roofline(st, workload):
                 "v_mfma_f32_16x16x4f32 products per complex update)", "gemm128")
     gemm = mfma("k_dense_gemm (the other trailing updates: 64x64 tiles, four real MFMA products per complex "
                 "update)", "gemm")
-    apply_ = {"kernel": "k_lu_apply_part (the preconditioner apply's products with the FP32 factor blocks)",
+    apply_ = {"kernel": "k_lu_apply_mfma (the preconditioner apply's products with the FP32 factor blocks on "
+                        "v_mfma_f32_16x16x4f32, all four block kinds)",
               "bound": "hbm", "achieved": st["apply_bytes"] / st["apply_s"] / 1e9 if st["apply_s"] else None,
               "peak": HBM_PEAK_GBS, "unit": "GB/s", "seconds": st["apply_s"], "launches": st["apply_launches"],
               "bytes": st["apply_bytes"]}
@@ -128,7 +129,7 @@ def _lu_roofline(st, workload):
             "traffic": None if ratio is None or dom is not apply_ else ratio * work_per_launch,
             "per_launch": work_per_launch, "launch_seconds": dom["seconds"] / max(1.0, dom["launches"]),
             "launches": dom["launches"]}
-    return roof, {"k_dense_gemm_t128": deep, "k_dense_gemm": gemm, "k_lu_apply_part": apply_}
+    return roof, {"k_dense_gemm_t128": deep, "k_dense_gemm": gemm, "k_lu_apply_mfma": apply_}
 
 
 def _uneven(args):

@@ -159,7 +159,7 @@ def c4lu(share=1, fp64_points=(0, 21, 42, 63)):
         s.close()
 
 
-def lu_pmc(nfreq=4, share=2):
+def lu_pmc(nfreq=8, share=4):
     """The sparse-LU kernels of the config-4 sweep at a counter-friendly size, for rocprofv3 --pmc passes: one mixed
     sweep of the first `nfreq` frequencies of the 64-point plan (FP32 factors of the mean matrix of each `share`
     adjacent frequencies) on a fresh context, then its opem_lu_stats totals (trailing-update flops, apply matrix
