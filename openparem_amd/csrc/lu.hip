@@ -1242,7 +1242,7 @@ struct LuFactor::Impl {
     double gemm_flops;  // real flops of the step's trailing updates, per matrix (8 per complex multiply-add)
     int64_t gemm_tiles128;  // 128 x 128 tiles of the same jobs (prefix in h_gprefix128)
     bool deep;              // the trailing update of an outer block (FP32 fronts may take the 128-tile kernel)
-    bool deep_fit;          // at least 80% of the 128-tile area is matrix (the default rule for taking it)
+    bool deep_fit;          // at least 85% of the 128-tile area is matrix (the default rule for taking it)
   };
   struct DLevel {
     std::vector<Step> lu, inv;
