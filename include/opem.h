@@ -147,6 +147,87 @@ int opem_port_project(opem_ctx *ctx, int attr, const double *mode_E, const doubl
 int opem_port_mode(opem_ctx *ctx, int attr, double freq, const double *vpath, double *gamma, double *mode_E,
                    double *mode_H, double *V, double *Z);
 
+/* ---- Radiation patterns (src/OpenParEM3D/port.cpp:1043-1265, 8077-8160; pattern.cpp) ----------------------- */
+
+/* Boundary::calculateRadiationCurrents (port.cpp:1043-1265) for the radiation boundaries attrs[0..nattr) (nattr 0:
+ * every radiation boundary of the context), which must share one wave impedance eta (fem3D.cpp:1904-1919):
+ * J = n x H_t and M = -n x E_t (n the outward unit normal) at the centres of the boundary triangles refined until
+ * no edge is longer than limit wavelengths (lambda = eta / (mu0 freq), fem3D.cpp:1926-1929), each with its area.
+ * x, h: the E and H DOF vectors (complex ndof, opem_sweep's x_out / h_out layout).  Each face whose longest edge
+ * is L is split into s*s similar triangles, s = ceil(L / (limit lambda)) (the reference bisects with MFEM's
+ * GeneralRefinement; both are midpoint rules of the same surface integrals); the centres follow the face order,
+ * and within a face i = 0..s-1, j = 0..s-1-i: the triangle at lattice corner (i,j), then, when i+j <= s-2, the
+ * inverted one next to it.  Call with xyz == NULL to get the count in *ncur, then with arrays of that size:
+ * xyz (ncur*3), J, M (complex ncur*3), area (ncur).  eta_out (optional): the boundaries' wave impedance. */
+int opem_radiation_currents(opem_ctx *ctx, int nattr, const int32_t *attrs, double freq, const double *x,
+                            const double *h, double limit, int64_t *ncur, double *xyz, double *J, double *M,
+                            double *area, double *eta_out);
+
+/* BoundaryDatabase::calculateFarField (port.cpp:8077-8160; Balanis 1989 pp. 288-289) on the GPU: the far E and H
+ * of the currents at the points pts (npts*3) about center: fields = complex npts*4 in the order E_phi, E_theta,
+ * H_phi, H_theta of OPEMpoint::set_fields, with theta, phi of each point about center.  beta = 2 pi freq mu0 / eta
+ * (fem3D.cpp:1926).  kernel_ms (optional): the GPU time of the summation kernels. */
+int opem_far_field(opem_ctx *ctx, int64_t ncur, const double *xyz, const double *J, const double *M,
+                   const double *area, const double *center, double beta, double eta, int64_t npts,
+                   const double *pts, double *fields, double *kernel_ms);
+
+/* Sphere::create, refine, setMetrics, allocateAreasToPoints (pattern.cpp:618-800), host only: the refined
+ * icosahedron about the origin (refine 0..9), its points (the vertices in creation order, then one centre per
+ * triangle), point areas (vertices only; the centres carry 0), triangles and spherical triangle areas about
+ * center.  Call with pts == NULL for the counts *npts, *ntri. */
+int opem_pattern_sphere(double radius, const double *center, int refine, int64_t *npts, double *pts,
+                        double *point_area, int64_t *ntri, int32_t *tris, double *tri_area);
+
+/* Circle::create (pattern.cpp:1468-1504), host only: npoints points of the circle of the given latitude about
+ * center in the x-y plane, rotated by theta about y and then phi about z (about the origin, as rotate_point). */
+int opem_pattern_circle(double radius, const double *center, double theta, double phi, double latitude,
+                        int64_t npoints, double *pts);
+
+/* Pattern metrics (pattern.hpp:98, pattern.cpp:858-868, 1144-1176, 2529-2535), host only: point power
+ * 0.5 (E_theta H_phi* - E_phi H_theta*), radiated power sum(power * point_area) (complex [1]), per-point gain
+ * Re(power / (accepted / total_area)) and directivity Re(power / (radiated / total_area)) (optional, npts each),
+ * summary = {max gain dB, max directivity dB, radiation efficiency Re(radiated / accepted)}. */
+int opem_pattern_metrics(int64_t npts, const double *fields, const double *point_area, double total_area,
+                         const double *accepted_power, double *radiated_power, double *gain, double *directivity,
+                         double *summary);
+
+/* BoundaryDatabase::calculateAcceptedPower (port.cpp:7475-7497), host only: |Cm|^2 Pz at the driven S-port minus
+ * |Cp|^2 Pz of the driven port's mode under every other driving set.  Cp, Cm: opem_sweep's layout for one
+ * frequency (complex nports*nports, [driving set][port]); Pz: the driven port's modal power (complex); sport
+ * 0-based.  (The reference evaluates this inside its driving-set loop, before the later sets are solved; here the
+ * splits of every set are available.) */
+int opem_accepted_power(int nports, const double *Cp, const double *Cm, const double *Pz, int sport,
+                        double *accepted);
+
+/* ---- Mesh errors for adaptive refinement (src/OpenParEM3D/fem3D.cpp:1162-1380, OPEM_L2ZZErrorEstimator.cpp) ---- */
+
+/* fem3D::calculateMeshErrors with OPEM_L2ZZErrorEstimator (OPEM_L2ZZErrorEstimator.cpp:18-164) for nsys H fields
+ * h (complex nsys*ndof, one per driven port; opem_sweep's h_out layout): per element K and field, the flux
+ * (1/mu_r) curl H on K (CurlCurlIntegrator(Inv_mur), fem3D.cpp:1167) is L2-projected onto ND_p by CG on the mass
+ * matrix to the relative residual tol (the reference: 1e-12), the error is int_K |s - flux| for the real and the
+ * imaginary part (norm p = 1), combined as sqrt(e_re^2 + e_im^2) (fem3D.cpp:1221-1224), and errors (ne) keeps the
+ * largest over the fields (fem3D.cpp:1263-1267); errors_sys (optional, ne*nsys) each field's.  The integral uses a
+ * collapsed Gauss-Jacobi rule exact to degree 2p+3 (MFEM's rule differs: the integrand |.| is not polynomial). */
+int opem_mesh_errors(opem_ctx *ctx, int nsys, const double *h, double tol, double *errors, double *errors_sys);
+
+/* The marking of fem3D::calculateMeshErrors (fem3D.cpp:1300-1325), host only: the count = NE * fraction
+ * (truncated, at least 1) elements of largest error, largest first (equal errors in ascending element order),
+ * into elements[count] (optional), and the largest error (maxMeshError, optional). */
+int opem_mark_elements(int64_t ne, const double *errors, double fraction, int64_t *count, int32_t *elements,
+                       double *max_error);
+
+/* Conforming local refinement of the marked tets (ParMesh::GeneralRefinement in fem3D::refineMesh,
+ * fem3D.cpp:1384-1447), host only: each marked tet not yet split is bisected at the midpoint of its longest edge,
+ * and every tet and boundary triangle holding that edge is split with it, so the mesh stays conforming; children
+ * keep their parent's attribute and orientation.  (MFEM bisects along its own refinement edges; the refined meshes
+ * differ, both conforming.)  Outputs: vertices (the input ones first), tets, boundary faces and attributes;
+ * parent (optional, nt_out): the input tet of each output tet.  Call with xyz_out == NULL for the counts. */
+int opem_refine_mesh(int64_t nv, const double *xyz, int64_t nt, const int32_t *tets, const int32_t *tet_attr,
+                     int64_t nbf, const int32_t *bfaces, const int32_t *bface_attr, int64_t nmarked,
+                     const int32_t *marked, int64_t *nv_out, double *xyz_out, int64_t *nt_out, int32_t *tets_out,
+                     int32_t *tet_attr_out, int64_t *nbf_out, int32_t *bfaces_out, int32_t *bface_attr_out,
+                     int32_t *parent);
+
 /* Allocation-free variant of opem_spmv (same arguments, same kernel, identical y).  It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The caller's rowptr, colind, vals, x and y are staged in a 64 MiB

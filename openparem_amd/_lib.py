@@ -16,7 +16,9 @@ SYMBOLS = [
     "opem_krylov_bench", "opem_sweep_apply", "opem_csr_spmv_bench", "opem_spmv_noalloc",
     "opem_lu_stats", "opem_solve", "opem_hfield", "opem_port_project", "opem_results_error", "opem_s2z",
     "opem_z2s", "opem_renormalize", "opem_write_csv", "opem_write_touchstone", "opem_save_as_test",
-    "opem_process_tests", "opem_port_mode",
+    "opem_process_tests", "opem_port_mode", "opem_radiation_currents", "opem_far_field", "opem_pattern_sphere",
+    "opem_pattern_circle", "opem_pattern_metrics", "opem_accepted_power", "opem_mesh_errors", "opem_mark_elements",
+    "opem_refine_mesh",
 ]
 
 BDR = {"pec": 1, "pmc": 2, "port": 3, "zs": 4, "radiation": 5}
@@ -87,6 +89,18 @@ def lib():
                                   i32p, i32p]
         L.opem_port_project.argtypes = [P, C.c_int, f64p, f64p, f64p, f64p, f64p]
         L.opem_port_mode.argtypes = [P, C.c_int, C.c_double, f64p, f64p, f64p, f64p, f64p, f64p]
+        L.opem_radiation_currents.argtypes = [P, C.c_int, i32p, C.c_double, f64p, f64p, C.c_double, i64p, f64p,
+                                              f64p, f64p, f64p, f64p]
+        L.opem_far_field.argtypes = [P, C.c_int64, f64p, f64p, f64p, f64p, f64p, C.c_double, C.c_double, C.c_int64,
+                                     f64p, f64p, f64p]
+        L.opem_pattern_sphere.argtypes = [C.c_double, f64p, C.c_int, i64p, f64p, f64p, i64p, i32p, f64p]
+        L.opem_pattern_circle.argtypes = [C.c_double, f64p, C.c_double, C.c_double, C.c_double, C.c_int64, f64p]
+        L.opem_pattern_metrics.argtypes = [C.c_int64, f64p, f64p, C.c_double, f64p, f64p, f64p, f64p, f64p]
+        L.opem_accepted_power.argtypes = [C.c_int, f64p, f64p, f64p, C.c_int, f64p]
+        L.opem_mesh_errors.argtypes = [P, C.c_int, f64p, C.c_double, f64p, f64p]
+        L.opem_mark_elements.argtypes = [C.c_int64, f64p, C.c_double, i64p, i32p, f64p]
+        L.opem_refine_mesh.argtypes = [C.c_int64, f64p, C.c_int64, i32p, i32p, C.c_int64, i32p, i32p, C.c_int64, i32p,
+                                       i64p, f64p, i64p, i32p, i32p, i64p, i32p, i32p, i32p]
         L.opem_element_matrices.argtypes = [P, C.c_int, C.c_int, C.c_int64, f64p, f64p, f64p, C.c_double,
                                             C.c_int, f64p, f64p]
         L.opem_sweep.argtypes = [P, C.POINTER(SweepIn), C.POINTER(SweepOut)]

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libopem.so")
 SOURCES = ["kernels.hip", "elemmat.hip", "opem.hip", "lu.hip", "nd_tables.cpp", "space.cpp", "quadrature.cpp",
-           "coarse.cpp", "poly.cpp", "lu_symbolic.cpp", "results.cpp"]
+           "coarse.cpp", "poly.cpp", "lu_symbolic.cpp", "results.cpp", "farfield.hip", "amr.hip", "refine.cpp"]
 ARCH = "gfx950"
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -42,6 +42,16 @@ namespace opem {
 static const double kMu0 = 4e-7 * M_PI;
 static const double kEps0 = 8.8541878176e-12;  // OpenParEM3D.cpp:216
 
+// element errors of the ZZ estimate on the GPU (amr.hip)
+void zz_errors_run(hipStream_t st, int p, int64_t ne, int S, int64_t n, const std::vector<double> &coords,
+                   const int32_t *d_edofs, const std::vector<double> &inv_mur, const double2 *d_H, const double2 *d_S,
+                   double *err, double *err_sys);
+
+// far field of surface currents on the GPU (farfield.hip)
+void far_field_run(hipStream_t st, int64_t ncur, const double *xyz, const double *J, const double *M,
+                   const double *area, const double *center, double beta, double eta, int64_t npts,
+                   const double *pts, double *out, double *kernel_ms);
+
 struct Material {
   double er = 1.0, tand = 0.0, mur = 1.0;
 };
@@ -1330,7 +1340,23 @@ static void build_h_patches(Ctx &c) {
   c.hpatch_state = 1;
 }
 
-static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, double htol, DevBuf &h) {
+// Qs = sum_slot Q_slot / (mu0 mur_slot): the H rhs is (j/w) Qs x (materials must be set)
+static void build_qs(Ctx &c) {
+  const int64_t nnz = c.sp.nnz;
+  const int nslot = (int)c.slot_attr.size();
+  std::vector<double> wq(nslot);
+  for (int sl = 0; sl < nslot; sl++) wq[sl] = 1.0 / (kMu0 * c.mat.at(c.slot_attr[sl]).mur);
+  DevBuf dwq;
+  dwq.upload(wq, c.st);
+  c.d_Qs.alloc((size_t)nnz * 16);
+  hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for(nnz, 65536)), dim3(256), 0, c.st, nnz, nslot, c.d_Q.as<double>(),
+                     dwq.as<double>(), c.d_Qs.as<double2>());
+  HIPCHK(hipStreamSynchronize(c.st));
+}
+
+// P h = scale_i Qs x_i for the S systems x (device, system-major): the H projection (scale j/omega) and the
+// smooth flux of the mesh-error estimate (scale mu0), by batched CG on the unit ND mass P.
+static void h_solve(Ctx &c, const double2 *x, int S, const std::vector<cplx> &scale, double htol, DevBuf &h) {
   const Space &s = c.sp;
   hipStream_t st = c.st;
   const int64_t n = s.ndof, nnz = s.nnz;
@@ -1387,14 +1413,14 @@ static void h_solve(Ctx &c, Work &W, int S, const std::vector<double> &omega, do
                          a);
     }
   };
-  // b = Qs x, then scale by j/w per system
+  // b = Qs x, then scale per system
   launch_spmv<double2, double2>(st, S, 1, n, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), c.d_Qs.as<double2>(),
-                                nnz, c.d_onesmask.as<uint8_t>(), W.x.as<double2>(), b.as<double2>(), nullptr, nullptr,
+                                nnz, c.d_onesmask.as<uint8_t>(), x, b.as<double2>(), nullptr, nullptr,
                                 0, c.lpr, 0);
   std::vector<double> sc(S * 2);
   for (int i = 0; i < S; i++) {
-    sc[i * 2] = 0.0;
-    sc[i * 2 + 1] = 1.0 / omega[i];
+    sc[i * 2] = scale[i].real();
+    sc[i * 2 + 1] = scale[i].imag();
   }
   DevBuf dsc;
   dsc.upload(sc, st);
@@ -1623,16 +1649,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
   SweepOperator sop;
   build_sweep_operator(c, ov, false, sop);
 
-  // Qs = sum_slot Q_slot / (mu0 mur_slot): H rhs = (j/w) Qs x
-  {
-    std::vector<double> wq(nslot);
-    for (int sl = 0; sl < nslot; sl++) wq[sl] = 1.0 / (kMu0 * c.mat.at(c.slot_attr[sl]).mur);
-    DevBuf dwq;
-    dwq.upload(wq, st);
-    c.d_Qs.alloc((size_t)nnz * 16);
-    hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for(nnz, 65536)), dim3(256), 0, st, nnz, nslot, c.d_Q.as<double>(),
-                       dwq.as<double>(), c.d_Qs.as<double2>());
-  }
+  build_qs(c);
   // Dirichlet masks per excitation: PEC U driven port
   std::vector<uint8_t> pec(n, 0);
   for (int64_t f = 0; f < c.mesh.nbf; f++) {
@@ -2023,10 +2040,10 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     const double tc = now_s();
     t_solve += tc - tb;
     // H projection
-    std::vector<double> omega(S);
-    for (int i = 0; i < S; i++) omega[i] = 2 * M_PI * in->freqs[ford[f0 + i / R]];
+    std::vector<cplx> hscale(S);
+    for (int i = 0; i < S; i++) hscale[i] = cplx(0.0, 1.0 / (2 * M_PI * in->freqs[ford[f0 + i / R]]));
     DevBuf h;
-    h_solve(c, W, S, omega, htol, h);
+    h_solve(c, W.x.as<double2>(), S, hscale, htol, h);
     // projections: e0 (x with uE), h0 (h with uH)
     DevBuf du, dout;
     du.upload(uvec, st);
@@ -3223,6 +3240,175 @@ int opem_port_project(opem_ctx *ctx, int attr, const double *mode_E, const doubl
       splits[2 * k + 1] = out[k].imag();
     }
   });
+}
+
+int opem_radiation_currents(opem_ctx *ctx, int nattr, const int32_t *attrs, double freq, const double *x,
+                            const double *h, double limit, int64_t *ncur, double *xyz, double *J, double *M,
+                            double *area, double *eta_out) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (!(freq > 0) || !(limit > 0) || !ncur || nattr < 0 || (nattr > 0 && !attrs))
+      throw std::invalid_argument("bad arguments");
+    const bool fill = xyz != nullptr;
+    if (fill && (!x || !h || !J || !M || !area)) throw std::invalid_argument("bad arguments");
+    // the radiation boundaries taken, all with one wave impedance (fem3D.cpp:1904-1919)
+    std::vector<int> sel;
+    if (nattr == 0) {
+      for (auto &kv : ctx->bdr)
+        if (kv.second.kind == OPEM_BDR_RADIATION) sel.push_back(kv.first);
+    } else {
+      for (int k = 0; k < nattr; k++) {
+        auto f = ctx->bdr.find(attrs[k]);
+        if (f == ctx->bdr.end() || f->second.kind != OPEM_BDR_RADIATION)
+          throw std::invalid_argument("attribute " + std::to_string(attrs[k]) + " is not a radiation boundary");
+        sel.push_back(attrs[k]);
+      }
+    }
+    if (sel.empty()) throw std::invalid_argument("no radiation boundary");
+    double eta = -1;
+    for (int a : sel) {
+      double e = ctx->bdr.at(a).value;
+      if (e <= 0) e = std::sqrt(M_PI * 4e-7 / kEps0);  // port.cpp:725-727 default
+      if (eta > 0 && std::fabs(e - eta) > 1e-12 * eta)
+        throw std::invalid_argument("radiation boundaries with different wave impedances");
+      eta = e;
+    }
+    if (eta_out) *eta_out = eta;
+    const double lam = eta / (kMu0 * freq);  // 2 pi / beta, beta = 2 pi f mu0 / eta (fem3D.cpp:1926-1929)
+    const Mesh &m = ctx->mesh;
+    const Space &s = ctx->sp;
+    const int ndf = s.ndf;
+    std::vector<double> phi((size_t)ndf * 3);
+    int64_t n = 0;
+    for (int64_t f = 0; f < m.nbf; f++) {
+      if (std::find(sel.begin(), sel.end(), m.bface_attr[f]) == sel.end()) continue;
+      double v[3][3];
+      for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++) v[a][d] = m.xyz[3 * (int64_t)s.bf_sorted[3 * f + a] + d];
+      double L = 0;
+      for (int a = 0; a < 3; a++)
+        for (int b = a + 1; b < 3; b++) {
+          const double dx = v[b][0] - v[a][0], dy = v[b][1] - v[a][1], dz = v[b][2] - v[a][2];
+          L = std::max(L, std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+      // uniform split into sp*sp similar triangles, no edge longer than limit wavelengths
+      const int sp = std::max(1, (int)std::ceil(L / (limit * lam)));
+      if ((int64_t)sp * sp > 1000000) throw std::invalid_argument("limit too small for the boundary faces");
+      if (!fill) {
+        n += (int64_t)sp * sp;
+        continue;
+      }
+      // outward unit normal: away from the owning tet's vertex that is not on the face
+      double e1[3], e2[3], nn[3], c[3], opp[3] = {0, 0, 0};
+      for (int d = 0; d < 3; d++) {
+        e1[d] = v[1][d] - v[0][d];
+        e2[d] = v[2][d] - v[0][d];
+        c[d] = (v[0][d] + v[1][d] + v[2][d]) / 3.0;
+      }
+      nn[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      nn[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      nn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      const double a2 = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+      const int32_t *tv = &s.tets_sorted[4 * (int64_t)s.bface_owner[f]];
+      for (int q = 0; q < 4; q++) {
+        const int32_t w = tv[q];
+        if (w != s.bf_sorted[3 * f] && w != s.bf_sorted[3 * f + 1] && w != s.bf_sorted[3 * f + 2])
+          for (int d = 0; d < 3; d++) opp[d] = m.xyz[3 * (int64_t)w + d];
+      }
+      const double sg = (nn[0] * (c[0] - opp[0]) + nn[1] * (c[1] - opp[1]) + nn[2] * (c[2] - opp[2])) < 0 ? -1 : 1;
+      for (int d = 0; d < 3; d++) nn[d] *= sg / a2;
+      const double sub_area = 0.5 * a2 / ((double)sp * sp);
+      const int32_t *fd = &s.bface_dofs[(int64_t)f * ndf];
+      auto emit = [&](double bx, double by) {
+        const double Lb[3] = {1.0 - bx - by, bx, by};
+        tri_basis_eval(s.p, v, Lb, phi.data());
+        cplx Et[3] = {0, 0, 0}, Ht[3] = {0, 0, 0};
+        for (int a = 0; a < ndf; a++) {
+          const int64_t g = fd[a];
+          const cplx xe(x[2 * g], x[2 * g + 1]), xh(h[2 * g], h[2 * g + 1]);
+          for (int d = 0; d < 3; d++) {
+            Et[d] += phi[3 * a + d] * xe;
+            Ht[d] += phi[3 * a + d] * xh;
+          }
+        }
+        const cplx Jv[3] = {nn[1] * Ht[2] - nn[2] * Ht[1], nn[2] * Ht[0] - nn[0] * Ht[2], nn[0] * Ht[1] - nn[1] * Ht[0]};
+        const cplx Mv[3] = {-(nn[1] * Et[2] - nn[2] * Et[1]), -(nn[2] * Et[0] - nn[0] * Et[2]),
+                            -(nn[0] * Et[1] - nn[1] * Et[0])};
+        for (int d = 0; d < 3; d++) {
+          xyz[3 * n + d] = Lb[0] * v[0][d] + Lb[1] * v[1][d] + Lb[2] * v[2][d];
+          J[6 * n + 2 * d] = Jv[d].real();
+          J[6 * n + 2 * d + 1] = Jv[d].imag();
+          M[6 * n + 2 * d] = Mv[d].real();
+          M[6 * n + 2 * d + 1] = Mv[d].imag();
+        }
+        area[n] = sub_area;
+        n++;
+      };
+      for (int i = 0; i < sp; i++)
+        for (int j = 0; j < sp - i; j++) {
+          emit((3 * i + 1) / (3.0 * sp), (3 * j + 1) / (3.0 * sp));
+          if (i + j <= sp - 2) emit((3 * i + 2) / (3.0 * sp), (3 * j + 2) / (3.0 * sp));
+        }
+    }
+    if (fill && n != *ncur) throw std::invalid_argument("ncur does not match the count of the first call");
+    *ncur = n;
+  });
+}
+
+int opem_far_field(opem_ctx *ctx, int64_t ncur, const double *xyz, const double *J, const double *M,
+                   const double *area, const double *center, double beta, double eta, int64_t npts,
+                   const double *pts, double *fields, double *kernel_ms) {
+  return guarded(ctx, [&] {
+    if (ncur < 0 || npts < 0 || !center || !(beta > 0) || !(eta > 0) || !fields || (npts > 0 && !pts) ||
+        (ncur > 0 && (!xyz || !J || !M || !area)))
+      throw std::invalid_argument("bad arguments");
+    if (ncur > ((int64_t)1 << 40) / 128 || npts > ((int64_t)1 << 31)) throw std::invalid_argument("too large");
+    far_field_run(ctx->st, ncur, xyz, J, M, area, center, beta, eta, npts, pts, fields, kernel_ms);
+  });
+}
+
+int opem_mesh_errors(opem_ctx *ctx, int nsys, const double *h, double tol, double *errors, double *errors_sys) {
+  return guarded(ctx, [&] {
+    if (!ctx->have_space) throw std::invalid_argument("space not built");
+    if (nsys < 1 || nsys > 256 || !h || !errors || !(tol > 0 && tol < 1)) throw std::invalid_argument("bad arguments");
+    Ctx &c = *ctx;
+    for (int sl = 0; sl < (int)c.slot_attr.size(); sl++)
+      if (!c.mat.count(c.slot_attr[sl])) throw std::invalid_argument("material not set");
+    const Space &s = c.sp;
+    const int64_t n = s.ndof, ne = c.mesh.nt;
+    build_qs(c);
+    DevBuf dh, sm;
+    dh.alloc((size_t)nsys * n * 16);
+    HIPCHK(hipMemcpyAsync(dh.p, h, (size_t)nsys * n * 16, hipMemcpyHostToDevice, c.st));
+    // smooth flux: P s = b, b = sum_K int_K (curl H / mu_r) . phi = mu0 Qs H (OPEM_L2ZZErrorEstimator.cpp:62-117)
+    h_solve(c, dh.as<double2>(), nsys, std::vector<cplx>(nsys, cplx(kMu0, 0.0)), tol, sm);
+    std::vector<double> coords((size_t)ne * 12), inv_mur(ne);
+    for (int64_t e = 0; e < ne; e++) {
+      for (int a = 0; a < 4; a++)
+        for (int d = 0; d < 3; d++) coords[e * 12 + 3 * a + d] = c.mesh.xyz[3 * (int64_t)s.tets_sorted[4 * e + a] + d];
+      inv_mur[e] = 1.0 / c.mat.at(c.slot_attr[c.elem_slot[e]]).mur;
+    }
+    zz_errors_run(c.st, s.p, ne, nsys, n, coords, c.d_elem_dofs.as<int32_t>(), inv_mur, dh.as<double2>(),
+                  sm.as<double2>(), errors, errors_sys);
+  });
+}
+
+int opem_mark_elements(int64_t ne, const double *errors, double fraction, int64_t *count, int32_t *elements,
+                       double *max_error) {
+  if (ne < 1 || !errors || !(fraction >= 0 && fraction <= 1) || !count) return -2;
+  // refinementCount = NE * fraction truncated, at least 1 (fem3D.cpp:1300-1301)
+  int64_t k = (int64_t)((double)ne * fraction);
+  if (k < 1) k = 1;
+  *count = k;
+  std::vector<int32_t> idx(ne);
+  for (int64_t e = 0; e < ne; e++) idx[e] = (int32_t)e;
+  // largest first; equal errors in ascending element order
+  std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](int32_t a, int32_t b) {
+    return errors[a] > errors[b] || (errors[a] == errors[b] && a < b);
+  });
+  if (elements) std::memcpy(elements, idx.data(), (size_t)k * 4);
+  if (max_error) *max_error = errors[idx[0]];
+  return 0;
 }
 
 int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {

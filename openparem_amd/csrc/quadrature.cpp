@@ -91,4 +91,23 @@ void tri_rule(int n, std::vector<double> &L, std::vector<double> &w) {
     }
 }
 
+void tet_rule(int n, std::vector<double> &L, std::vector<double> &w) {
+  std::vector<double> u, wu, v, wv, s, ws;
+  gauss_jacobi01(n, 2.0, u, wu);
+  gauss_jacobi01(n, 1.0, v, wv);
+  gauss_jacobi01(n, 0.0, s, ws);
+  L.clear();
+  w.clear();
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      for (int k = 0; k < n; k++) {
+        const double x = u[i], y = v[j] * (1.0 - u[i]), z = s[k] * (1.0 - u[i]) * (1.0 - v[j]);
+        L.push_back(1.0 - x - y - z);
+        L.push_back(x);
+        L.push_back(y);
+        L.push_back(z);
+        w.push_back(wu[i] * wv[j] * ws[k]);
+      }
+}
+
 }  // namespace opem

@@ -9,6 +9,8 @@
 
     python -m openparem_amd.measure kernels  # the roofline kernels alone at the guide benchmark's shape (profiling)
 
+    python -m openparem_amd.measure far_field  # the far-field kernels at the config-4 radiation lid's size
+
     python -m openparem_amd.measure c5   # the seeded mixed set of workloads.c5_instances(): one batched mixed
                                          # sweep per instance, plain FP64 at its middle frequency
 
@@ -253,6 +255,42 @@ def kernels(restart=300, ns=3):
     torch.cuda.synchronize()
 
 
+def far_field(solver=None, freq=20e9, limit=0.05, refine=4, nrep=5):
+    """The far-field kernels (opem_far_field) at the size of the config-4 microstrip's radiation lid: its currents
+    at ``freq`` split to ``limit`` wavelengths (synthetic values at the real positions and areas) on the pattern
+    sphere refined ``refine`` times; FP64, bound by the vector FP64 rate (78.6 TFLOP/s: 53 flops per current-point
+    pair plus one sincos, which is not counted)."""
+    from . import pattern as pat
+    own = solver is None
+    if own:
+        w = workloads.make("microstrip-c4")
+        solver = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+    try:
+        eta = np.sqrt(4e-7 * np.pi / 8.8541878176e-12)
+        n = solver.ndof
+        z = np.zeros(n, np.complex128)
+        cur = pat.radiation_currents(solver, z, z, freq, limit)
+        rng = np.random.default_rng(0)
+        nc = cur["area"].size
+        cur["J"] = rng.normal(size=(nc, 3)) + 1j * rng.normal(size=(nc, 3))
+        cur["M"] = rng.normal(size=(nc, 3)) + 1j * rng.normal(size=(nc, 3))
+        beta = 2 * np.pi * freq * 4e-7 * np.pi / eta
+        xyz = solver.mesh.xyz
+        center = 0.5 * (xyz.min(axis=0) + xyz.max(axis=0))
+        sph = pat.Sphere(1e5 * 2 * np.pi / beta, center, refine)
+        pat.far_field(solver, cur, center, beta, eta, sph.pts)
+        ms = min(pat.far_field(solver, cur, center, beta, eta, sph.pts, timing=True)[1] for _ in range(nrep))
+        pairs = nc * sph.pts.shape[0]
+        d = dict(kernel="k_far_field_part + k_far_field_sum (FP64, current-point pairs staged through LDS)",
+                 currents=int(nc), points=int(sph.pts.shape[0]), ms=ms, pairs_per_s=pairs / (ms * 1e-3),
+                 tflops=pairs * 53 / (ms * 1e-3) / 1e12, peak_tflops=78.6,
+                 frac=pairs * 53 / (ms * 1e-3) / 1e12 / 78.6)
+        return d
+    finally:
+        if own:
+            solver.close()
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c3"
     if which == "lu_pmc":
@@ -260,4 +298,4 @@ if __name__ == "__main__":
     elif which == "c4lu":
         c4lu(share=int(sys.argv[2]) if len(sys.argv) > 2 else 1)
     else:
-        {"c2": c2, "c3": c3, "c4": c4, "c5": c5, "kernels": kernels}[which]()
+        {"c2": c2, "c3": c3, "c4": c4, "c5": c5, "kernels": kernels, "far_field": lambda: _emit(far_field())}[which]()
