@@ -44,8 +44,10 @@ class LuFactor {
   // z = A^{-1} v for S = ninst * nrhs systems: system s uses instance s / nrhs and excitation s % nexc.
   // v: system-major, v[s * ldv + dof]; z: system-interleaved, z[dof * S + s] (rows outside the factored set and the
   // driven port's rows are zero).  X = double2 or float2.
+  // active (optional, device, ninst * nrhs): per system; an instance none of whose systems is active is skipped
+  // (its z entries are left zero), so converged systems cost no factor traffic.
   template <typename X>
-  void solve(int nrhs, const X *v, int64_t ldv, X *z);
+  void solve(int nrhs, const X *v, int64_t ldv, X *z, const int32_t *active = nullptr);
 
   // Accumulated kernel timings (HIP events, only while OPEM_LU_TIMING is set in the environment) and counts:
   // out[0..2] trailing-update seconds, real flops, launches; out[3..4] panel seconds, launches; out[5..7] apply

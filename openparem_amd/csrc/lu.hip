@@ -91,6 +91,7 @@ struct LuDev {
   const int64_t *dbeg, *uoff, *foff, *fvoff;
   const int32_t *uidx, *umap, *parent, *kid_off, *kids;
   int64_t n, ppqs, fstride, fvstride;
+  const int32_t *iact;  // apply: per instance, 0 when none of its systems is active (its blocks return at once)
 };
 
 // ---------------------------------------------------------------- factorization kernels
@@ -796,8 +797,9 @@ __global__ void k_lu_root_gather(const T *__restrict__ S, int64_t sstride, int n
 
 template <typename X>
 __global__ void k_lu_gather_rhs(int64_t n, int nrhs, const int32_t *__restrict__ perm, const X *__restrict__ v,
-                                int64_t ldv, double2 *__restrict__ b) {
+                                int64_t ldv, double2 *__restrict__ b, const int32_t *__restrict__ iact) {
   const int q = blockIdx.y;
+  if (iact && !iact[q]) return;
   const int64_t tot = n * nrhs;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t pos = e / nrhs;
@@ -808,8 +810,9 @@ __global__ void k_lu_gather_rhs(int64_t n, int nrhs, const int32_t *__restrict__
 
 template <typename X>
 __global__ void k_lu_scatter(int64_t n, int nrhs, int64_t S, const int32_t *__restrict__ perm,
-                             const double2 *__restrict__ x, X *__restrict__ z) {
+                             const double2 *__restrict__ x, X *__restrict__ z, const int32_t *__restrict__ iact) {
   const int q = blockIdx.y;
+  if (iact && !iact[q]) return;
   const int64_t tot = n * nrhs;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t pos = e / nrhs;
@@ -818,12 +821,22 @@ __global__ void k_lu_scatter(int64_t n, int nrhs, int64_t S, const int32_t *__re
   }
 }
 
+// Instance q is active when any of its right-hand sides (systems q nrhs .. q nrhs + nrhs - 1) is.
+__global__ void k_lu_inst_active(int ninst, int nrhs, const int32_t *__restrict__ act, int32_t *__restrict__ iact) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ninst) return;
+  int a = 0;
+  for (int j = 0; j < nrhs; j++) a |= act[(int64_t)q * nrhs + j] != 0;
+  iact[q] = a;
+}
+
 // Front vectors of the nodes of one level: pivot rows from b, update rows zero, then each child's update part
 // added at its umap rows (children one after another inside the workgroup).
 __global__ __launch_bounds__(256) void k_lu_fwd_gather(const int32_t *__restrict__ nodes, LuDev D, int nrhs,
                                                        const double2 *__restrict__ ball, double2 *__restrict__ fvall) {
-  const int t = nodes[blockIdx.x];
   const int q = blockIdx.y;
+  if (D.iact && !D.iact[q]) return;
+  const int t = nodes[blockIdx.x];
   const int64_t nd = D.dbeg[t + 1] - D.dbeg[t], nf = nd + (D.uoff[t + 1] - D.uoff[t]);
   double2 *fv = fvall + q * D.fvstride + D.fvoff[t] * nrhs;
   const double2 *b = ball + q * D.n * nrhs + D.dbeg[t] * nrhs;
@@ -907,8 +920,9 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
                                                        const double2 *__restrict__ yrall, double2 *__restrict__ part,
                                                        int64_t part_qstride, int kch) {
   extern __shared__ double2 sin[];  // kKC x nsub staged inputs
-  const int4 tk = tasks[blockIdx.x];
   const int q = blockIdx.y;
+  if (D.iact && !D.iact[q]) return;
+  const int4 tk = tasks[blockIdx.x];
   const int rows_wg = 256 / ty_n;
   const int tx = threadIdx.x % rows_wg, ty = threadIdx.x / rows_wg;
   int t, e;
@@ -1008,8 +1022,11 @@ __global__ __launch_bounds__(256) void k_lu_apply_part(const int4 *__restrict__ 
 // complex products by the three-product form (P1 = Ar.Xr, P2 = Ai.Xi, P3 = (Ar + Ai)(Xr + Xi); Re = P1 - P2,
 // Im = P3 - P1 - P2).  Sums over a chunk are FP32 (the factor and the inner Krylov vectors are FP32); the slots are
 // added in FP64 by k_lu_apply_sum.  The triangular blocks are masked per element (unit lower: k < i, diagonal 1;
-// upper: k >= i).
-template <int KIND>
+// upper: k >= i).  NARROW (at most 8 right-hand sides): the second operand holds the real parts of the 8 inputs in
+// columns 0-7 and their imaginary parts in columns 8-15, so two products P1 = Ar.[Xr|Xi], P2 = Ai.[Xr|Xi] give
+// Re = P1[:, m] - P2[:, m + 8] and Im = P1[:, m + 8] + P2[:, m] (lanes m and m + 8 exchange in the epilogue): two
+// MFMAs per block instead of three and 32 accumulators instead of 48.
+template <int KIND, bool NARROW>
 __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
                                                        const float2 *__restrict__ Fall,
                                                        const float2 *__restrict__ Eall,
@@ -1020,9 +1037,11 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
                                                        const double2 *__restrict__ xall,
                                                        const double2 *__restrict__ yrall, double2 *__restrict__ part,
                                                        int64_t part_qstride, int kch) {
-  __shared__ float2 sX[kKC][16];  // staged inputs: column k of the chunk, right-hand side m
-  const int4 tk = tasks[blockIdx.x];
+  constexpr int kKM = 128;         // columns staged per pass (one pass for the default chunk of 128 columns)
+  __shared__ float2 sX[kKM][16];  // staged inputs: column k of the chunk, right-hand side m
   const int q = blockIdx.y;
+  if (D.iact && !D.iact[q]) return;
+  const int4 tk = tasks[blockIdx.x];
   int t, e;
   int64_t rows, cols, ld, dbeg, nd, nf;
   apply_geom<KIND>(tk, D, Rt, rows, cols, ld, t, e, dbeg, nd, nf);
@@ -1049,57 +1068,39 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
   const int64_t rbase = tk.y + 64 * w;  // first row of this wave
   const int64_t c_beg = tk.z, c_end = min((int64_t)tk.z + kch, cols);
-  lu_f4 p1[4], p2[4], p3[4];
+  lu_f4 p1[4], p2[4], p3[NARROW ? 1 : 4];
 #pragma unroll
   for (int rb = 0; rb < 4; rb++) {
     p1[rb] = lu_f4{0, 0, 0, 0};
     p2[rb] = lu_f4{0, 0, 0, 0};
-    p3[rb] = lu_f4{0, 0, 0, 0};
+    if (!NARROW) p3[rb] = lu_f4{0, 0, 0, 0};
   }
   // the rows this lane's first-operand values come from (clamped, so every load stays inside the block)
   int64_t arow[4];
 #pragma unroll
   for (int rb = 0; rb < 4; rb++) arow[rb] = min(rbase + 16 * rb + li, rows - 1);
-  for (int64_t kc = c_beg; kc < c_end; kc += kKC) {
-    const int kn = (int)min((int64_t)kKC, c_end - kc);
-    for (int s = threadIdx.x; s < kKC * 16; s += 256) {
-      const int kk = s >> 4, m = s & 15;
-      float2 v = make_float2(0.f, 0.f);
-      if (kk < kn && m < nsub) {
-        const int j = j0 + m * rstep;
-        const int64_t k = kc + kk;
-        double2 dv;
-        if (KIND == K_FWD_L)
-          dv = fv[(D.fvoff[t] + pp[k]) * nrhs + j];
-        else if (KIND == K_FWD_L21)
-          dv = y[(dbeg + k) * nrhs + j];
-        else if (KIND == K_BWD_U12)
-          dv = x[(int64_t)D.uidx[D.uoff[t] + k] * nrhs + j];
-        else if (KIND == K_BWD_U)
-          dv = tt[(dbeg + k) * nrhs + j];
-        else if (KIND == K_ROOT_L)
-          dv = fv[(D.fvoff[t] + O[rpp[k]]) * nrhs + j];
-        else
-          dv = yr[k * nrhs + j];
-        v = make_float2((float)dv.x, (float)dv.y);
-      }
-      sX[kk][m] = v;
-    }
-    __syncthreads();
-    const int nk4 = (kn + 3) >> 2;
-#pragma unroll 4
-    for (int kb = 0; kb < nk4; kb++) {
-      const int kk = 4 * kb + lk;  // this lane's column of the 16 x 4 block (and row of the 4 x 16 one)
-      const int64_t kg = kc + min(kk, kn - 1);
-      const float2 xv = sX[kk][li];
-      const float xs = xv.x + xv.y;
-      float2 av[4];
+  constexpr int kGB = 2;
+  // one group = kGB column blocks of 4 (4 kGB columns): 4 kGB factor values per lane, loaded before they are used
+  auto load_group = [&](int g, float2 *buf, int64_t kc, int kn) {
 #pragma unroll
-      for (int rb = 0; rb < 4; rb++) av[rb] = M[arow[rb] + ld * kg];
+    for (int u = 0; u < kGB; u++) {
+      const int64_t kg = kc + min(4 * kGB * g + 4 * u + lk, kn - 1);
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) buf[4 * u + rb] = M[arow[rb] + ld * kg];
+    }
+  };
+  auto mfma_group = [&](int g, const float2 *buf, int64_t kc, int kn) {
+#pragma unroll
+    for (int u = 0; u < kGB; u++) {
+      const int kk = 4 * kGB * g + 4 * u + lk;  // this lane's column of the 16 x 4 block (and row of the 4 x 16 one)
+      const int64_t kg = kc + min(kk, kn - 1);
+      const float2 xv = sX[min(kk, kKM - 1)][NARROW ? (li & 7) : li];
+      const float xs = xv.x + xv.y;
+      const float xq = li < 8 ? xv.x : xv.y;  // NARROW: [Xr | Xi]
 #pragma unroll
       for (int rb = 0; rb < 4; rb++) {
         const int64_t ig = rbase + 16 * rb + li;
-        float2 a = av[rb];
+        float2 a = buf[4 * u + rb];
         bool use = kk < kn;
         if (KIND == K_FWD_L || KIND == K_ROOT_L) {
           if (kg == ig) a = make_float2(1.f, 0.f);
@@ -1108,16 +1109,95 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
           use = use && kg >= ig;
         }
         if (!use) a = make_float2(0.f, 0.f);
-        p1[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, xv.x, p1[rb], 0, 0, 0);
-        p2[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, xv.y, p2[rb], 0, 0, 0);
-        p3[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x + a.y, xs, p3[rb], 0, 0, 0);
+        if (NARROW) {
+          p1[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, xq, p1[rb], 0, 0, 0);
+          p2[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, xq, p2[rb], 0, 0, 0);
+        } else {
+          p1[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, xv.x, p1[rb], 0, 0, 0);
+          p2[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, xv.y, p2[rb], 0, 0, 0);
+          p3[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x + a.y, xs, p3[rb], 0, 0, 0);
+        }
       }
     }
+  };
+  for (int64_t kc = c_beg; kc < c_end; kc += kKM) {
+    const int kn = (int)min((int64_t)kKM, c_end - kc);
+    // the first factor group does not depend on the inputs: its loads go out before the staging
+    float2 bufA[4 * kGB], bufB[4 * kGB];
+    load_group(0, bufA, kc, kn);
+    // stage the inputs: every load issued (clamped, unconditional) before the LDS stores; at most 8 right-hand
+    // sides (the usual case) take 4 loads per thread, 9 to 16 take 8
+    auto in_at = [&](int kk, int m) -> double2 {
+      const int j = j0 + m * rstep;
+      const int64_t k = kc + kk;
+      if (KIND == K_FWD_L) return fv[(D.fvoff[t] + pp[k]) * nrhs + j];
+      if (KIND == K_FWD_L21) return y[(dbeg + k) * nrhs + j];
+      if (KIND == K_BWD_U12) return x[(int64_t)D.uidx[D.uoff[t] + k] * nrhs + j];
+      if (KIND == K_BWD_U) return tt[(dbeg + k) * nrhs + j];
+      if (KIND == K_ROOT_L) return fv[(D.fvoff[t] + O[rpp[k]]) * nrhs + j];
+      return yr[k * nrhs + j];
+    };
+    // half h of the right-hand sides (m = 8 h .. 8 h + 7); the second half only when there are more than 8
+    for (int h = 0; h < (nsub > 8 ? 2 : 1); h++) {
+      constexpr int kSt = kKM * 8 / 256;
+      double2 dv[kSt];
+#pragma unroll
+      for (int u = 0; u < kSt; u++) {
+        const int s = threadIdx.x + 256 * u;
+        dv[u] = in_at(min(s >> 3, kn - 1), min(8 * h + (s & 7), nsub - 1));
+      }
+#pragma unroll
+      for (int u = 0; u < kSt; u++) {
+        const int s = threadIdx.x + 256 * u;
+        const int kk = s >> 3, m = 8 * h + (s & 7);
+        sX[kk][m] = (kk < kn && m < nsub) ? make_float2((float)dv[u].x, (float)dv[u].y) : make_float2(0.f, 0.f);
+        if (nsub <= 8) sX[kk][m + 8] = make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    // two register buffers: the next group's loads are in flight while the current group runs on MFMA
+    const int ng = (kn + 4 * kGB - 1) / (4 * kGB);
+    int g = 0;
+    // (the scheduling barriers keep the compiler from sinking the next group's loads below the MFMAs)
+    for (; g + 1 < ng; g += 2) {
+      load_group(g + 1, bufB, kc, kn);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_group(g, bufA, kc, kn);
+      __builtin_amdgcn_sched_barrier(0);
+      if (g + 2 < ng) load_group(g + 2, bufA, kc, kn);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_group(g + 1, bufB, kc, kn);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (g < ng) mfma_group(g, bufA, kc, kn);
     __syncthreads();
   }
   // D[i][m] of row block rb: lane l, register r: i = 4 (l / 16) + r, m = l % 16
   double2 *pq = part + q * part_qstride + (int64_t)tk.w * 256 * nrhs;
-  if (li < nsub) {
+  if (NARROW) {
+    // lane m (< 8) holds P1[:, m], P2[:, m]; lane m + 8 holds P1[:, m + 8], P2[:, m + 8]
+    float o1[4][4], o2[4][4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        o1[rb][r] = __shfl_xor(p1[rb][r], 8, 16);
+        o2[rb][r] = __shfl_xor(p2[rb][r], 8, 16);
+      }
+    if (li < 8 && li < nsub) {
+      const int j = j0 + li * rstep;
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int64_t ig = rbase + 16 * rb + 4 * lk + r;
+          if (ig < rows) {
+            const float re = p1[rb][r] - o2[rb][r], im = o1[rb][r] + p2[rb][r];
+            pq[(ig - tk.y) * nrhs + j] = make_double2((double)re, (double)im);
+          }
+        }
+    }
+  } else if (li < nsub) {
     const int j = j0 + li * rstep;
 #pragma unroll
     for (int rb = 0; rb < 4; rb++)
@@ -1138,8 +1218,9 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
                                                       double2 *__restrict__ yall, double2 *__restrict__ tall,
                                                       double2 *__restrict__ xall, double2 *__restrict__ yrall,
                                                       const double2 *__restrict__ part, int64_t part_qstride) {
-  const int4 tk = tasks[blockIdx.x];
   const int q = blockIdx.y;
+  if (D.iact && !D.iact[q]) return;
+  const int4 tk = tasks[blockIdx.x];
   const int rows_wg = 256 / ty_n;
   int t, e;
   int64_t rows, cols, ld, dbeg, nd, nf;
@@ -1203,8 +1284,10 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
   }
 }
 
-__global__ void k_lu_zero_rows(int64_t first, int64_t count, int nrhs, int64_t qstride, double2 *__restrict__ x) {
+__global__ void k_lu_zero_rows(int64_t first, int64_t count, int nrhs, int64_t qstride, double2 *__restrict__ x,
+                               const int32_t *__restrict__ iact) {
   const int q = blockIdx.y;
+  if (iact && !iact[q]) return;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count * nrhs;
        e += (int64_t)gridDim.x * blockDim.x)
     x[q * qstride + first * nrhs + e] = make_double2(0.0, 0.0);
@@ -1271,6 +1354,7 @@ struct LuFactor::Impl {
     int64_t max_slots = 1;                  // partial-sum slots of the largest launch
   };
   DevBuf part;
+  DevBuf iact;  // per instance: any system active (apply)
   std::map<int, ApplyTasks> atasks;  // keyed by ty_n
   // kernel timing (OPEM_LU_TIMING set): HIP events on the stream around each trailing update, panel and apply
   // launch, drained when the stream is synchronised; class 0 trailing updates on the 64-tile kernel (work =
@@ -1347,6 +1431,7 @@ struct LuFactor::Impl {
     D.ppqs = ppqs;
     D.fstride = fstride;
     D.fvstride = fvtot * nrhs;
+    D.iact = nullptr;
     return D;
   }
 };
@@ -1893,7 +1978,8 @@ void LuFactor::factor(int prec, int ninst, const double *vals, int64_t vals_stri
 }
 
 template <typename T, typename X>
-static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, const X *v, int64_t ldv, X *z) {
+static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, const X *v, int64_t ldv, X *z,
+                      const int32_t *act) {
   if (nrhs > 8 * kMaxRhsThread) throw std::invalid_argument("lu: at most 64 right-hand sides per matrix");
   hipStream_t st = I.st;
   const int64_t n = Tr.n, S = (int64_t)ninst * nrhs;
@@ -1924,6 +2010,11 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
     // rows x cols of each kind's block and its triangle: 1 = lower with unit diagonal (k <= i), 2 = upper (k >= i)
     for (int kind = 0; kind < 6; kind++) A.elems[kind].assign(L, 0.0);
     double *cur_elems = nullptr;
+    // OPEM_LU_APPLY_STATS (diagnostic): per kind, the phase-1 tasks, their mean fill of the rows x kch tile and the
+    // share of tasks with at most 64 valid rows or 32 valid columns, printed once per row tiling
+    const bool apply_stats = getenv("OPEM_LU_APPLY_STATS") != nullptr;
+    double st_tasks[6] = {0}, st_fill[6] = {0}, st_small[6] = {0}, st_narrow[6] = {0};
+    int cur_kind = 0;
     auto add_block = [&](int id, int64_t rows, int64_t cols, int tri, int &slot) {
       *cur_elems += tri == 0 ? (double)rows * cols : tri == 1 ? 0.5 * rows * (rows - 1) : 0.5 * rows * (rows + 1);
       for (int64_t r0 = 0; r0 < rows; r0 += rows_wg) {
@@ -1933,7 +2024,16 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
         if (tri == 2) k0 = r0;
         const int slot0 = slot;
         int nch = 0;
-        for (int64_t c0 = k0; c0 < k1; c0 += I.kch, nch++) A.all1.push_back(make_int4(id, (int)r0, (int)c0, slot++));
+        for (int64_t c0 = k0; c0 < k1; c0 += I.kch, nch++) {
+          A.all1.push_back(make_int4(id, (int)r0, (int)c0, slot++));
+          if (apply_stats) {  // diagnostic: valid rows and columns of each phase-1 task
+            const double vr = (double)(r1 - r0), vc = (double)(std::min(k1, c0 + I.kch) - c0);
+            st_tasks[cur_kind] += 1;
+            st_fill[cur_kind] += vr * vc / ((double)rows_wg * I.kch);
+            st_small[cur_kind] += vr <= 64;
+            st_narrow[cur_kind] += vc <= 32;
+          }
+        }
         A.all2.push_back(make_int4(id, (int)r0, nch, slot0));
       }
     };
@@ -1942,6 +2042,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
         A.off1[kind][l] = (int64_t)A.all1.size();
         A.off2[kind][l] = (int64_t)A.all2.size();
         cur_elems = &A.elems[kind][l];
+        cur_kind = kind;
         int slot = 0;
         if (kind <= K_BWD_U) {
           for (int t : I.level_nodes[l]) {
@@ -1962,12 +2063,37 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
     }
     A.d1.upload(A.all1, st);
     A.d2.upload(A.all2, st);
+    if (apply_stats)
+      for (int kind = 0; kind < 6; kind++) {
+        double el = 0;
+        for (double v : A.elems[kind]) el += v;
+        fprintf(stderr, "lu apply kind %d: %.0f tasks, %.3e factor entries, mean tile fill %.3f, rows<=64 %.3f, "
+                "cols<=32 %.3f\n", kind, st_tasks[kind], el, st_fill[kind] / std::max(1.0, st_tasks[kind]),
+                st_small[kind] / std::max(1.0, st_tasks[kind]), st_narrow[kind] / std::max(1.0, st_tasks[kind]));
+      }
   }
   auto &A = it->second;
   const int64_t part_qstride = A.max_slots * rows_wg * nrhs;
   I.part.alloc((size_t)ninst * part_qstride * 16);
   double2 *part = I.part.as<double2>();
   LuDev D = I.dev(Tr, nrhs);
+  // instances whose systems are all inactive (converged) are skipped by every apply kernel
+  int nact = ninst;
+  if (act) {
+    I.iact.alloc((size_t)ninst * 4);
+    hipLaunchKernelGGL(k_lu_inst_active, dim3((ninst + 63) / 64), dim3(64), 0, st, ninst, nrhs, act,
+                       I.iact.as<int32_t>());
+    LU_LAUNCH_CHECK();
+    D.iact = I.iact.as<int32_t>();
+    if (I.timing) {  // the bytes attributed to the timed launches count the active instances only
+      std::vector<int32_t> ia(ninst);
+      HIPCHK(hipMemcpyAsync(ia.data(), I.iact.p, (size_t)ninst * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      nact = 0;
+      for (int32_t a : ia) nact += a != 0;
+    }
+  }
+  const int32_t *iactp = D.iact;
   RootDev Rt{};
   Rt.node = I.root;
   Rt.nexc = I.nexc;
@@ -1982,20 +2108,24 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   double2 *b = I.b.as<double2>(), *y = I.y.as<double2>(), *tt = I.t.as<double2>(), *x = I.x.as<double2>(),
           *fv = I.fv.as<double2>(), *yr = I.yr.as<double2>();
   hipLaunchKernelGGL((k_lu_gather_rhs<X>), dim3(grid_for(n * nrhs), ninst), dim3(256), 0, st, n, nrhs,
-                     I.perm.as<int32_t>(), v, ldv, b);
+                     I.perm.as<int32_t>(), v, ldv, b, iactp);
   LU_LAUNCH_CHECK();
   auto launch = [&](auto kind_c, int l) {
     constexpr int KIND = decltype(kind_c)::value;
     const int64_t a0 = A.off1[KIND][l], a1 = A.off1[KIND][l + 1];
     const int64_t b0 = A.off2[KIND][l], b1 = A.off2[KIND][l + 1];
     const size_t lds = (size_t)kKC * nrhs * sizeof(double2);
-    if (a1 > a0) I.timed(2, A.elems[KIND][l] * sizeof(T) * ninst, [&] {
+    if (a1 > a0) I.timed(2, A.elems[KIND][l] * sizeof(T) * nact, [&] {
       const dim3 g((unsigned)(a1 - a0), ninst);
       const int4 *tk = A.d1.as<int4>() + a0;
       if constexpr (std::is_same<T, float2>::value) {
         if (ty_n == 1 && I.apply_mfma) {  // FP32 factor, every right-hand side of the task in one MFMA tile
-          hipLaunchKernelGGL((k_lu_apply_mfma<KIND>), g, dim3(256), 0, st, tk, D, Rt, nrhs, F, E, pp, fv, y, tt, x,
-                             yr, part, part_qstride, I.kch);
+          if (nrhs <= 8)
+            hipLaunchKernelGGL((k_lu_apply_mfma<KIND, true>), g, dim3(256), 0, st, tk, D, Rt, nrhs, F, E, pp, fv, y,
+                               tt, x, yr, part, part_qstride, I.kch);
+          else
+            hipLaunchKernelGGL((k_lu_apply_mfma<KIND, false>), g, dim3(256), 0, st, tk, D, Rt, nrhs, F, E, pp, fv, y,
+                               tt, x, yr, part, part_qstride, I.kch);
           return;
         }
       }
@@ -2033,7 +2163,7 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   }
   if (I.root >= 0) {
     hipLaunchKernelGGL(k_lu_zero_rows, dim3(grid_for(Tr.nd(I.root) * nrhs), ninst), dim3(256), 0, st,
-                       (int64_t)Tr.dbeg[I.root], (int64_t)Tr.nd(I.root), nrhs, n * nrhs, x);
+                       (int64_t)Tr.dbeg[I.root], (int64_t)Tr.nd(I.root), nrhs, n * nrhs, x, iactp);
     LU_LAUNCH_CHECK();
   }
   for (int l = L - 1; l >= 0; l--) {
@@ -2044,18 +2174,17 @@ static void solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs,
   const int64_t ndof = Tr.ndof;
   hipLaunchKernelGGL((k_lu_zero<X>), dim3(grid_for(ndof * S, 65536)), dim3(256), 0, st, ndof * S, z);
   hipLaunchKernelGGL((k_lu_scatter<X>), dim3(grid_for(n * nrhs), ninst), dim3(256), 0, st, n, nrhs, S,
-                     I.perm.as<int32_t>(), x, z);
+                     I.perm.as<int32_t>(), x, z, iactp);
   LU_LAUNCH_CHECK();
+  return nact;
 }
 
 template <typename X>
-void LuFactor::solve(int nrhs, const X *v, int64_t ldv, X *z) {
+void LuFactor::solve(int nrhs, const X *v, int64_t ldv, X *z, const int32_t *active) {
   if (!T_ || ninst_ < 1) throw std::logic_error("lu: factor first");
-  if (prec_ == 0)
-    solve_impl<double2, X>(*d_, *T_, ninst_, nrhs, v, ldv, z);
-  else
-    solve_impl<float2, X>(*d_, *T_, ninst_, nrhs, v, ldv, z);
-  d_->solves += ninst_;
+  const int nact = prec_ == 0 ? solve_impl<double2, X>(*d_, *T_, ninst_, nrhs, v, ldv, z, active)
+                              : solve_impl<float2, X>(*d_, *T_, ninst_, nrhs, v, ldv, z, active);
+  d_->solves += nact;
 }
 void LuFactor::kernel_stats(double *out, bool reset) {
   Impl &I = *d_;
@@ -2074,7 +2203,7 @@ void LuFactor::kernel_stats(double *out, bool reset) {
   }
 }
 
-template void LuFactor::solve<double2>(int, const double2 *, int64_t, double2 *);
-template void LuFactor::solve<float2>(int, const float2 *, int64_t, float2 *);
+template void LuFactor::solve<double2>(int, const double2 *, int64_t, double2 *, const int32_t *);
+template void LuFactor::solve<float2>(int, const float2 *, int64_t, float2 *, const int32_t *);
 
 }  // namespace opem

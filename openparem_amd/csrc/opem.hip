@@ -983,7 +983,7 @@ static void gmres_batch(Ctx &c, Work &W, const BatchOp &op, int64_t n, int S, in
       return;
     }
     if (use_lu) {
-      c.lu_pc->solve<X>(c.lu_nrhs, v, ldv_sys, z);
+      c.lu_pc->solve<X>(c.lu_nrhs, v, ldv_sys, z, act);
       return;
     }
     if (use_patch)
