@@ -141,6 +141,11 @@ struct Ctx {
   bool have_lu_tree = false;
   std::vector<uint8_t> lu_excl, lu_last;
   LuFactor lu;
+  // sparse LU of the unit mass P over every DOF (FP64): the H projection's preconditioner when the sweep runs on
+  // the LU path (built once per space; P does not depend on the frequency or the materials)
+  LuTree p_tree;
+  LuFactor p_lu;
+  bool have_p_lu = false;
   LuFactor *lu_pc = nullptr;  // set while a Krylov solve is preconditioned by lu
   int lu_nrhs = 0;            // systems per factor
   int lu_leaf = 8;            // dissection leaves: at most this many mesh vertices
@@ -1394,17 +1399,34 @@ static void h_solve(Ctx &c, const double2 *x, int S, const std::vector<cplx> &sc
   const SweepMatrix<double> MP{n, nnz, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(), pbase.as<double>(),
                                pbase.as<double>() + nnz, nullptr, c.d_rowptr.as<int32_t>(), c.d_colind.as<int32_t>(),
                                pbase.as<double>(), 0, 0, blk16_max(s)};
-  // preconditioner: vertex patches of P when every patch fits (S <= 256), else point Jacobi
-  build_h_patches(c);
-  const bool hp = c.hpatch_state == 1 && S <= 256;
+  // preconditioner: on the LU path the FP64 sparse LU of P (an exact inverse: one CG step), else vertex patches of
+  // P when every patch fits (S <= 256), else point Jacobi
+  const bool plu = c.precond == 2;
+  if (plu && !c.have_p_lu) {
+    const std::vector<uint8_t> none((size_t)n, 0);
+    lu_analyse_mesh(c.mesh, s, none, none, c.lu_leaf, c.p_tree);
+    c.p_lu.setup(c.p_tree, 1, {}, st);
+    c.p_lu.factor(0, 1, c.d_P.as<double>(), nnz, nullptr);
+    c.have_p_lu = true;
+  }
+  if (!plu) build_h_patches(c);
+  const bool hp = !plu && c.hpatch_state == 1 && S <= 256;
   const PatchLayout HL = hp ? h_layout(c) : PatchLayout{};
   DevBuf hy, hzil;
   if (hp) {
     hy.alloc((size_t)S * HL.ytot * 16);
     hzil.alloc((size_t)S * n * 16);
   }
+  constexpr int kPluRhs = 64;  // right-hand sides per LU solve call
+  if (plu) hzil.alloc((size_t)std::min(S, kPluRhs) * n * 16);
   auto hprec = [&](const double2 *rr, double2 *zz, const int32_t *a) {
-    if (hp) {
+    if (plu) {
+      for (int s0 = 0; s0 < S; s0 += kPluRhs) {
+        const int cnt = std::min(kPluRhs, S - s0);
+        c.p_lu.solve<double2>(cnt, rr + (int64_t)s0 * n, n, hzil.as<double2>(), a ? a + s0 : nullptr);
+        launch_transpose<double2>(st, n, cnt, hzil.as<double2>(), zz + (int64_t)s0 * n, false);
+      }
+    } else if (hp) {
       launch_patch_apply<__half2, double2>(st, HL, S, 1, n, c.dh_inv.as<__half2>(), c.dh_scale.as<float>(), rr, n,
                                            hzil.as<double2>(), a, (void *)hy.p);
       launch_transpose<double2>(st, n, S, hzil.as<double2>(), zz, false);
@@ -1459,7 +1481,7 @@ static void h_solve(Ctx &c, const double2 *x, int S, const std::vector<cplx> &sc
                        nullptr, h.as<double2>(), act.as<int32_t>());
     hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n), S), dim3(256), 0, st, n, nal.as<double2>(), q.as<double2>(),
                        nullptr, r.as<double2>(), act.as<int32_t>());
-    if ((it % 8) == 7) {
+    if (plu || (it % 8) == 7) {  // an exact inverse converges at once: test every step
       launch_norm2<double2>(st, S, n, r.as<double2>(), part.as<double>(), nblk, act.as<int32_t>());
       launch_reduce(st, S, nblk, 0, part.as<double>(), part2.as<double>(), 1, nullptr, 0, act.as<int32_t>());
       HIPCHK(hipMemcpyAsync(rn2.data(), part2.p, S * 8, hipMemcpyDeviceToHost, st));
@@ -2866,6 +2888,7 @@ int opem_space_build(opem_ctx *ctx, int order, int64_t *ndof, int64_t *nnz) {
     ctx->have_patches = false;
     ctx->have_coarse = false;
     ctx->hpatch_state = 0;
+    ctx->have_p_lu = false;
     ctx->spmv_form.clear();
     ctx->have_lu_tree = false;
     build_space(ctx->mesh, order, ctx->sp, nth);
@@ -2874,6 +2897,7 @@ int opem_space_build(opem_ctx *ctx, int order, int64_t *ndof, int64_t *nnz) {
     ctx->have_patches = false;  // patch and coarse structures belong to the previous space
     ctx->have_coarse = false;
     ctx->hpatch_state = 0;
+    ctx->have_p_lu = false;
     if (ndof) *ndof = ctx->sp.ndof;
     if (nnz) *nnz = ctx->sp.nnz;
   });
