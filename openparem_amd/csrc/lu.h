@@ -38,8 +38,11 @@ class LuFactor {
   void setup(const LuTree &T, int nexc, const std::vector<int32_t> &root_group, hipStream_t st);
 
   // Factor ninst matrices (complex FP64 values on the device, [ninst][vals_stride], laid out on the pattern the
-  // tree was analysed on).  prec 0: FP64 fronts, 1: FP32 fronts.
-  void factor(int prec, int ninst, const double *vals, int64_t vals_stride, LuFactorStats *stats);
+  // tree was analysed on).  prec 0: FP64 fronts, 1: FP32 fronts.  symmetric: the caller guarantees A = A^T (the
+  // complex-symmetric Maxwell matrices); then FP32 factors (FP64 too under OPEM_LU_SYM=2) compute the lower
+  // triangles of the trailing updates only and take pivots inside each 128-column outer block.
+  void factor(int prec, int ninst, const double *vals, int64_t vals_stride, LuFactorStats *stats,
+              bool symmetric = false);
 
   // z = A^{-1} v for S = ninst * nrhs systems: system s uses instance s / nrhs and excitation s % nexc.
   // v: system-major, v[s * ldv + dof]; z: system-interleaved, z[dof * S + s] (rows outside the factored set and the
@@ -58,6 +61,7 @@ class LuFactor {
   void kernel_stats(double *out, bool reset);
 
   int prec() const { return prec_; }
+  bool symmetric() const { return sym_; }  // the last factorization used the symmetric form
   int ninst() const { return ninst_; }
   const LuTree *tree() const { return T_; }
   // Device bytes one factored matrix needs (fronts, port blocks, inverse scratch, pivots, and the apply's vectors
@@ -70,6 +74,7 @@ class LuFactor {
   Impl *d_;
   const LuTree *T_ = nullptr;
   int prec_ = 0, ninst_ = 0;
+  bool sym_ = false;
 };
 
 }  // namespace opem

@@ -117,9 +117,11 @@ __global__ void k_lu_assemble(int64_t nnz, const int64_t *__restrict__ asm_pos, 
 
 // Task: (child, first column of its update block).  The child's update block (rows and columns nd_c .. nf_c-1
 // of its front) is added into the parent's front at the rows and columns umap gives.
+// sym: the update blocks hold their lower triangles only (symmetric factorization); umap is increasing, so entry
+// (a, b) with a >= b lands in the parent's lower triangle, and only those entries are added.
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_extend_add(const int2 *__restrict__ tasks, LuDev D, T *__restrict__ F,
-                                                       int cb) {
+                                                       int cb, int sym) {
   const int2 tk = tasks[blockIdx.x];
   const int q = blockIdx.y;
   const int c = tk.x, p = D.parent[c];
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(256) void k_lu_extend_add(const int2 *__restrict__ 
   for (int64_t b = tk.y; b < b1; b++) {
     const int64_t pc = (int64_t)um[b] * nfp;
     const T *src = Fc + ndc + nfc * (ndc + b);
-    for (int64_t a = threadIdx.x; a < nuc; a += blockDim.x) {
+    for (int64_t a = (sym ? b : 0) + threadIdx.x; a < nuc; a += blockDim.x) {
       T v = Fp[um[a] + pc];
       const T s = src[a];
       v.x += s.x;
@@ -180,15 +182,18 @@ __device__ __forceinline__ T *dense_ptr(const DenseBase &B, int sel, int64_t off
 // solved afterwards by k_dense_lrows.
 constexpr int kGrp = 8;
 
+// sym (symmetric factorization): pivots are searched among the rows of the current outer block only, whose
+// rows were completed from the lower triangle by k_dense_mirror; the rows below it hold their lower parts only.
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_dense_panel(const int32_t *__restrict__ list,
                                                                const DenseFront *__restrict__ fr, DenseBase B, int k0,
-                                                               unsigned long long *__restrict__ zero_pivots) {
+                                                               unsigned long long *__restrict__ zero_pivots, int sym) {
   using R = decltype(T().x);
   const DenseFront f = fr[list[blockIdx.x]];
   const int q = blockIdx.y;
   const int nf = f.nf, nd = f.nd;
   const int nb = min(kNB, nd - k0);
+  const int pend = sym ? min(nd, (k0 / kOuter + 1) * kOuter) : nd;  // pivot rows searched: j .. pend-1
   T *F = dense_ptr<T>(B, f.sel, f.off, q);
   int32_t *pp = B.pp + q * B.ppqs + f.pp_off;
   int32_t *sw = B.sw + q * B.ppqs + f.pp_off;
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_dense_panel(const int32_t *__
     for (int j = g0; j < g1; j++) {
       double best = -1.0;
       int bi = j;
-      for (int i = j + tid; i < nd; i += kPanelThreads) {
+      for (int i = j + tid; i < pend; i += kPanelThreads) {
         const double v = cabs1(F[i + (int64_t)nf * j]);
         if (v > best) {
           best = v;
@@ -309,16 +314,101 @@ __global__ __launch_bounds__(kPanelThreads) void k_dense_panel(const int32_t *__
   }
 }
 
-// Update rows of a factored panel: rows nd .. nf-1 of columns k0 .. k0+nb-1 times U_kk^-1 (from the right).
-// Task = (front, first row after nd); one thread per row.
+// Panel of the symmetric factorization: pivots are searched among rows k0 .. pend-1 only (pend = the end of the
+// outer block, at most 128 rows), so those rows are staged in LDS, one row per thread, and the 32-column
+// elimination runs there with three barriers per column instead of global-memory passes over every pivot row.
+// Rows from pend on are solved afterwards by k_dense_lrows (x U_kk^-1), exactly as the update rows are.
 template <typename T>
-__global__ __launch_bounds__(256) void k_dense_lrows(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr,
-                                                     DenseBase B, int k0) {
-  const int2 tk = tasks[blockIdx.x];
-  const DenseFront f = fr[tk.x];
+__global__ __launch_bounds__(kOuter) void k_dense_panel_blk(const int32_t *__restrict__ list,
+                                                            const DenseFront *__restrict__ fr, DenseBase B, int k0,
+                                                            unsigned long long *__restrict__ zero_pivots) {
+  using R = decltype(T().x);
+  const DenseFront f = fr[list[blockIdx.x]];
   const int q = blockIdx.y;
   const int nf = f.nf, nd = f.nd;
   const int nb = min(kNB, nd - k0);
+  const int pend = min(nd, (k0 / kOuter + 1) * kOuter), nr = pend - k0;
+  T *F = dense_ptr<T>(B, f.sel, f.off, q);
+  int32_t *pp = B.pp + q * B.ppqs + f.pp_off;
+  int32_t *sw = B.sw + q * B.ppqs + f.pp_off;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  __shared__ T P[kOuter][kNB + 1];
+  __shared__ double red_v[kOuter / 64];
+  __shared__ int red_i[kOuter / 64];
+  if (k0 == 0)
+    for (int k = t; k < nd; k += kOuter) pp[k] = k;
+  for (int c = 0; c < nb; c++)  // column c of the panel: consecutive threads on consecutive rows
+    if (t < nr) P[t][c] = F[(k0 + t) + (int64_t)nf * (k0 + c)];
+  __syncthreads();
+  for (int j = 0; j < nb; j++) {
+    double best = (t >= j && t < nr) ? cabs1(P[t][j]) : -1.0;
+    int bi = t;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v2 = __shfl_xor(best, o, 64);
+      const int i2 = __shfl_xor(bi, o, 64);
+      if (v2 > best || (v2 == best && i2 < bi)) {
+        best = v2;
+        bi = i2;
+      }
+    }
+    if (lane == 0) {
+      red_v[w] = best;
+      red_i[w] = bi;
+    }
+    __syncthreads();
+    double bb = red_v[0];
+    int p = red_i[0];
+#pragma unroll
+    for (int w2 = 1; w2 < kOuter / 64; w2++)
+      if (red_v[w2] > bb || (red_v[w2] == bb && red_i[w2] < p)) {
+        bb = red_v[w2];
+        p = red_i[w2];
+      }
+    const bool zero = !(bb > 0.0);
+    if (p != j && t < nb) {  // rows j and p trade their panel values, one column per thread
+      const T a = P[j][t];
+      P[j][t] = P[p][t];
+      P[p][t] = a;
+    }
+    if (t == 0) {
+      sw[k0 + j] = k0 + p;
+      if (p != j) {
+        const int32_t s2 = pp[k0 + j];
+        pp[k0 + j] = pp[k0 + p];
+        pp[k0 + p] = s2;
+      }
+      if (zero) atomicAdd(zero_pivots, 1ull);
+    }
+    __syncthreads();
+    if (zero && t == 0) {
+      P[j][j].x = std::is_same<R, float>::value ? (R)1e-30 : (R)1e-290;
+      P[j][j].y = 0;
+    }
+    __syncthreads();
+    if (t > j && t < nr) {
+      const T l = cmul_t(P[t][j], cinv_t(P[j][j]));
+      P[t][j] = l;
+      for (int c = j + 1; c < nb; c++) P[t][c] = csub_t(P[t][c], cmul_t(l, P[j][c]));
+    }
+    __syncthreads();
+  }
+  for (int c = 0; c < nb; c++)
+    if (t < nr) F[(k0 + t) + (int64_t)nf * (k0 + c)] = P[t][c];
+}
+
+// Update rows of a factored panel: rows nd .. nf-1 of columns k0 .. k0+nb-1 times U_kk^-1 (from the right).
+// Task = (front, first row after nd); one thread per row.
+// sym: the rows from the end of the outer block on (not pivot candidates there) instead of from nd; task rows
+// count from that row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_lrows(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr,
+                                                     DenseBase B, int k0, int sym) {
+  const int2 tk = tasks[blockIdx.x];
+  const DenseFront f = fr[tk.x];
+  const int q = blockIdx.y;
+  const int nf = f.nf, nd = sym ? min(f.nd, (k0 / kOuter + 1) * kOuter) : f.nd;
+  const int nb = min(kNB, f.nd - k0);
   T *F = dense_ptr<T>(B, f.sel, f.off, q);
   __shared__ T Us[kNB * kNB];
   for (int e = threadIdx.x; e < nb * nb; e += 256) Us[e] = F[(k0 + e % nb) + (int64_t)nf * (k0 + e / nb)];
@@ -424,8 +514,10 @@ __global__ __launch_bounds__(256) void k_dense_trsm(const int2 *__restrict__ tas
 // preconditioner).  Operand layouts of the 16x16x4 MFMA: first operand
 // P[i][k] in lane i + 16 k, second Q[k][j] in lane j + 16 k; D[i][j] in register i % 4 of lane j + 16 (i / 4)
 // (f32) or register i / 4 of lane j + 16 (i % 4) (f64).
+// tri = 1 marks a square trailing update on a front's diagonal (a symmetric Schur complement): launches of the
+// symmetric factorization number only its lower tiles (tile row >= tile column).
 struct GemmJob {
-  int asel, bsel, csel, pad;
+  int asel, bsel, csel, tri;
   int m, n, k, lda;
   int ldb, ldc, pad2, pad3;
   int64_t aoff, boff, coff;
@@ -433,6 +525,21 @@ struct GemmJob {
 
 typedef float lu_f4 __attribute__((ext_vector_type(4)));
 typedef double lu_d4 __attribute__((ext_vector_type(4)));
+
+// Tile (row, column) of a job's local tile index: row-major over ntc tile columns, or, for a triangular job of a
+// triangular launch, the lower tiles row by row (local = r (r + 1) / 2 + c, c <= r).
+__device__ __forceinline__ void gemm_tile(int64_t local, int ntc, bool tri, int &tr, int &tc) {
+  if (tri) {
+    int r = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+    while ((int64_t)r * (r + 1) / 2 > local) r--;
+    while ((int64_t)(r + 1) * (r + 2) / 2 <= local) r++;
+    tr = r;
+    tc = (int)(local - (int64_t)r * (r + 1) / 2);
+  } else {
+    tr = (int)(local / ntc);
+    tc = (int)(local % ntc);
+  }
+}
 
 template <typename T>
 struct MfmaAcc;
@@ -455,7 +562,7 @@ struct MfmaAcc<double2> {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_gemm(const GemmJob *__restrict__ jobs, const int64_t *__restrict__ prefix,
-                                                    int njobs, DenseBase B) {
+                                                    int njobs, DenseBase B, int tri) {
   using M = MfmaAcc<T>;
   using V = typename M::V;
   __shared__ T As[kNB][kTile];
@@ -472,8 +579,9 @@ __global__ __launch_bounds__(256) void k_dense_gemm(const GemmJob *__restrict__ 
   }
   const GemmJob J = jobs[lo];
   const int64_t local = b - prefix[lo];
-  const int ntc = (J.n + kTile - 1) / kTile;
-  const int r0 = (int)(local / ntc) * kTile, c0 = (int)(local % ntc) * kTile;
+  int tr, tc;
+  gemm_tile(local, (J.n + kTile - 1) / kTile, tri && J.tri, tr, tc);
+  const int r0 = tr * kTile, c0 = tc * kTile;
   const T *A = dense_ptr<T>(B, J.asel, J.aoff, q);
   const T *Bm = dense_ptr<T>(B, J.bsel, J.boff, q);
   T *C = dense_ptr<T>(B, J.csel, J.coff, q);
@@ -620,7 +728,7 @@ __device__ __forceinline__ int swz_b(int s, int c) { return c ^ (s & 15) ^ ((s &
 
 __global__ __launch_bounds__(256) void k_dense_gemm_t128(const GemmJob *__restrict__ jobs,
                                                          const int64_t *__restrict__ prefix, int njobs,
-                                                         int64_t ntiles, int ninst, DenseBase B) {
+                                                         int64_t ntiles, int ninst, DenseBase B, int tri) {
   __shared__ float2 As[kBK][kBT];
   __shared__ float2 Bs[kBK][kBT];
   // XCD-contiguous order of the flat (matrix, tile) index: hardware block b runs on XCD b % 8
@@ -639,8 +747,9 @@ __global__ __launch_bounds__(256) void k_dense_gemm_t128(const GemmJob *__restri
   }
   const GemmJob J = jobs[lo];
   const int64_t local = tile - prefix[lo];
-  const int ntc = (J.n + kBT - 1) / kBT;
-  const int r0 = (int)(local / ntc) * kBT, c0 = (int)(local % ntc) * kBT;
+  int tr, tc;
+  gemm_tile(local, (J.n + kBT - 1) / kBT, tri && J.tri, tr, tc);
+  const int r0 = tr * kBT, c0 = tc * kBT;
   const float2 *A = dense_ptr<float2>(B, J.asel, J.aoff, q);
   const float2 *Bm = dense_ptr<float2>(B, J.bsel, J.boff, q);
   float2 *C = dense_ptr<float2>(B, J.csel, J.coff, q);
@@ -781,14 +890,48 @@ __global__ void k_dense_put_inverses(const int2 *__restrict__ tasks, const Dense
 }
 
 // Excitation e's port block: E[i + no*k] = S[O[i] + nfr * O[k]], S the root front (the ports' Schur complement).
+// Symmetric factorization: before an outer block [ob0, ob1) is factored, its rows right of the diagonal are
+// filled from its columns below it, F[i][j] = F[j][i] for ob0 <= i < ob1, j > i.  The trailing updates of a
+// symmetric front compute lower tiles only (the Schur complement of a pivot block does not depend on the row
+// interchanges inside it, so it stays symmetric), and the row interchanges stay inside the outer block, so these
+// rows are the only upper entries the factorization reads.  Task = (front, first row i0 (a multiple of 64 from
+// ob0), first column j0 >= i0); a 64 x 64 tile is transposed through LDS.  Reads are at j > i, writes at i < j:
+// no tile writes what another reads.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_mirror(const int4 *__restrict__ tasks,
+                                                      const DenseFront *__restrict__ fr, DenseBase B, int ob0) {
+  const int4 tk = tasks[blockIdx.x];
+  const DenseFront f = fr[tk.x];
+  const int q = blockIdx.y;
+  const int nf = f.nf, ob1 = min(ob0 + kOuter, f.nd), i0 = tk.y, j0 = tk.z;
+  T *F = dense_ptr<T>(B, f.sel, f.off, q);
+  __shared__ T tile[64][65];
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int t = 0; t < 16; t++) {  // read F[j][i]: consecutive threads on consecutive j (rows of column i)
+    const int e = tid + 256 * t, jj = e & 63, ii = e >> 6;
+    const int i = i0 + ii, j = j0 + jj;
+    if (i < ob1 && j < nf && j > i) tile[ii][jj] = F[j + (int64_t)nf * i];
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int t = 0; t < 16; t++) {  // write F[i][j]: consecutive threads on consecutive i (rows of column j)
+    const int e = tid + 256 * t, ii = e & 63, jj = e >> 6;
+    const int i = i0 + ii, j = j0 + jj;
+    if (i < ob1 && j < nf && j > i) F[i + (int64_t)nf * j] = tile[ii][jj];
+  }
+}
+
+// sym: the root front holds its lower triangle only; entry (i, k) is read at (max, min).
 template <typename T>
 __global__ void k_lu_root_gather(const T *__restrict__ S, int64_t sstride, int nfr, const int32_t *__restrict__ O,
-                                 int no, T *__restrict__ E, int64_t estride) {
+                                 int no, T *__restrict__ E, int64_t estride, int sym) {
   const int q = blockIdx.y;
   const int64_t tot = (int64_t)no * no;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e % no), k = (int)(e / no);
-    E[q * estride + e] = S[q * sstride + O[i] + (int64_t)nfr * O[k]];
+    const int r = sym ? max(O[i], O[k]) : O[i], c = sym ? min(O[i], O[k]) : O[k];
+    E[q * estride + e] = S[q * sstride + r + (int64_t)nfr * c];
   }
 }
 
@@ -1326,6 +1469,13 @@ struct LuFactor::Impl {
     int64_t gemm_tiles128;  // 128 x 128 tiles of the same jobs (prefix in h_gprefix128)
     bool deep;              // the trailing update of an outer block (FP32 fronts may take the 128-tile kernel)
     bool deep_fit;          // at least 85% of the 128-tile area is matrix (the default rule for taking it)
+    // symmetric factorization: tiles of the same jobs with triangular jobs cut to their lower tiles (prefixes in
+    // h_gprefix_sym / h_gprefix128_sym), the real flops of those lower triangles (diagonal included), and the
+    // mirror tasks run before the step (the first step of each outer block)
+    int64_t gemm_tiles_sym, gemm_tiles128_sym;
+    double gemm_flops_sym;
+    int64_t mirror_off, mirror_n;
+    int64_t lrows_sym_off, lrows_sym_n;  // rows from the end of the outer block on, for k_dense_lrows (sym)
   };
   struct DLevel {
     std::vector<Step> lu, inv;
@@ -1336,11 +1486,13 @@ struct LuFactor::Impl {
   std::vector<DenseFront> dfr;
   std::vector<DLevel> dlev;    // tree levels, then one level of port blocks
   std::vector<int32_t> h_list;
-  std::vector<int2> h_laswp, h_trsm_tasks, h_lrows, h_cols;
+  std::vector<int2> h_laswp, h_trsm_tasks, h_lrows, h_lrows_sym, h_cols;
   std::vector<TrsmJob> h_trsm;
   std::vector<GemmJob> h_gemm;
-  std::vector<int64_t> h_gprefix, h_gprefix128;
-  DevBuf d_dfr, d_list, d_laswp, d_trsm_tasks, d_trsm, d_gemm, d_gprefix, d_gprefix128, d_lrows, d_cols;
+  std::vector<int64_t> h_gprefix, h_gprefix128, h_gprefix_sym, h_gprefix128_sym;
+  std::vector<int4> h_mirror;
+  DevBuf d_dfr, d_list, d_laswp, d_trsm_tasks, d_trsm, d_gemm, d_gprefix, d_gprefix128, d_gprefix_sym,
+      d_gprefix128_sym, d_lrows, d_lrows_sym, d_cols, d_mirror;
   int64_t xmax = 1, ppqs = 1;
   // numeric storage
   DevBuf fronts, pp, sw, zp, E, X, Y;
@@ -1371,6 +1523,9 @@ struct LuFactor::Impl {
   // deep trailing updates of FP32 fronts: OPEM_LU_DEEP_GEMM=0 keeps them on the 64-tile kernel, 2 puts every one
   // on the 128-tile kernel, 1 (default) those whose jobs fill at least 85% of the 128-tile area
   int deep_gemm = getenv("OPEM_LU_DEEP_GEMM") ? std::max(0, std::min(2, atoi(getenv("OPEM_LU_DEEP_GEMM")))) : 1;
+  // symmetric factorization of matrices the caller declares symmetric: OPEM_LU_SYM=0 never, 1 (default) FP32
+  // factors, 2 FP32 and FP64 factors
+  int sym_mode = getenv("OPEM_LU_SYM") ? std::max(0, std::min(2, atoi(getenv("OPEM_LU_SYM")))) : 1;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   struct Pending {
@@ -1475,15 +1630,22 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
   for (int id : ids)
     for (int c0 = 0; c0 < dfr[id].nd; c0 += 16) h_cols.push_back(make_int2(id, c0));
   L.cols_n = (int64_t)h_cols.size() - L.cols_off;
-  double *cur_flops = nullptr;  // the step being filled
-  int64_t *cur_tiles128 = nullptr;
-  auto gemm_push = [&](const GemmJob &J, int64_t &tiles) {
-    h_gprefix.push_back(tiles);
-    h_gprefix128.push_back(cur_tiles128 ? *cur_tiles128 : 0);
+  // a job joins step S: tile prefixes of the full and the triangular numbering, 64- and 128-tile kernels
+  auto gemm_push = [&](GemmJob J, Step &S, bool tri) {
+    J.tri = tri ? 1 : 0;
+    h_gprefix.push_back(S.gemm_tiles);
+    h_gprefix128.push_back(S.gemm_tiles128);
+    h_gprefix_sym.push_back(S.gemm_tiles_sym);
+    h_gprefix128_sym.push_back(S.gemm_tiles128_sym);
     h_gemm.push_back(J);
-    tiles += (int64_t)((J.m + kTile - 1) / kTile) * ((J.n + kTile - 1) / kTile);
-    if (cur_tiles128) *cur_tiles128 += (int64_t)((J.m + kBT - 1) / kBT) * ((J.n + kBT - 1) / kBT);
-    if (cur_flops) *cur_flops += 8.0 * J.m * J.n * J.k;
+    const int64_t r64 = (J.m + kTile - 1) / kTile, c64 = (J.n + kTile - 1) / kTile;
+    const int64_t r128 = (J.m + kBT - 1) / kBT, c128 = (J.n + kBT - 1) / kBT;
+    S.gemm_tiles += r64 * c64;
+    S.gemm_tiles128 += r128 * c128;
+    S.gemm_tiles_sym += tri ? r64 * (r64 + 1) / 2 : r64 * c64;
+    S.gemm_tiles128_sym += tri ? r128 * (r128 + 1) / 2 : r128 * c128;
+    S.gemm_flops += 8.0 * J.m * J.n * J.k;
+    S.gemm_flops_sym += tri ? 8.0 * J.k * (0.5 * J.m * (J.m + 1.0)) : 8.0 * J.m * J.n * J.k;
   };
   auto trsm_push = [&](const TrsmJob &J) {
     const int jid = (int)h_trsm.size();
@@ -1501,6 +1663,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     S.laswp_off = (int64_t)h_laswp.size();
     S.trsm_off = (int64_t)h_trsm_tasks.size();
     S.gemm_off = (int64_t)h_gemm.size();
+    S.mirror_off = (int64_t)h_mirror.size();
+    S.lrows_sym_off = (int64_t)h_lrows_sym.size();
     return S;
   };
   auto close_step = [&](Step &S) {
@@ -1514,21 +1678,35 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
   for (int ob0 = 0; ob0 < maxnd; ob0 += kOuter) {
     for (int k0 = ob0; k0 < ob0 + kOuter && k0 < maxnd; k0 += kNB) {
       Step S = empty_step(k0);
+      if (k0 == ob0)  // symmetric factorization: rows ob0 .. ob1-1 right of the diagonal from the columns below
+        for (int id : ids) {
+          const DenseFront &f = dfr[id];
+          if (f.nd <= ob0) continue;
+          const int ob1 = std::min(ob0 + kOuter, f.nd);
+          for (int i0 = ob0; i0 < ob1; i0 += 64)
+            for (int j0 = i0; j0 < f.nf; j0 += 64) h_mirror.push_back(make_int4(id, i0, j0, 0));
+        }
+      S.mirror_n = (int64_t)h_mirror.size() - S.mirror_off;
+      S.panel_off = (int64_t)h_list.size();
       for (int id : ids)
         if (dfr[id].nd > k0) h_list.push_back(id);
       S.lrows_off = (int64_t)h_lrows.size();
       for (int id : ids)
         if (dfr[id].nd > k0)
           for (int r0 = 0; r0 < dfr[id].nf - dfr[id].nd; r0 += 256) h_lrows.push_back(make_int2(id, r0));
+      S.lrows_sym_off = (int64_t)h_lrows_sym.size();
+      for (int id : ids)
+        if (dfr[id].nd > k0) {
+          const int pend = std::min(dfr[id].nd, ob0 + kOuter);
+          for (int r0 = 0; r0 < dfr[id].nf - pend; r0 += 256) h_lrows_sym.push_back(make_int2(id, r0));
+        }
+      S.lrows_sym_n = (int64_t)h_lrows_sym.size() - S.lrows_sym_off;
       S.laswp_off = (int64_t)h_laswp.size();
       for (int id : ids)
         if (dfr[id].nd > k0)
           for (int c0 = 0; c0 < dfr[id].nf; c0 += 256) h_laswp.push_back(make_int2(id, c0));
       S.trsm_off = (int64_t)h_trsm_tasks.size();
       S.gemm_off = (int64_t)h_gemm.size();
-      S.gemm_tiles = 0;
-      S.gemm_flops = 0;
-      cur_flops = &S.gemm_flops;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
         if (f.nd <= k0) continue;
@@ -1552,16 +1730,13 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
         G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
         G.boff = T.xoff;
         G.coff = f.off + (k0 + nb) + (int64_t)f.nf * (k0 + nb);
-        gemm_push(G, S.gemm_tiles);
+        gemm_push(G, S, false);
       }
       close_step(S);
     }
     // rows ob0 .. ob1 of U over the trailing columns ob1 .. nf: block forward substitution by inner blocks
     for (int k0 = ob0; k0 < ob0 + kOuter && k0 < maxnd; k0 += kNB) {
       Step S = empty_step(k0);
-      S.gemm_tiles = 0;
-      S.gemm_flops = 0;
-      cur_flops = &S.gemm_flops;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
         if (f.nd <= k0) continue;
@@ -1586,7 +1761,7 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
           G.aoff = f.off + (k0 + nb) + (int64_t)f.nf * k0;
           G.boff = T.xoff;
           G.coff = f.off + (k0 + nb) + (int64_t)f.nf * ob1;
-          gemm_push(G, S.gemm_tiles);
+          gemm_push(G, S, false);
         }
       }
       close_step(S);
@@ -1594,10 +1769,6 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     // trailing update of depth ob1 - ob0
     {
       Step S = empty_step(ob0);
-      S.gemm_tiles = 0;
-      S.gemm_flops = 0;
-      cur_flops = &S.gemm_flops;
-      cur_tiles128 = &S.gemm_tiles128;
       S.deep = true;
       for (int id : ids) {
         const DenseFront &f = dfr[id];
@@ -1612,9 +1783,8 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
         G.aoff = f.off + ob1 + (int64_t)f.nf * ob0;
         G.boff = f.off + ob0 + (int64_t)f.nf * ob1;
         G.coff = f.off + ob1 + (int64_t)f.nf * ob1;
-        gemm_push(G, S.gemm_tiles);
+        gemm_push(G, S, true);
       }
-      cur_tiles128 = nullptr;
       // the 128-tile kernel where its padding wastes little: at least 85% of its tile area is matrix
       double area = 0, area128 = 0;
       for (int64_t g = S.gemm_off; g < (int64_t)h_gemm.size(); g++) {
@@ -1632,9 +1802,6 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
     Step S{};
     S.trsm_off = (int64_t)h_trsm_tasks.size();
     S.gemm_off = (int64_t)h_gemm.size();
-    S.gemm_tiles = 0;
-    S.gemm_flops = 0;
-    cur_flops = &S.gemm_flops;
     std::vector<GemmJob> gj;
     for (int id : ids) {
       const DenseFront &f = dfr[id];
@@ -1697,7 +1864,7 @@ void LuFactor::Impl::schedule_level(const std::vector<int32_t> &ids, DLevel &L) 
         }
       }
     }
-    for (auto &G : gj) gemm_push(G, S.gemm_tiles);
+    for (auto &G : gj) gemm_push(G, S, false);
     S.trsm_n = (int64_t)h_trsm_tasks.size() - S.trsm_off;
     S.gemm_n = (int64_t)h_gemm.size() - S.gemm_off;
     L.inv.push_back(S);
@@ -1809,6 +1976,10 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   I.h_gemm.clear();
   I.h_gprefix.clear();
   I.h_gprefix128.clear();
+  I.h_gprefix_sym.clear();
+  I.h_gprefix128_sym.clear();
+  I.h_mirror.clear();
+  I.h_lrows_sym.clear();
   I.h_lrows.clear();
   I.h_cols.clear();
   I.dlev.assign(L + 1, Impl::DLevel{});
@@ -1833,6 +2004,10 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   I.d_gemm.upload(I.h_gemm, st);
   I.d_gprefix.upload(I.h_gprefix, st);
   I.d_gprefix128.upload(I.h_gprefix128, st);
+  I.d_gprefix_sym.upload(I.h_gprefix_sym, st);
+  I.d_gprefix128_sym.upload(I.h_gprefix128_sym, st);
+  I.d_mirror.upload(I.h_mirror, st);
+  I.d_lrows_sym.upload(I.h_lrows_sym, st);
   I.d_lrows.upload(I.h_lrows, st);
   I.d_cols.upload(I.h_cols, st);
   I.atasks.clear();
@@ -1840,19 +2015,28 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
 }
 
 template <typename T>
-static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int ninst, const DenseBase &B) {
+static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int ninst, const DenseBase &B, bool sym) {
   hipStream_t st = I.st;
   if (L.list_n == 0) return;
   const DenseFront *fr = I.d_dfr.as<DenseFront>();
+  const int tri = sym ? 1 : 0;
   for (const auto &S : L.lu) {
+    if (sym && S.mirror_n > 0)
+      hipLaunchKernelGGL((k_dense_mirror<T>), dim3((unsigned)S.mirror_n, ninst), dim3(256), 0, st,
+                         I.d_mirror.as<int4>() + S.mirror_off, fr, B, S.k0);
     if (S.panel_n > 0)
       I.timed(1, 0.0, [&] {
-        hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
-                           I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
+        if (sym)
+          hipLaunchKernelGGL((k_dense_panel_blk<T>), dim3((unsigned)S.panel_n, ninst), dim3(kOuter), 0, st,
+                             I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>());
+        else
+          hipLaunchKernelGGL((k_dense_panel<T>), dim3((unsigned)S.panel_n, ninst), dim3(kPanelThreads), 0, st,
+                             I.d_list.as<int32_t>() + S.panel_off, fr, B, S.k0, I.zp.as<unsigned long long>(), 0);
       });
-    if (S.lrows_n > 0)
-      hipLaunchKernelGGL((k_dense_lrows<T>), dim3((unsigned)S.lrows_n, ninst), dim3(256), 0, st,
-                         I.d_lrows.as<int2>() + S.lrows_off, fr, B, S.k0);
+    if (sym ? S.lrows_sym_n > 0 : S.lrows_n > 0)
+      hipLaunchKernelGGL((k_dense_lrows<T>), dim3((unsigned)(sym ? S.lrows_sym_n : S.lrows_n), ninst), dim3(256), 0,
+                         st, (sym ? I.d_lrows_sym : I.d_lrows).as<int2>() + (sym ? S.lrows_sym_off : S.lrows_off), fr,
+                         B, S.k0, tri);
     if (S.laswp_n > 0)
       hipLaunchKernelGGL((k_dense_laswp<T>), dim3((unsigned)S.laswp_n, ninst), dim3(256), 0, st,
                          I.d_laswp.as<int2>() + S.laswp_off, fr, B, S.k0);
@@ -1860,17 +2044,19 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
       hipLaunchKernelGGL((k_dense_trsm<T>), dim3((unsigned)S.trsm_n, ninst), dim3(256), 0, st,
                          I.d_trsm_tasks.as<int2>() + S.trsm_off, I.d_trsm.as<TrsmJob>(), B);
     if (S.gemm_tiles > 0) {
+      const double flops = sym ? S.gemm_flops_sym : S.gemm_flops;
+      const int64_t t64 = sym ? S.gemm_tiles_sym : S.gemm_tiles, t128 = sym ? S.gemm_tiles128_sym : S.gemm_tiles128;
+      const int64_t *p64 = (sym ? I.d_gprefix_sym : I.d_gprefix).as<int64_t>() + S.gemm_off;
+      const int64_t *p128 = (sym ? I.d_gprefix128_sym : I.d_gprefix128).as<int64_t>() + S.gemm_off;
       if (std::is_same<T, float2>::value && S.deep && (I.deep_gemm == 2 || (I.deep_gemm == 1 && S.deep_fit)))
-        I.timed(3, S.gemm_flops * ninst, [&] {
-          hipLaunchKernelGGL(k_dense_gemm_t128, dim3((unsigned)(S.gemm_tiles128 * ninst)), dim3(256), 0, st,
-                             I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix128.as<int64_t>() + S.gemm_off,
-                             (int)S.gemm_n, S.gemm_tiles128, ninst, B);
+        I.timed(3, flops * ninst, [&] {
+          hipLaunchKernelGGL(k_dense_gemm_t128, dim3((unsigned)(t128 * ninst)), dim3(256), 0, st,
+                             I.d_gemm.as<GemmJob>() + S.gemm_off, p128, (int)S.gemm_n, t128, ninst, B, tri);
         });
       else
-        I.timed(0, S.gemm_flops * ninst, [&] {
-          hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
-                             I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
-                             (int)S.gemm_n, B);
+        I.timed(0, flops * ninst, [&] {
+          hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)t64, ninst), dim3(256), 0, st,
+                             I.d_gemm.as<GemmJob>() + S.gemm_off, p64, (int)S.gemm_n, B, tri);
         });
     }
     LU_LAUNCH_CHECK();
@@ -1885,7 +2071,7 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
       I.timed(0, S.gemm_flops * ninst, [&] {
         hipLaunchKernelGGL((k_dense_gemm<T>), dim3((unsigned)S.gemm_tiles, ninst), dim3(256), 0, st,
                            I.d_gemm.as<GemmJob>() + S.gemm_off, I.d_gprefix.as<int64_t>() + S.gemm_off,
-                           (int)S.gemm_n, B);
+                           (int)S.gemm_n, B, 0);
       });
     LU_LAUNCH_CHECK();
   }
@@ -1896,7 +2082,7 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
 
 template <typename T>
 static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const double *vals, int64_t vstride,
-                        LuFactorStats *stats) {
+                        LuFactorStats *stats, bool sym) {
   hipStream_t st = I.st;
   const int64_t fs = I.fstride;
   const int64_t es = (int64_t)I.no_max * I.no_max;
@@ -1933,10 +2119,10 @@ static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const do
       const int64_t a = I.ea_off[(size_t)l * MC + s], b = I.ea_off[(size_t)l * MC + s + 1];
       if (b > a)
         hipLaunchKernelGGL((k_lu_extend_add<T>), dim3((unsigned)(b - a), ninst), dim3(256), 0, st,
-                           I.d_ea.as<int2>() + a, D, F, I.ea_cb);
+                           I.d_ea.as<int2>() + a, D, F, I.ea_cb, sym ? 1 : 0);
     }
     LU_LAUNCH_CHECK();
-    dense_level<T>(I, I.dlev[l], ninst, B);
+    dense_level<T>(I, I.dlev[l], ninst, B, sym);
   }
   if (I.root >= 0) {
     const int r = I.root;
@@ -1947,10 +2133,10 @@ static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const do
       if (no == 0) continue;
       hipLaunchKernelGGL((k_lu_root_gather<T>), dim3(grid_for((int64_t)no * no, 16384), ninst), dim3(256), 0, st,
                          F + Tr.foff[r], fs, nfr, I.d_O.as<int32_t>() + (int64_t)e * I.no_max, no, E + (int64_t)e * es,
-                         (int64_t)I.nexc * es);
+                         (int64_t)I.nexc * es, sym ? 1 : 0);
       LU_LAUNCH_CHECK();
     }
-    dense_level<T>(I, I.dlev[Tr.nlevels], ninst, B);
+    dense_level<T>(I, I.dlev[Tr.nlevels], ninst, B, sym);
   }
   unsigned long long zp = 0;
   HIPCHK(hipMemcpyAsync(&zp, I.zp.p, 8, hipMemcpyDeviceToHost, st));
@@ -1960,16 +2146,19 @@ static void factor_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, const do
   }
 }
 
-void LuFactor::factor(int prec, int ninst, const double *vals, int64_t vals_stride, LuFactorStats *stats) {
+void LuFactor::factor(int prec, int ninst, const double *vals, int64_t vals_stride, LuFactorStats *stats,
+                      bool symmetric) {
   if (!T_) throw std::logic_error("lu: setup first");
   if (ninst < 1) throw std::invalid_argument("lu: ninst must be >= 1");
   const auto t0 = std::chrono::steady_clock::now();
   prec_ = prec;
   ninst_ = ninst;
+  const bool sym = symmetric && (d_->sym_mode == 2 || (d_->sym_mode == 1 && prec != 0));
+  sym_ = sym;
   if (prec == 0)
-    factor_impl<double2>(*d_, *T_, ninst, vals, vals_stride, stats);
+    factor_impl<double2>(*d_, *T_, ninst, vals, vals_stride, stats, sym);
   else
-    factor_impl<float2>(*d_, *T_, ninst, vals, vals_stride, stats);
+    factor_impl<float2>(*d_, *T_, ninst, vals, vals_stride, stats, sym);
   const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) stats->seconds_factor = dt;
   d_->factor_wall += dt;

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -158,11 +160,11 @@ struct Ctx {
 };
 
 // ------------------------------------------------------------------ small kernels
-__global__ void k_scatter_vals(int S, int64_t n, int npd, const int32_t *__restrict__ pd,
-                               const double2 *__restrict__ vals, double2 *__restrict__ x) {
-  const int s = blockIdx.y;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npd; i += gridDim.x * blockDim.x)
-    x[(int64_t)s * n + pd[(int64_t)s * npd + i]] = vals[(int64_t)s * npd + i];
+// x[pos[i]] = vals[i]: the Dirichlet values of every system of a batch at once
+__global__ void k_scatter_pos(int64_t count, const int64_t *__restrict__ pos, const double2 *__restrict__ vals,
+                              double2 *__restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    x[pos[i]] = vals[i];
 }
 
 // warm start: x[s][i] = (1 - w[s]) stash[ia[s]][i] + w[s] stash[ib[s]][i] on the free rows of system s
@@ -1406,7 +1408,7 @@ static void h_solve(Ctx &c, const double2 *x, int S, const std::vector<cplx> &sc
     const std::vector<uint8_t> none((size_t)n, 0);
     lu_analyse_mesh(c.mesh, s, none, none, c.lu_leaf, c.p_tree);
     c.p_lu.setup(c.p_tree, 1, {}, st);
-    c.p_lu.factor(0, 1, c.d_P.as<double>(), nnz, nullptr);
+    c.p_lu.factor(0, 1, c.d_P.as<double>(), nnz, nullptr, true);  // the mass matrix P is symmetric
     c.have_p_lu = true;
   }
   if (!plu) build_h_patches(c);
@@ -1853,36 +1855,56 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     std::vector<std::vector<cplx>> mE((size_t)F * np), mH((size_t)F * np);
     std::vector<cplx> e2((size_t)F * np), h2((size_t)F * np);
     std::vector<double> uvec((size_t)S * pd.size() * 2, 0.0);
-    for (int fi = 0; fi < F; fi++)
-      for (int k = 0; k < np; k++) {
-        const auto &P = c.ports[port_of[k]];
-        const double *Es = in->mode_E + ((int64_t)ford[f0 + fi] * mstride + moff[k]) * 2;
-        const double *Hs = in->mode_H + ((int64_t)ford[f0 + fi] * mstride + moff[k]) * 2;
-        auto &me = mE[fi * np + k];
-        auto &mh = mH[fi * np + k];
-        port_project(P, s.ndf, Es, true, me);
-        port_project(P, s.ndf, Hs, false, mh);
-        std::vector<cplx> ue, uh;
-        port_mass_apply(P, me, ue);
-        port_mass_apply(P, mh, uh);
-        cplx a = 0, b = 0;
-        for (size_t i = 0; i < me.size(); i++) {
-          a += std::conj(me[i]) * ue[i];
-          b += std::conj(mh[i]) * uh[i];
-        }
-        e2[fi * np + k] = a;
-        h2[fi * np + k] = b;
-        // u for every system of this frequency: E part goes to x-dots, H part to h-dots
-        for (int r = 0; r < R; r++) {
-          const int sidx = fi * R + r;
-          for (size_t i = 0; i < ue.size(); i++) {
-            uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2] = ue[i].real();
-            uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2 + 1] = ue[i].imag();
-          }
-        }
-        // stash uh in mh's place for later (kept in mH via a second vector)
-        mh.insert(mh.end(), uh.begin(), uh.end());
+    // the (frequency, port) projections are independent host work (CG on each port's trace mass): spread over
+    // the host threads; every task writes its own entries of mE, mH, e2, h2 and uvec
+    auto project_one = [&](int task) {
+      const int fi = task / np, k = task % np;
+      const auto &P = c.ports[port_of[k]];
+      const double *Es = in->mode_E + ((int64_t)ford[f0 + fi] * mstride + moff[k]) * 2;
+      const double *Hs = in->mode_H + ((int64_t)ford[f0 + fi] * mstride + moff[k]) * 2;
+      auto &me = mE[fi * np + k];
+      auto &mh = mH[fi * np + k];
+      port_project(P, s.ndf, Es, true, me);
+      port_project(P, s.ndf, Hs, false, mh);
+      std::vector<cplx> ue, uh;
+      port_mass_apply(P, me, ue);
+      port_mass_apply(P, mh, uh);
+      cplx a = 0, b = 0;
+      for (size_t i = 0; i < me.size(); i++) {
+        a += std::conj(me[i]) * ue[i];
+        b += std::conj(mh[i]) * uh[i];
       }
+      e2[fi * np + k] = a;
+      h2[fi * np + k] = b;
+      // u for every system of this frequency: E part goes to x-dots, H part to h-dots
+      for (int r = 0; r < R; r++) {
+        const int sidx = fi * R + r;
+        for (size_t i = 0; i < ue.size(); i++) {
+          uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2] = ue[i].real();
+          uvec[((size_t)sidx * pd.size() + pd_off[k] + i) * 2 + 1] = ue[i].imag();
+        }
+      }
+      // stash uh in mh's place for later (kept in mH via a second vector)
+      mh.insert(mh.end(), uh.begin(), uh.end());
+    };
+    {
+      const int ntask = F * np;
+      const int nth = std::max(1, std::min<int>(ntask, (int)std::min(16u, std::thread::hardware_concurrency())));
+      std::atomic<int> next(0);
+      std::vector<std::thread> th;
+      std::vector<std::exception_ptr> err(nth);
+      for (int t = 0; t < nth; t++)
+        th.emplace_back([&, t] {
+          try {
+            for (int k; (k = next.fetch_add(1)) < ntask;) project_one(k);
+          } catch (...) {
+            err[t] = std::current_exception();
+          }
+        });
+      for (auto &t : th) t.join();
+      for (auto &e : err)
+        if (e) std::rethrow_exception(e);
+    }
     // coefficients and A(w)
     std::vector<double> cK, cM, cT;
     for (int fi = 0; fi < F; fi++) {
@@ -1934,26 +1956,32 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     W.x.alloc((size_t)S * n * 16);
     HIPCHK(hipMemsetAsync(W.x.p, 0, (size_t)S * n * 16, st));
     std::vector<double> bn(S, 0.0);
-    for (int fi = 0; fi < F; fi++)
-      for (int r = 0; r < R; r++) {
-        const auto &P = c.ports[port_of[r]];
-        const auto &me = mE[fi * np + r];
-        std::vector<double> vals(me.size() * 2);
-        double xn = 0;
-        for (size_t i = 0; i < me.size(); i++) {
-          vals[i * 2] = me[i].real();
-          vals[i * 2 + 1] = me[i].imag();
-          xn += std::norm(me[i]);
+    {  // every system's Dirichlet values in one upload and one launch: positions s n + dof
+      std::vector<int64_t> spos;
+      std::vector<double> svals;
+      for (int fi = 0; fi < F; fi++)
+        for (int r = 0; r < R; r++) {
+          const auto &P = c.ports[port_of[r]];
+          const auto &me = mE[fi * np + r];
+          double xn = 0;
+          for (size_t i = 0; i < me.size(); i++) {
+            spos.push_back((int64_t)(fi * R + r) * n + P.dofs[i]);
+            svals.push_back(me[i].real());
+            svals.push_back(me[i].imag());
+            xn += std::norm(me[i]);
+          }
+          bn[fi * R + r] = xn;
         }
-        bn[fi * R + r] = xn;
+      if (!spos.empty()) {
         DevBuf dv, dp;
-        dv.upload(vals, st);
-        dp.upload(P.dofs, st);
-        hipLaunchKernelGGL(k_scatter_vals, dim3(blocks_for((int64_t)me.size()), 1), dim3(256), 0, st, 1, n,
-                           (int)me.size(), dp.as<int32_t>(), dv.as<double2>(),
-                           W.x.as<double2>() + (int64_t)(fi * R + r) * n);
-        HIPCHK(hipStreamSynchronize(st));
+        dv.upload(svals, st);
+        dp.upload(spos, st);
+        hipLaunchKernelGGL(k_scatter_pos, dim3(blocks_for((int64_t)spos.size()), 1), dim3(256), 0, st,
+                           (int64_t)spos.size(), dp.as<int64_t>(), dv.as<double2>(), W.x.as<double2>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));  // dv and dp are freed at the end of this block
       }
+    }
     // ||b~||^2 = ||mask .* A x_D||^2 + ||x_D||^2
     {
       const int nblk = reduce_blocks(n);
@@ -2033,7 +2061,7 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
         vals = W.Amean.as<double>();
       }
       LuFactorStats lst;
-      c.lu.factor(mixed ? 1 : 0, nq, vals, nnz, &lst);
+      c.lu.factor(mixed ? 1 : 0, nq, vals, nnz, &lst, true);  // A(w) is complex symmetric
       lu_stats_zero += lst.zero_pivots;
       lu_zero_batch = lst.zero_pivots;
       lu_seconds += lst.seconds_factor;

@@ -234,3 +234,64 @@ def test_lu_apply_mfma_matches_vector_kernel():
     f = np.linspace(9.0e9, 9.3e9, 4)
     o = pr.solve_point(f[2], {a: te10(m, a, f[2]) for a in pr.ports})
     assert np.abs(S1[2] - o["S"]).max() < 1e-8
+
+
+SYM_CHILD = r"""
+import json
+import numpy as np
+from openparem_amd.mesh import tee_mesh, waveguide_mesh
+from openparem_amd.solver import Solver, lu_stats
+out = {}
+for name, m in (("tee", tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=False)),
+                ("guide", waveguide_mesh(nx=6, ny=3, nz=12, seed=7))):
+    f = np.array([9e9, 11e9])
+    with Solver(m, 2) as s:
+        r = s.sweep(f, precision="fp64", tol=1e-12, batch=2, restart=30, precond="lu")
+        rm = s.sweep(f, precision="mixed", tol=1e-11, batch=2, restart=30, precond="lu", orth_passes=1)
+        st = lu_stats(s)
+    out[name] = {"S": [[float(z.real), float(z.imag)] for z in r["S"].ravel()],
+                 "Sm": [[float(z.real), float(z.imag)] for z in rm["S"].ravel()], "shape": list(r["S"].shape),
+                 "converged": int(r["converged"].all() and rm["converged"].all()), "iters": r["iters"].tolist(),
+                 "iters_mixed": rm["iters"].tolist(), "flops": st["gemm_flops"] + st["gemm128_flops"]}
+print(json.dumps(out), flush=True)
+"""
+
+
+def test_lu_symmetric_factorization_against_oracle():
+    """The symmetric factorization (lower tiles of every trailing update, rows of each outer block mirrored from its
+    columns, pivots inside the outer block, lower-triangle extend-add and root gather) forced onto the FP64 factors
+    too (OPEM_LU_SYM=2) and switched off (=0), in child processes, on the three-port tee (port root gathered per
+    excitation) and a guide whose fronts span several 128-column outer blocks: an FP64 factor of the frequency's own
+    matrix is exact, so GMRES must stop within a few iterations (a stale mirrored row would not), S must match the
+    oracle's direct solve, and the symmetric form must count fewer trailing-update flops."""
+    import subprocess
+    import sys
+    root = os.path.dirname(HERE)
+    out = {}
+    for mode in ("0", "2"):
+        env = dict(os.environ, OPEM_LU_SYM=mode, OPEM_LU_TIMING="1")
+        env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        print(f"symmetric factorization mode {mode}", flush=True)
+        p = subprocess.run([sys.executable, "-c", SYM_CHILD], cwd=root, env=env, capture_output=True, text=True,
+                           timeout=150)
+        assert p.returncode == 0, p.stderr[-2000:]
+        out[mode] = json.loads(p.stdout.strip().splitlines()[-1])
+        for name, d in out[mode].items():
+            print(f"mode {mode} {name}: iters fp64 {d['iters']} mixed {d['iters_mixed']} flops {d['flops']:.3e}",
+                  flush=True)
+            assert d["converged"] == 1
+            assert max(max(it) for it in d["iters"]) <= 6
+            assert max(max(it) for it in d["iters_mixed"]) <= 20
+    for name, m in (("tee", tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=False)),
+                    ("guide", waveguide_mesh(nx=6, ny=3, nz=12, seed=7))):
+        assert out["2"][name]["flops"] < 0.8 * out["0"][name]["flops"]
+        pr = Problem(m, 2)
+        f = np.array([9e9, 11e9])
+        for mode in ("0", "2"):
+            d = out[mode][name]
+            S = np.array([complex(a, b) for a, b in d["S"]]).reshape(d["shape"])
+            Sm = np.array([complex(a, b) for a, b in d["Sm"]]).reshape(d["shape"])
+            for i, fi in enumerate(f):
+                o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+                assert np.abs(S[i] - o["S"]).max() < 1e-9
+                assert np.abs(Sm[i] - o["S"]).max() < 1e-8
