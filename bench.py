@@ -134,68 +134,128 @@ def _lu_roofline(st, workload):
 
 def _uneven(args):
     """The uneven set: 16 seeded guide, microstrip, tee and cross instances (ND1-3, 2-4 ports, 8-32 frequencies,
-    50 k - 1.5 M DOF), one after another.  Per instance: a one-frequency warm-up sweep (nested-dissection analysis,
-    kernel loading), the timed mixed sweep of all its frequencies, then the plain FP64 path on every frequency.  A
-    point counts when both converged and |dS| < 1e-6; value = counted points / summed mixed seconds."""
+    50 k - 1.5 M DOF).  ``--concurrent K`` instances share the GPU at a time: K host threads, each with its own
+    context (its own HIP stream and workspace, limited to 3/(4K) of the device memory), take instances largest first
+    and, per instance, build the context (mesh, space, nested-dissection analysis), run a one-frequency warm-up sweep
+    and the timed mixed sweep of all its frequencies.  value = passing points / wall time of that whole phase (set-up
+    and warm-up included).  Then the plain FP64 path sweeps every instance the same way (its own timed phase).  A
+    point passes when both converged and |dS| < 1e-6.  Lanes: with K > 1, lane 0 takes the large instances (those
+    estimated at 7% or more of the set's work) one at a time with 55% of the device memory, then helps with the rest;
+    lanes 1 .. K-1 share 20% and take the other instances largest first, so large factors never meet in memory."""
+    import threading
+
     from openparem_amd import workloads
     from openparem_amd.solver import Solver, lu_stats
 
     keys = ("gemm_s", "gemm_flops", "gemm_launches", "panel_s", "panel_launches", "apply_s", "apply_bytes",
             "apply_launches", "factor_wall_s", "matrices_factored", "matrix_applies", "gemm128_s", "gemm128_flops",
             "gemm128_launches")
-    agg = {k: 0.0 for k in keys}
-    per, tot_t, tot64_t, tot_pass, tot_pts, tot_conv = [], 0.0, 0.0, 0, 0, 0
-    for i, w in enumerate(workloads.c5_instances()):
+    inst = workloads.c5_instances()
+    K = max(1, int(args.concurrent))
+    dof_per_tet = {1: 1.2, 2: 5.3, 3: 13.5}
+    cost = [(len(w["mesh"].tets) * dof_per_tet[w["order"]]) ** 1.4 * len(w["freqs"]) for w in inst]
+    order = sorted(range(len(inst)), key=lambda i: -cost[i])  # largest first
+    big = [i for i in order if K > 1 and cost[i] >= 0.07 * sum(cost)]
+    small = [i for i in order if i not in big]
+
+    def share_of(f):
+        return 2 if len(f) > 1 and f[1] - f[0] <= 0.5e9 else 1
+
+    def phase(fp64):
+        out, errs = {}, []
+        queues = {"big": list(big), "small": list(small)}
+        lock = threading.Lock()
+
+        def worker(lane):
+            frac = 0.75 if K == 1 else (0.55 if lane == 0 else 0.2 / (K - 1))
+            while True:
+                with lock:
+                    if errs:
+                        return
+                    q = queues["big"] if lane == 0 and queues["big"] else queues["small"]
+                    if not q:
+                        return
+                    i = q.pop(0)
+                w = inst[i]
+                f = w["freqs"]
+                try:
+                    ta = time.perf_counter()
+                    s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
+                    try:
+                        opts = dict(restart=args.restart, tol=args.tol, maxit=600, precond="lu",
+                                    mem_fraction=frac)
+                        if fp64:
+                            run = dict(precision="fp64", batch=1, orth_passes=2, **opts)
+                        else:
+                            run = dict(precision="mixed", precond_share=share_of(f), orth_passes=1, **opts)
+                        s.sweep(f[:1], **run)
+                        lu_stats(s, reset=True)
+                        t0 = time.perf_counter()
+                        r = s.sweep(f, **run)
+                        t1 = time.perf_counter()
+                        st = lu_stats(s, reset=True)
+                        out[i] = dict(r=r, sweep_s=t1 - t0, total_s=t1 - ta, ndof=s.ndof, ports=len(s.ports),
+                                      stats=st)
+                    finally:
+                        s.close()
+                    _progress(f"{'fp64' if fp64 else 'mixed'} instance {i} ({w['family']} ND{w['order']}, "
+                              f"{out[i]['ndof']} DOF, {len(f)} freqs): sweep {t1 - t0:.2f} s, with set-up "
+                              f"{t1 - ta:.2f} s")
+                except Exception as e:  # noqa: BLE001 - reported after the phase
+                    with lock:
+                        errs.append((i, repr(e)))
+                    return
+
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=worker, args=(lane,)) for lane in range(K)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        wall = time.perf_counter() - t0
+        if errs:
+            raise RuntimeError(f"uneven set failed: {errs}")
+        return out, wall
+
+    mix, wall = phase(False)
+    _progress(f"mixed phase: {wall:.2f} s wall with {K} concurrent instance(s)")
+    p64, wall64 = phase(True)
+    _progress(f"plain FP64 phase: {wall64:.2f} s wall")
+    agg = {k: sum(mix[i]["stats"][k] for i in mix) for k in keys}
+    per, tot_pass, tot_pts, tot_conv = [], 0, 0, 0
+    for i, w in enumerate(inst):
         f = w["freqs"]
-        s = Solver(w["mesh"], w["order"], boundaries=w["boundaries"])
-        try:
-            share = 2 if len(f) > 1 and f[1] - f[0] <= 0.5e9 else 1
-            opts = dict(restart=args.restart, tol=args.tol, maxit=600, precond="lu")
-            mixed = dict(precision="mixed", precond_share=share, orth_passes=1, **opts)
-            s.sweep(f[:1], **mixed)
-            lu_stats(s, reset=True)
-            t0 = time.perf_counter()
-            r = s.sweep(f, **mixed)
-            dt = time.perf_counter() - t0
-            st = lu_stats(s, reset=True)
-            for k in keys:
-                agg[k] += st[k]
-            _progress(f"instance {i}: mixed sweep {dt:.2f} s")
-            t0 = time.perf_counter()
-            r64 = s.sweep(f, precision="fp64", batch=1, orth_passes=2, **opts)
-            dt64 = time.perf_counter() - t0
-            dS = np.abs(r["S"] - r64["S"]).max(axis=(1, 2))
-            conv = np.all(r["converged"].astype(bool), axis=1)
-            ok = conv & np.all(r64["converged"].astype(bool), axis=1) & (dS < 1e-6)
-            per.append({"instance": i, "family": w["family"], "order": w["order"], "ndof": s.ndof,
-                        "ports": len(s.ports), "freqs": len(f), "band_GHz": [f[0] / 1e9, f[-1] / 1e9],
-                        "share": share, "mixed_s": round(dt, 3), "fp64_s": round(dt64, 3),
-                        "points_converged": int(conv.sum()), "points_passing": int(ok.sum()),
-                        "iters_mean": float(r["iters"].mean()), "max_abs_dS_vs_fp64": float(dS.max())})
-            tot_t += dt
-            tot64_t += dt64
-            tot_pass += int(ok.sum())
-            tot_conv += int(conv.sum())
-            tot_pts += len(f)
-            _progress(f"instance {i} ({w['family']} ND{w['order']}, {s.ndof} DOF, {len(f)} freqs): mixed {dt:.2f} s, "
-                      f"fp64 {dt64:.2f} s, {int(ok.sum())}/{len(f)} pass, max |dS| {dS.max():.2e}")
-        finally:
-            s.close()
+        r, r64 = mix[i]["r"], p64[i]["r"]
+        dS = np.abs(r["S"] - r64["S"]).max(axis=(1, 2))
+        conv = np.all(r["converged"].astype(bool), axis=1)
+        ok = conv & np.all(r64["converged"].astype(bool), axis=1) & (dS < 1e-6)
+        per.append({"instance": i, "family": w["family"], "order": w["order"], "ndof": mix[i]["ndof"],
+                    "ports": mix[i]["ports"], "freqs": len(f), "band_GHz": [f[0] / 1e9, f[-1] / 1e9],
+                    "share": share_of(f), "mixed_s": round(mix[i]["sweep_s"], 3),
+                    "mixed_with_setup_s": round(mix[i]["total_s"], 3), "fp64_s": round(p64[i]["sweep_s"], 3),
+                    "points_converged": int(conv.sum()), "points_passing": int(ok.sum()),
+                    "iters_mean": float(r["iters"].mean()), "max_abs_dS_vs_fp64": float(dS.max())})
+        tot_pass += int(ok.sum())
+        tot_conv += int(conv.sum())
+        tot_pts += len(f)
     roof, kern = _lu_roofline(agg, "uneven-c5")
-    value = tot_pass / tot_t
-    fp64_value = tot_pts / tot64_t
+    value = tot_pass / wall
+    fp64_value = tot_pts / wall64
     line = {
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",
         "value": value, "unit": "freq-points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": 1e3 * tot_t, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c-fp32 factor and inner GMRES / c-fp64 refinement", "data": "synthetic",
-        "config": {"workload": f"uneven-c5: {len(per)} instances one after another, ND1-3, 2-4 ports, "
-                               f"{tot_pts} frequency points, {min(p['ndof'] for p in per)}-"
-                               f"{max(p['ndof'] for p in per)} DOF",
+        "config": {"workload": f"uneven-c5: {len(per)} instances, {K} at a time on one GPU (one context and stream "
+                               f"each), ND1-3, 2-4 ports, {tot_pts} frequency points, "
+                               f"{min(p['ndof'] for p in per)}-{max(p['ndof'] for p in per)} DOF; wall time of "
+                               f"the whole phase including each instance's set-up and warm-up",
+                   "concurrent": K,
                    "precond": "sparse LU per instance (FP32 factor, shared by 2 adjacent frequencies where they are "
                               f"at most 0.5 GHz apart), GMRES({args.restart}) inside FP64 refinement"},
         "points": tot_pts, "points_converged": tot_conv, "points_passing": tot_pass,
-        "fp64_plain": {"value": fp64_value, "unit": "freq-points/s", "points": tot_pts},
+        "mixed_sweep_seconds_sum": sum(mix[i]["sweep_s"] for i in mix),
+        "fp64_plain": {"value": fp64_value, "unit": "freq-points/s", "points": tot_pts, "wall_s": wall64},
         "speedup_vs_fp64_plain": value / fp64_value,
         "max_abs_dS_vs_fp64": max(p["max_abs_dS_vs_fp64"] for p in per),
         "roofline": roof, "kernels": kern,
@@ -216,6 +276,8 @@ def main():
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--share", type=int, default=4, help="adjacent frequencies sharing one FP32 LU factor")
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--concurrent", type=int, default=3,
+                    help="uneven-c5: instances sharing the GPU at a time (own context and stream each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the systems of a sweep are "

@@ -322,6 +322,9 @@ typedef struct {
                                 /*    index, every 2^L-th frequency first, then the midpoints, each     */
                                 /*    started from the linear interpolation in frequency of its solved  */
                                 /*    neighbours (free rows); 0: off                                   */
+  double mem_fraction;          /* share of the device memory the batches of this sweep may fill by the */
+                                /*    planner's estimate (0 -> 0.75); contexts that sweep on one GPU at */
+                                /*    the same time divide it between them                             */
 } opem_sweep_in;
 
 typedef struct {

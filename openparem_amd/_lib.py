@@ -40,7 +40,7 @@ class SweepIn(C.Structure):
                 ("coarse_size", C.c_int), ("orth_passes", C.c_int),
                 ("precond_share", C.c_int), ("poly_degree", C.c_int),
                 ("arnoldi_step", C.c_int),
-                ("warm_start", C.c_int)]
+                ("warm_start", C.c_int), ("mem_fraction", C.c_double)]
 
 
 class SolveOpts(C.Structure):

@@ -1799,11 +1799,16 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
                  (double)s.nnz * 16 / (g * R);
     } else
       per_sys += patch ? (double)c.patches.idx.size() * 16 : (double)s.nd * n * 16;
-    const double fit = std::floor(0.75 * (double)mtotal / (per_sys * R));
+    // the share of the device memory the estimate may fill: the caller's (several contexts on one GPU), else 3/4
+    const double frac = in->mem_fraction > 0 ? std::min(0.95, in->mem_fraction) : 0.75;
+    const double fit = std::floor(frac * (double)mtotal / (per_sys * R));
     if (fit < batch) batch = (int)std::max(1.0, fit);
     // an LU batch holds whole groups of precond_share frequencies
     const int g = std::max(1, in->precond_share);
     if (use_lu && g > 1 && batch > g) batch -= batch % g;
+    if (use_lu && getenv("OPEM_LU_STATS"))
+      fprintf(stderr, "opem: batch plan: %.3g GB per system (estimate), %.3g GB free of %.3g GB, %d frequencies "
+              "per batch\n", per_sys / 1e9, mfree / 1e9, mtotal / 1e9, batch);
   }
   const int passes = in->orth_passes == 1 ? 1 : 2;
   const double tol = in->tol > 0 ? in->tol : 1e-10;
@@ -2175,10 +2180,13 @@ static void sweep(Ctx &c, const opem_sweep_in *in, opem_sweep_out *out) {
     HIPCHK(hipStreamSynchronize(st));
     t_post += now_s() - tc;
   }
-  if (use_lu && getenv("OPEM_LU_STATS"))
+  if (use_lu && getenv("OPEM_LU_STATS")) {
+    size_t mfree = 0, mtotal = 0;
+    HIPCHK(hipMemGetInfo(&mfree, &mtotal));
     fprintf(stderr, "opem: sparse LU %d nodes, max front %lld, %.3g GFLOP per factor, factor time %.3f s, %lld zero "
-            "pivots\n", c.lu_tree.nnodes, (long long)c.lu_tree.max_front, c.lu_tree.flops / 1e9, lu_seconds,
-            (long long)lu_stats_zero);
+            "pivots, %.3g GB of device memory in use\n", c.lu_tree.nnodes, (long long)c.lu_tree.max_front,
+            c.lu_tree.flops / 1e9, lu_seconds, (long long)lu_stats_zero, (mtotal - mfree) / 1e9);
+  }
   out->seconds_total = now_s() - t0;
   out->seconds_setup = t_setup;
   out->seconds_solve = t_solve;
@@ -3479,6 +3487,7 @@ int opem_sweep(opem_ctx *ctx, const opem_sweep_in *in, opem_sweep_out *out) {
     if (in->poly_degree < 0 || in->poly_degree > 32) throw std::invalid_argument("poly_degree must be 0..32");
     if (in->arnoldi_step < 0 || in->arnoldi_step > 4) throw std::invalid_argument("arnoldi_step must be 0..4");
     if (in->warm_start < 0 || in->warm_start > 4) throw std::invalid_argument("warm_start must be 0..4");
+    if (!(in->mem_fraction >= 0 && in->mem_fraction <= 1)) throw std::invalid_argument("mem_fraction must be 0..1");
     for (int f = 0; f < in->nfreq; f++)
       if (!(in->freqs[f] > 0)) throw std::invalid_argument("frequencies must be positive");
     // the context keeps the options only once every argument has passed

@@ -181,7 +181,7 @@ class Solver:
     # ------------------------------------------------------------------ hot path
     def sweep(self, freqs, precision="fp64", batch=0, restart=150, maxit=20000, tol=1e-10, reciprocal=True,
               h_tol=1e-13, want_fields=False, mode_fn=None, precond="patch", coarse=0, orth_passes=2,
-              precond_share=1, poly_degree=0, arnoldi_step=1, warm_start=0):
+              precond_share=1, poly_degree=0, arnoldi_step=1, warm_start=0, mem_fraction=0.0):
         """One ``opem_sweep``: S (forced reciprocal unless ``reciprocal=False``) for every frequency, all port
         excitations.  ``precision`` "fp64" or "mixed" (FP32 inner GMRES, FP64 refinement); ``batch`` frequencies per
         batched solve (0: all); GMRES(``restart``) to the true residual ``tol``; ``precond`` "patch" (vertex
@@ -189,7 +189,8 @@ class Solver:
         precision); ``precond_share`` adjacent frequencies per set of patch inverses or per LU factor (their mean
         matrix);
         ``arnoldi_step`` Krylov vectors per Gram-Schmidt pass; ``warm_start`` bisection levels over the frequency
-        index (interpolated starting iterates).  ``mode_fn(mesh, attr, f)`` supplies each port's modal E, H, gamma,
+        index (interpolated starting iterates); ``mem_fraction`` the share of device memory the batch planner may
+        fill (0: three quarters; contexts sweeping on one GPU at once divide it).  ``mode_fn(mesh, attr, f)`` supplies each port's modal E, H, gamma,
         V, Z (default: the mesh family's provider).  Returns S, S_raw, Cp, Cm, iters, true_res, converged,
         fallback (FP64 GMRES finished a system whose FP32 refinement stagnated), singular (the system's LU batch met
         exactly zero pivots) and, with ``want_fields``, the E and H DOFs x, h [freq][excitation][ndof]."""
@@ -219,7 +220,8 @@ class Solver:
                            reciprocal=int(bool(reciprocal)), h_tol=float(h_tol),
                            precond={"patch": 0, "element": 1, "lu": 2}[precond], coarse_size=int(coarse), orth_passes=int(orth_passes),
                            precond_share=int(precond_share), poly_degree=int(poly_degree),
-                           arnoldi_step=int(arnoldi_step), warm_start=int(warm_start))
+                           arnoldi_step=int(arnoldi_step), warm_start=int(warm_start),
+                           mem_fraction=float(mem_fraction))
         S = np.zeros((nf, np_, np_), np.complex128)
         Sr = np.zeros_like(S)
         Cp = np.zeros_like(S)

@@ -89,6 +89,8 @@ def test_misuse_is_reported_and_leaves_the_context_usable():
             (dict(poly_degree=33), "poly_degree"),
             (dict(arnoldi_step=5), "arnoldi_step"),
             (dict(warm_start=-1), "warm_start"),
+            (dict(mem_fraction=1.5), "mem_fraction"),
+            (dict(mem_fraction=float("nan")), "mem_fraction"),
             (dict(precond_share=-1), "precond_share"),
         ]
         for kw, msg in cases:
