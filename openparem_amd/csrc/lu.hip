@@ -973,31 +973,44 @@ __global__ void k_lu_inst_active(int ninst, int nrhs, const int32_t *__restrict_
   iact[q] = a;
 }
 
-// Front vectors of the nodes of one level: pivot rows from b, update rows zero, then each child's update part
-// added at its umap rows (children one after another inside the workgroup).
-__global__ __launch_bounds__(256) void k_lu_fwd_gather(const int32_t *__restrict__ nodes, LuDev D, int nrhs,
-                                                       const double2 *__restrict__ ball, double2 *__restrict__ fvall) {
+// Front vectors of the nodes of one level (forward solve): k_lu_fwd_init fills kFvRows rows of a front's vector per
+// task (the right-hand side on the pivot rows, zero on the update rows), then k_lu_fwd_add, once per child slot,
+// adds kFvRows update rows of one child per task at the rows umap gives (children of one slot have different parents and umap
+// is injective, so no two threads add to one entry; the slots are added in a fixed order, so the sums do not
+// depend on scheduling).  Task = (node, first row).
+constexpr int kFvRows = 32;
+
+__global__ __launch_bounds__(256) void k_lu_fwd_init(const int2 *__restrict__ tasks, LuDev D, int nrhs,
+                                                     const double2 *__restrict__ ball, double2 *__restrict__ fvall) {
   const int q = blockIdx.y;
   if (D.iact && !D.iact[q]) return;
-  const int t = nodes[blockIdx.x];
+  const int2 tk = tasks[blockIdx.x];
+  const int t = tk.x;
   const int64_t nd = D.dbeg[t + 1] - D.dbeg[t], nf = nd + (D.uoff[t + 1] - D.uoff[t]);
   double2 *fv = fvall + q * D.fvstride + D.fvoff[t] * nrhs;
   const double2 *b = ball + q * D.n * nrhs + D.dbeg[t] * nrhs;
-  for (int64_t e = threadIdx.x; e < nf * nrhs; e += blockDim.x) fv[e] = e < nd * nrhs ? b[e] : make_double2(0.0, 0.0);
-  __syncthreads();
-  for (int k = D.kid_off[t]; k < D.kid_off[t + 1]; k++) {
-    const int c = D.kids[k];
-    const int64_t ndc = D.dbeg[c + 1] - D.dbeg[c], nuc = D.uoff[c + 1] - D.uoff[c];
-    const double2 *fc = fvall + q * D.fvstride + (D.fvoff[c] + ndc) * nrhs;
-    const int32_t *um = D.umap + D.uoff[c];
-    for (int64_t e = threadIdx.x; e < nuc * nrhs; e += blockDim.x) {
-      const int64_t a = e / nrhs;
-      const int j = (int)(e % nrhs);
-      double2 &dst = fv[(int64_t)um[a] * nrhs + j];
-      dst.x += fc[e].x;
-      dst.y += fc[e].y;
-    }
-    __syncthreads();
+  const int64_t e1 = min((int64_t)tk.y + kFvRows, nf) * nrhs;
+  for (int64_t e = (int64_t)tk.y * nrhs + threadIdx.x; e < e1; e += blockDim.x)
+    fv[e] = e < nd * nrhs ? b[e] : make_double2(0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_lu_fwd_add(const int2 *__restrict__ tasks, LuDev D, int nrhs,
+                                                    double2 *__restrict__ fvall) {
+  const int q = blockIdx.y;
+  if (D.iact && !D.iact[q]) return;
+  const int2 tk = tasks[blockIdx.x];
+  const int c = tk.x, p = D.parent[c];
+  const int64_t ndc = D.dbeg[c + 1] - D.dbeg[c], nuc = D.uoff[c + 1] - D.uoff[c];
+  const double2 *fc = fvall + q * D.fvstride + (D.fvoff[c] + ndc) * nrhs;
+  double2 *fp = fvall + q * D.fvstride + D.fvoff[p] * nrhs;
+  const int32_t *um = D.umap + D.uoff[c];
+  const int64_t e1 = min((int64_t)tk.y + kFvRows, nuc) * nrhs;
+  for (int64_t e = (int64_t)tk.y * nrhs + threadIdx.x; e < e1; e += blockDim.x) {
+    const int64_t a = e / nrhs;
+    const int j = (int)(e - a * nrhs);
+    double2 &dst = fp[(int64_t)um[a] * nrhs + j];
+    dst.x += fc[e].x;
+    dst.y += fc[e].y;
   }
 }
 
@@ -1178,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
                                                        const double2 *__restrict__ yall,
                                                        const double2 *__restrict__ tall,
                                                        const double2 *__restrict__ xall,
-                                                       const double2 *__restrict__ yrall, double2 *__restrict__ part,
+                                                       const double2 *__restrict__ yrall, float2 *__restrict__ part,
                                                        int64_t part_qstride, int kch) {
   constexpr int kKM = 128;         // columns staged per pass (one pass for the default chunk of 128 columns)
   __shared__ float2 sX[kKM][16];  // staged inputs: column k of the chunk, right-hand side m
@@ -1315,8 +1328,9 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
     if (g < ng) mfma_group(g, bufA, kc, kn);
     __syncthreads();
   }
-  // D[i][m] of row block rb: lane l, register r: i = 4 (l / 16) + r, m = l % 16
-  double2 *pq = part + q * part_qstride + (int64_t)tk.w * 256 * nrhs;
+  // D[i][m] of row block rb: lane l, register r: i = 4 (l / 16) + r, m = l % 16; the FP32 chunk sums are stored in
+  // FP32 (k_lu_apply_sum adds them in FP64)
+  float2 *pq = part + q * part_qstride + (int64_t)tk.w * 256 * nrhs;
   if (NARROW) {
     // lane m (< 8) holds P1[:, m], P2[:, m]; lane m + 8 holds P1[:, m + 8], P2[:, m + 8]
     float o1[4][4], o2[4][4];
@@ -1336,7 +1350,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
           const int64_t ig = rbase + 16 * rb + 4 * lk + r;
           if (ig < rows) {
             const float re = p1[rb][r] - o2[rb][r], im = o1[rb][r] + p2[rb][r];
-            pq[(ig - tk.y) * nrhs + j] = make_double2((double)re, (double)im);
+            pq[(ig - tk.y) * nrhs + j] = make_float2(re, im);
           }
         }
     }
@@ -1349,18 +1363,19 @@ __global__ __launch_bounds__(256) void k_lu_apply_mfma(const int4 *__restrict__ 
         const int64_t ig = rbase + 16 * rb + 4 * lk + r;
         if (ig < rows) {
           const float re = p1[rb][r] - p2[rb][r], im = p3[rb][r] - p1[rb][r] - p2[rb][r];
-          pq[(ig - tk.y) * nrhs + j] = make_double2((double)re, (double)im);
+          pq[(ig - tk.y) * nrhs + j] = make_float2(re, im);
         }
       }
   }
 }
 
-template <int KIND>
+// P: the partial-sum type (double2 from the vector kernels, float2 from the MFMA kernel); sums are formed in FP64.
+template <int KIND, typename P>
 __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ tasks, LuDev D, RootDev Rt, int nrhs,
                                                       int ty_n, double2 *__restrict__ fvall,
                                                       double2 *__restrict__ yall, double2 *__restrict__ tall,
                                                       double2 *__restrict__ xall, double2 *__restrict__ yrall,
-                                                      const double2 *__restrict__ part, int64_t part_qstride) {
+                                                      const P *__restrict__ part, int64_t part_qstride) {
   const int q = blockIdx.y;
   if (D.iact && !D.iact[q]) return;
   const int4 tk = tasks[blockIdx.x];
@@ -1378,7 +1393,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
   double2 *fv = fvall + q * D.fvstride;
   double2 *y = yall + q * nrows_all, *tt = tall + q * nrows_all, *x = xall + q * nrows_all;
   const int32_t *O = (KIND == K_ROOT_L || KIND == K_ROOT_U) ? Rt.O + (int64_t)e * Rt.no_max : nullptr;
-  const double2 *pq = part + q * part_qstride;
+  const P *pq = part + q * part_qstride;
   const int nrow = (int)min((int64_t)rows_wg, rows - tk.y);
   // one (row, right-hand side) pair per thread and pass, right-hand sides fastest, so consecutive threads read
   // consecutive partial sums of a slot; the slots are summed four at a time with independent loads
@@ -1386,13 +1401,13 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
     const int tx = pi / nsub, m = pi % nsub;
     const int64_t i = tk.y + tx;
     const int j = j0 + m * rstep;
-    const double2 *src = pq + ((int64_t)tk.w * rows_wg + tx) * nrhs + j;
+    const P *src = pq + ((int64_t)tk.w * rows_wg + tx) * nrhs + j;
     const int64_t cstride = (int64_t)rows_wg * nrhs;
     double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
     int c = 0;
     for (; c + 4 <= tk.z; c += 4) {
-      const double2 v0 = src[(c + 0) * cstride], v1 = src[(c + 1) * cstride];
-      const double2 v2 = src[(c + 2) * cstride], v3 = src[(c + 3) * cstride];
+      const P v0 = src[(c + 0) * cstride], v1 = src[(c + 1) * cstride];
+      const P v2 = src[(c + 2) * cstride], v3 = src[(c + 3) * cstride];
       a0.x += v0.x;
       a0.y += v0.y;
       a1.x += v1.x;
@@ -1403,7 +1418,7 @@ __global__ __launch_bounds__(256) void k_lu_apply_sum(const int4 *__restrict__ t
       a3.y += v3.y;
     }
     for (; c < tk.z; c++) {
-      const double2 v = src[c * cstride];
+      const P v = src[c * cstride];
       a0.x += v.x;
       a0.y += v.y;
     }
@@ -1451,11 +1466,12 @@ struct LuFactor::Impl {
   int64_t fstride = 0, fvtot = 0;
   // schedule
   std::vector<std::vector<int32_t>> level_nodes;  // per level
-  DevBuf d_levels;                                // node lists, concatenated
-  std::vector<int64_t> level_off;
   DevBuf d_ea;
   std::vector<int64_t> ea_off;
   int ea_cb = 16;
+  // apply's forward gather: per level, (node, first row) init tasks; per (level, child slot), (child, first row)
+  DevBuf d_fvi, d_fva;
+  std::vector<int64_t> fvi_off, fva_off;
   // root
   int root = -1, nexc = 1, no_max = 0;
   std::vector<int32_t> no;
@@ -1929,13 +1945,6 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   const int L = T.nlevels;
   I.level_nodes.assign(L, {});
   for (int t = 0; t < nn; t++) I.level_nodes[T.level[t]].push_back(t);
-  std::vector<int32_t> cat;
-  I.level_off.assign(L + 1, 0);
-  for (int l = 0; l < L; l++) {
-    cat.insert(cat.end(), I.level_nodes[l].begin(), I.level_nodes[l].end());
-    I.level_off[l + 1] = (int64_t)cat.size();
-  }
-  I.d_levels.upload(cat, st);
   // extend-add tasks: children of the level-l nodes, by slot
   const int MC = std::max(1, T.max_children);
   std::vector<int2> ea;
@@ -1951,6 +1960,27 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
       I.ea_off[(size_t)l * MC + s + 1] = (int64_t)ea.size();
     }
   I.d_ea.upload(ea, st);
+  {
+    std::vector<int2> fvi, fva;
+    I.fvi_off.assign((size_t)L + 1, 0);
+    I.fva_off.assign((size_t)L * MC + 1, 0);
+    for (int l = 0; l < L; l++) {
+      for (int p : I.level_nodes[l])
+        for (int64_t r = 0; r < T.nf(p); r += kFvRows) fvi.push_back(make_int2(p, (int)r));
+      I.fvi_off[(size_t)l + 1] = (int64_t)fvi.size();
+      for (int s = 0; s < MC; s++) {
+        for (int p : I.level_nodes[l])
+          for (int k = kid_off[p]; k < kid_off[p + 1]; k++) {
+            const int c = kids[k];
+            if (T.slot[c] != s) continue;
+            for (int64_t r = 0; r < T.nu(c); r += kFvRows) fva.push_back(make_int2(c, (int)r));
+          }
+        I.fva_off[(size_t)l * MC + s + 1] = (int64_t)fva.size();
+      }
+    }
+    I.d_fvi.upload(fvi, st);
+    I.d_fva.upload(fva, st);
+  }
   // dense fronts: tree nodes (the port root is only gathered), then one block per excitation
   I.dfr.assign(nn + I.nexc, DenseFront{});
   for (int t = 0; t < nn; t++) {
@@ -2265,6 +2295,9 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
   const int64_t part_qstride = A.max_slots * rows_wg * nrhs;
   I.part.alloc((size_t)ninst * part_qstride * 16);
   double2 *part = I.part.as<double2>();
+  // FP32 factors on the MFMA kernel: every right-hand side of a task in one MFMA tile, FP32 partial sums
+  const bool mfma_path = std::is_same<T, float2>::value && ty_n == 1 && I.apply_mfma;
+  float2 *part32 = I.part.as<float2>();
   LuDev D = I.dev(Tr, nrhs);
   // instances whose systems are all inactive (converged) are skipped by every apply kernel
   int nact = ninst;
@@ -2308,13 +2341,13 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
       const dim3 g((unsigned)(a1 - a0), ninst);
       const int4 *tk = A.d1.as<int4>() + a0;
       if constexpr (std::is_same<T, float2>::value) {
-        if (ty_n == 1 && I.apply_mfma) {  // FP32 factor, every right-hand side of the task in one MFMA tile
+        if (mfma_path) {
           if (nrhs <= 8)
             hipLaunchKernelGGL((k_lu_apply_mfma<KIND, true>), g, dim3(256), 0, st, tk, D, Rt, nrhs, F, E, pp, fv, y,
-                               tt, x, yr, part, part_qstride, I.kch);
+                               tt, x, yr, part32, part_qstride, I.kch);
           else
             hipLaunchKernelGGL((k_lu_apply_mfma<KIND, false>), g, dim3(256), 0, st, tk, D, Rt, nrhs, F, E, pp, fv, y,
-                               tt, x, yr, part, part_qstride, I.kch);
+                               tt, x, yr, part32, part_qstride, I.kch);
           return;
         }
       }
@@ -2328,9 +2361,14 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
         hipLaunchKernelGGL((k_lu_apply_part<T, KIND, kMaxRhsThread>), g, dim3(256), lds, st, tk, D, Rt, nrhs, ty_n, F,
                            E, pp, fv, y, tt, x, yr, part, part_qstride, I.kch);
     });
-    if (b1 > b0)
-      hipLaunchKernelGGL((k_lu_apply_sum<KIND>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
-                         A.d2.as<int4>() + b0, D, Rt, nrhs, ty_n, fv, y, tt, x, yr, part, part_qstride);
+    if (b1 > b0) {
+      if (mfma_path)
+        hipLaunchKernelGGL((k_lu_apply_sum<KIND, float2>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
+                           A.d2.as<int4>() + b0, D, Rt, nrhs, ty_n, fv, y, tt, x, yr, part32, part_qstride);
+      else
+        hipLaunchKernelGGL((k_lu_apply_sum<KIND, double2>), dim3((unsigned)(b1 - b0), ninst), dim3(256), 0, st,
+                           A.d2.as<int4>() + b0, D, Rt, nrhs, ty_n, fv, y, tt, x, yr, part, part_qstride);
+    }
     LU_LAUNCH_CHECK();
   };
   using KL = std::integral_constant<int, K_FWD_L>;
@@ -2340,11 +2378,20 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
   using KRL = std::integral_constant<int, K_ROOT_L>;
   using KRU = std::integral_constant<int, K_ROOT_U>;
   const int L = Tr.nlevels;
+  const int MC = std::max(1, Tr.max_children);
   for (int l = 0; l < L; l++) {
-    const int64_t a0 = I.level_off[l], a1 = I.level_off[l + 1];
-    if (a1 > a0)
-      hipLaunchKernelGGL(k_lu_fwd_gather, dim3((unsigned)(a1 - a0), ninst), dim3(256), 0, st,
-                         I.d_levels.as<int32_t>() + a0, D, nrhs, b, fv);
+    {
+      const int64_t a0 = I.fvi_off[l], a1 = I.fvi_off[l + 1];
+      if (a1 > a0)
+        hipLaunchKernelGGL(k_lu_fwd_init, dim3((unsigned)(a1 - a0), ninst), dim3(256), 0, st,
+                           I.d_fvi.as<int2>() + a0, D, nrhs, b, fv);
+      for (int s2 = 0; s2 < MC; s2++) {
+        const int64_t c0 = I.fva_off[(size_t)l * MC + s2], c1 = I.fva_off[(size_t)l * MC + s2 + 1];
+        if (c1 > c0)
+          hipLaunchKernelGGL(k_lu_fwd_add, dim3((unsigned)(c1 - c0), ninst), dim3(256), 0, st,
+                             I.d_fva.as<int2>() + c0, D, nrhs, fv);
+      }
+    }
     LU_LAUNCH_CHECK();
     launch(KL(), l);
     launch(KL21(), l);
