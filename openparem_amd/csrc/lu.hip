@@ -2284,6 +2284,10 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
     A.d2.upload(A.all2, st);
     if (apply_stats)
       for (int kind = 0; kind < 6; kind++) {
+        for (int l = 0; l < L; l++)
+          if (A.off1[kind][l + 1] > A.off1[kind][l])
+            fprintf(stderr, "lu apply kind %d level %d: %lld tasks, %.4e factor entries\n", kind, l,
+                    (long long)(A.off1[kind][l + 1] - A.off1[kind][l]), A.elems[kind][l]);
         double el = 0;
         for (double v : A.elems[kind]) el += v;
         fprintf(stderr, "lu apply kind %d: %.0f tasks, %.3e factor entries, mean tile fill %.3f, rows<=64 %.3f, "

@@ -150,7 +150,8 @@ struct Ctx {
   bool have_p_lu = false;
   LuFactor *lu_pc = nullptr;  // set while a Krylov solve is preconditioned by lu
   int lu_nrhs = 0;            // systems per factor
-  int lu_leaf = 8;            // dissection leaves: at most this many mesh vertices
+  // dissection leaves: at most this many mesh vertices (OPEM_LU_LEAF overrides, 1..4096)
+  int lu_leaf = getenv("OPEM_LU_LEAF") ? std::max(1, std::min(4096, atoi(getenv("OPEM_LU_LEAF")))) : 8;
   // preconditioner rung of a caller-matrix solve: 0 the configured one, 2 point Jacobi (d_jacinv), 3 none
   int pc_override = 0;
   DevBuf d_jacinv;
