@@ -30,8 +30,9 @@ HBM_PEAK_GBS = 8000.0
 
 
 def _pmc_traffic_ratio(kernel, shape):
-    """HBM traffic / algorithmic bytes of a kernel at a shape, as measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
-    passes and recorded in profiles/roofline_pmc.md ("traffic_ratio <kernel> <shape> <ratio>"); None if absent."""
+    """HBM traffic of a kernel at a shape per unit of its algorithmic work (bytes per algorithmic byte, or per flop
+    for an MFMA-bound kernel), as measured by rocprofv3 FETCH_SIZE/WRITE_SIZE passes and recorded in
+    profiles/roofline_pmc.md ("traffic_ratio <kernel> <shape> <ratio>"); None if absent."""
     path = os.path.join(ROOT, "profiles", "roofline_pmc.md")
     if not os.path.exists(path):
         return None
@@ -126,7 +127,8 @@ def _lu_roofline(st, workload):
     work_per_launch = (dom["bytes"] if dom is apply_ else dom["flops"]) / max(1.0, dom["launches"])
     roof = {"kernel": dom["kernel"], "bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
             "unit": dom["unit"], "frac": dom["frac"],
-            "traffic": None if ratio is None or dom is not apply_ else ratio * work_per_launch,
+            # PMC traffic per launch: the apply's ratio to its algorithmic bytes, or the GEMM's bytes per flop
+            "traffic": None if ratio is None or dom is gemm else ratio * work_per_launch,
             "per_launch": work_per_launch, "launch_seconds": dom["seconds"] / max(1.0, dom["launches"]),
             "launches": dom["launches"]}
     return roof, {"k_dense_gemm_t128": deep, "k_dense_gemm": gemm, "k_lu_apply_mfma": apply_}
