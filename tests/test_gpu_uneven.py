@@ -64,3 +64,46 @@ def test_uneven_microstrip_band_share_group_of_one():
         s.close()
     assert r["converged"].all()
     assert np.abs(r["S"] - So[:7]).max() < 1e-8
+
+
+def test_uneven_instances_sharing_the_gpu():
+    """The four reduced instances swept at the same time from four host threads, each on its own context (own HIP
+    stream and workspace, a quarter of the planner's memory share), as the uneven set runs them: every instance's S
+    against the oracle's direct solve at every frequency, and against the same instance swept alone."""
+    import threading
+
+    from openparem_amd.solver import Solver
+    meshes = [make() for _, make, _, _ in CASES]
+
+    def sweep(k, out, frac):
+        _, _, order, freqs = CASES[k]
+        s = Solver(meshes[k], order)
+        try:
+            out[k] = s.sweep(freqs, precision="mixed", tol=1e-11, batch=len(freqs), restart=30, precond="lu",
+                             precond_share=2, orth_passes=1, mem_fraction=frac)
+        except Exception as e:  # noqa: BLE001 - checked below
+            out[k] = e
+        finally:
+            s.close()
+
+    alone, together = {}, {}
+    for k in range(len(CASES)):
+        sweep(k, alone, 0.0)
+    print("instances swept alone", flush=True)
+    th = [threading.Thread(target=sweep, args=(k, together, 0.75 / len(CASES))) for k in range(len(CASES))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    print("instances swept together", flush=True)
+    for k, (name, _, order, freqs) in enumerate(CASES):
+        assert not isinstance(together[k], Exception), (name, together[k])
+        r, r1 = together[k], alone[k]
+        assert r["converged"].all()
+        assert np.abs(r["S"] - r1["S"]).max() < 1e-9
+        pr = Problem(meshes[k], order)
+        for i, f in enumerate(freqs):
+            o = pr.solve_point(f, {a: te10(meshes[k], a, f) for a in pr.ports})
+            dS = np.abs(r["S"][i] - o["S"]).max()
+            print(f"{name} {f / 1e9:.2f} GHz: |dS| vs oracle {dS:.2e}", flush=True)
+            assert dS < 1e-8
