@@ -1542,6 +1542,14 @@ struct LuFactor::Impl {
   // symmetric factorization of matrices the caller declares symmetric: OPEM_LU_SYM=0 never, 1 (default) FP32
   // factors, 2 FP32 and FP64 factors
   int sym_mode = getenv("OPEM_LU_SYM") ? std::max(0, std::min(2, atoi(getenv("OPEM_LU_SYM")))) : 1;
+  // the apply's level sweeps replayed from HIP graphs (OPEM_LU_GRAPH=0: launched one by one; always one by one
+  // while kernel timing is on, whose events sit between the launches)
+  bool use_graph = !(getenv("OPEM_LU_GRAPH") && getenv("OPEM_LU_GRAPH")[0] == '0');
+  std::map<std::vector<int64_t>, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   struct Pending {
@@ -1585,6 +1593,7 @@ struct LuFactor::Impl {
   }
   void schedule_level(const std::vector<int32_t> &ids, DLevel &L);
   ~Impl() {
+    drop_graphs();
     for (auto e : evpool) (void)hipEventDestroy(e);
   }
   LuDev dev(const LuTree &T, int nrhs) const {
@@ -2041,6 +2050,7 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   I.d_lrows.upload(I.h_lrows, st);
   I.d_cols.upload(I.h_cols, st);
   I.atasks.clear();
+  I.drop_graphs();  // captured launch sequences belong to the previous tree
   HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -2383,6 +2393,8 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
   using KRU = std::integral_constant<int, K_ROOT_U>;
   const int L = Tr.nlevels;
   const int MC = std::max(1, Tr.max_children);
+  // the level sweeps: a fixed launch sequence for a given tree, batch shape and set of buffers
+  auto levels = [&] {
   for (int l = 0; l < L; l++) {
     {
       const int64_t a0 = I.fvi_off[l], a1 = I.fvi_off[l + 1];
@@ -2410,6 +2422,38 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
     launch(KRU(), l);
     launch(KU12(), l);
     launch(KU(), l);
+  }
+  };
+  if (I.use_graph && !I.timing) {
+    // captured once per (precision, batch shape, buffers) and replayed: the kernels read the right-hand sides,
+    // activity flags and factors through these fixed buffers, so only the launch sequence is frozen
+    const std::vector<int64_t> key = {
+        std::is_same<T, float2>::value ? 1 : 0, ty_n, ninst, nrhs, mfma_path ? 1 : 0, (int64_t)(uintptr_t)iactp,
+        (int64_t)(uintptr_t)b, (int64_t)(uintptr_t)fv, (int64_t)(uintptr_t)y, (int64_t)(uintptr_t)tt,
+        (int64_t)(uintptr_t)x, (int64_t)(uintptr_t)yr, (int64_t)(uintptr_t)part, (int64_t)(uintptr_t)F,
+        (int64_t)(uintptr_t)E, (int64_t)(uintptr_t)pp};
+    auto g = I.graphs.find(key);
+    if (g == I.graphs.end()) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        levels();
+      } catch (...) {
+        hipGraph_t aborted = nullptr;
+        (void)hipStreamEndCapture(st, &aborted);
+        if (aborted) (void)hipGraphDestroy(aborted);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(st, &graph));
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(ie);
+      g = I.graphs.emplace(key, exec).first;
+    }
+    HIPCHK(hipGraphLaunch(g->second, st));
+  } else {
+    levels();
   }
   const int64_t ndof = Tr.ndof;
   hipLaunchKernelGGL((k_lu_zero<X>), dim3(grid_for(ndof * S, 65536)), dim3(256), 0, st, ndof * S, z);

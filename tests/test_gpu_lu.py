@@ -295,3 +295,55 @@ def test_lu_symmetric_factorization_against_oracle():
                 o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
                 assert np.abs(S[i] - o["S"]).max() < 1e-9
                 assert np.abs(Sm[i] - o["S"]).max() < 1e-8
+
+
+GRAPH_CHILD = r"""
+import json
+import numpy as np
+from openparem_amd.mesh import tee_mesh
+from openparem_amd.solver import Solver
+m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=False)
+f = np.linspace(9.0e9, 9.3e9, 4)
+out = []
+with Solver(m, 2) as s:
+    for rep in range(2):  # the second sweep replays the graphs the first one captured
+        r = s.sweep(f, precision="mixed", tol=1e-11, batch=4, restart=30, precond="lu", orth_passes=1,
+                    precond_share=2)
+        out.append({"S": [[float(z.real), float(z.imag)] for z in r["S"].ravel()], "iters": r["iters"].tolist(),
+                    "converged": int(r["converged"].all())})
+print(json.dumps(out), flush=True)
+"""
+
+
+def test_lu_apply_graph_replay_matches_launches():
+    """The apply's level sweeps replayed from captured HIP graphs (the default while kernel timing is off) against
+    the same launches issued one by one (OPEM_LU_GRAPH=0), in child processes: two sweeps each on the three-port
+    tee, so every apply of the first sweep after the capture and every apply of the second replays a graph between
+    the Krylov solver's own launches and host synchronisations; the iteration counts and S must be identical, and S
+    must match the oracle's direct solve."""
+    import subprocess
+    import sys
+    root = os.path.dirname(HERE)
+    out = {}
+    for mode in ("0", "1"):
+        env = {k: v for k, v in os.environ.items() if k != "OPEM_LU_TIMING"}
+        env["OPEM_LU_GRAPH"] = mode
+        env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        print(f"apply graph {mode}", flush=True)
+        p = subprocess.run([sys.executable, "-c", GRAPH_CHILD], cwd=root, env=env, capture_output=True, text=True,
+                           timeout=150)
+        assert p.returncode == 0, p.stderr[-2000:]
+        out[mode] = json.loads(p.stdout.strip().splitlines()[-1])
+        for d in out[mode]:
+            print(f"mode {mode}: iters {d['iters']}", flush=True)
+            assert d["converged"] == 1
+    for rep in range(2):
+        assert out["0"][rep]["iters"] == out["1"][rep]["iters"]
+        assert out["0"][rep]["S"] == out["1"][rep]["S"]
+    S = np.array([complex(a, b) for a, b in out["1"][1]["S"]]).reshape(4, 3, 3)
+    m = tee_mesh(h=4e-3, ny=2, length=50e-3, arm=20e-3, cross=False)
+    pr = Problem(m, 2)
+    f = np.linspace(9.0e9, 9.3e9, 4)
+    for i in (0, 3):
+        o = pr.solve_point(f[i], {a: te10(m, a, f[i]) for a in pr.ports})
+        assert np.abs(S[i] - o["S"]).max() < 1e-8
