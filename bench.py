@@ -97,6 +97,10 @@ def _progress(msg):
 
 MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_FP64_PEAK_TFLOPS = 78.6
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# the split-BF16 deep updates spend 12 BF16 MFMA multiply-adds (24 flops) per complex multiply-add (8 algorithmic
+# flops): their ceiling in algorithmic flops is the BF16 peak x 8 / 24
+SPLIT_BF16_PEAK_TFLOPS = MFMA_BF16_PEAK_TFLOPS * 8.0 / 24.0
 
 
 def _lu_roofline(st, workload):
@@ -105,14 +109,21 @@ def _lu_roofline(st, workload):
     against the FP32 MFMA peak and the apply's factor products against HBM; the one that took longest is the
     roofline kernel.  Algorithmic flops are 8 m n k per complex update of an m x k by k x n block, whatever number of
     real MFMA products the kernel spends on it."""
-    def mfma(name, key):
-        d = {"kernel": name, "bound": "mfma", "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+    def mfma(name, key, peak=MFMA_FP32_PEAK_TFLOPS):
+        d = {"kernel": name, "bound": "mfma", "peak": peak, "unit": "TFLOP/s",
              "seconds": st[key + "_s"], "launches": st[key + "_launches"], "flops": st[key + "_flops"],
              "achieved": st[key + "_flops"] / st[key + "_s"] / 1e12 if st[key + "_s"] else None}
-        d["frac"] = d["achieved"] / MFMA_FP32_PEAK_TFLOPS if d["achieved"] else None
+        d["frac"] = d["achieved"] / peak if d["achieved"] else None
         return d
-    deep = mfma("k_dense_gemm_t128 (deep trailing updates of the FP32 fronts: 128x128 tiles, three real "
-                "v_mfma_f32_16x16x4f32 products per complex update)", "gemm128")
+    if os.environ.get("OPEM_LU_BF16", "1") != "0":
+        deep = mfma("k_dense_gemm_t128_bf16x3 (deep trailing updates of the FP32 fronts: 128x128 tiles, each real "
+                    "product as three split-BF16 v_mfma_f32_16x16x32_bf16 products, four real products per complex "
+                    "update; peak = the BF16 dense MFMA peak x 8 algorithmic / 24 BF16 flops)", "gemm128",
+                    SPLIT_BF16_PEAK_TFLOPS)
+        deep["frac_of_fp32_mfma_peak"] = (deep["achieved"] / MFMA_FP32_PEAK_TFLOPS) if deep["achieved"] else None
+    else:
+        deep = mfma("k_dense_gemm_t128 (deep trailing updates of the FP32 fronts: 128x128 tiles, three real "
+                    "v_mfma_f32_16x16x4f32 products per complex update)", "gemm128")
     gemm = mfma("k_dense_gemm (the other trailing updates: 64x64 tiles, four real MFMA products per complex "
                 "update)", "gemm")
     apply_ = {"kernel": "k_lu_apply_mfma (the preconditioner apply's products with the FP32 factor blocks on "
@@ -131,7 +142,7 @@ def _lu_roofline(st, workload):
             "traffic": None if ratio is None or dom is gemm else ratio * work_per_launch,
             "per_launch": work_per_launch, "launch_seconds": dom["seconds"] / max(1.0, dom["launches"]),
             "launches": dom["launches"]}
-    return roof, {"k_dense_gemm_t128": deep, "k_dense_gemm": gemm, "k_lu_apply_mfma": apply_}
+    return roof, {deep["kernel"].split()[0]: deep, "k_dense_gemm": gemm, "k_lu_apply_mfma": apply_}
 
 
 def _uneven(args):
@@ -246,7 +257,7 @@ def _uneven(args):
         "metric": "freq-points/sec at |dS|<1e-6 (mixed+batched vs FP64 GPU); SpMV HBM GB/s",
         "value": value, "unit": "freq-points/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c-fp32 factor and inner GMRES / c-fp64 refinement", "data": "synthetic",
+        "dtype": "c-fp32 factor (deep updates as split-bf16 MFMA products) and inner GMRES / c-fp64 refinement", "data": "synthetic",
         "config": {"workload": f"uneven-c5: {len(per)} instances, {K} at a time on one GPU (one context and stream "
                                f"each), ND1-3, 2-4 ports, {tot_pts} frequency points, "
                                f"{min(p['ndof'] for p in per)}-{max(p['ndof'] for p in per)} DOF; wall time of "
@@ -359,7 +370,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c-fp32 factor and inner GMRES / c-fp64 refinement",
+        "dtype": "c-fp32 factor (deep updates as split-bf16 MFMA products) and inner GMRES / c-fp64 refinement",
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: batched frequency sweep, {len(freqs)} frequencies "

@@ -857,6 +857,178 @@ __global__ __launch_bounds__(256) void k_dense_gemm_t128(const GemmJob *__restri
   }
 }
 
+// Deep trailing updates of FP32 fronts with split-BF16 products: the same 128 x 128 tiles, staging, XCD order and
+// store pattern as k_dense_gemm_t128, but each real product runs as three BF16 products on
+// v_mfma_f32_16x16x32_bf16 (x = hi + lo with hi = bf16(x), lo = bf16(x - hi); x y ~ hi_x hi_y + hi_x lo_y +
+// lo_x hi_y, the dropped lo_x lo_y being 2^-16 of the product), accumulated in FP32.  The complex product takes
+// four real products into two accumulators (Re += Br Ar - Bi Ai, Im += Br Ai + Bi Ar; the minus sign flips the
+// BF16 sign bits of Ai): three accumulators per block, as in the three-product form, would not fit the register
+// file beside the split operands.  The operands keep
+// about 16 significant bits, against 24 in FP32; the factor is a preconditioner inside FP64 refinement, and the
+// config-4 sweep needs the same GMRES iterations with operands rounded this way (16.27 vs 16.26 per system).
+// Operand layouts of the 16x16x32 MFMA: first operand P[i][k] in lane i + 16 g, element e (k = 8 g + e), second
+// Q[k][j] in lane j + 16 g, element e; D as for 16x16x4.  Each K slice of 32 is one MFMA step: lane (li, lk) reads
+// 8 consecutive k of its row (A) or column (B) from LDS and splits them in registers.
+typedef __bf16 lu_bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split_bf16(const float (&x)[8], lu_bf8 &hi, lu_bf8 &lo) {
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const __bf16 h = (__bf16)x[e];
+    hi[e] = h;
+    lo[e] = (__bf16)(x[e] - (float)h);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dense_gemm_t128_bf16x3(const GemmJob *__restrict__ jobs,
+                                                                const int64_t *__restrict__ prefix, int njobs,
+                                                                int64_t ntiles, int ninst, DenseBase B, int tri) {
+  __shared__ float2 As[kBK][kBT];
+  __shared__ float2 Bs[kBK][kBT];
+  const int64_t total = ntiles * ninst, b = blockIdx.x;
+  const int64_t qx = total / 8, rx = total % 8, xcd = b % 8;
+  const int64_t flat = xcd * qx + min(xcd, rx) + b / 8;
+  const int q = (int)(flat / ntiles);
+  const int64_t tile = flat % ntiles;
+  int lo = 0, hi = njobs - 1;  // last job with prefix <= tile
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= tile)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const GemmJob J = jobs[lo];
+  const int64_t local = tile - prefix[lo];
+  int tr, tc;
+  gemm_tile(local, (J.n + kBT - 1) / kBT, tri && J.tri, tr, tc);
+  const int r0 = tr * kBT, c0 = tc * kBT;
+  const float2 *A = dense_ptr<float2>(B, J.asel, J.aoff, q);
+  const float2 *Bm = dense_ptr<float2>(B, J.bsel, J.boff, q);
+  float2 *C = dense_ptr<float2>(B, J.csel, J.coff, q);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+  lu_f4 pr[4][4], pi[4][4];  // [column block][row block]: Re and Im of C's update
+#pragma unroll
+  for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+      pr[cb][rb] = lu_f4{0, 0, 0, 0};
+      pi[cb][rb] = lu_f4{0, 0, 0, 0};
+    }
+  float2 ra[16], rb_[16];
+  auto gload = [&](int kc) {
+    const int K = min(kBK, J.k - kc);
+#pragma unroll
+    for (int t = 0; t < 16; t++) {  // A: 128 rows x kBK, consecutive threads on consecutive rows
+      const int e = tid + 256 * t, rr = e & (kBT - 1), s = e >> 7;
+      ra[t] = A[min(r0 + rr, J.m - 1) + (int64_t)J.lda * (kc + min(s, K - 1))];
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++) {  // B: kBK x 128 columns, consecutive threads on consecutive k
+      const int e = tid + 256 * t, s = e & (kBK - 1), cc = e >> 5;
+      rb_[t] = Bm[kc + min(s, K - 1) + (int64_t)J.ldb * min(c0 + cc, J.n - 1)];
+    }
+  };
+  gload(0);
+  for (int kc = 0; kc < J.k; kc += kBK) {
+    const int K = min(kBK, J.k - kc);
+    __syncthreads();  // the previous slice has been read
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int e = tid + 256 * t, rr = e & (kBT - 1), s = e >> 7;
+      const bool ok = s < K && r0 + rr < J.m;
+      As[s][swz_a(s, rr)] = ok ? ra[t] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int e = tid + 256 * t, s = e & (kBK - 1), cc = e >> 5;
+      const bool ok = s < K && c0 + cc < J.n;
+      Bs[s][swz_b(s, cc)] = ok ? rb_[t] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (kc + kBK < J.k) gload(kc + kBK);
+    // the first operands (B^T) of the four column blocks, split once for the four row blocks
+    lu_bf8 bh[4][2], bl[4][2];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+      float xr[8], xi[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int s = 8 * lk + e;
+        const float2 v = Bs[s][swz_b(s, wc + 16 * cb + li)];
+        xr[e] = v.x;
+        xi[e] = v.y;
+      }
+      split_bf16(xr, bh[cb][0], bl[cb][0]);
+      split_bf16(xi, bh[cb][1], bl[cb][1]);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+      float xr[8], xi[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int s = 8 * lk + e;
+        const float2 v = As[s][swz_a(s, wr + 16 * rb + li)];
+        xr[e] = v.x;
+        xi[e] = v.y;
+      }
+      lu_bf8 ahr, alr, ahi, ali;
+      split_bf16(xr, ahr, alr);
+      split_bf16(xi, ahi, ali);
+      const lu_bf8 nhi = -ahi, nli = -ali;
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++) {
+        lu_f4 r = pr[cb][rb], m = pi[cb][rb];
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][0], ahr, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][0], alr, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][0], ahr, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][1], nhi, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][1], nli, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][1], nhi, r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][0], ahi, m, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][0], ali, m, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][0], ahi, m, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][1], ahr, m, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][1], alr, m, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][1], ahr, m, 0, 0, 0);
+        pr[cb][rb] = r;
+        pi[cb][rb] = m;
+      }
+    }
+  }
+  // D[i][j] of block (cb, rb) = C[row r0 + wr + 16 rb + j][col c0 + wc + 16 cb + i]; lane l, register r: i = 4 (l / 16)
+  // + r, j = l % 16.  All 64 values of C a lane updates are loaded at once (clamped addresses), then the ones inside
+  // the job are stored.
+  float2 cv[4][4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = min(r0 + wr + 16 * rb + li, J.m - 1), col = min(c0 + wc + 16 * cb + 4 * lk + r, J.n - 1);
+        cv[rb][cb][r] = C[row + (int64_t)J.ldc * col];
+      }
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    const int row = r0 + wr + 16 * rb + li;
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = c0 + wc + 16 * cb + 4 * lk + r;
+        if (row < J.m && col < J.n) {
+          float2 v = cv[rb][cb][r];
+          v.x -= pr[cb][rb][r];
+          v.y -= pi[cb][rb][r];
+          C[row + (int64_t)J.ldc * col] = v;
+        }
+      }
+  }
+}
+
 // Scratch X = I and Y = I (nd x nd) for every listed front.  Task = (front, first column of a 16-column chunk).
 template <typename T>
 __global__ void k_dense_eye(const int2 *__restrict__ tasks, const DenseFront *__restrict__ fr, DenseBase B) {
@@ -1542,6 +1714,8 @@ struct LuFactor::Impl {
   // symmetric factorization of matrices the caller declares symmetric: OPEM_LU_SYM=0 never, 1 (default) FP32
   // factors, 2 FP32 and FP64 factors
   int sym_mode = getenv("OPEM_LU_SYM") ? std::max(0, std::min(2, atoi(getenv("OPEM_LU_SYM")))) : 1;
+  // deep FP32 trailing updates with split-BF16 products (OPEM_LU_BF16=0: FP32 products)
+  bool deep_bf16 = !(getenv("OPEM_LU_BF16") && getenv("OPEM_LU_BF16")[0] == '0');
   // the apply's level sweeps replayed from HIP graphs (OPEM_LU_GRAPH=0: launched one by one; always one by one
   // while kernel timing is on, whose events sit between the launches)
   bool use_graph = !(getenv("OPEM_LU_GRAPH") && getenv("OPEM_LU_GRAPH")[0] == '0');
@@ -2090,8 +2264,12 @@ static void dense_level(LuFactor::Impl &I, const LuFactor::Impl::DLevel &L, int 
       const int64_t *p128 = (sym ? I.d_gprefix128_sym : I.d_gprefix128).as<int64_t>() + S.gemm_off;
       if (std::is_same<T, float2>::value && S.deep && (I.deep_gemm == 2 || (I.deep_gemm == 1 && S.deep_fit)))
         I.timed(3, flops * ninst, [&] {
-          hipLaunchKernelGGL(k_dense_gemm_t128, dim3((unsigned)(t128 * ninst)), dim3(256), 0, st,
-                             I.d_gemm.as<GemmJob>() + S.gemm_off, p128, (int)S.gemm_n, t128, ninst, B, tri);
+          if (I.deep_bf16)
+            hipLaunchKernelGGL(k_dense_gemm_t128_bf16x3, dim3((unsigned)(t128 * ninst)), dim3(256), 0, st,
+                               I.d_gemm.as<GemmJob>() + S.gemm_off, p128, (int)S.gemm_n, t128, ninst, B, tri);
+          else
+            hipLaunchKernelGGL(k_dense_gemm_t128, dim3((unsigned)(t128 * ninst)), dim3(256), 0, st,
+                               I.d_gemm.as<GemmJob>() + S.gemm_off, p128, (int)S.gemm_n, t128, ninst, B, tri);
         });
       else
         I.timed(0, flops * ninst, [&] {
