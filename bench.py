@@ -200,7 +200,7 @@ def _uneven(args):
                         if fp64:
                             run = dict(precision="fp64", batch=1, orth_passes=2, **opts)
                         else:
-                            run = dict(precision="mixed", precond_share=share_of(f), orth_passes=1, **opts)
+                            run = dict(precision="mixed", precond_share=share_of(f), orth_passes=2, **opts)
                         lu_stats(s, reset=True)
                         t0 = time.perf_counter()
                         r = s.sweep(f, **run)
@@ -310,7 +310,8 @@ def main():
     s = Solver(work["mesh"], work["order"], boundaries=work["boundaries"])
     nports = len(s.ports)
     opts = dict(restart=args.restart, tol=args.tol, maxit=600, precond="lu")
-    mixed = dict(precision="mixed", precond_share=args.share, orth_passes=1, **opts)
+    # two classical Gram-Schmidt passes, the projections accumulated in FP64 (re-orthogonalisation)
+    mixed = dict(precision="mixed", precond_share=args.share, orth_passes=2, **opts)
 
     for _ in range(args.warmup):
         s.sweep(freqs, **mixed)
