@@ -71,13 +71,39 @@ def test_scaling_and_rotation(p):
     np.testing.assert_allclose(M3, M, atol=1e-12 * abs(M).max())
 
 
-def test_mixed_curl_is_mass_times_curl():
-    """Q_e[a][b] = int curl(phi_b).phi_a equals M-weighted projection of curl; check on fields:
-    for u = sum u_b phi_b, int curl(u).phi_a computed two ways (quadrature degree raised)."""
-    v = REF_TET + np.random.default_rng(3).uniform(-0.1, 0.1, (1, 4, 3))
-    Q = nd.mixed_curl_matrices(v, 2)
-    Q2 = nd.mixed_curl_matrices(v, 2, nquad=7)
-    np.testing.assert_allclose(Q, Q2, atol=1e-14 * abs(Q).max())
+def _load(space, field):
+    """b_i = int field(x) . phi_i over the mesh (element quadrature of degree p + 2, scattered by elem_dofs)."""
+    v = space.sorted_coords()
+    L, w = nd.tet_rule(space.p + 2)
+    g, det = nd.tet_gradients(v)
+    phi, _ = nd.eval_basis(nd.tet_local_basis(space.p), L, g)
+    xq = np.einsum("qv,evk->eqk", L, v)
+    be = np.einsum("q,e,eqk,eqak->ea", w, np.abs(det), field(xq), phi)
+    b = np.zeros(space.ndof)
+    np.add.at(b, space.elem_dofs.ravel(), be.ravel())
+    return b
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_mixed_curl_reproduces_a_constant_curl(p):
+    """The mixed curl Q_e[a][b] = int curl(phi_b) . phi_a pinned by the de Rham property: E = c x x / 2 + a has
+    the constant curl c and lies in ND_p (p >= 1), so its L2 projection x_E = M^-1 b(E) is E itself and
+    Q x_E = int curl(E) . phi_i = b(c), the load of the constant field c.  Q with test and trial swapped (Q^T), a
+    curl missing its constant term or a flipped sign fails (checked on Q^T)."""
+    from scipy.sparse.linalg import spsolve
+    m = waveguide_mesh(nx=2, ny=2, nz=3, seed=4)
+    S = oasm.Space(m, p)
+    v = S.sorted_coords()
+    _, Me = nd.element_matrices(v, p)
+    Qe = nd.mixed_curl_matrices(v, p)
+    M = oasm._scatter(S.elem_dofs, Me, S.ndof).tocsc()
+    Q = oasm._scatter(S.elem_dofs, Qe, S.ndof).tocsr()
+    c = np.array([0.3, -1.1, 0.7]) * 1e2
+    a = np.array([0.5, 0.2, -0.4])
+    xE = spsolve(M, _load(S, lambda x: 0.5 * np.cross(c, x) + a))
+    bc = _load(S, lambda x: np.broadcast_to(c, x.shape))
+    np.testing.assert_allclose(Q @ xE, bc, atol=1e-10 * np.abs(bc).max())
+    assert np.abs(Q.T @ xE - bc).max() > 1e-3 * np.abs(bc).max()
 
 
 def test_global_curl_of_gradient_vanishes():
