@@ -4,6 +4,7 @@ import pytest
 import scipy.sparse as sp
 
 from openparem_amd.mesh import microstrip_mesh, random_tets, waveguide_mesh
+from openparem_amd.ports import te10
 from oracle import assemble as oasm
 from oracle import nd as ond
 from oracle.pipeline import Problem
@@ -274,6 +275,7 @@ def test_sweep_waveguide_fp64(p, precond):
 
 
 def test_sweep_mixed_vs_fp64():
+    """Mixed (FP32 inner, FP64 refinement, batched) against the plain FP64 path and both against the oracle."""
     m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
     f = np.array([8e9, 10e9, 12e9])
     s = _solver(m, 2)
@@ -285,6 +287,11 @@ def test_sweep_mixed_vs_fp64():
     assert r64["converged"].all() and r32["converged"].all()
     assert np.abs(r64["S"] - r32["S"]).max() < 1e-6
     assert (r32["true_res"] <= 1e-10).all()
+    pr = Problem(m, 2)
+    for i, fi in enumerate(f):
+        o = pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})
+        assert np.abs(r64["S"][i] - o["S"]).max() < 1e-9
+        assert np.abs(r32["S"][i] - o["S"]).max() < 1e-8
 
 
 def test_reference_mesh_gpu():
@@ -459,7 +466,9 @@ def test_sweep_nd3():
 
 @pytest.fixture(scope="module")
 def nine_point_fp64():
-    """Nine frequencies solved one at a time in FP64 (the batched runs below are checked against it)."""
+    """Nine frequencies solved one at a time in FP64 on the GPU and by the oracle's direct solve: the GPU's S must
+    equal the oracle's to 1e-9, and the batched runs below are checked against the oracle's S (r1["S"]; the
+    GPU's one-at-a-time S is r1["S_gpu"])."""
     m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
     f = np.linspace(8e9, 12e9, 9)
     s = _solver(m, 2)
@@ -468,6 +477,10 @@ def nine_point_fp64():
     finally:
         s.close()
     assert r1["converged"].all()
+    pr = Problem(m, 2)
+    So = np.array([pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})["S"] for fi in f])
+    assert np.abs(r1["S"] - So).max() < 1e-9
+    r1 = dict(r1, S_gpu=r1["S"], S=So)
     return m, f, r1
 
 
@@ -544,6 +557,10 @@ def test_sweep_splits_large_batches():
         s.close()
     assert ra["converged"].all() and rb["converged"].all()
     assert np.abs(ra["S"] - rb["S"]).max() < 1e-9
+    pr = Problem(m, 1)
+    for i in (0, 64, 127, 128, 129):  # the first batch, its last frequency, and the split-off batch
+        o = pr.solve_point(f[i], {a: te10(m, a, f[i]) for a in pr.ports})
+        assert np.abs(ra["S"][i] - o["S"]).max() < 1e-9
 
 
 @pytest.mark.parametrize("deg,prec", [(3, "mixed"), (4, "mixed"), (5, "fp64")])

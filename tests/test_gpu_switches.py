@@ -1,6 +1,7 @@
 """The library's diagnostic switches (environment variables read once per process) select measured kernel variants:
-each one runs a batched mixed sweep in a child process and must give the S of the plain FP64 one-frequency-at-a-
-time solve, to the |dS| < 1e-6 bar of the mixed path.  One child at a time; each prints one JSON line."""
+each one runs a batched mixed sweep in a child process and must give the oracle's S (which the plain FP64
+one-frequency-at-a-time GPU solve matches to 1e-9), to the |dS| < 1e-6 bar of the mixed path.  One child at a
+time; each prints one JSON line."""
 import json
 import os
 import subprocess
@@ -31,13 +32,20 @@ print(json.dumps({"S": [[float(z.real), float(z.imag)] for z in r["S"].ravel()],
 
 @pytest.fixture(scope="module")
 def reference_s():
+    """The oracle's S at the nine frequencies (direct solve), after checking the GPU's plain FP64 one-frequency-at-
+    a-time S against it to 1e-9."""
+    from openparem_amd.ports import te10
     from openparem_amd.solver import Solver
+    from oracle.pipeline import Problem
     m = waveguide_mesh(nx=4, ny=2, nz=6, seed=4)
     f = np.linspace(8e9, 12e9, 9)
     with Solver(m, 2) as s:
         r = s.sweep(f, precision="fp64", tol=1e-10, batch=1)
     assert r["converged"].all()
-    return r["S"]
+    pr = Problem(m, 2)
+    So = np.array([pr.solve_point(fi, {a: te10(m, a, fi) for a in pr.ports})["S"] for fi in f])
+    assert np.abs(r["S"] - So).max() < 1e-9
+    return So
 
 
 @pytest.mark.parametrize("env", [
