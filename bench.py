@@ -288,7 +288,7 @@ def main():
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--share", type=int, default=4, help="adjacent frequencies sharing one FP32 LU factor")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--concurrent", type=int, default=3,
+    ap.add_argument("--concurrent", type=int, default=4,
                     help="uneven-c5: instances sharing the GPU at a time (own context and stream each)")
     args = ap.parse_args()
     if args.gpus != 1:
