@@ -1124,15 +1124,28 @@ __global__ void k_lu_gather_rhs(int64_t n, int nrhs, const int32_t *__restrict__
 }
 
 template <typename X>
+// z at the factored rows: x of an active instance, zero for an instance whose systems are all inactive (the rows
+// outside the factored set are cleared by k_lu_zero_list, so every entry of z is written once per apply).
 __global__ void k_lu_scatter(int64_t n, int nrhs, int64_t S, const int32_t *__restrict__ perm,
                              const double2 *__restrict__ x, X *__restrict__ z, const int32_t *__restrict__ iact) {
   const int q = blockIdx.y;
-  if (iact && !iact[q]) return;
+  const bool active = !iact || iact[q];
   const int64_t tot = n * nrhs;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t pos = e / nrhs;
     const int j = (int)(e % nrhs);
-    z[(int64_t)perm[pos] * S + (int64_t)q * nrhs + j] = cvt<X>(x[q * tot + e]);
+    z[(int64_t)perm[pos] * S + (int64_t)q * nrhs + j] =
+        active ? cvt<X>(x[q * tot + e]) : cvt<X>(make_double2(0.0, 0.0));
+  }
+}
+
+// z = 0 at the rows outside the factored set (rows[0 .. count)), every system.
+template <typename X>
+__global__ void k_lu_zero_list(int64_t count, const int32_t *__restrict__ rows, int64_t S, X *__restrict__ z) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count * S;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / S;
+    z[(int64_t)rows[r] * S + (e - r * S)] = cvt<X>(make_double2(0.0, 0.0));
   }
 }
 
@@ -1634,7 +1647,7 @@ inline int grid_for(int64_t n, int cap = 8192) {
 struct LuFactor::Impl {
   hipStream_t st = nullptr;
   // tree on the device
-  DevBuf dbeg, uoff, foff, fvoff, uidx, umap, parent, kid_off, kids, perm, asm_pos;
+  DevBuf dbeg, uoff, foff, fvoff, uidx, umap, parent, kid_off, kids, perm, asm_pos, excl;
   int64_t fstride = 0, fvtot = 0;
   // schedule
   std::vector<std::vector<int32_t>> level_nodes;  // per level
@@ -2099,6 +2112,13 @@ void LuFactor::setup(const LuTree &T, int nexc, const std::vector<int32_t> &root
   I.kids.upload(kids, st);
   I.perm.upload(T.perm, st);
   I.asm_pos.upload(T.asm_pos, st);
+  {
+    std::vector<int32_t> ex;  // rows outside the factored set (cleared in every apply's output)
+    for (int64_t d = 0; d < T.ndof; d++)
+      if (T.iperm[d] < 0) ex.push_back((int32_t)d);
+    if ((int64_t)ex.size() != T.ndof - T.n) throw std::logic_error("lu: excluded rows do not match the tree");
+    I.excl.upload(ex, st);
+  }
   I.fstride = T.foff[nn];
   // root
   I.root = T.last_node;
@@ -2633,8 +2653,10 @@ static int solve_impl(LuFactor::Impl &I, const LuTree &Tr, int ninst, int nrhs, 
   } else {
     levels();
   }
-  const int64_t ndof = Tr.ndof;
-  hipLaunchKernelGGL((k_lu_zero<X>), dim3(grid_for(ndof * S, 65536)), dim3(256), 0, st, ndof * S, z);
+  const int64_t nex = Tr.ndof - Tr.n;  // rows outside the factored set
+  if (nex > 0)
+    hipLaunchKernelGGL((k_lu_zero_list<X>), dim3(grid_for(nex * S, 65536)), dim3(256), 0, st, nex,
+                       I.excl.as<int32_t>(), S, z);
   hipLaunchKernelGGL((k_lu_scatter<X>), dim3(grid_for(n * nrhs), ninst), dim3(256), 0, st, n, nrhs, S,
                      I.perm.as<int32_t>(), x, z, iactp);
   LU_LAUNCH_CHECK();
